@@ -1,0 +1,92 @@
+/*
+ * pintt_gpu.h - C ABI of the MI355X (gfx950) exact big-integer product engine.
+ *
+ * Magnitudes only: little-endian arrays of 64-bit limbs (GMP's mp_limb_t
+ * layout on x86-64, as returned by mpz_limbs_read), least significant first.
+ * Signs, zero operands and the small-operand threshold are the caller's
+ * business (the C++ HipNttMultiplier does them, like the reference wrapper at
+ * src/arithmetic/int_ntt_multiplier.cpp:115-159).  No exceptions cross this
+ * boundary; every entry point returns a status code (0 = success).
+ *
+ * Products are exact: the engine computes the cyclic convolution of 32-bit
+ * coefficients modulo three primes below 2^31 (transform length N = 2^k,
+ * 2^10 <= N <= 2^25 with N >= 2*an + 2*bn - 1), reconstructs every
+ * coefficient with Garner's CRT and propagates carries on the device.  The
+ * result is bit-identical to mpz_mul.
+ *
+ * Thread safety: every entry point may be called from any thread; calls on one
+ * device are serialised inside the library (one product at a time).
+ */
+#ifndef PINTT_GPU_H
+#define PINTT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PINTT_OK = 0,
+    PINTT_ERR_NO_DEVICE = 1,  /* no usable gfx950 device / init failed            */
+    PINTT_ERR_TOO_LARGE = 2,  /* transform would exceed 2^25 (an + bn > 2^24)      */
+    PINTT_ERR_NO_MEMORY = 3,  /* device allocation failed                          */
+    PINTT_ERR_HIP = 4,        /* a HIP runtime call or kernel launch failed        */
+    PINTT_ERR_ARGS = 5        /* null pointer, zero length, aliasing               */
+};
+
+/* Select and initialise a device (default: device 0 on first use).
+ * Replaces the engine constructors gpu::IntNttEngine(int device_id)
+ * (src/gpu/int_ntt.h:48) and gpu::NttEngine (src/gpu/ntt_engine.h). */
+int gpu_ntt_init(int device);
+
+/* Release all device memory, tables and streams. */
+void gpu_ntt_shutdown(void);
+
+/* out[0 .. an+bn) = a[0 .. an) * b[0 .. bn), host pointers, synchronous.
+ * an, bn >= 1; out must not overlap a or b.  a == b with an == bn runs the
+ * squaring path (one forward transform).
+ * Replaces gpu::IntNttEngine::multiply(const uint32_t*, size_t, const uint32_t*,
+ * size_t, uint32_t*, size_t, uint64_t) (src/gpu/int_ntt.h:66-69,
+ * src/gpu/int_ntt.cu:301-380) and gpu::NttEngine::multiply
+ * (src/gpu/ntt_engine.h:58-60, src/gpu/ntt_engine.cu:109-158), which took
+ * base-2^15 / 2^12 digit vectors built on the host. */
+int gpu_ntt_mul(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out);
+
+/* out[0 .. 2an) = a^2.  Same contract as gpu_ntt_mul(a, an, a, an, out).
+ * Replaces Multiplier::square's default route multiply(r, a, a)
+ * (src/arithmetic/multiplier.h:36-38) on the device side. */
+int gpu_ntt_sqr(const uint64_t* a, size_t an, uint64_t* out);
+
+/* Device-resident variant: d_a, d_b, d_out are device pointers on the current
+ * device, `stream` a hipStream_t (NULL = the library's own stream).  The work
+ * is enqueued and the call returns without waiting; products issued on any
+ * stream are ordered one after another inside the library.  d_a == d_b with
+ * an == bn squares.  No reference counterpart: the reference copies host digit
+ * vectors on every call (src/gpu/int_ntt.cu:329-330,348-349). */
+int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
+                       uint64_t* d_out, void* stream);
+
+/* log2 of the transform length used for an an-limb by bn-limb product,
+ * or -1 when it exceeds 2^25. */
+int gpu_ntt_transform_log2(size_t an, size_t bn);
+
+/* Largest an + bn accepted. */
+size_t gpu_ntt_max_limbs(void);
+
+/* Message for the last failure on the calling thread ("" if none). */
+const char* gpu_ntt_last_error(void);
+
+/* Per-pass timing with HIP events on the product's stream.  When enabled,
+ * every product records events around its passes; gpu_ntt_pass_times writes
+ * the last product's times in milliseconds (waiting for it) in the order
+ * forward-columns, row-convolution, inverse-columns+CRT, carry, and returns
+ * the number of passes written. */
+void gpu_ntt_set_timing(int enable);
+int gpu_ntt_pass_times(double* ms, int max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

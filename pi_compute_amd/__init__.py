@@ -1,0 +1,181 @@
+"""pi_compute_amd - MI355X (gfx950) exact big-integer product engine.
+
+The product path is the HIP library ``libpintt.so`` built from ``csrc/`` and
+reached through the C ABI declared in ``include/pintt_gpu.h``.  This module is
+the thin Python view of that ABI used by the tests, the smoke check and the
+benchmark: it loads the library (and fails loudly if it was not built), binds
+the entry points with ctypes and provides the seeded operand generator.
+
+There is no Python or CPU fallback for a product: if the library or the device
+is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libpintt.so"
+
+PINTT_OK = 0
+STATUS_NAMES = {
+    0: "PINTT_OK",
+    1: "PINTT_ERR_NO_DEVICE",
+    2: "PINTT_ERR_TOO_LARGE",
+    3: "PINTT_ERR_NO_MEMORY",
+    4: "PINTT_ERR_HIP",
+    5: "PINTT_ERR_ARGS",
+}
+
+# Every symbol include/pintt_gpu.h declares, with (restype, argtypes).
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+ABI = {
+    "gpu_ntt_init": (ctypes.c_int, [ctypes.c_int]),
+    "gpu_ntt_shutdown": (None, []),
+    "gpu_ntt_mul": (ctypes.c_int, [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]),
+    "gpu_ntt_sqr": (ctypes.c_int, [_u64p, ctypes.c_size_t, _u64p]),
+    "gpu_ntt_mul_device": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p],
+    ),
+    "gpu_ntt_transform_log2": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t]),
+    "gpu_ntt_max_limbs": (ctypes.c_size_t, []),
+    "gpu_ntt_last_error": (ctypes.c_char_p, []),
+    "gpu_ntt_set_timing": (None, [ctypes.c_int]),
+    "gpu_ntt_pass_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+}
+
+PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns_crt", "carry")
+
+_lib = None
+
+
+class PinttError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
+    """Load libpintt.so and bind every ABI symbol.  Raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build the HIP engine first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in ABI.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(lib, status: int) -> None:
+    if status != PINTT_OK:
+        raise PinttError(status, lib.gpu_ntt_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u64p)
+
+
+def mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """|a| * |b| for little-endian uint64 limb arrays, on the GPU (host buffers)."""
+    lib = load_library()
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.empty(a.size + b.size, dtype=np.uint64)
+    _check(lib, lib.gpu_ntt_mul(_ptr(a), a.size, _ptr(b), b.size, _ptr(out)))
+    return out
+
+
+def sqr(a: np.ndarray) -> np.ndarray:
+    """|a|^2 on the GPU (host buffers)."""
+    lib = load_library()
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    out = np.empty(2 * a.size, dtype=np.uint64)
+    _check(lib, lib.gpu_ntt_sqr(_ptr(a), a.size, _ptr(out)))
+    return out
+
+
+def mul_device(d_a: int, an: int, d_b: int, bn: int, d_out: int, stream: int = 0) -> None:
+    """Enqueue a product on device pointers (e.g. torch tensors' data_ptr())."""
+    lib = load_library()
+    _check(lib, lib.gpu_ntt_mul_device(d_a, an, d_b, bn, d_out, stream or None))
+
+
+def transform_log2(an: int, bn: int) -> int:
+    return load_library().gpu_ntt_transform_log2(an, bn)
+
+
+def set_timing(enable: bool) -> None:
+    load_library().gpu_ntt_set_timing(1 if enable else 0)
+
+
+def pass_times() -> dict:
+    lib = load_library()
+    buf = (ctypes.c_double * 8)()
+    k = lib.gpu_ntt_pass_times(buf, 8)
+    return {PASS_NAMES[i]: buf[i] for i in range(k)}
+
+
+# ---------------------------------------------------------------------------
+# Seeded operands: splitmix64 stream (Steele, Lea, Flood 2014), limbs drawn
+# least significant first, top limb masked to the exact bit length with the top
+# bit set; operand a first, then b, from one stream.
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, start: int, count: int) -> np.ndarray:
+    """Outputs start .. start+count-1 of the splitmix64 stream seeded with `seed`."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def limbs_for_bits(bits: int) -> int:
+    return (bits + 63) // 64
+
+
+def draw_operand(seed: int, start: int, bits: int) -> np.ndarray:
+    """`bits`-bit operand (top bit set) from stream outputs start.. ."""
+    n = limbs_for_bits(bits)
+    x = splitmix64(seed, start, n)
+    top = bits - 64 * (n - 1)
+    if top < 64:
+        x[-1] &= np.uint64((1 << top) - 1)
+    x[-1] |= np.uint64(1 << (top - 1))
+    return x
+
+
+def operands(seed: int, bits_a: int, bits_b: int):
+    """Operands a (bits_a bits) then b (bits_b bits) from one splitmix64 stream."""
+    a = draw_operand(seed, 0, bits_a)
+    b = draw_operand(seed, a.size, bits_b)
+    return a, b
+
+
+def to_int(limbs: np.ndarray) -> int:
+    return int.from_bytes(np.ascontiguousarray(limbs, dtype="<u8").tobytes(), "little")
+
+
+def from_int(x: int, nlimbs: int | None = None) -> np.ndarray:
+    n = nlimbs if nlimbs is not None else max(1, (x.bit_length() + 63) // 64)
+    return np.frombuffer(x.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)

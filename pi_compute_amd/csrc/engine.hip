@@ -1,0 +1,321 @@
+// engine.hip - host side of the C ABI (include/pintt_gpu.h).
+//
+// Owns, per process, one device context: a stream, the twiddle tables (one per
+// prime, grown to the largest transform seen), two residue arenas of 3 * N
+// words, the carry scratch and the staging buffers of the host entry points.
+// Products are serialised by a mutex; device-pointer products issued on other
+// streams are ordered behind the previous product with an event.
+//
+// Reference counterparts: IntNttEngine::ensure_buffers / TwiddleCache::build
+// (src/gpu/int_ntt.cu:110-186,282-299), which rebuilt the tables whenever the
+// length changed, and the per-GPU mutex of the wrappers
+// (src/arithmetic/int_ntt_multiplier.cpp:137-144).
+#include "pintt_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "ntt_kernels.h"
+
+namespace pintt {
+namespace {
+
+thread_local std::string t_error;
+
+struct Engine {
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int tw_log2 = 0;
+    uint2* tw_fwd[kPrimes] = {};
+    uint2* tw_inv[kPrimes] = {};
+    size_t arena_n = 0;  // per-prime capacity of bufA / bufB
+    uint32_t* bufA = nullptr;
+    uint32_t* bufB = nullptr;
+    size_t carry_cap = 0;  // blocks
+    uint32_t* carry_fn = nullptr;
+    uint32_t* carry_cin = nullptr;
+    size_t io_cap = 0;  // limbs per staging buffer
+    uint64_t* d_a = nullptr;
+    uint64_t* d_b = nullptr;
+    uint64_t* d_out = nullptr;
+    bool timing = false;
+    hipEvent_t ev[5] = {};
+    bool have_times = false;
+};
+
+Engine g;
+std::mutex g_mu;
+
+int fail(int code, const std::string& what) {
+    t_error = what;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    return fail(e == hipErrorOutOfMemory ? PINTT_ERR_NO_MEMORY : PINTT_ERR_HIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define PINTT_HIP(call)                                     \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+    } while (0)
+
+uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {
+    uint64_t r = 1, x = b % p;
+    while (e) {
+        if (e & 1) r = r * x % p;
+        x = x * x % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+uint32_t invmod(uint32_t a, uint32_t p) { return powmod(a % p, p - 2, p); }
+uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((uint64_t)a * b % p); }
+uint2 shoup(uint32_t w, uint32_t p) { return make_uint2(w, (uint32_t)(((uint64_t)w << 32) / p)); }
+
+CrtConst crt_constants(int log2n) {
+    const uint32_t p0 = kP[0], p1 = kP[1], p2 = kP[2];
+    auto scale = [&](uint32_t p) {  // 2^32 / N mod p
+        return mulmod((uint32_t)((1ull << 32) % p), invmod((uint32_t)((1ull << log2n) % p), p), p);
+    };
+    CrtConst c;
+    c.k0 = shoup(scale(p0), p0);
+    uint32_t ip0_1 = invmod(p0 % p1, p1);
+    c.k1 = shoup(mulmod(scale(p1), ip0_1, p1), p1);
+    c.c1 = shoup(ip0_1, p1);
+    uint32_t ip01_2 = invmod(mulmod(p0 % p2, p1 % p2, p2), p2);
+    c.k2 = shoup(mulmod(scale(p2), ip01_2, p2), p2);
+    c.c20 = shoup(ip01_2, p2);
+    c.c21 = shoup(invmod(p1 % p2, p2), p2);
+    return c;
+}
+
+int ensure_init(int device) {
+    if (g.ready) {
+        if (device < 0 || device == g.device) return PINTT_OK;
+        return fail(PINTT_ERR_ARGS, "engine already initialised on another device");
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(PINTT_ERR_NO_DEVICE, "no HIP device available");
+    int dev = device < 0 ? 0 : device;
+    if (dev >= count) return fail(PINTT_ERR_NO_DEVICE, "device index out of range");
+    PINTT_HIP(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    PINTT_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PINTT_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                             ", the kernels are built for gfx950 only");
+    PINTT_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    PINTT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    for (auto& e : g.ev) PINTT_HIP(hipEventCreate(&e));
+    g.device = dev;
+    g.ready = true;
+    return PINTT_OK;
+}
+
+template <class T>
+int grow(T*& ptr, size_t count) {
+    if (ptr) {
+        PINTT_HIP(hipFree(ptr));
+        ptr = nullptr;
+    }
+    PINTT_HIP(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)));
+    return PINTT_OK;
+}
+
+int ensure_capacity(int log2n, size_t nwords) {
+    int rc;
+    if (log2n > g.tw_log2) {
+        int tl = log2n < 16 ? 16 : log2n;
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        size_t cnt = (size_t)1 << tl;
+        for (int i = 0; i < kPrimes; ++i) {
+            if ((rc = grow(g.tw_fwd[i], cnt))) return rc;
+            if ((rc = grow(g.tw_inv[i], cnt))) return rc;
+            PINTT_HIP(launch_build_twiddles(g.tw_fwd[i], g.tw_inv[i], i, tl, g.stream));
+        }
+        g.tw_log2 = tl;
+    }
+    size_t n = (size_t)1 << log2n;
+    if (n > g.arena_n) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        if ((rc = grow(g.bufA, kPrimes * n))) return rc;
+        if ((rc = grow(g.bufB, kPrimes * n))) return rc;
+        g.arena_n = n;
+    }
+    size_t nblk = (nwords + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock;
+    if (nblk > g.carry_cap) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        if ((rc = grow(g.carry_fn, nblk))) return rc;
+        if ((rc = grow(g.carry_cin, nblk))) return rc;
+        g.carry_cap = nblk;
+    }
+    return PINTT_OK;
+}
+
+int log2_for(size_t an, size_t bn) {
+    size_t need = 2 * an + 2 * bn - 1;
+    int k = kMinLog2N;
+    while (k <= kMaxLog2N && ((size_t)1 << k) < need) ++k;
+    return k > kMaxLog2N ? -1 : k;
+}
+
+Twiddles twiddles() {
+    Twiddles t;
+    for (int i = 0; i < kPrimes; ++i) {
+        t.fwd[i] = g.tw_fwd[i];
+        t.inv[i] = g.tw_inv[i];
+    }
+    return t;
+}
+
+// Enqueue one product on `s`; operands and output are device pointers.
+int run_product(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
+                bool square, hipStream_t s) {
+    const int log2n = log2_for(an, bn);
+    if (log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+    const size_t nwords = 2 * (an + bn);
+    int rc = ensure_capacity(log2n, nwords);
+    if (rc) return rc;
+    const Plan pl = make_plan(log2n);
+    const Twiddles tw = twiddles();
+    const size_t n = (size_t)1 << log2n;
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[0], s));
+    PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const uint32_t*>(d_a), (uint32_t)(2 * an),
+                                     reinterpret_cast<const uint32_t*>(d_b), (uint32_t)(2 * bn),
+                                     g.bufA, g.bufB, square ? 1 : 2, tw, s));
+    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[1], s));
+    PINTT_HIP(launch_row_convolution(pl, g.bufA, g.bufB, square ? 1 : 0, tw, s));
+    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[2], s));
+    PINTT_HIP(launch_inverse_columns_crt(pl, g.bufA, g.bufB, tw, crt_constants(log2n), s));
+    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[3], s));
+    PINTT_HIP(launch_carry(g.bufB, n, reinterpret_cast<uint32_t*>(d_out), nwords, g.carry_fn,
+                           g.carry_cin, s));
+    if (g.timing) {
+        PINTT_HIP(hipEventRecord(g.ev[4], s));
+        g.have_times = true;
+    }
+    PINTT_HIP(hipEventRecord(g.done, s));
+    return PINTT_OK;
+}
+
+int host_product(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out,
+                 bool square) {
+    if (!a || !b || !out || an == 0 || bn == 0) return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+    if ((out < a + an && a < out + an + bn) || (out < b + bn && b < out + an + bn))
+        return fail(PINTT_ERR_ARGS, "output overlaps an operand");
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc = ensure_init(-1);
+    if (rc) return rc;
+    if (log2_for(an, bn) < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+    PINTT_HIP(hipSetDevice(g.device));
+    size_t need = an + bn;
+    if (need > g.io_cap) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        if ((rc = grow(g.d_a, need))) return rc;
+        if ((rc = grow(g.d_b, need))) return rc;
+        if ((rc = grow(g.d_out, need))) return rc;
+        g.io_cap = need;
+    }
+    PINTT_HIP(hipMemcpyAsync(g.d_a, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+    if (!square) PINTT_HIP(hipMemcpyAsync(g.d_b, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+    rc = run_product(g.d_a, an, square ? g.d_a : g.d_b, bn, g.d_out, square, g.stream);
+    if (rc) return rc;
+    PINTT_HIP(hipMemcpyAsync(out, g.d_out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    return PINTT_OK;
+}
+
+}  // namespace
+}  // namespace pintt
+
+using namespace pintt;
+
+extern "C" {
+
+int gpu_ntt_init(int device) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    return ensure_init(device);
+}
+
+void gpu_ntt_shutdown(void) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g.ready) return;
+    (void)hipSetDevice(g.device);
+    (void)hipStreamSynchronize(g.stream);
+    for (int i = 0; i < kPrimes; ++i) {
+        (void)hipFree(g.tw_fwd[i]);
+        (void)hipFree(g.tw_inv[i]);
+    }
+    (void)hipFree(g.bufA);
+    (void)hipFree(g.bufB);
+    (void)hipFree(g.carry_fn);
+    (void)hipFree(g.carry_cin);
+    (void)hipFree(g.d_a);
+    (void)hipFree(g.d_b);
+    (void)hipFree(g.d_out);
+    for (auto& e : g.ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(g.done);
+    (void)hipStreamDestroy(g.stream);
+    g = Engine{};
+}
+
+int gpu_ntt_mul(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out) {
+    return host_product(a, an, b, bn, out, a == b && an == bn);
+}
+
+int gpu_ntt_sqr(const uint64_t* a, size_t an, uint64_t* out) {
+    return host_product(a, an, a, an, out, true);
+}
+
+int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
+                       uint64_t* d_out, void* stream) {
+    if (!d_a || !d_b || !d_out || an == 0 || bn == 0)
+        return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc = ensure_init(-1);
+    if (rc) return rc;
+    PINTT_HIP(hipSetDevice(g.device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
+    return run_product(d_a, an, d_b, bn, d_out, d_a == d_b && an == bn, s);
+}
+
+int gpu_ntt_transform_log2(size_t an, size_t bn) {
+    if (an == 0 || bn == 0) return -1;
+    return log2_for(an, bn);
+}
+
+size_t gpu_ntt_max_limbs(void) { return ((size_t)1 << (kMaxLog2N - 1)); }
+
+const char* gpu_ntt_last_error(void) { return t_error.c_str(); }
+
+void gpu_ntt_set_timing(int enable) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g.timing = enable != 0;
+    g.have_times = false;
+}
+
+int gpu_ntt_pass_times(double* ms, int max) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g.ready || !g.have_times || !ms) return 0;
+    if (hipEventSynchronize(g.ev[4]) != hipSuccess) return 0;
+    int k = 0;
+    for (int i = 0; i < 4 && k < max; ++i) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, g.ev[i], g.ev[i + 1]) != hipSuccess) return k;
+        ms[k++] = t;
+    }
+    return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// ntt_kernels.h - device side of the three-prime NTT product (gfx950).
+//
+// Transform layout (length N = R * C, four-step without transposes):
+//   forward  = log2(R) Cooley-Tukey layers on every column (stride C), then
+//              log2(C) layers on every row; the result sits in bit-reversed
+//              order, which the pointwise product does not care about;
+//   inverse  = the same layers undone in reverse order (Gentleman-Sande), so
+//              natural order comes back without any bit-reversal pass.
+// Layer s of the length-N transform splits block b (of size N / 2^s) with the
+// twiddle Z[2^s + b] = w_{2^(s+1)}^bitrev_s(b).  That table does not depend on
+// N, so one table of 2^25 (w, w') pairs per prime serves every length.  The
+// first log2(R) layers only pair elements at distances that are multiples of C,
+// i.e. inside one column; the rest stay inside one row (block = row index).
+//
+// Each workgroup owns a tile in LDS; a thread keeps 16 elements in registers
+// and runs up to four layers on them between LDS exchanges (radix-16 steps).
+//
+// Replaces: bit_reverse_kernel, ntt_butterfly_kernel (one launch per layer),
+// pointwise_mod_mul_kernel, scale_kernel (src/gpu/int_ntt.cu:57-104,192-220)
+// and the host CRT + carry loop (src/gpu/int_ntt.cu:226-245,352-377).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "modarith.h"
+
+namespace pintt {
+
+struct Twiddles {
+    const uint2* fwd[kPrimes];  // (w, floor(w 2^32 / p)) in Cooley-Tukey order
+    const uint2* inv[kPrimes];  // inverses, same indexing
+};
+
+// Constants for Garner's CRT at one transform length N.  The residues coming
+// out of the inverse transform equal N * x * 2^-32 (Montgomery pointwise
+// product), so the first multiplier of each prime also carries 2^32 / N.
+struct CrtConst {
+    uint2 k0;        // 2^32 N^-1                  mod p0
+    uint2 k1;        // 2^32 N^-1 p0^-1            mod p1
+    uint2 c1;        // p0^-1                      mod p1
+    uint2 k2;        // 2^32 N^-1 (p0 p1)^-1       mod p2
+    uint2 c20;       // (p0 p1)^-1                 mod p2
+    uint2 c21;       // p1^-1                      mod p2
+};
+
+constexpr int kElemsPerThread = 16;
+constexpr int kMinLog2N = 10;
+constexpr int kCarryThreads = 256;
+constexpr int kCarryWordsPerThread = 16;
+constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
+constexpr int kScanThreads = 1024;
+
+// Row/column split and tile shapes for a transform of length 2^n.
+struct Plan {
+    int log2n, log2r, log2c;
+    int log2w;     // columns per workgroup in the column passes
+    int log2rows;  // rows per workgroup in the row pass
+};
+
+__host__ inline Plan make_plan(int log2n) {
+    Plan pl{};
+    pl.log2n = log2n;
+    if (log2n <= 24) {
+        pl.log2r = log2n / 2 < 10 ? log2n / 2 : 10;
+    } else {
+        pl.log2r = 11;
+    }
+    pl.log2c = log2n - pl.log2r;
+    if (pl.log2r == 11) {
+        pl.log2w = 3;
+    } else {
+        int w = 12 - pl.log2r;  // 4096 / R elements wide
+        if (w < 4) w = 4;
+        if (w > pl.log2c) w = pl.log2c;
+        pl.log2w = w;
+    }
+    int rows = 12 - pl.log2c;
+    if (rows < 0) rows = 0;
+    if (rows > pl.log2r) rows = pl.log2r;
+    pl.log2rows = rows;
+    return pl;
+}
+
+// Launch entry points (defined in ntt_kernels.hip).
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s);
+hipError_t launch_forward_columns(const Plan& pl, const uint32_t* src0, uint32_t ncoef0,
+                                  const uint32_t* src1, uint32_t ncoef1, uint32_t* dst0,
+                                  uint32_t* dst1, int nops, const Twiddles& tw, hipStream_t s);
+hipError_t launch_row_convolution(const Plan& pl, uint32_t* bufA, const uint32_t* bufB, int square,
+                                  const Twiddles& tw, hipStream_t s);
+hipError_t launch_inverse_columns_crt(const Plan& pl, const uint32_t* bufA, uint32_t* planes,
+                                      const Twiddles& tw, const CrtConst& cc, hipStream_t s);
+hipError_t launch_carry(const uint32_t* planes, size_t n, uint32_t* out_words, size_t nwords,
+                        uint32_t* block_fn, uint32_t* block_cin, hipStream_t s);
+
+}  // namespace pintt

@@ -1,0 +1,110 @@
+import ctypes
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class Oracle:
+    """ctypes view of oracle/liboracle.so (the checker)."""
+
+    def __init__(self, path):
+        L = ctypes.CDLL(str(path))
+        L.orc_prime.restype = ctypes.c_uint32
+        L.orc_generator.restype = ctypes.c_uint32
+        L.orc_root.restype = ctypes.c_uint32
+        L.orc_root.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_zeta_table.argtypes = [ctypes.c_int, ctypes.c_int, _u32p, _u32p]
+        L.orc_dft_textbook.argtypes = [_u32p, _u32p, ctypes.c_int, ctypes.c_int]
+        for f in (L.orc_ntt_ct_layers, L.orc_ntt_gs_layers):
+            f.argtypes = [_u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_crt3.argtypes = [_u32p, _u32p]
+        L.orc_ntt_mul.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
+        L.orc_gmp_mul.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
+        L.orc_conv_residues.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, ctypes.c_int, _u32p]
+        self.L = L
+
+    @staticmethod
+    def _p64(a):
+        return a.ctypes.data_as(_u64p)
+
+    @staticmethod
+    def _p32(a):
+        return a.ctypes.data_as(_u32p)
+
+    def prime(self, i):
+        return self.L.orc_prime(i)
+
+    def gmp_mul(self, a, b):
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b, np.uint64)
+        out = np.zeros(a.size + b.size, np.uint64)
+        self.L.orc_gmp_mul(self._p64(a), a.size, self._p64(b), b.size, self._p64(out))
+        return out
+
+    def ntt_mul(self, a, b):
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b, np.uint64)
+        out = np.zeros(a.size + b.size, np.uint64)
+        rc = self.L.orc_ntt_mul(self._p64(a), a.size, self._p64(b), b.size, self._p64(out))
+        assert rc == 0, rc
+        return out
+
+    def ct_layers(self, x, log2n, prime, lo, hi):
+        x = np.ascontiguousarray(x, np.uint32).copy()
+        self.L.orc_ntt_ct_layers(self._p32(x), log2n, prime, lo, hi)
+        return x
+
+    def gs_layers(self, x, log2n, prime, lo, hi):
+        x = np.ascontiguousarray(x, np.uint32).copy()
+        self.L.orc_ntt_gs_layers(self._p32(x), log2n, prime, lo, hi)
+        return x
+
+    def dft(self, x, log2n, prime):
+        x = np.ascontiguousarray(x, np.uint32)
+        X = np.zeros(1 << log2n, np.uint32)
+        self.L.orc_dft_textbook(self._p32(x), self._p32(X), log2n, prime)
+        return X
+
+    def zeta(self, prime, log2n):
+        z = np.zeros(1 << log2n, np.uint32)
+        zi = np.zeros(1 << log2n, np.uint32)
+        self.L.orc_zeta_table(prime, log2n, self._p32(z), self._p32(zi))
+        return z, zi
+
+    def crt3(self, r):
+        r = np.ascontiguousarray(r, np.uint32)
+        out = np.zeros(3, np.uint32)
+        self.L.orc_crt3(self._p32(r), self._p32(out))
+        return int(out[0]) | (int(out[1]) << 32) | (int(out[2]) << 64)
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    path = ROOT / "oracle" / "liboracle.so"
+    if not path.exists():
+        pytest.fail("oracle/liboracle.so is not built (run __graft_entry__.build())")
+    return Oracle(path)
+
+
+@pytest.fixture(scope="session")
+def engine():
+    import pi_compute_amd as pc
+
+    lib = pc.load_library()
+    rc = lib.gpu_ntt_init(0)
+    if rc != 0:
+        pytest.fail(f"gpu_ntt_init failed: {lib.gpu_ntt_last_error().decode()}")
+    return pc

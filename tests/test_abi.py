@@ -1,0 +1,78 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, answers the size queries, and refuses to compute without a
+device (there is no CPU fallback inside the product path)."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pi_compute_amd as pc
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(gpu_ntt_\w+)\s*\(", text, flags=re.M))
+    return names
+
+
+def test_header_declares_the_abi():
+    assert _declared_symbols() == set(pc.ABI)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pc.load_library()
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_transform_length_rule():
+    lib = pc.load_library()
+    assert lib.gpu_ntt_transform_log2(1, 1) == 10  # minimum 2^10
+    assert lib.gpu_ntt_transform_log2(65536, 65536) == 18  # 2^22-bit operands
+    assert lib.gpu_ntt_transform_log2(1 << 20, 349526) == 22  # 2^26 x 2^26/3 bits
+    assert lib.gpu_ntt_transform_log2(1 << 20, 1 << 20) == 22  # 2^26-bit square
+    assert lib.gpu_ntt_transform_log2(1 << 23, 1 << 23) == 25
+    assert lib.gpu_ntt_transform_log2(1 << 23, (1 << 23) + 1) == -1
+    assert lib.gpu_ntt_transform_log2(0, 5) == -1
+    assert lib.gpu_ntt_max_limbs() == 1 << 24
+    for an in (1, 7, 1000, 123457):
+        for bn in (1, 3, 999):
+            k = lib.gpu_ntt_transform_log2(an, bn)
+            assert (1 << k) >= 2 * (an + bn) - 1
+            assert k == 10 or (1 << (k - 1)) < 2 * (an + bn) - 1
+
+
+def test_bad_arguments_are_rejected():
+    lib = pc.load_library()
+    a = np.ones(4, np.uint64)
+    out = np.zeros(8, np.uint64)
+    rc = lib.gpu_ntt_mul(a.ctypes.data_as(pc._u64p), 0, a.ctypes.data_as(pc._u64p), 4,
+                         out.ctypes.data_as(pc._u64p))
+    assert rc == 5
+    rc = lib.gpu_ntt_mul(a.ctypes.data_as(pc._u64p), 4, a.ctypes.data_as(pc._u64p), 4,
+                         a.ctypes.data_as(pc._u64p))
+    assert rc == 5  # output overlaps an operand
+    assert "overlap" in lib.gpu_ntt_last_error().decode()
+
+
+def test_no_silent_cpu_fallback():
+    lib = pc.load_library()
+    rc = lib.gpu_ntt_init(0)
+    if rc == 0:
+        pytest.skip("a gfx950 device is present; covered by the gpu tests")
+    assert rc == 1
+    assert lib.gpu_ntt_last_error().decode()
+    with pytest.raises(pc.PinttError) as ei:
+        pc.mul(np.array([3], np.uint64), np.array([5], np.uint64))
+    assert ei.value.status == 1
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError, match="missing"):
+        pc.load_library(tmp_path / "libpintt.so")

@@ -1,0 +1,152 @@
+"""GPU parity: products through the C ABI against GMP mpz_mul (bit-exact).
+
+Mirrors tests/test_gpu_ntt_multiplier.cpp of the reference (every check is
+"GPU result == mpz_mul result", :38-45) and extends it with seeded random
+operands at every transform length the engine plans (2^10 .. 2^25), squares,
+adversarial carry chains and the device-pointer entry point.
+"""
+import numpy as np
+import pytest
+
+import pi_compute_amd as pc
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(engine, oracle, a, b, label):
+    got = engine.mul(a, b)
+    want = oracle.gmp_mul(a, b)
+    assert got.dtype == np.uint64 and got.size == a.size + b.size
+    if not np.array_equal(got, want):
+        bad = np.nonzero(got != want)[0]
+        pytest.fail(f"{label}: {bad.size} limbs differ, first at {bad[0]} of {got.size}")
+
+
+def _pow_minus_one(base, exp):
+    return pc.from_int(base**exp - 1)
+
+
+# Reference operands: tests/test_gpu_ntt_multiplier.cpp:65-125.
+@pytest.mark.parametrize(
+    "x,y",
+    [
+        (12345, 67890),
+        (999999, 1),
+        (123456789012345678901234567890, 1),
+        (2**3321 - 1, 3**2095 - 1),
+        (2**33219 - 1, 3**20959 - 1),
+        (2**332192 - 1, 3**209590 - 1),
+    ],
+    ids=["small", "by_one", "30digit_by_one", "1000digit", "10000digit", "100000digit"],
+)
+def test_reference_operands(engine, oracle, x, y):
+    _check(engine, oracle, pc.from_int(x), pc.from_int(y), f"{x.bit_length()}x{y.bit_length()} bits")
+
+
+def test_reference_square(engine, oracle):
+    a = _pow_minus_one(2, 33219)
+    got = engine.sqr(a)
+    assert np.array_equal(got, oracle.gmp_mul(a, a))
+    # a == b with an == bn takes the squaring path through gpu_ntt_mul as well
+    assert np.array_equal(engine.mul(a, a), got)
+
+
+# One product per planned transform length 2^10 .. 2^24 (balanced and 3:1).
+@pytest.mark.parametrize("log2n", list(range(10, 25)))
+def test_every_plan_length(engine, oracle, log2n):
+    total = (1 << log2n) // 2  # an + bn limbs filling about half the transform
+    an = max(1, (3 * total) // 4)
+    bn = max(1, total - an)
+    a, b = pc.operands(log2n, 64 * an, 64 * bn)
+    assert engine.transform_log2(a.size, b.size) == log2n
+    _check(engine, oracle, a, b, f"n=2^{log2n} {a.size}x{b.size}")
+    sq = engine.sqr(a)
+    assert np.array_equal(sq, oracle.gmp_mul(a, a)), f"square n=2^{log2n}"
+
+
+def test_largest_transform(engine, oracle):
+    # an + bn just above 2^23 limbs needs N = 2^25, the ceiling of the prime set.
+    a, b = pc.operands(25, 64 * (7 << 20), 64 * ((1 << 20) + 1))
+    assert engine.transform_log2(a.size, b.size) == 25
+    _check(engine, oracle, a, b, "n=2^25")
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 1000, 65536])
+def test_all_ones_square(engine, oracle, n):
+    # (2^64n - 1)^2 = 2^128n - 2^(64n+1) + 1: carries ripple through every word.
+    a = np.full(n, np.uint64(0xFFFFFFFFFFFFFFFF))
+    got = engine.sqr(a)
+    assert np.array_equal(got, oracle.gmp_mul(a, a))
+
+
+@pytest.mark.parametrize("an,bn", [(1, 1), (1, 5000), (5000, 1), (3, 2), (17, 4099), (65536, 65536)])
+def test_unbalanced_and_tiny(engine, oracle, an, bn):
+    a, b = pc.operands(an * 31 + bn, 64 * an, 64 * bn)
+    _check(engine, oracle, a, b, f"{an}x{bn}")
+
+
+def test_random_limbs_with_zero_runs(engine, oracle):
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 2**64, 40000, dtype=np.uint64)
+    b = rng.integers(0, 2**64, 30000, dtype=np.uint64)
+    a[1000:9000] = 0
+    b[:5000] = 0
+    a[-1] |= np.uint64(1)
+    b[-1] |= np.uint64(1)
+    _check(engine, oracle, a, b, "zero runs")
+
+
+def test_config1_balanced_2_22_bits(engine, oracle):
+    # Two exactly-2^22-bit operands (65,536 limbs), seed 1: a*b and a*a.
+    a, b = pc.operands(1, 1 << 22, 1 << 22)
+    assert a.size == 65536 and b.size == 65536
+    _check(engine, oracle, a, b, "config1 mul")
+    assert np.array_equal(engine.sqr(a), oracle.gmp_mul(a, a)), "config1 sqr"
+
+
+def test_repeated_calls_alternate_sizes(engine, oracle):
+    # Buffers and tables grow and are reused; alternating sizes must not leak state.
+    cases = [(5000, 3000), (200, 100), (70000, 9000), (5000, 3000), (1, 1)]
+    for i, (an, bn) in enumerate(cases):
+        a, b = pc.operands(100 + i, 64 * an, 64 * bn)
+        _check(engine, oracle, a, b, f"repeat {i}")
+
+
+def test_device_entry_point(engine, oracle):
+    torch = pytest.importorskip("torch")
+    a, b = pc.operands(3, 1 << 20, 1 << 19)
+    da = torch.from_numpy(a.view(np.int64)).cuda()
+    db = torch.from_numpy(b.view(np.int64)).cuda()
+    dout = torch.empty(a.size + b.size, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    engine.mul_device(da.data_ptr(), a.size, db.data_ptr(), b.size, dout.data_ptr(), stream.cuda_stream)
+    got = dout.cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, oracle.gmp_mul(a, b))
+    # squaring through the device entry point (same pointer, same length)
+    dsq = torch.empty(2 * a.size, dtype=torch.int64, device="cuda")
+    engine.mul_device(da.data_ptr(), a.size, da.data_ptr(), a.size, dsq.data_ptr(), stream.cuda_stream)
+    assert np.array_equal(dsq.cpu().numpy().view(np.uint64), oracle.gmp_mul(a, a))
+
+
+def test_too_large_is_an_error_not_a_launch(engine):
+    lib = engine.load_library()
+    assert lib.gpu_ntt_transform_log2(1 << 23, (1 << 23) + 1) == -1
+    a = np.zeros(1, np.uint64)
+    with pytest.raises(pc.PinttError) as ei:
+        # lengths checked before any copy: the pointers are never read
+        lib_rc = lib.gpu_ntt_mul(a.ctypes.data_as(pc._u64p), 1 << 23, a.ctypes.data_as(pc._u64p),
+                                 (1 << 23) + 1, a.ctypes.data_as(pc._u64p))
+        pc._check(lib, lib_rc)
+    assert ei.value.status in (2, 5)
+
+
+def test_pass_timing(engine):
+    a, b = pc.operands(9, 1 << 22, 1 << 21)
+    engine.set_timing(True)
+    try:
+        engine.mul(a, b)
+        t = engine.pass_times()
+    finally:
+        engine.set_timing(False)
+    assert set(t) == set(pc.PASS_NAMES)
+    assert all(v > 0 for v in t.values())
