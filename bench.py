@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of exact big-integer products on one MI355X.
+
+Workload (the size-sweep configuration at k = 26): a has exactly 2^26 bits,
+b exactly floor(2^26 / 3) bits (1,048,576 x 349,526 limbs), limbs from the
+splitmix64 stream seeded with 26; the product a*b is the timed step, a^2 is
+reported beside it.  Operands and outputs stay resident in HBM; each step is
+one gpu_ntt_mul_device call on a HIP stream, timed with HIP events recorded on
+that stream (device memory, stream and events come from the HIP runtime the
+engine itself uses, see pi_compute_amd/hiprt.py).
+
+Prints one JSON line: mults/s, the dominant pass's roofline (algorithmic bytes
+over its HIP-event time), and the reference's own GmpMultiplier timed on this
+host's cores on a bounded sample as the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import pi_compute_amd as pc  # noqa: E402
+from pi_compute_amd import hiprt  # noqa: E402
+
+METRIC = "exact mults/s at 2^26 bits (%HBM roofline); 100M-digit pi time, GPU vs GMP"
+HBM_PEAK_GBS = 8000.0
+K_BITS = 26
+SEED = 26
+
+
+def pass_bytes(an: int, bn: int, log2n: int, square: bool) -> dict:
+    """Compulsory HBM bytes of each pass (4-byte residues, 3 primes)."""
+    n = 1 << log2n
+    ops = 1 if square else 2
+    limbs_in = an if square else an + bn
+    return {
+        "forward_columns": 8 * limbs_in + ops * 12 * n,        # read limbs, write residues
+        "row_convolution": ops * 12 * n + 12 * n,              # read residues, write product
+        "inverse_columns_crt": 12 * n + 12 * n,                # read product, write 3 word planes
+        "carry": 12 * n + 8 * (an + bn),                       # read planes, write limbs
+    }
+
+
+def survey_bytes(an: int, bn: int, log2n: int, square: bool) -> int:
+    """Per-product algorithmic bytes of the canonical three-prime four-step
+    pipeline (2 read+write sweeps per transform, pointwise and CRT fused)."""
+    n = 1 << log2n
+    return 24 * an + 96 * n if square else 16 * (an + bn) + 144 * n
+
+
+def time_gpu(stream, da, an, db, bn, dout, steps: int, warmup: int) -> float:
+    for _ in range(warmup):
+        pc.mul_device(da.ptr, an, db.ptr, bn, dout.ptr, stream.handle)
+    stream.synchronize()
+    start, stop = hiprt.Event(), hiprt.Event()
+    start.record(stream)
+    for _ in range(steps):
+        pc.mul_device(da.ptr, an, db.ptr, bn, dout.ptr, stream.handle)
+    stop.record(stream)
+    stop.synchronize()
+    ms = start.elapsed_ms(stop) / steps
+    start.close()
+    stop.close()
+    return ms
+
+
+def time_passes(stream, da, an, db, bn, dout, reps: int) -> dict:
+    pc.set_timing(True)
+    acc = {}
+    try:
+        for _ in range(reps):
+            pc.mul_device(da.ptr, an, db.ptr, bn, dout.ptr, stream.handle)
+            for k, v in pc.pass_times().items():
+                acc[k] = acc.get(k, 0.0) + v
+    finally:
+        pc.set_timing(False)
+    return {k: v / reps for k, v in acc.items()}
+
+
+def cpu_baseline(a: np.ndarray, b: np.ndarray, gpu_out: np.ndarray, budget_s: float):
+    """GmpMultiplier (mpz_mul) on this host: the reference's own source built
+    under oracle/_ref when present (kind "reference"), else the oracle's GMP
+    restatement (kind "port").  c threads run independent products."""
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    ref = ROOT / "oracle" / "_ref" / "libgmpmul_ref.so"
+    if ref.exists():
+        L = ctypes.CDLL(str(ref))
+        L.ref_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p, ctypes.c_int]
+        L.ref_gmp_time.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        L.ref_gmp_time.restype = ctypes.c_double
+        kind = "reference"
+
+        def product(out):
+            L.ref_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
+                          out.ctypes.data_as(u64p), 0)
+
+        def timed(reps):
+            return L.ref_gmp_time(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size, 0, reps)
+    else:
+        L = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+        L.orc_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p]
+        kind = "port"
+
+        def product(out):
+            L.orc_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
+                          out.ctypes.data_as(u64p))
+
+        def timed(reps):
+            out = np.zeros(a.size + b.size, np.uint64)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                product(out)
+            return (time.perf_counter() - t0) / reps
+
+    want = np.zeros(a.size + b.size, np.uint64)
+    t0 = time.perf_counter()
+    product(want)
+    single = time.perf_counter() - t0
+    exact = bool(np.array_equal(want, gpu_out))
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+    reps = max(1, int(budget_s / max(single, 1e-6)))
+    per = [0.0] * cores
+    threads = [threading.Thread(target=lambda i=i: per.__setitem__(i, timed(reps))) for i in range(cores)]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    value = cores * reps / wall
+    sample = (f"{cores} threads x {reps} products of the same k=26 multiply "
+              f"({a.size}x{b.size} limbs); one-thread latency {single * 1e3:.1f} ms")
+    return {"value": round(value, 4), "unit": "mults/s", "cores": cores, "kind": kind,
+            "sample": sample, "single_thread_ms": round(single * 1e3, 3),
+            "bit_exact_vs_gpu": exact}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(products shard across GPUs as independent calls; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    lib = pc.load_library()
+    if hiprt.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    if lib.gpu_ntt_init(0) != 0:
+        print("gpu_ntt_init failed: " + lib.gpu_ntt_last_error().decode(), file=sys.stderr)
+        return 1
+
+    a, b = pc.operands(SEED, 1 << K_BITS, (1 << K_BITS) // 3)
+    an, bn = a.size, b.size
+    log2n = pc.transform_log2(an, bn)
+    da, db = hiprt.DeviceBuffer.from_array(a), hiprt.DeviceBuffer.from_array(b)
+    dout, dsq = hiprt.DeviceBuffer(8 * (an + bn)), hiprt.DeviceBuffer(16 * an)
+    stream = hiprt.Stream()
+
+    ms_mul = time_gpu(stream, da, an, db, bn, dout, args.steps, args.warmup)
+    gpu_out = dout.to_array()
+    ms_sqr = time_gpu(stream, da, an, da, an, dsq, args.steps, args.warmup)
+    passes = time_passes(stream, da, an, db, bn, dout, max(5, min(50, args.steps)))
+    passes_sq = time_passes(stream, da, an, da, an, dsq, max(5, min(50, args.steps)))
+
+    pb = pass_bytes(an, bn, log2n, False)
+    dom = max(passes, key=passes.get)
+    achieved = pb[dom] / (passes[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes": pb[dom], "kernel_ms": round(passes[dom], 4)}
+    b_alg = survey_bytes(an, bn, log2n, False)
+    result = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_mul, 2),
+        "unit": "mults/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mul, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (splitmix64 seed 26 operands, resident in HBM)",
+        "config": {"workload": "exact multiply 2^26 x floor(2^26/3) bits",
+                   "limbs": [an, bn], "transform": f"N=2^{log2n}, 3 primes < 2^31",
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "product_roofline": {"algorithmic_bytes": b_alg,
+                             "achieved_GBs": round(b_alg / (ms_mul * 1e-3) / 1e9, 1),
+                             "frac": round(b_alg / (ms_mul * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "square": {"mults_per_s": round(1e3 / ms_sqr, 2), "ms": round(ms_sqr, 5),
+                   "product_roofline_frac": round(
+                       survey_bytes(an, an, log2n, True) / (ms_sqr * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "passes_ms": {k: round(v, 4) for k, v in passes.items()},
+        "passes_ms_square": {k: round(v, 4) for k, v in passes_sq.items()},
+        "pass_GBs": {k: round(pb[k] / (passes[k] * 1e-3) / 1e9, 1) for k in passes},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(a, b, gpu_out, args.cpu_budget)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

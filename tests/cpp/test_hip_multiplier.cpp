@@ -1,0 +1,152 @@
+// test_hip_multiplier.cpp - HipNttMultiplier against GmpMultiplier (mpz_cmp),
+// the cases of the reference's GPU suite (tests/test_gpu_ntt_multiplier.cpp:65-145)
+// plus signs, aliasing, seeded sizes and the fallback paths.  Needs a gfx950
+// device; run by tests/test_gpu_host.py.  Exit status = number of failures.
+#include <gmp.h>
+
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "gmp_multiplier.h"
+#include "hip_ntt_multiplier.h"
+
+namespace {
+
+int failures = 0;
+
+void check(bool ok, const std::string& name) {
+    std::printf("%s %s\n", ok ? "PASS" : "FAIL", name.c_str());
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+bool same_product(pi::Multiplier& m, mpz_t a, mpz_t b) {
+    mpz_t got, want;
+    mpz_inits(got, want, NULL);
+    m.multiply(got, a, b);
+    mpz_mul(want, a, b);
+    bool ok = mpz_cmp(got, want) == 0;
+    mpz_clears(got, want, NULL);
+    return ok;
+}
+
+void pow_minus_one(mpz_t r, unsigned long base, unsigned long e) {
+    mpz_ui_pow_ui(r, base, e);
+    mpz_sub_ui(r, r, 1);
+}
+
+// splitmix64-seeded operand of exactly `bits` bits
+void seeded(mpz_t r, uint64_t seed, size_t bits) {
+    size_t n = (bits + 63) / 64;
+    std::vector<uint64_t> limbs(n);
+    uint64_t s = seed;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        limbs[i] = z ^ (z >> 31);
+    }
+    size_t top = bits - 64 * (n - 1);
+    if (top < 64) limbs[n - 1] &= (1ull << top) - 1;
+    limbs[n - 1] |= 1ull << (top - 1);
+    mpz_import(r, n, -1, 8, 0, 0, limbs.data());
+}
+
+}  // namespace
+
+int main() {
+    pi::HipNttMultiplier gpu(1);  // threshold 1: every non-trivial product on the GPU
+    mpz_t a, b, r, e;
+    mpz_inits(a, b, r, e, NULL);
+
+    mpz_set_ui(a, 12345);
+    mpz_set_ui(b, 67890);
+    check(same_product(gpu, a, b), "MultiplySmallNumbers");
+    mpz_set_ui(a, 999999);
+    mpz_set_ui(b, 0);
+    check(same_product(gpu, a, b), "MultiplyByZero");
+    mpz_set_str(a, "123456789012345678901234567890", 10);
+    mpz_set_ui(b, 1);
+    check(same_product(gpu, a, b), "MultiplyByOne");
+    mpz_set_si(a, -12345);
+    mpz_set_si(b, 67890);
+    check(same_product(gpu, a, b), "MultiplyNegative");
+    mpz_set_si(a, -7);
+    mpz_set_si(b, -13);
+    check(same_product(gpu, a, b), "MultiplyBothNegative");
+    pow_minus_one(a, 2, 3321);
+    pow_minus_one(b, 3, 2095);
+    check(same_product(gpu, a, b), "Multiply1000DigitNumbers");
+    pow_minus_one(a, 2, 33219);
+    pow_minus_one(b, 3, 20959);
+    check(same_product(gpu, a, b), "Multiply10000DigitNumbers");
+    pow_minus_one(a, 2, 332192);
+    pow_minus_one(b, 3, 209590);
+    check(same_product(gpu, a, b), "Multiply100000DigitNumbers");
+    mpz_neg(b, b);
+    check(same_product(gpu, a, b), "Multiply100000DigitMixedSign");
+
+    pow_minus_one(a, 2, 33219);
+    gpu.square(r, a);
+    mpz_mul(e, a, a);
+    check(mpz_cmp(r, e) == 0, "SquareMatchesCpu");
+    mpz_neg(a, a);
+    gpu.square(r, a);
+    check(mpz_cmp(r, e) == 0, "SquareNegative");
+
+    {
+        pi::HipNttMultiplier high(999999);
+        mpz_set_ui(a, 12345);
+        mpz_set_ui(b, 67890);
+        high.multiply(r, a, b);
+        check(mpz_cmp_ui(r, 838102050ul) == 0 && high.stats().cpu_calls.load() == 1 &&
+                  high.stats().gpu_calls.load() == 0,
+              "ThresholdFallback");
+    }
+
+    // Aliasing: result is one of the operands (allowed by mpz semantics).
+    seeded(a, 1, 1 << 20);
+    seeded(b, 2, 700000);
+    mpz_mul(e, a, b);
+    mpz_set(r, a);
+    gpu.multiply(r, r, b);
+    check(mpz_cmp(r, e) == 0, "AliasResultIsA");
+    mpz_set(r, b);
+    gpu.multiply(r, a, r);
+    check(mpz_cmp(r, e) == 0, "AliasResultIsB");
+    mpz_mul(e, a, a);
+    mpz_set(r, a);
+    gpu.multiply(r, r, r);
+    check(mpz_cmp(r, e) == 0, "AliasSquareInPlace");
+    mpz_set(r, a);
+    gpu.square(r, r);
+    check(mpz_cmp(r, e) == 0, "AliasSquareCall");
+
+    // Seeded sweep over sizes (bits of a, bits of b).
+    const size_t sizes[][2] = {{64, 64},         {4096, 100},        {1 << 16, 1 << 16},
+                               {1 << 20, 1 << 18}, {1 << 22, 1 << 22}, {(1 << 24) + 17, 999983}};
+    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
+        seeded(a, 10 + i, sizes[i][0]);
+        seeded(b, 20 + i, sizes[i][1]);
+        check(same_product(gpu, a, b), "Seeded" + std::to_string(sizes[i][0]) + "x" +
+                                           std::to_string(sizes[i][1]));
+    }
+
+    // Oversize: above the engine's limit the wrapper falls back to mpz_mul.
+    {
+        pi::HipNttMultiplier m(1);
+        seeded(a, 77, (size_t)64 << 23);
+        seeded(b, 78, ((size_t)64 << 23) + 64);
+        m.multiply(r, a, b);
+        mpz_mul(e, a, b);
+        check(mpz_cmp(r, e) == 0 && m.stats().fallback_calls.load() == 1, "OversizeFallsBackExact");
+    }
+
+    mpz_clears(a, b, r, e, NULL);
+    gpu.print_stats();
+    std::printf("%d failure(s)\n", failures);
+    return failures;
+}

@@ -113,19 +113,28 @@ def test_repeated_calls_alternate_sizes(engine, oracle):
 
 
 def test_device_entry_point(engine, oracle):
-    torch = pytest.importorskip("torch")
+    from pi_compute_amd import hiprt
+
     a, b = pc.operands(3, 1 << 20, 1 << 19)
-    da = torch.from_numpy(a.view(np.int64)).cuda()
-    db = torch.from_numpy(b.view(np.int64)).cuda()
-    dout = torch.empty(a.size + b.size, dtype=torch.int64, device="cuda")
-    stream = torch.cuda.current_stream()
-    engine.mul_device(da.data_ptr(), a.size, db.data_ptr(), b.size, dout.data_ptr(), stream.cuda_stream)
-    got = dout.cpu().numpy().view(np.uint64)
-    assert np.array_equal(got, oracle.gmp_mul(a, b))
-    # squaring through the device entry point (same pointer, same length)
-    dsq = torch.empty(2 * a.size, dtype=torch.int64, device="cuda")
-    engine.mul_device(da.data_ptr(), a.size, da.data_ptr(), a.size, dsq.data_ptr(), stream.cuda_stream)
-    assert np.array_equal(dsq.cpu().numpy().view(np.uint64), oracle.gmp_mul(a, a))
+    da, db = hiprt.DeviceBuffer.from_array(a), hiprt.DeviceBuffer.from_array(b)
+    dout = hiprt.DeviceBuffer(8 * (a.size + b.size))
+    dsq = hiprt.DeviceBuffer(16 * a.size)
+    stream = hiprt.Stream()
+    try:
+        engine.mul_device(da.ptr, a.size, db.ptr, b.size, dout.ptr, stream.handle)
+        # squaring through the device entry point (same pointer, same length)
+        engine.mul_device(da.ptr, a.size, da.ptr, a.size, dsq.ptr, stream.handle)
+        stream.synchronize()
+        assert np.array_equal(dout.to_array(), oracle.gmp_mul(a, b))
+        assert np.array_equal(dsq.to_array(), oracle.gmp_mul(a, a))
+        # library stream (NULL) after a caller stream: ordered, same result
+        engine.mul_device(da.ptr, a.size, db.ptr, b.size, dout.ptr, 0)
+        hiprt.check(hiprt.runtime().hipDeviceSynchronize(), "sync")
+        assert np.array_equal(dout.to_array(), oracle.gmp_mul(a, b))
+    finally:
+        stream.close()
+        for buf in (da, db, dout, dsq):
+            buf.close()
 
 
 def test_too_large_is_an_error_not_a_launch(engine):
