@@ -46,8 +46,8 @@ def pass_bytes(an: int, bn: int, log2n: int, square: bool) -> dict:
     return {
         "forward_columns": 8 * limbs_in + ops * 12 * n,        # read limbs, write residues
         "row_convolution": ops * 12 * n + 12 * n,              # read residues, write product
-        "inverse_columns_crt": 12 * n + 12 * n,                # read product, write 3 word planes
-        "carry": 12 * n + 8 * (an + bn),                       # read planes, write limbs
+        "inverse_columns": 12 * n + 12 * n,                    # read and write the product residues
+        "crt_carry": 12 * min(n, 2 * (an + bn)) + 8 * (an + bn),  # read residues, write limbs
     }
 
 

@@ -81,7 +81,7 @@ const char* gpu_ntt_last_error(void);
 /* Per-pass timing with HIP events on the product's stream.  When enabled,
  * every product records events around its passes; gpu_ntt_pass_times writes
  * the last product's times in milliseconds (waiting for it) in the order
- * forward-columns, row-convolution, inverse-columns+CRT, carry, and returns
+ * forward-columns, row-convolution, inverse-columns, CRT+carry, and returns
  * the number of passes written. */
 void gpu_ntt_set_timing(int enable);
 int gpu_ntt_pass_times(double* ms, int max);

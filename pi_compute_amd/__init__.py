@@ -48,7 +48,7 @@ ABI = {
     "gpu_ntt_pass_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
 }
 
-PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns_crt", "carry")
+PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns", "crt_carry")
 
 _lib = None
 

@@ -37,9 +37,9 @@ struct Engine {
     size_t arena_n = 0;  // per-prime capacity of bufA / bufB
     uint32_t* bufA = nullptr;
     uint32_t* bufB = nullptr;
-    size_t carry_cap = 0;  // blocks
-    uint32_t* carry_fn = nullptr;
-    uint32_t* carry_cin = nullptr;
+    size_t carry_cap = 0;  // carry blocks covered by `status`
+    uint64_t* status = nullptr;
+    CarrySync* sync = nullptr;
     size_t io_cap = 0;  // limbs per staging buffer
     uint64_t* d_a = nullptr;
     uint64_t* d_b = nullptr;
@@ -152,11 +152,16 @@ int ensure_capacity(int log2n, size_t nwords) {
         if ((rc = grow(g.bufB, kPrimes * n))) return rc;
         g.arena_n = n;
     }
+    if (!g.sync) {
+        if ((rc = grow(g.sync, 1))) return rc;
+        PINTT_HIP(hipMemset(g.sync, 0, sizeof(CarrySync)));
+    }
     size_t nblk = (nwords + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock;
     if (nblk > g.carry_cap) {
+        // Status words start at zero; epochs start at 1, so stale words never match.
         PINTT_HIP(hipStreamSynchronize(g.stream));
-        if ((rc = grow(g.carry_fn, nblk))) return rc;
-        if ((rc = grow(g.carry_cin, nblk))) return rc;
+        if ((rc = grow(g.status, nblk))) return rc;
+        PINTT_HIP(hipMemset(g.status, 0, nblk * sizeof(uint64_t)));
         g.carry_cap = nblk;
     }
     return PINTT_OK;
@@ -197,10 +202,10 @@ int run_product(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, 
     if (g.timing) PINTT_HIP(hipEventRecord(g.ev[1], s));
     PINTT_HIP(launch_row_convolution(pl, g.bufA, g.bufB, square ? 1 : 0, tw, s));
     if (g.timing) PINTT_HIP(hipEventRecord(g.ev[2], s));
-    PINTT_HIP(launch_inverse_columns_crt(pl, g.bufA, g.bufB, tw, crt_constants(log2n), s));
+    PINTT_HIP(launch_inverse_columns(pl, g.bufA, tw, g.sync, s));
     if (g.timing) PINTT_HIP(hipEventRecord(g.ev[3], s));
-    PINTT_HIP(launch_carry(g.bufB, n, reinterpret_cast<uint32_t*>(d_out), nwords, g.carry_fn,
-                           g.carry_cin, s));
+    PINTT_HIP(launch_crt_carry(g.bufA, n, reinterpret_cast<uint32_t*>(d_out), nwords, crt_constants(log2n),
+                               g.sync, g.status, s));
     if (g.timing) {
         PINTT_HIP(hipEventRecord(g.ev[4], s));
         g.have_times = true;
@@ -232,7 +237,10 @@ int host_product(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uin
     rc = run_product(g.d_a, an, square ? g.d_a : g.d_b, bn, g.d_out, square, g.stream);
     if (rc) return rc;
     PINTT_HIP(hipMemcpyAsync(out, g.d_out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+    uint32_t err = 0;
+    PINTT_HIP(hipMemcpyAsync(&err, &g.sync->error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
     PINTT_HIP(hipStreamSynchronize(g.stream));
+    if (err) return fail(PINTT_ERR_HIP, "carry look-back gave up (device state corrupted)");
     return PINTT_OK;
 }
 
@@ -259,8 +267,8 @@ void gpu_ntt_shutdown(void) {
     }
     (void)hipFree(g.bufA);
     (void)hipFree(g.bufB);
-    (void)hipFree(g.carry_fn);
-    (void)hipFree(g.carry_cin);
+    (void)hipFree(g.status);
+    (void)hipFree(g.sync);
     (void)hipFree(g.d_a);
     (void)hipFree(g.d_b);
     (void)hipFree(g.d_out);

@@ -47,9 +47,17 @@ struct CrtConst {
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
 constexpr int kCarryThreads = 256;
-constexpr int kCarryWordsPerThread = 16;
+constexpr int kCarryWordsPerThread = 32;
 constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
-constexpr int kScanThreads = 1024;
+
+// Device-side state of the carry pass: block tickets and the epoch that tags
+// status words (reset/incremented on the device by the inverse column pass of
+// each product, so a captured graph can be replayed).  `error` is set if a look-back ever gave up.
+struct CarrySync {
+    unsigned long long ticket;
+    uint32_t epoch;
+    uint32_t error;
+};
 
 // Row/column split and tile shapes for a transform of length 2^n.
 struct Plan {
@@ -89,9 +97,11 @@ hipError_t launch_forward_columns(const Plan& pl, const uint32_t* src0, uint32_t
                                   uint32_t* dst1, int nops, const Twiddles& tw, hipStream_t s);
 hipError_t launch_row_convolution(const Plan& pl, uint32_t* bufA, const uint32_t* bufB, int square,
                                   const Twiddles& tw, hipStream_t s);
-hipError_t launch_inverse_columns_crt(const Plan& pl, const uint32_t* bufA, uint32_t* planes,
-                                      const Twiddles& tw, const CrtConst& cc, hipStream_t s);
-hipError_t launch_carry(const uint32_t* planes, size_t n, uint32_t* out_words, size_t nwords,
-                        uint32_t* block_fn, uint32_t* block_cin, hipStream_t s);
+// The inverse column pass also arms `sync` for the CRT/carry pass after it.
+hipError_t launch_inverse_columns(const Plan& pl, uint32_t* bufA, const Twiddles& tw, CarrySync* sync,
+                                  hipStream_t s);
+// `status` holds one word per carry block (zero-initialised once).
+hipError_t launch_crt_carry(const uint32_t* res, size_t n, uint32_t* out_words, size_t nwords,
+                            const CrtConst& cc, CarrySync* sync, uint64_t* status, hipStream_t s);
 
 }  // namespace pintt

@@ -5,11 +5,10 @@
 //                     (both operands in one launch, grid.z = operand)
 //   2. k_row_conv     forward layers of each row of a and b, Montgomery
 //                     pointwise product, inverse layers of the row
-//   3. k_col_inv_crt  inverse layers per column for all three primes, Garner
-//                     CRT to a 96-bit value per position, three word planes
-//   4. k_carry_fn / k_carry_scan / k_carry_apply
-//                     word sums s_j = X0[j] + X1[j-1] + X2[j-2] and a
-//                     parallel carry scan to the output limbs
+//   3. k_col_inv      inverse layers per column, one prime per workgroup
+//   4. k_crt_carry    Garner CRT to a 96-bit value per position, word sums
+//                     s_j = X0[j] + X1[j-1] + X2[j-2] and a single-pass carry
+//                     scan (decoupled look-back) to the output limbs
 // See ntt_kernels.h for the transform layout.
 #include "ntt_kernels.h"
 
@@ -225,71 +224,52 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT))
     }
 }
 
-__device__ __forceinline__ void crt3(uint32_t r0, uint32_t r1, uint32_t r2, const CrtConst& cc,
-                                     uint32_t& x0, uint32_t& x1, uint32_t& x2) {
-    constexpr uint32_t p0 = kP[0], p1 = kP[1], p2 = kP[2];
-    uint32_t y0 = mul_shoup(r0, cc.k0.x, cc.k0.y, p0);
-    uint32_t y1 = sub_mod(mul_shoup(r1, cc.k1.x, cc.k1.y, p1), mul_shoup(y0, cc.c1.x, cc.c1.y, p1), p1);
-    uint32_t y2 = sub_mod(mul_shoup(r2, cc.k2.x, cc.k2.y, p2), mul_shoup(y0, cc.c20.x, cc.c20.y, p2), p2);
-    y2 = sub_mod(y2, mul_shoup(y1, cc.c21.x, cc.c21.y, p2), p2);
-    // x = y0 + p0 * (y1 + p1 * y2) < p0 p1 p2 < 2^93
-    uint64_t t = (uint64_t)y2 * p1 + y1;
-    uint64_t lo = (uint64_t)(uint32_t)t * p0 + y0;
-    uint64_t hi = (t >> 32) * (uint64_t)p0 + (lo >> 32);
-    x0 = (uint32_t)lo;
-    x1 = (uint32_t)hi;
-    x2 = (uint32_t)(hi >> 32);
-}
 
-// Pass 3: inverse column layers for the three primes, then CRT.
+// Pass 3: inverse column layers, one prime per workgroup, in place.
 template <int LOGR, int LOGW>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
-    k_col_inv_crt(const uint32_t* __restrict__ bufA, uint32_t* __restrict__ planes, int log2c,
-                  Twiddles tw, CrtConst cc) {
+    k_col_inv(uint32_t* __restrict__ bufA, int log2c, Twiddles tw, CarrySync* __restrict__ sync) {
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
-    const size_t n = (size_t)1 << (LOGR + log2c);
-    const uint32_t c0 = blockIdx.x << LOGW;
-    uint32_t res[kPrimes][kElemsPerThread];
-#pragma unroll
-    for (int prime = 0; prime < kPrimes; ++prime) {
-        const uint32_t p = prime_of(prime);
-        const uint32_t* src = bufA + prime * n;
-#pragma unroll
-        for (int e = 0; e < kElemsPerThread; ++e) {
-            uint32_t idx = e * TL::NT + threadIdx.x;
-            uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
-            lds[TL::addr(w, r)] = src[(r << log2c) + c0 + w];
-        }
-        __syncthreads();
-        gs_run<TL, 0>(lds, tw.inv[prime], 0, 0, p);
-#pragma unroll
-        for (int e = 0; e < kElemsPerThread; ++e) {
-            uint32_t idx = e * TL::NT + threadIdx.x;
-            uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
-            res[prime][e] = lds[TL::addr(w, r)];
-        }
-        __syncthreads();
+    // Arm the carry pass that follows on this stream: fresh tickets, new epoch.
+    if (sync && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        sync->ticket = 0;
+        sync->epoch = sync->epoch + 1;
     }
+    const int prime = blockIdx.y;
+    const uint32_t p = prime_of(prime);
+    uint32_t* buf = bufA + ((size_t)prime << (LOGR + log2c));
+    const uint32_t c0 = blockIdx.x << LOGW;
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
-        size_t k = ((size_t)r << log2c) + c0 + w;
-        uint32_t x0, x1, x2;
-        crt3(res[0][e], res[1][e], res[2][e], cc, x0, x1, x2);
-        planes[k] = x0;
-        planes[n + k] = x1;
-        planes[2 * n + k] = x2;
+        lds[TL::addr(w, r)] = buf[(r << log2c) + c0 + w];
+    }
+    __syncthreads();
+    gs_run<TL, 0>(lds, tw.inv[prime], 0, 0, p);
+#pragma unroll
+    for (int e = 0; e < kElemsPerThread; ++e) {
+        uint32_t idx = e * TL::NT + threadIdx.x;
+        uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
+        buf[(r << log2c) + c0 + w] = lds[TL::addr(w, r)];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Carry propagation.  Word j of the product before carries is
-// s_j = X0[j] + X1[j-1] + X2[j-2] < 3 * 2^32.  Along any run of words the
-// carry stays below 4, so the effect of a run on an incoming carry c in
-// {0,1,2,3} is a map {0..3} -> {0..3}, packed as four 2-bit entries.  Maps
-// compose associatively, which gives a parallel scan.
+// Pass 4: CRT and carry propagation in one pass.
+//
+// Word j of the product before carries is s_j = X0[j] + X1[j-1] + X2[j-2]
+// < 3 * 2^32, where X0..X2 are the 32-bit words of the CRT value at position
+// k.  Along any run of words the carry stays below 4, so the effect of a run on
+// an incoming carry c in {0,1,2,3} is a map {0..3} -> {0..3}, packed as four
+// 2-bit entries.  Maps compose associatively: a wave-shuffle scan inside the
+// block, then a decoupled look-back across blocks (blocks are numbered in the
+// order they start, from a ticket counter, so every block only waits for
+// blocks that already run and publish their aggregate before waiting).
+// A status word carries epoch, flag and map in one 64-bit atomic, so relaxed
+// atomics suffice: release/acquire would add L2 write-backs and invalidations
+// on every publication and look-back round.
 
 __device__ __forceinline__ uint32_t fn_apply(uint32_t f, uint32_t c) { return (f >> (2 * c)) & 3u; }
 
@@ -302,51 +282,26 @@ __device__ __forceinline__ uint32_t fn_compose(uint32_t later, uint32_t earlier)
 }
 
 constexpr uint32_t kFnIdentity = 0u | (1u << 2) | (2u << 4) | (3u << 6);
+constexpr uint64_t kFlagAggregate = 1, kFlagPrefix = 2;
+constexpr uint32_t kLookbackSpinLimit = 1u << 26;
 
-__device__ __forceinline__ uint64_t word_sum(const uint32_t* __restrict__ planes, size_t n, size_t j) {
-    uint64_t s = 0;
-    if (j < n) s += planes[j];
-    if (j >= 1 && j - 1 < n) s += planes[n + j - 1];
-    if (j >= 2 && j - 2 < n) s += planes[2 * n + j - 2];
-    return s;
+__device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint64_t flag, uint32_t fn) {
+    return ((uint64_t)epoch << 32) | (flag << 8) | fn;
 }
 
-// The carry map of words [j0, j0 + 16) (words at or past nwords are zero).
-__device__ __forceinline__ uint32_t thread_fn(const uint32_t* __restrict__ planes, size_t n,
-                                              size_t nwords, size_t j0) {
-    uint64_t c = 0;
-    uint32_t w0 = 0;
-    bool ones = true;
-#pragma unroll
-    for (int e = 0; e < kCarryWordsPerThread; ++e) {
-        size_t j = j0 + e;
-        uint64_t v = (j < nwords ? word_sum(planes, n, j) : 0) + c;
-        uint32_t o = (uint32_t)v;
-        c = v >> 32;
-        if (e == 0) w0 = o;
-        else ones = ones && (o == 0xFFFFFFFFu);
-    }
-    uint32_t f = 0;
-#pragma unroll
-    for (uint32_t cin = 0; cin < 4; ++cin) {
-        uint32_t over = (ones && (uint64_t)w0 + cin > 0xFFFFFFFFull) ? 1u : 0u;
-        f |= ((uint32_t)c + over) << (2 * cin);
-    }
-    return f;
-}
-
-// Inclusive scan of carry maps over the block in thread order; returns the
-// thread's inclusive prefix and leaves the block total in *total.
+// Inclusive scan of carry maps over the block in thread order.  Returns the
+// thread's exclusive prefix; *total receives the block aggregate.
 template <int NT>
 __device__ __forceinline__ uint32_t block_scan_fn(uint32_t f, uint32_t* lds_waves, uint32_t* total) {
     constexpr int W = 64;
     const int lane = threadIdx.x & (W - 1), wave = threadIdx.x / W;
+    uint32_t incl = f;
 #pragma unroll
     for (int off = 1; off < W; off <<= 1) {
-        uint32_t other = __shfl_up(f, off, W);
-        if (lane >= off) f = fn_compose(f, other);
+        uint32_t other = __shfl_up(incl, off, W);
+        if (lane >= off) incl = fn_compose(incl, other);
     }
-    if (lane == W - 1) lds_waves[wave] = f;
+    if (lane == W - 1) lds_waves[wave] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = kFnIdentity;
@@ -356,65 +311,187 @@ __device__ __forceinline__ uint32_t block_scan_fn(uint32_t f, uint32_t* lds_wave
         }
     }
     __syncthreads();
-    if (wave > 0) f = fn_compose(f, lds_waves[wave - 1]);
+    uint32_t excl = __shfl_up(incl, 1, W);
+    if (lane == 0) excl = kFnIdentity;
+    if (wave > 0) excl = fn_compose(excl, lds_waves[wave - 1]);
     *total = lds_waves[NT / W - 1];
-    return f;
+    return excl;
+}
+
+// Garner CRT of one position: the exact value x < p0 p1 p2 < 2^93 as 3 words.
+__device__ __forceinline__ void crt_words(uint32_t r0, uint32_t r1, uint32_t r2, const CrtConst& cc,
+                                          uint32_t& x0, uint32_t& x1, uint32_t& x2) {
+    constexpr uint32_t p0 = kP[0], p1 = kP[1], p2 = kP[2];
+    uint32_t y0 = mul_shoup(r0, cc.k0.x, cc.k0.y, p0);
+    uint32_t y1 = sub_mod(mul_shoup(r1, cc.k1.x, cc.k1.y, p1), mul_shoup(y0, cc.c1.x, cc.c1.y, p1), p1);
+    uint32_t y2 = sub_mod(mul_shoup(r2, cc.k2.x, cc.k2.y, p2), mul_shoup(y0, cc.c20.x, cc.c20.y, p2), p2);
+    y2 = sub_mod(y2, mul_shoup(y1, cc.c21.x, cc.c21.y, p2), p2);
+    // x = y0 + p0 * (y1 + p1 * y2)
+    uint64_t t = (uint64_t)y2 * p1 + y1;
+    uint64_t lo = (uint64_t)(uint32_t)t * p0 + y0;
+    uint64_t hi = (t >> 32) * (uint64_t)p0 + (lo >> 32);
+    x0 = (uint32_t)lo;
+    x1 = (uint32_t)hi;
+    x2 = (uint32_t)(hi >> 32);
+}
+
+// Ordered composition over the block of one map per thread, thread 0 being the
+// latest: returns f_0 o f_1 o ... o f_255 to every thread.
+__device__ __forceinline__ uint32_t block_compose_desc(uint32_t g, uint32_t* lds_waves) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t o = __shfl_down(g, off, 64);
+        if (lane + off < 64) g = fn_compose(g, o);
+    }
+    if (lane == 0) lds_waves[wave] = g;
+    __syncthreads();
+    uint32_t r = kFnIdentity;
+#pragma unroll
+    for (int w = kCarryThreads / 64 - 1; w >= 0; --w) r = fn_compose(lds_waves[w], r);
+    __syncthreads();
+    return r;
 }
 
 __global__ void __launch_bounds__(kCarryThreads)
-    k_carry_fn(const uint32_t* __restrict__ planes, size_t n, size_t nwords, uint32_t* __restrict__ block_fn) {
+    k_crt_carry(const uint32_t* __restrict__ res, size_t n, size_t nwords, uint32_t* __restrict__ out,
+                CrtConst cc, CarrySync* __restrict__ sync, uint64_t* __restrict__ status) {
+    constexpr int WB = kCarryWordsPerBlock;
+    constexpr int WT = kCarryWordsPerThread;
     __shared__ uint32_t waves[kCarryThreads / 64];
-    size_t j0 = (size_t)blockIdx.x * kCarryWordsPerBlock + (size_t)threadIdx.x * kCarryWordsPerThread;
-    uint32_t f = thread_fn(planes, n, nwords, j0);
-    uint32_t total;
-    block_scan_fn<kCarryThreads>(f, waves, &total);
-    if (threadIdx.x == 0) block_fn[blockIdx.x] = total;
-}
+    __shared__ uint32_t sh_blk, sh_epoch, sh_first_prefix, sh_first_wait;
+    if (threadIdx.x == 0) {
+        sh_blk = (uint32_t)atomicAdd(&sync->ticket, 1ull);
+        sh_epoch = __hip_atomic_load(&sync->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t blk = sh_blk, epoch = sh_epoch;
+    const size_t j0 = (size_t)blk * WB + (size_t)WT * threadIdx.x;  // this thread's first word
+    const size_t lim = n < nwords ? n : nwords;  // CRT positions at or past lim are zero
 
-// One workgroup: carry into every block from the maps of the blocks before it.
-__global__ void __launch_bounds__(kScanThreads)
-    k_carry_scan(const uint32_t* __restrict__ block_fn, uint32_t nblk, uint32_t* __restrict__ block_cin) {
-    __shared__ uint32_t waves[kScanThreads / 64];
-    const uint32_t per = (nblk + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = threadIdx.x * per;
-    uint32_t f = kFnIdentity;
-    for (uint32_t i = 0; i < per; ++i)
-        if (b0 + i < nblk) f = fn_compose(block_fn[b0 + i], f);
-    uint32_t total;
-    uint32_t incl = block_scan_fn<kScanThreads>(f, waves, &total);
-    // exclusive prefix = inclusive prefix of the previous thread
-    uint32_t excl = __shfl_up(incl, 1, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    if (lane == 0) excl = wave > 0 ? waves[wave - 1] : kFnIdentity;
-    uint32_t c = fn_apply(excl, 0);
-    for (uint32_t i = 0; i < per; ++i) {
-        if (b0 + i < nblk) {
-            block_cin[b0 + i] = c;
-            c = fn_apply(block_fn[b0 + i], c);
+    // Word sums s_j = X0[j] + X1[j-1] + X2[j-2] for j in [j0, j0 + WT).
+    uint64_t s[WT];
+    uint32_t xm1[3] = {0, 0, 0}, xm2[3] = {0, 0, 0};  // x at positions j-1 and j-2
+    if (j0 >= 2 && j0 - 2 < lim) crt_words(res[j0 - 2], res[n + j0 - 2], res[2 * n + j0 - 2], cc, xm2[0], xm2[1], xm2[2]);
+    if (j0 >= 1 && j0 - 1 < lim) crt_words(res[j0 - 1], res[n + j0 - 1], res[2 * n + j0 - 1], cc, xm1[0], xm1[1], xm1[2]);
+    const bool full = j0 + WT <= lim;
+#pragma unroll
+    for (int q = 0; q < WT / 4; ++q) {
+        uint32_t r0[4], r1[4], r2[4];
+        if (full) {
+            const uint4 a = *reinterpret_cast<const uint4*>(res + j0 + 4 * q);
+            const uint4 b = *reinterpret_cast<const uint4*>(res + n + j0 + 4 * q);
+            const uint4 c = *reinterpret_cast<const uint4*>(res + 2 * n + j0 + 4 * q);
+            r0[0] = a.x; r0[1] = a.y; r0[2] = a.z; r0[3] = a.w;
+            r1[0] = b.x; r1[1] = b.y; r1[2] = b.z; r1[3] = b.w;
+            r2[0] = c.x; r2[1] = c.y; r2[2] = c.z; r2[3] = c.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                size_t k = j0 + 4 * q + u;
+                bool in = k < lim;
+                r0[u] = in ? res[k] : 0u;
+                r1[u] = in ? res[n + k] : 0u;
+                r2[u] = in ? res[2 * n + k] : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = 4 * q + u;
+            uint32_t x0, x1, x2;
+            crt_words(r0[u], r1[u], r2[u], cc, x0, x1, x2);
+            if (j0 + e >= lim) x0 = x1 = x2 = 0;  // r = 0 maps to x = 0 anyway; keep it explicit
+            s[e] = (j0 + e < nwords) ? (uint64_t)x0 + xm1[1] + xm2[2] : 0;
+            xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
+            xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
         }
     }
-}
 
-__global__ void __launch_bounds__(kCarryThreads)
-    k_carry_apply(const uint32_t* __restrict__ planes, size_t n, size_t nwords,
-                  const uint32_t* __restrict__ block_cin, uint32_t* __restrict__ out) {
-    __shared__ uint32_t waves[kCarryThreads / 64];
-    size_t j0 = (size_t)blockIdx.x * kCarryWordsPerBlock + (size_t)threadIdx.x * kCarryWordsPerThread;
-    uint32_t f = thread_fn(planes, n, nwords, j0);
-    uint32_t total;
-    uint32_t incl = block_scan_fn<kCarryThreads>(f, waves, &total);
-    uint32_t excl = __shfl_up(incl, 1, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    if (lane == 0) excl = wave > 0 ? waves[wave - 1] : kFnIdentity;
-    uint64_t c = fn_apply(excl, block_cin[blockIdx.x]);
+    // Carry map of this thread's run (computed with carry-in 0).
+    uint32_t f;
+    {
+        uint64_t c = 0;
+        uint32_t w0 = 0;
+        bool ones = true;
 #pragma unroll
-    for (int e = 0; e < kCarryWordsPerThread; ++e) {
-        size_t j = j0 + e;
-        if (j < nwords) {
-            uint64_t v = word_sum(planes, n, j) + c;
-            out[j] = (uint32_t)v;
+        for (int e = 0; e < WT; ++e) {
+            uint64_t v = s[e] + c;
+            uint32_t o = (uint32_t)v;
             c = v >> 32;
+            if (e == 0) w0 = o;
+            else ones = ones && (o == 0xFFFFFFFFu);
         }
+        f = 0;
+#pragma unroll
+        for (uint32_t cin = 0; cin < 4; ++cin) {
+            uint32_t over = (ones && (uint64_t)w0 + cin > 0xFFFFFFFFull) ? 1u : 0u;
+            f |= ((uint32_t)c + over) << (2 * cin);
+        }
+    }
+    uint32_t agg;
+    const uint32_t excl = block_scan_fn<kCarryThreads>(f, waves, &agg);
+    __syncthreads();  // block_scan_fn's lds_waves are reused below
+
+    // Look-back: each round, thread t reads the status of block hi - t; the maps
+    // up to the nearest published inclusive prefix are composed in order.
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&status[blk], status_word(epoch, blk == 0 ? kFlagPrefix : kFlagAggregate, agg),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t before = kFnIdentity;  // composition of every earlier block's map
+    long long hi = (long long)blk - 1;
+    uint32_t spins = 0;
+    while (hi >= 0) {
+        const long long idx = hi - (long long)threadIdx.x;
+        uint32_t flag = (uint32_t)kFlagPrefix, fm = kFnIdentity;  // before block 0: identity
+        if (idx >= 0) {
+            uint64_t v = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            flag = ((uint32_t)(v >> 32) == epoch) ? (uint32_t)((v >> 8) & 0xFF) : 0u;
+            fm = (uint32_t)(v & 0xFF);
+        }
+        if (threadIdx.x == 0) {
+            sh_first_prefix = kCarryThreads;
+            sh_first_wait = kCarryThreads;
+        }
+        __syncthreads();
+        if (flag == (uint32_t)kFlagPrefix) atomicMin(&sh_first_prefix, threadIdx.x);
+        if (flag == 0) atomicMin(&sh_first_wait, threadIdx.x);
+        __syncthreads();
+        const uint32_t first_prefix = sh_first_prefix, first_wait = sh_first_wait;
+        __syncthreads();
+        if (first_wait <= first_prefix && first_wait < kCarryThreads) {
+            if (++spins >= kLookbackSpinLimit) {  // cannot happen while blocks make progress
+                if (threadIdx.x == 0) sync->error = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const uint32_t g = threadIdx.x <= first_prefix ? fm : kFnIdentity;
+        before = fn_compose(before, block_compose_desc(g, waves));
+        if (first_prefix < kCarryThreads) break;
+        hi -= kCarryThreads;
+    }
+    if (threadIdx.x == 0 && blk > 0)
+        __hip_atomic_store(&status[blk], status_word(epoch, kFlagPrefix, fn_compose(agg, before)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    // Final words with the thread's true carry-in.
+    uint64_t c = fn_apply(excl, fn_apply(before, 0));
+    uint32_t w[WT];
+#pragma unroll
+    for (int e = 0; e < WT; ++e) {
+        uint64_t v = s[e] + c;
+        w[e] = (uint32_t)v;
+        c = v >> 32;
+    }
+    if (j0 + WT <= nwords) {
+#pragma unroll
+        for (int q = 0; q < WT / 4; ++q)
+            *reinterpret_cast<uint4*>(out + j0 + 4 * q) = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < WT; ++e)
+            if (j0 + e < nwords) out[j0 + e] = w[e];
     }
 }
 
@@ -506,13 +583,13 @@ hipError_t launch_row_convolution(const Plan& pl, uint32_t* bufA, const uint32_t
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inverse_columns_crt(const Plan& pl, const uint32_t* bufA, uint32_t* planes,
-                                      const Twiddles& tw, const CrtConst& cc, hipStream_t s) {
-    dim3 grid(1u << (pl.log2c - pl.log2w));
+hipError_t launch_inverse_columns(const Plan& pl, uint32_t* bufA, const Twiddles& tw, CarrySync* sync,
+                                  hipStream_t s) {
+    dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes);
 #define PINTT_CASE(LR, LW)                                                                          \
     if (pl.log2r == LR && pl.log2w == LW) {                                                         \
-        hipLaunchKernelGGL((k_col_inv_crt<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, bufA, \
-                           planes, pl.log2c, tw, cc);                                               \
+        hipLaunchKernelGGL((k_col_inv<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, bufA,    \
+                           pl.log2c, tw, sync);                                                     \
         return hipGetLastError();                                                                   \
     }
     PINTT_COL_CASES(PINTT_CASE)
@@ -520,13 +597,11 @@ hipError_t launch_inverse_columns_crt(const Plan& pl, const uint32_t* bufA, uint
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_carry(const uint32_t* planes, size_t n, uint32_t* out_words, size_t nwords,
-                        uint32_t* block_fn, uint32_t* block_cin, hipStream_t s) {
+hipError_t launch_crt_carry(const uint32_t* res, size_t n, uint32_t* out_words, size_t nwords,
+                            const CrtConst& cc, CarrySync* sync, uint64_t* status, hipStream_t s) {
     uint32_t nblk = (uint32_t)((nwords + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock);
-    hipLaunchKernelGGL(k_carry_fn, dim3(nblk), dim3(kCarryThreads), 0, s, planes, n, nwords, block_fn);
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(kScanThreads), 0, s, block_fn, nblk, block_cin);
-    hipLaunchKernelGGL(k_carry_apply, dim3(nblk), dim3(kCarryThreads), 0, s, planes, n, nwords,
-                       block_cin, out_words);
+    hipLaunchKernelGGL(k_crt_carry, dim3(nblk), dim3(kCarryThreads), 0, s, res, n, nwords, out_words, cc,
+                       sync, status);
     return hipGetLastError();
 }
 
