@@ -152,6 +152,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -173,6 +175,12 @@ def main() -> int:
     dout, dsq = hiprt.DeviceBuffer(8 * (an + bn)), hiprt.DeviceBuffer(16 * an)
     stream = hiprt.Stream()
 
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = hiprt.DeviceBuffer(nbytes), hiprt.DeviceBuffer(nbytes)
+        hiprt.check(hiprt.runtime().hipMemcpy(dst.ptr, src.ptr, nbytes, hiprt.D2D), "calibration copy")
+        src.close()
+        dst.close()
     ms_mul = time_gpu(stream, da, an, db, bn, dout, args.steps, args.warmup)
     gpu_out = dout.to_array()
     ms_sqr = time_gpu(stream, da, an, da, an, dsq, args.steps, args.warmup)

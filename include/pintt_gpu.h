@@ -78,6 +78,22 @@ size_t gpu_ntt_max_limbs(void);
 /* Message for the last failure on the calling thread ("" if none). */
 const char* gpu_ntt_last_error(void);
 
+/* Tile plans.  A transform of length 2^n splits into R x C (R = 2^log2r
+ * column length, C = 2^(n - log2r) row length) with 2^log2w columns per
+ * column tile and 2^log2rows rows per row tile.  By default the library times
+ * every compiled plan the first time a length is used and keeps the fastest
+ * (set PINTT_AUTOTUNE=0 for the fixed heuristic).  Tuning waits on the device,
+ * so call gpu_ntt_autotune beforehand when products will be enqueued inside a
+ * stream capture.  No reference counterpart: the reference had one fixed
+ * radix-2 layout (src/gpu/int_ntt.cu:192-220). */
+int gpu_ntt_autotune(int log2n_min, int log2n_max);
+int gpu_ntt_get_plan(int log2n, int* log2r, int* log2w, int* log2rows, float* ms);
+/* Force a plan (log2r <= 0 selects the heuristic); PINTT_ERR_ARGS if that
+ * tile shape is not compiled in. */
+int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows);
+/* Writes up to max_plans (log2r, log2w, log2rows) triples; returns the count. */
+int gpu_ntt_list_plans(int log2n, int* out, int max_plans);
+
 /* Per-pass timing with HIP events on the product's stream.  When enabled,
  * every product records events around its passes; gpu_ntt_pass_times writes
  * the last product's times in milliseconds (waiting for it) in the order

@@ -46,6 +46,14 @@ ABI = {
     "gpu_ntt_last_error": (ctypes.c_char_p, []),
     "gpu_ntt_set_timing": (None, [ctypes.c_int]),
     "gpu_ntt_pass_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "gpu_ntt_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "gpu_ntt_get_plan": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+         ctypes.POINTER(ctypes.c_float)],
+    ),
+    "gpu_ntt_set_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "gpu_ntt_list_plans": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
 }
 
 PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns", "crt_carry")
@@ -117,6 +125,30 @@ def mul_device(d_a: int, an: int, d_b: int, bn: int, d_out: int, stream: int = 0
 
 def transform_log2(an: int, bn: int) -> int:
     return load_library().gpu_ntt_transform_log2(an, bn)
+
+
+def list_plans(log2n: int) -> list:
+    buf = (ctypes.c_int * 3 * 256)()
+    k = load_library().gpu_ntt_list_plans(log2n, ctypes.cast(buf, ctypes.POINTER(ctypes.c_int)), 256)
+    return [tuple(buf[i]) for i in range(k)]
+
+
+def get_plan(log2n: int) -> dict:
+    lib = load_library()
+    r, w, rows, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+    _check(lib, lib.gpu_ntt_get_plan(log2n, ctypes.byref(r), ctypes.byref(w), ctypes.byref(rows), ctypes.byref(ms)))
+    return {"log2n": log2n, "log2r": r.value, "log2c": log2n - r.value, "log2w": w.value,
+            "log2rows": rows.value, "tuned_ms": ms.value}
+
+
+def set_plan(log2n: int, log2r: int, log2w: int, log2rows: int) -> None:
+    lib = load_library()
+    _check(lib, lib.gpu_ntt_set_plan(log2n, log2r, log2w, log2rows))
+
+
+def autotune(lo: int, hi: int) -> None:
+    lib = load_library()
+    _check(lib, lib.gpu_ntt_autotune(lo, hi))
 
 
 def set_timing(enable: bool) -> None:

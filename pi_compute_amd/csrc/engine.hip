@@ -15,9 +15,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "ntt_kernels.h"
 
@@ -47,6 +49,15 @@ struct Engine {
     bool timing = false;
     hipEvent_t ev[5] = {};
     bool have_times = false;
+    // Tile plan per transform length: tuned on the device at first use
+    // (unless disabled), or set explicitly.
+    bool autotune = true;
+    bool have_plan[kMaxLog2N + 1] = {};
+    Plan plan[kMaxLog2N + 1] = {};
+    float plan_ms[kMaxLog2N + 1] = {};
+    size_t tune_cap = 0;  // limbs in each tuning operand buffer
+    uint64_t* tune_in = nullptr;
+    uint64_t* tune_out = nullptr;
 };
 
 Engine g;
@@ -117,6 +128,8 @@ int ensure_init(int device) {
     PINTT_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     PINTT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
     for (auto& e : g.ev) PINTT_HIP(hipEventCreate(&e));
+    const char* at = std::getenv("PINTT_AUTOTUNE");
+    g.autotune = !(at && at[0] == '0');
     g.device = dev;
     g.ready = true;
     return PINTT_OK;
@@ -183,34 +196,111 @@ Twiddles twiddles() {
     return t;
 }
 
+std::vector<Plan> candidate_plans(int log2n) {
+    std::vector<Plan> out;
+    for (int r = 5; r <= 11; ++r) {
+        int c = log2n - r;
+        if (c < 5 || c > 14) continue;
+        for (int w = 3; w <= 6; ++w)
+            for (int rows = 0; rows <= 8; ++rows) {
+                Plan pl{log2n, r, c, w, rows};
+                if (plan_valid(pl)) out.push_back(pl);
+            }
+    }
+    return out;
+}
+
+// Enqueue one product on `s` with tile plan `pl`; device pointers throughout.
+int enqueue_product(const Plan& pl, const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
+                    uint64_t* d_out, bool square, hipStream_t s, bool timed) {
+    const int log2n = pl.log2n;
+    const size_t nwords = 2 * (an + bn);
+    const Twiddles tw = twiddles();
+    const size_t n = (size_t)1 << log2n;
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
+    PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const uint32_t*>(d_a), (uint32_t)(2 * an),
+                                     reinterpret_cast<const uint32_t*>(d_b), (uint32_t)(2 * bn),
+                                     g.bufA, g.bufB, square ? 1 : 2, tw, s));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
+    PINTT_HIP(launch_row_convolution(pl, g.bufA, g.bufB, square ? 1 : 0, tw, s));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
+    PINTT_HIP(launch_inverse_columns(pl, g.bufA, tw, g.sync, s));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
+    PINTT_HIP(launch_crt_carry(g.bufA, n, reinterpret_cast<uint32_t*>(d_out), nwords, crt_constants(log2n),
+                               g.sync, g.status, s));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
+    PINTT_HIP(hipEventRecord(g.done, s));
+    return PINTT_OK;
+}
+
+// Time every compiled tile plan for length 2^log2n on synthetic operands
+// (an = bn = N/4 limbs) and keep the fastest.  Runs on the library stream and
+// waits for it, so it must not happen inside a stream capture.
+int tune(int log2n) {
+    const size_t n = (size_t)1 << log2n;
+    const size_t an = n / 4, bn = n / 4;
+    int rc = ensure_capacity(log2n, 2 * (an + bn));
+    if (rc) return rc;
+    if (g.tune_cap < an) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        if ((rc = grow(g.tune_in, 2 * an))) return rc;
+        if ((rc = grow(g.tune_out, 2 * an))) return rc;
+        PINTT_HIP(hipMemset(g.tune_in, 0xA7, 2 * an * sizeof(uint64_t)));
+        g.tune_cap = an;
+    }
+    Plan best = make_plan(log2n);
+    float best_ms = 1e30f;
+    for (const Plan& pl : candidate_plans(log2n)) {
+        const int reps = log2n >= 22 ? 3 : 5;
+        // warm-up, then `reps` timed products
+        if ((rc = enqueue_product(pl, g.tune_in, an, g.tune_in + an, bn, g.tune_out, false, g.stream, false)))
+            return rc;
+        PINTT_HIP(hipEventRecord(g.ev[0], g.stream));
+        for (int i = 0; i < reps; ++i)
+            if ((rc = enqueue_product(pl, g.tune_in, an, g.tune_in + an, bn, g.tune_out, false, g.stream, false)))
+                return rc;
+        PINTT_HIP(hipEventRecord(g.ev[4], g.stream));
+        PINTT_HIP(hipEventSynchronize(g.ev[4]));
+        float ms = 0.f;
+        PINTT_HIP(hipEventElapsedTime(&ms, g.ev[0], g.ev[4]));
+        ms /= reps;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = pl;
+        }
+    }
+    g.plan[log2n] = best;
+    g.plan_ms[log2n] = best_ms;
+    g.have_plan[log2n] = true;
+    return PINTT_OK;
+}
+
+int plan_for(int log2n, Plan* out) {
+    if (!g.have_plan[log2n]) {
+        if (g.autotune) {
+            int rc = tune(log2n);
+            if (rc) return rc;
+        } else {
+            g.plan[log2n] = make_plan(log2n);
+            g.have_plan[log2n] = true;
+        }
+    }
+    *out = g.plan[log2n];
+    return PINTT_OK;
+}
+
 // Enqueue one product on `s`; operands and output are device pointers.
 int run_product(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
                 bool square, hipStream_t s) {
     const int log2n = log2_for(an, bn);
     if (log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
-    const size_t nwords = 2 * (an + bn);
-    int rc = ensure_capacity(log2n, nwords);
+    int rc = ensure_capacity(log2n, 2 * (an + bn));
     if (rc) return rc;
-    const Plan pl = make_plan(log2n);
-    const Twiddles tw = twiddles();
-    const size_t n = (size_t)1 << log2n;
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
-    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[0], s));
-    PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const uint32_t*>(d_a), (uint32_t)(2 * an),
-                                     reinterpret_cast<const uint32_t*>(d_b), (uint32_t)(2 * bn),
-                                     g.bufA, g.bufB, square ? 1 : 2, tw, s));
-    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[1], s));
-    PINTT_HIP(launch_row_convolution(pl, g.bufA, g.bufB, square ? 1 : 0, tw, s));
-    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[2], s));
-    PINTT_HIP(launch_inverse_columns(pl, g.bufA, tw, g.sync, s));
-    if (g.timing) PINTT_HIP(hipEventRecord(g.ev[3], s));
-    PINTT_HIP(launch_crt_carry(g.bufA, n, reinterpret_cast<uint32_t*>(d_out), nwords, crt_constants(log2n),
-                               g.sync, g.status, s));
-    if (g.timing) {
-        PINTT_HIP(hipEventRecord(g.ev[4], s));
-        g.have_times = true;
-    }
-    PINTT_HIP(hipEventRecord(g.done, s));
+    Plan pl;
+    if ((rc = plan_for(log2n, &pl))) return rc;
+    if ((rc = enqueue_product(pl, d_a, an, d_b, bn, d_out, square, s, g.timing))) return rc;
+    if (g.timing) g.have_times = true;
     return PINTT_OK;
 }
 
@@ -272,6 +362,8 @@ void gpu_ntt_shutdown(void) {
     (void)hipFree(g.d_a);
     (void)hipFree(g.d_b);
     (void)hipFree(g.d_out);
+    (void)hipFree(g.tune_in);
+    (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(g.done);
     (void)hipStreamDestroy(g.stream);
@@ -306,6 +398,55 @@ int gpu_ntt_transform_log2(size_t an, size_t bn) {
 size_t gpu_ntt_max_limbs(void) { return ((size_t)1 << (kMaxLog2N - 1)); }
 
 const char* gpu_ntt_last_error(void) { return t_error.c_str(); }
+
+int gpu_ntt_autotune(int log2n_min, int log2n_max) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc = ensure_init(-1);
+    if (rc) return rc;
+    if (log2n_min < kMinLog2N) log2n_min = kMinLog2N;
+    if (log2n_max > kMaxLog2N) log2n_max = kMaxLog2N;
+    PINTT_HIP(hipSetDevice(g.device));
+    for (int k = log2n_min; k <= log2n_max; ++k)
+        if ((rc = tune(k))) return rc;
+    return PINTT_OK;
+}
+
+int gpu_ntt_get_plan(int log2n, int* log2r, int* log2w, int* log2rows, float* ms) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (log2n < kMinLog2N || log2n > kMaxLog2N) return fail(PINTT_ERR_ARGS, "length out of range");
+    Plan pl = g.have_plan[log2n] ? g.plan[log2n] : make_plan(log2n);
+    if (log2r) *log2r = pl.log2r;
+    if (log2w) *log2w = pl.log2w;
+    if (log2rows) *log2rows = pl.log2rows;
+    if (ms) *ms = g.have_plan[log2n] ? g.plan_ms[log2n] : 0.f;
+    return PINTT_OK;
+}
+
+int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (log2n < kMinLog2N || log2n > kMaxLog2N) return fail(PINTT_ERR_ARGS, "length out of range");
+    Plan pl{log2n, log2r, log2n - log2r, log2w, log2rows};
+    if (log2r <= 0) pl = make_plan(log2n);
+    if (!plan_valid(pl)) return fail(PINTT_ERR_ARGS, "tile plan not compiled into this library");
+    g.plan[log2n] = pl;
+    g.plan_ms[log2n] = 0.f;
+    g.have_plan[log2n] = true;
+    return PINTT_OK;
+}
+
+int gpu_ntt_list_plans(int log2n, int* out, int max_plans) {
+    if (log2n < kMinLog2N || log2n > kMaxLog2N) return 0;
+    std::vector<Plan> c = candidate_plans(log2n);
+    int k = 0;
+    for (const Plan& pl : c) {
+        if (k >= max_plans) break;
+        out[3 * k] = pl.log2r;
+        out[3 * k + 1] = pl.log2w;
+        out[3 * k + 2] = pl.log2rows;
+        ++k;
+    }
+    return k;
+}
 
 void gpu_ntt_set_timing(int enable) {
     std::lock_guard<std::mutex> lock(g_mu);

@@ -1,0 +1,300 @@
+// ntt_carry.hip - CRT + carry pass and the twiddle-table builder (gfx950).
+#include "ntt_device.h"
+
+namespace pintt {
+namespace {
+
+// Pass 4: CRT and carry propagation in one pass.
+//
+// Word j of the product before carries is s_j = X0[j] + X1[j-1] + X2[j-2]
+// < 3 * 2^32, where X0..X2 are the 32-bit words of the CRT value at position
+// k.  Along any run of words the carry stays below 4, so the effect of a run on
+// an incoming carry c in {0,1,2,3} is a map {0..3} -> {0..3}, packed as four
+// 2-bit entries.  Maps compose associatively: a wave-shuffle scan inside the
+// block, then a decoupled look-back across blocks (blocks are numbered in the
+// order they start, from a ticket counter, so every block only waits for
+// blocks that already run and publish their aggregate before waiting).
+// A status word carries epoch, flag and map in one 64-bit atomic, so relaxed
+// atomics suffice: release/acquire would add L2 write-backs and invalidations
+// on every publication and look-back round.
+
+__device__ __forceinline__ uint32_t fn_apply(uint32_t f, uint32_t c) { return (f >> (2 * c)) & 3u; }
+
+// later o earlier
+__device__ __forceinline__ uint32_t fn_compose(uint32_t later, uint32_t earlier) {
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c) r |= fn_apply(later, fn_apply(earlier, c)) << (2 * c);
+    return r;
+}
+
+constexpr uint32_t kFnIdentity = 0u | (1u << 2) | (2u << 4) | (3u << 6);
+constexpr uint64_t kFlagAggregate = 1, kFlagPrefix = 2;
+constexpr uint32_t kLookbackSpinLimit = 1u << 26;
+
+__device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint64_t flag, uint32_t fn) {
+    return ((uint64_t)epoch << 32) | (flag << 8) | fn;
+}
+
+// Inclusive scan of carry maps over the block in thread order.  Returns the
+// thread's exclusive prefix; *total receives the block aggregate.
+template <int NT>
+__device__ __forceinline__ uint32_t block_scan_fn(uint32_t f, uint32_t* lds_waves, uint32_t* total) {
+    constexpr int W = 64;
+    const int lane = threadIdx.x & (W - 1), wave = threadIdx.x / W;
+    uint32_t incl = f;
+#pragma unroll
+    for (int off = 1; off < W; off <<= 1) {
+        uint32_t other = __shfl_up(incl, off, W);
+        if (lane >= off) incl = fn_compose(incl, other);
+    }
+    if (lane == W - 1) lds_waves[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = kFnIdentity;
+        for (int i = 0; i < NT / W; ++i) {
+            acc = fn_compose(lds_waves[i], acc);
+            lds_waves[i] = acc;  // inclusive prefix over waves
+        }
+    }
+    __syncthreads();
+    uint32_t excl = __shfl_up(incl, 1, W);
+    if (lane == 0) excl = kFnIdentity;
+    if (wave > 0) excl = fn_compose(excl, lds_waves[wave - 1]);
+    *total = lds_waves[NT / W - 1];
+    return excl;
+}
+
+// Garner CRT of one position: the exact value x < p0 p1 p2 < 2^93 as 3 words.
+__device__ __forceinline__ void crt_words(uint32_t r0, uint32_t r1, uint32_t r2, const CrtConst& cc,
+                                          uint32_t& x0, uint32_t& x1, uint32_t& x2) {
+    constexpr uint32_t p0 = kP[0], p1 = kP[1], p2 = kP[2];
+    uint32_t y0 = mul_shoup(r0, cc.k0.x, cc.k0.y, p0);
+    uint32_t y1 = sub_mod(mul_shoup(r1, cc.k1.x, cc.k1.y, p1), mul_shoup(y0, cc.c1.x, cc.c1.y, p1), p1);
+    uint32_t y2 = sub_mod(mul_shoup(r2, cc.k2.x, cc.k2.y, p2), mul_shoup(y0, cc.c20.x, cc.c20.y, p2), p2);
+    y2 = sub_mod(y2, mul_shoup(y1, cc.c21.x, cc.c21.y, p2), p2);
+    // x = y0 + p0 * (y1 + p1 * y2)
+    uint64_t t = (uint64_t)y2 * p1 + y1;
+    uint64_t lo = (uint64_t)(uint32_t)t * p0 + y0;
+    uint64_t hi = (t >> 32) * (uint64_t)p0 + (lo >> 32);
+    x0 = (uint32_t)lo;
+    x1 = (uint32_t)hi;
+    x2 = (uint32_t)(hi >> 32);
+}
+
+// Ordered composition over the block of one map per thread, thread 0 being the
+// latest: returns f_0 o f_1 o ... o f_255 to every thread.
+__device__ __forceinline__ uint32_t block_compose_desc(uint32_t g, uint32_t* lds_waves) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t o = __shfl_down(g, off, 64);
+        if (lane + off < 64) g = fn_compose(g, o);
+    }
+    if (lane == 0) lds_waves[wave] = g;
+    __syncthreads();
+    uint32_t r = kFnIdentity;
+#pragma unroll
+    for (int w = kCarryThreads / 64 - 1; w >= 0; --w) r = fn_compose(lds_waves[w], r);
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kCarryThreads)
+    k_crt_carry(const uint32_t* __restrict__ res, size_t n, size_t nwords, uint32_t* __restrict__ out,
+                CrtConst cc, CarrySync* __restrict__ sync, uint64_t* __restrict__ status) {
+    constexpr int WB = kCarryWordsPerBlock;
+    constexpr int WT = kCarryWordsPerThread;
+    __shared__ uint32_t waves[kCarryThreads / 64];
+    __shared__ uint32_t sh_blk, sh_epoch, sh_first_prefix, sh_first_wait;
+    if (threadIdx.x == 0) {
+        sh_blk = (uint32_t)atomicAdd(&sync->ticket, 1ull);
+        sh_epoch = __hip_atomic_load(&sync->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t blk = sh_blk, epoch = sh_epoch;
+    const size_t j0 = (size_t)blk * WB + (size_t)WT * threadIdx.x;  // this thread's first word
+    const size_t lim = n < nwords ? n : nwords;  // CRT positions at or past lim are zero
+
+    // Word sums s_j = X0[j] + X1[j-1] + X2[j-2] for j in [j0, j0 + WT).
+    uint64_t s[WT];
+    uint32_t xm1[3] = {0, 0, 0}, xm2[3] = {0, 0, 0};  // x at positions j-1 and j-2
+    if (j0 >= 2 && j0 - 2 < lim) crt_words(res[j0 - 2], res[n + j0 - 2], res[2 * n + j0 - 2], cc, xm2[0], xm2[1], xm2[2]);
+    if (j0 >= 1 && j0 - 1 < lim) crt_words(res[j0 - 1], res[n + j0 - 1], res[2 * n + j0 - 1], cc, xm1[0], xm1[1], xm1[2]);
+    const bool full = j0 + WT <= lim;
+#pragma unroll
+    for (int q = 0; q < WT / 4; ++q) {
+        uint32_t r0[4], r1[4], r2[4];
+        if (full) {
+            const uint4 a = *reinterpret_cast<const uint4*>(res + j0 + 4 * q);
+            const uint4 b = *reinterpret_cast<const uint4*>(res + n + j0 + 4 * q);
+            const uint4 c = *reinterpret_cast<const uint4*>(res + 2 * n + j0 + 4 * q);
+            r0[0] = a.x; r0[1] = a.y; r0[2] = a.z; r0[3] = a.w;
+            r1[0] = b.x; r1[1] = b.y; r1[2] = b.z; r1[3] = b.w;
+            r2[0] = c.x; r2[1] = c.y; r2[2] = c.z; r2[3] = c.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                size_t k = j0 + 4 * q + u;
+                bool in = k < lim;
+                r0[u] = in ? res[k] : 0u;
+                r1[u] = in ? res[n + k] : 0u;
+                r2[u] = in ? res[2 * n + k] : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = 4 * q + u;
+            uint32_t x0, x1, x2;
+            crt_words(r0[u], r1[u], r2[u], cc, x0, x1, x2);
+            if (j0 + e >= lim) x0 = x1 = x2 = 0;  // r = 0 maps to x = 0 anyway; keep it explicit
+            s[e] = (j0 + e < nwords) ? (uint64_t)x0 + xm1[1] + xm2[2] : 0;
+            xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
+            xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
+        }
+    }
+
+    // Carry map of this thread's run (computed with carry-in 0).
+    uint32_t f;
+    {
+        uint64_t c = 0;
+        uint32_t w0 = 0;
+        bool ones = true;
+#pragma unroll
+        for (int e = 0; e < WT; ++e) {
+            uint64_t v = s[e] + c;
+            uint32_t o = (uint32_t)v;
+            c = v >> 32;
+            if (e == 0) w0 = o;
+            else ones = ones && (o == 0xFFFFFFFFu);
+        }
+        f = 0;
+#pragma unroll
+        for (uint32_t cin = 0; cin < 4; ++cin) {
+            uint32_t over = (ones && (uint64_t)w0 + cin > 0xFFFFFFFFull) ? 1u : 0u;
+            f |= ((uint32_t)c + over) << (2 * cin);
+        }
+    }
+    uint32_t agg;
+    const uint32_t excl = block_scan_fn<kCarryThreads>(f, waves, &agg);
+    __syncthreads();  // block_scan_fn's lds_waves are reused below
+
+    // Look-back: each round, thread t reads the status of block hi - t; the maps
+    // up to the nearest published inclusive prefix are composed in order.
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&status[blk], status_word(epoch, blk == 0 ? kFlagPrefix : kFlagAggregate, agg),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t before = kFnIdentity;  // composition of every earlier block's map
+    long long hi = (long long)blk - 1;
+    uint32_t spins = 0;
+    while (hi >= 0) {
+        const long long idx = hi - (long long)threadIdx.x;
+        uint32_t flag = (uint32_t)kFlagPrefix, fm = kFnIdentity;  // before block 0: identity
+        if (idx >= 0) {
+            uint64_t v = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            flag = ((uint32_t)(v >> 32) == epoch) ? (uint32_t)((v >> 8) & 0xFF) : 0u;
+            fm = (uint32_t)(v & 0xFF);
+        }
+        if (threadIdx.x == 0) {
+            sh_first_prefix = kCarryThreads;
+            sh_first_wait = kCarryThreads;
+        }
+        __syncthreads();
+        if (flag == (uint32_t)kFlagPrefix) atomicMin(&sh_first_prefix, threadIdx.x);
+        if (flag == 0) atomicMin(&sh_first_wait, threadIdx.x);
+        __syncthreads();
+        const uint32_t first_prefix = sh_first_prefix, first_wait = sh_first_wait;
+        __syncthreads();
+        if (first_wait <= first_prefix && first_wait < kCarryThreads) {
+            if (++spins >= kLookbackSpinLimit) {  // cannot happen while blocks make progress
+                if (threadIdx.x == 0) sync->error = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const uint32_t g = threadIdx.x <= first_prefix ? fm : kFnIdentity;
+        before = fn_compose(before, block_compose_desc(g, waves));
+        if (first_prefix < kCarryThreads) break;
+        hi -= kCarryThreads;
+    }
+    if (threadIdx.x == 0 && blk > 0)
+        __hip_atomic_store(&status[blk], status_word(epoch, kFlagPrefix, fn_compose(agg, before)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    // Final words with the thread's true carry-in.
+    uint64_t c = fn_apply(excl, fn_apply(before, 0));
+    uint32_t w[WT];
+#pragma unroll
+    for (int e = 0; e < WT; ++e) {
+        uint64_t v = s[e] + c;
+        w[e] = (uint32_t)v;
+        c = v >> 32;
+    }
+    if (j0 + WT <= nwords) {
+#pragma unroll
+        for (int q = 0; q < WT / 4; ++q)
+            *reinterpret_cast<uint4*>(out + j0 + 4 * q) = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < WT; ++e)
+            if (j0 + e < nwords) out[j0 + e] = w[e];
+    }
+}
+
+__device__ __forceinline__ uint32_t powmod_dev(uint32_t b, uint32_t e, uint32_t p) {
+    uint64_t r = 1, x = b;
+    while (e) {
+        if (e & 1) r = (r * x) % p;
+        x = (x * x) % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+// Z[i], i = 2^s + b: w_{2^(s+1)}^bitrev_s(b) = w25^(bitrev_s(b) * 2^(24 - s)).
+__global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ inv, uint32_t p,
+                                 uint32_t w25, uint32_t count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t ef = 0;
+    if (i > 1) {
+        int s = 31 - __clz(i);
+        uint32_t b = i - (1u << s);
+        uint32_t rev = __brev(b) >> (32 - s);
+        ef = rev << (24 - s);
+    }
+    uint32_t ei = (ef == 0) ? 0u : ((1u << 25) - ef);
+    uint32_t z = (i == 0) ? 1u : powmod_dev(w25, ef, p);
+    uint32_t zi = (i == 0) ? 1u : powmod_dev(w25, ei, p);
+    fwd[i] = make_uint2(z, (uint32_t)(((uint64_t)z << 32) / p));
+    inv[i] = make_uint2(zi, (uint32_t)(((uint64_t)zi << 32) / p));
+}
+
+}  // namespace
+
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s) {
+    const uint32_t p = kP[prime];
+    // w25: a primitive 2^25-th root of unity, g^((p-1)/2^25).
+    uint64_t r = 1, x = kGen[prime];
+    uint64_t e = (uint64_t)(p - 1) >> 25;
+    while (e) {
+        if (e & 1) r = (r * x) % p;
+        x = (x * x) % p;
+        e >>= 1;
+    }
+    uint32_t count = 1u << log2n;
+    hipLaunchKernelGGL(k_build_twiddles, dim3((count + 255) / 256), dim3(256), 0, s, fwd, inv, p,
+                       (uint32_t)r, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_crt_carry(const uint32_t* res, size_t n, uint32_t* out_words, size_t nwords,
+                            const CrtConst& cc, CarrySync* sync, uint64_t* status, hipStream_t s) {
+    uint32_t nblk = (uint32_t)((nwords + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock);
+    hipLaunchKernelGGL(k_crt_carry, dim3(nblk), dim3(kCarryThreads), 0, s, res, n, nwords, out_words, cc,
+                       sync, status);
+    return hipGetLastError();
+}
+
+}  // namespace pintt

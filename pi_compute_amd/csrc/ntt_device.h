@@ -1,0 +1,153 @@
+// ntt_device.h - device building blocks shared by the NTT passes (gfx950).
+//
+// A workgroup owns a tile of SEQ sequences of length L in LDS; every thread
+// keeps 16 elements in registers and runs up to four butterfly layers on them
+// between LDS exchanges (radix-16 steps).  Forward layers are Cooley-Tukey
+// (natural order in, bit-reversed out), inverse layers Gentleman-Sande; the
+// twiddle of layer s, block b is Z[2^s + b] (see ntt_kernels.h).
+#pragma once
+
+#include "ntt_kernels.h"
+
+namespace pintt {
+__device__ __forceinline__ uint32_t prime_of(int i) {
+    return i == 0 ? kP[0] : (i == 1 ? kP[1] : kP[2]);
+}
+
+// One pad word after every 16 keeps radix-16 strides off a single LDS bank.
+__device__ __forceinline__ uint32_t pad(uint32_t x) { return x + (x >> 4); }
+
+// A tile of SEQ sequences of length L in LDS.  COLS: the column pass, where
+// element (seq w, index r) sits at r * SEQ + w so that neighbouring threads
+// touch neighbouring words; otherwise sequences are stored one after another.
+template <int LOGL_, int LOGSEQ_, bool COLS_>
+struct Tile {
+    static constexpr int LOGL = LOGL_;
+    static constexpr int LOGSEQ = LOGSEQ_;
+    static constexpr bool COLS = COLS_;
+    static constexpr int L = 1 << LOGL;
+    static constexpr int SEQ = 1 << LOGSEQ;
+    static constexpr int NT = L * SEQ / kElemsPerThread;
+    static constexpr int WORDS = L * SEQ + ((L * SEQ) >> 4);
+    __device__ static __forceinline__ uint32_t addr(uint32_t seq, uint32_t idx) {
+        return COLS ? pad((idx << LOGSEQ) + seq) : pad((seq << LOGL) + idx);
+    }
+};
+
+// Group g of a stage covering local layers [S0, S0 + K): the 2^K elements
+// base + t * DD (t < 2^K) of one sequence, DD = L / 2^(S0+K).  `bblk` is the
+// block index of the group at local layer S0.
+template <class TL, int S0, int K>
+struct Stage {
+    static constexpr int G = 1 << K;
+    static constexpr int NG = kElemsPerThread / G;
+    static constexpr int DD = TL::L >> (S0 + K);
+    static constexpr int GPS = TL::L / G;  // groups per sequence
+    __device__ static __forceinline__ void locate(uint32_t gi, uint32_t& seq, uint32_t& base,
+                                                  uint32_t& bblk) {
+        uint32_t grp;
+        if constexpr (TL::COLS) {
+            seq = gi & (TL::SEQ - 1);
+            grp = gi >> TL::LOGSEQ;
+        } else {
+            grp = gi & (GPS - 1);
+            seq = gi / GPS;
+        }
+        uint32_t j = grp % DD;
+        bblk = grp / DD;
+        base = bblk * (DD * G) + j;
+    }
+};
+
+// Forward (Cooley-Tukey) layers [S0, S0+K) on one or two tiles.  Twiddle of
+// local layer S0+l, sub-block `sub` of the group: global layer sb + S0 + l,
+// global block ((row0 + seq) << (S0 + l)) + (bblk << l) + sub, where row0 is
+// the block index of sequence 0 at layer sb (0 for columns, the row for rows).
+template <class TL, int S0, int K, int NB>
+__device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, const uint2* __restrict__ Z,
+                                         int sb, uint32_t row0, uint32_t p) {
+    using ST = Stage<TL, S0, K>;
+#pragma unroll
+    for (int q = 0; q < ST::NG; ++q) {
+        uint32_t seq, base, bblk;
+        ST::locate(q * TL::NT + threadIdx.x, seq, base, bblk);
+        uint32_t blk = TL::COLS ? row0 : row0 + seq;
+        uint32_t v0[ST::G], v1[ST::G];
+#pragma unroll
+        for (int t = 0; t < ST::G; ++t) {
+            v0[t] = t0[TL::addr(seq, base + t * ST::DD)];
+            if constexpr (NB == 2) v1[t] = t1[TL::addr(seq, base + t * ST::DD)];
+        }
+#pragma unroll
+        for (int l = 0; l < K; ++l) {
+            const int h = ST::G >> (l + 1);
+#pragma unroll
+            for (int sub = 0; sub < (1 << l); ++sub) {
+                uint2 z = Z[(1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub];
+#pragma unroll
+                for (int t = 0; t < h; ++t) {
+                    ct_bfly(v0[sub * 2 * h + t], v0[sub * 2 * h + t + h], z, p);
+                    if constexpr (NB == 2) ct_bfly(v1[sub * 2 * h + t], v1[sub * 2 * h + t + h], z, p);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < ST::G; ++t) {
+            t0[TL::addr(seq, base + t * ST::DD)] = v0[t];
+            if constexpr (NB == 2) t1[TL::addr(seq, base + t * ST::DD)] = v1[t];
+        }
+    }
+}
+
+// Inverse (Gentleman-Sande) layers S0+K-1 down to S0 on one tile.
+template <class TL, int S0, int K>
+__device__ __forceinline__ void gs_stage(uint32_t* t0, const uint2* __restrict__ Zi, int sb,
+                                         uint32_t row0, uint32_t p) {
+    using ST = Stage<TL, S0, K>;
+#pragma unroll
+    for (int q = 0; q < ST::NG; ++q) {
+        uint32_t seq, base, bblk;
+        ST::locate(q * TL::NT + threadIdx.x, seq, base, bblk);
+        uint32_t blk = TL::COLS ? row0 : row0 + seq;
+        uint32_t v[ST::G];
+#pragma unroll
+        for (int t = 0; t < ST::G; ++t) v[t] = t0[TL::addr(seq, base + t * ST::DD)];
+#pragma unroll
+        for (int l = K - 1; l >= 0; --l) {
+            const int h = ST::G >> (l + 1);
+#pragma unroll
+            for (int sub = 0; sub < (1 << l); ++sub) {
+                uint2 z = Zi[(1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub];
+#pragma unroll
+                for (int t = 0; t < h; ++t) gs_bfly(v[sub * 2 * h + t], v[sub * 2 * h + t + h], z, p);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < ST::G; ++t) t0[TL::addr(seq, base + t * ST::DD)] = v[t];
+    }
+}
+
+template <class TL, int S0, int NB>
+__device__ __forceinline__ void ct_run(uint32_t* t0, uint32_t* t1, const uint2* __restrict__ Z,
+                                       int sb, uint32_t row0, uint32_t p) {
+    if constexpr (S0 < TL::LOGL) {
+        constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
+        ct_stage<TL, S0, K, NB>(t0, t1, Z, sb, row0, p);
+        __syncthreads();
+        ct_run<TL, S0 + K, NB>(t0, t1, Z, sb, row0, p);
+    }
+}
+
+template <class TL, int S0>
+__device__ __forceinline__ void gs_run(uint32_t* t0, const uint2* __restrict__ Zi, int sb,
+                                       uint32_t row0, uint32_t p) {
+    if constexpr (S0 < TL::LOGL) {
+        constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
+        gs_run<TL, S0 + K>(t0, Zi, sb, row0, p);
+        gs_stage<TL, S0, K>(t0, Zi, sb, row0, p);
+        __syncthreads();
+    }
+}
+
+
+}  // namespace pintt

@@ -90,7 +90,17 @@ __host__ inline Plan make_plan(int log2n) {
     return pl;
 }
 
-// Launch entry points (defined in ntt_kernels.hip).
+// Tile shapes compiled into the library (ntt_cols.hip / ntt_rows.hip).
+bool col_case_available(int log2r, int log2w);
+bool row_case_available(int log2c, int log2rows);
+
+// A plan with an explicit split, checked against the compiled tile shapes.
+__host__ inline bool plan_valid(const Plan& pl) {
+    return pl.log2r + pl.log2c == pl.log2n && pl.log2w <= pl.log2c && pl.log2rows <= pl.log2r &&
+           col_case_available(pl.log2r, pl.log2w) && row_case_available(pl.log2c, pl.log2rows);
+}
+
+// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip).
 hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s);
 hipError_t launch_forward_columns(const Plan& pl, const uint32_t* src0, uint32_t ncoef0,
                                   const uint32_t* src1, uint32_t ncoef1, uint32_t* dst0,

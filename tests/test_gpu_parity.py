@@ -159,3 +159,33 @@ def test_pass_timing(engine):
         engine.set_timing(False)
     assert set(t) == set(pc.PASS_NAMES)
     assert all(v > 0 for v in t.values())
+
+
+@pytest.mark.parametrize("log2n", [10, 12, 14, 16, 18, 20, 22, 23, 24])
+def test_every_compiled_tile_plan(engine, oracle, log2n):
+    """Force each compiled (split, tile) plan at this length; all must be bit-exact."""
+    plans = engine.list_plans(log2n)
+    assert plans, f"no compiled plan for 2^{log2n}"
+    total = (1 << log2n) // 2
+    a, b = pc.operands(1000 + log2n, 64 * ((2 * total) // 3), 64 * (total - (2 * total) // 3))
+    assert engine.transform_log2(a.size, b.size) == log2n
+    want = oracle.gmp_mul(a, b)
+    want_sq = oracle.gmp_mul(b, b) if log2n <= 20 else None
+    saved = engine.get_plan(log2n)
+    try:
+        for (r, w, rows) in plans:
+            engine.set_plan(log2n, r, w, rows)
+            got = engine.mul(a, b)
+            assert np.array_equal(got, want), f"plan r={r} w={w} rows={rows} at 2^{log2n}"
+            if want_sq is not None:
+                assert np.array_equal(engine.sqr(b), want_sq), f"square plan r={r} w={w} rows={rows}"
+    finally:
+        engine.set_plan(log2n, saved["log2r"], saved["log2w"], saved["log2rows"])
+
+
+def test_autotune_picks_a_compiled_plan(engine):
+    engine.autotune(16, 17)
+    for n in (16, 17):
+        p = engine.get_plan(n)
+        assert (p["log2r"], p["log2w"], p["log2rows"]) in engine.list_plans(n)
+        assert p["tuned_ms"] > 0
