@@ -68,7 +68,28 @@ int gpu_ntt_sqr(const uint64_t* a, size_t an, uint64_t* out);
 int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
                        uint64_t* d_out, void* stream);
 
-/* log2 of the transform length used for an an-limb by bn-limb product,
+/* Batches: count independent products out[i] = a[i] * b[i] in one call
+ * (a[i] == b[i] with an[i] == bn[i] squares).  The engine buckets them by
+ * transform length and runs each bucket as one launch per pass, so a level of
+ * binary splitting costs four launches per distinct length instead of four
+ * per product.  Host-pointer form is synchronous.  Replaces the per-product
+ * calls of BinarySplitting::merge_parallel (src/engine/binary_splitting.cpp:
+ * 161-293), which each paid a full engine round trip. */
+int gpu_ntt_mul_batch(size_t count, const uint64_t* const* a, const size_t* an, const uint64_t* const* b,
+                      const size_t* bn, uint64_t* const* out);
+/* Device-pointer form: enqueued on `stream` (NULL = library stream). */
+int gpu_ntt_mul_batch_device(size_t count, const uint64_t* const* d_a, const size_t* an,
+                             const uint64_t* const* d_b, const size_t* bn, uint64_t* const* d_out,
+                             void* stream);
+
+/* Very unbalanced operands: the longer one may be cut into up to 4 chunks that
+ * share one transform of the shorter one.  mode 1 (default): when the tuned
+ * timings say that is cheaper; 0 (or PINTT_SPLIT=0): never; 2..4: that many
+ * chunks whenever the shapes allow it (for tests).  Products are identical
+ * either way. */
+void gpu_ntt_set_split(int mode);
+
+/* log2 of the transform length an unsplit an-limb by bn-limb product uses,
  * or -1 when it exceeds 2^25. */
 int gpu_ntt_transform_log2(size_t an, size_t bn);
 

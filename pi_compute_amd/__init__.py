@@ -54,6 +54,18 @@ ABI = {
     ),
     "gpu_ntt_set_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gpu_ntt_list_plans": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+    "gpu_ntt_mul_batch": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
+         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p)],
+    ),
+    "gpu_ntt_mul_batch_device": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_void_p),
+         ctypes.c_void_p],
+    ),
+    "gpu_ntt_set_split": (None, [ctypes.c_int]),
 }
 
 PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns", "crt_carry")
@@ -115,6 +127,41 @@ def sqr(a: np.ndarray) -> np.ndarray:
     out = np.empty(2 * a.size, dtype=np.uint64)
     _check(lib, lib.gpu_ntt_sqr(_ptr(a), a.size, _ptr(out)))
     return out
+
+
+def mul_batch(pairs) -> list:
+    """Products of a list of (a, b) limb-array pairs in one batch (host buffers).
+    A pair whose two members are the same array object is squared."""
+    lib = load_library()
+    ops = []
+    for a, b in pairs:
+        same = a is b
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = a if same else np.ascontiguousarray(b, dtype=np.uint64)
+        ops.append((a, b, np.empty(a.size + b.size, dtype=np.uint64)))
+    n = len(ops)
+    pa = (_u64p * n)(*[_ptr(a) for a, _, _ in ops])
+    pb = (_u64p * n)(*[_ptr(b) for _, b, _ in ops])
+    po = (_u64p * n)(*[_ptr(o) for _, _, o in ops])
+    na = (ctypes.c_size_t * n)(*[a.size for a, _, _ in ops])
+    nb = (ctypes.c_size_t * n)(*[b.size for _, b, _ in ops])
+    _check(lib, lib.gpu_ntt_mul_batch(n, pa, na, pb, nb, po))
+    return [o for _, _, o in ops]
+
+
+def mul_batch_device(d_a, an, d_b, bn, d_out, stream: int = 0) -> None:
+    """Enqueue a batch of products on device pointers (lists of equal length)."""
+    lib = load_library()
+    n = len(d_a)
+    vp = ctypes.c_void_p
+    _check(lib, lib.gpu_ntt_mul_batch_device(
+        n, (vp * n)(*d_a), (ctypes.c_size_t * n)(*an), (vp * n)(*d_b), (ctypes.c_size_t * n)(*bn),
+        (vp * n)(*d_out), stream or None))
+
+
+def set_split(mode: int) -> None:
+    """0 never cut operands, 1 cut when cheaper (default), 2..4 force that many chunks."""
+    load_library().gpu_ntt_set_split(int(mode))
 
 
 def mul_device(d_a: int, an: int, d_b: int, bn: int, d_out: int, stream: int = 0) -> None:

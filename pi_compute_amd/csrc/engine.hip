@@ -1,10 +1,19 @@
 // engine.hip - host side of the C ABI (include/pintt_gpu.h).
 //
-// Owns, per process, one device context: a stream, the twiddle tables (one per
-// prime, grown to the largest transform seen), two residue arenas of 3 * N
-// words, the carry scratch and the staging buffers of the host entry points.
-// Products are serialised by a mutex; device-pointer products issued on other
-// streams are ordered behind the previous product with an event.
+// One device context per process: a stream, the twiddle tables (one per
+// prime, grown to the largest transform seen), a residue arena, the carry
+// look-back state, descriptor buffers and staging buffers for host operands.
+//
+// Every call is a batch (a single product is a batch of one):
+//   1. each product becomes a group: the longer operand is cut into m chunks
+//      when that makes the transforms cheaper (unbalanced operands), all
+//      sharing the one transform of the shorter operand;
+//   2. groups are bucketed by transform length N;
+//   3. per bucket, four launches (forward columns, row convolution, inverse
+//      columns, CRT + carry) process every group of the bucket, driven by job
+//      descriptors uploaded once per batch.
+// Calls are serialised by a mutex; work enqueued on a caller's stream is
+// ordered behind the previous batch with an event.
 //
 // Reference counterparts: IntNttEngine::ensure_buffers / TwiddleCache::build
 // (src/gpu/int_ntt.cu:110-186,282-299), which rebuilt the tables whenever the
@@ -14,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -32,20 +42,28 @@ struct Engine {
     bool ready = false;
     int device = -1;
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t done = nullptr;  // end of the last batch
     int tw_log2 = 0;
     uint2* tw_fwd[kPrimes] = {};
     uint2* tw_inv[kPrimes] = {};
-    size_t arena_n = 0;  // per-prime capacity of bufA / bufB
-    uint32_t* bufA = nullptr;
-    uint32_t* bufB = nullptr;
-    size_t carry_cap = 0;  // carry blocks covered by `status`
+    size_t arena_words = 0;
+    uint32_t* arena = nullptr;
+    CarryGlobal* glob = nullptr;
+    size_t ticket_cap = 0;
+    unsigned long long* tickets = nullptr;
+    size_t status_cap = 0;
     uint64_t* status = nullptr;
-    CarrySync* sync = nullptr;
-    size_t io_cap = 0;  // limbs per staging buffer
-    uint64_t* d_a = nullptr;
-    uint64_t* d_b = nullptr;
-    uint64_t* d_out = nullptr;
+    // Ring of descriptor buffers, so the host can prepare a few batches ahead:
+    // slot k is reused only after its previous upload has completed.
+    static constexpr int kDescRing = 8;
+    int desc_next = 0;
+    size_t desc_cap[kDescRing] = {};               // bytes
+    unsigned char* desc_host[kDescRing] = {};      // pinned
+    unsigned char* desc_dev[kDescRing] = {};
+    hipEvent_t desc_done[kDescRing] = {};
+    size_t io_cap = 0;  // limbs
+    uint64_t* io_in = nullptr;
+    uint64_t* io_out = nullptr;
     bool timing = false;
     hipEvent_t ev[5] = {};
     bool have_times = false;
@@ -55,7 +73,8 @@ struct Engine {
     bool have_plan[kMaxLog2N + 1] = {};
     Plan plan[kMaxLog2N + 1] = {};
     float plan_ms[kMaxLog2N + 1] = {};
-    size_t tune_cap = 0;  // limbs in each tuning operand buffer
+    int split_mode = 1;  // 0: never cut operands into chunks, 1: when cheaper, m >= 2: m chunks when valid
+    size_t tune_cap = 0;
     uint64_t* tune_in = nullptr;
     uint64_t* tune_out = nullptr;
 };
@@ -73,10 +92,16 @@ int hip_fail(hipError_t e, const char* where) {
                 std::string(where) + ": " + hipGetErrorString(e));
 }
 
-#define PINTT_HIP(call)                                     \
-    do {                                                    \
-        hipError_t e_ = (call);                             \
-        if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+#define PINTT_HIP(call)                                   \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+#define PINTT_TRY(expr)      \
+    do {                     \
+        int rc_ = (expr);    \
+        if (rc_) return rc_; \
     } while (0)
 
 uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {
@@ -127,17 +152,24 @@ int ensure_init(int device) {
                                              ", the kernels are built for gfx950 only");
     PINTT_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     PINTT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    for (auto& e : g.desc_done) PINTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : g.ev) PINTT_HIP(hipEventCreate(&e));
+    PINTT_HIP(hipMalloc(reinterpret_cast<void**>(&g.glob), sizeof(CarryGlobal)));
+    PINTT_HIP(hipMemset(g.glob, 0, sizeof(CarryGlobal)));
     const char* at = std::getenv("PINTT_AUTOTUNE");
     g.autotune = !(at && at[0] == '0');
+    const char* sp = std::getenv("PINTT_SPLIT");
+    g.split_mode = (sp && sp[0] == '0') ? 0 : 1;
     g.device = dev;
     g.ready = true;
     return PINTT_OK;
 }
 
+// Replace a device allocation by a larger one (contents are not kept).
 template <class T>
 int grow(T*& ptr, size_t count) {
     if (ptr) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
         PINTT_HIP(hipFree(ptr));
         ptr = nullptr;
     }
@@ -145,38 +177,52 @@ int grow(T*& ptr, size_t count) {
     return PINTT_OK;
 }
 
-int ensure_capacity(int log2n, size_t nwords) {
-    int rc;
-    if (log2n > g.tw_log2) {
-        int tl = log2n < 16 ? 16 : log2n;
-        PINTT_HIP(hipStreamSynchronize(g.stream));
-        size_t cnt = (size_t)1 << tl;
-        for (int i = 0; i < kPrimes; ++i) {
-            if ((rc = grow(g.tw_fwd[i], cnt))) return rc;
-            if ((rc = grow(g.tw_inv[i], cnt))) return rc;
-            PINTT_HIP(launch_build_twiddles(g.tw_fwd[i], g.tw_inv[i], i, tl, g.stream));
-        }
-        g.tw_log2 = tl;
+int ensure_twiddles(int log2n) {
+    if (log2n <= g.tw_log2) return PINTT_OK;
+    int tl = log2n < 16 ? 16 : log2n;
+    size_t cnt = (size_t)1 << tl;
+    for (int i = 0; i < kPrimes; ++i) {
+        PINTT_TRY(grow(g.tw_fwd[i], cnt));
+        PINTT_TRY(grow(g.tw_inv[i], cnt));
+        PINTT_HIP(launch_build_twiddles(g.tw_fwd[i], g.tw_inv[i], i, tl, g.stream));
     }
-    size_t n = (size_t)1 << log2n;
-    if (n > g.arena_n) {
-        PINTT_HIP(hipStreamSynchronize(g.stream));
-        if ((rc = grow(g.bufA, kPrimes * n))) return rc;
-        if ((rc = grow(g.bufB, kPrimes * n))) return rc;
-        g.arena_n = n;
+    g.tw_log2 = tl;
+    return PINTT_OK;
+}
+
+int ensure_arena(size_t words) {
+    if (words <= g.arena_words) return PINTT_OK;
+    PINTT_TRY(grow(g.arena, words));
+    g.arena_words = words;
+    return PINTT_OK;
+}
+
+// Look-back state: zero when allocated; stale words never match a later epoch.
+int ensure_carry_state(size_t groups, size_t blocks) {
+    if (groups > g.ticket_cap) {
+        PINTT_TRY(grow(g.tickets, groups));
+        PINTT_HIP(hipMemset(g.tickets, 0, groups * sizeof(unsigned long long)));
+        g.ticket_cap = groups;
     }
-    if (!g.sync) {
-        if ((rc = grow(g.sync, 1))) return rc;
-        PINTT_HIP(hipMemset(g.sync, 0, sizeof(CarrySync)));
+    if (blocks > g.status_cap) {
+        PINTT_TRY(grow(g.status, blocks));
+        PINTT_HIP(hipMemset(g.status, 0, blocks * sizeof(uint64_t)));
+        g.status_cap = blocks;
     }
-    size_t nblk = (nwords + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock;
-    if (nblk > g.carry_cap) {
-        // Status words start at zero; epochs start at 1, so stale words never match.
-        PINTT_HIP(hipStreamSynchronize(g.stream));
-        if ((rc = grow(g.status, nblk))) return rc;
-        PINTT_HIP(hipMemset(g.status, 0, nblk * sizeof(uint64_t)));
-        g.carry_cap = nblk;
-    }
+    return PINTT_OK;
+}
+
+// Ring slot `k` with room for `bytes`, its previous upload finished.
+int ensure_descriptors(int k, size_t bytes) {
+    PINTT_HIP(hipEventSynchronize(g.desc_done[k]));
+    if (bytes <= g.desc_cap[k]) return PINTT_OK;
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    if (g.desc_host[k]) PINTT_HIP(hipHostFree(g.desc_host[k]));
+    g.desc_host[k] = nullptr;
+    size_t cap = std::max(bytes, (size_t)1 << 16);
+    PINTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&g.desc_host[k]), cap, hipHostMallocDefault));
+    PINTT_TRY(grow(g.desc_dev[k], cap));
+    g.desc_cap[k] = cap;
     return PINTT_OK;
 }
 
@@ -210,77 +256,310 @@ std::vector<Plan> candidate_plans(int log2n) {
     return out;
 }
 
-// Enqueue one product on `s` with tile plan `pl`; device pointers throughout.
-int enqueue_product(const Plan& pl, const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
-                    uint64_t* d_out, bool square, hipStream_t s, bool timed) {
-    const int log2n = pl.log2n;
-    const size_t nwords = 2 * (an + bn);
+// ---------------------------------------------------------------------------
+// Batch planning.
+
+struct Request {  // one product, device pointers
+    const uint64_t* a;
+    size_t an;
+    const uint64_t* b;
+    size_t bn;
+    uint64_t* out;
+    bool square;
+};
+
+struct Group {
+    Request r;    // a is the longer operand (swapped if needed)
+    int log2n;
+    uint32_t m;   // chunks of a
+    size_t h;     // limbs per chunk (multiple of 16 when m > 1)
+    size_t nblk;  // carry blocks
+};
+
+int plan_for(int log2n, Plan* out);
+
+// Estimated cost of `transforms` transforms of length 2^k: a third of the
+// tuned time of a full product (three transforms) when known, else N log N.
+double transform_cost(int k, int transforms) {
+    double base = g.have_plan[k] && g.plan_ms[k] > 0 ? g.plan_ms[k] / 3.0
+                                                     : 1e-7 * (double)((size_t)1 << k) * k;
+    return base * transforms;
+}
+
+int plan_group(const Request& req, Group* gr) {
+    Request r = req;
+    if (!r.square && r.an < r.bn) {
+        std::swap(r.a, r.b);
+        std::swap(r.an, r.bn);
+    }
+    gr->r = r;
+    gr->m = 1;
+    gr->h = r.an;
+    gr->log2n = log2_for(r.an, r.bn);
+    if (gr->log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+    if (!r.square && g.split_mode) {
+        if (g.autotune && g.split_mode == 1 && !g.have_plan[gr->log2n]) {
+            Plan tmp;
+            PINTT_TRY(plan_for(gr->log2n, &tmp));
+        }
+        double best = transform_cost(gr->log2n, 3);
+        for (uint32_t m = 2; m <= (uint32_t)kMaxChunks; ++m) {
+            size_t h = (r.an + m - 1) / m;
+            h = (h + 15) & ~(size_t)15;  // chunk offsets stay multiples of 32 coefficients
+            if (h < r.bn || (m - 1) * h >= r.an) continue;  // at most two chunk products per word
+            int k = log2_for(h, r.bn);
+            if (k < 0 || k >= gr->log2n) continue;
+            if (g.autotune && g.split_mode == 1 && !g.have_plan[k]) {
+                Plan tmp;
+                PINTT_TRY(plan_for(k, &tmp));
+            }
+            double c = transform_cost(k, 2 * (int)m + 1);
+            if (g.split_mode >= 2) c = (m == (uint32_t)g.split_mode) ? -1.0 : 1e300;  // forced
+            if (c < best) {
+                best = c;
+                gr->m = m;
+                gr->h = h;
+                gr->log2n = k;
+            }
+        }
+    }
+    gr->nblk = (2 * (r.an + r.bn) + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock;
+    return PINTT_OK;
+}
+
+struct RowLaunch {
+    RowKind kind;
+    size_t off;  // byte offset of its RowJob array in the descriptor blob
+    uint32_t count;
+};
+
+struct BucketLayout {
+    int log2n;
+    std::vector<size_t> groups;                   // indices into the group list
+    size_t fwd_off, inv_off, carry_off;           // byte offsets in the descriptor blob
+    uint32_t nfwd, ninv, nblocks;
+    uint32_t max_m;
+    std::vector<RowLaunch> rows;                  // in launch order
+};
+
+template <class T>
+size_t align_up(size_t x) {
+    return (x + alignof(T) - 1) / alignof(T) * alignof(T);
+}
+
+// Enqueue a batch on `s`.  Every pointer in `reqs` is a device pointer.
+int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
+    if (reqs.empty()) return PINTT_OK;
+    std::vector<Group> groups(reqs.size());
+    for (size_t i = 0; i < reqs.size(); ++i) PINTT_TRY(plan_group(reqs[i], &groups[i]));
+
+    // Buckets by transform length.
+    std::vector<BucketLayout> buckets;
+    for (size_t i = 0; i < groups.size(); ++i) {
+        auto it = std::find_if(buckets.begin(), buckets.end(),
+                               [&](const BucketLayout& b) { return b.log2n == groups[i].log2n; });
+        if (it == buckets.end()) {
+            buckets.push_back(BucketLayout{});
+            it = buckets.end() - 1;
+            it->log2n = groups[i].log2n;
+        }
+        it->groups.push_back(i);
+    }
+    // Sizes: descriptors, arena (buckets run one after another on one stream
+    // and reuse the arena), tickets, status words.
+    size_t bytes = 0, arena = 0, total_groups = 0, total_blocks = 0;
+    int max_log2n = 0;
+    for (auto& b : buckets) {
+        b.nfwd = b.ninv = b.nblocks = b.max_m = 0;
+        size_t slots = 0, nsq = 0;
+        for (size_t gi : b.groups) {
+            const Group& gr = groups[gi];
+            b.nfwd += gr.m + (gr.r.square ? 0 : 1);
+            b.ninv += gr.m;
+            b.nblocks += (uint32_t)gr.nblk;
+            slots += gr.m + (gr.r.square ? 0 : 1);
+            b.max_m = std::max(b.max_m, gr.r.square ? 0u : gr.m);
+            nsq += gr.r.square ? 1 : 0;
+        }
+        bytes = align_up<FwdJob>(bytes);
+        b.fwd_off = bytes;
+        bytes += b.nfwd * sizeof(FwdJob);
+        // Row launches: squares; first chunks (transform b); later chunks (reuse b).
+        b.rows.clear();
+        auto add_rows = [&](RowKind kind, uint32_t count) {
+            if (!count) return;
+            bytes = align_up<RowJob>(bytes);
+            b.rows.push_back(RowLaunch{kind, bytes, count});
+            bytes += count * sizeof(RowJob);
+        };
+        add_rows(kRowSquare, (uint32_t)nsq);
+        for (uint32_t i = 0; i < b.max_m; ++i) {
+            uint32_t cnt = 0;
+            for (size_t gi : b.groups) cnt += (!groups[gi].r.square && groups[gi].m > i) ? 1 : 0;
+            add_rows(i == 0 ? kRowTransformB : kRowReuseB, cnt);
+        }
+        bytes = align_up<InvJob>(bytes);
+        b.inv_off = bytes;
+        bytes += b.ninv * sizeof(InvJob);
+        bytes = align_up<CarryJob>(bytes);
+        b.carry_off = bytes;
+        bytes += b.groups.size() * sizeof(CarryJob);
+        arena = std::max(arena, slots * kPrimes * ((size_t)1 << b.log2n));
+        total_groups += b.groups.size();
+        total_blocks += b.nblocks;
+        max_log2n = std::max(max_log2n, b.log2n);
+    }
+    std::vector<Plan> plans(buckets.size());
+    for (size_t bi = 0; bi < buckets.size(); ++bi) PINTT_TRY(plan_for(buckets[bi].log2n, &plans[bi]));
+    PINTT_TRY(ensure_twiddles(max_log2n));
+    PINTT_TRY(ensure_arena(arena));
+    PINTT_TRY(ensure_carry_state(total_groups, total_blocks));
+    const int ring = g.desc_next;
+    g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
+    PINTT_TRY(ensure_descriptors(ring, bytes));
+
+    // Fill the descriptors.
+    unsigned char* H = g.desc_host[ring];
+    unsigned char* D = g.desc_dev[ring];
+    size_t ticket_next = 0, status_next = 0;
+    for (auto& b : buckets) {
+        const size_t n = (size_t)1 << b.log2n;
+        const size_t slot = kPrimes * n;
+        FwdJob* fj = reinterpret_cast<FwdJob*>(H + b.fwd_off);
+        // row launch i of kind: squares first (if any), then chunk index 0, 1, ...
+        std::vector<RowJob*> rj;
+        std::vector<uint32_t> rfill(b.rows.size(), 0);
+        for (const RowLaunch& rl : b.rows) rj.push_back(reinterpret_cast<RowJob*>(H + rl.off));
+        const size_t first_chunk_launch = (b.rows.size() && b.rows[0].kind == kRowSquare) ? 1 : 0;
+        InvJob* ij = reinterpret_cast<InvJob*>(H + b.inv_off);
+        CarryJob* cj = reinterpret_cast<CarryJob*>(H + b.carry_off);
+        size_t next_slot = 0;
+        uint32_t f = 0, inv = 0, blk0 = 0;
+        for (size_t k = 0; k < b.groups.size(); ++k) {
+            const Group& gr = groups[b.groups[k]];
+            const Request& r = gr.r;
+            CarryJob car{};
+            uint32_t* bslot = nullptr;
+            if (!r.square) bslot = g.arena + slot * (next_slot + gr.m);
+            unsigned long long* ticket = g.tickets + ticket_next++;
+            for (uint32_t i = 0; i < gr.m; ++i) {
+                uint32_t* dst = g.arena + slot * next_slot++;
+                size_t lo = i * gr.h;
+                size_t limbs = std::min(gr.h, r.an - lo);
+                fj[f++] = FwdJob{reinterpret_cast<const uint32_t*>(r.a + lo), (uint32_t)(2 * limbs), dst};
+                if (r.square) {
+                    rj[0][rfill[0]++] = RowJob{dst, nullptr, 0};
+                } else {
+                    size_t li = first_chunk_launch + i;
+                    rj[li][rfill[li]++] = RowJob{dst, bslot, (i == 0 && gr.m > 1) ? 1u : 0u};
+                }
+                ij[inv++] = InvJob{dst, i == 0 ? ticket : nullptr};
+                car.res[i] = dst;
+                car.off[i] = (uint32_t)(2 * lo);
+                car.len[i] = (uint32_t)std::min(n, 2 * limbs + 2 * r.bn - 1);
+            }
+            if (!r.square) {
+                next_slot++;  // bslot
+                fj[f++] = FwdJob{reinterpret_cast<const uint32_t*>(r.b), (uint32_t)(2 * r.bn), bslot};
+            }
+            car.m = gr.m;
+            car.blk0 = blk0;
+            car.out = reinterpret_cast<uint32_t*>(r.out);
+            car.nwords = 2 * (r.an + r.bn);
+            car.ticket = ticket;
+            car.status = g.status + status_next;
+            cj[k] = car;
+            blk0 += (uint32_t)gr.nblk;
+            status_next += gr.nblk;
+        }
+    }
+    PINTT_HIP(hipMemcpyAsync(D, H, bytes, hipMemcpyHostToDevice, g.stream));
+    PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
+
+    // Launches (a caller stream first waits for the upload, which itself is
+    // ordered after the previous batch on the library stream).
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
     const Twiddles tw = twiddles();
-    const size_t n = (size_t)1 << log2n;
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
-    if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
-    PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const uint32_t*>(d_a), (uint32_t)(2 * an),
-                                     reinterpret_cast<const uint32_t*>(d_b), (uint32_t)(2 * bn),
-                                     g.bufA, g.bufB, square ? 1 : 2, tw, s));
-    if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
-    PINTT_HIP(launch_row_convolution(pl, g.bufA, g.bufB, square ? 1 : 0, tw, s));
-    if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
-    PINTT_HIP(launch_inverse_columns(pl, g.bufA, tw, g.sync, s));
-    if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
-    PINTT_HIP(launch_crt_carry(g.bufA, n, reinterpret_cast<uint32_t*>(d_out), nwords, crt_constants(log2n),
-                               g.sync, g.status, s));
-    if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
+    const bool timed = g.timing;
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        const BucketLayout& b = buckets[bi];
+        const Plan& pl = plans[bi];
+        if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
+        PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const FwdJob*>(D + b.fwd_off), b.nfwd, tw, s));
+        if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
+        for (const RowLaunch& rl : b.rows)
+            PINTT_HIP(launch_row_convolution(pl, reinterpret_cast<const RowJob*>(D + rl.off), rl.count, rl.kind,
+                                             tw, s));
+        if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
+        PINTT_HIP(launch_inverse_columns(pl, reinterpret_cast<const InvJob*>(D + b.inv_off), b.ninv, g.glob,
+                                         tw, s));
+        if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
+        PINTT_HIP(launch_crt_carry(reinterpret_cast<const CarryJob*>(D + b.carry_off), (uint32_t)b.groups.size(),
+                                   b.nblocks, b.log2n, crt_constants(b.log2n), g.glob, s));
+        if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
+    }
+    if (timed) g.have_times = true;
     PINTT_HIP(hipEventRecord(g.done, s));
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
     return PINTT_OK;
 }
 
 // Time every compiled tile plan for length 2^log2n on synthetic operands
-// (an = bn = N/4 limbs) and keep the fastest.  Runs on the library stream and
-// waits for it, so it must not happen inside a stream capture.
+// (an = bn = N/4 limbs, no chunking) and keep the fastest.  Runs on the
+// library stream and waits for it, so it must not happen inside a capture.
 int tune(int log2n) {
     const size_t n = (size_t)1 << log2n;
     const size_t an = n / 4, bn = n / 4;
-    int rc = ensure_capacity(log2n, 2 * (an + bn));
-    if (rc) return rc;
     if (g.tune_cap < an) {
-        PINTT_HIP(hipStreamSynchronize(g.stream));
-        if ((rc = grow(g.tune_in, 2 * an))) return rc;
-        if ((rc = grow(g.tune_out, 2 * an))) return rc;
+        PINTT_TRY(grow(g.tune_in, 2 * an));
+        PINTT_TRY(grow(g.tune_out, 2 * an));
         PINTT_HIP(hipMemset(g.tune_in, 0xA7, 2 * an * sizeof(uint64_t)));
         g.tune_cap = an;
     }
+    const int saved_split = g.split_mode;
+    const bool saved_timing = g.timing;
+    g.split_mode = 0;
+    g.timing = false;
     Plan best = make_plan(log2n);
     float best_ms = 1e30f;
+    const std::vector<Request> one{Request{g.tune_in, an, g.tune_in + an, bn, g.tune_out, false}};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess
+                 ? PINTT_OK
+                 : fail(PINTT_ERR_HIP, "hipEventCreate");
     for (const Plan& pl : candidate_plans(log2n)) {
+        if (rc) break;
+        g.plan[log2n] = pl;
+        g.have_plan[log2n] = true;
         const int reps = log2n >= 22 ? 3 : 5;
-        // warm-up, then `reps` timed products
-        if ((rc = enqueue_product(pl, g.tune_in, an, g.tune_in + an, bn, g.tune_out, false, g.stream, false)))
-            return rc;
-        PINTT_HIP(hipEventRecord(g.ev[0], g.stream));
-        for (int i = 0; i < reps; ++i)
-            if ((rc = enqueue_product(pl, g.tune_in, an, g.tune_in + an, bn, g.tune_out, false, g.stream, false)))
-                return rc;
-        PINTT_HIP(hipEventRecord(g.ev[4], g.stream));
-        PINTT_HIP(hipEventSynchronize(g.ev[4]));
+        if ((rc = enqueue_batch(one, g.stream))) break;  // warm-up
+        if (hipEventRecord(t0, g.stream) != hipSuccess) rc = fail(PINTT_ERR_HIP, "hipEventRecord");
+        for (int i = 0; i < reps && !rc; ++i) rc = enqueue_batch(one, g.stream);
         float ms = 0.f;
-        PINTT_HIP(hipEventElapsedTime(&ms, g.ev[0], g.ev[4]));
+        if (!rc && (hipEventRecord(t1, g.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+            rc = fail(PINTT_ERR_HIP, "timing a tile plan");
+        if (rc) break;
         ms /= reps;
         if (ms < best_ms) {
             best_ms = ms;
             best = pl;
         }
     }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    g.split_mode = saved_split;
+    g.timing = saved_timing;
     g.plan[log2n] = best;
-    g.plan_ms[log2n] = best_ms;
+    g.plan_ms[log2n] = best_ms < 1e29f ? best_ms : 0.f;
     g.have_plan[log2n] = true;
-    return PINTT_OK;
+    return rc;
 }
 
 int plan_for(int log2n, Plan* out) {
     if (!g.have_plan[log2n]) {
         if (g.autotune) {
-            int rc = tune(log2n);
-            if (rc) return rc;
+            PINTT_TRY(tune(log2n));
         } else {
             g.plan[log2n] = make_plan(log2n);
             g.have_plan[log2n] = true;
@@ -290,48 +569,69 @@ int plan_for(int log2n, Plan* out) {
     return PINTT_OK;
 }
 
-// Enqueue one product on `s`; operands and output are device pointers.
-int run_product(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
-                bool square, hipStream_t s) {
-    const int log2n = log2_for(an, bn);
-    if (log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
-    int rc = ensure_capacity(log2n, 2 * (an + bn));
-    if (rc) return rc;
-    Plan pl;
-    if ((rc = plan_for(log2n, &pl))) return rc;
-    if ((rc = enqueue_product(pl, d_a, an, d_b, bn, d_out, square, s, g.timing))) return rc;
-    if (g.timing) g.have_times = true;
+int ensure_io(size_t limbs) {
+    if (limbs <= g.io_cap) return PINTT_OK;
+    PINTT_TRY(grow(g.io_in, limbs));
+    PINTT_TRY(grow(g.io_out, limbs));
+    g.io_cap = limbs;
     return PINTT_OK;
 }
 
-int host_product(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out,
-                 bool square) {
-    if (!a || !b || !out || an == 0 || bn == 0) return fail(PINTT_ERR_ARGS, "null pointer or zero length");
-    if ((out < a + an && a < out + an + bn) || (out < b + bn && b < out + an + bn))
-        return fail(PINTT_ERR_ARGS, "output overlaps an operand");
-    std::lock_guard<std::mutex> lock(g_mu);
-    int rc = ensure_init(-1);
-    if (rc) return rc;
-    if (log2_for(an, bn) < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
-    PINTT_HIP(hipSetDevice(g.device));
-    size_t need = an + bn;
-    if (need > g.io_cap) {
-        PINTT_HIP(hipStreamSynchronize(g.stream));
-        if ((rc = grow(g.d_a, need))) return rc;
-        if ((rc = grow(g.d_b, need))) return rc;
-        if ((rc = grow(g.d_out, need))) return rc;
-        g.io_cap = need;
-    }
-    PINTT_HIP(hipMemcpyAsync(g.d_a, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
-    if (!square) PINTT_HIP(hipMemcpyAsync(g.d_b, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
-    rc = run_product(g.d_a, an, square ? g.d_a : g.d_b, bn, g.d_out, square, g.stream);
-    if (rc) return rc;
-    PINTT_HIP(hipMemcpyAsync(out, g.d_out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+int check_device_error() {
     uint32_t err = 0;
-    PINTT_HIP(hipMemcpyAsync(&err, &g.sync->error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
+    PINTT_HIP(hipMemcpyAsync(&err, &g.glob->error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
     PINTT_HIP(hipStreamSynchronize(g.stream));
     if (err) return fail(PINTT_ERR_HIP, "carry look-back gave up (device state corrupted)");
     return PINTT_OK;
+}
+
+bool overlaps(const uint64_t* x, size_t xn, const uint64_t* y, size_t yn) { return x < y + yn && y < x + xn; }
+
+// Host operands: staged into device memory, products computed as one batch,
+// results copied back.
+int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const uint64_t* const* b,
+               const size_t* bn, uint64_t* const* out) {
+    if (!a || !an || !b || !bn || !out) return fail(PINTT_ERR_ARGS, "null pointer");
+    size_t total_in = 0, total_out = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (!a[i] || !b[i] || !out[i] || an[i] == 0 || bn[i] == 0)
+            return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+        if (overlaps(out[i], an[i] + bn[i], a[i], an[i]) || overlaps(out[i], an[i] + bn[i], b[i], bn[i]))
+            return fail(PINTT_ERR_ARGS, "output overlaps an operand");
+        if (an[i] + bn[i] > ((size_t)1 << (kMaxLog2N - 1)))
+            return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+        bool sq = a[i] == b[i] && an[i] == bn[i];
+        total_in += an[i] + (sq ? 0 : bn[i]);
+        total_out += an[i] + bn[i];
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    PINTT_TRY(ensure_io(std::max(total_in, total_out)));
+    std::vector<Request> reqs(count);
+    size_t in_off = 0, out_off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        bool sq = a[i] == b[i] && an[i] == bn[i];
+        uint64_t* da = g.io_in + in_off;
+        PINTT_HIP(hipMemcpyAsync(da, a[i], an[i] * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+        in_off += an[i];
+        uint64_t* db = da;
+        if (!sq) {
+            db = g.io_in + in_off;
+            PINTT_HIP(hipMemcpyAsync(db, b[i], bn[i] * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+            in_off += bn[i];
+        }
+        reqs[i] = Request{da, an[i], db, bn[i], g.io_out + out_off, sq};
+        out_off += an[i] + bn[i];
+    }
+    PINTT_TRY(enqueue_batch(reqs, g.stream));
+    out_off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        PINTT_HIP(hipMemcpyAsync(out[i], g.io_out + out_off, (an[i] + bn[i]) * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost, g.stream));
+        out_off += an[i] + bn[i];
+    }
+    return check_device_error();
 }
 
 }  // namespace
@@ -355,13 +655,17 @@ void gpu_ntt_shutdown(void) {
         (void)hipFree(g.tw_fwd[i]);
         (void)hipFree(g.tw_inv[i]);
     }
-    (void)hipFree(g.bufA);
-    (void)hipFree(g.bufB);
+    (void)hipFree(g.arena);
+    (void)hipFree(g.glob);
+    (void)hipFree(g.tickets);
     (void)hipFree(g.status);
-    (void)hipFree(g.sync);
-    (void)hipFree(g.d_a);
-    (void)hipFree(g.d_b);
-    (void)hipFree(g.d_out);
+    for (int k = 0; k < Engine::kDescRing; ++k) {
+        (void)hipFree(g.desc_dev[k]);
+        if (g.desc_host[k]) (void)hipHostFree(g.desc_host[k]);
+        (void)hipEventDestroy(g.desc_done[k]);
+    }
+    (void)hipFree(g.io_in);
+    (void)hipFree(g.io_out);
     (void)hipFree(g.tune_in);
     (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);
@@ -371,23 +675,41 @@ void gpu_ntt_shutdown(void) {
 }
 
 int gpu_ntt_mul(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out) {
-    return host_product(a, an, b, bn, out, a == b && an == bn);
+    return host_batch(1, &a, &an, &b, &bn, &out);
 }
 
 int gpu_ntt_sqr(const uint64_t* a, size_t an, uint64_t* out) {
-    return host_product(a, an, a, an, out, true);
+    return host_batch(1, &a, &an, &a, &an, &out);
 }
 
-int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
-                       uint64_t* d_out, void* stream) {
-    if (!d_a || !d_b || !d_out || an == 0 || bn == 0)
-        return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+int gpu_ntt_mul_batch(size_t count, const uint64_t* const* a, const size_t* an, const uint64_t* const* b,
+                      const size_t* bn, uint64_t* const* out) {
+    if (count == 0) return PINTT_OK;
+    return host_batch(count, a, an, b, bn, out);
+}
+
+int gpu_ntt_mul_batch_device(size_t count, const uint64_t* const* d_a, const size_t* an,
+                             const uint64_t* const* d_b, const size_t* bn, uint64_t* const* d_out,
+                             void* stream) {
+    if (count == 0) return PINTT_OK;
+    if (!d_a || !an || !d_b || !bn || !d_out) return fail(PINTT_ERR_ARGS, "null pointer");
+    std::vector<Request> reqs(count);
+    for (size_t i = 0; i < count; ++i) {
+        if (!d_a[i] || !d_b[i] || !d_out[i] || an[i] == 0 || bn[i] == 0)
+            return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+        if (an[i] + bn[i] > ((size_t)1 << (kMaxLog2N - 1)))
+            return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+        reqs[i] = Request{d_a[i], an[i], d_b[i], bn[i], d_out[i], d_a[i] == d_b[i] && an[i] == bn[i]};
+    }
     std::lock_guard<std::mutex> lock(g_mu);
-    int rc = ensure_init(-1);
-    if (rc) return rc;
+    PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipSetDevice(g.device));
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
-    return run_product(d_a, an, d_b, bn, d_out, d_a == d_b && an == bn, s);
+    return enqueue_batch(reqs, stream ? static_cast<hipStream_t>(stream) : g.stream);
+}
+
+int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
+                       void* stream) {
+    return gpu_ntt_mul_batch_device(1, &d_a, &an, &d_b, &bn, &d_out, stream);
 }
 
 int gpu_ntt_transform_log2(size_t an, size_t bn) {
@@ -401,13 +723,11 @@ const char* gpu_ntt_last_error(void) { return t_error.c_str(); }
 
 int gpu_ntt_autotune(int log2n_min, int log2n_max) {
     std::lock_guard<std::mutex> lock(g_mu);
-    int rc = ensure_init(-1);
-    if (rc) return rc;
+    PINTT_TRY(ensure_init(-1));
     if (log2n_min < kMinLog2N) log2n_min = kMinLog2N;
     if (log2n_max > kMaxLog2N) log2n_max = kMaxLog2N;
     PINTT_HIP(hipSetDevice(g.device));
-    for (int k = log2n_min; k <= log2n_max; ++k)
-        if ((rc = tune(k))) return rc;
+    for (int k = log2n_min; k <= log2n_max; ++k) PINTT_TRY(tune(k));
     return PINTT_OK;
 }
 
@@ -429,13 +749,12 @@ int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows) {
     if (log2r <= 0) pl = make_plan(log2n);
     if (!plan_valid(pl)) return fail(PINTT_ERR_ARGS, "tile plan not compiled into this library");
     g.plan[log2n] = pl;
-    g.plan_ms[log2n] = 0.f;
     g.have_plan[log2n] = true;
     return PINTT_OK;
 }
 
 int gpu_ntt_list_plans(int log2n, int* out, int max_plans) {
-    if (log2n < kMinLog2N || log2n > kMaxLog2N) return 0;
+    if (log2n < kMinLog2N || log2n > kMaxLog2N || !out) return 0;
     std::vector<Plan> c = candidate_plans(log2n);
     int k = 0;
     for (const Plan& pl : c) {
@@ -446,6 +765,11 @@ int gpu_ntt_list_plans(int log2n, int* out, int max_plans) {
         ++k;
     }
     return k;
+}
+
+void gpu_ntt_set_split(int mode) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g.split_mode = mode < 0 ? 0 : (mode > kMaxChunks ? kMaxChunks : mode);
 }
 
 void gpu_ntt_set_timing(int enable) {

@@ -4,36 +4,40 @@
 namespace pintt {
 namespace {
 
-// Pass 4: CRT and carry propagation in one pass.
+// Pass 4: CRT and carry propagation in one pass, per group.
 //
-// Word j of the product before carries is s_j = X0[j] + X1[j-1] + X2[j-2]
-// < 3 * 2^32, where X0..X2 are the 32-bit words of the CRT value at position
-// k.  Along any run of words the carry stays below 4, so the effect of a run on
-// an incoming carry c in {0,1,2,3} is a map {0..3} -> {0..3}, packed as four
-// 2-bit entries.  Maps compose associatively: a wave-shuffle scan inside the
-// block, then a decoupled look-back across blocks (blocks are numbered in the
-// order they start, from a ticket counter, so every block only waits for
-// blocks that already run and publish their aggregate before waiting).
+// Word j of a group's product before carries is the sum, over its chunk
+// products i, of X0_i[k] + X1_i[k-1] + X2_i[k-2] with k = j - off_i, where
+// X0..X2 are the 32-bit words of the CRT value of chunk product i at position k.
+// At most two chunk products overlap at any word, so a word sum is below
+// 6 * 2^32 and the carry between words stays below 8: the effect of a run of
+// words on an incoming carry c in {0..7} is a map {0..7} -> {0..7}, packed as
+// eight 3-bit entries.  Maps compose associatively: a wave-shuffle scan inside
+// the block, then a decoupled look-back across the group's blocks (numbered in
+// the order they start, from the group's ticket counter, so a block only waits
+// for blocks that already run and publish their aggregate before waiting).
 // A status word carries epoch, flag and map in one 64-bit atomic, so relaxed
 // atomics suffice: release/acquire would add L2 write-backs and invalidations
 // on every publication and look-back round.
 
-__device__ __forceinline__ uint32_t fn_apply(uint32_t f, uint32_t c) { return (f >> (2 * c)) & 3u; }
+constexpr uint32_t kCarryStates = 8;
+__device__ __forceinline__ uint32_t fn_apply(uint32_t f, uint32_t c) { return (f >> (3 * c)) & 7u; }
 
 // later o earlier
 __device__ __forceinline__ uint32_t fn_compose(uint32_t later, uint32_t earlier) {
     uint32_t r = 0;
 #pragma unroll
-    for (uint32_t c = 0; c < 4; ++c) r |= fn_apply(later, fn_apply(earlier, c)) << (2 * c);
+    for (uint32_t c = 0; c < kCarryStates; ++c) r |= fn_apply(later, fn_apply(earlier, c)) << (3 * c);
     return r;
 }
 
-constexpr uint32_t kFnIdentity = 0u | (1u << 2) | (2u << 4) | (3u << 6);
+constexpr uint32_t kFnIdentity = 0u | (1u << 3) | (2u << 6) | (3u << 9) | (4u << 12) | (5u << 15) |
+                                 (6u << 18) | (7u << 21);
 constexpr uint64_t kFlagAggregate = 1, kFlagPrefix = 2;
 constexpr uint32_t kLookbackSpinLimit = 1u << 26;
 
 __device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint64_t flag, uint32_t fn) {
-    return ((uint64_t)epoch << 32) | (flag << 8) | fn;
+    return ((uint64_t)epoch << 32) | (flag << 24) | fn;
 }
 
 // Inclusive scan of carry maps over the block in thread order.  Returns the
@@ -100,43 +104,33 @@ __device__ __forceinline__ uint32_t block_compose_desc(uint32_t g, uint32_t* lds
     return r;
 }
 
-__global__ void __launch_bounds__(kCarryThreads)
-    k_crt_carry(const uint32_t* __restrict__ res, size_t n, size_t nwords, uint32_t* __restrict__ out,
-                CrtConst cc, CarrySync* __restrict__ sync, uint64_t* __restrict__ status) {
-    constexpr int WB = kCarryWordsPerBlock;
-    constexpr int WT = kCarryWordsPerThread;
-    __shared__ uint32_t waves[kCarryThreads / 64];
-    __shared__ uint32_t sh_blk, sh_epoch, sh_first_prefix, sh_first_wait;
-    if (threadIdx.x == 0) {
-        sh_blk = (uint32_t)atomicAdd(&sync->ticket, 1ull);
-        sh_epoch = __hip_atomic_load(&sync->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const uint32_t blk = sh_blk, epoch = sh_epoch;
-    const size_t j0 = (size_t)blk * WB + (size_t)WT * threadIdx.x;  // this thread's first word
-    const size_t lim = n < nwords ? n : nwords;  // CRT positions at or past lim are zero
-
-    // Word sums s_j = X0[j] + X1[j-1] + X2[j-2] for j in [j0, j0 + WT).
-    uint64_t s[WT];
-    uint32_t xm1[3] = {0, 0, 0}, xm2[3] = {0, 0, 0};  // x at positions j-1 and j-2
-    if (j0 >= 2 && j0 - 2 < lim) crt_words(res[j0 - 2], res[n + j0 - 2], res[2 * n + j0 - 2], cc, xm2[0], xm2[1], xm2[2]);
-    if (j0 >= 1 && j0 - 1 < lim) crt_words(res[j0 - 1], res[n + j0 - 1], res[2 * n + j0 - 1], cc, xm1[0], xm1[1], xm1[2]);
-    const bool full = j0 + WT <= lim;
+// Adds the word contributions of one chunk product (residues `res`, N = 2^log2n
+// positions of which `len` are non-zero, first word at `off`) to the thread's
+// word sums s[0 .. WT) of words j0 ...
+template <int WT>
+__device__ __forceinline__ void add_chunk_words(uint64_t* s, const uint32_t* __restrict__ res, size_t n,
+                                                uint32_t len, long long k0, const CrtConst& cc) {
+    // k0: position of word j0 inside this chunk product
+    if (k0 + WT <= 0 || k0 - 2 >= (long long)len) return;
+    uint32_t xm1[3] = {0, 0, 0}, xm2[3] = {0, 0, 0};  // x at positions k-1, k-2
+    if (k0 >= 2 && k0 - 2 < len) crt_words(res[k0 - 2], res[n + k0 - 2], res[2 * n + k0 - 2], cc, xm2[0], xm2[1], xm2[2]);
+    if (k0 >= 1 && k0 - 1 < len) crt_words(res[k0 - 1], res[n + k0 - 1], res[2 * n + k0 - 1], cc, xm1[0], xm1[1], xm1[2]);
+    const bool full = k0 >= 0 && k0 + WT <= (long long)len;
 #pragma unroll
     for (int q = 0; q < WT / 4; ++q) {
         uint32_t r0[4], r1[4], r2[4];
         if (full) {
-            const uint4 a = *reinterpret_cast<const uint4*>(res + j0 + 4 * q);
-            const uint4 b = *reinterpret_cast<const uint4*>(res + n + j0 + 4 * q);
-            const uint4 c = *reinterpret_cast<const uint4*>(res + 2 * n + j0 + 4 * q);
+            const uint4 a = *reinterpret_cast<const uint4*>(res + k0 + 4 * q);
+            const uint4 b = *reinterpret_cast<const uint4*>(res + n + k0 + 4 * q);
+            const uint4 c = *reinterpret_cast<const uint4*>(res + 2 * n + k0 + 4 * q);
             r0[0] = a.x; r0[1] = a.y; r0[2] = a.z; r0[3] = a.w;
             r1[0] = b.x; r1[1] = b.y; r1[2] = b.z; r1[3] = b.w;
             r2[0] = c.x; r2[1] = c.y; r2[2] = c.z; r2[3] = c.w;
         } else {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                size_t k = j0 + 4 * q + u;
-                bool in = k < lim;
+                long long k = k0 + 4 * q + u;
+                bool in = k >= 0 && k < (long long)len;
                 r0[u] = in ? res[k] : 0u;
                 r1[u] = in ? res[n + k] : 0u;
                 r2[u] = in ? res[2 * n + k] : 0u;
@@ -146,13 +140,48 @@ __global__ void __launch_bounds__(kCarryThreads)
         for (int u = 0; u < 4; ++u) {
             const int e = 4 * q + u;
             uint32_t x0, x1, x2;
-            crt_words(r0[u], r1[u], r2[u], cc, x0, x1, x2);
-            if (j0 + e >= lim) x0 = x1 = x2 = 0;  // r = 0 maps to x = 0 anyway; keep it explicit
-            s[e] = (j0 + e < nwords) ? (uint64_t)x0 + xm1[1] + xm2[2] : 0;
+            crt_words(r0[u], r1[u], r2[u], cc, x0, x1, x2);  // residues 0 give x = 0
+            s[e] += (uint64_t)x0 + xm1[1] + xm2[2];
             xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
             xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
         }
     }
+}
+
+// grid = total carry blocks of the launch; jobs sorted by blk0.
+__global__ void __launch_bounds__(kCarryThreads)
+    k_crt_carry(const CarryJob* __restrict__ jobs, uint32_t njobs, int log2n, CrtConst cc,
+                CarryGlobal* __restrict__ glob) {
+    constexpr int WB = kCarryWordsPerBlock;
+    constexpr int WT = kCarryWordsPerThread;
+    __shared__ uint32_t waves[kCarryThreads / 64];
+    __shared__ uint32_t sh_blk, sh_epoch, sh_job, sh_first_prefix, sh_first_wait;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = njobs - 1;  // last job with blk0 <= blockIdx.x
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        sh_job = lo;
+        sh_blk = (uint32_t)atomicAdd(jobs[lo].ticket, 1ull);
+        sh_epoch = __hip_atomic_load(&glob->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const CarryJob& job = jobs[sh_job];
+    const uint32_t blk = sh_blk, epoch = sh_epoch;
+    const size_t n = (size_t)1 << log2n;
+    const uint64_t j0 = (uint64_t)blk * WB + (uint64_t)WT * threadIdx.x;  // this thread's first word
+    uint64_t* status = job.status;
+
+    uint64_t s[WT];
+#pragma unroll
+    for (int e = 0; e < WT; ++e) s[e] = 0;
+    for (uint32_t i = 0; i < job.m; ++i)
+        add_chunk_words<WT>(s, job.res[i], n, job.len[i], (long long)j0 - (long long)job.off[i], cc);
+#pragma unroll
+    for (int e = 0; e < WT; ++e)
+        if (j0 + e >= job.nwords) s[e] = 0;
 
     // Carry map of this thread's run (computed with carry-in 0).
     uint32_t f;
@@ -170,9 +199,9 @@ __global__ void __launch_bounds__(kCarryThreads)
         }
         f = 0;
 #pragma unroll
-        for (uint32_t cin = 0; cin < 4; ++cin) {
+        for (uint32_t cin = 0; cin < kCarryStates; ++cin) {
             uint32_t over = (ones && (uint64_t)w0 + cin > 0xFFFFFFFFull) ? 1u : 0u;
-            f |= ((uint32_t)c + over) << (2 * cin);
+            f |= ((uint32_t)c + over) << (3 * cin);
         }
     }
     uint32_t agg;
@@ -192,8 +221,8 @@ __global__ void __launch_bounds__(kCarryThreads)
         uint32_t flag = (uint32_t)kFlagPrefix, fm = kFnIdentity;  // before block 0: identity
         if (idx >= 0) {
             uint64_t v = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            flag = ((uint32_t)(v >> 32) == epoch) ? (uint32_t)((v >> 8) & 0xFF) : 0u;
-            fm = (uint32_t)(v & 0xFF);
+            flag = ((uint32_t)(v >> 32) == epoch) ? (uint32_t)((v >> 24) & 0xFF) : 0u;
+            fm = (uint32_t)(v & 0xFFFFFF);
         }
         if (threadIdx.x == 0) {
             sh_first_prefix = kCarryThreads;
@@ -207,7 +236,7 @@ __global__ void __launch_bounds__(kCarryThreads)
         __syncthreads();
         if (first_wait <= first_prefix && first_wait < kCarryThreads) {
             if (++spins >= kLookbackSpinLimit) {  // cannot happen while blocks make progress
-                if (threadIdx.x == 0) sync->error = 1;
+                if (threadIdx.x == 0) glob->error = 1;
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
@@ -231,17 +260,19 @@ __global__ void __launch_bounds__(kCarryThreads)
         w[e] = (uint32_t)v;
         c = v >> 32;
     }
-    if (j0 + WT <= nwords) {
+    uint32_t* out = job.out;
+    if (j0 + WT <= job.nwords) {
 #pragma unroll
         for (int q = 0; q < WT / 4; ++q)
             *reinterpret_cast<uint4*>(out + j0 + 4 * q) = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
     } else {
 #pragma unroll
         for (int e = 0; e < WT; ++e)
-            if (j0 + e < nwords) out[j0 + e] = w[e];
+            if (j0 + e < job.nwords) out[j0 + e] = w[e];
     }
 }
 
+// ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t powmod_dev(uint32_t b, uint32_t e, uint32_t p) {
     uint64_t r = 1, x = b;
     while (e) {
@@ -289,11 +320,9 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, h
     return hipGetLastError();
 }
 
-hipError_t launch_crt_carry(const uint32_t* res, size_t n, uint32_t* out_words, size_t nwords,
-                            const CrtConst& cc, CarrySync* sync, uint64_t* status, hipStream_t s) {
-    uint32_t nblk = (uint32_t)((nwords + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock);
-    hipLaunchKernelGGL(k_crt_carry, dim3(nblk), dim3(kCarryThreads), 0, s, res, n, nwords, out_words, cc,
-                       sync, status);
+hipError_t launch_crt_carry(const CarryJob* jobs, uint32_t njobs, uint32_t nblocks, int log2n,
+                            const CrtConst& cc, CarryGlobal* glob, hipStream_t s) {
+    hipLaunchKernelGGL(k_crt_carry, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc, glob);
     return hipGetLastError();
 }
 

@@ -1,6 +1,6 @@
 // ntt_cols.hip - column passes of the four-step NTT (gfx950).
 //   k_col_fwd  limbs -> residues (unpack + reduce), log2(R) forward layers per
-//              column, both operands in one launch (grid.z = operand)
+//              column, every operand chunk of a bucket in one launch
 //   k_col_inv  log2(R) inverse layers per column, in place; also arms the
 //              CRT/carry pass that follows
 #include "ntt_device.h"
@@ -13,18 +13,18 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bx, uint32_t nb) {
 }
 
 // Pass 1: forward column layers, fused with limb unpacking and reduction.
+// grid = (column tiles, prime, job).
 template <int LOGR, int LOGW>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
-    k_col_fwd(const uint32_t* __restrict__ src0, uint32_t ncoef0, const uint32_t* __restrict__ src1,
-              uint32_t ncoef1, uint32_t* __restrict__ dst0, uint32_t* __restrict__ dst1, int log2c,
-              Twiddles tw) {
+    k_col_fwd(const FwdJob* __restrict__ jobs, int log2c, Twiddles tw) {
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
+    __shared__ uint2 zt[TL::L];  // the R - 1 twiddles of the column layers (entry 0 unused)
+    const FwdJob job = jobs[blockIdx.z];
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
-    const uint32_t* src = blockIdx.z ? src1 : src0;
-    const uint32_t ncoef = blockIdx.z ? ncoef1 : ncoef0;
-    uint32_t* dst = (blockIdx.z ? dst1 : dst0) + ((size_t)prime << (LOGR + log2c));
+    uint32_t* dst = job.dst + ((size_t)prime << (LOGR + log2c));
+    for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw.fwd[prime][i];
     // Neighbouring column tiles share 128-byte lines; give each XCD (workgroups
     // are dealt to the 8 XCDs round-robin) a contiguous range of tiles.
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
@@ -33,11 +33,11 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
         uint32_t k = (r << log2c) + c0 + w;
-        uint32_t v = k < ncoef ? src[k] : 0u;
+        uint32_t v = k < job.ncoef ? job.src[k] : 0u;
         lds[TL::addr(w, r)] = reduce_u32(v, p);
     }
     __syncthreads();
-    ct_run<TL, 0, 1>(lds, nullptr, tw.fwd[prime], 0, 0, p);
+    ct_run<TL, 0, 1>(lds, nullptr, zt, 0, 0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -46,23 +46,25 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     }
 }
 
-// Pass 3: inverse column layers, one prime per workgroup, in place.
+// Pass 3: inverse column layers, one prime of one chunk product per workgroup,
+// in place.  grid = (column tiles, prime, job).  It also arms the carry pass
+// that follows on the stream: a new epoch, fresh tickets for every group.
 template <int LOGR, int LOGW>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
-    k_col_inv(uint32_t* __restrict__ bufA, int log2c, Twiddles tw, CarrySync* __restrict__ sync) {
+    k_col_inv(const InvJob* __restrict__ jobs, int log2c, Twiddles tw, CarryGlobal* __restrict__ glob) {
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
-    // Arm the carry pass that follows on this stream: fresh tickets, new epoch.
-    if (sync && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        sync->ticket = 0;
-        sync->epoch = sync->epoch + 1;
+    __shared__ uint2 zt[TL::L];  // inverse twiddles of the column layers
+    const InvJob job = jobs[blockIdx.z];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (job.arm) *job.arm = 0;
+        if (blockIdx.z == 0) glob->epoch = glob->epoch + 1;
     }
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
-    uint32_t* buf = bufA + ((size_t)prime << (LOGR + log2c));
-    // Neighbouring column tiles share 128-byte lines; give each XCD (workgroups
-    // are dealt to the 8 XCDs round-robin) a contiguous range of tiles.
+    uint32_t* buf = job.buf + ((size_t)prime << (LOGR + log2c));
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
+    for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw.inv[prime][i];
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -70,7 +72,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         lds[TL::addr(w, r)] = buf[(r << log2c) + c0 + w];
     }
     __syncthreads();
-    gs_run<TL, 0>(lds, tw.inv[prime], 0, 0, p);
+    gs_run<TL, 0>(lds, zt, 0, 0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -91,28 +93,27 @@ bool col_case_available(int log2r, int log2w) {
     return false;
 }
 
-hipError_t launch_forward_columns(const Plan& pl, const uint32_t* src0, uint32_t ncoef0,
-                                  const uint32_t* src1, uint32_t ncoef1, uint32_t* dst0,
-                                  uint32_t* dst1, int nops, const Twiddles& tw, hipStream_t s) {
-    dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, nops);
-#define PINTT_CASE(LR, LW)                                                                         \
-    if (pl.log2r == LR && pl.log2w == LW) {                                                        \
-        hipLaunchKernelGGL((k_col_fwd<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, src0,   \
-                           ncoef0, src1, ncoef1, dst0, dst1, pl.log2c, tw);                        \
-        return hipGetLastError();                                                                  \
+hipError_t launch_forward_columns(const Plan& pl, const FwdJob* jobs, uint32_t njobs, const Twiddles& tw,
+                                  hipStream_t s) {
+    dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
+#define PINTT_CASE(LR, LW)                                                                          \
+    if (pl.log2r == LR && pl.log2w == LW) {                                                         \
+        hipLaunchKernelGGL((k_col_fwd<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs,    \
+                           pl.log2c, tw);                                                           \
+        return hipGetLastError();                                                                   \
     }
     PINTT_COL_CASES(PINTT_CASE)
 #undef PINTT_CASE
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inverse_columns(const Plan& pl, uint32_t* bufA, const Twiddles& tw, CarrySync* sync,
-                                  hipStream_t s) {
-    dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes);
+hipError_t launch_inverse_columns(const Plan& pl, const InvJob* jobs, uint32_t njobs, CarryGlobal* glob,
+                                  const Twiddles& tw, hipStream_t s) {
+    dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
 #define PINTT_CASE(LR, LW)                                                                          \
     if (pl.log2r == LR && pl.log2w == LW) {                                                         \
-        hipLaunchKernelGGL((k_col_inv<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, bufA,    \
-                           pl.log2c, tw, sync);                                                     \
+        hipLaunchKernelGGL((k_col_inv<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs,    \
+                           pl.log2c, tw, glob);                                                     \
         return hipGetLastError();                                                                   \
     }
     PINTT_COL_CASES(PINTT_CASE)

@@ -46,15 +46,56 @@ struct CrtConst {
 
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
-constexpr int kCarryThreads = 256;
-constexpr int kCarryWordsPerThread = 32;
+constexpr int kCarryThreads = 1024;
+constexpr int kCarryWordsPerThread = 8;
 constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
+constexpr int kMaxChunks = 4;  // pieces the longer operand of one product may be cut into
 
-// Device-side state of the carry pass: block tickets and the epoch that tags
-// status words (reset/incremented on the device by the inverse column pass of
-// each product, so a captured graph can be replayed).  `error` is set if a look-back ever gave up.
-struct CarrySync {
-    unsigned long long ticket;
+// Work descriptors.  One launch of a pass processes a bucket of products that
+// share the transform length N; each product is a group: its longer operand
+// cut into m chunks (m = 1 unless the operands are very unbalanced), all
+// multiplied by the one transform of the other operand, and the chunk
+// convolutions summed at their offsets by the carry pass.  Residues live in
+// arena slots of 3 planes x N words.
+
+struct FwdJob {             // forward column pass of one operand chunk
+    const uint32_t* src;    // 32-bit coefficients, little endian
+    uint32_t ncoef;         // coefficients present; the rest of the slot is zero
+    uint32_t* dst;          // arena slot
+};
+
+// Row pass of one chunk product a_i * b (or a_i^2).  Launches are split by
+// kind: squares, products whose b still needs its row layers (the first chunk
+// of a group; it writes b's transformed rows back when more chunks follow),
+// and products whose b slot already holds transformed rows.
+enum RowKind { kRowSquare = 0, kRowTransformB = 1, kRowReuseB = 2 };
+struct RowJob {
+    uint32_t* a;       // chunk slot, overwritten by the chunk product
+    uint32_t* b;       // slot of the shared operand (unused when squaring)
+    uint32_t write_b;  // kRowTransformB: store b's transformed rows back
+};
+
+struct InvJob {             // inverse column pass of one chunk product, in place
+    uint32_t* buf;
+    unsigned long long* arm;  // the group's carry ticket (reset here), or null
+};
+
+struct CarryJob {           // CRT + carry of one group
+    const uint32_t* res[kMaxChunks];  // chunk product slots
+    uint32_t off[kMaxChunks];         // coefficient offset of each chunk product (multiple of 32)
+    uint32_t len[kMaxChunks];         // length of each chunk product (<= N)
+    uint32_t m;
+    uint32_t blk0;                    // first carry block of the group in this launch
+    uint32_t* out;                    // product words
+    uint64_t nwords;
+    unsigned long long* ticket;       // block tickets of the group
+    uint64_t* status;                 // one look-back word per carry block of the group
+};
+
+// Device-wide state of the carry pass: `epoch` tags look-back status words and
+// is advanced on the device by every inverse column launch (so a captured graph
+// can be replayed); `error` is set if a look-back ever gave up.
+struct CarryGlobal {
     uint32_t epoch;
     uint32_t error;
 };
@@ -100,18 +141,17 @@ __host__ inline bool plan_valid(const Plan& pl) {
            col_case_available(pl.log2r, pl.log2w) && row_case_available(pl.log2c, pl.log2rows);
 }
 
-// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip).
+// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip).  Job
+// arrays are device pointers; counts are host values.
 hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s);
-hipError_t launch_forward_columns(const Plan& pl, const uint32_t* src0, uint32_t ncoef0,
-                                  const uint32_t* src1, uint32_t ncoef1, uint32_t* dst0,
-                                  uint32_t* dst1, int nops, const Twiddles& tw, hipStream_t s);
-hipError_t launch_row_convolution(const Plan& pl, uint32_t* bufA, const uint32_t* bufB, int square,
-                                  const Twiddles& tw, hipStream_t s);
-// The inverse column pass also arms `sync` for the CRT/carry pass after it.
-hipError_t launch_inverse_columns(const Plan& pl, uint32_t* bufA, const Twiddles& tw, CarrySync* sync,
+hipError_t launch_forward_columns(const Plan& pl, const FwdJob* jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
-// `status` holds one word per carry block (zero-initialised once).
-hipError_t launch_crt_carry(const uint32_t* res, size_t n, uint32_t* out_words, size_t nwords,
-                            const CrtConst& cc, CarrySync* sync, uint64_t* status, hipStream_t s);
+hipError_t launch_row_convolution(const Plan& pl, const RowJob* jobs, uint32_t njobs, RowKind kind,
+                                  const Twiddles& tw, hipStream_t s);
+// Also advances glob->epoch and resets the tickets the jobs name.
+hipError_t launch_inverse_columns(const Plan& pl, const InvJob* jobs, uint32_t njobs, CarryGlobal* glob,
+                                  const Twiddles& tw, hipStream_t s);
+hipError_t launch_crt_carry(const CarryJob* jobs, uint32_t njobs, uint32_t nblocks, int log2n,
+                            const CrtConst& cc, CarryGlobal* glob, hipStream_t s);
 
 }  // namespace pintt

@@ -1,46 +1,62 @@
 // ntt_rows.hip - row pass of the four-step NTT (gfx950): forward layers on the
-// rows of a and b, Montgomery pointwise product, inverse layers, in place.
+// rows of the operands, Montgomery pointwise products, inverse layers, in place.
 #include "ntt_device.h"
 
 namespace pintt {
 namespace {
 
-// Pass 2: forward row layers of a (and b), pointwise product, inverse row layers.
-template <int LOGC, int LOGROWS, bool SQUARE>
-__global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT))
-    k_row_conv(uint32_t* __restrict__ bufA, const uint32_t* __restrict__ bufB, int log2r,
-               Twiddles tw) {
+// Pass 2, per chunk product and row tile: forward row layers of a (and of b
+// unless they are already done), Montgomery pointwise product, inverse row
+// layers, written back in place.  grid = (row tiles, prime, job).
+// Three waves per SIMD (at most 168 VGPRs): without the cap the compiler hoists
+// twiddle loads of the paired transforms into ~250 registers and halves occupancy.
+// Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
+// ~250 registers on the paired transforms and halves occupancy.
+template <int LOGC, int LOGROWS, int KIND>
+__global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute__((amdgpu_waves_per_eu(3)))
+    k_row_conv(const RowJob* __restrict__ jobs, int log2r, Twiddles tw) {
     using TL = Tile<LOGC, LOGROWS, false>;
     __shared__ uint32_t ldsA[TL::WORDS];
-    __shared__ uint32_t ldsB[SQUARE ? 1 : TL::WORDS];
+    __shared__ uint32_t ldsB[KIND == kRowSquare ? 1 : TL::WORDS];
+    const RowJob job = jobs[blockIdx.z];
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
     const uint32_t pinv = neg_inv_mod_2_32(p);
     const size_t base = ((size_t)prime << (log2r + LOGC)) + ((size_t)blockIdx.x << (LOGROWS + LOGC));
     const uint32_t row0 = blockIdx.x << LOGROWS;
+    uint32_t* a = job.a + base;
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
-        ldsA[pad(idx)] = bufA[base + idx];
-        if constexpr (!SQUARE) ldsB[pad(idx)] = bufB[base + idx];
+        ldsA[pad(idx)] = a[idx];
+        if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = job.b[base + idx];
     }
     __syncthreads();
-    ct_run<TL, 0, SQUARE ? 1 : 2>(ldsA, ldsB, tw.fwd[prime], log2r, row0, p);
+    ct_run<TL, 0, KIND == kRowTransformB ? 2 : 1>(ldsA, ldsB, tw.fwd[prime], log2r, row0, p);
+    if constexpr (KIND == kRowTransformB) {
+        if (job.write_b) {
+#pragma unroll
+            for (int e = 0; e < kElemsPerThread; ++e) {
+                uint32_t idx = e * TL::NT + threadIdx.x;
+                job.b[base + idx] = ldsB[pad(idx)];
+            }
+        }
+    }
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
-        uint32_t a = ldsA[pad(e * TL::NT + threadIdx.x)];
-        uint32_t b = SQUARE ? a : ldsB[pad(e * TL::NT + threadIdx.x)];
-        ldsA[pad(e * TL::NT + threadIdx.x)] = mul_mont(a, b, p, pinv);
+        uint32_t idx = pad(e * TL::NT + threadIdx.x);
+        uint32_t x = ldsA[idx];
+        uint32_t y = KIND == kRowSquare ? x : ldsB[idx];
+        ldsA[idx] = mul_mont(x, y, p, pinv);
     }
     __syncthreads();
     gs_run<TL, 0>(ldsA, tw.inv[prime], log2r, row0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
-        bufA[base + idx] = ldsA[pad(idx)];
+        a[idx] = ldsA[pad(idx)];
     }
 }
-
 
 }  // namespace
 
@@ -58,21 +74,22 @@ bool row_case_available(int log2c, int log2rows) {
     return false;
 }
 
-hipError_t launch_row_convolution(const Plan& pl, uint32_t* bufA, const uint32_t* bufB, int square,
+hipError_t launch_row_convolution(const Plan& pl, const RowJob* jobs, uint32_t njobs, RowKind kind,
                                   const Twiddles& tw, hipStream_t s) {
-    dim3 grid(1u << (pl.log2r - pl.log2rows), kPrimes);
-#define PINTT_CASE(LC, LROWS)                                                                        \
-    if (pl.log2c == LC && pl.log2rows == LROWS) {                                                    \
-        if (square)                                                                                  \
-            hipLaunchKernelGGL((k_row_conv<LC, LROWS, true>), grid, dim3(Tile<LC, LROWS, false>::NT), \
-                               0, s, bufA, bufB, pl.log2r, tw);                                      \
-        else                                                                                         \
-            hipLaunchKernelGGL((k_row_conv<LC, LROWS, false>), grid,                                 \
-                               dim3(Tile<LC, LROWS, false>::NT), 0, s, bufA, bufB, pl.log2r, tw);    \
-        return hipGetLastError();                                                                    \
+    dim3 grid(1u << (pl.log2r - pl.log2rows), kPrimes, njobs);
+#define PINTT_LAUNCH(LC, LROWS, K)                                                                \
+    hipLaunchKernelGGL((k_row_conv<LC, LROWS, K>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s, \
+                       jobs, pl.log2r, tw)
+#define PINTT_CASE(LC, LROWS)                                                   \
+    if (pl.log2c == LC && pl.log2rows == LROWS) {                               \
+        if (kind == kRowSquare) PINTT_LAUNCH(LC, LROWS, kRowSquare);            \
+        else if (kind == kRowTransformB) PINTT_LAUNCH(LC, LROWS, kRowTransformB); \
+        else PINTT_LAUNCH(LC, LROWS, kRowReuseB);                               \
+        return hipGetLastError();                                               \
     }
     PINTT_ROW_CASES(PINTT_CASE)
 #undef PINTT_CASE
+#undef PINTT_LAUNCH
     return hipErrorInvalidValue;
 }
 

@@ -189,3 +189,66 @@ def test_autotune_picks_a_compiled_plan(engine):
         p = engine.get_plan(n)
         assert (p["log2r"], p["log2w"], p["log2rows"]) in engine.list_plans(n)
         assert p["tuned_ms"] > 0
+
+
+@pytest.mark.parametrize("chunks", [2, 3, 4])
+@pytest.mark.parametrize("an,bn", [(200000, 30000), (1 << 20, 349526), (99999, 5000), (4096, 1000)])
+def test_split_products(engine, oracle, chunks, an, bn):
+    """The longer operand cut into chunks sharing one transform: bit-exact."""
+    a, b = pc.operands(an + 7 * bn, 64 * an, 64 * bn)
+    want = oracle.gmp_mul(a, b)
+    engine.set_split(chunks)
+    try:
+        assert np.array_equal(engine.mul(a, b), want), f"{chunks} chunks"
+        assert np.array_equal(engine.mul(b, a), want), f"{chunks} chunks, swapped"
+    finally:
+        engine.set_split(1)
+    assert np.array_equal(engine.mul(a, b), want), "automatic choice"
+
+
+def test_split_all_ones_carry_chains(engine, oracle):
+    # Overlapping chunk products whose sums ripple carries across chunk borders.
+    a = np.full(50000, np.uint64(0xFFFFFFFFFFFFFFFF))
+    b = np.full(9000, np.uint64(0xFFFFFFFFFFFFFFFF))
+    want = oracle.gmp_mul(a, b)
+    for chunks in (2, 3, 4):
+        engine.set_split(chunks)
+        try:
+            assert np.array_equal(engine.mul(a, b), want), f"{chunks} chunks"
+        finally:
+            engine.set_split(1)
+
+
+def test_batch_mixed_sizes_and_squares(engine, oracle):
+    rng = np.random.default_rng(77)
+    pairs = []
+    for i in range(60):
+        an = int(rng.integers(1, 40000))
+        bn = int(rng.integers(1, 40000))
+        a, b = pc.operands(5000 + i, 64 * an, 64 * bn)
+        pairs.append((a, a) if i % 3 == 2 else (a, b))
+    pairs.append((pc.operands(1, 1 << 22, 1 << 20)))  # one large unbalanced product
+    outs = engine.mul_batch(pairs)
+    for i, ((a, b), got) in enumerate(zip(pairs, outs)):
+        assert np.array_equal(got, oracle.gmp_mul(a, b)), f"batch member {i}"
+
+
+def test_batch_device_pointers(engine, oracle):
+    from pi_compute_amd import hiprt
+
+    ops = [pc.operands(900 + i, 64 * (1000 + 777 * i), 64 * (500 + 333 * i)) for i in range(8)]
+    bufs = []
+    try:
+        da, an, db, bn, do = [], [], [], [], []
+        for a, b in ops:
+            xa, xb = hiprt.DeviceBuffer.from_array(a), hiprt.DeviceBuffer.from_array(b)
+            xo = hiprt.DeviceBuffer(8 * (a.size + b.size))
+            bufs += [xa, xb, xo]
+            da.append(xa.ptr); an.append(a.size); db.append(xb.ptr); bn.append(b.size); do.append(xo.ptr)
+        engine.mul_batch_device(da, an, db, bn, do, 0)
+        hiprt.check(hiprt.runtime().hipDeviceSynchronize(), "sync")
+        for i, (a, b) in enumerate(ops):
+            assert np.array_equal(bufs[3 * i + 2].to_array(), oracle.gmp_mul(a, b)), f"member {i}"
+    finally:
+        for x in bufs:
+            x.close()
