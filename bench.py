@@ -145,6 +145,135 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, gpu_out: np.ndarray, budget_s: fl
             "bit_exact_vs_gpu": exact}
 
 
+def batch_replay(stream, cpu_budget: float, reps: int):
+    """The tree-level replay batch: 4,096 independent products (lognormal bit
+    lengths, median 2^18, every third a square), operands resident in HBM.
+    Times one product per call, one batch call, and a captured-graph replay."""
+    from pi_compute_amd import workloads
+
+    prods = workloads.tree_level_replay()
+    n = len(prods)
+    total_in = workloads.total_limbs(prods)
+    total_out = sum(a.size + b.size for a, b, _ in prods)
+    host_in = np.empty(total_in, np.uint64)
+    a_off, b_off, o_off = [], [], []
+    pos = opos = 0
+    for a, b, sq in prods:
+        host_in[pos:pos + a.size] = a
+        a_off.append(pos)
+        pos += a.size
+        if sq:
+            b_off.append(a_off[-1])
+        else:
+            host_in[pos:pos + b.size] = b
+            b_off.append(pos)
+            pos += b.size
+        o_off.append(opos)
+        opos += a.size + b.size
+    din = hiprt.DeviceBuffer.from_array(host_in)
+    dout = hiprt.DeviceBuffer(8 * total_out)
+    d_a = [din.ptr + 8 * x for x in a_off]
+    d_b = [din.ptr + 8 * x for x in b_off]
+    d_o = [dout.ptr + 8 * x for x in o_off]
+    an = [a.size for a, _, _ in prods]
+    bn = [b.size for _, b, _ in prods]
+
+    def timed(fn, k):
+        fn()  # warm-up (tunes any new length)
+        stream.synchronize()
+        t0, t1 = hiprt.Event(), hiprt.Event()
+        t0.record(stream)
+        for _ in range(k):
+            fn()
+        t1.record(stream)
+        t1.synchronize()
+        ms = t0.elapsed_ms(t1) / k
+        t0.close()
+        t1.close()
+        return ms
+
+    def per_call():
+        for i in range(n):
+            pc.mul_device(d_a[i], an[i], d_b[i], bn[i], d_o[i], stream.handle)
+
+    ms_calls = timed(per_call, 1)
+    ms_batch = timed(lambda: pc.mul_batch_device(d_a, an, d_b, bn, d_o, stream.handle), reps)
+    graph = pc.BatchGraph(d_a, an, d_b, bn, d_o)
+    ms_graph = timed(lambda: graph.launch(stream.handle), reps)
+    pc.synchronize()
+    out = dout.to_array()
+    graph_nodes = graph.nodes
+    graph.close()
+    res = {
+        "workload": "4096 products, bits lognormal(median 2^18, sigma 0.6) clipped [2^12, 2^24], "
+                    "every third a square, splitmix64 seed 7",
+        "products": n, "limbs_in": int(total_in), "limbs_out": int(total_out),
+        "per_call": {"ms": round(ms_calls, 3), "mults_per_s": round(n / ms_calls * 1e3, 1)},
+        "batch_call": {"ms": round(ms_batch, 3), "mults_per_s": round(n / ms_batch * 1e3, 1)},
+        "graph_replay": {"ms": round(ms_graph, 3), "mults_per_s": round(n / ms_graph * 1e3, 1),
+                         "graph_nodes": int(graph_nodes)},
+    }
+    if cpu_budget > 0:
+        res["cpu_baseline"] = batch_cpu_baseline(prods, out, o_off, cpu_budget)
+    din.close()
+    dout.close()
+    return res
+
+
+def batch_cpu_baseline(prods, out, o_off, budget_s: float):
+    """mpz_mul (the reference's GmpMultiplier when built) over the batch in
+    order on c host threads until the time budget runs out; every product it
+    reaches is also compared with the GPU's."""
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    ref = ROOT / "oracle" / "_ref" / "libgmpmul_ref.so"
+    if ref.exists():
+        L = ctypes.CDLL(str(ref))
+        L.ref_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p, ctypes.c_int]
+        kind = "reference"
+
+        def product(a, b, sq, dst):
+            L.ref_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
+                          dst.ctypes.data_as(u64p), 1 if sq else 0)
+    else:
+        L = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+        L.orc_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p]
+        kind = "port"
+
+        def product(a, b, sq, dst):
+            L.orc_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
+                          dst.ctypes.data_as(u64p))
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0, "exact": True}
+    deadline = time.perf_counter() + budget_s
+
+    def worker():
+        while time.perf_counter() < deadline:
+            with lock:
+                i = state["next"]
+                if i >= len(prods):
+                    return
+                state["next"] += 1
+            a, b, sq = prods[i]
+            dst = np.zeros(a.size + b.size, np.uint64)
+            product(a, b, sq, dst)
+            ok = np.array_equal(dst, out[o_off[i]:o_off[i] + dst.size])
+            with lock:
+                state["done"] += 1
+                state["exact"] = state["exact"] and ok
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    return {"value": round(state["done"] / wall, 2), "unit": "mults/s", "cores": cores, "kind": kind,
+            "sample": f"first {state['done']} of {len(prods)} batch products on {cores} threads ({wall:.1f} s)",
+            "bit_exact_vs_gpu": state["exact"], "products_checked": state["done"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +281,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the tree-level replay batch leg")
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -223,6 +354,8 @@ def main() -> int:
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(a, b, gpu_out, args.cpu_budget)
+    if not args.no_batch:
+        result["batch_replay"] = batch_replay(stream, 0 if args.no_cpu else args.cpu_budget / 2, args.batch_reps)
     print(json.dumps(result), flush=True)
     return 0
 

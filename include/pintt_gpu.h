@@ -82,6 +82,21 @@ int gpu_ntt_mul_batch_device(size_t count, const uint64_t* const* d_a, const siz
                              const uint64_t* const* d_b, const size_t* bn, uint64_t* const* d_out,
                              void* stream);
 
+/* A batch captured once into a HIP graph and replayed: the device pointers
+ * and sizes are fixed at creation (the graph owns its scratch arena, carry
+ * state and descriptors); gpu_ntt_graph_launch enqueues one replay on
+ * `stream` (NULL = library stream), ordered with all other products.  Creating
+ * a graph may tune tile plans first (it waits on the device). */
+int gpu_ntt_graph_create(size_t count, const uint64_t* const* d_a, const size_t* an, const uint64_t* const* d_b,
+                         const size_t* bn, uint64_t* const* d_out, void** graph);
+int gpu_ntt_graph_launch(void* graph, void* stream);
+size_t gpu_ntt_graph_nodes(void* graph);
+void gpu_ntt_graph_destroy(void* graph);
+
+/* Wait for all enqueued products; PINTT_ERR_HIP if the device reported a
+ * failed carry look-back. */
+int gpu_ntt_synchronize(void);
+
 /* Very unbalanced operands: the longer one may be cut into up to 4 chunks that
  * share one transform of the shorter one.  mode 1 (default): when the tuned
  * timings say that is cheaper; 0 (or PINTT_SPLIT=0): never; 2..4: that many

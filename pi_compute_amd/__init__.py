@@ -66,6 +66,16 @@ ABI = {
          ctypes.c_void_p],
     ),
     "gpu_ntt_set_split": (None, [ctypes.c_int]),
+    "gpu_ntt_graph_create": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_void_p),
+         ctypes.POINTER(ctypes.c_void_p)],
+    ),
+    "gpu_ntt_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_ntt_graph_nodes": (ctypes.c_size_t, [ctypes.c_void_p]),
+    "gpu_ntt_graph_destroy": (None, [ctypes.c_void_p]),
+    "gpu_ntt_synchronize": (ctypes.c_int, []),
 }
 
 PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns", "crt_carry")
@@ -157,6 +167,35 @@ def mul_batch_device(d_a, an, d_b, bn, d_out, stream: int = 0) -> None:
     _check(lib, lib.gpu_ntt_mul_batch_device(
         n, (vp * n)(*d_a), (ctypes.c_size_t * n)(*an), (vp * n)(*d_b), (ctypes.c_size_t * n)(*bn),
         (vp * n)(*d_out), stream or None))
+
+
+class BatchGraph:
+    """A batch of device-pointer products captured into a HIP graph."""
+
+    def __init__(self, d_a, an, d_b, bn, d_out):
+        lib = load_library()
+        n = len(d_a)
+        vp = ctypes.c_void_p
+        h = vp()
+        _check(lib, lib.gpu_ntt_graph_create(
+            n, (vp * n)(*d_a), (ctypes.c_size_t * n)(*an), (vp * n)(*d_b), (ctypes.c_size_t * n)(*bn),
+            (vp * n)(*d_out), ctypes.byref(h)))
+        self.handle = h.value
+        self.nodes = lib.gpu_ntt_graph_nodes(self.handle)
+
+    def launch(self, stream: int = 0) -> None:
+        lib = load_library()
+        _check(lib, lib.gpu_ntt_graph_launch(self.handle, stream or None))
+
+    def close(self) -> None:
+        if self.handle:
+            load_library().gpu_ntt_graph_destroy(self.handle)
+            self.handle = None
+
+
+def synchronize() -> None:
+    lib = load_library()
+    _check(lib, lib.gpu_ntt_synchronize())
 
 
 def set_split(mode: int) -> None:

@@ -177,9 +177,11 @@ int grow(T*& ptr, size_t count) {
     return PINTT_OK;
 }
 
+// The tables are built once at the full 2^25 entries (1.5 GiB for the three
+// primes, forward and inverse) so that captured graphs never see them move.
 int ensure_twiddles(int log2n) {
     if (log2n <= g.tw_log2) return PINTT_OK;
-    int tl = log2n < 16 ? 16 : log2n;
+    int tl = kMaxLog2N;
     size_t cnt = (size_t)1 << tl;
     for (int i = 0; i < kPrimes; ++i) {
         PINTT_TRY(grow(g.tw_fwd[i], cnt));
@@ -347,14 +349,21 @@ size_t align_up(size_t x) {
     return (x + alignof(T) - 1) / alignof(T) * alignof(T);
 }
 
-// Enqueue a batch on `s`.  Every pointer in `reqs` is a device pointer.
-int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
-    if (reqs.empty()) return PINTT_OK;
-    std::vector<Group> groups(reqs.size());
-    for (size_t i = 0; i < reqs.size(); ++i) PINTT_TRY(plan_group(reqs[i], &groups[i]));
-
-    // Buckets by transform length.
+struct BatchPlan {
+    std::vector<Group> groups;
     std::vector<BucketLayout> buckets;
+    std::vector<Plan> plans;
+    size_t bytes = 0, arena_words = 0, total_groups = 0, total_blocks = 0;
+    int max_log2n = 0;
+};
+
+// Groups, buckets, descriptor layout and resource sizes of a batch.  May tune
+// tile plans for lengths seen for the first time (which waits on the device).
+int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
+    bp->groups.resize(reqs.size());
+    for (size_t i = 0; i < reqs.size(); ++i) PINTT_TRY(plan_group(reqs[i], &bp->groups[i]));
+    const std::vector<Group>& groups = bp->groups;
+    std::vector<BucketLayout>& buckets = bp->buckets;
     for (size_t i = 0; i < groups.size(); ++i) {
         auto it = std::find_if(buckets.begin(), buckets.end(),
                                [&](const BucketLayout& b) { return b.log2n == groups[i].log2n; });
@@ -365,10 +374,8 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
         }
         it->groups.push_back(i);
     }
-    // Sizes: descriptors, arena (buckets run one after another on one stream
-    // and reuse the arena), tickets, status words.
-    size_t bytes = 0, arena = 0, total_groups = 0, total_blocks = 0;
-    int max_log2n = 0;
+    // Buckets run one after another on one stream and reuse the arena.
+    size_t bytes = 0;
     for (auto& b : buckets) {
         b.nfwd = b.ninv = b.nblocks = b.max_m = 0;
         size_t slots = 0, nsq = 0;
@@ -404,29 +411,27 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
         bytes = align_up<CarryJob>(bytes);
         b.carry_off = bytes;
         bytes += b.groups.size() * sizeof(CarryJob);
-        arena = std::max(arena, slots * kPrimes * ((size_t)1 << b.log2n));
-        total_groups += b.groups.size();
-        total_blocks += b.nblocks;
-        max_log2n = std::max(max_log2n, b.log2n);
+        bp->arena_words = std::max(bp->arena_words, slots * kPrimes * ((size_t)1 << b.log2n));
+        bp->total_groups += b.groups.size();
+        bp->total_blocks += b.nblocks;
+        bp->max_log2n = std::max(bp->max_log2n, b.log2n);
     }
-    std::vector<Plan> plans(buckets.size());
-    for (size_t bi = 0; bi < buckets.size(); ++bi) PINTT_TRY(plan_for(buckets[bi].log2n, &plans[bi]));
-    PINTT_TRY(ensure_twiddles(max_log2n));
-    PINTT_TRY(ensure_arena(arena));
-    PINTT_TRY(ensure_carry_state(total_groups, total_blocks));
-    const int ring = g.desc_next;
-    g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
-    PINTT_TRY(ensure_descriptors(ring, bytes));
+    bp->bytes = bytes;
+    bp->plans.resize(buckets.size());
+    for (size_t bi = 0; bi < buckets.size(); ++bi) PINTT_TRY(plan_for(buckets[bi].log2n, &bp->plans[bi]));
+    return PINTT_OK;
+}
 
-    // Fill the descriptors.
-    unsigned char* H = g.desc_host[ring];
-    unsigned char* D = g.desc_dev[ring];
+// Write the descriptors of `bp` into H, pointing at the given arena, tickets
+// and status words.
+void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, unsigned long long* tickets,
+                      uint64_t* status) {
     size_t ticket_next = 0, status_next = 0;
-    for (auto& b : buckets) {
+    for (const auto& b : bp.buckets) {
         const size_t n = (size_t)1 << b.log2n;
         const size_t slot = kPrimes * n;
         FwdJob* fj = reinterpret_cast<FwdJob*>(H + b.fwd_off);
-        // row launch i of kind: squares first (if any), then chunk index 0, 1, ...
+        // row launches: squares first (if any), then chunk index 0, 1, ...
         std::vector<RowJob*> rj;
         std::vector<uint32_t> rfill(b.rows.size(), 0);
         for (const RowLaunch& rl : b.rows) rj.push_back(reinterpret_cast<RowJob*>(H + rl.off));
@@ -436,14 +441,13 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
         size_t next_slot = 0;
         uint32_t f = 0, inv = 0, blk0 = 0;
         for (size_t k = 0; k < b.groups.size(); ++k) {
-            const Group& gr = groups[b.groups[k]];
+            const Group& gr = bp.groups[b.groups[k]];
             const Request& r = gr.r;
             CarryJob car{};
-            uint32_t* bslot = nullptr;
-            if (!r.square) bslot = g.arena + slot * (next_slot + gr.m);
-            unsigned long long* ticket = g.tickets + ticket_next++;
+            uint32_t* bslot = r.square ? nullptr : arena + slot * (next_slot + gr.m);
+            unsigned long long* ticket = tickets + ticket_next++;
             for (uint32_t i = 0; i < gr.m; ++i) {
-                uint32_t* dst = g.arena + slot * next_slot++;
+                uint32_t* dst = arena + slot * next_slot++;
                 size_t lo = i * gr.h;
                 size_t limbs = std::min(gr.h, r.an - lo);
                 fj[f++] = FwdJob{reinterpret_cast<const uint32_t*>(r.a + lo), (uint32_t)(2 * limbs), dst};
@@ -467,23 +471,20 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
             car.out = reinterpret_cast<uint32_t*>(r.out);
             car.nwords = 2 * (r.an + r.bn);
             car.ticket = ticket;
-            car.status = g.status + status_next;
+            car.status = status + status_next;
             cj[k] = car;
             blk0 += (uint32_t)gr.nblk;
             status_next += gr.nblk;
         }
     }
-    PINTT_HIP(hipMemcpyAsync(D, H, bytes, hipMemcpyHostToDevice, g.stream));
-    PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
+}
 
-    // Launches (a caller stream first waits for the upload, which itself is
-    // ordered after the previous batch on the library stream).
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
+// The kernel launches of a batch whose descriptors sit at device address D.
+int enqueue_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t s, bool timed) {
     const Twiddles tw = twiddles();
-    const bool timed = g.timing;
-    for (size_t bi = 0; bi < buckets.size(); ++bi) {
-        const BucketLayout& b = buckets[bi];
-        const Plan& pl = plans[bi];
+    for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
+        const BucketLayout& b = bp.buckets[bi];
+        const Plan& pl = bp.plans[bi];
         if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
         PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const FwdJob*>(D + b.fwd_off), b.nfwd, tw, s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
@@ -498,9 +499,96 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
                                    b.nblocks, b.log2n, crt_constants(b.log2n), g.glob, s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
     }
-    if (timed) g.have_times = true;
+    return PINTT_OK;
+}
+
+// Enqueue a batch on `s`.  Every pointer in `reqs` is a device pointer.
+int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
+    if (reqs.empty()) return PINTT_OK;
+    BatchPlan bp;
+    PINTT_TRY(plan_batch(reqs, &bp));
+    PINTT_TRY(ensure_twiddles(bp.max_log2n));
+    PINTT_TRY(ensure_arena(bp.arena_words));
+    PINTT_TRY(ensure_carry_state(bp.total_groups, bp.total_blocks));
+    const int ring = g.desc_next;
+    g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
+    PINTT_TRY(ensure_descriptors(ring, bp.bytes));
+    fill_descriptors(bp, g.desc_host[ring], g.arena, g.tickets, g.status);
+    PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bp.bytes, hipMemcpyHostToDevice, g.stream));
+    PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
+    // A caller stream first waits for the upload, which itself is ordered
+    // after the previous batch on the library stream.
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
+    PINTT_TRY(enqueue_launches(bp, g.desc_dev[ring], s, g.timing));
+    if (g.timing) g.have_times = true;
     PINTT_HIP(hipEventRecord(g.done, s));
     if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
+    return PINTT_OK;
+}
+
+// A batch captured once into a HIP graph, with its own arena, carry state and
+// descriptors, replayed on demand (the operand and output pointers are fixed).
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t* arena = nullptr;
+    unsigned long long* tickets = nullptr;
+    uint64_t* status = nullptr;
+    unsigned char* desc = nullptr;
+    size_t nodes = 0;
+};
+
+void free_graph(BatchGraph* bg) {
+    if (bg->exec) (void)hipGraphExecDestroy(bg->exec);
+    if (bg->graph) (void)hipGraphDestroy(bg->graph);
+    (void)hipFree(bg->arena);
+    (void)hipFree(bg->tickets);
+    (void)hipFree(bg->status);
+    (void)hipFree(bg->desc);
+    delete bg;
+}
+
+int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
+    BatchPlan bp;
+    PINTT_TRY(plan_batch(reqs, &bp));
+    PINTT_TRY(ensure_twiddles(bp.max_log2n));
+    BatchGraph* bg = new BatchGraph;
+    auto cleanup = [&](int rc) {
+        free_graph(bg);
+        return rc;
+    };
+    if (hipMalloc(reinterpret_cast<void**>(&bg->arena), std::max<size_t>(bp.arena_words, 1) * 4) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&bg->tickets), bp.total_groups * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&bg->status), std::max<size_t>(bp.total_blocks, 1) * 8) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&bg->desc), bp.bytes) != hipSuccess)
+        return cleanup(fail(PINTT_ERR_NO_MEMORY, "graph buffers"));
+    if (hipMemset(bg->tickets, 0, bp.total_groups * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(bg->status, 0, std::max<size_t>(bp.total_blocks, 1) * 8) != hipSuccess)
+        return cleanup(fail(PINTT_ERR_HIP, "graph state init"));
+    std::vector<unsigned char> H(bp.bytes);
+    fill_descriptors(bp, H.data(), bg->arena, bg->tickets, bg->status);
+    if (hipMemcpy(bg->desc, H.data(), bp.bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(PINTT_ERR_HIP, "graph descriptor upload"));
+    // Capture the launches on a private stream (nothing else runs on it).
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(PINTT_ERR_HIP, "capture stream"));
+    int rc = PINTT_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = fail(PINTT_ERR_HIP, "hipStreamBeginCapture");
+    } else {
+        rc = enqueue_launches(bp, bg->desc, cs, false);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+        bg->graph = graph;
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc) return cleanup(rc);
+    if (hipGraphGetNodes(bg->graph, nullptr, &bg->nodes) != hipSuccess) bg->nodes = 0;
+    hipError_t e = hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipGraphInstantiate"));
+    *out = bg;
     return PINTT_OK;
 }
 
@@ -710,6 +798,57 @@ int gpu_ntt_mul_batch_device(size_t count, const uint64_t* const* d_a, const siz
 int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
                        void* stream) {
     return gpu_ntt_mul_batch_device(1, &d_a, &an, &d_b, &bn, &d_out, stream);
+}
+
+int gpu_ntt_graph_create(size_t count, const uint64_t* const* d_a, const size_t* an, const uint64_t* const* d_b,
+                         const size_t* bn, uint64_t* const* d_out, void** graph) {
+    if (!graph || count == 0 || !d_a || !an || !d_b || !bn || !d_out) return fail(PINTT_ERR_ARGS, "bad arguments");
+    std::vector<Request> reqs(count);
+    for (size_t i = 0; i < count; ++i) {
+        if (!d_a[i] || !d_b[i] || !d_out[i] || an[i] == 0 || bn[i] == 0)
+            return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+        if (an[i] + bn[i] > ((size_t)1 << (kMaxLog2N - 1)))
+            return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+        reqs[i] = Request{d_a[i], an[i], d_b[i], bn[i], d_out[i], d_a[i] == d_b[i] && an[i] == bn[i]};
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    BatchGraph* bg = nullptr;
+    PINTT_TRY(create_graph(reqs, &bg));
+    *graph = bg;
+    return PINTT_OK;
+}
+
+int gpu_ntt_graph_launch(void* graph, void* stream) {
+    if (!graph) return fail(PINTT_ERR_ARGS, "null graph");
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    BatchGraph* bg = static_cast<BatchGraph*>(graph);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
+    // Serialised with every other product: all share the carry epoch.
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+    PINTT_HIP(hipGraphLaunch(bg->exec, s));
+    PINTT_HIP(hipEventRecord(g.done, s));
+    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
+    return PINTT_OK;
+}
+
+size_t gpu_ntt_graph_nodes(void* graph) { return graph ? static_cast<BatchGraph*>(graph)->nodes : 0; }
+
+void gpu_ntt_graph_destroy(void* graph) {
+    if (!graph) return;
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g.ready) (void)hipStreamSynchronize(g.stream);
+    (void)hipDeviceSynchronize();
+    free_graph(static_cast<BatchGraph*>(graph));
+}
+
+int gpu_ntt_synchronize(void) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g.ready) return PINTT_OK;
+    return check_device_error();
 }
 
 int gpu_ntt_transform_log2(size_t an, size_t bn) {

@@ -252,3 +252,57 @@ def test_batch_device_pointers(engine, oracle):
     finally:
         for x in bufs:
             x.close()
+
+
+def _upload_batch(prods):
+    from pi_compute_amd import hiprt
+
+    bufs, d_a, an, d_b, bn, d_o = [], [], [], [], [], []
+    for a, b, sq in prods:
+        xa = hiprt.DeviceBuffer.from_array(a)
+        xb = xa if sq else hiprt.DeviceBuffer.from_array(b)
+        xo = hiprt.DeviceBuffer(8 * (a.size + b.size))
+        bufs.append((xa, xb, xo))
+        d_a.append(xa.ptr); an.append(a.size); d_b.append(xb.ptr); bn.append(b.size); d_o.append(xo.ptr)
+    return bufs, d_a, an, d_b, bn, d_o
+
+
+def test_graph_replays_match_eager(engine, oracle):
+    """A captured batch replayed several times, with eager products and host
+    synchronisations in between; every replay is checked."""
+    from pi_compute_amd import workloads
+
+    prods = workloads.tree_level_replay(count=96, seed=11, median_bits=1 << 15, hi_bits=1 << 20)
+    want = [oracle.gmp_mul(a, b) for a, b, _ in prods]
+    bufs, d_a, an, d_b, bn, d_o = _upload_batch(prods)
+    graph = engine.BatchGraph(d_a, an, d_b, bn, d_o)
+    try:
+        assert graph.nodes >= 4
+        for rep in range(4):
+            for _, _, xo in bufs:  # clear outputs so each replay must rewrite them
+                xo_zero = np.zeros(xo.nbytes // 8, np.uint64)
+                from pi_compute_amd import hiprt
+                hiprt.check(hiprt.runtime().hipMemcpy(xo.ptr, xo_zero.ctypes.data, xo.nbytes, hiprt.H2D), "clear")
+            graph.launch(0)
+            engine.synchronize()
+            for i, (_, _, xo) in enumerate(bufs):
+                assert np.array_equal(xo.to_array(), want[i]), f"replay {rep}, product {i}"
+            # eager work between replays (shares the carry epoch and tables)
+            x, y = pc.operands(50 + rep, 64 * 3000, 64 * 2000)
+            assert np.array_equal(engine.mul(x, y), oracle.gmp_mul(x, y))
+    finally:
+        graph.close()
+        for xa, xb, xo in bufs:
+            xa.close()
+            if xb is not xa:
+                xb.close()
+            xo.close()
+
+
+def test_tree_level_batch_reduced(engine, oracle):
+    from pi_compute_amd import workloads
+
+    prods = workloads.tree_level_replay(count=400, seed=7)
+    outs = engine.mul_batch([(a, a) if sq else (a, b) for a, b, sq in prods])
+    for i, ((a, b, _), got) in enumerate(zip(prods, outs)):
+        assert np.array_equal(got, oracle.gmp_mul(a, b)), f"product {i}"

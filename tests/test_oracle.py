@@ -248,3 +248,17 @@ def test_operand_generator_shape():
     # splitmix64 reference outputs for seed 0 (Vigna's published generator)
     assert int(pc.splitmix64(0, 0, 1)[0]) == 0xE220A8397B1DCDAF
     assert int(pc.splitmix64(0, 1, 1)[0]) == 0x6E789E6AA1B965F4
+
+
+def test_tree_level_workload_shape():
+    from pi_compute_amd import workloads
+
+    ba, bb, sq = workloads.tree_level_sizes()
+    assert ba.size == 4096 and sq.sum() == 4096 // 3
+    assert ba.min() >= 1 << 12 and ba.max() <= 1 << 24
+    assert abs(np.log2(np.median(ba)) - 18) < 0.1
+    assert np.all(bb[sq] == ba[sq])
+    ps = workloads.tree_level_replay(count=30)
+    for (a, b, s), x, y in zip(ps, ba[:30], bb[:30]):
+        assert int(a[-1]).bit_length() + 64 * (a.size - 1) == x
+        assert (b is a) == s
