@@ -30,8 +30,10 @@ STATUS_NAMES = {
     5: "PINTT_ERR_ARGS",
 }
 
-# Every symbol include/pintt_gpu.h declares, with (restype, argtypes).
+# Every symbol include/*.h declares, with (restype, argtypes).
 _u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
 ABI = {
     "gpu_ntt_init": (ctypes.c_int, [ctypes.c_int]),
     "gpu_ntt_shutdown": (None, []),
@@ -76,6 +78,21 @@ ABI = {
     "gpu_ntt_graph_nodes": (ctypes.c_size_t, [ctypes.c_void_p]),
     "gpu_ntt_graph_destroy": (None, [ctypes.c_void_p]),
     "gpu_ntt_synchronize": (ctypes.c_int, []),
+    # include/pintt_mlp.h
+    "gpu_mlp_param_count": (ctypes.c_size_t, [ctypes.c_int]),
+    "gpu_mlp_forward": (
+        ctypes.c_int,
+        [_f32p, ctypes.c_int, _f32p, ctypes.c_size_t, _f32p],
+    ),
+    "gpu_mlp_gradient": (
+        ctypes.c_int,
+        [_f32p, ctypes.c_int, _f32p, _i32p, _f32p, ctypes.c_size_t, _f32p, _f32p],
+    ),
+    "gpu_mlp_train": (
+        ctypes.c_int,
+        [_f32p, ctypes.c_int, _f32p, _i32p, _f32p, ctypes.c_size_t, ctypes.c_int, ctypes.c_float, _f32p],
+    ),
+    "gpu_mlp_last_error": (ctypes.c_char_p, []),
 }
 
 PASS_NAMES = ("forward_columns", "row_convolution", "inverse_columns", "crt_carry")
@@ -297,3 +314,50 @@ def to_int(limbs: np.ndarray) -> int:
 def from_int(x: int, nlimbs: int | None = None) -> np.ndarray:
     n = nlimbs if nlimbs is not None else max(1, (x.bit_length() + 63) // 64)
     return np.frombuffer(x.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)
+
+
+# ---- dispatch cost model (include/pintt_mlp.h) ----
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _mlp_check(lib, status: int) -> None:
+    if status != PINTT_OK:
+        raise PinttError(status, lib.gpu_mlp_last_error().decode())
+
+
+def mlp_forward(params, outputs: int, x) -> np.ndarray:
+    """Device inference: x[n][4] -> y[n][outputs] (float32)."""
+    lib = load_library()
+    w, x = _f32(params), _f32(x)
+    y = np.zeros((x.shape[0], outputs), np.float32)
+    _mlp_check(lib, lib.gpu_mlp_forward(w.ctypes.data_as(_f32p), outputs, x.ctypes.data_as(_f32p), x.shape[0],
+                                        y.ctypes.data_as(_f32p)))
+    return y
+
+
+def mlp_gradient(params, outputs: int, x, path, target):
+    """Device gradient and loss of the masked squared error."""
+    lib = load_library()
+    w, x, t = _f32(params), _f32(x), _f32(target)
+    p = np.ascontiguousarray(path, np.int32)
+    g = np.zeros(w.size, np.float32)
+    loss = np.zeros(1, np.float32)
+    _mlp_check(lib, lib.gpu_mlp_gradient(w.ctypes.data_as(_f32p), outputs, x.ctypes.data_as(_f32p),
+                                         p.ctypes.data_as(_i32p), t.ctypes.data_as(_f32p), x.shape[0],
+                                         g.ctypes.data_as(_f32p), loss.ctypes.data_as(_f32p)))
+    return g, float(loss[0])
+
+
+def mlp_train(params, outputs: int, x, path, target, steps: int, lr: float):
+    """Full-batch Adam on the device; returns (params, loss history)."""
+    lib = load_library()
+    w = _f32(params).copy()
+    x, t = _f32(x), _f32(target)
+    p = np.ascontiguousarray(path, np.int32)
+    hist = np.zeros(max(steps, 1), np.float32)
+    _mlp_check(lib, lib.gpu_mlp_train(w.ctypes.data_as(_f32p), outputs, x.ctypes.data_as(_f32p),
+                                      p.ctypes.data_as(_i32p), t.ctypes.data_as(_f32p), x.shape[0], steps, lr,
+                                      hist.ctypes.data_as(_f32p)))
+    return w, hist[:steps]

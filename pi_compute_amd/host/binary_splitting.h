@@ -1,0 +1,69 @@
+// binary_splitting.h - Chudnovsky P/Q/R binary splitting over a Multiplier.
+//
+// Same recursion as the reference (src/engine/binary_splitting.cpp:137-293):
+//   leaf k:  P = -(6k-5)(2k-1)(6k-1), Q = k^3 * 10939058860032000,
+//            R = P * (13591409 + 545140134 k);  leaf 0: P = Q = 1, R = 13591409
+//   merge:   P = Pl*Pr, Q = Ql*Qr, R = Qr*Rl + Pl*Rr.
+// P(a,b), Q(a,b) and R(a,b) do not depend on where the range is split (R is
+// the sum over k of P(a,k+1)*(13591409 + 545140134 k)*Q(k+1,b)), so any tree
+// shape yields identical integers and identical digits.
+//
+// compute() evaluates the tree level by level: small subtrees of `leaf_span`
+// terms on host threads, then every merge of a level issues its products as
+// one Multiplier::multiply_batch (one engine call per level on the GPU).
+// compute_recursive() is the depth-first order of the reference, one
+// Multiplier::multiply per product.
+#pragma once
+
+#include <gmp.h>
+
+#include <cstddef>
+#include <vector>
+
+#include "multiplier.h"
+
+namespace pi {
+
+struct BSResult {
+    mpz_t P, Q, R;
+    BSResult();
+    ~BSResult();
+    BSResult(const BSResult&) = delete;
+    BSResult& operator=(const BSResult&) = delete;
+    BSResult(BSResult&& o) noexcept;
+    BSResult& operator=(BSResult&& o) noexcept;
+};
+
+class BinarySplitting {
+public:
+    explicit BinarySplitting(Multiplier& m, unsigned threads = 0);
+
+    // [a, b); need_top_p = false skips the final P product (unused by pi).
+    BSResult compute(unsigned long a, unsigned long b, bool need_top_p = true);
+    BSResult compute_recursive(unsigned long a, unsigned long b);
+
+    static unsigned long terms_needed(size_t digits);
+    static void base_case(unsigned long k, BSResult& r);
+    // Host-only recursion with mpz_mul (the bottom of the tree).
+    static void compute_small(unsigned long a, unsigned long b, BSResult& r);
+
+    void set_leaf_span(unsigned long s) { leaf_span_ = s ? s : 1; }
+
+    struct Level {
+        size_t merges, products;
+        size_t max_limbs;
+        double seconds;
+    };
+    const std::vector<Level>& levels() const { return levels_; }
+
+private:
+    void merge(BSResult& left, BSResult& right, BSResult& out, bool need_p);
+    BSResult recurse(unsigned long a, unsigned long b, int depth);
+
+    Multiplier& mult_;
+    unsigned threads_;
+    unsigned long leaf_span_ = 256;
+    std::vector<Level> levels_;
+};
+
+}  // namespace pi

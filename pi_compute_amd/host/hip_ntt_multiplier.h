@@ -3,18 +3,24 @@
 // Drop-in beside GmpMultiplier.  Semantics follow the reference GPU wrappers
 // (src/arithmetic/int_ntt_multiplier.cpp:115-159,
 //  src/arithmetic/gpu_ntt_multiplier.cpp:144-210):
-//   * min(|a|, |b|) in limbs below the threshold -> mpz_mul on the host;
+//   * the dispatcher (default: smaller operand below `threshold` limbs) sends
+//     a product to mpz_mul on the host;
 //   * sign = sgn(a) * sgn(b), zero operand -> result 0;
 //   * engine refusal (too large, device error) -> mpz_mul, product still exact.
 // Differences: operands cross as raw GMP limbs (no digit vectors), squaring
-// takes one forward transform, and result may alias a or b.
+// takes one forward transform, result may alias a or b, and multiply_batch
+// sends a whole set of products to the device in one engine call while the
+// host-path products of the set run on host threads at the same time.
 #pragma once
 
 #include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <mutex>
 #include <string>
 
+#include "dispatch.h"
 #include "multiplier.h"
 
 namespace pi {
@@ -26,31 +32,56 @@ public:
     // Throws std::runtime_error when no usable device exists, like the
     // reference constructors (src/arithmetic/int_ntt_multiplier.cpp:26-31).
     explicit HipNttMultiplier(size_t threshold = 10000, int device = 0);
-    ~HipNttMultiplier() override = default;
+    ~HipNttMultiplier() override;
 
     void multiply(mpz_t result, const mpz_t a, const mpz_t b) override;
     void square(mpz_t result, const mpz_t a) override;
+    void multiply_batch(const MulJob* jobs, size_t count, unsigned threads = 0) override;
 
-    size_t threshold() const { return threshold_; }
-    void set_threshold(size_t t) { threshold_ = t; }
+    size_t threshold() const { return heuristic_.threshold(); }
+    void set_threshold(size_t t) { heuristic_.set_threshold(t); }
+    // Replace the path choice (not owned; nullptr restores the threshold rule).
+    void set_dispatcher(const Dispatcher* d) { dispatcher_ = d ? d : &heuristic_; }
+    const Dispatcher& dispatcher() const { return *dispatcher_; }
+    // When false, multiply_batch issues its device products one engine call
+    // each (the per-call mode the batching is measured against).
+    void set_batching(bool on) { batching_ = on; }
     std::string device_name() const;
 
+    // Per-product timing log, one JSON object per line:
+    //   {"an":..,"bn":..,"batch":..,"square":0|1,"path":"gpu"|"cpu","ms":..}
+    // For a device batch, "ms" is the batch wall time shared out in proportion
+    // to each product's transform work.  These lines train the learned dispatcher.
+    bool open_log(const std::string& path);
+    void close_log();
+    // At most this many evenly spaced lines per path and batch (0: all).
+    void set_log_limit(size_t n) { log_limit_ = n; }
+
     struct Stats {
-        std::atomic<uint64_t> gpu_calls{0};
-        std::atomic<uint64_t> cpu_calls{0};       // below threshold
+        std::atomic<uint64_t> gpu_calls{0};       // products on the device
+        std::atomic<uint64_t> gpu_batches{0};     // engine calls
+        std::atomic<uint64_t> cpu_calls{0};       // products sent to the host by the dispatcher
         std::atomic<uint64_t> fallback_calls{0};  // engine refused or failed
         std::atomic<uint64_t> gpu_ns{0};
+        std::atomic<uint64_t> cpu_ns{0};
     };
     const Stats& stats() const { return stats_; }
     void reset_stats();
     void print_stats() const;
 
 private:
-    bool use_gpu(const mpz_t a, const mpz_t b) const;
     void gpu_product(mpz_t result, const mpz_t a, const mpz_t b, bool square);
+    void host_product(mpz_ptr result, mpz_srcptr a, mpz_srcptr b, size_t batch, bool log);
+    bool log_pick(size_t k, size_t n) const;
+    void log_line(size_t an, size_t bn, size_t batch, bool square, bool gpu, double ms);
 
-    size_t threshold_;
+    ThresholdDispatcher heuristic_;
+    const Dispatcher* dispatcher_;
     int device_;
+    bool batching_ = true;
+    std::FILE* log_ = nullptr;
+    size_t log_limit_ = 0;
+    std::mutex log_mu_;
     mutable Stats stats_;
 };
 

@@ -1,0 +1,120 @@
+// main.cpp - pi_compute_hip: Chudnovsky pi with the gfx950 multiplier.
+// Options mirror the reference CLI (src/main.cpp:28-55) where they overlap.
+//
+//   pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]
+//                  [--threads T] [--tree levels|recursive] [--per-call]
+//                  [--dispatch threshold|mlp] [--model FILE] [--log FILE]
+//                  [--verbose]
+//
+//   --gpu            HipNttMultiplier (default: GmpMultiplier, the GMP-only run)
+//   --tree levels    level-synchronous tree, one multiply_batch per level (default)
+//   --tree recursive depth-first tree of the reference, one multiply per product
+//   --per-call       with --tree levels: device products one engine call each
+//   --dispatch mlp   learned cost model from --model (threshold rule otherwise)
+//   --log FILE       per-product timing lines (training data for the cost model)
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <string>
+
+#include "gmp_multiplier.h"
+#include "hip_ntt_multiplier.h"
+#include "mlp_dispatch.h"
+#include "pi_engine.h"
+
+namespace {
+
+void usage() {
+    std::cout << "usage: pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]\n"
+                 "                      [--threads T] [--tree levels|recursive] [--per-call]\n"
+                 "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE] [--verbose]\n";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    pi::PiConfig cfg;
+    cfg.output_file = "pi_digits.txt";
+    bool gpu = false, per_call = false;
+    size_t threshold = 10000;
+    std::string dispatch = "threshold", model, log;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() -> std::string {
+            if (i + 1 >= argc) {
+                std::cerr << "missing value for " << a << std::endl;
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--digits") cfg.digits = std::stoul(next());
+        else if (a == "--output") cfg.output_file = next();
+        else if (a == "--gpu") gpu = true;
+        else if (a == "--gpu-threshold") threshold = std::stoul(next());
+        else if (a == "--threads") cfg.num_threads = std::stoul(next());
+        else if (a == "--tree") {
+            std::string t = next();
+            if (t == "levels") cfg.level_batches = true;
+            else if (t == "recursive") cfg.level_batches = false;
+            else { usage(); return 2; }
+        } else if (a == "--per-call") per_call = true;
+        else if (a == "--dispatch") dispatch = next();
+        else if (a == "--model") model = next();
+        else if (a == "--log") log = next();
+        else if (a == "--verbose" || a == "-v") cfg.verbose = true;
+        else if (a == "--help" || a == "-h") { usage(); return 0; }
+        else { std::cerr << "unknown option " << a << std::endl; usage(); return 2; }
+    }
+    if (cfg.digits == 0) {
+        std::cerr << "--digits must be > 0" << std::endl;
+        return 2;
+    }
+    try {
+        std::unique_ptr<pi::Multiplier> m;
+        pi::HipNttMultiplier* hip = nullptr;
+        std::unique_ptr<pi::MlpDispatcher> mlp;
+        std::string mode = "GMP";
+        if (gpu) {
+            auto h = std::make_unique<pi::HipNttMultiplier>(threshold);
+            hip = h.get();
+            hip->set_batching(!per_call);
+            if (dispatch == "mlp") {
+                if (model.empty()) throw std::runtime_error("--dispatch mlp needs --model FILE");
+                mlp = std::make_unique<pi::MlpDispatcher>(model, &hip->dispatcher());
+                hip->set_dispatcher(mlp.get());
+            } else if (dispatch != "threshold") {
+                throw std::runtime_error("unknown dispatcher " + dispatch);
+            }
+            if (!log.empty() && !hip->open_log(log)) throw std::runtime_error("cannot open log " + log);
+            mode = std::string("HIP NTT, ") + hip->device_name() + ", " + hip->dispatcher().name() +
+                   " dispatch, " + (cfg.level_batches ? (per_call ? "levels/per-call" : "levels/batched") : "recursive");
+            m = std::move(h);
+        } else {
+            m = std::make_unique<pi::GmpMultiplier>();
+        }
+        pi::PiEngine engine(*m);
+        pi::PiResult r = engine.compute(cfg);
+        if (hip) {
+            hip->close_log();
+            if (cfg.verbose) hip->print_stats();
+        }
+        std::cout << "Computed " << cfg.digits << " digits of pi in " << r.elapsed_seconds << " seconds ("
+                  << r.terms_used << " terms, " << mode << ")." << std::endl;
+        std::cout << "{\"digits\":" << cfg.digits << ",\"seconds\":" << r.elapsed_seconds
+                  << ",\"bs_seconds\":" << r.bs_seconds << ",\"final_seconds\":" << r.final_seconds
+                  << ",\"conv_seconds\":" << r.conv_seconds;
+        if (hip)
+            std::cout << ",\"gpu_calls\":" << hip->stats().gpu_calls.load()
+                      << ",\"gpu_batches\":" << hip->stats().gpu_batches.load()
+                      << ",\"cpu_calls\":" << hip->stats().cpu_calls.load()
+                      << ",\"fallback_calls\":" << hip->stats().fallback_calls.load();
+        std::cout << "}" << std::endl;
+        std::cout << "Output written to: " << cfg.output_file << std::endl;
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
+}

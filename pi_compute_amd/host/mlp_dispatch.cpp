@@ -1,0 +1,160 @@
+// mlp_dispatch.cpp - see mlp_dispatch.h.
+#include "mlp_dispatch.h"
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <thread>
+
+#include "pintt_mlp.h"
+
+namespace pi {
+namespace {
+constexpr int IN = PINTT_MLP_INPUTS, H = PINTT_MLP_HIDDEN;
+constexpr int oB1 = H * IN, oW2 = oB1 + H, oB2 = oW2 + H * H, oW3 = oB2 + H;
+}  // namespace
+
+std::array<float, 4> MlpModel::features(const ProductShape& s) {
+    const double big = (double)(s.an > s.bn ? s.an : s.bn), small = (double)(s.an < s.bn ? s.an : s.bn);
+    return {(float)std::log2(big), (float)std::log2(small), (float)std::log2((double)(s.batch ? s.batch : 1)),
+            s.square ? 1.f : 0.f};
+}
+
+bool MlpModel::in_range(const std::array<float, 4>& f) const {
+    for (int j = 0; j < 3; ++j)
+        if (f[j] < lo[j] || f[j] > hi[j]) return false;
+    return true;
+}
+
+std::array<float, 4> MlpModel::standardise(const std::array<float, 4>& f) const {
+    std::array<float, 4> x;
+    for (int j = 0; j < 4; ++j) x[j] = (f[j] - mean[j]) / scale[j];
+    return x;
+}
+
+void MlpModel::forward(const float* x, float* y) const {
+    const float* w = params.data();
+    float h1[H], h2[H];
+    for (int i = 0; i < H; ++i) {
+        float s = w[oB1 + i];
+        for (int j = 0; j < IN; ++j) s += w[i * IN + j] * x[j];
+        h1[i] = s > 0.f ? s : 0.f;
+    }
+    for (int i = 0; i < H; ++i) {
+        float s = w[oB2 + i];
+        for (int j = 0; j < H; ++j) s += w[oW2 + i * H + j] * h1[j];
+        h2[i] = s > 0.f ? s : 0.f;
+    }
+    for (int k = 0; k < outputs; ++k) {
+        float s = w[oW3 + H * outputs + k];
+        for (int j = 0; j < H; ++j) s += w[oW3 + k * H + j] * h2[j];
+        y[k] = s;
+    }
+}
+
+// Text format:
+//   pintt-mlp 1
+//   outputs K
+//   mean m0 m1 m2 m3 / scale ... / lo ... / hi ...
+//   params N
+//   N floats
+bool MlpModel::load(const std::string& path, std::string* err) {
+    std::ifstream f(path);
+    auto bad = [&](const char* what) {
+        if (err) *err = path + ": " + what;
+        return false;
+    };
+    if (!f) return bad("cannot open");
+    std::string tag, key;
+    int version = 0;
+    if (!(f >> tag >> version) || tag != "pintt-mlp" || version != 1) return bad("not a pintt-mlp v1 file");
+    if (!(f >> key >> outputs) || key != "outputs" || outputs < 2 || outputs > PINTT_MLP_MAX_OUTPUTS)
+        return bad("bad outputs");
+    for (auto* arr : {&mean, &scale, &lo, &hi}) {
+        if (!(f >> key)) return bad("truncated");
+        for (float& v : *arr)
+            if (!(f >> v)) return bad("truncated");
+    }
+    size_t n = 0;
+    if (!(f >> key >> n) || key != "params" || n != gpu_mlp_param_count(outputs)) return bad("bad parameter count");
+    params.resize(n);
+    for (float& v : params)
+        if (!(f >> v)) return bad("truncated parameters");
+    for (float s : scale)
+        if (!(s > 0.f)) return bad("non-positive scale");
+    return true;
+}
+
+bool MlpModel::save(const std::string& path) const {
+    std::FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "pintt-mlp 1\noutputs %d\n", outputs);
+    const char* names[4] = {"mean", "scale", "lo", "hi"};
+    const std::array<float, 4>* arrs[4] = {&mean, &scale, &lo, &hi};
+    for (int a = 0; a < 4; ++a) {
+        std::fprintf(f, "%s", names[a]);
+        for (float v : *arrs[a]) std::fprintf(f, " %.9g", v);
+        std::fprintf(f, "\n");
+    }
+    std::fprintf(f, "params %zu\n", params.size());
+    for (size_t i = 0; i < params.size(); ++i) std::fprintf(f, "%.9g%c", params[i], (i % 8 == 7) ? '\n' : ' ');
+    std::fprintf(f, "\n");
+    return std::fclose(f) == 0;
+}
+
+MlpDispatcher::MlpDispatcher(MlpModel model, const Dispatcher* fallback, unsigned cpu_threads)
+    : model_(std::move(model)), fallback_(fallback) {
+    if (!fallback_) throw std::invalid_argument("MlpDispatcher needs a fallback dispatcher");
+    if (model_.params.size() != gpu_mlp_param_count(model_.outputs))
+        throw std::invalid_argument("MlpDispatcher: parameter count does not match the layout");
+    unsigned t = cpu_threads ? cpu_threads : std::max(1u, std::thread::hardware_concurrency());
+    log_threads_ = std::log((float)t);
+}
+
+static MlpModel load_or_throw(const std::string& path) {
+    MlpModel m;
+    std::string err;
+    if (!m.load(path, &err)) throw std::runtime_error("MlpDispatcher: " + err);
+    return m;
+}
+
+MlpDispatcher::MlpDispatcher(const std::string& model_path, const Dispatcher* fallback, unsigned cpu_threads)
+    : MlpDispatcher(load_or_throw(model_path), fallback, cpu_threads) {}
+
+Path MlpDispatcher::decide(const ProductShape& s, const float* y) const {
+    const float cpu = y[0] - (s.batch > 1 ? log_threads_ : 0.f);
+    return y[1] < cpu ? Path::Gpu : Path::Cpu;
+}
+
+Path MlpDispatcher::choose(const ProductShape& s) const {
+    const auto f = MlpModel::features(s);
+    if (!model_.in_range(f)) return fallback_->choose(s);
+    const auto x = model_.standardise(f);
+    float y[PINTT_MLP_MAX_OUTPUTS];
+    model_.forward(x.data(), y);
+    return decide(s, y);
+}
+
+void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) const {
+    if (n < device_batch_) {
+        for (size_t i = 0; i < n; ++i) out[i] = choose(s[i]);
+        return;
+    }
+    std::vector<float> x(n * IN), y(n * model_.outputs);
+    std::vector<char> ok(n);
+    for (size_t i = 0; i < n; ++i) {
+        const auto f = MlpModel::features(s[i]);
+        ok[i] = model_.in_range(f);
+        const auto z = model_.standardise(f);
+        for (int j = 0; j < IN; ++j) x[i * IN + j] = z[j];
+    }
+    if (gpu_mlp_forward(model_.params.data(), model_.outputs, x.data(), n, y.data()) != PINTT_OK) {
+        fallback_->choose_batch(s, n, out);
+        return;
+    }
+    for (size_t i = 0; i < n; ++i) out[i] = ok[i] ? decide(s[i], &y[i * model_.outputs]) : fallback_->choose(s[i]);
+}
+
+}  // namespace pi

@@ -1,0 +1,63 @@
+// mlp_dispatch.h - learned CPU/GPU dispatch: a small FP32 network predicts the
+// log run time of a product on each path; the cheaper path wins.
+//
+// Replaces the reference's fixed crossover (src/arithmetic/int_ntt_multiplier.cpp:56-59)
+// while keeping it as the fallback: shapes outside the range the model was
+// trained on, and any model that fails to load, go to the threshold rule.
+// Either path returns the exact product; only wall time depends on the choice.
+//
+// Inputs (standardised with the model's mean/scale):
+//   log2(larger operand limbs), log2(smaller operand limbs), log2(batch), square
+// Outputs: predicted ln(milliseconds) per product, output 0 = host GMP,
+// output 1 = GPU engine (a device batch's time shared out by transform work).
+// Host products of a batch run on `cpu_threads` threads at once, so within a
+// batch the host prediction is divided by that concurrency.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "dispatch.h"
+
+namespace pi {
+
+struct MlpModel {
+    int outputs = 2;
+    std::array<float, 4> mean{}, scale{}, lo{}, hi{};  // feature standardisation and trained range
+    std::vector<float> params;                         // layout of include/pintt_mlp.h
+
+    static std::array<float, 4> features(const ProductShape& s);
+    bool in_range(const std::array<float, 4>& f) const;
+    std::array<float, 4> standardise(const std::array<float, 4>& f) const;
+    // Host forward pass, same arithmetic order as the device kernel.
+    void forward(const float* x, float* y) const;
+
+    bool load(const std::string& path, std::string* err = nullptr);
+    bool save(const std::string& path) const;
+};
+
+class MlpDispatcher : public Dispatcher {
+public:
+    // Throws std::runtime_error if the model cannot be loaded.
+    MlpDispatcher(const std::string& model_path, const Dispatcher* fallback, unsigned cpu_threads = 0);
+    MlpDispatcher(MlpModel model, const Dispatcher* fallback, unsigned cpu_threads = 0);
+
+    Path choose(const ProductShape& s) const override;
+    void choose_batch(const ProductShape* s, size_t n, Path* out) const override;
+    const char* name() const override { return "mlp"; }
+
+    const MlpModel& model() const { return model_; }
+    // Batches of at least this many products are scored by the device kernel.
+    void set_device_batch(size_t n) { device_batch_ = n; }
+
+private:
+    Path decide(const ProductShape& s, const float* y) const;
+
+    MlpModel model_;
+    const Dispatcher* fallback_;
+    float log_threads_;
+    size_t device_batch_ = 64;
+};
+
+}  // namespace pi

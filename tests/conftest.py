@@ -15,6 +15,8 @@ def pytest_configure(config):
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
 
 
 class Oracle:
@@ -34,6 +36,14 @@ class Oracle:
         L.orc_ntt_mul.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
         L.orc_gmp_mul.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
         L.orc_conv_residues.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, ctypes.c_int, _u32p]
+        L.orc_mlp_param_count.restype = ctypes.c_size_t
+        L.orc_mlp_param_count.argtypes = [ctypes.c_int]
+        L.orc_mlp_forward.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_size_t, _f64p]
+        L.orc_mlp_loss.restype = ctypes.c_double
+        L.orc_mlp_loss.argtypes = [_f64p, ctypes.c_int, _f64p, _i32p, _f64p, ctypes.c_size_t]
+        L.orc_mlp_gradient.restype = ctypes.c_double
+        L.orc_mlp_gradient.argtypes = [_f64p, ctypes.c_int, _f64p, _i32p, _f64p, ctypes.c_size_t, _f64p]
+        L.orc_mlp_adam.argtypes = [_f64p, _f64p, _f64p, _f64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double]
         self.L = L
 
     @staticmethod
@@ -83,6 +93,42 @@ class Oracle:
         zi = np.zeros(1 << log2n, np.uint32)
         self.L.orc_zeta_table(prime, log2n, self._p32(z), self._p32(zi))
         return z, zi
+
+    # --- dispatch cost model (double precision) ---
+    def mlp_param_count(self, k):
+        return self.L.orc_mlp_param_count(k)
+
+    @staticmethod
+    def _f64(a):
+        return np.ascontiguousarray(a, np.float64)
+
+    def mlp_forward(self, w, k, x):
+        w, x = self._f64(w), self._f64(x)
+        y = np.zeros((x.shape[0], k), np.float64)
+        self.L.orc_mlp_forward(w.ctypes.data_as(_f64p), k, x.ctypes.data_as(_f64p), x.shape[0],
+                               y.ctypes.data_as(_f64p))
+        return y
+
+    def mlp_loss(self, w, k, x, path, t):
+        w, x, t = self._f64(w), self._f64(x), self._f64(t)
+        path = np.ascontiguousarray(path, np.int32)
+        return self.L.orc_mlp_loss(w.ctypes.data_as(_f64p), k, x.ctypes.data_as(_f64p),
+                                   path.ctypes.data_as(_i32p), t.ctypes.data_as(_f64p), x.shape[0])
+
+    def mlp_gradient(self, w, k, x, path, t):
+        w, x, t = self._f64(w), self._f64(x), self._f64(t)
+        path = np.ascontiguousarray(path, np.int32)
+        g = np.zeros(w.size, np.float64)
+        L = self.L.orc_mlp_gradient(w.ctypes.data_as(_f64p), k, x.ctypes.data_as(_f64p),
+                                    path.ctypes.data_as(_i32p), t.ctypes.data_as(_f64p), x.shape[0],
+                                    g.ctypes.data_as(_f64p))
+        return g, L
+
+    def mlp_adam(self, w, g, m, v, t, lr):
+        for a in (w, g, m, v):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        self.L.orc_mlp_adam(w.ctypes.data_as(_f64p), g.ctypes.data_as(_f64p), m.ctypes.data_as(_f64p),
+                            v.ctypes.data_as(_f64p), w.size, t, lr)
 
     def crt3(self, r):
         r = np.ascontiguousarray(r, np.uint32)

@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "binary_splitting.h"
+#include "dispatch.h"
 #include "gmp_multiplier.h"
 #include "hip_ntt_multiplier.h"
 
@@ -143,6 +145,63 @@ int main() {
         m.multiply(r, a, b);
         mpz_mul(e, a, b);
         check(mpz_cmp(r, e) == 0 && m.stats().fallback_calls.load() == 1, "OversizeFallsBackExact");
+    }
+
+    // multiply_batch: mixed sizes, signs, zeros and squares in one call, batched
+    // and per-call, with host-path products running beside the device batch.
+    {
+        const size_t n = 40;
+        std::vector<__mpz_struct> x(n), y(n), z(n);
+        for (size_t i = 0; i < n; ++i) {
+            mpz_init(&x[i]);
+            mpz_init(&y[i]);
+            mpz_init(&z[i]);
+            seeded(&x[i], 100 + i, 64 + ((i * 7919) % 40) * 20000);
+            seeded(&y[i], 200 + i, 64 + ((i * 104729) % 37) * 9000);
+            if (i % 5 == 1) mpz_neg(&x[i], &x[i]);
+            if (i % 7 == 3) mpz_neg(&y[i], &y[i]);
+            if (i == 11) mpz_set_ui(&y[i], 0);
+        }
+        std::vector<pi::MulJob> jobs(n);
+        for (size_t i = 0; i < n; ++i) jobs[i] = pi::MulJob{&z[i], &x[i], (i % 3 == 2) ? &x[i] : &y[i]};
+        for (int mode = 0; mode < 3; ++mode) {
+            pi::HipNttMultiplier m(mode == 2 ? 5000 : 1);  // mode 2: part of the batch on the host
+            m.set_batching(mode != 1);
+            for (size_t i = 0; i < n; ++i) mpz_set_ui(&z[i], 12345);
+            m.multiply_batch(jobs.data(), n, 4);
+            bool ok = true;
+            for (size_t i = 0; i < n; ++i) {
+                mpz_mul(e, jobs[i].a, jobs[i].b);
+                ok = ok && mpz_cmp(&z[i], e) == 0;
+            }
+            const char* names[3] = {"BatchMixedSignsZerosSquares", "BatchPerCallMode", "BatchSplitHostAndDevice"};
+            bool split_ok = mode != 2 || (m.stats().cpu_calls.load() > 0 && m.stats().gpu_calls.load() > 0);
+            bool batch_ok = mode != 0 || m.stats().gpu_batches.load() == 1;
+            check(ok && split_ok && batch_ok, names[mode]);
+        }
+        for (size_t i = 0; i < n; ++i) {
+            mpz_clear(&x[i]);
+            mpz_clear(&y[i]);
+            mpz_clear(&z[i]);
+        }
+    }
+
+    // Binary splitting through the GPU multiplier equals the host recursion.
+    {
+        pi::HipNttMultiplier m(64);
+        pi::BinarySplitting bs(m, 8);
+        bs.set_leaf_span(64);
+        pi::BSResult want;
+        pi::BinarySplitting::compute_small(0, 30000, want);
+        pi::BSResult lv = bs.compute(0, 30000);
+        m.set_batching(false);
+        pi::BSResult pc = bs.compute(0, 30000);
+        pi::BSResult rc = bs.compute_recursive(0, 30000);
+        auto eq = [](const pi::BSResult& u, const pi::BSResult& v) {
+            return mpz_cmp(u.P, v.P) == 0 && mpz_cmp(u.Q, v.Q) == 0 && mpz_cmp(u.R, v.R) == 0;
+        };
+        check(eq(lv, want) && eq(pc, want) && eq(rc, want) && m.stats().gpu_calls.load() > 0,
+              "BinarySplittingOnGpuMatchesHost");
     }
 
     mpz_clears(a, b, r, e, NULL);

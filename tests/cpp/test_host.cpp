@@ -1,0 +1,228 @@
+// test_host.cpp - the host layer on the CPU (GMP multiplier only): binary
+// splitting known values (reference tests/test_binary_splitting.cpp:30-110),
+// tree orders agreeing, one batch per level, decimal conversion, pi digits
+// (reference tests/test_validation.cpp:29-60), dispatch rules and the model
+// file.  Usage: test_host PI_1000_FILE.  Exit status = number of failures.
+#include <gmp.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iterator>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "binary_splitting.h"
+#include "decimal.h"
+#include "dispatch.h"
+#include "gmp_multiplier.h"
+#include "mlp_dispatch.h"
+#include "operands.h"
+#include "pi_engine.h"
+#include "pintt_mlp.h"
+
+namespace {
+
+int failures = 0;
+
+void check(bool ok, const std::string& name) {
+    std::printf("%s %s\n", ok ? "PASS" : "FAIL", name.c_str());
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+// GMP multiplier that records how products arrive.
+class CountingMultiplier : public pi::GmpMultiplier {
+public:
+    void multiply(mpz_t r, const mpz_t a, const mpz_t b) override {
+        singles.fetch_add(1);
+        mpz_mul(r, a, b);
+    }
+    void multiply_batch(const pi::MulJob* jobs, size_t count, unsigned threads) override {
+        {
+            std::lock_guard<std::mutex> l(mu);
+            batch_sizes.push_back(count);
+        }
+        (void)threads;
+        for (size_t i = 0; i < count; ++i) mpz_mul(jobs[i].result, jobs[i].a, jobs[i].b);
+    }
+    std::atomic<size_t> singles{0};
+    std::vector<size_t> batch_sizes;
+    std::mutex mu;
+};
+
+bool same(const pi::BSResult& x, const pi::BSResult& y, bool with_p = true) {
+    return (!with_p || mpz_cmp(x.P, y.P) == 0) && mpz_cmp(x.Q, y.Q) == 0 && mpz_cmp(x.R, y.R) == 0;
+}
+
+std::string slurp(const char* path) {
+    std::ifstream f(path, std::ios::binary);
+    return std::string(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    pi::GmpMultiplier gmp;
+    using BS = pi::BinarySplitting;
+
+    check(BS::terms_needed(1000000) >= 70000 && BS::terms_needed(1000000) <= 71000, "TermsNeededFor1MillionDigits");
+    check(BS::terms_needed(10000100) == 705145, "TermsNeeded10MDigitsWithGuard");
+    check(BS::terms_needed(100000100) == 7051375, "TermsNeeded100MDigitsWithGuard");
+
+    {
+        BS bs(gmp, 2);
+        pi::BSResult r0 = bs.compute(0, 1);
+        check(mpz_cmp_ui(r0.P, 1) == 0 && mpz_cmp_ui(r0.Q, 1) == 0 && mpz_cmp_ui(r0.R, 13591409) == 0,
+              "BaseCaseTerm0");
+        pi::BSResult r1 = bs.compute(1, 2);
+        mpz_t q, r;
+        mpz_init_set_str(q, "10939058860032000", 10);
+        mpz_init_set_str(r, "-2793657715", 10);
+        check(mpz_cmp_si(r1.P, -5) == 0 && mpz_cmp(r1.Q, q) == 0 && mpz_cmp(r1.R, r) == 0, "BaseCaseTerm1");
+        pi::BSResult r2 = bs.compute(2, 3);
+        mpz_set_str(q, "87512470880256000", 10);
+        check(mpz_cmp_si(r2.P, -231) == 0 && mpz_cmp(r2.Q, q) == 0, "BaseCaseTerm2");
+        // [0,2): P = P0 P1, Q = Q0 Q1, R = Q1 R0 + P0 R1
+        bs.set_leaf_span(1);
+        pi::BSResult m = bs.compute(0, 2);
+        mpz_set_str(q, "10939058860032000", 10);
+        mpz_mul_ui(r, q, 13591409);
+        mpz_sub_ui(r, r, 2793657715ul);
+        check(mpz_cmp_si(m.P, -5) == 0 && mpz_cmp(m.Q, q) == 0 && mpz_cmp(m.R, r) == 0, "MergeTwoTerms");
+        mpz_clears(q, r, NULL);
+    }
+
+    // Every tree order yields the same integers.
+    for (unsigned long b : {2ul, 3ul, 17ul, 1000ul, 4099ul}) {
+        pi::BSResult ref;
+        BS::compute_small(0, b, ref);
+        for (unsigned long span : {1ul, 16ul, 256ul}) {
+            BS bs(gmp, 4);
+            bs.set_leaf_span(span);
+            pi::BSResult lv = bs.compute(0, b);
+            pi::BSResult lvq = bs.compute(0, b, false);
+            pi::BSResult rc = bs.compute_recursive(0, b);
+            check(same(lv, ref) && same(lvq, ref, false) && same(rc, ref),
+                  "TreeOrdersAgree b=" + std::to_string(b) + " span=" + std::to_string(span));
+        }
+    }
+
+    // One multiply_batch per level, 4 products per merge (3 at the top without P).
+    {
+        CountingMultiplier cm;
+        BS bs(cm, 4);
+        bs.set_leaf_span(8);
+        pi::BSResult r = bs.compute(0, 8 * 37, false);  // 37 leaves
+        size_t expect_merges = 0;
+        bool ok = cm.singles == 0 && cm.batch_sizes.size() == bs.levels().size();
+        for (size_t i = 0; ok && i < cm.batch_sizes.size(); ++i) {
+            const auto& L = bs.levels()[i];
+            bool top = i + 1 == cm.batch_sizes.size();
+            ok = L.products == cm.batch_sizes[i] && L.products == 4 * L.merges - (top ? 1 : 0);
+            expect_merges += L.merges;
+        }
+        check(ok && expect_merges == 36, "OneBatchPerLevel");
+    }
+
+    // Decimal conversion against mpz_get_str.
+    {
+        pi::SplitMix64 g(99);
+        bool ok = true;
+        mpz_t x;
+        mpz_init(x);
+        std::vector<size_t> bits = {1, 5, 64, 1000, 20000, 70000, 333333, 1000003};
+        for (size_t nb : bits) {
+            pi::draw_operand(x, g, nb);
+            char* s = mpz_get_str(nullptr, 10, x);
+            ok = ok && pi::to_decimal(x, 4) == s;
+            free(s);
+        }
+        for (unsigned long k : {1ul, 4095ul, 4096ul, 4097ul, 10000ul, 65537ul}) {
+            for (int d : {-1, 0, 1}) {
+                mpz_ui_pow_ui(x, 10, k);
+                if (d < 0) mpz_sub_ui(x, x, 1);
+                if (d > 0) mpz_add_ui(x, x, 1);
+                char* s = mpz_get_str(nullptr, 10, x);
+                ok = ok && pi::to_decimal(x, 3) == s;
+                free(s);
+            }
+        }
+        mpz_set_ui(x, 0);
+        ok = ok && pi::to_decimal(x) == "0";
+        mpz_clear(x);
+        check(ok, "DecimalMatchesMpzGetStr");
+    }
+
+    // Pi digits.
+    {
+        pi::PiEngine engine(gmp);
+        check(engine.compute_digits(50) == "3.14159265358979323846264338327950288419716939937510", "First50Digits");
+        check(engine.compute_digits(100) ==
+                  "3.14159265358979323846264338327950288419716939937510"
+                  "58209749445923078164062862089986280348253421170679",
+              "First100Digits");
+        if (argc > 1) {
+            std::string want = slurp(argv[1]);
+            pi::PiConfig c;
+            c.digits = 1000;
+            c.level_batches = true;
+            std::string lv = engine.compute(c).digits;
+            c.level_batches = false;
+            std::string rc = engine.compute(c).digits;
+            check(want.size() == 1002 && lv == want && rc == want, "First1000DigitsMatchReferenceFile");
+        }
+    }
+
+    // Dispatch rules.
+    {
+        pi::ThresholdDispatcher t(10000);
+        check(t.choose({9999, 50000, 1, false}) == pi::Path::Cpu && t.choose({10000, 10000, 1, true}) == pi::Path::Gpu &&
+                  t.choose({50000, 9999, 8, false}) == pi::Path::Cpu,
+              "ThresholdRule");
+        pi::MlpModel m;
+        m.outputs = 2;
+        m.mean = {10, 9, 3, 0.3f};
+        m.scale = {4, 4, 2, 0.5f};
+        m.lo = {3, 3, 0, 0};
+        m.hi = {25, 25, 12, 1};
+        m.params.resize(gpu_mlp_param_count(2));
+        pi::SplitMix64 g(5);
+        for (auto& v : m.params) v = (float)(g.uniform() - 0.5);
+        const char* path = "test_host_model.tmp";
+        pi::MlpModel back;
+        bool ok = m.save(path) && back.load(path) && back.params == m.params && back.mean == m.mean &&
+                  back.scale == m.scale && back.lo == m.lo && back.hi == m.hi;
+        std::remove(path);
+        check(ok, "ModelFileRoundTrip");
+        // Out-of-range shapes go to the fallback.
+        pi::FixedDispatcher cpu(pi::Path::Cpu);
+        pi::MlpModel big = m;
+        const int oB3 = 1216 + 32 * 2;
+        for (auto& v : big.params) v = 0;
+        big.params[oB3 + 0] = 5.f;  // host predicted slow, GPU fast
+        big.params[oB3 + 1] = 0.f;
+        pi::MlpDispatcher d(big, &cpu, 16);
+        check(d.choose({1000, 1000, 1, false}) == pi::Path::Gpu &&
+                  d.choose({1u << 27, 1000, 1, false}) == pi::Path::Cpu,
+              "MlpFallbackOutsideTrainedRange");
+    }
+
+    // parallel_for visits every index once.
+    {
+        std::vector<std::atomic<int>> hits(1000);
+        std::vector<double> cost(1000);
+        for (size_t i = 0; i < cost.size(); ++i) cost[i] = (double)((i * 7919) % 1000);
+        pi::parallel_for(
+            hits.size(), 7, [](void* p, size_t i) { (*static_cast<std::vector<std::atomic<int>>*>(p))[i]++; }, &hits,
+            cost.data());
+        bool ok = true;
+        for (auto& h : hits) ok = ok && h.load() == 1;
+        check(ok, "ParallelForCoversEachIndexOnce");
+    }
+
+    std::printf("%d failure(s)\n", failures);
+    return failures;
+}

@@ -17,7 +17,7 @@ def _declared_symbols():
     for h in (ROOT / "include").glob("*.h"):
         text = h.read_text()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names |= set(re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(gpu_ntt_\w+)\s*\(", text, flags=re.M))
+        names |= set(re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(gpu_(?:ntt|mlp)_\w+)\s*\(", text, flags=re.M))
     return names
 
 
@@ -76,3 +76,19 @@ def test_no_silent_cpu_fallback():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="missing"):
         pc.load_library(tmp_path / "libpintt.so")
+
+
+def test_cost_model_abi_without_device():
+    lib = pc.load_library()
+    assert lib.gpu_mlp_param_count(2) == 1216 + 66
+    assert lib.gpu_mlp_param_count(0) == 0 and lib.gpu_mlp_param_count(9) == 0
+    w = np.zeros(1282, np.float32)
+    x = np.zeros((3, 4), np.float32)
+    y = np.zeros((3, 2), np.float32)
+    f = pc._f32p
+    assert lib.gpu_mlp_forward(w.ctypes.data_as(f), 9, x.ctypes.data_as(f), 3, y.ctypes.data_as(f)) == 5
+    from pi_compute_amd import hiprt
+
+    if hiprt.device_count() == 0:
+        # no device: the kernels refuse rather than computing on the host
+        assert lib.gpu_mlp_forward(w.ctypes.data_as(f), 2, x.ctypes.data_as(f), 3, y.ctypes.data_as(f)) == 1

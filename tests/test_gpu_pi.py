@@ -1,0 +1,56 @@
+"""Pi digit files through the GPU multiplier equal the GMP-only run, byte for
+byte, in every dispatch mode (level batches, per-call, depth-first), and the
+C++ host test (tests/cpp/test_hip_multiplier.cpp) covers multiply_batch."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+BIN = ROOT / "pi_compute_amd" / "bin"
+REF_PI = ROOT / "oracle" / "_ref" / "pi_compute_ref"
+MODEL = ROOT / "pi_compute_amd" / "data" / "cost_model.txt"
+pytestmark = pytest.mark.gpu
+
+
+def _run(args, out):
+    print("run:", " ".join(args), flush=True)
+    r = subprocess.run([*args, "--output", str(out)], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.strip().splitlines()[-2], flush=True)
+    return out.read_bytes()
+
+
+@pytest.fixture(scope="module")
+def gmp_digits(tmp_path_factory):
+    d = tmp_path_factory.mktemp("pi")
+    out = {}
+    for digits in (1000000,):
+        exe = [str(REF_PI)] if REF_PI.exists() else [str(BIN / "pi_compute_hip")]
+        out[digits] = _run([*exe, "--digits", str(digits)], d / f"gmp_{digits}.txt")
+    return out
+
+
+@pytest.mark.parametrize(
+    "mode",
+    [
+        ["--tree", "levels"],
+        ["--tree", "levels", "--per-call"],
+        ["--tree", "recursive"],
+        ["--tree", "levels", "--gpu-threshold", "64"],
+        ["--tree", "recursive", "--gpu-threshold", "64"],
+    ],
+    ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64"],
+)
+def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
+    digits = 1000000
+    got = _run([str(BIN / "pi_compute_hip"), "--digits", str(digits), "--gpu", *mode], tmp_path / "gpu.txt")
+    assert got == gmp_digits[digits]
+
+
+def test_pi_gpu_learned_dispatch_matches_gmp_run(tmp_path, gmp_digits):
+    if not MODEL.exists():
+        pytest.skip("no trained cost model in the tree")
+    got = _run([str(BIN / "pi_compute_hip"), "--digits", "1000000", "--gpu", "--dispatch", "mlp", "--model",
+                str(MODEL)], tmp_path / "mlp.txt")
+    assert got == gmp_digits[1000000]
