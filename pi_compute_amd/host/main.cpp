@@ -82,7 +82,7 @@ int main(int argc, char** argv) {
             hip->set_batching(!per_call);
             if (dispatch == "mlp") {
                 if (model.empty()) throw std::runtime_error("--dispatch mlp needs --model FILE");
-                mlp = std::make_unique<pi::MlpDispatcher>(model, &hip->dispatcher());
+                mlp = std::make_unique<pi::MlpDispatcher>(model, &hip->dispatcher(), cfg.num_threads);
                 hip->set_dispatcher(mlp.get());
             } else if (dispatch != "threshold") {
                 throw std::runtime_error("unknown dispatcher " + dispatch);

@@ -1,6 +1,7 @@
 // mlp_dispatch.cpp - see mlp_dispatch.h.
 #include "mlp_dispatch.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -137,12 +138,20 @@ Path MlpDispatcher::choose(const ProductShape& s) const {
     return decide(s, y);
 }
 
+// A batch runs its device products and its host products at the same time,
+// so the batch takes max(sum of device times, sum of host times / threads).
+// Products are ranked by predicted host/device time ratio and the device takes
+// the prefix of that ranking that minimises the larger of the two sums.  With
+// products divisible this ranking is optimal (exchange argument); whole
+// products make the split at most one product's time above the best subset.
 void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) const {
-    if (n < device_batch_) {
-        for (size_t i = 0; i < n; ++i) out[i] = choose(s[i]);
+    if (n == 0) return;
+    if (n == 1) {
+        out[0] = choose(s[0]);
         return;
     }
-    std::vector<float> x(n * IN), y(n * model_.outputs);
+    const int K = model_.outputs;
+    std::vector<float> x(n * IN), y(n * K);
     std::vector<char> ok(n);
     for (size_t i = 0; i < n; ++i) {
         const auto f = MlpModel::features(s[i]);
@@ -150,11 +159,38 @@ void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) con
         const auto z = model_.standardise(f);
         for (int j = 0; j < IN; ++j) x[i * IN + j] = z[j];
     }
-    if (gpu_mlp_forward(model_.params.data(), model_.outputs, x.data(), n, y.data()) != PINTT_OK) {
-        fallback_->choose_batch(s, n, out);
-        return;
+    bool scored = false;
+    if (n >= device_batch_)
+        scored = gpu_mlp_forward(model_.params.data(), K, x.data(), n, y.data()) == PINTT_OK;
+    if (!scored)
+        for (size_t i = 0; i < n; ++i) model_.forward(&x[i * IN], &y[i * K]);
+    const double threads = std::exp((double)log_threads_);
+    std::vector<size_t> free_idx;
+    for (size_t i = 0; i < n; ++i) {
+        if (ok[i]) {
+            free_idx.push_back(i);
+            continue;
+        }
+        out[i] = fallback_->choose(s[i]);  // outside the trained range: the rule decides, cost unknown
     }
-    for (size_t i = 0; i < n; ++i) out[i] = ok[i] ? decide(s[i], &y[i * model_.outputs]) : fallback_->choose(s[i]);
+    std::sort(free_idx.begin(), free_idx.end(), [&](size_t a, size_t b) {
+        return (y[a * K] - y[a * K + 1]) > (y[b * K] - y[b * K + 1]);
+    });
+    double host = 0, dev = 0;
+    for (size_t i : free_idx) host += std::exp((double)y[i * K]);
+    double best = std::max(dev, host / threads);
+    size_t best_k = 0;
+    for (size_t k = 0; k < free_idx.size(); ++k) {
+        const size_t i = free_idx[k];
+        dev += std::exp((double)y[i * K + 1]);
+        host -= std::exp((double)y[i * K]);
+        const double t = std::max(dev, host / threads);
+        if (t < best) {
+            best = t;
+            best_k = k + 1;
+        }
+    }
+    for (size_t k = 0; k < free_idx.size(); ++k) out[free_idx[k]] = k < best_k ? Path::Gpu : Path::Cpu;
 }
 
 }  // namespace pi

@@ -10,8 +10,9 @@
 //   log2(larger operand limbs), log2(smaller operand limbs), log2(batch), square
 // Outputs: predicted ln(milliseconds) per product, output 0 = host GMP,
 // output 1 = GPU engine (a device batch's time shared out by transform work).
-// Host products of a batch run on `cpu_threads` threads at once, so within a
-// batch the host prediction is divided by that concurrency.
+// A single product goes to the path with the lower prediction.  A batch is
+// split so that its device share and its host share (run on `cpu_threads`
+// threads at the same time) finish together as early as the predictions allow.
 #pragma once
 
 #include <array>

@@ -5,7 +5,9 @@
 // file.  Usage: test_host PI_1000_FILE.  Exit status = number of failures.
 #include <gmp.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -208,6 +210,54 @@ int main(int argc, char** argv) {
         check(d.choose({1000, 1000, 1, false}) == pi::Path::Gpu &&
                   d.choose({1u << 27, 1000, 1, false}) == pi::Path::Cpu,
               "MlpFallbackOutsideTrainedRange");
+    }
+
+    // The batch split is within one product of the best max(device sum,
+    // host sum / threads) over all subsets (brute force over 2^12 assignments,
+    // host-side scoring), and never worse than all-host or all-device.
+    {
+        pi::MlpModel m;
+        m.outputs = 2;
+        m.mean = {12, 10, 4, 0.3f};
+        m.scale = {4, 4, 3, 0.5f};
+        m.lo = {0, 0, 0, 0};
+        m.hi = {30, 30, 20, 1};
+        m.params.resize(gpu_mlp_param_count(2));
+        pi::SplitMix64 g(17);
+        for (auto& v : m.params) v = (float)(0.6 * (g.uniform() - 0.5));
+        pi::FixedDispatcher cpu(pi::Path::Cpu);
+        const unsigned threads = 3;
+        pi::MlpDispatcher d(m, &cpu, threads);
+        bool ok = true;
+        for (int trial = 0; trial < 20 && ok; ++trial) {
+            const size_t n = 12;
+            std::vector<pi::ProductShape> sh(n);
+            std::vector<double> c(n), gg(n);
+            for (size_t i = 0; i < n; ++i) {
+                size_t an = 16 + (size_t)(g.uniform() * 200000), bn = 16 + (size_t)(g.uniform() * 200000);
+                sh[i] = pi::ProductShape{an, bn, n, g.uniform() < 0.3};
+                auto z = m.standardise(pi::MlpModel::features(sh[i]));
+                float y[2];
+                m.forward(z.data(), y);
+                c[i] = std::exp((double)y[0]);
+                gg[i] = std::exp((double)y[1]);
+            }
+            std::vector<pi::Path> out(n);
+            d.choose_batch(sh.data(), n, out.data());
+            auto cost = [&](unsigned mask) {
+                double dv = 0, hs = 0;
+                for (size_t i = 0; i < n; ++i) (mask >> i & 1) ? dv += gg[i] : hs += c[i];
+                return std::max(dv, hs / threads);
+            };
+            unsigned chosen = 0;
+            for (size_t i = 0; i < n; ++i) chosen |= (out[i] == pi::Path::Gpu ? 1u : 0u) << i;
+            double best = 1e300, slack = 0;
+            for (unsigned mask = 0; mask < (1u << n); ++mask) best = std::min(best, cost(mask));
+            for (size_t i = 0; i < n; ++i) slack = std::max(slack, std::max(gg[i], c[i] / threads));
+            ok = cost(chosen) <= (best + slack) * (1 + 1e-6) && cost(chosen) <= cost(0) * (1 + 1e-6) &&
+                 cost(chosen) <= cost((1u << n) - 1) * (1 + 1e-6);
+        }
+        check(ok, "BatchSplitWithinOneProductOfBest");
     }
 
     // parallel_for visits every index once.
