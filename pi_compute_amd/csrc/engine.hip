@@ -29,6 +29,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ntt_kernels.h"
@@ -64,6 +65,12 @@ struct Engine {
     size_t io_cap = 0;  // limbs
     uint64_t* io_in = nullptr;
     uint64_t* io_out = nullptr;
+    // Pinned host staging for host operands: gathered by host threads, one
+    // DMA each way per batch (pageable copies go through the runtime's own
+    // small staging buffers, one call per operand).
+    size_t pin_cap = 0;  // limbs
+    uint64_t* pin_in = nullptr;
+    uint64_t* pin_out = nullptr;
     bool timing = false;
     hipEvent_t ev[5] = {};
     bool have_times = false;
@@ -665,6 +672,50 @@ int ensure_io(size_t limbs) {
     return PINTT_OK;
 }
 
+int ensure_pinned(size_t limbs) {
+    if (limbs <= g.pin_cap) return PINTT_OK;
+    if (g.pin_in) (void)hipHostFree(g.pin_in);
+    if (g.pin_out) (void)hipHostFree(g.pin_out);
+    g.pin_in = g.pin_out = nullptr;
+    g.pin_cap = 0;
+    const size_t cap = std::max(limbs, (size_t)1 << 20) + (limbs >> 3);
+    if (hipHostMalloc((void**)&g.pin_in, cap * sizeof(uint64_t)) != hipSuccess ||
+        hipHostMalloc((void**)&g.pin_out, cap * sizeof(uint64_t)) != hipSuccess)
+        return fail(PINTT_ERR_NO_MEMORY, "hipHostMalloc of the staging buffers failed");
+    g.pin_cap = cap;
+    return PINTT_OK;
+}
+
+// Copies pieces (dst, src, limbs) with up to `threads` host threads, each
+// taking an equal share of the total bytes (pieces are cut where needed).
+struct Piece {
+    uint64_t* dst;
+    const uint64_t* src;
+    size_t n;
+};
+
+void parallel_copy(const std::vector<Piece>& pieces, size_t total) {
+    const size_t kPerThread = (size_t)1 << 17;  // 1 MiB of limbs at least per thread
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    size_t nt = std::min<size_t>({(size_t)hw, (size_t)16, std::max<size_t>(1, total / kPerThread)});
+    auto run = [&](size_t lo, size_t hi) {  // limbs [lo, hi) of the concatenation
+        size_t pos = 0;
+        for (const Piece& p : pieces) {
+            size_t a = std::max(lo, pos), b = std::min(hi, pos + p.n);
+            if (a < b) std::memcpy(p.dst + (a - pos), p.src + (a - pos), (b - a) * sizeof(uint64_t));
+            pos += p.n;
+            if (pos >= hi) break;
+        }
+    };
+    if (nt <= 1) {
+        run(0, total);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; ++t) th.emplace_back(run, total * t / nt, total * (t + 1) / nt);
+    for (auto& t : th) t.join();
+}
+
 int check_device_error() {
     uint32_t err = 0;
     PINTT_HIP(hipMemcpyAsync(&err, &g.glob->error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
@@ -696,30 +747,36 @@ int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const u
     PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipSetDevice(g.device));
     PINTT_TRY(ensure_io(std::max(total_in, total_out)));
+    PINTT_TRY(ensure_pinned(std::max(total_in, total_out)));
+    // The previous batch may still read pin_in / write pin_out: it ended with
+    // a synchronize, so the buffers are free here.
     std::vector<Request> reqs(count);
+    std::vector<Piece> gather, scatter;
+    gather.reserve(2 * count);
+    scatter.reserve(count);
     size_t in_off = 0, out_off = 0;
     for (size_t i = 0; i < count; ++i) {
         bool sq = a[i] == b[i] && an[i] == bn[i];
         uint64_t* da = g.io_in + in_off;
-        PINTT_HIP(hipMemcpyAsync(da, a[i], an[i] * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+        gather.push_back(Piece{g.pin_in + in_off, a[i], an[i]});
         in_off += an[i];
         uint64_t* db = da;
         if (!sq) {
             db = g.io_in + in_off;
-            PINTT_HIP(hipMemcpyAsync(db, b[i], bn[i] * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+            gather.push_back(Piece{g.pin_in + in_off, b[i], bn[i]});
             in_off += bn[i];
         }
         reqs[i] = Request{da, an[i], db, bn[i], g.io_out + out_off, sq};
+        scatter.push_back(Piece{out[i], g.pin_out + out_off, an[i] + bn[i]});
         out_off += an[i] + bn[i];
     }
+    parallel_copy(gather, total_in);
+    PINTT_HIP(hipMemcpyAsync(g.io_in, g.pin_in, total_in * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
     PINTT_TRY(enqueue_batch(reqs, g.stream));
-    out_off = 0;
-    for (size_t i = 0; i < count; ++i) {
-        PINTT_HIP(hipMemcpyAsync(out[i], g.io_out + out_off, (an[i] + bn[i]) * sizeof(uint64_t),
-                                 hipMemcpyDeviceToHost, g.stream));
-        out_off += an[i] + bn[i];
-    }
-    return check_device_error();
+    PINTT_HIP(hipMemcpyAsync(g.pin_out, g.io_out, total_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+    PINTT_TRY(check_device_error());
+    parallel_copy(scatter, total_out);
+    return PINTT_OK;
 }
 
 }  // namespace
@@ -754,6 +811,8 @@ void gpu_ntt_shutdown(void) {
     }
     (void)hipFree(g.io_in);
     (void)hipFree(g.io_out);
+    if (g.pin_in) (void)hipHostFree(g.pin_in);
+    if (g.pin_out) (void)hipHostFree(g.pin_out);
     (void)hipFree(g.tune_in);
     (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);

@@ -1,10 +1,15 @@
 // decimal.cpp - see decimal.h.
 #include "decimal.h"
 
+#include <algorithm>
+
 #include <cstring>
 #include <future>
+#include <memory>
 #include <thread>
 #include <vector>
+
+#include "newton.h"
 
 namespace pi {
 namespace {
@@ -85,6 +90,143 @@ std::string to_decimal(mpz_srcptr n, unsigned threads) {
         convert(a, &out[0], width, pw, depth + 1);
     }
     mpz_clear(a);
+    size_t z = out.find_first_not_of('0');
+    if (z != 0) out.erase(0, z);
+    return out;
+}
+
+namespace {
+
+struct Node {
+    mpz_t x;
+    size_t offset, width;  // digits [offset, offset + width) of the output; x < 10^width
+    Node() { mpz_init(x); }
+    ~Node() { mpz_clear(x); }
+    Node(const Node&) = delete;
+    Node& operator=(const Node&) = delete;
+};
+
+}  // namespace
+
+std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t leaf_digits) {
+    if (mpz_sgn(n) == 0) return "0";
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    leaf_digits = std::max<size_t>(leaf_digits, kLeafDigits);
+    const size_t width = mpz_sizeinbase(n, 10);
+    if (width <= 2 * leaf_digits) return to_decimal(n, threads);
+    std::string out(width, '0');
+
+    int k = 0;
+    while (((size_t)2 << k) < width) ++k;  // 2^k < width <= 2^(k+1)
+    // Powers 10^(2^j), j <= k, by squaring through m.
+    std::vector<__mpz_struct> pw(k + 1);
+    for (int j = 0; j <= k; ++j) {
+        mpz_init(&pw[j]);
+        if (j == 0)
+            mpz_set_ui(&pw[j], 10);
+        else
+            m.square(&pw[j], &pw[j - 1]);
+    }
+
+    std::vector<std::unique_ptr<Node>> nodes;
+    nodes.emplace_back(new Node);
+    mpz_abs(nodes[0]->x, n);
+    nodes[0]->offset = 0;
+    nodes[0]->width = width;
+
+    mpz_t inv;
+    mpz_init(inv);
+    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
+        const size_t low = (size_t)1 << j;
+        std::vector<size_t> split;
+        for (size_t i = 0; i < nodes.size(); ++i)
+            if (nodes[i]->width > low) split.push_back(i);
+        if (split.empty()) continue;
+        mpz_srcptr P = &pw[j];
+        const size_t np = mpz_sizeinbase(P, 2);
+        newton_reciprocal(inv, P, np, m);  // ~ 2^(2 np) / P, within 4
+        const size_t sx = np > 64 ? np - 64 : 0;
+        const size_t ns = split.size();
+        std::vector<__mpz_struct> hi(ns), q(ns), qp(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            mpz_init(&hi[s]);
+            mpz_init(&q[s]);
+            mpz_init(&qp[s]);
+            mpz_tdiv_q_2exp(&hi[s], nodes[split[s]]->x, sx);
+        }
+        std::vector<MulJob> jobs(ns);
+        for (size_t s = 0; s < ns; ++s) jobs[s] = MulJob{&q[s], &hi[s], inv};
+        m.multiply_batch(jobs.data(), ns, threads);
+        for (size_t s = 0; s < ns; ++s) {
+            mpz_fdiv_q_2exp(&q[s], &q[s], 2 * np - sx);
+            jobs[s] = MulJob{&qp[s], &q[s], P};
+        }
+        m.multiply_batch(jobs.data(), ns, threads);
+        // Corrections, then the node becomes its high part and a low node joins.
+        std::vector<std::unique_ptr<Node>> lows(ns);
+        struct Ctx {
+            std::vector<std::unique_ptr<Node>>* nodes;
+            const std::vector<size_t>* split;
+            std::vector<__mpz_struct>* q;
+            std::vector<__mpz_struct>* qp;
+            std::vector<std::unique_ptr<Node>>* lows;
+            mpz_srcptr P;
+            size_t low;
+        } ctx{&nodes, &split, &q, &qp, &lows, P, low};
+        parallel_for(
+            ns, threads,
+            [](void* p, size_t s) {
+                auto* c = static_cast<Ctx*>(p);
+                Node& nd = *(*c->nodes)[(*c->split)[s]];
+                mpz_ptr qq = &(*c->q)[s];
+                auto* lo = new Node;
+                mpz_sub(lo->x, nd.x, &(*c->qp)[s]);  // remainder estimate
+                while (mpz_sgn(lo->x) < 0) {
+                    mpz_add(lo->x, lo->x, c->P);
+                    mpz_sub_ui(qq, qq, 1);
+                }
+                while (mpz_cmp(lo->x, c->P) >= 0) {
+                    mpz_sub(lo->x, lo->x, c->P);
+                    mpz_add_ui(qq, qq, 1);
+                }
+                lo->width = c->low;
+                lo->offset = nd.offset + nd.width - c->low;
+                nd.width -= c->low;
+                mpz_swap(nd.x, qq);
+                (*c->lows)[s].reset(lo);
+            },
+            &ctx);
+        for (auto& lo : lows) nodes.push_back(std::move(lo));
+        for (size_t s = 0; s < ns; ++s) {
+            mpz_clear(&hi[s]);
+            mpz_clear(&q[s]);
+            mpz_clear(&qp[s]);
+        }
+    }
+    mpz_clear(inv);
+    for (auto& p : pw) mpz_clear(&p);
+
+    // Leaves: host conversion, one node per task (largest first).
+    Powers leaf_pw(2 * leaf_digits);  // shared read-only by every task
+    struct LeafCtx {
+        std::vector<std::unique_ptr<Node>>* nodes;
+        std::string* out;
+        const Powers* pw;
+    } lctx{&nodes, &out, &leaf_pw};
+    std::vector<double> cost(nodes.size());
+    for (size_t i = 0; i < nodes.size(); ++i) cost[i] = (double)nodes[i]->width;
+    parallel_for(
+        nodes.size(), threads,
+        [](void* p, size_t i) {
+            auto* c = static_cast<LeafCtx*>(p);
+            Node& nd = *(*c->nodes)[i];
+            if (nd.width <= kLeafDigits) {
+                leaf(nd.x, &(*c->out)[nd.offset], nd.width);
+            } else {
+                convert(nd.x, &(*c->out)[nd.offset], nd.width, *c->pw, 0);
+            }
+        },
+        &lctx, cost.data());
     size_t z = out.find_first_not_of('0');
     if (z != 0) out.erase(0, z);
     return out;

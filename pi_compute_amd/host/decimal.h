@@ -13,8 +13,18 @@
 #include <cstddef>
 #include <string>
 
+#include "multiplier.h"
+
 namespace pi {
 
 std::string to_decimal(mpz_srcptr n, unsigned threads = 0);
+
+// Same digits; the splits above `leaf_digits` run level by level, every node
+// of a level divided by the same power 10^(2^j) at once: quotient estimate
+// q = (top bits of x) * (Newton reciprocal of 10^(2^j)) / 2^s, remainder
+// x - q * 10^(2^j), both products of a level as one multiply_batch, then at
+// most a few +-1 corrections on the host.  Below leaf_digits the host
+// conversion above takes over, one subtree per thread.
+std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads = 0, size_t leaf_digits = 1 << 15);
 
 }  // namespace pi

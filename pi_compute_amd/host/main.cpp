@@ -12,6 +12,10 @@
 //   --per-call       with --tree levels: device products one engine call each
 //   --dispatch mlp   learned cost model from --model (threshold rule otherwise)
 //   --log FILE       per-product timing lines (training data for the cost model)
+//   --final mpf|newton        final quotient and square root: GMP mpf (the
+//                             reference's sequence) or Newton through the
+//                             multiplier (default: newton with --gpu)
+//   --convert host|multiplier decimal conversion (default: multiplier with --gpu)
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -29,7 +33,8 @@ namespace {
 void usage() {
     std::cout << "usage: pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]\n"
                  "                      [--threads T] [--tree levels|recursive] [--per-call]\n"
-                 "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE] [--verbose]\n";
+                 "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
+                 "                      [--final mpf|newton] [--convert host|multiplier] [--verbose]\n";
 }
 
 }  // namespace
@@ -39,7 +44,7 @@ int main(int argc, char** argv) {
     cfg.output_file = "pi_digits.txt";
     bool gpu = false, per_call = false;
     size_t threshold = 10000;
-    std::string dispatch = "threshold", model, log;
+    std::string dispatch = "threshold", model, log, final_step, convert;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -63,10 +68,20 @@ int main(int argc, char** argv) {
         else if (a == "--dispatch") dispatch = next();
         else if (a == "--model") model = next();
         else if (a == "--log") log = next();
+        else if (a == "--final") final_step = next();
+        else if (a == "--convert") convert = next();
         else if (a == "--verbose" || a == "-v") cfg.verbose = true;
         else if (a == "--help" || a == "-h") { usage(); return 0; }
         else { std::cerr << "unknown option " << a << std::endl; usage(); return 2; }
     }
+    if (final_step.empty()) final_step = gpu ? "newton" : "mpf";
+    if (convert.empty()) convert = gpu ? "multiplier" : "host";
+    if ((final_step != "mpf" && final_step != "newton") || (convert != "host" && convert != "multiplier")) {
+        usage();
+        return 2;
+    }
+    cfg.final_step = final_step == "newton" ? pi::PiConfig::FinalStep::Newton : pi::PiConfig::FinalStep::Mpf;
+    cfg.conversion = convert == "multiplier" ? pi::PiConfig::Conversion::Multiplier : pi::PiConfig::Conversion::Host;
     if (cfg.digits == 0) {
         std::cerr << "--digits must be > 0" << std::endl;
         return 2;
@@ -89,7 +104,8 @@ int main(int argc, char** argv) {
             }
             if (!log.empty() && !hip->open_log(log)) throw std::runtime_error("cannot open log " + log);
             mode = std::string("HIP NTT, ") + hip->device_name() + ", " + hip->dispatcher().name() +
-                   " dispatch, " + (cfg.level_batches ? (per_call ? "levels/per-call" : "levels/batched") : "recursive");
+                   " dispatch, " + (cfg.level_batches ? (per_call ? "levels/per-call" : "levels/batched") : "recursive") +
+                   ", final " + final_step + ", conversion " + convert;
             m = std::move(h);
         } else {
             m = std::make_unique<pi::GmpMultiplier>();

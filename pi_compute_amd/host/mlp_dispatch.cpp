@@ -141,9 +141,10 @@ Path MlpDispatcher::choose(const ProductShape& s) const {
 // A batch runs its device products and its host products at the same time,
 // so the batch takes max(sum of device times, sum of host times / threads).
 // Products are ranked by predicted host/device time ratio and the device takes
-// the prefix of that ranking that minimises the larger of the two sums.  With
-// products divisible this ranking is optimal (exchange argument); whole
-// products make the split at most one product's time above the best subset.
+// the prefix of that ranking that minimises the batch time, taken as
+// max(device sum, host sum / threads, longest host product).  With products
+// divisible this ranking is optimal (exchange argument); whole products make
+// the split at most one product's time above the best subset.
 void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) const {
     if (n == 0) return;
     if (n == 1) {
@@ -176,15 +177,20 @@ void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) con
     std::sort(free_idx.begin(), free_idx.end(), [&](size_t a, size_t b) {
         return (y[a * K] - y[a * K + 1]) > (y[b * K] - y[b * K + 1]);
     });
+    // A host product runs on one thread: the host share takes at least its
+    // longest product (suffix maxima of the ranking).
+    const size_t nf = free_idx.size();
+    std::vector<double> longest(nf + 1, 0.0);
+    for (size_t k = nf; k-- > 0;) longest[k] = std::max(longest[k + 1], std::exp((double)y[free_idx[k] * K]));
     double host = 0, dev = 0;
     for (size_t i : free_idx) host += std::exp((double)y[i * K]);
-    double best = std::max(dev, host / threads);
+    double best = std::max({dev, host / threads, longest[0]});
     size_t best_k = 0;
-    for (size_t k = 0; k < free_idx.size(); ++k) {
+    for (size_t k = 0; k < nf; ++k) {
         const size_t i = free_idx[k];
         dev += std::exp((double)y[i * K + 1]);
         host -= std::exp((double)y[i * K]);
-        const double t = std::max(dev, host / threads);
+        const double t = std::max({dev, host / threads, longest[k + 1]});
         if (t < best) {
             best = t;
             best_k = k + 1;

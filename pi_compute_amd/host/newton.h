@@ -1,0 +1,34 @@
+// newton.h - reciprocal, inverse square root and powers with every large
+// product sent through a Multiplier (the GPU engine when it is one).
+//
+// The reference computes the final quotient and square root with GMP's mpf
+// (NewtonDivider::sqrt_to_precision, src/arithmetic/newton_divider.cpp:34-39;
+// src/engine/pi_engine.cpp:433-470), single-threaded.  These routines are the
+// fixed-point Newton iterations (precision doubling; Brent & Zimmermann,
+// "Modern Computer Arithmetic", sections 3.4 and 3.5) whose large products
+// are ordinary Multiplier calls.  Results carry a bounded error in the last
+// place; callers keep guard bits.
+#pragma once
+
+#include <gmp.h>
+
+#include <cstddef>
+
+#include "multiplier.h"
+
+namespace pi {
+
+// y ~ 2^(n+b) / d, n = bit length of d > 0; |y - 2^(n+b)/d| <= 4.
+void newton_reciprocal(mpz_t y, mpz_srcptr d, size_t b, Multiplier& m);
+
+// y ~ 2^b / sqrt(a), a > 0; |y - 2^b/sqrt(a)| <= 4.
+void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m);
+
+// y = base^e exactly; the squarings go through m.
+void pow_ui(mpz_t y, unsigned long base, unsigned long e, Multiplier& m);
+
+// y = |x| >> s with s chosen so that y keeps the top `bits` bits of |x|
+// (s = 0 when |x| is shorter); returns s.  y may alias x.
+size_t truncate_top(mpz_t y, mpz_srcptr x, size_t bits);
+
+}  // namespace pi

@@ -7,6 +7,7 @@
 #include <stdexcept>
 
 #include "decimal.h"
+#include "newton.h"
 
 namespace pi {
 
@@ -35,36 +36,44 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const auto t1 = Clock::now();
     mpz_realloc2(r.P, 0);
     const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
-    mpf_t s, num, den, x;
-    mpf_init2(s, bits);
-    mpf_init2(num, bits);
-    mpf_init2(den, bits);
-    mpf_init2(x, bits);
-    mpf_set_ui(s, 10005);
-    mpf_sqrt(s, s);
-    mpz_mul_ui(r.Q, r.Q, 426880);
-    mpf_set_z(num, r.Q);
-    mpz_realloc2(r.Q, 0);
-    mpf_set_z(den, r.R);
-    mpz_realloc2(r.R, 0);
-    mpf_mul(x, num, s);
-    mpf_div(x, x, den);
-    mpz_t scale, pi_int;
-    mpz_init(scale);
+    mpz_t pi_int;
     mpz_init(pi_int);
-    mpz_ui_pow_ui(scale, 10, cfg.digits);
-    mpf_set_z(num, scale);
-    mpz_clear(scale);
-    mpf_mul(x, x, num);
-    mpz_set_f(pi_int, x);
-    mpf_clear(s);
-    mpf_clear(num);
-    mpf_clear(den);
-    mpf_clear(x);
+    if (cfg.final_step == PiConfig::FinalStep::Newton && mpz_sgn(r.R) > 0) {
+        pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, bits + 64, mult_);
+        mpz_realloc2(r.Q, 0);
+        mpz_realloc2(r.R, 0);
+    } else {
+        mpf_t s, num, den, x;
+        mpf_init2(s, bits);
+        mpf_init2(num, bits);
+        mpf_init2(den, bits);
+        mpf_init2(x, bits);
+        mpf_set_ui(s, 10005);
+        mpf_sqrt(s, s);
+        mpz_mul_ui(r.Q, r.Q, 426880);
+        mpf_set_z(num, r.Q);
+        mpz_realloc2(r.Q, 0);
+        mpf_set_z(den, r.R);
+        mpz_realloc2(r.R, 0);
+        mpf_mul(x, num, s);
+        mpf_div(x, x, den);
+        mpz_t scale;
+        mpz_init(scale);
+        mpz_ui_pow_ui(scale, 10, cfg.digits);
+        mpf_set_z(num, scale);
+        mpz_clear(scale);
+        mpf_mul(x, x, num);
+        mpz_set_f(pi_int, x);
+        mpf_clear(s);
+        mpf_clear(num);
+        mpf_clear(den);
+        mpf_clear(x);
+    }
     const double fin_s = since(t1);
 
     const auto t2 = Clock::now();
-    std::string d = to_decimal(pi_int, cfg.num_threads);
+    std::string d = cfg.conversion == PiConfig::Conversion::Multiplier ? to_decimal(pi_int, mult_, cfg.num_threads)
+                                                                      : to_decimal(pi_int, cfg.num_threads);
     mpz_clear(pi_int);
     std::string out;
     out.reserve(cfg.digits + 2);
@@ -83,6 +92,34 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     if (cfg.verbose)
         std::cout << "  final step: " << fin_s << " s, decimal conversion: " << conv_s << " s" << std::endl;
     return PiResult{std::move(out), since(t0), terms, bs_s, fin_s, conv_s};
+}
+
+void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t w, Multiplier& m) {
+    // 426880 * sqrt(10005) * Q * 10^D / R
+    //   = 10005 * (426880 Q) * (2^w / sqrt(10005)) * 10^D * (2^(nr+w) / Rt) / 2^(...)
+    // with every operand cut to its top w bits and the exponents of 2 tracked.
+    mpz_t a, y, t, inv;
+    mpz_inits(a, y, t, inv, NULL);
+    long ex = 0;  // value = integer * 2^ex
+    mpz_mul_ui(a, Q, 426880);
+    ex += (long)truncate_top(a, a, w);
+    newton_inv_sqrt_ui(y, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
+    m.multiply(t, a, y);
+    ex += (long)truncate_top(t, t, w) - (long)w;
+    pow_ui(a, 10, digits, m);
+    m.multiply(y, t, a);
+    ex += (long)truncate_top(y, y, w);
+    const size_t sr = truncate_top(t, R, w);
+    const size_t nr = mpz_sizeinbase(t, 2);
+    newton_reciprocal(inv, t, w, m);  // 1/R ~ inv / 2^(nr + w + sr)
+    ex -= (long)(nr + w + sr);
+    m.multiply(a, y, inv);
+    mpz_mul_ui(a, a, 10005);
+    if (ex >= 0)
+        mpz_mul_2exp(out, a, (mp_bitcnt_t)ex);
+    else
+        mpz_fdiv_q_2exp(out, a, (mp_bitcnt_t)(-ex));
+    mpz_clears(a, y, t, inv, NULL);
 }
 
 std::string PiEngine::compute_digits(size_t digits) {

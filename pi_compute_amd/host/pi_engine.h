@@ -6,6 +6,13 @@
 //   (P,Q,R) = binary splitting over [0, terms)
 //   pi      = 426880 * Q * sqrt(10005) / R       at (D+100)*3.3219281 + 64 bits
 //   digits  = "3." + the first D decimals of floor(pi * 10^D), no newline
+//
+// Final step: FinalStep::Mpf is the reference's mpf sequence; FinalStep::Newton
+// computes the same quotient with fixed-point Newton iterations (newton.h)
+// whose products go through the Multiplier, 64 bits beyond that precision.
+// Conversion::Multiplier splits the decimal conversion with Newton quotients
+// through the Multiplier as well (decimal.h).  Either way the digits are the
+// truncation of the same value: the error is below 10^-(D+90) of pi * 10^D.
 #pragma once
 
 #include <cstddef>
@@ -22,6 +29,8 @@ struct PiConfig {
     bool verbose = false;
     unsigned num_threads = 0;  // 0: hardware concurrency
     bool level_batches = true;  // level-synchronous tree with one batch per level
+    enum class FinalStep { Mpf, Newton } final_step = FinalStep::Mpf;
+    enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
 };
 
 struct PiResult {
@@ -30,6 +39,10 @@ struct PiResult {
     unsigned long terms_used;
     double bs_seconds, final_seconds, conv_seconds;
 };
+
+// floor(426880 * sqrt(10005) * Q * 10^digits / R) with `bits` of working
+// precision, products through m (R > 0).  Used by FinalStep::Newton.
+void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t bits, Multiplier& m);
 
 class PiEngine {
 public:

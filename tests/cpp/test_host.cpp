@@ -21,6 +21,7 @@
 #include "dispatch.h"
 #include "gmp_multiplier.h"
 #include "mlp_dispatch.h"
+#include "newton.h"
 #include "operands.h"
 #include "pi_engine.h"
 #include "pintt_mlp.h"
@@ -158,6 +159,76 @@ int main(int argc, char** argv) {
         check(ok, "DecimalMatchesMpzGetStr");
     }
 
+    // Newton reciprocal / inverse square root within 4 units of the exact floor.
+    {
+        pi::SplitMix64 g(31);
+        bool ok = true;
+        mpz_t d, y, want, t;
+        mpz_inits(d, y, want, t, NULL);
+        const size_t dbits[] = {1, 7, 64, 300, 5000, 70001, 400000};
+        const size_t bs[] = {1, 100, 257, 1000, 33333, 300000};
+        for (size_t nb : dbits)
+            for (size_t b : bs) {
+                pi::draw_operand(d, g, nb);
+                pi::newton_reciprocal(y, d, b, gmp);
+                size_t n = mpz_sizeinbase(d, 2);
+                mpz_set_ui(t, 1);
+                mpz_mul_2exp(t, t, n + b);
+                mpz_tdiv_q(want, t, d);
+                mpz_sub(t, y, want);
+                mpz_abs(t, t);
+                ok = ok && mpz_cmp_ui(t, 4) <= 0;
+            }
+        check(ok, "NewtonReciprocalWithin4");
+        ok = true;
+        for (unsigned long a : {2ul, 3ul, 10005ul, 999999937ul})
+            for (size_t b : bs) {
+                pi::newton_inv_sqrt_ui(y, a, b, gmp);
+                mpz_set_ui(t, 1);
+                mpz_mul_2exp(t, t, 2 * b);
+                mpz_tdiv_q_ui(t, t, a);
+                mpz_sqrt(want, t);
+                mpz_sub(t, y, want);
+                mpz_abs(t, t);
+                ok = ok && mpz_cmp_ui(t, 4) <= 0;
+            }
+        check(ok, "NewtonInverseSqrtWithin4");
+        ok = true;
+        for (unsigned long e : {0ul, 1ul, 2ul, 77ul, 1000ul, 123457ul}) {
+            pi::pow_ui(y, 10, e, gmp);
+            mpz_ui_pow_ui(want, 10, e);
+            ok = ok && mpz_cmp(y, want) == 0;
+        }
+        check(ok, "PowerBySquaring");
+        mpz_clears(d, y, want, t, NULL);
+    }
+
+    // Decimal conversion with Newton quotients (small leaves force many levels).
+    {
+        pi::SplitMix64 g(77);
+        bool ok = true;
+        mpz_t x;
+        mpz_init(x);
+        for (size_t nb : {20000ul, 100000ul, 654321ul, 2000000ul}) {
+            pi::draw_operand(x, g, nb);
+            char* s = mpz_get_str(nullptr, 10, x);
+            ok = ok && pi::to_decimal(x, gmp, 4, 4096) == s;
+            free(s);
+        }
+        for (unsigned long k : {8192ul, 65536ul, 100000ul, 131072ul}) {
+            for (int d : {-1, 0, 1}) {
+                mpz_ui_pow_ui(x, 10, k);
+                if (d < 0) mpz_sub_ui(x, x, 1);
+                if (d > 0) mpz_add_ui(x, x, 1);
+                char* s = mpz_get_str(nullptr, 10, x);
+                ok = ok && pi::to_decimal(x, gmp, 3, 4096) == s;
+                free(s);
+            }
+        }
+        mpz_clear(x);
+        check(ok, "DecimalWithNewtonQuotients");
+    }
+
     // Pi digits.
     {
         pi::PiEngine engine(gmp);
@@ -175,6 +246,17 @@ int main(int argc, char** argv) {
             c.level_batches = false;
             std::string rc = engine.compute(c).digits;
             check(want.size() == 1002 && lv == want && rc == want, "First1000DigitsMatchReferenceFile");
+            c.final_step = pi::PiConfig::FinalStep::Newton;
+            c.conversion = pi::PiConfig::Conversion::Multiplier;
+            check(engine.compute(c).digits == want, "First1000DigitsNewtonFinalStep");
+        }
+        for (size_t digits : {1ul, 2ul, 17ul, 5000ul, 250000ul}) {
+            pi::PiConfig a, b;
+            a.digits = b.digits = digits;
+            b.final_step = pi::PiConfig::FinalStep::Newton;
+            b.conversion = pi::PiConfig::Conversion::Multiplier;
+            check(engine.compute(a).digits == engine.compute(b).digits,
+                  "NewtonFinalStepMatchesMpf D=" + std::to_string(digits));
         }
     }
 
@@ -213,8 +295,9 @@ int main(int argc, char** argv) {
     }
 
     // The batch split is within one product of the best max(device sum,
-    // host sum / threads) over all subsets (brute force over 2^12 assignments,
-    // host-side scoring), and never worse than all-host or all-device.
+    // host sum / threads, longest host product) over all subsets (brute force
+    // over 2^12 assignments, host-side scoring), and never worse than all-host
+    // or all-device.
     {
         pi::MlpModel m;
         m.outputs = 2;
@@ -245,9 +328,16 @@ int main(int argc, char** argv) {
             std::vector<pi::Path> out(n);
             d.choose_batch(sh.data(), n, out.data());
             auto cost = [&](unsigned mask) {
-                double dv = 0, hs = 0;
-                for (size_t i = 0; i < n; ++i) (mask >> i & 1) ? dv += gg[i] : hs += c[i];
-                return std::max(dv, hs / threads);
+                double dv = 0, hs = 0, lg = 0;
+                for (size_t i = 0; i < n; ++i) {
+                    if (mask >> i & 1) {
+                        dv += gg[i];
+                    } else {
+                        hs += c[i];
+                        lg = std::max(lg, c[i]);
+                    }
+                }
+                return std::max({dv, hs / threads, lg});
             };
             unsigned chosen = 0;
             for (size_t i = 0; i < n; ++i) chosen |= (out[i] == pi::Path::Gpu ? 1u : 0u) << i;

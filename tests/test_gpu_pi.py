@@ -1,6 +1,7 @@
 """Pi digit files through the GPU multiplier equal the GMP-only run, byte for
-byte, in every dispatch mode (level batches, per-call, depth-first), and the
-C++ host test (tests/cpp/test_hip_multiplier.cpp) covers multiply_batch."""
+byte, in every dispatch mode (level batches, per-call, depth-first), with the
+Newton final step and the Newton-quotient decimal conversion (the defaults
+with --gpu) and with the reference's mpf final step."""
 import subprocess
 from pathlib import Path
 
@@ -39,8 +40,11 @@ def gmp_digits(tmp_path_factory):
         ["--tree", "recursive"],
         ["--tree", "levels", "--gpu-threshold", "64"],
         ["--tree", "recursive", "--gpu-threshold", "64"],
+        ["--tree", "levels", "--final", "mpf", "--convert", "host"],
+        ["--tree", "levels", "--gpu-threshold", "64", "--convert", "multiplier"],
     ],
-    ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64"],
+    ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64", "levels-mpf-host",
+         "levels-t64-convert"],
 )
 def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
     digits = 1000000
