@@ -10,16 +10,21 @@ that stream (device memory, stream and events come from the HIP runtime the
 engine itself uses, see pi_compute_amd/hiprt.py).
 
 Prints one JSON line: mults/s, the dominant pass's roofline (algorithmic bytes
-over its HIP-event time), and the reference's own GmpMultiplier timed on this
-host's cores on a bounded sample as the CPU baseline.
+over its HIP-event time), the reference's own GmpMultiplier timed on this
+host's cores on a bounded sample as the CPU baseline, the tree-level replay
+batch (4,096 products: per call, one batch call, one graph replay), and the
+100M-digit pi leg (GMP-only run against the GPU runs, digit files compared).
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
+import hashlib
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import threading
 import time
 from pathlib import Path
@@ -274,6 +279,54 @@ def batch_cpu_baseline(prods, out, o_off, budget_s: float):
             "bit_exact_vs_gpu": state["exact"], "products_checked": state["done"]}
 
 
+def _pi_run(args, out: Path) -> dict:
+    """One pi_compute_hip run; returns its JSON summary line plus the wall time."""
+    t0 = time.perf_counter()
+    r = subprocess.run([*args, "--output", str(out)], capture_output=True, text=True, timeout=900)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(args)} failed: {r.stderr.strip()[-400:]}")
+    line = next((l for l in r.stdout.splitlines() if l.startswith("{")), "{}")
+    info = json.loads(line)
+    info["wall_s"] = round(wall, 3)
+    return info
+
+
+def pi_leg(digits: int) -> dict:
+    """Chudnovsky pi at `digits`: the GMP-only run (the reference's algorithm,
+    our host build) against GPU runs with level batches + threshold dispatch,
+    level batches + learned dispatch, and per-call dispatch; every digit file
+    must equal the GMP-only file byte for byte."""
+    exe = str(ROOT / "pi_compute_amd" / "bin" / "pi_compute_hip")
+    model = ROOT / "pi_compute_amd" / "data" / "cost_model.txt"
+    runs = {
+        "gmp_only": [exe, "--digits", str(digits)],
+        "gpu_levels_threshold": [exe, "--digits", str(digits), "--gpu"],
+        "gpu_per_call_threshold": [exe, "--digits", str(digits), "--gpu", "--per-call"],
+    }
+    if model.exists():
+        runs["gpu_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", "--dispatch", "mlp", "--model", str(model)]
+    out = {"digits": digits, "runs": {}}
+    with tempfile.TemporaryDirectory(prefix="pi_bench_") as d:
+        ref_bytes = None
+        for name, args in runs.items():
+            print(f"pi leg: {name}", file=sys.stderr, flush=True)
+            path = Path(d) / f"{name}.txt"
+            info = _pi_run(args, path)
+            data = path.read_bytes()
+            if ref_bytes is None:
+                ref_bytes = data
+                info["sha256"] = hashlib.sha256(data).hexdigest()
+            else:
+                info["identical_to_gmp_only"] = data == ref_bytes
+            path.unlink()
+            out["runs"][name] = {k: info[k] for k in info if k != "digits"}
+    g = out["runs"]["gmp_only"]["seconds"]
+    out["speedup_vs_gmp_only"] = {k: round(g / v["seconds"], 2) for k, v in out["runs"].items() if k != "gmp_only"}
+    out["all_identical"] = all(v.get("identical_to_gmp_only", True) for v in out["runs"].values())
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +336,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the tree-level replay batch leg")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--pi-digits", type=int, default=100_000_000,
+                    help="digits of the pi leg (GMP-only vs GPU runs of pi_compute_hip); 0 skips it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -356,6 +411,8 @@ def main() -> int:
         result["cpu_baseline"] = cpu_baseline(a, b, gpu_out, args.cpu_budget)
     if not args.no_batch:
         result["batch_replay"] = batch_replay(stream, 0 if args.no_cpu else args.cpu_budget / 2, args.batch_reps)
+    if args.pi_digits:
+        result["pi"] = pi_leg(args.pi_digits)
     print(json.dumps(result), flush=True)
     return 0
 

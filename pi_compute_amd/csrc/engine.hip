@@ -30,6 +30,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "ntt_kernels.h"
@@ -44,6 +45,7 @@ struct Engine {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;  // end of the last batch
+    hipStream_t last = nullptr; // stream that batch ran on
     int tw_log2 = 0;
     uint2* tw_fwd[kPrimes] = {};
     uint2* tw_inv[kPrimes] = {};
@@ -487,23 +489,42 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, un
 }
 
 // The kernel launches of a batch whose descriptors sit at device address D.
-int enqueue_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t s, bool timed) {
+// True when every launch of the plan can carry its jobs in kernel arguments.
+bool fits_inline(const BatchPlan& bp) {
+    for (const BucketLayout& b : bp.buckets) {
+        if (b.nfwd > (uint32_t)kInlineJobs || b.ninv > (uint32_t)kInlineJobs || b.groups.size() > (size_t)kInlineJobs)
+            return false;
+        for (const RowLaunch& rl : b.rows)
+            if (rl.count > (uint32_t)kInlineJobs) return false;
+    }
+    return true;
+}
+
+// D: device copy of the descriptors (may be null when H is given and every
+// launch fits inline); H: host copy (null inside captures, which always read
+// device descriptors).
+int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned char* H, hipStream_t s,
+                     bool timed) {
     const Twiddles tw = twiddles();
+    auto list = [&](auto* tag, size_t off, uint32_t n) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        if (H && n <= (uint32_t)kInlineJobs) return inline_jobs<T>(reinterpret_cast<const T*>(H + off), n);
+        return device_jobs<T>(reinterpret_cast<const T*>(D + off));
+    };
     for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
         const BucketLayout& b = bp.buckets[bi];
         const Plan& pl = bp.plans[bi];
         if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
-        PINTT_HIP(launch_forward_columns(pl, reinterpret_cast<const FwdJob*>(D + b.fwd_off), b.nfwd, tw, s));
+        PINTT_HIP(launch_forward_columns(pl, list((FwdJob*)nullptr, b.fwd_off, b.nfwd), b.nfwd, tw, s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
         for (const RowLaunch& rl : b.rows)
-            PINTT_HIP(launch_row_convolution(pl, reinterpret_cast<const RowJob*>(D + rl.off), rl.count, rl.kind,
-                                             tw, s));
+            PINTT_HIP(launch_row_convolution(pl, list((RowJob*)nullptr, rl.off, rl.count), rl.count, rl.kind, tw, s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
-        PINTT_HIP(launch_inverse_columns(pl, reinterpret_cast<const InvJob*>(D + b.inv_off), b.ninv, g.glob,
-                                         tw, s));
+        PINTT_HIP(launch_inverse_columns(pl, list((InvJob*)nullptr, b.inv_off, b.ninv), b.ninv, g.glob, tw, s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
-        PINTT_HIP(launch_crt_carry(reinterpret_cast<const CarryJob*>(D + b.carry_off), (uint32_t)b.groups.size(),
-                                   b.nblocks, b.log2n, crt_constants(b.log2n), g.glob, s));
+        const uint32_t ng = (uint32_t)b.groups.size();
+        PINTT_HIP(launch_crt_carry(list((CarryJob*)nullptr, b.carry_off, ng), ng, b.nblocks, b.log2n,
+                                   crt_constants(b.log2n), g.glob, s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
     }
     return PINTT_OK;
@@ -517,19 +538,29 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
     PINTT_TRY(ensure_twiddles(bp.max_log2n));
     PINTT_TRY(ensure_arena(bp.arena_words));
     PINTT_TRY(ensure_carry_state(bp.total_groups, bp.total_blocks));
-    const int ring = g.desc_next;
-    g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
-    PINTT_TRY(ensure_descriptors(ring, bp.bytes));
-    fill_descriptors(bp, g.desc_host[ring], g.arena, g.tickets, g.status);
-    PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bp.bytes, hipMemcpyHostToDevice, g.stream));
-    PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
-    // A caller stream first waits for the upload, which itself is ordered
-    // after the previous batch on the library stream.
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
-    PINTT_TRY(enqueue_launches(bp, g.desc_dev[ring], s, g.timing));
+    if (fits_inline(bp)) {
+        // Every job list rides in kernel arguments: nothing to upload.
+        std::vector<unsigned char> H(bp.bytes);
+        fill_descriptors(bp, H.data(), g.arena, g.tickets, g.status);
+        // Ordered after the previous batch (arena, tickets and epoch are shared).
+        if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+        PINTT_TRY(enqueue_launches(bp, nullptr, H.data(), s, g.timing));
+    } else {
+        const int ring = g.desc_next;
+        g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
+        PINTT_TRY(ensure_descriptors(ring, bp.bytes));
+        fill_descriptors(bp, g.desc_host[ring], g.arena, g.tickets, g.status);
+        PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bp.bytes, hipMemcpyHostToDevice, g.stream));
+        PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
+        // A caller stream first waits for the upload, which itself is ordered
+        // after the previous batch on the library stream.
+        if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
+        PINTT_TRY(enqueue_launches(bp, g.desc_dev[ring], g.desc_host[ring], s, g.timing));
+    }
     if (g.timing) g.have_times = true;
     PINTT_HIP(hipEventRecord(g.done, s));
     if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
+    g.last = s;
     return PINTT_OK;
 }
 
@@ -584,7 +615,7 @@ int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
     if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         rc = fail(PINTT_ERR_HIP, "hipStreamBeginCapture");
     } else {
-        rc = enqueue_launches(bp, bg->desc, cs, false);
+        rc = enqueue_launches(bp, bg->desc, nullptr, cs, false);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(cs, &graph);
         if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
@@ -887,10 +918,11 @@ int gpu_ntt_graph_launch(void* graph, void* stream) {
     BatchGraph* bg = static_cast<BatchGraph*>(graph);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
     // Serialised with every other product: all share the carry epoch.
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+    if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
     PINTT_HIP(hipGraphLaunch(bg->exec, s));
     PINTT_HIP(hipEventRecord(g.done, s));
     if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
+    g.last = s;
     return PINTT_OK;
 }
 

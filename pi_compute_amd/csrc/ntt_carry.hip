@@ -10,9 +10,12 @@ namespace {
 // products i, of X0_i[k] + X1_i[k-1] + X2_i[k-2] with k = j - off_i, where
 // X0..X2 are the 32-bit words of the CRT value of chunk product i at position k.
 // At most two chunk products overlap at any word, so a word sum is below
-// 6 * 2^32 and the carry between words stays below 8: the effect of a run of
-// words on an incoming carry c in {0..7} is a map {0..7} -> {0..7}, packed as
-// eight 3-bit entries.  Maps compose associatively: a wave-shuffle scan inside
+// 6 * 2^32 and the carry between words stays at most 6.  The effect of a run of
+// words on an incoming carry c in {0..7} is c -> b + [c >= t]: b is the run's
+// carry-out for carry-in 0, and a larger carry-in adds exactly one more carry
+// when it overflows the run's first word while every later word is all ones
+// (t = 8: never).  These step maps are closed under composition and compose
+// with a few compares: a wave-shuffle scan inside
 // the block, then a decoupled look-back across the group's blocks (numbered in
 // the order they start, from the group's ticket counter, so a block only waits
 // for blocks that already run and publish their aggregate before waiting).
@@ -20,19 +23,22 @@ namespace {
 // atomics suffice: release/acquire would add L2 write-backs and invalidations
 // on every publication and look-back round.
 
-constexpr uint32_t kCarryStates = 8;
-__device__ __forceinline__ uint32_t fn_apply(uint32_t f, uint32_t c) { return (f >> (3 * c)) & 7u; }
+// Map encoding: b in bits 0-2, t in bits 3-6; bit 7 marks the identity.
+constexpr uint32_t kFnIdentity = 1u << 7;
+__device__ __forceinline__ uint32_t fn_make(uint32_t b, uint32_t t) { return b | (t << 3); }
+__device__ __forceinline__ uint32_t fn_apply(uint32_t f, uint32_t c) {
+    return (f & kFnIdentity) ? c : (f & 7u) + (c >= ((f >> 3) & 15u) ? 1u : 0u);
+}
 
 // later o earlier
 __device__ __forceinline__ uint32_t fn_compose(uint32_t later, uint32_t earlier) {
-    uint32_t r = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < kCarryStates; ++c) r |= fn_apply(later, fn_apply(earlier, c)) << (3 * c);
-    return r;
+    const uint32_t gb = earlier & 7u, gt = (earlier >> 3) & 15u;
+    const uint32_t fb = later & 7u, ft = (later >> 3) & 15u;
+    const uint32_t lo = fb + (gb >= ft ? 1u : 0u), hi = fb + (gb + 1 >= ft ? 1u : 0u);
+    const uint32_t r = fn_make(lo, (gt >= 8 || lo == hi) ? 8u : gt);
+    return (earlier & kFnIdentity) ? later : ((later & kFnIdentity) ? earlier : r);
 }
 
-constexpr uint32_t kFnIdentity = 0u | (1u << 3) | (2u << 6) | (3u << 9) | (4u << 12) | (5u << 15) |
-                                 (6u << 18) | (7u << 21);
 constexpr uint64_t kFlagAggregate = 1, kFlagPrefix = 2;
 constexpr uint32_t kLookbackSpinLimit = 1u << 26;
 
@@ -108,7 +114,7 @@ __device__ __forceinline__ uint32_t block_compose_desc(uint32_t g, uint32_t* lds
 // positions of which `len` are non-zero, first word at `off`) to the thread's
 // word sums s[0 .. WT) of words j0 ...
 template <int WT>
-__device__ __forceinline__ void add_chunk_words(uint64_t* s, const uint32_t* __restrict__ res, size_t n,
+__device__ __forceinline__ void add_chunk_words(uint64_t* s, global_ptr<const uint32_t> res, size_t n,
                                                 uint32_t len, long long k0, const CrtConst& cc) {
     // k0: position of word j0 inside this chunk product
     if (k0 + WT <= 0 || k0 - 2 >= (long long)len) return;
@@ -120,9 +126,9 @@ __device__ __forceinline__ void add_chunk_words(uint64_t* s, const uint32_t* __r
     for (int q = 0; q < WT / 4; ++q) {
         uint32_t r0[4], r1[4], r2[4];
         if (full) {
-            const uint4 a = *reinterpret_cast<const uint4*>(res + k0 + 4 * q);
-            const uint4 b = *reinterpret_cast<const uint4*>(res + n + k0 + 4 * q);
-            const uint4 c = *reinterpret_cast<const uint4*>(res + 2 * n + k0 + 4 * q);
+            const u32x4 a = *(global_ptr<const u32x4>)(res + k0 + 4 * q);
+            const u32x4 b = *(global_ptr<const u32x4>)(res + n + k0 + 4 * q);
+            const u32x4 c = *(global_ptr<const u32x4>)(res + 2 * n + k0 + 4 * q);
             r0[0] = a.x; r0[1] = a.y; r0[2] = a.z; r0[3] = a.w;
             r1[0] = b.x; r1[1] = b.y; r1[2] = b.z; r1[3] = b.w;
             r2[0] = c.x; r2[1] = c.y; r2[2] = c.z; r2[3] = c.w;
@@ -150,35 +156,52 @@ __device__ __forceinline__ void add_chunk_words(uint64_t* s, const uint32_t* __r
 
 // grid = total carry blocks of the launch; jobs sorted by blk0.
 __global__ void __launch_bounds__(kCarryThreads)
-    k_crt_carry(const CarryJob* __restrict__ jobs, uint32_t njobs, int log2n, CrtConst cc,
+    k_crt_carry(const JobList<CarryJob> jobs, uint32_t njobs, int log2n, CrtConst cc,
                 CarryGlobal* __restrict__ glob) {
     constexpr int WB = kCarryWordsPerBlock;
     constexpr int WT = kCarryWordsPerThread;
     __shared__ uint32_t waves[kCarryThreads / 64];
     __shared__ uint32_t sh_blk, sh_epoch, sh_job, sh_first_prefix, sh_first_wait;
     if (threadIdx.x == 0) {
-        uint32_t lo = 0, hi = njobs - 1;  // last job with blk0 <= blockIdx.x
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
-            else hi = mid - 1;
+        uint32_t lo = 0;  // last job with blk0 <= blockIdx.x
+        if (jobs.ninl) {
+#pragma unroll
+            for (int i = 1; i < kInlineJobs; ++i)
+                if ((uint32_t)i < jobs.ninl && jobs.inl[i].blk0 <= blockIdx.x) lo = i;
+        } else {
+            uint32_t hi = njobs - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (jobs.dev[mid].blk0 <= blockIdx.x) lo = mid;
+                else hi = mid - 1;
+            }
         }
         sh_job = lo;
-        sh_blk = (uint32_t)atomicAdd(jobs[lo].ticket, 1ull);
+        unsigned long long* ticket = jobs.ninl ? nullptr : jobs.dev[lo].ticket;
+        if (jobs.ninl) {
+#pragma unroll
+            for (int i = 0; i < kInlineJobs; ++i)
+                if ((uint32_t)i == lo) ticket = jobs.inl[i].ticket;
+        }
+        sh_blk = (uint32_t)__hip_atomic_fetch_add(as_global(ticket), 1ull, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
         sh_epoch = __hip_atomic_load(&glob->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    const CarryJob& job = jobs[sh_job];
+    const CarryJob job = PINTT_JOB(CarryJob, jobs, sh_job);
     const uint32_t blk = sh_blk, epoch = sh_epoch;
     const size_t n = (size_t)1 << log2n;
     const uint64_t j0 = (uint64_t)blk * WB + (uint64_t)WT * threadIdx.x;  // this thread's first word
-    uint64_t* status = job.status;
+    const global_ptr<uint64_t> status = as_global(job.status);
 
     uint64_t s[WT];
 #pragma unroll
     for (int e = 0; e < WT; ++e) s[e] = 0;
-    for (uint32_t i = 0; i < job.m; ++i)
-        add_chunk_words<WT>(s, job.res[i], n, job.len[i], (long long)j0 - (long long)job.off[i], cc);
+    // Constant chunk indices keep the job (a kernel-argument copy) in registers.
+#pragma unroll
+    for (uint32_t i = 0; i < (uint32_t)kMaxChunks; ++i)
+        if (i < job.m)
+            add_chunk_words<WT>(s, as_global(job.res[i]), n, job.len[i], (long long)j0 - (long long)job.off[i], cc);
 #pragma unroll
     for (int e = 0; e < WT; ++e)
         if (j0 + e >= job.nwords) s[e] = 0;
@@ -197,12 +220,10 @@ __global__ void __launch_bounds__(kCarryThreads)
             if (e == 0) w0 = o;
             else ones = ones && (o == 0xFFFFFFFFu);
         }
-        f = 0;
-#pragma unroll
-        for (uint32_t cin = 0; cin < kCarryStates; ++cin) {
-            uint32_t over = (ones && (uint64_t)w0 + cin > 0xFFFFFFFFull) ? 1u : 0u;
-            f |= ((uint32_t)c + over) << (3 * cin);
-        }
+        // smallest carry-in that overflows word 0: 2^32 - w0 (8 stands for never)
+        const uint32_t room = 0u - w0;  // 2^32 - w0 mod 2^32; 0 when w0 == 0
+        const uint32_t t = (ones && w0 != 0 && room <= 7u) ? room : 8u;
+        f = fn_make((uint32_t)c, t);
     }
     uint32_t agg;
     const uint32_t excl = block_scan_fn<kCarryThreads>(f, waves, &agg);
@@ -260,11 +281,11 @@ __global__ void __launch_bounds__(kCarryThreads)
         w[e] = (uint32_t)v;
         c = v >> 32;
     }
-    uint32_t* out = job.out;
+    const global_ptr<uint32_t> out = as_global(job.out);
     if (j0 + WT <= job.nwords) {
 #pragma unroll
         for (int q = 0; q < WT / 4; ++q)
-            *reinterpret_cast<uint4*>(out + j0 + 4 * q) = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+            *(global_ptr<u32x4>)(out + j0 + 4 * q) = u32x4{w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]};
     } else {
 #pragma unroll
         for (int e = 0; e < WT; ++e)
@@ -320,7 +341,7 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, h
     return hipGetLastError();
 }
 
-hipError_t launch_crt_carry(const CarryJob* jobs, uint32_t njobs, uint32_t nblocks, int log2n,
+hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
                             const CrtConst& cc, CarryGlobal* glob, hipStream_t s) {
     hipLaunchKernelGGL(k_crt_carry, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc, glob);
     return hipGetLastError();

@@ -16,15 +16,17 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bx, uint32_t nb) {
 // grid = (column tiles, prime, job).
 template <int LOGR, int LOGW>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
-    k_col_fwd(const FwdJob* __restrict__ jobs, int log2c, Twiddles tw) {
+    k_col_fwd(const JobList<FwdJob> jobs, int log2c, Twiddles tw) {
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
     __shared__ uint2 zt[TL::L];  // the R - 1 twiddles of the column layers (entry 0 unused)
-    const FwdJob job = jobs[blockIdx.z];
+    const FwdJob job = PINTT_JOB(FwdJob, jobs, blockIdx.z);
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
-    uint32_t* dst = job.dst + ((size_t)prime << (LOGR + log2c));
-    for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw.fwd[prime][i];
+    global_ptr<uint32_t> dst = as_global(job.dst) + ((size_t)prime << (LOGR + log2c));
+    global_ptr<const uint32_t> src = as_global(job.src);
+    const TwTable z = PINTT_FWD_TABLE(tw, prime);
+    for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw_at(z, i);
     // Neighbouring column tiles share 128-byte lines; give each XCD (workgroups
     // are dealt to the 8 XCDs round-robin) a contiguous range of tiles.
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
@@ -33,7 +35,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
         uint32_t k = (r << log2c) + c0 + w;
-        uint32_t v = k < job.ncoef ? job.src[k] : 0u;
+        uint32_t v = k < job.ncoef ? src[k] : 0u;
         lds[TL::addr(w, r)] = reduce_u32(v, p);
     }
     __syncthreads();
@@ -51,20 +53,21 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 // that follows on the stream: a new epoch, fresh tickets for every group.
 template <int LOGR, int LOGW>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
-    k_col_inv(const InvJob* __restrict__ jobs, int log2c, Twiddles tw, CarryGlobal* __restrict__ glob) {
+    k_col_inv(const JobList<InvJob> jobs, int log2c, Twiddles tw, CarryGlobal* __restrict__ glob) {
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
     __shared__ uint2 zt[TL::L];  // inverse twiddles of the column layers
-    const InvJob job = jobs[blockIdx.z];
+    const InvJob job = PINTT_JOB(InvJob, jobs, blockIdx.z);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         if (job.arm) *job.arm = 0;
         if (blockIdx.z == 0) glob->epoch = glob->epoch + 1;
     }
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
-    uint32_t* buf = job.buf + ((size_t)prime << (LOGR + log2c));
+    global_ptr<uint32_t> buf = as_global(job.buf) + ((size_t)prime << (LOGR + log2c));
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
-    for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw.inv[prime][i];
+    const TwTable z = PINTT_INV_TABLE(tw, prime);
+    for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw_at(z, i);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -93,7 +96,7 @@ bool col_case_available(int log2r, int log2w) {
     return false;
 }
 
-hipError_t launch_forward_columns(const Plan& pl, const FwdJob* jobs, uint32_t njobs, const Twiddles& tw,
+hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
 #define PINTT_CASE(LR, LW)                                                                          \
@@ -107,7 +110,7 @@ hipError_t launch_forward_columns(const Plan& pl, const FwdJob* jobs, uint32_t n
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inverse_columns(const Plan& pl, const InvJob* jobs, uint32_t njobs, CarryGlobal* glob,
+hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, CarryGlobal* glob,
                                   const Twiddles& tw, hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
 #define PINTT_CASE(LR, LW)                                                                          \

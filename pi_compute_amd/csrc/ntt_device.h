@@ -14,6 +14,35 @@ __device__ __forceinline__ uint32_t prime_of(int i) {
     return i == 0 ? kP[0] : (i == 1 ? kP[1] : kP[2]);
 }
 
+// Pointers read from job descriptors are generic to the compiler, which then
+// emits flat loads (counted against both the vector-memory and the LDS wait
+// counters, so every LDS wait also waits for them).  Device buffers here are
+// always global memory: say so.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ global_ptr<T> as_global(T* p) {
+    return (global_ptr<T>)p;
+}
+// Four words as one 128-bit access (a clang vector type: it loads and stores
+// through address-space-qualified pointers, unlike uint4).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Twiddle table of one prime, picked with scalar selects: indexing the
+// kernel-argument arrays with a run-time prime would copy them to scratch.
+// A table entry (w, w') is read as one 64-bit word (w in the low half).
+using TwTable = global_ptr<const unsigned long long>;
+__device__ __forceinline__ TwTable pick(const uint2* t0, const uint2* t1, const uint2* t2, int prime) {
+    return (TwTable)(prime == 0 ? t0 : (prime == 1 ? t1 : t2));
+}
+__device__ __forceinline__ uint2 tw_at(TwTable z, uint32_t i) {
+    const unsigned long long v = z[i];
+    return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+}
+__device__ __forceinline__ uint2 tw_at(const uint2* z, uint32_t i) { return z[i]; }
+#define PINTT_FWD_TABLE(tw, prime) pick((tw).fwd[0], (tw).fwd[1], (tw).fwd[2], (prime))
+#define PINTT_INV_TABLE(tw, prime) pick((tw).inv[0], (tw).inv[1], (tw).inv[2], (prime))
+
 // One pad word after every 16 keeps radix-16 strides off a single LDS bank.
 __device__ __forceinline__ uint32_t pad(uint32_t x) { return x + (x >> 4); }
 
@@ -63,9 +92,8 @@ struct Stage {
 // local layer S0+l, sub-block `sub` of the group: global layer sb + S0 + l,
 // global block ((row0 + seq) << (S0 + l)) + (bblk << l) + sub, where row0 is
 // the block index of sequence 0 at layer sb (0 for columns, the row for rows).
-template <class TL, int S0, int K, int NB>
-__device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, const uint2* __restrict__ Z,
-                                         int sb, uint32_t row0, uint32_t p) {
+template <class TL, int S0, int K, int NB, class ZP>
+__device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, ZP Z, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
     for (int q = 0; q < ST::NG; ++q) {
@@ -83,7 +111,7 @@ __device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, const uint2
             const int h = ST::G >> (l + 1);
 #pragma unroll
             for (int sub = 0; sub < (1 << l); ++sub) {
-                uint2 z = Z[(1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub];
+                uint2 z = tw_at(Z, (1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub);
 #pragma unroll
                 for (int t = 0; t < h; ++t) {
                     ct_bfly(v0[sub * 2 * h + t], v0[sub * 2 * h + t + h], z, p);
@@ -100,9 +128,8 @@ __device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, const uint2
 }
 
 // Inverse (Gentleman-Sande) layers S0+K-1 down to S0 on one tile.
-template <class TL, int S0, int K>
-__device__ __forceinline__ void gs_stage(uint32_t* t0, const uint2* __restrict__ Zi, int sb,
-                                         uint32_t row0, uint32_t p) {
+template <class TL, int S0, int K, class ZP>
+__device__ __forceinline__ void gs_stage(uint32_t* t0, ZP Zi, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
     for (int q = 0; q < ST::NG; ++q) {
@@ -117,7 +144,7 @@ __device__ __forceinline__ void gs_stage(uint32_t* t0, const uint2* __restrict__
             const int h = ST::G >> (l + 1);
 #pragma unroll
             for (int sub = 0; sub < (1 << l); ++sub) {
-                uint2 z = Zi[(1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub];
+                uint2 z = tw_at(Zi, (1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub);
 #pragma unroll
                 for (int t = 0; t < h; ++t) gs_bfly(v[sub * 2 * h + t], v[sub * 2 * h + t + h], z, p);
             }
@@ -127,9 +154,8 @@ __device__ __forceinline__ void gs_stage(uint32_t* t0, const uint2* __restrict__
     }
 }
 
-template <class TL, int S0, int NB>
-__device__ __forceinline__ void ct_run(uint32_t* t0, uint32_t* t1, const uint2* __restrict__ Z,
-                                       int sb, uint32_t row0, uint32_t p) {
+template <class TL, int S0, int NB, class ZP>
+__device__ __forceinline__ void ct_run(uint32_t* t0, uint32_t* t1, ZP Z, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
         ct_stage<TL, S0, K, NB>(t0, t1, Z, sb, row0, p);
@@ -138,9 +164,8 @@ __device__ __forceinline__ void ct_run(uint32_t* t0, uint32_t* t1, const uint2* 
     }
 }
 
-template <class TL, int S0>
-__device__ __forceinline__ void gs_run(uint32_t* t0, const uint2* __restrict__ Zi, int sb,
-                                       uint32_t row0, uint32_t p) {
+template <class TL, int S0, class ZP>
+__device__ __forceinline__ void gs_run(uint32_t* t0, ZP Zi, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
         gs_run<TL, S0 + K>(t0, Zi, sb, row0, p);

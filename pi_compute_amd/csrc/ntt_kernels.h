@@ -141,17 +141,57 @@ __host__ inline bool plan_valid(const Plan& pl) {
            col_case_available(pl.log2r, pl.log2w) && row_case_available(pl.log2c, pl.log2rows);
 }
 
-// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip).  Job
-// arrays are device pointers; counts are host values.
+// A launch's job list: up to kInlineJobs jobs travel inside the kernel
+// arguments (no descriptor upload, so a single product is only kernels on the
+// stream); longer lists are read from device memory.
+constexpr int kInlineJobs = 4;
+template <class T>
+struct JobList {
+    const T* dev;   // device array (count jobs) when ninl == 0
+    uint32_t ninl;  // jobs held in `inl`
+    T inl[kInlineJobs];
+};
+template <class T>
+inline JobList<T> device_jobs(const T* dev) {
+    JobList<T> l{};
+    l.dev = dev;
+    return l;
+}
+template <class T>
+inline JobList<T> inline_jobs(const T* host, uint32_t n) {
+    JobList<T> l{};
+    l.ninl = n;
+    for (uint32_t i = 0; i < n; ++i) l.inl[i] = host[i];
+    return l;
+}
+
+// Job z of a launch (z uniform across the workgroup).  A statement expression
+// over the kernel argument itself: passing the list by reference to a function
+// would make the compiler copy the kernel argument block to scratch.
+#define PINTT_JOB(T, list, z)                                                        \
+    ({                                                                               \
+        const uint32_t z_ = __builtin_amdgcn_readfirstlane(z);                      \
+        T r_;                                                                        \
+        if ((list).ninl == 0) {                                                      \
+            r_ = (list).dev[z_];                                                     \
+        } else {                                                                     \
+            r_ = (list).inl[0];                                                      \
+            _Pragma("unroll") for (int i_ = 1; i_ < kInlineJobs; ++i_) if (z_ == (uint32_t)i_) r_ = (list).inl[i_]; \
+        }                                                                            \
+        r_;                                                                          \
+    })
+
+// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip); counts are
+// host values.
 hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s);
-hipError_t launch_forward_columns(const Plan& pl, const FwdJob* jobs, uint32_t njobs, const Twiddles& tw,
+hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
-hipError_t launch_row_convolution(const Plan& pl, const RowJob* jobs, uint32_t njobs, RowKind kind,
+hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,
                                   const Twiddles& tw, hipStream_t s);
 // Also advances glob->epoch and resets the tickets the jobs name.
-hipError_t launch_inverse_columns(const Plan& pl, const InvJob* jobs, uint32_t njobs, CarryGlobal* glob,
+hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, CarryGlobal* glob,
                                   const Twiddles& tw, hipStream_t s);
-hipError_t launch_crt_carry(const CarryJob* jobs, uint32_t njobs, uint32_t nblocks, int log2n,
+hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
                             const CrtConst& cc, CarryGlobal* glob, hipStream_t s);
 
 }  // namespace pintt

@@ -8,37 +8,36 @@ namespace {
 // Pass 2, per chunk product and row tile: forward row layers of a (and of b
 // unless they are already done), Montgomery pointwise product, inverse row
 // layers, written back in place.  grid = (row tiles, prime, job).
-// Three waves per SIMD (at most 168 VGPRs): without the cap the compiler hoists
-// twiddle loads of the paired transforms into ~250 registers and halves occupancy.
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
 // ~250 registers on the paired transforms and halves occupancy.
 template <int LOGC, int LOGROWS, int KIND>
 __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute__((amdgpu_waves_per_eu(3)))
-    k_row_conv(const RowJob* __restrict__ jobs, int log2r, Twiddles tw) {
+    k_row_conv(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
     using TL = Tile<LOGC, LOGROWS, false>;
     __shared__ uint32_t ldsA[TL::WORDS];
     __shared__ uint32_t ldsB[KIND == kRowSquare ? 1 : TL::WORDS];
-    const RowJob job = jobs[blockIdx.z];
+    const RowJob job = PINTT_JOB(RowJob, jobs, blockIdx.z);
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
     const uint32_t pinv = neg_inv_mod_2_32(p);
     const size_t base = ((size_t)prime << (log2r + LOGC)) + ((size_t)blockIdx.x << (LOGROWS + LOGC));
     const uint32_t row0 = blockIdx.x << LOGROWS;
-    uint32_t* a = job.a + base;
+    global_ptr<uint32_t> a = as_global(job.a) + base;
+    global_ptr<uint32_t> b = as_global(job.b) + base;
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         ldsA[pad(idx)] = a[idx];
-        if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = job.b[base + idx];
+        if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
     }
     __syncthreads();
-    ct_run<TL, 0, KIND == kRowTransformB ? 2 : 1>(ldsA, ldsB, tw.fwd[prime], log2r, row0, p);
+    ct_run<TL, 0, KIND == kRowTransformB ? 2 : 1>(ldsA, ldsB, PINTT_FWD_TABLE(tw, prime), log2r, row0, p);
     if constexpr (KIND == kRowTransformB) {
         if (job.write_b) {
 #pragma unroll
             for (int e = 0; e < kElemsPerThread; ++e) {
                 uint32_t idx = e * TL::NT + threadIdx.x;
-                job.b[base + idx] = ldsB[pad(idx)];
+                b[idx] = ldsB[pad(idx)];
             }
         }
     }
@@ -50,7 +49,7 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         ldsA[idx] = mul_mont(x, y, p, pinv);
     }
     __syncthreads();
-    gs_run<TL, 0>(ldsA, tw.inv[prime], log2r, row0, p);
+    gs_run<TL, 0>(ldsA, PINTT_INV_TABLE(tw, prime), log2r, row0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -74,7 +73,7 @@ bool row_case_available(int log2c, int log2rows) {
     return false;
 }
 
-hipError_t launch_row_convolution(const Plan& pl, const RowJob* jobs, uint32_t njobs, RowKind kind,
+hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,
                                   const Twiddles& tw, hipStream_t s) {
     dim3 grid(1u << (pl.log2r - pl.log2rows), kPrimes, njobs);
 #define PINTT_LAUNCH(LC, LROWS, K)                                                                \
