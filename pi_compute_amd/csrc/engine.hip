@@ -83,6 +83,7 @@ struct Engine {
     Plan plan[kMaxLog2N + 1] = {};
     float plan_ms[kMaxLog2N + 1] = {};
     int split_mode = 1;  // 0: never cut operands into chunks, 1: when cheaper, m >= 2: m chunks when valid
+    bool pair_rows = true;  // first two chunks of a group in one row launch (PINTT_PAIR=0: off)
     size_t tune_cap = 0;
     uint64_t* tune_in = nullptr;
     uint64_t* tune_out = nullptr;
@@ -169,6 +170,8 @@ int ensure_init(int device) {
     g.autotune = !(at && at[0] == '0');
     const char* sp = std::getenv("PINTT_SPLIT");
     g.split_mode = (sp && sp[0] == '0') ? 0 : 1;
+    const char* pr = std::getenv("PINTT_PAIR");
+    g.pair_rows = !(pr && pr[0] == '0');
     g.device = dev;
     g.ready = true;
     return PINTT_OK;
@@ -342,6 +345,7 @@ struct RowLaunch {
     RowKind kind;
     size_t off;  // byte offset of its RowJob array in the descriptor blob
     uint32_t count;
+    uint32_t chunk;  // chunk index it serves (kRowPair: chunks 0 and 1)
 };
 
 struct BucketLayout {
@@ -351,6 +355,7 @@ struct BucketLayout {
     uint32_t nfwd, ninv, nblocks;
     uint32_t max_m;
     std::vector<RowLaunch> rows;                  // in launch order
+    bool pair;                                    // chunks 0 and 1 of a group share one row launch
 };
 
 template <class T>
@@ -383,9 +388,14 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
         }
         it->groups.push_back(i);
     }
+    // Tile plans first: the row launches depend on them.
+    bp->plans.resize(buckets.size());
+    for (size_t bi = 0; bi < buckets.size(); ++bi) PINTT_TRY(plan_for(buckets[bi].log2n, &bp->plans[bi]));
     // Buckets run one after another on one stream and reuse the arena.
     size_t bytes = 0;
-    for (auto& b : buckets) {
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        BucketLayout& b = buckets[bi];
+        const Plan& pl = bp->plans[bi];
         b.nfwd = b.ninv = b.nblocks = b.max_m = 0;
         size_t slots = 0, nsq = 0;
         for (size_t gi : b.groups) {
@@ -397,22 +407,35 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
             b.max_m = std::max(b.max_m, gr.r.square ? 0u : gr.m);
             nsq += gr.r.square ? 1 : 0;
         }
+        b.pair = g.pair_rows && pair_case_available(pl.log2c, pl.log2rows);
         bytes = align_up<FwdJob>(bytes);
         b.fwd_off = bytes;
         bytes += b.nfwd * sizeof(FwdJob);
-        // Row launches: squares; first chunks (transform b); later chunks (reuse b).
+        // Row launches: squares; first chunks (transform b), or first pairs of
+        // chunks; later chunks (reuse b's transformed rows).
         b.rows.clear();
-        auto add_rows = [&](RowKind kind, uint32_t count) {
+        auto add_rows = [&](RowKind kind, uint32_t count, uint32_t chunk) {
             if (!count) return;
             bytes = align_up<RowJob>(bytes);
-            b.rows.push_back(RowLaunch{kind, bytes, count});
+            b.rows.push_back(RowLaunch{kind, bytes, count, chunk});
             bytes += count * sizeof(RowJob);
         };
-        add_rows(kRowSquare, (uint32_t)nsq);
-        for (uint32_t i = 0; i < b.max_m; ++i) {
-            uint32_t cnt = 0;
-            for (size_t gi : b.groups) cnt += (!groups[gi].r.square && groups[gi].m > i) ? 1 : 0;
-            add_rows(i == 0 ? kRowTransformB : kRowReuseB, cnt);
+        auto count_groups = [&](auto pred) {
+            uint32_t c = 0;
+            for (size_t gi : b.groups)
+                if (!groups[gi].r.square && pred(groups[gi].m)) ++c;
+            return c;
+        };
+        add_rows(kRowSquare, (uint32_t)nsq, 0);
+        if (b.pair) {
+            add_rows(kRowTransformB, count_groups([](uint32_t m) { return m == 1; }), 0);
+            add_rows(kRowPair, count_groups([](uint32_t m) { return m >= 2; }), 0);
+            for (uint32_t i = 2; i < b.max_m; ++i)
+                add_rows(kRowReuseB, count_groups([i](uint32_t m) { return m > i; }), i);
+        } else {
+            add_rows(kRowTransformB, count_groups([](uint32_t m) { return m >= 1; }), 0);
+            for (uint32_t i = 1; i < b.max_m; ++i)
+                add_rows(kRowReuseB, count_groups([i](uint32_t m) { return m > i; }), i);
         }
         bytes = align_up<InvJob>(bytes);
         b.inv_off = bytes;
@@ -426,8 +449,6 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
         bp->max_log2n = std::max(bp->max_log2n, b.log2n);
     }
     bp->bytes = bytes;
-    bp->plans.resize(buckets.size());
-    for (size_t bi = 0; bi < buckets.size(); ++bi) PINTT_TRY(plan_for(buckets[bi].log2n, &bp->plans[bi]));
     return PINTT_OK;
 }
 
@@ -440,11 +461,14 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, un
         const size_t n = (size_t)1 << b.log2n;
         const size_t slot = kPrimes * n;
         FwdJob* fj = reinterpret_cast<FwdJob*>(H + b.fwd_off);
-        // row launches: squares first (if any), then chunk index 0, 1, ...
         std::vector<RowJob*> rj;
         std::vector<uint32_t> rfill(b.rows.size(), 0);
         for (const RowLaunch& rl : b.rows) rj.push_back(reinterpret_cast<RowJob*>(H + rl.off));
-        const size_t first_chunk_launch = (b.rows.size() && b.rows[0].kind == kRowSquare) ? 1 : 0;
+        auto launch_of = [&](RowKind kind, uint32_t chunk) {
+            for (size_t li = 0; li < b.rows.size(); ++li)
+                if (b.rows[li].kind == kind && b.rows[li].chunk == chunk) return li;
+            return b.rows.size();  // not reached: plan_batch made every launch a job needs
+        };
         InvJob* ij = reinterpret_cast<InvJob*>(H + b.inv_off);
         CarryJob* cj = reinterpret_cast<CarryJob*>(H + b.carry_off);
         size_t next_slot = 0;
@@ -455,16 +479,27 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, un
             CarryJob car{};
             uint32_t* bslot = r.square ? nullptr : arena + slot * (next_slot + gr.m);
             unsigned long long* ticket = tickets + ticket_next++;
+            const size_t first = next_slot;
+            const bool pair = b.pair && !r.square && gr.m >= 2;
             for (uint32_t i = 0; i < gr.m; ++i) {
                 uint32_t* dst = arena + slot * next_slot++;
                 size_t lo = i * gr.h;
                 size_t limbs = std::min(gr.h, r.an - lo);
                 fj[f++] = FwdJob{reinterpret_cast<const uint32_t*>(r.a + lo), (uint32_t)(2 * limbs), dst};
                 if (r.square) {
-                    rj[0][rfill[0]++] = RowJob{dst, nullptr, 0};
+                    const size_t li = launch_of(kRowSquare, 0);
+                    rj[li][rfill[li]++] = RowJob{dst, nullptr, nullptr, 0};
+                } else if (pair && i < 2) {
+                    if (i == 0) {
+                        const size_t li = launch_of(kRowPair, 0);
+                        rj[li][rfill[li]++] = RowJob{dst, bslot, arena + slot * (first + 1), gr.m > 2 ? 1u : 0u};
+                    }
+                } else if (i == 0) {
+                    const size_t li = launch_of(kRowTransformB, 0);
+                    rj[li][rfill[li]++] = RowJob{dst, bslot, nullptr, gr.m > 1 ? 1u : 0u};
                 } else {
-                    size_t li = first_chunk_launch + i;
-                    rj[li][rfill[li]++] = RowJob{dst, bslot, (i == 0 && gr.m > 1) ? 1u : 0u};
+                    const size_t li = launch_of(kRowReuseB, i);
+                    rj[li][rfill[li]++] = RowJob{dst, bslot, nullptr, 0};
                 }
                 ij[inv++] = InvJob{dst, i == 0 ? ticket : nullptr};
                 car.res[i] = dst;

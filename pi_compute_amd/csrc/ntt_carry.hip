@@ -110,47 +110,66 @@ __device__ __forceinline__ uint32_t block_compose_desc(uint32_t g, uint32_t* lds
     return r;
 }
 
-// Adds the word contributions of one chunk product (residues `res`, N = 2^log2n
-// positions of which `len` are non-zero, first word at `off`) to the thread's
-// word sums s[0 .. WT) of words j0 ...
+// Residues of one chunk product that a thread needs: positions k0-2, k0-1 (the
+// words they spill into) and k0 .. k0+WT-1, three primes each; zero outside the
+// chunk product (residues 0 give x = 0).
 template <int WT>
-__device__ __forceinline__ void add_chunk_words(uint64_t* s, global_ptr<const uint32_t> res, size_t n,
-                                                uint32_t len, long long k0, const CrtConst& cc) {
-    // k0: position of word j0 inside this chunk product
-    if (k0 + WT <= 0 || k0 - 2 >= (long long)len) return;
-    uint32_t xm1[3] = {0, 0, 0}, xm2[3] = {0, 0, 0};  // x at positions k-1, k-2
-    if (k0 >= 2 && k0 - 2 < len) crt_words(res[k0 - 2], res[n + k0 - 2], res[2 * n + k0 - 2], cc, xm2[0], xm2[1], xm2[2]);
-    if (k0 >= 1 && k0 - 1 < len) crt_words(res[k0 - 1], res[n + k0 - 1], res[2 * n + k0 - 1], cc, xm1[0], xm1[1], xm1[2]);
-    const bool full = k0 >= 0 && k0 + WT <= (long long)len;
+struct ChunkRes {
+    uint32_t h[2][3];
+    uint32_t r[3][WT];
+    bool any;  // some position in range
+};
+
+// Loads only, so that every chunk's loads are in flight before any CRT.
+// k0: position of the thread's first word inside the chunk product.
+template <int WT>
+__device__ __forceinline__ void load_chunk(ChunkRes<WT>& c, global_ptr<const uint32_t> res, size_t n, uint32_t len,
+                                           long long k0) {
+    c.any = !(k0 + WT <= 0 || k0 - 2 >= (long long)len);
 #pragma unroll
-    for (int q = 0; q < WT / 4; ++q) {
-        uint32_t r0[4], r1[4], r2[4];
-        if (full) {
-            const u32x4 a = *(global_ptr<const u32x4>)(res + k0 + 4 * q);
-            const u32x4 b = *(global_ptr<const u32x4>)(res + n + k0 + 4 * q);
-            const u32x4 c = *(global_ptr<const u32x4>)(res + 2 * n + k0 + 4 * q);
-            r0[0] = a.x; r0[1] = a.y; r0[2] = a.z; r0[3] = a.w;
-            r1[0] = b.x; r1[1] = b.y; r1[2] = b.z; r1[3] = b.w;
-            r2[0] = c.x; r2[1] = c.y; r2[2] = c.z; r2[3] = c.w;
-        } else {
+    for (int d = 0; d < 2; ++d) {
+        const long long k = k0 - 2 + d;
+        const bool in = c.any && k >= 0 && k < (long long)len;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                long long k = k0 + 4 * q + u;
-                bool in = k >= 0 && k < (long long)len;
-                r0[u] = in ? res[k] : 0u;
-                r1[u] = in ? res[n + k] : 0u;
-                r2[u] = in ? res[2 * n + k] : 0u;
+        for (int q = 0; q < 3; ++q) c.h[d][q] = in ? res[q * n + k] : 0u;
+    }
+    const bool full = c.any && k0 >= 0 && k0 + WT <= (long long)len;
+    if (full) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int v = 0; v < WT / 4; ++v) {
+                const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k0 + 4 * v);
+                c.r[q][4 * v] = x.x;
+                c.r[q][4 * v + 1] = x.y;
+                c.r[q][4 * v + 2] = x.z;
+                c.r[q][4 * v + 3] = x.w;
             }
-        }
+    } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = 4 * q + u;
-            uint32_t x0, x1, x2;
-            crt_words(r0[u], r1[u], r2[u], cc, x0, x1, x2);  // residues 0 give x = 0
-            s[e] += (uint64_t)x0 + xm1[1] + xm2[2];
-            xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
-            xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
+        for (int e = 0; e < WT; ++e) {
+            const long long k = k0 + e;
+            const bool in = c.any && k >= 0 && k < (long long)len;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) c.r[q][e] = in ? res[q * n + k] : 0u;
         }
+    }
+}
+
+// Adds the chunk product's word contributions X0[k] + X1[k-1] + X2[k-2] to the
+// thread's word sums s[0 .. WT).
+template <int WT>
+__device__ __forceinline__ void add_chunk_words(uint64_t* s, const ChunkRes<WT>& c, const CrtConst& cc) {
+    uint32_t xm1[3], xm2[3];  // x at positions k-1, k-2
+    crt_words(c.h[0][0], c.h[0][1], c.h[0][2], cc, xm2[0], xm2[1], xm2[2]);
+    crt_words(c.h[1][0], c.h[1][1], c.h[1][2], cc, xm1[0], xm1[1], xm1[2]);
+#pragma unroll
+    for (int e = 0; e < WT; ++e) {
+        uint32_t x0, x1, x2;
+        crt_words(c.r[0][e], c.r[1][e], c.r[2][e], cc, x0, x1, x2);
+        s[e] += (uint64_t)x0 + xm1[1] + xm2[2];
+        xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
+        xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
     }
 }
 
@@ -197,11 +216,21 @@ __global__ void __launch_bounds__(kCarryThreads)
     uint64_t s[WT];
 #pragma unroll
     for (int e = 0; e < WT; ++e) s[e] = 0;
+    // Chunk products two at a time: both loads issued, then both CRTs.
     // Constant chunk indices keep the job (a kernel-argument copy) in registers.
 #pragma unroll
-    for (uint32_t i = 0; i < (uint32_t)kMaxChunks; ++i)
-        if (i < job.m)
-            add_chunk_words<WT>(s, as_global(job.res[i]), n, job.len[i], (long long)j0 - (long long)job.off[i], cc);
+    for (uint32_t i = 0; i < (uint32_t)kMaxChunks; i += 2) {
+        if (i < job.m) {
+            ChunkRes<WT> c0, c1;
+            load_chunk<WT>(c0, as_global(job.res[i]), n, job.len[i], (long long)j0 - (long long)job.off[i]);
+            const bool two = i + 1 < job.m;
+            if (two)
+                load_chunk<WT>(c1, as_global(job.res[i + 1]), n, job.len[i + 1],
+                               (long long)j0 - (long long)job.off[i + 1]);
+            if (c0.any) add_chunk_words<WT>(s, c0, cc);
+            if (two && c1.any) add_chunk_words<WT>(s, c1, cc);
+        }
+    }
 #pragma unroll
     for (int e = 0; e < WT; ++e)
         if (j0 + e >= job.nwords) s[e] = 0;

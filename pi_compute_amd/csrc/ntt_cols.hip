@@ -39,7 +39,8 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         lds[TL::addr(w, r)] = reduce_u32(v, p);
     }
     __syncthreads();
-    ct_run<TL, 0, 1>(lds, nullptr, zt, 0, 0, p);
+    uint32_t* const tiles[1] = {lds};
+    ct_run<TL, 0, 1>(tiles, zt, 0, 0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -75,7 +76,8 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         lds[TL::addr(w, r)] = buf[(r << log2c) + c0 + w];
     }
     __syncthreads();
-    gs_run<TL, 0>(lds, zt, 0, 0, p);
+    uint32_t* const tiles[1] = {lds};
+    gs_run<TL, 0, 1>(tiles, zt, 0, 0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;

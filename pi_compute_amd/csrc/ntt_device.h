@@ -88,24 +88,24 @@ struct Stage {
     }
 };
 
-// Forward (Cooley-Tukey) layers [S0, S0+K) on one or two tiles.  Twiddle of
-// local layer S0+l, sub-block `sub` of the group: global layer sb + S0 + l,
-// global block ((row0 + seq) << (S0 + l)) + (bblk << l) + sub, where row0 is
-// the block index of sequence 0 at layer sb (0 for columns, the row for rows).
+// Forward (Cooley-Tukey) layers [S0, S0+K) on NB tiles (1..3) that share
+// their twiddles.  Twiddle of local layer S0+l, sub-block `sub` of the group:
+// global layer sb + S0 + l, global block ((row0 + seq) << (S0 + l)) +
+// (bblk << l) + sub, where row0 is the block index of sequence 0 at layer sb
+// (0 for columns, the row for rows).
 template <class TL, int S0, int K, int NB, class ZP>
-__device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, ZP Z, int sb, uint32_t row0, uint32_t p) {
+__device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
     for (int q = 0; q < ST::NG; ++q) {
         uint32_t seq, base, bblk;
         ST::locate(q * TL::NT + threadIdx.x, seq, base, bblk);
         uint32_t blk = TL::COLS ? row0 : row0 + seq;
-        uint32_t v0[ST::G], v1[ST::G];
+        uint32_t v[NB][ST::G];
 #pragma unroll
-        for (int t = 0; t < ST::G; ++t) {
-            v0[t] = t0[TL::addr(seq, base + t * ST::DD)];
-            if constexpr (NB == 2) v1[t] = t1[TL::addr(seq, base + t * ST::DD)];
-        }
+        for (int i = 0; i < ST::G; ++i)
+#pragma unroll
+            for (int u = 0; u < NB; ++u) v[u][i] = t[u][TL::addr(seq, base + i * ST::DD)];
 #pragma unroll
         for (int l = 0; l < K; ++l) {
             const int h = ST::G >> (l + 1);
@@ -113,32 +113,32 @@ __device__ __forceinline__ void ct_stage(uint32_t* t0, uint32_t* t1, ZP Z, int s
             for (int sub = 0; sub < (1 << l); ++sub) {
                 uint2 z = tw_at(Z, (1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub);
 #pragma unroll
-                for (int t = 0; t < h; ++t) {
-                    ct_bfly(v0[sub * 2 * h + t], v0[sub * 2 * h + t + h], z, p);
-                    if constexpr (NB == 2) ct_bfly(v1[sub * 2 * h + t], v1[sub * 2 * h + t + h], z, p);
-                }
+                for (int i = 0; i < h; ++i)
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) ct_bfly(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
             }
         }
 #pragma unroll
-        for (int t = 0; t < ST::G; ++t) {
-            t0[TL::addr(seq, base + t * ST::DD)] = v0[t];
-            if constexpr (NB == 2) t1[TL::addr(seq, base + t * ST::DD)] = v1[t];
-        }
+        for (int i = 0; i < ST::G; ++i)
+#pragma unroll
+            for (int u = 0; u < NB; ++u) t[u][TL::addr(seq, base + i * ST::DD)] = v[u][i];
     }
 }
 
-// Inverse (Gentleman-Sande) layers S0+K-1 down to S0 on one tile.
-template <class TL, int S0, int K, class ZP>
-__device__ __forceinline__ void gs_stage(uint32_t* t0, ZP Zi, int sb, uint32_t row0, uint32_t p) {
+// Inverse (Gentleman-Sande) layers S0+K-1 down to S0 on NB tiles (1..2).
+template <class TL, int S0, int K, int NB, class ZP>
+__device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
     for (int q = 0; q < ST::NG; ++q) {
         uint32_t seq, base, bblk;
         ST::locate(q * TL::NT + threadIdx.x, seq, base, bblk);
         uint32_t blk = TL::COLS ? row0 : row0 + seq;
-        uint32_t v[ST::G];
+        uint32_t v[NB][ST::G];
 #pragma unroll
-        for (int t = 0; t < ST::G; ++t) v[t] = t0[TL::addr(seq, base + t * ST::DD)];
+        for (int i = 0; i < ST::G; ++i)
+#pragma unroll
+            for (int u = 0; u < NB; ++u) v[u][i] = t[u][TL::addr(seq, base + i * ST::DD)];
 #pragma unroll
         for (int l = K - 1; l >= 0; --l) {
             const int h = ST::G >> (l + 1);
@@ -146,33 +146,36 @@ __device__ __forceinline__ void gs_stage(uint32_t* t0, ZP Zi, int sb, uint32_t r
             for (int sub = 0; sub < (1 << l); ++sub) {
                 uint2 z = tw_at(Zi, (1u << (sb + S0 + l)) + (blk << (S0 + l)) + (bblk << l) + sub);
 #pragma unroll
-                for (int t = 0; t < h; ++t) gs_bfly(v[sub * 2 * h + t], v[sub * 2 * h + t + h], z, p);
+                for (int i = 0; i < h; ++i)
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) gs_bfly(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
             }
         }
 #pragma unroll
-        for (int t = 0; t < ST::G; ++t) t0[TL::addr(seq, base + t * ST::DD)] = v[t];
+        for (int i = 0; i < ST::G; ++i)
+#pragma unroll
+            for (int u = 0; u < NB; ++u) t[u][TL::addr(seq, base + i * ST::DD)] = v[u][i];
     }
 }
 
 template <class TL, int S0, int NB, class ZP>
-__device__ __forceinline__ void ct_run(uint32_t* t0, uint32_t* t1, ZP Z, int sb, uint32_t row0, uint32_t p) {
+__device__ __forceinline__ void ct_run(uint32_t* const (&t)[NB], ZP Z, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
-        ct_stage<TL, S0, K, NB>(t0, t1, Z, sb, row0, p);
+        ct_stage<TL, S0, K, NB>(t, Z, sb, row0, p);
         __syncthreads();
-        ct_run<TL, S0 + K, NB>(t0, t1, Z, sb, row0, p);
+        ct_run<TL, S0 + K, NB>(t, Z, sb, row0, p);
     }
 }
 
-template <class TL, int S0, class ZP>
-__device__ __forceinline__ void gs_run(uint32_t* t0, ZP Zi, int sb, uint32_t row0, uint32_t p) {
+template <class TL, int S0, int NB, class ZP>
+__device__ __forceinline__ void gs_run(uint32_t* const (&t)[NB], ZP Zi, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
-        gs_run<TL, S0 + K>(t0, Zi, sb, row0, p);
-        gs_stage<TL, S0, K>(t0, Zi, sb, row0, p);
+        gs_run<TL, S0 + K, NB>(t, Zi, sb, row0, p);
+        gs_stage<TL, S0, K, NB>(t, Zi, sb, row0, p);
         __syncthreads();
     }
 }
-
 
 }  // namespace pintt

@@ -46,8 +46,8 @@ struct CrtConst {
 
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
-constexpr int kCarryThreads = 1024;
-constexpr int kCarryWordsPerThread = 8;
+constexpr int kCarryThreads = 512;
+constexpr int kCarryWordsPerThread = 16;
 constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
 constexpr int kMaxChunks = 4;  // pieces the longer operand of one product may be cut into
 
@@ -64,15 +64,18 @@ struct FwdJob {             // forward column pass of one operand chunk
     uint32_t* dst;          // arena slot
 };
 
-// Row pass of one chunk product a_i * b (or a_i^2).  Launches are split by
+// Row pass of one chunk product a_i * b (or a_i^2), or of two, a_i * b and
+// a_(i+1) * b, sharing b's row layers and the twiddles.  Launches are split by
 // kind: squares, products whose b still needs its row layers (the first chunk
-// of a group; it writes b's transformed rows back when more chunks follow),
-// and products whose b slot already holds transformed rows.
-enum RowKind { kRowSquare = 0, kRowTransformB = 1, kRowReuseB = 2 };
+// of a group, or its first two chunks as a pair; b's transformed rows are
+// written back when more chunks follow), and products whose b slot already
+// holds transformed rows.
+enum RowKind { kRowSquare = 0, kRowTransformB = 1, kRowReuseB = 2, kRowPair = 3 };
 struct RowJob {
     uint32_t* a;       // chunk slot, overwritten by the chunk product
     uint32_t* b;       // slot of the shared operand (unused when squaring)
-    uint32_t write_b;  // kRowTransformB: store b's transformed rows back
+    uint32_t* a2;      // kRowPair: the second chunk slot
+    uint32_t write_b;  // kRowTransformB, kRowPair: store b's transformed rows back
 };
 
 struct InvJob {             // inverse column pass of one chunk product, in place
@@ -134,6 +137,8 @@ __host__ inline Plan make_plan(int log2n) {
 // Tile shapes compiled into the library (ntt_cols.hip / ntt_rows.hip).
 bool col_case_available(int log2r, int log2w);
 bool row_case_available(int log2c, int log2rows);
+// Pair row kernels hold three tiles in LDS: rows of at most 2^13 words per workgroup.
+inline bool pair_case_available(int log2c, int log2rows) { return log2c + log2rows <= 13; }
 
 // A plan with an explicit split, checked against the compiled tile shapes.
 __host__ inline bool plan_valid(const Plan& pl) {

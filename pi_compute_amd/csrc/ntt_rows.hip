@@ -5,17 +5,20 @@
 namespace pintt {
 namespace {
 
-// Pass 2, per chunk product and row tile: forward row layers of a (and of b
-// unless they are already done), Montgomery pointwise product, inverse row
-// layers, written back in place.  grid = (row tiles, prime, job).
+// Pass 2, per row tile: forward row layers of the chunk(s) (and of b unless
+// they are already done), Montgomery pointwise products, inverse row layers,
+// written back in place.  grid = (row tiles, prime, job).
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
-// ~250 registers on the paired transforms and halves occupancy.
+// ~250 registers on the paired transforms and halves occupancy.  The pair
+// kind (three tiles in flight) gets two waves and 256 registers instead.
 template <int LOGC, int LOGROWS, int KIND>
-__global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute__((amdgpu_waves_per_eu(KIND == kRowPair ? 2 : 3)))
     k_row_conv(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
     using TL = Tile<LOGC, LOGROWS, false>;
+    constexpr bool kPair = KIND == kRowPair;
     __shared__ uint32_t ldsA[TL::WORDS];
     __shared__ uint32_t ldsB[KIND == kRowSquare ? 1 : TL::WORDS];
+    __shared__ uint32_t ldsC[kPair ? TL::WORDS : 1];
     const RowJob job = PINTT_JOB(RowJob, jobs, blockIdx.z);
     const int prime = blockIdx.y;
     const uint32_t p = prime_of(prime);
@@ -24,15 +27,27 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     const uint32_t row0 = blockIdx.x << LOGROWS;
     global_ptr<uint32_t> a = as_global(job.a) + base;
     global_ptr<uint32_t> b = as_global(job.b) + base;
+    global_ptr<uint32_t> a2 = as_global(job.a2) + base;
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         ldsA[pad(idx)] = a[idx];
         if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
+        if constexpr (kPair) ldsC[pad(idx)] = a2[idx];
     }
     __syncthreads();
-    ct_run<TL, 0, KIND == kRowTransformB ? 2 : 1>(ldsA, ldsB, PINTT_FWD_TABLE(tw, prime), log2r, row0, p);
-    if constexpr (KIND == kRowTransformB) {
+    const auto zf = PINTT_FWD_TABLE(tw, prime);
+    if constexpr (kPair) {
+        uint32_t* const t[3] = {ldsA, ldsB, ldsC};
+        ct_run<TL, 0, 3>(t, zf, log2r, row0, p);
+    } else if constexpr (KIND == kRowTransformB) {
+        uint32_t* const t[2] = {ldsA, ldsB};
+        ct_run<TL, 0, 2>(t, zf, log2r, row0, p);
+    } else {
+        uint32_t* const t[1] = {ldsA};
+        ct_run<TL, 0, 1>(t, zf, log2r, row0, p);
+    }
+    if constexpr (KIND == kRowTransformB || kPair) {
         if (job.write_b) {
 #pragma unroll
             for (int e = 0; e < kElemsPerThread; ++e) {
@@ -47,13 +62,22 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         uint32_t x = ldsA[idx];
         uint32_t y = KIND == kRowSquare ? x : ldsB[idx];
         ldsA[idx] = mul_mont(x, y, p, pinv);
+        if constexpr (kPair) ldsC[idx] = mul_mont(ldsC[idx], y, p, pinv);
     }
     __syncthreads();
-    gs_run<TL, 0>(ldsA, PINTT_INV_TABLE(tw, prime), log2r, row0, p);
+    const auto zi = PINTT_INV_TABLE(tw, prime);
+    if constexpr (kPair) {
+        uint32_t* const t[2] = {ldsA, ldsC};
+        gs_run<TL, 0, 2>(t, zi, log2r, row0, p);
+    } else {
+        uint32_t* const t[1] = {ldsA};
+        gs_run<TL, 0, 1>(t, zi, log2r, row0, p);
+    }
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         a[idx] = ldsA[pad(idx)];
+        if constexpr (kPair) a2[idx] = ldsC[pad(idx)];
     }
 }
 
@@ -64,6 +88,13 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     X(7, 5) X(7, 6) X(8, 3) X(8, 4) X(8, 5) X(9, 2) X(9, 3) X(9, 4) \
     X(10, 1) X(10, 2) X(10, 3) X(11, 0) X(11, 1) X(11, 2) X(12, 0) X(12, 1) \
     X(13, 0) X(14, 0)
+
+// Pair kernels: the row cases with at most 2^13 words per workgroup.
+#define PINTT_PAIR_CASES(X) \
+    X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(6, 5) X(6, 6) X(6, 7) X(7, 4) \
+    X(7, 5) X(7, 6) X(8, 3) X(8, 4) X(8, 5) X(9, 2) X(9, 3) X(9, 4) \
+    X(10, 1) X(10, 2) X(10, 3) X(11, 0) X(11, 1) X(11, 2) X(12, 0) X(12, 1) \
+    X(13, 0)
 
 bool row_case_available(int log2c, int log2rows) {
 #define PINTT_CASE(LC, LROWS) \
@@ -83,10 +114,18 @@ hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, u
     if (pl.log2c == LC && pl.log2rows == LROWS) {                               \
         if (kind == kRowSquare) PINTT_LAUNCH(LC, LROWS, kRowSquare);            \
         else if (kind == kRowTransformB) PINTT_LAUNCH(LC, LROWS, kRowTransformB); \
-        else PINTT_LAUNCH(LC, LROWS, kRowReuseB);                               \
+        else if (kind == kRowReuseB) PINTT_LAUNCH(LC, LROWS, kRowReuseB);       \
+        else return hipErrorInvalidValue;                                       \
         return hipGetLastError();                                               \
     }
+#define PINTT_PAIR_CASE(LC, LROWS)                                              \
+    if (kind == kRowPair && pl.log2c == LC && pl.log2rows == LROWS) {          \
+        PINTT_LAUNCH(LC, LROWS, kRowPair);                                      \
+        return hipGetLastError();                                               \
+    }
+    PINTT_PAIR_CASES(PINTT_PAIR_CASE)
     PINTT_ROW_CASES(PINTT_CASE)
+#undef PINTT_PAIR_CASE
 #undef PINTT_CASE
 #undef PINTT_LAUNCH
     return hipErrorInvalidValue;

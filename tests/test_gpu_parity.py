@@ -171,6 +171,11 @@ def test_every_compiled_tile_plan(engine, oracle, log2n):
     assert engine.transform_log2(a.size, b.size) == log2n
     want = oracle.gmp_mul(a, b)
     want_sq = oracle.gmp_mul(b, b) if log2n <= 20 else None
+    # A product cut into two chunks whose chunk products have length 2^log2n:
+    # runs the paired row kernels (two chunks sharing b) of the plan.
+    h, bn2 = total // 2, total // 3
+    a2, b2 = pc.operands(2000 + log2n, 64 * 2 * h, 64 * bn2)
+    want2 = oracle.gmp_mul(a2, b2) if log2n <= 22 else None
     saved = engine.get_plan(log2n)
     try:
         for (r, w, rows) in plans:
@@ -179,6 +184,13 @@ def test_every_compiled_tile_plan(engine, oracle, log2n):
             assert np.array_equal(got, want), f"plan r={r} w={w} rows={rows} at 2^{log2n}"
             if want_sq is not None:
                 assert np.array_equal(engine.sqr(b), want_sq), f"square plan r={r} w={w} rows={rows}"
+            if want2 is not None:
+                engine.set_split(2)
+                try:
+                    got2 = engine.mul(a2, b2)
+                finally:
+                    engine.set_split(1)
+                assert np.array_equal(got2, want2), f"two-chunk plan r={r} w={w} rows={rows} at 2^{log2n}"
     finally:
         engine.set_plan(log2n, saved["log2r"], saved["log2w"], saved["log2rows"])
 
