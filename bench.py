@@ -43,17 +43,44 @@ K_BITS = 26
 SEED = 26
 
 
-def pass_bytes(an: int, bn: int, log2n: int, square: bool) -> dict:
-    """Compulsory HBM bytes of each pass (4-byte residues, 3 primes)."""
+def pass_bytes(an: int, bn: int, log2n: int, square: bool, chunks: int = 1, chunk_limbs: int = 0) -> dict:
+    """Compulsory HBM bytes of each pass (4-byte residues, 3 primes) for the
+    pipeline the engine runs: the longer operand in `chunks` pieces of
+    `chunk_limbs` limbs, each multiplied by one transform of the other operand
+    at length 2^log2n."""
     n = 1 << log2n
-    ops = 1 if square else 2
-    limbs_in = an if square else an + bn
+    if square:
+        return {
+            "forward_columns": 8 * an + 12 * n,                    # read limbs, write residues
+            "row_convolution": 12 * n + 12 * n,                    # read residues, write product
+            "inverse_columns": 24 * n,                             # read and write the product residues
+            "crt_carry": 12 * min(n, 4 * an - 1) + 16 * an,        # read residues, write limbs
+        }
+    m = chunks
+    h = chunk_limbs or an
+    pieces = [min(h, an - i * h) for i in range(m)]
+    lens = [min(n, 2 * p + 2 * bn - 1) for p in pieces]
     return {
-        "forward_columns": 8 * limbs_in + ops * 12 * n,        # read limbs, write residues
-        "row_convolution": ops * 12 * n + 12 * n,              # read residues, write product
-        "inverse_columns": 12 * n + 12 * n,                    # read and write the product residues
-        "crt_carry": 12 * min(n, 2 * (an + bn)) + 8 * (an + bn),  # read residues, write limbs
+        "forward_columns": 8 * (an + bn) + (m + 1) * 12 * n,
+        "row_convolution": (m + 1) * 12 * n + m * 12 * n,
+        "inverse_columns": m * 24 * n,
+        "crt_carry": 12 * sum(lens) + 8 * (an + bn),
     }
+
+
+def measured_traffic(path: str, pass_name: str, plans: dict):
+    """HBM bytes of one launch set of `pass_name` per product, as measured by
+    rocprofv3 counters (FETCH_SIZE + WRITE_SIZE, scaled by a device copy of
+    known size) for the same workload and tile plans; None when the summary is
+    absent or was taken with other plans."""
+    try:
+        t = json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return None
+    if t.get("tile_plans") != plans or pass_name not in t.get("passes", {}):
+        return None
+    p = t["passes"][pass_name]
+    return int(p["fetch_bytes"] + p["write_bytes"])
 
 
 def survey_bytes(an: int, bn: int, log2n: int, square: bool) -> int:
@@ -336,6 +363,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the tree-level replay batch leg")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--plan", action="append", default=[], metavar="N:R,W,ROWS",
+                    help="force the tile plan of transform length 2^N (else autotuned); repeatable")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "k26_multiply_traffic.json"),
+                    help="measured HBM bytes per pass (rocprofv3 FETCH_SIZE/WRITE_SIZE summary); its tile "
+                         "plans are used unless --plan or --autotune is given")
+    ap.add_argument("--autotune", action="store_true", help="tune the tile plans instead of using the profiled ones")
     ap.add_argument("--pi-digits", type=int, default=100_000_000,
                     help="digits of the pi leg (GMP-only vs GPU runs of pi_compute_hip); 0 skips it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -354,6 +387,17 @@ def main() -> int:
         print("gpu_ntt_init failed: " + lib.gpu_ntt_last_error().decode(), file=sys.stderr)
         return 1
 
+    specs = list(args.plan)
+    if not specs and not args.autotune:
+        try:
+            prof = json.loads(Path(args.traffic).read_text())
+            specs = [f"{n}:{','.join(map(str, v))}" for n, v in prof["tile_plans"].items()]
+        except (OSError, ValueError, KeyError):
+            pass
+    for spec in specs:
+        n, rest = spec.split(":")
+        r, w, rows = (int(x) for x in rest.split(","))
+        pc.set_plan(int(n), r, w, rows)
     a, b = pc.operands(SEED, 1 << K_BITS, (1 << K_BITS) // 3)
     an, bn = a.size, b.size
     log2n = pc.transform_log2(an, bn)
@@ -373,11 +417,14 @@ def main() -> int:
     passes = time_passes(stream, da, an, db, bn, dout, max(5, min(50, args.steps)))
     passes_sq = time_passes(stream, da, an, da, an, dsq, max(5, min(50, args.steps)))
 
-    pb = pass_bytes(an, bn, log2n, False)
+    plans = {str(n): [pc.get_plan(n)[k] for k in ("log2r", "log2w", "log2rows")] for n in (log2n - 1, log2n)}
+    pp = pc.product_plan(an, bn)
+    pb = pass_bytes(an, bn, pp["log2n"], False, pp["chunks"], pp["chunk_limbs"])
     dom = max(passes, key=passes.get)
     achieved = pb[dom] / (passes[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(args.traffic, dom, plans),
             "algorithmic_bytes": pb[dom], "kernel_ms": round(passes[dom], 4)}
     b_alg = survey_bytes(an, bn, log2n, False)
     result = {
@@ -394,8 +441,9 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic (splitmix64 seed 26 operands, resident in HBM)",
         "config": {"workload": "exact multiply 2^26 x floor(2^26/3) bits",
-                   "limbs": [an, bn], "transform": f"N=2^{log2n}, 3 primes < 2^31",
-                   "parallelism": "single GPU"},
+                   "limbs": [an, bn],
+                   "transform": f"{pp['chunks']} chunk product(s) at N=2^{pp['log2n']}, 3 primes < 2^31",
+                   "tile_plans": plans, "product_plan": pp, "parallelism": "single GPU"},
         "roofline": roof,
         "product_roofline": {"algorithmic_bytes": b_alg,
                              "achieved_GBs": round(b_alg / (ms_mul * 1e-3) / 1e9, 1),

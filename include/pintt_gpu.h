@@ -108,6 +108,13 @@ void gpu_ntt_set_split(int mode);
  * or -1 when it exceeds 2^25. */
 int gpu_ntt_transform_log2(size_t an, size_t bn);
 
+/* How gpu_ntt_mul would run an an-limb by bn-limb product (not a square):
+ * transform length 2^*log2n, the longer operand cut into *chunks pieces of
+ * *chunk_limbs limbs (may be null) sharing one transform of the other.  May
+ * tune tile plans first (waits on the device).  No reference counterpart: the
+ * reference sizes one transform per call (src/gpu/int_ntt.cu:305-314). */
+int gpu_ntt_product_plan(size_t an, size_t bn, int* log2n, int* chunks, size_t* chunk_limbs);
+
 /* Largest an + bn accepted. */
 size_t gpu_ntt_max_limbs(void);
 

@@ -45,6 +45,11 @@ ABI = {
     ),
     "gpu_ntt_transform_log2": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t]),
     "gpu_ntt_max_limbs": (ctypes.c_size_t, []),
+    "gpu_ntt_product_plan": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+         ctypes.POINTER(ctypes.c_size_t)],
+    ),
     "gpu_ntt_last_error": (ctypes.c_char_p, []),
     "gpu_ntt_set_timing": (None, [ctypes.c_int]),
     "gpu_ntt_pass_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
@@ -242,6 +247,14 @@ def get_plan(log2n: int) -> dict:
     _check(lib, lib.gpu_ntt_get_plan(log2n, ctypes.byref(r), ctypes.byref(w), ctypes.byref(rows), ctypes.byref(ms)))
     return {"log2n": log2n, "log2r": r.value, "log2c": log2n - r.value, "log2w": w.value,
             "log2rows": rows.value, "tuned_ms": ms.value}
+
+
+def product_plan(an: int, bn: int) -> dict:
+    """Transform length and chunking gpu_ntt_mul would use for an an x bn product."""
+    lib = load_library()
+    k, m, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    _check(lib, lib.gpu_ntt_product_plan(an, bn, ctypes.byref(k), ctypes.byref(m), ctypes.byref(h)))
+    return {"log2n": k.value, "chunks": m.value, "chunk_limbs": h.value}
 
 
 def set_plan(log2n: int, log2r: int, log2w: int, log2rows: int) -> None:

@@ -982,6 +982,20 @@ int gpu_ntt_transform_log2(size_t an, size_t bn) {
     return log2_for(an, bn);
 }
 
+int gpu_ntt_product_plan(size_t an, size_t bn, int* log2n, int* chunks, size_t* chunk_limbs) {
+    if (an == 0 || bn == 0 || !log2n || !chunks) return fail(PINTT_ERR_ARGS, "bad arguments");
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    Group gr;
+    static const uint64_t dummy = 0;
+    PINTT_TRY(plan_group(Request{&dummy, an, &dummy + (an == bn ? 1 : 0), bn, nullptr, false}, &gr));
+    *log2n = gr.log2n;
+    *chunks = (int)gr.m;
+    if (chunk_limbs) *chunk_limbs = gr.h;
+    return PINTT_OK;
+}
+
 size_t gpu_ntt_max_limbs(void) { return ((size_t)1 << (kMaxLog2N - 1)); }
 
 const char* gpu_ntt_last_error(void) { return t_error.c_str(); }
