@@ -93,6 +93,26 @@ int gpu_ntt_graph_launch(void* graph, void* stream);
 size_t gpu_ntt_graph_nodes(void* graph);
 void gpu_ntt_graph_destroy(void* graph);
 
+/* Device buffers and batched copies, for callers that keep operands on the
+ * device between products (binary splitting's upper tree levels).  No
+ * reference counterpart: the reference copies every operand per call
+ * (src/gpu/int_ntt.cu:321-350).  Upload and download go through pinned
+ * staging, gathered and scattered by host threads, and return when done. */
+int gpu_ntt_dev_alloc(size_t limbs, uint64_t** d_ptr);
+int gpu_ntt_dev_free(uint64_t* d_ptr);  /* waits for the library stream first */
+int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* h_src, const size_t* limbs);
+int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const* d_src, const size_t* limbs);
+int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs);  /* enqueued */
+
+/* out[i] = x[i] + y[i] (max(xn, yn) + 1 limbs) or, when subtract[i], x[i] -
+ * y[i] (max(xn, yn) limbs, requires x >= y; lengths may count zero top limbs):
+ * magnitudes, device pointers, outputs distinct from operands.  The signed
+ * R = Qr*Rl + Pl*Rr of binary-splitting merges (src/engine/binary_splitting.cpp:199,229,258,280)
+ * on device-resident values.  Waits for completion; PINTT_ERR_ARGS if a
+ * subtraction found x < y (the outputs are then undefined). */
+int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
+                             const size_t* yn, const int* subtract, uint64_t* const* d_out);
+
 /* Wait for all enqueued products; PINTT_ERR_HIP if the device reported a
  * failed carry look-back. */
 int gpu_ntt_synchronize(void);
@@ -136,6 +156,10 @@ int gpu_ntt_get_plan(int log2n, int* log2r, int* log2w, int* log2rows, float* ms
 int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows);
 /* Writes up to max_plans (log2r, log2w, log2rows) triples; returns the count. */
 int gpu_ntt_list_plans(int log2n, int* out, int max_plans);
+/* Tile plans (and their tuned times) to / from a text file, so that a
+ * process can skip tuning lengths another run already tuned. */
+int gpu_ntt_save_plans(const char* path);
+int gpu_ntt_load_plans(const char* path);
 
 /* Per-pass timing with HIP events on the product's stream.  When enabled,
  * every product records events around its passes; gpu_ntt_pass_times writes

@@ -83,6 +83,22 @@ ABI = {
     "gpu_ntt_graph_nodes": (ctypes.c_size_t, [ctypes.c_void_p]),
     "gpu_ntt_graph_destroy": (None, [ctypes.c_void_p]),
     "gpu_ntt_synchronize": (ctypes.c_int, []),
+    "gpu_ntt_save_plans": (ctypes.c_int, [ctypes.c_char_p]),
+    "gpu_ntt_load_plans": (ctypes.c_int, [ctypes.c_char_p]),
+    "gpu_ntt_dev_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_u64p)]),
+    "gpu_ntt_dev_free": (ctypes.c_int, [_u64p]),
+    "gpu_ntt_upload": (
+        ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t)]
+    ),
+    "gpu_ntt_download": (
+        ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t)]
+    ),
+    "gpu_ntt_dev_copy": (ctypes.c_int, [_u64p, _u64p, ctypes.c_size_t]),
+    "gpu_ntt_add_batch_device": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
+         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p)],
+    ),
     # include/pintt_mlp.h
     "gpu_mlp_param_count": (ctypes.c_size_t, [ctypes.c_int]),
     "gpu_mlp_forward": (
@@ -327,6 +343,40 @@ def to_int(limbs: np.ndarray) -> int:
 def from_int(x: int, nlimbs: int | None = None) -> np.ndarray:
     n = nlimbs if nlimbs is not None else max(1, (x.bit_length() + 63) // 64)
     return np.frombuffer(x.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)
+
+
+# ---- device-resident limb arrays (binary splitting's upper levels) ----
+
+def add_batch(pairs, subtract) -> list:
+    """Device add/subtract of magnitudes: [(x, y)] numpy uint64 limb arrays,
+    subtract[i] -> x - y (needs x >= y).  Uploads, runs one batch, downloads."""
+    lib = load_library()
+    n = len(pairs)
+    xs = [np.ascontiguousarray(x, np.uint64) for x, _ in pairs]
+    ys = [np.ascontiguousarray(y, np.uint64) for _, y in pairs]
+    widths = [max(x.size, y.size) + (0 if s else 1) for (x, y), s in zip(zip(xs, ys), subtract)]
+    dev = []
+    try:
+        def alloc(k):
+            p = _u64p()
+            _check(lib, lib.gpu_ntt_dev_alloc(k, ctypes.byref(p)))
+            dev.append(p)
+            return p
+        dx = [alloc(x.size) for x in xs]
+        dy = [alloc(y.size) for y in ys]
+        do = [alloc(w) for w in widths]
+        arr = lambda ps: (_u64p * len(ps))(*ps)
+        sz = lambda v: (ctypes.c_size_t * len(v))(*v)
+        _check(lib, lib.gpu_ntt_upload(n, arr(dx), arr([_ptr(x) for x in xs]), sz([x.size for x in xs])))
+        _check(lib, lib.gpu_ntt_upload(n, arr(dy), arr([_ptr(y) for y in ys]), sz([y.size for y in ys])))
+        _check(lib, lib.gpu_ntt_add_batch_device(n, arr(dx), sz([x.size for x in xs]), arr(dy), sz([y.size for y in ys]),
+                                                 (ctypes.c_int * n)(*[1 if s else 0 for s in subtract]), arr(do)))
+        outs = [np.zeros(w, np.uint64) for w in widths]
+        _check(lib, lib.gpu_ntt_download(n, arr([_ptr(o) for o in outs]), arr(do), sz(widths)))
+        return outs
+    finally:
+        for p in dev:
+            lib.gpu_ntt_dev_free(p)
 
 
 # ---- dispatch cost model (include/pintt_mlp.h) ----

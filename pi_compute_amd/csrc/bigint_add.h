@@ -1,0 +1,32 @@
+// bigint_add.h - descriptors and launcher of the batched device add/subtract.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pintt {
+
+constexpr int kAddThreads = 256;
+constexpr int kAddLimbsPerThread = 16;
+constexpr int kAddLimbsPerBlock = kAddThreads * kAddLimbsPerThread;
+
+struct AddJob {
+    const uint64_t* x;
+    const uint64_t* y;
+    uint64_t* out;
+    uint64_t xn, yn;  // limbs (>= 1)
+    uint32_t sub;     // 1: out = x - y over max(xn, yn) limbs (x >= y); 0: out = x + y over max(xn, yn) + 1 limbs
+    uint32_t blk0;    // first block of the job in the launch
+    uint32_t nblk;    // blocks of the job: ceil(max(xn, yn) / kAddLimbsPerBlock)
+    uint32_t pad;
+};
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+
+// jobs: device array sorted by blk0; maps, carry_in: nblocks bytes each;
+// error: set to nonzero when a subtraction's x < y.
+hipError_t launch_add_batch(const AddJob* jobs, uint32_t njobs, uint32_t nblocks, uint8_t* maps, uint8_t* carry_in,
+                            uint32_t* error, hipStream_t s);
+
+}  // namespace pintt

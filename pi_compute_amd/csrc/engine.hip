@@ -33,6 +33,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "bigint_add.h"
 #include "ntt_kernels.h"
 
 namespace pintt {
@@ -73,6 +74,14 @@ struct Engine {
     size_t pin_cap = 0;  // limbs
     uint64_t* pin_in = nullptr;
     uint64_t* pin_out = nullptr;
+    // Batched add/subtract: job descriptors (pinned + device), per-block maps.
+    size_t add_job_cap = 0;
+    AddJob* add_jobs_host = nullptr;
+    AddJob* add_jobs_dev = nullptr;
+    size_t add_blk_cap = 0;
+    uint8_t* add_maps = nullptr;
+    uint8_t* add_carry = nullptr;
+    uint32_t* add_error = nullptr;
     bool timing = false;
     hipEvent_t ev[5] = {};
     bool have_times = false;
@@ -546,16 +555,29 @@ int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned
         if (H && n <= (uint32_t)kInlineJobs) return inline_jobs<T>(reinterpret_cast<const T*>(H + off), n);
         return device_jobs<T>(reinterpret_cast<const T*>(D + off));
     };
+    // Grid z carries the job index: launches of more than 65535 jobs go in pieces.
+    constexpr uint32_t kMaxJobs = 65535;
     for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
         const BucketLayout& b = bp.buckets[bi];
         const Plan& pl = bp.plans[bi];
         if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
-        PINTT_HIP(launch_forward_columns(pl, list((FwdJob*)nullptr, b.fwd_off, b.nfwd), b.nfwd, tw, s));
+        for (uint32_t k = 0; k < b.nfwd; k += kMaxJobs) {
+            const uint32_t c = std::min(kMaxJobs, b.nfwd - k);
+            PINTT_HIP(launch_forward_columns(pl, list((FwdJob*)nullptr, b.fwd_off + k * sizeof(FwdJob), c), c, tw, s));
+        }
         if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
         for (const RowLaunch& rl : b.rows)
-            PINTT_HIP(launch_row_convolution(pl, list((RowJob*)nullptr, rl.off, rl.count), rl.count, rl.kind, tw, s));
+            for (uint32_t k = 0; k < rl.count; k += kMaxJobs) {
+                const uint32_t c = std::min(kMaxJobs, rl.count - k);
+                PINTT_HIP(launch_row_convolution(pl, list((RowJob*)nullptr, rl.off + k * sizeof(RowJob), c), c,
+                                                 rl.kind, tw, s));
+            }
         if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
-        PINTT_HIP(launch_inverse_columns(pl, list((InvJob*)nullptr, b.inv_off, b.ninv), b.ninv, g.glob, tw, s));
+        for (uint32_t k = 0; k < b.ninv; k += kMaxJobs) {
+            const uint32_t c = std::min(kMaxJobs, b.ninv - k);
+            PINTT_HIP(launch_inverse_columns(pl, list((InvJob*)nullptr, b.inv_off + k * sizeof(InvJob), c), c,
+                                             g.glob, tw, s));
+        }
         if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
         const uint32_t ng = (uint32_t)b.groups.size();
         PINTT_HIP(launch_crt_carry(list((CarryJob*)nullptr, b.carry_off, ng), ng, b.nblocks, b.log2n,
@@ -879,6 +901,11 @@ void gpu_ntt_shutdown(void) {
     (void)hipFree(g.io_out);
     if (g.pin_in) (void)hipHostFree(g.pin_in);
     if (g.pin_out) (void)hipHostFree(g.pin_out);
+    if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
+    (void)hipFree(g.add_jobs_dev);
+    (void)hipFree(g.add_maps);
+    (void)hipFree(g.add_carry);
+    (void)hipFree(g.add_error);
     (void)hipFree(g.tune_in);
     (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);
@@ -961,6 +988,149 @@ int gpu_ntt_graph_launch(void* graph, void* stream) {
     return PINTT_OK;
 }
 
+int gpu_ntt_dev_alloc(size_t limbs, uint64_t** d_ptr) {
+    if (!d_ptr || limbs == 0) return fail(PINTT_ERR_ARGS, "bad arguments");
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    if (hipMalloc(reinterpret_cast<void**>(d_ptr), limbs * sizeof(uint64_t)) != hipSuccess) {
+        *d_ptr = nullptr;
+        return fail(PINTT_ERR_NO_MEMORY, "hipMalloc failed");
+    }
+    return PINTT_OK;
+}
+
+int gpu_ntt_dev_free(uint64_t* d_ptr) {
+    if (!d_ptr) return PINTT_OK;
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    PINTT_HIP(hipFree(d_ptr));
+    return PINTT_OK;
+}
+
+// Host <-> device copies of many arrays through the pinned staging buffers
+// (gathered / scattered by host threads, one DMA each way).
+int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* h_src, const size_t* limbs) {
+    if (count == 0) return PINTT_OK;
+    if (!d_dst || !h_src || !limbs) return fail(PINTT_ERR_ARGS, "null pointer");
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) total += limbs[i];
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    PINTT_TRY(ensure_pinned(total));
+    std::vector<Piece> gather;
+    gather.reserve(count);
+    size_t off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        gather.push_back(Piece{g.pin_in + off, h_src[i], limbs[i]});
+        off += limbs[i];
+    }
+    parallel_copy(gather, total);
+    off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (limbs[i])
+            PINTT_HIP(hipMemcpyAsync(d_dst[i], g.pin_in + off, limbs[i] * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                     g.stream));
+        off += limbs[i];
+    }
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    return PINTT_OK;
+}
+
+int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const* d_src, const size_t* limbs) {
+    if (count == 0) return PINTT_OK;
+    if (!h_dst || !d_src || !limbs) return fail(PINTT_ERR_ARGS, "null pointer");
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) total += limbs[i];
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    PINTT_TRY(ensure_pinned(total));
+    std::vector<Piece> scatter;
+    scatter.reserve(count);
+    size_t off = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (limbs[i])
+            PINTT_HIP(hipMemcpyAsync(g.pin_out + off, d_src[i], limbs[i] * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                     g.stream));
+        scatter.push_back(Piece{h_dst[i], g.pin_out + off, limbs[i]});
+        off += limbs[i];
+    }
+    PINTT_TRY(check_device_error());
+    parallel_copy(scatter, total);
+    return PINTT_OK;
+}
+
+int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs) {
+    if (!d_dst || !d_src) return fail(PINTT_ERR_ARGS, "null pointer");
+    if (limbs == 0) return PINTT_OK;
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipMemcpyAsync(d_dst, d_src, limbs * sizeof(uint64_t), hipMemcpyDeviceToDevice, g.stream));
+    return PINTT_OK;
+}
+
+int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
+                             const size_t* yn, const int* subtract, uint64_t* const* d_out) {
+    if (count == 0) return PINTT_OK;
+    if (!d_x || !xn || !d_y || !yn || !subtract || !d_out) return fail(PINTT_ERR_ARGS, "null pointer");
+    std::vector<AddJob> jobs(count);
+    size_t blocks = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (!d_x[i] || !d_y[i] || !d_out[i] || xn[i] == 0 || yn[i] == 0)
+            return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+        const size_t w = std::max(xn[i], yn[i]);
+        const size_t nb = (w + kAddLimbsPerBlock - 1) / kAddLimbsPerBlock;
+        if (blocks + nb > 0xFFFFFFFFull) return fail(PINTT_ERR_TOO_LARGE, "too many add blocks");
+        jobs[i] = AddJob{d_x[i], d_y[i], d_out[i], xn[i], yn[i], subtract[i] ? 1u : 0u, (uint32_t)blocks,
+                         (uint32_t)nb, 0u};
+        blocks += nb;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    PINTT_HIP(hipStreamSynchronize(g.stream));  // the staging buffers below may be reused
+    if (count > g.add_job_cap) {
+        if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
+        (void)hipFree(g.add_jobs_dev);
+        g.add_jobs_host = nullptr;
+        g.add_jobs_dev = nullptr;
+        g.add_job_cap = 0;
+        const size_t cap = std::max<size_t>(count, 1024);
+        if (hipHostMalloc(reinterpret_cast<void**>(&g.add_jobs_host), cap * sizeof(AddJob)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&g.add_jobs_dev), cap * sizeof(AddJob)) != hipSuccess)
+            return fail(PINTT_ERR_NO_MEMORY, "add descriptors");
+        g.add_job_cap = cap;
+    }
+    if (blocks > g.add_blk_cap) {
+        (void)hipFree(g.add_maps);
+        (void)hipFree(g.add_carry);
+        g.add_maps = g.add_carry = nullptr;
+        g.add_blk_cap = 0;
+        const size_t cap = std::max<size_t>(blocks, 1 << 16);
+        if (hipMalloc(reinterpret_cast<void**>(&g.add_maps), cap) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&g.add_carry), cap) != hipSuccess)
+            return fail(PINTT_ERR_NO_MEMORY, "add block state");
+        g.add_blk_cap = cap;
+    }
+    if (!g.add_error && hipMalloc(reinterpret_cast<void**>(&g.add_error), sizeof(uint32_t)) != hipSuccess)
+        return fail(PINTT_ERR_NO_MEMORY, "add error flag");
+    std::memcpy(g.add_jobs_host, jobs.data(), count * sizeof(AddJob));
+    PINTT_HIP(hipMemcpyAsync(g.add_jobs_dev, g.add_jobs_host, count * sizeof(AddJob), hipMemcpyHostToDevice, g.stream));
+    // A fill kernel, not hipMemsetAsync: no memset node if a caller ever captures this.
+    PINTT_HIP(launch_fill_u32(g.add_error, 0u, 1, g.stream));
+    PINTT_HIP(launch_add_batch(g.add_jobs_dev, (uint32_t)count, (uint32_t)blocks, g.add_maps, g.add_carry, g.add_error,
+                               g.stream));
+    uint32_t err = 0;
+    PINTT_HIP(hipMemcpyAsync(&err, g.add_error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    if (err) return fail(PINTT_ERR_ARGS, "subtraction with |x| < |y|");
+    return PINTT_OK;
+}
+
 size_t gpu_ntt_graph_nodes(void* graph) { return graph ? static_cast<BatchGraph*>(graph)->nodes : 0; }
 
 void gpu_ntt_graph_destroy(void* graph) {
@@ -1030,6 +1200,45 @@ int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows) {
     g.plan[log2n] = pl;
     g.have_plan[log2n] = true;
     return PINTT_OK;
+}
+
+int gpu_ntt_save_plans(const char* path) {
+    if (!path) return fail(PINTT_ERR_ARGS, "null path");
+    std::lock_guard<std::mutex> lock(g_mu);
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return fail(PINTT_ERR_ARGS, std::string("cannot write ") + path);
+    std::fprintf(f, "# log2n log2r log2w log2rows tuned_ms\n");
+    for (int n = kMinLog2N; n <= kMaxLog2N; ++n)
+        if (g.have_plan[n])
+            std::fprintf(f, "%d %d %d %d %.6f\n", n, g.plan[n].log2r, g.plan[n].log2w, g.plan[n].log2rows,
+                         (double)g.plan_ms[n]);
+    return std::fclose(f) == 0 ? PINTT_OK : fail(PINTT_ERR_ARGS, "write failed");
+}
+
+int gpu_ntt_load_plans(const char* path) {
+    if (!path) return fail(PINTT_ERR_ARGS, "null path");
+    std::lock_guard<std::mutex> lock(g_mu);
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return fail(PINTT_ERR_ARGS, std::string("cannot read ") + path);
+    char line[256];
+    int rc = PINTT_OK;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int n, r, w, rows;
+        double ms;
+        if (std::sscanf(line, "%d %d %d %d %lf", &n, &r, &w, &rows, &ms) != 5) continue;
+        if (n < kMinLog2N || n > kMaxLog2N) continue;
+        Plan pl{n, r, n - r, w, rows};
+        if (!plan_valid(pl)) {
+            rc = fail(PINTT_ERR_ARGS, "plan file names a tile plan not compiled into this library");
+            continue;
+        }
+        g.plan[n] = pl;
+        g.plan_ms[n] = (float)ms;
+        g.have_plan[n] = true;
+    }
+    std::fclose(f);
+    return rc;
 }
 
 int gpu_ntt_list_plans(int log2n, int* out, int max_plans) {

@@ -96,10 +96,9 @@ void BinarySplitting::merge(BSResult& l, BSResult& r, BSResult& out, bool need_p
     mult_.multiply(out.Q, l.Q, r.Q);
 }
 
-BSResult BinarySplitting::compute(unsigned long a, unsigned long b, bool need_top_p) {
-    levels_.clear();
-    using Clock = std::chrono::steady_clock;
+std::vector<BSResult> BinarySplitting::leaves(unsigned long a, unsigned long b) {
     // Bottom: subtrees of leaf_span terms on host threads.
+    const auto t0 = std::chrono::steady_clock::now();
     const unsigned long span = leaf_span_;
     const size_t nleaf = (size_t)((b - a + span - 1) / span);
     std::vector<BSResult> nodes(nleaf);
@@ -115,37 +114,48 @@ BSResult BinarySplitting::compute(unsigned long a, unsigned long b, bool need_to
             compute_small(lo, hi, (*c->nodes)[i]);
         },
         &lctx);
-    // Levels: adjacent pairs merge; an odd last node moves up unchanged.
-    while (nodes.size() > 1) {
-        auto t0 = Clock::now();
-        const size_t pairs = nodes.size() / 2;
-        const bool top = nodes.size() == 2;
-        std::vector<BSResult> next(pairs + (nodes.size() & 1));
-        std::vector<__mpz_struct> tmp(pairs);
-        std::vector<MulJob> jobs;
-        jobs.reserve(4 * pairs);
-        size_t max_limbs = 0;
-        for (size_t k = 0; k < pairs; ++k) {
-            BSResult& l = nodes[2 * k];
-            BSResult& r = nodes[2 * k + 1];
-            BSResult& o = next[k];
-            mpz_init(&tmp[k]);
-            jobs.push_back(MulJob{&tmp[k], r.Q, l.R});
-            jobs.push_back(MulJob{o.R, l.P, r.R});
-            if (!(top && !need_top_p)) jobs.push_back(MulJob{o.P, l.P, r.P});
-            jobs.push_back(MulJob{o.Q, l.Q, r.Q});
-            max_limbs = std::max({max_limbs, mpz_size(l.Q), mpz_size(r.Q), mpz_size(l.R), mpz_size(r.R)});
-        }
-        mult_.multiply_batch(jobs.data(), jobs.size(), threads_);
-        for (size_t k = 0; k < pairs; ++k) {
-            mpz_add(next[k].R, next[k].R, &tmp[k]);
-            mpz_clear(&tmp[k]);
-        }
-        if (nodes.size() & 1) next.back() = std::move(nodes.back());
-        levels_.push_back(Level{pairs, jobs.size(), max_limbs,
-                                std::chrono::duration<double>(Clock::now() - t0).count()});
-        nodes.swap(next);
+    leaf_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return nodes;
+}
+
+// One level: adjacent pairs merge (all products as one multiply_batch); an odd
+// last node moves up unchanged.
+void BinarySplitting::host_level(std::vector<BSResult>& nodes, bool need_top_p) {
+    using Clock = std::chrono::steady_clock;
+    auto t0 = Clock::now();
+    const size_t pairs = nodes.size() / 2;
+    const bool top = nodes.size() == 2;
+    std::vector<BSResult> next(pairs + (nodes.size() & 1));
+    std::vector<__mpz_struct> tmp(pairs);
+    std::vector<MulJob> jobs;
+    jobs.reserve(4 * pairs);
+    size_t max_limbs = 0;
+    for (size_t k = 0; k < pairs; ++k) {
+        BSResult& l = nodes[2 * k];
+        BSResult& r = nodes[2 * k + 1];
+        BSResult& o = next[k];
+        mpz_init(&tmp[k]);
+        jobs.push_back(MulJob{&tmp[k], r.Q, l.R});
+        jobs.push_back(MulJob{o.R, l.P, r.R});
+        if (!(top && !need_top_p)) jobs.push_back(MulJob{o.P, l.P, r.P});
+        jobs.push_back(MulJob{o.Q, l.Q, r.Q});
+        max_limbs = std::max({max_limbs, mpz_size(l.Q), mpz_size(r.Q), mpz_size(l.R), mpz_size(r.R)});
     }
+    mult_.multiply_batch(jobs.data(), jobs.size(), threads_);
+    for (size_t k = 0; k < pairs; ++k) {
+        mpz_add(next[k].R, next[k].R, &tmp[k]);
+        mpz_clear(&tmp[k]);
+    }
+    if (nodes.size() & 1) next.back() = std::move(nodes.back());
+    levels_.push_back(Level{pairs, jobs.size(), max_limbs,
+                            std::chrono::duration<double>(Clock::now() - t0).count()});
+    nodes.swap(next);
+}
+
+BSResult BinarySplitting::compute(unsigned long a, unsigned long b, bool need_top_p) {
+    levels_.clear();
+    std::vector<BSResult> nodes = leaves(a, b);
+    while (nodes.size() > 1) host_level(nodes, need_top_p);
     return std::move(nodes[0]);
 }
 

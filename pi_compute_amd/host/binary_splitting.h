@@ -42,6 +42,14 @@ public:
     BSResult compute(unsigned long a, unsigned long b, bool need_top_p = true);
     BSResult compute_recursive(unsigned long a, unsigned long b);
 
+    // Level-synchronous like compute(), but from the first level whose smallest
+    // operand has at least `device_from_limbs` limbs upward, P, Q and R stay in
+    // device memory: each level is one device product batch (gpu_ntt_mul_batch_device)
+    // and one device add batch for R, host copies only at the switch and at
+    // the top.  Every such product runs on the GPU engine.  Throws
+    // std::runtime_error if the device fails (the caller can rerun on the host).
+    BSResult compute_device(unsigned long a, unsigned long b, bool need_top_p = true, size_t device_from_limbs = 0);
+
     static unsigned long terms_needed(size_t digits);
     static void base_case(unsigned long k, BSResult& r);
     // Host-only recursion with mpz_mul (the bottom of the tree).
@@ -55,15 +63,20 @@ public:
         double seconds;
     };
     const std::vector<Level>& levels() const { return levels_; }
+    double leaf_seconds() const { return leaf_s_; }      // host subtrees of the last compute*
+    double upload_seconds() const { return upload_s_; }  // compute_device: host -> device switch
 
 private:
     void merge(BSResult& left, BSResult& right, BSResult& out, bool need_p);
+    std::vector<BSResult> leaves(unsigned long a, unsigned long b);
+    void host_level(std::vector<BSResult>& nodes, bool need_top_p);
     BSResult recurse(unsigned long a, unsigned long b, int depth);
 
     Multiplier& mult_;
     unsigned threads_;
     unsigned long leaf_span_ = 256;
     std::vector<Level> levels_;
+    double leaf_s_ = 0, upload_s_ = 0;
 };
 
 }  // namespace pi

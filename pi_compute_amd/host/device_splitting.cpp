@@ -1,0 +1,257 @@
+// device_splitting.cpp - BinarySplitting::compute_device: the upper tree
+// levels with P, Q and R resident in device memory (see binary_splitting.h).
+//
+// Per level: the level's results (P, Q, R of every merge, odd node copied up)
+// go to one buffer and the temporaries t = Qr*Rl, u = Pl*Rr to another; the
+// products run as one gpu_ntt_mul_batch_device, then R = t +- u as one
+// gpu_ntt_add_batch_device.  The buffers are reused level after level.
+// Magnitudes live on the device, signs on the host.  Lengths are upper
+// bounds: a product has an + bn limbs, a sum max + 1, so a top limb may be 0.
+//
+// Signs: Q > 0; P(k) < 0 for every leaf k >= 1; R(a, b) has the sign of its
+// first term (the series alternates with terms shrinking ~1.5e14-fold), so
+// when t and u differ in sign |t| > |u| and R = sign(t) * (|t| - |u|).  The
+// device add reports a subtraction that would go negative; then the level
+// is redone here on the host from downloaded t and u.
+#include <gmp.h>
+
+#include <algorithm>
+#include <utility>
+#include <chrono>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "binary_splitting.h"
+#include "pintt_gpu.h"
+
+namespace pi {
+namespace {
+
+struct DevVal {
+    uint64_t* d = nullptr;
+    size_t n = 0;  // limbs (upper bound)
+    int sign = 1;
+};
+struct DevNode {
+    DevVal P, Q, R;
+};
+
+void check(int rc, const char* what) {
+    if (rc != PINTT_OK) throw std::runtime_error(std::string(what) + ": " + gpu_ntt_last_error());
+}
+
+// A grow-only device buffer carved into consecutive values; reset() makes it
+// reusable for the next level (its previous contents are no longer needed).
+struct Arena {
+    uint64_t* base = nullptr;
+    size_t used = 0, cap = 0;
+    Arena() = default;
+    ~Arena() {
+        if (base) (void)gpu_ntt_dev_free(base);
+    }
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    void reset(size_t limbs) {
+        used = 0;
+        if (limbs <= cap) return;
+        if (base) check(gpu_ntt_dev_free(base), "gpu_ntt_dev_free");
+        base = nullptr;
+        cap = 0;
+        const size_t want = limbs + limbs / 4;  // some room: later levels grow slowly
+        check(gpu_ntt_dev_alloc(want, &base), "gpu_ntt_dev_alloc");
+        cap = want;
+    }
+    uint64_t* take(size_t limbs) {
+        if (used + limbs > cap) throw std::runtime_error("device arena overflow");
+        uint64_t* p = base + used;
+        used += limbs;
+        return p;
+    }
+};
+
+size_t bytes_of(mpz_srcptr x) { return std::max<size_t>(mpz_size(x), 1); }
+
+}  // namespace
+
+BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs) {
+    using Clock = std::chrono::steady_clock;
+    levels_.clear();
+    upload_s_ = 0;
+    std::vector<BSResult> nodes = leaves(a, b);
+    // Host levels while the operands are small.
+    while (nodes.size() > 1) {
+        size_t smallest = SIZE_MAX;
+        for (const auto& n : nodes) smallest = std::min({smallest, mpz_size(n.P), mpz_size(n.Q), mpz_size(n.R)});
+        if (smallest >= device_from_limbs) break;
+        host_level(nodes, need_top_p);
+    }
+    if (nodes.size() == 1) return std::move(nodes[0]);
+
+    // Upload the level: magnitudes into one arena, signs kept here.
+    const auto tu = Clock::now();
+    size_t total = 0;
+    for (const auto& n : nodes) total += bytes_of(n.P) + bytes_of(n.Q) + bytes_of(n.R);
+    // Three buffers reused across levels: this level's values, the next
+    // level's, and the temporaries t and u.
+    Arena buf[3];
+    Arena* cur = &buf[0];
+    Arena* nxt = &buf[1];
+    Arena& tmp = buf[2];
+    cur->reset(total);
+    std::vector<DevNode> dn(nodes.size());
+    {
+        std::vector<uint64_t*> dst;
+        std::vector<const uint64_t*> src;
+        std::vector<size_t> len;
+        std::vector<uint64_t> zero(1, 0);
+        auto put = [&](mpz_srcptr x, DevVal& v) {
+            v.n = bytes_of(x);
+            v.d = cur->take(v.n);
+            v.sign = mpz_sgn(x) < 0 ? -1 : 1;
+            dst.push_back(v.d);
+            src.push_back(mpz_size(x) ? reinterpret_cast<const uint64_t*>(mpz_limbs_read(x)) : zero.data());
+            len.push_back(v.n);
+        };
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            put(nodes[i].P, dn[i].P);
+            put(nodes[i].Q, dn[i].Q);
+            put(nodes[i].R, dn[i].R);
+        }
+        check(gpu_ntt_upload(dst.size(), dst.data(), src.data(), len.data()), "gpu_ntt_upload");
+    }
+    nodes.clear();
+    upload_s_ = std::chrono::duration<double>(Clock::now() - tu).count();
+
+    while (dn.size() > 1) {
+        auto t0 = Clock::now();
+        const size_t pairs = dn.size() / 2;
+        const bool top = dn.size() == 2;
+        const bool want_p = !(top && !need_top_p);
+        std::vector<DevNode> next(pairs + (dn.size() & 1));
+        size_t need = 0, tmp_need = 0, max_limbs = 0;
+        for (size_t k = 0; k < pairs; ++k) {
+            const DevNode &l = dn[2 * k], &r = dn[2 * k + 1];
+            const size_t tn = r.Q.n + l.R.n, un = l.P.n + r.R.n;
+            const bool same = l.R.sign == l.P.sign * r.R.sign;
+            tmp_need += tn + un;
+            need += (want_p ? l.P.n + r.P.n : 0) + l.Q.n + r.Q.n + std::max(tn, un) + (same ? 1 : 0);
+            max_limbs = std::max({max_limbs, l.Q.n, r.Q.n, l.R.n, r.R.n});
+        }
+        if (dn.size() & 1) need += dn.back().P.n + dn.back().Q.n + dn.back().R.n;
+        nxt->reset(need);
+        tmp.reset(tmp_need);
+        // Products.
+        std::vector<const uint64_t*> A, B;
+        std::vector<size_t> AN, BN;
+        std::vector<uint64_t*> O;
+        std::vector<DevVal> T(pairs), U(pairs);
+        auto job = [&](const DevVal& x, const DevVal& y, DevVal& out, Arena& ar) {
+            out.n = x.n + y.n;
+            out.d = ar.take(out.n);
+            out.sign = x.sign * y.sign;
+            A.push_back(x.d);
+            AN.push_back(x.n);
+            B.push_back(y.d);
+            BN.push_back(y.n);
+            O.push_back(out.d);
+        };
+        for (size_t k = 0; k < pairs; ++k) {
+            const DevNode &l = dn[2 * k], &r = dn[2 * k + 1];
+            job(r.Q, l.R, T[k], tmp);
+            job(l.P, r.R, U[k], tmp);
+            if (want_p) job(l.P, r.P, next[k].P, *nxt);
+            job(l.Q, r.Q, next[k].Q, *nxt);
+        }
+        check(gpu_ntt_mul_batch_device(O.size(), A.data(), AN.data(), B.data(), BN.data(), O.data(), nullptr),
+              "gpu_ntt_mul_batch_device");
+        // R = t +- u.
+        std::vector<const uint64_t*> X, Y;
+        std::vector<size_t> XN, YN;
+        std::vector<int> SUB;
+        std::vector<uint64_t*> RO;
+        for (size_t k = 0; k < pairs; ++k) {
+            const bool same = T[k].sign == U[k].sign;
+            DevVal& R = next[k].R;
+            R.n = std::max(T[k].n, U[k].n) + (same ? 1 : 0);
+            R.d = nxt->take(R.n);
+            R.sign = T[k].sign;
+            X.push_back(T[k].d);
+            XN.push_back(T[k].n);
+            Y.push_back(U[k].d);
+            YN.push_back(U[k].n);
+            SUB.push_back(same ? 0 : 1);
+            RO.push_back(R.d);
+        }
+        int rc = gpu_ntt_add_batch_device(pairs, X.data(), XN.data(), Y.data(), YN.data(), SUB.data(), RO.data());
+        if (rc == PINTT_ERR_ARGS) {
+            // Some |t| < |u|: redo every R of this level on the host.
+            for (size_t k = 0; k < pairs; ++k) {
+                mpz_t t, u;
+                mpz_inits(t, u, NULL);
+                uint64_t* tp = reinterpret_cast<uint64_t*>(mpz_limbs_write(t, T[k].n));
+                uint64_t* up = reinterpret_cast<uint64_t*>(mpz_limbs_write(u, U[k].n));
+                const uint64_t* src[2] = {T[k].d, U[k].d};
+                uint64_t* dst[2] = {tp, up};
+                size_t len[2] = {T[k].n, U[k].n};
+                check(gpu_ntt_download(2, dst, src, len), "gpu_ntt_download");
+                size_t tn = T[k].n, un = U[k].n;
+                while (tn && !tp[tn - 1]) --tn;
+                while (un && !up[un - 1]) --un;
+                mpz_limbs_finish(t, (mp_size_t)tn);
+                mpz_limbs_finish(u, (mp_size_t)un);
+                if (T[k].sign < 0) mpz_neg(t, t);
+                if (U[k].sign < 0) mpz_neg(u, u);
+                mpz_add(t, t, u);
+                DevVal& R = next[k].R;
+                R.sign = mpz_sgn(t) < 0 ? -1 : 1;
+                std::vector<uint64_t> mag(R.n, 0);
+                size_t cnt = 0;
+                mpz_export(mag.data(), &cnt, -1, 8, 0, 0, t);
+                if (cnt > R.n) throw std::runtime_error("R larger than its bound");
+                const uint64_t* ms = mag.data();
+                uint64_t* rd = R.d;
+                size_t rn = R.n;
+                check(gpu_ntt_upload(1, &rd, &ms, &rn), "gpu_ntt_upload");
+                mpz_clears(t, u, NULL);
+            }
+        } else {
+            check(rc, "gpu_ntt_add_batch_device");
+        }
+        if (dn.size() & 1) {
+            const DevNode& o = dn.back();
+            DevNode& m = next.back();
+            for (auto pr : {std::make_pair(&o.P, &m.P), std::make_pair(&o.Q, &m.Q), std::make_pair(&o.R, &m.R)}) {
+                pr.second->n = pr.first->n;
+                pr.second->sign = pr.first->sign;
+                pr.second->d = nxt->take(pr.first->n);
+                check(gpu_ntt_dev_copy(pr.second->d, pr.first->d, pr.first->n), "gpu_ntt_dev_copy");
+            }
+        }
+        check(gpu_ntt_synchronize(), "gpu_ntt_synchronize");
+        std::swap(cur, nxt);  // the old level's buffer is reused two levels on
+        dn.swap(next);
+        levels_.push_back(Level{pairs, O.size(), max_limbs,
+                                std::chrono::duration<double>(Clock::now() - t0).count()});
+    }
+
+    // Top: download.
+    BSResult res;
+    auto get = [&](const DevVal& v, mpz_ptr x) {
+        uint64_t* p = reinterpret_cast<uint64_t*>(mpz_limbs_write(x, (mp_size_t)v.n));
+        const uint64_t* src = v.d;
+        size_t n = v.n;
+        check(gpu_ntt_download(1, &p, &src, &n), "gpu_ntt_download");
+        while (n && !p[n - 1]) --n;
+        mpz_limbs_finish(x, (mp_size_t)n);
+        if (v.sign < 0) mpz_neg(x, x);
+    };
+    if (need_top_p || dn[0].P.d) {
+        if (dn[0].P.d) get(dn[0].P, res.P);
+    }
+    get(dn[0].Q, res.Q);
+    get(dn[0].R, res.R);
+    return res;
+}
+
+}  // namespace pi

@@ -16,6 +16,11 @@
 //                             reference's sequence) or Newton through the
 //                             multiplier (default: newton with --gpu)
 //   --convert host|multiplier decimal conversion (default: multiplier with --gpu)
+//   --plans FILE              tile plans: loaded if FILE exists, saved after the run
+//   --device-levels [L]       with --gpu --tree levels: P, Q, R stay on the GPU from
+//                             the first level whose operands have >= L limbs (default 0)
+#include <unistd.h>
+
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -27,14 +32,27 @@
 #include "hip_ntt_multiplier.h"
 #include "mlp_dispatch.h"
 #include "pi_engine.h"
+#include "pintt_gpu.h"
 
 namespace {
+
+// pi_compute_amd/data, found from this executable's path (pi_compute_amd/bin/).
+std::string data_dir() {
+    char buf[4096];
+    const ssize_t n = readlink("/proc/self/exe", buf, sizeof buf - 1);
+    if (n <= 0) return "data";
+    std::string exe(buf, (size_t)n);
+    const size_t slash = exe.rfind('/');
+    const std::string bin = slash == std::string::npos ? "." : exe.substr(0, slash);
+    return bin + "/../data";
+}
 
 void usage() {
     std::cout << "usage: pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]\n"
                  "                      [--threads T] [--tree levels|recursive] [--per-call]\n"
                  "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
-                 "                      [--final mpf|newton] [--convert host|multiplier] [--verbose]\n";
+                 "                      [--final mpf|newton] [--convert host|multiplier] [--device-levels [L]]\n"
+                 "                      [--verbose]\n";
 }
 
 }  // namespace
@@ -44,7 +62,7 @@ int main(int argc, char** argv) {
     cfg.output_file = "pi_digits.txt";
     bool gpu = false, per_call = false;
     size_t threshold = 10000;
-    std::string dispatch = "threshold", model, log, final_step, convert;
+    std::string dispatch = "threshold", model, log, final_step, convert, plans;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -70,6 +88,11 @@ int main(int argc, char** argv) {
         else if (a == "--log") log = next();
         else if (a == "--final") final_step = next();
         else if (a == "--convert") convert = next();
+        else if (a == "--plans") plans = next();
+        else if (a == "--device-levels") {
+            cfg.device_levels = true;
+            if (i + 1 < argc && argv[i + 1][0] != '-') cfg.device_from_limbs = std::stoul(argv[++i]);
+        }
         else if (a == "--verbose" || a == "-v") cfg.verbose = true;
         else if (a == "--help" || a == "-h") { usage(); return 0; }
         else { std::cerr << "unknown option " << a << std::endl; usage(); return 2; }
@@ -82,6 +105,10 @@ int main(int argc, char** argv) {
     }
     cfg.final_step = final_step == "newton" ? pi::PiConfig::FinalStep::Newton : pi::PiConfig::FinalStep::Mpf;
     cfg.conversion = convert == "multiplier" ? pi::PiConfig::Conversion::Multiplier : pi::PiConfig::Conversion::Host;
+    if (cfg.device_levels && !gpu) {
+        std::cerr << "--device-levels needs --gpu" << std::endl;
+        return 2;
+    }
     if (cfg.digits == 0) {
         std::cerr << "--digits must be > 0" << std::endl;
         return 2;
@@ -94,6 +121,22 @@ int main(int argc, char** argv) {
         if (gpu) {
             auto h = std::make_unique<pi::HipNttMultiplier>(threshold);
             hip = h.get();
+            // Tile plans tuned on this GPU model ship in data/; a run tunes only lengths they miss.
+            const std::string shipped = data_dir() + "/plans_gfx950.txt";
+            if (plans.empty()) {
+                std::FILE* sf = std::fopen(shipped.c_str(), "r");
+                if (sf) {
+                    std::fclose(sf);
+                    (void)gpu_ntt_load_plans(shipped.c_str());
+                }
+            } else {
+                std::FILE* pf = std::fopen(plans.c_str(), "r");
+                if (pf) {
+                    std::fclose(pf);
+                    if (gpu_ntt_load_plans(plans.c_str()) != PINTT_OK)
+                        std::cerr << "plan file: " << gpu_ntt_last_error() << std::endl;
+                }
+            }
             hip->set_batching(!per_call);
             if (dispatch == "mlp") {
                 if (model.empty()) throw std::runtime_error("--dispatch mlp needs --model FILE");
@@ -105,7 +148,7 @@ int main(int argc, char** argv) {
             if (!log.empty() && !hip->open_log(log)) throw std::runtime_error("cannot open log " + log);
             mode = std::string("HIP NTT, ") + hip->device_name() + ", " + hip->dispatcher().name() +
                    " dispatch, " + (cfg.level_batches ? (per_call ? "levels/per-call" : "levels/batched") : "recursive") +
-                   ", final " + final_step + ", conversion " + convert;
+                   (cfg.device_levels ? ", device-resident levels" : "") + ", final " + final_step + ", conversion " + convert;
             m = std::move(h);
         } else {
             m = std::make_unique<pi::GmpMultiplier>();
@@ -113,6 +156,8 @@ int main(int argc, char** argv) {
         pi::PiEngine engine(*m);
         pi::PiResult r = engine.compute(cfg);
         if (hip) {
+            if (!plans.empty() && gpu_ntt_save_plans(plans.c_str()) != PINTT_OK)
+                std::cerr << "plan file: " << gpu_ntt_last_error() << std::endl;
             hip->close_log();
             if (cfg.verbose) hip->print_stats();
         }

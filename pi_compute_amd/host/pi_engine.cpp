@@ -22,10 +22,22 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     if (cfg.verbose) std::cout << "  terms: " << terms << std::endl;
 
     BinarySplitting bs(mult_, cfg.num_threads);
-    BSResult r = cfg.level_batches ? bs.compute(0, terms, false) : bs.compute_recursive(0, terms);
+    BSResult r;
+    if (cfg.level_batches && cfg.device_levels) {
+        try {
+            r = bs.compute_device(0, terms, false, cfg.device_from_limbs);
+        } catch (const std::exception& e) {
+            // Device trouble: the same tree on the host path, exact either way.
+            std::cerr << "  device-resident levels failed (" << e.what() << "); host levels instead" << std::endl;
+            r = bs.compute(0, terms, false);
+        }
+    } else {
+        r = cfg.level_batches ? bs.compute(0, terms, false) : bs.compute_recursive(0, terms);
+    }
     const double bs_s = since(t0);
     if (cfg.verbose) {
-        std::cout << "  binary splitting: " << bs_s << " s" << std::endl;
+        std::cout << "  binary splitting: " << bs_s << " s (host subtrees " << bs.leaf_seconds() << " s, upload "
+                  << bs.upload_seconds() << " s)" << std::endl;
         for (size_t i = 0; i < bs.levels().size(); ++i) {
             const auto& L = bs.levels()[i];
             std::cout << "    level " << i << ": " << L.products << " products, max " << L.max_limbs

@@ -29,6 +29,8 @@ struct PiConfig {
     bool verbose = false;
     unsigned num_threads = 0;  // 0: hardware concurrency
     bool level_batches = true;  // level-synchronous tree with one batch per level
+    bool device_levels = false;  // upper levels with P, Q, R resident on the GPU (level_batches only)
+    size_t device_from_limbs = 0;  // first device level: smallest operand at least this many limbs
     enum class FinalStep { Mpf, Newton } final_step = FinalStep::Mpf;
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
 };

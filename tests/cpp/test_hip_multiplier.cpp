@@ -202,6 +202,16 @@ int main() {
         };
         check(eq(lv, want) && eq(pc, want) && eq(rc, want) && m.stats().gpu_calls.load() > 0,
               "BinarySplittingOnGpuMatchesHost");
+        for (size_t from : {(size_t)0, (size_t)500, (size_t)5000}) {
+            for (unsigned long b : {30000ul, 4097ul, 3ul}) {
+                pi::BSResult w;
+                pi::BinarySplitting::compute_small(0, b, w);
+                pi::BSResult dv = bs.compute_device(0, b, true, from);
+                pi::BSResult dq = bs.compute_device(0, b, false, from);
+                check(eq(dv, w) && mpz_cmp(dq.Q, w.Q) == 0 && mpz_cmp(dq.R, w.R) == 0,
+                      "DeviceResidentLevels b=" + std::to_string(b) + " from=" + std::to_string(from));
+            }
+        }
     }
 
     mpz_clears(a, b, r, e, NULL);

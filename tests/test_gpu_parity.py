@@ -318,3 +318,62 @@ def test_tree_level_batch_reduced(engine, oracle):
     outs = engine.mul_batch([(a, a) if sq else (a, b) for a, b, sq in prods])
     for i, ((a, b, _), got) in enumerate(zip(prods, outs)):
         assert np.array_equal(got, oracle.gmp_mul(a, b)), f"product {i}"
+
+
+def _int_limbs(x: int, n: int) -> np.ndarray:
+    return pc.from_int(x, n)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_device_add_subtract_batch(engine, seed):
+    """Device add/subtract of magnitudes (the R = t +- u of device-resident
+    merges) against Python integers: random sizes across block borders,
+    unequal lengths both ways, zero top limbs, long carry and borrow chains."""
+    rng = np.random.default_rng(seed)
+    sizes = [1, 2, 17, 4095, 4096, 4097, 12289, 100000]
+    pairs, sub, want = [], [], []
+    for xn in sizes:
+        for yn in (1, max(1, xn // 3), xn, xn + 5):
+            x = rng.integers(0, 2**64, xn, dtype=np.uint64)
+            y = rng.integers(0, 2**64, yn, dtype=np.uint64)
+            x[-1] = 0 if xn > 3 and rng.random() < 0.3 else x[-1]  # zero top limb
+            X, Y = pc.to_int(x), pc.to_int(y)
+            pairs.append((x, y))
+            sub.append(False)
+            want.append(X + Y)
+            if X >= Y:
+                pairs.append((x, y))
+                sub.append(True)
+                want.append(X - Y)
+    for k in (1, 4096, 4097, 20000):  # carry through all ones; borrow through zeros; equal operands
+        ones = np.full(k, np.uint64(2**64 - 1))
+        one = np.array([1], np.uint64)
+        pairs += [(ones, one), (_int_limbs(1 << (64 * k), k + 1), one), (ones, ones)]
+        sub += [False, True, True]
+        want += [(1 << (64 * k)) - 1 + 1, (1 << (64 * k)) - 1, 0]
+    got = pc.add_batch(pairs, sub)
+    for g, w, (x, y), s in zip(got, want, pairs, sub):
+        assert g.size == max(x.size, y.size) + (0 if s else 1)
+        assert pc.to_int(g) == w, (x.size, y.size, s)
+
+
+def test_device_subtract_negative_is_reported(engine):
+    with pytest.raises(pc.PinttError) as ei:
+        pc.add_batch([(np.array([5], np.uint64), np.array([7], np.uint64))], [True])
+    assert ei.value.status == 5
+
+
+def test_plan_file_round_trip(engine, tmp_path):
+    lib = pc.load_library()
+    engine.autotune(12, 13)
+    before = {n: engine.get_plan(n) for n in (12, 13)}
+    path = str(tmp_path / "plans.txt").encode()
+    assert lib.gpu_ntt_save_plans(path) == 0
+    plans = engine.list_plans(12)
+    other = next(p for p in plans if p != (before[12]["log2r"], before[12]["log2w"], before[12]["log2rows"]))
+    engine.set_plan(12, *other)
+    assert lib.gpu_ntt_load_plans(path) == 0
+    after = engine.get_plan(12)
+    assert (after["log2r"], after["log2w"], after["log2rows"]) == (
+        before[12]["log2r"], before[12]["log2w"], before[12]["log2rows"])
+    assert lib.gpu_ntt_load_plans(str(tmp_path / "missing.txt").encode()) == 5

@@ -42,9 +42,11 @@ def gmp_digits(tmp_path_factory):
         ["--tree", "recursive", "--gpu-threshold", "64"],
         ["--tree", "levels", "--final", "mpf", "--convert", "host"],
         ["--tree", "levels", "--gpu-threshold", "64", "--convert", "multiplier"],
+        ["--tree", "levels", "--device-levels"],
+        ["--tree", "levels", "--device-levels", "2000"],
     ],
     ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64", "levels-mpf-host",
-         "levels-t64-convert"],
+         "levels-t64-convert", "device-levels", "device-levels-2000"],
 )
 def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
     digits = 1000000
