@@ -1029,12 +1029,15 @@ int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* 
         off += limbs[i];
     }
     parallel_copy(gather, total);
+    // One DMA per run of destinations that follow each other in device memory.
     off = 0;
-    for (size_t i = 0; i < count; ++i) {
-        if (limbs[i])
-            PINTT_HIP(hipMemcpyAsync(d_dst[i], g.pin_in + off, limbs[i] * sizeof(uint64_t), hipMemcpyHostToDevice,
-                                     g.stream));
-        off += limbs[i];
+    for (size_t i = 0; i < count;) {
+        size_t j = i + 1, run = limbs[i];
+        while (j < count && d_dst[j] == d_dst[j - 1] + limbs[j - 1]) run += limbs[j++];
+        if (run)
+            PINTT_HIP(hipMemcpyAsync(d_dst[i], g.pin_in + off, run * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+        off += run;
+        i = j;
     }
     PINTT_HIP(hipStreamSynchronize(g.stream));
     return PINTT_OK;
@@ -1052,12 +1055,16 @@ int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const
     std::vector<Piece> scatter;
     scatter.reserve(count);
     size_t off = 0;
-    for (size_t i = 0; i < count; ++i) {
-        if (limbs[i])
-            PINTT_HIP(hipMemcpyAsync(g.pin_out + off, d_src[i], limbs[i] * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                     g.stream));
-        scatter.push_back(Piece{h_dst[i], g.pin_out + off, limbs[i]});
-        off += limbs[i];
+    for (size_t i = 0; i < count;) {
+        size_t j = i + 1, run = limbs[i];
+        while (j < count && d_src[j] == d_src[j - 1] + limbs[j - 1]) run += limbs[j++];
+        if (run)
+            PINTT_HIP(hipMemcpyAsync(g.pin_out + off, d_src[i], run * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+        for (size_t k = i; k < j; ++k) {
+            scatter.push_back(Piece{h_dst[k], g.pin_out + off, limbs[k]});
+            off += limbs[k];
+        }
+        i = j;
     }
     PINTT_TRY(check_device_error());
     parallel_copy(scatter, total);

@@ -20,6 +20,7 @@
 #include <cstddef>
 #include <vector>
 
+#include "dispatch.h"
 #include "multiplier.h"
 
 namespace pi {
@@ -50,6 +51,11 @@ public:
     // std::runtime_error if the device fails (the caller can rerun on the host).
     BSResult compute_device(unsigned long a, unsigned long b, bool need_top_p = true, size_t device_from_limbs = 0);
 
+    // With a dispatcher set, compute_device switches to device-resident values
+    // at the first level whose products the dispatcher sends to the GPU (by
+    // transform work, at least half), instead of by device_from_limbs.
+    void set_switch_dispatcher(const Dispatcher* d) { switch_by_ = d; }
+
     static unsigned long terms_needed(size_t digits);
     static void base_case(unsigned long k, BSResult& r);
     // Host-only recursion with mpz_mul (the bottom of the tree).
@@ -77,6 +83,8 @@ private:
     unsigned long leaf_span_ = 256;
     std::vector<Level> levels_;
     double leaf_s_ = 0, upload_s_ = 0;
+    const Dispatcher* switch_by_ = nullptr;
+    bool level_prefers_device(const std::vector<BSResult>& nodes, bool need_top_p) const;
 };
 
 }  // namespace pi

@@ -5,7 +5,8 @@
 //   size sweep      |a| = 2^k bits, |b| = 2^k/3 bits and a^2, k = kmin..kmax,
 //                   single calls, reps per size (the crossover region)
 //   uniform batches B products of ~2^lg limbs (b from 1/4 to 1 of a, every
-//                   third a square), B = 2..4096, one multiply_batch each
+//                   third a square), B = 2..4096, one multiply_batch each,
+//                   and once more with device-resident operands
 //   tree level      4096 products, lognormal bit lengths (median 2^18,
 //                   sigma 0.6, clipped to [2^12, 2^24]), every third a square
 //
@@ -143,6 +144,7 @@ int main(int argc, char** argv) {
             auto jobs = bt.jobs();
             m.set_dispatcher(&gpu);
             m.multiply_batch(jobs.data(), jobs.size(), threads);
+            m.multiply_batch_resident(jobs.data(), jobs.size());
             if (bt.cpu_estimate_s() / std::max(1u, threads ? threads : 16u) < cpu_budget) {
                 m.set_dispatcher(&cpu);
                 m.multiply_batch(jobs.data(), jobs.size(), threads);
@@ -176,6 +178,7 @@ int main(int argc, char** argv) {
             m.set_dispatcher(d);
             m.multiply_batch(jobs.data(), jobs.size(), threads);
         }
+        m.multiply_batch_resident(jobs.data(), jobs.size());
         std::printf("tree level done (%.1f s)\n", seconds(t0));
     }
     m.close_log();

@@ -16,6 +16,7 @@
 #include <gmp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <utility>
 #include <chrono>
 #include <stdexcept>
@@ -79,11 +80,15 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
     levels_.clear();
     upload_s_ = 0;
     std::vector<BSResult> nodes = leaves(a, b);
-    // Host levels while the operands are small.
+    // Host levels while the operands are small (or the dispatcher keeps them on the host).
     while (nodes.size() > 1) {
-        size_t smallest = SIZE_MAX;
-        for (const auto& n : nodes) smallest = std::min({smallest, mpz_size(n.P), mpz_size(n.Q), mpz_size(n.R)});
-        if (smallest >= device_from_limbs) break;
+        if (switch_by_) {
+            if (level_prefers_device(nodes, need_top_p)) break;
+        } else {
+            size_t smallest = SIZE_MAX;
+            for (const auto& n : nodes) smallest = std::min({smallest, mpz_size(n.P), mpz_size(n.Q), mpz_size(n.R)});
+            if (smallest >= device_from_limbs) break;
+        }
         host_level(nodes, need_top_p);
     }
     if (nodes.size() == 1) return std::move(nodes[0]);
@@ -252,6 +257,35 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
     get(dn[0].Q, res.Q);
     get(dn[0].R, res.R);
     return res;
+}
+
+bool BinarySplitting::level_prefers_device(const std::vector<BSResult>& nodes, bool need_top_p) const {
+    const size_t pairs = nodes.size() / 2;
+    const bool top = nodes.size() == 2;
+    std::vector<ProductShape> shapes;
+    std::vector<double> work;
+    auto add = [&](mpz_srcptr x, mpz_srcptr y) {
+        const size_t xn = mpz_size(x), yn = mpz_size(y);
+        shapes.push_back(ProductShape{xn, yn, 0, false, true});
+        const double n = 2.0 * (double)(xn + yn);
+        work.push_back(n * std::log2(n + 2.0));
+    };
+    for (size_t k = 0; k < pairs; ++k) {
+        const BSResult &l = nodes[2 * k], &r = nodes[2 * k + 1];
+        add(r.Q, l.R);
+        add(l.P, r.R);
+        if (!(top && !need_top_p)) add(l.P, r.P);
+        add(l.Q, r.Q);
+    }
+    for (auto& sh : shapes) sh.batch = shapes.size();
+    std::vector<Path> paths(shapes.size());
+    switch_by_->choose_batch(shapes.data(), shapes.size(), paths.data());
+    double gpu = 0, all = 0;
+    for (size_t i = 0; i < shapes.size(); ++i) {
+        all += work[i];
+        if (paths[i] == Path::Gpu) gpu += work[i];
+    }
+    return gpu >= 0.5 * all;
 }
 
 }  // namespace pi

@@ -17,6 +17,7 @@ struct ProductShape {
     size_t an, bn;  // limbs
     size_t batch;   // products issued together with this one
     bool square;
+    bool resident = false;  // operands and result can stay in device memory (no host staging)
 };
 
 class Dispatcher {

@@ -47,11 +47,11 @@ void HipNttMultiplier::close_log() {
     log_ = nullptr;
 }
 
-void HipNttMultiplier::log_line(size_t an, size_t bn, size_t batch, bool square, bool gpu, double ms) {
+void HipNttMultiplier::log_line(size_t an, size_t bn, size_t batch, bool square, const char* path, double ms) {
     std::lock_guard<std::mutex> lock(log_mu_);
     if (!log_) return;
     std::fprintf(log_, "{\"an\":%zu,\"bn\":%zu,\"batch\":%zu,\"square\":%d,\"path\":\"%s\",\"ms\":%.6f}\n", an, bn,
-                 batch, square ? 1 : 0, gpu ? "gpu" : "cpu", ms);
+                 batch, square ? 1 : 0, path, ms);
 }
 
 bool HipNttMultiplier::log_pick(size_t k, size_t n) const {
@@ -66,7 +66,7 @@ void HipNttMultiplier::host_product(mpz_ptr result, mpz_srcptr a, mpz_srcptr b, 
     double ms = ms_since(t0);
     stats_.cpu_calls.fetch_add(1);
     stats_.cpu_ns.fetch_add((uint64_t)(ms * 1e6));
-    if (log) log_line(mpz_size(a), mpz_size(b), batch, a == b, false, ms);
+    if (log) log_line(mpz_size(a), mpz_size(b), batch, a == b, "cpu", ms);
 }
 
 // |result| = |a| * |b| (or |a|^2) through the C ABI, then the sign.  The ABI
@@ -111,7 +111,7 @@ void HipNttMultiplier::gpu_product(mpz_t result, const mpz_t a, const mpz_t b, b
         stats_.gpu_calls.fetch_add(1);
         stats_.gpu_batches.fetch_add(1);
         stats_.gpu_ns.fetch_add((uint64_t)(ms * 1e6));
-        log_line(an, bn, 1, square, true, ms);
+        log_line(an, bn, 1, square, "gpu", ms);
     }
     if (alias) {
         mpz_swap(result, tmp);
@@ -215,7 +215,7 @@ void HipNttMultiplier::multiply_batch(const MulJob* jobs, size_t count, unsigned
                     mpz_limbs_finish(j.result, static_cast<mp_size_t>(len));
                     if (mpz_sgn(j.a) * mpz_sgn(j.b) < 0) mpz_neg(j.result, j.result);
                     if (log_pick(k, n))
-                        log_line(an[k], bn[k], n, j.a == j.b, true, ms * work_of(an[k], bn[k]) / total_work);
+                        log_line(an[k], bn[k], n, j.a == j.b, "gpu", ms * work_of(an[k], bn[k]) / total_work);
                 }
                 stats_.gpu_calls.fetch_add(n);
                 stats_.gpu_batches.fetch_add(1);
@@ -224,6 +224,93 @@ void HipNttMultiplier::multiply_batch(const MulJob* jobs, size_t count, unsigned
         }
     }
     host_thread.join();
+}
+
+void HipNttMultiplier::multiply_batch_resident(const MulJob* jobs, size_t count) {
+    std::vector<size_t> live;
+    for (size_t i = 0; i < count; ++i) {
+        if (mpz_sgn(jobs[i].a) == 0 || mpz_sgn(jobs[i].b) == 0) mpz_set_ui(jobs[i].result, 0);
+        else live.push_back(i);
+    }
+    const size_t n = live.size();
+    if (n == 0) return;
+    size_t total = 0;
+    std::vector<size_t> an(n), bn(n);
+    for (size_t k = 0; k < n; ++k) {
+        const MulJob& j = jobs[live[k]];
+        an[k] = mpz_size(j.a);
+        bn[k] = mpz_size(j.b);
+        total += an[k] + (j.a == j.b ? 0 : bn[k]) + an[k] + bn[k];
+    }
+    uint64_t* base = nullptr;
+    bool ok = an.size() && total && gpu_ntt_dev_alloc(total, &base) == PINTT_OK;
+    std::vector<const uint64_t*> da(n), db(n), hs;
+    std::vector<uint64_t*> dout(n), dd;
+    std::vector<size_t> len;
+    if (ok) {
+        size_t off = 0;
+        for (size_t k = 0; k < n; ++k) {
+            const MulJob& j = jobs[live[k]];
+            uint64_t* pa = base + off;
+            off += an[k];
+            dd.push_back(pa);
+            hs.push_back(reinterpret_cast<const uint64_t*>(mpz_limbs_read(j.a)));
+            len.push_back(an[k]);
+            uint64_t* pb = pa;
+            if (j.a != j.b) {
+                pb = base + off;
+                off += bn[k];
+                dd.push_back(pb);
+                hs.push_back(reinterpret_cast<const uint64_t*>(mpz_limbs_read(j.b)));
+                len.push_back(bn[k]);
+            }
+            da[k] = pa;
+            db[k] = pb;
+            dout[k] = base + off;
+            off += an[k] + bn[k];
+        }
+        ok = gpu_ntt_upload(dd.size(), dd.data(), hs.data(), len.data()) == PINTT_OK;
+    }
+    double ms = 0;
+    if (ok) {
+        auto t0 = Clock::now();
+        ok = gpu_ntt_mul_batch_device(n, da.data(), an.data(), db.data(), bn.data(), dout.data(), nullptr) == PINTT_OK &&
+             gpu_ntt_synchronize() == PINTT_OK;
+        ms = ms_since(t0);
+    }
+    if (ok) {
+        std::vector<uint64_t*> hd(n);
+        std::vector<const uint64_t*> sd(n);
+        std::vector<size_t> ln(n);
+        for (size_t k = 0; k < n; ++k) {
+            hd[k] = reinterpret_cast<uint64_t*>(mpz_limbs_write(jobs[live[k]].result, an[k] + bn[k]));
+            sd[k] = dout[k];
+            ln[k] = an[k] + bn[k];
+        }
+        ok = gpu_ntt_download(n, hd.data(), sd.data(), ln.data()) == PINTT_OK;
+        for (size_t k = 0; k < n; ++k) {
+            const MulJob& j = jobs[live[k]];
+            size_t l = ok ? ln[k] : 0;
+            while (l && !hd[k][l - 1]) --l;
+            mpz_limbs_finish(j.result, (mp_size_t)l);
+            if (!ok) mpz_mul(j.result, j.a, j.b);
+            else if (mpz_sgn(j.a) * mpz_sgn(j.b) < 0) mpz_neg(j.result, j.result);
+        }
+    } else {
+        for (size_t k = 0; k < n; ++k) mpz_mul(jobs[live[k]].result, jobs[live[k]].a, jobs[live[k]].b);
+    }
+    if (base) (void)gpu_ntt_dev_free(base);
+    if (!ok) {
+        stats_.fallback_calls.fetch_add(n);
+        return;
+    }
+    double total_work = 0;
+    for (size_t k = 0; k < n; ++k) total_work += work_of(an[k], bn[k]);
+    for (size_t k = 0; k < n; ++k)
+        if (log_pick(k, n)) log_line(an[k], bn[k], n, jobs[live[k]].a == jobs[live[k]].b, "gpu_dev",
+                                     ms * work_of(an[k], bn[k]) / total_work);
+    stats_.gpu_calls.fetch_add(n);
+    stats_.gpu_batches.fetch_add(1);
 }
 
 void HipNttMultiplier::reset_stats() {

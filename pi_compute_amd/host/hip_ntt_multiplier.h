@@ -43,6 +43,11 @@ public:
     // Replace the path choice (not owned; nullptr restores the threshold rule).
     void set_dispatcher(const Dispatcher* d) { dispatcher_ = d ? d : &heuristic_; }
     const Dispatcher& dispatcher() const { return *dispatcher_; }
+    // Every product on the GPU with operands uploaded first and results
+    // downloaded after; only the device batch itself is timed and logged (path
+    // "gpu_dev"): the cost of products whose operands already live on the
+    // device (binary splitting's device-resident levels).
+    void multiply_batch_resident(const MulJob* jobs, size_t count);
     // When false, multiply_batch issues its device products one engine call
     // each (the per-call mode the batching is measured against).
     void set_batching(bool on) { batching_ = on; }
@@ -73,7 +78,7 @@ private:
     void gpu_product(mpz_t result, const mpz_t a, const mpz_t b, bool square);
     void host_product(mpz_ptr result, mpz_srcptr a, mpz_srcptr b, size_t batch, bool log);
     bool log_pick(size_t k, size_t n) const;
-    void log_line(size_t an, size_t bn, size_t batch, bool square, bool gpu, double ms);
+    void log_line(size_t an, size_t bn, size_t batch, bool square, const char* path, double ms);
 
     ThresholdDispatcher heuristic_;
     const Dispatcher* dispatcher_;

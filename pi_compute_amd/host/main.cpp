@@ -17,8 +17,9 @@
 //                             multiplier (default: newton with --gpu)
 //   --convert host|multiplier decimal conversion (default: multiplier with --gpu)
 //   --plans FILE              tile plans: loaded if FILE exists, saved after the run
-//   --device-levels [L]       with --gpu --tree levels: P, Q, R stay on the GPU from
-//                             the first level whose operands have >= L limbs (default 0)
+//   --device-levels [L|auto]  with --gpu --tree levels: P, Q, R stay on the GPU from
+//                             the first level whose operands have >= L limbs (default 0),
+//                             or (auto) whose products the dispatcher sends to the GPU
 #include <unistd.h>
 
 #include <chrono>
@@ -60,7 +61,7 @@ void usage() {
 int main(int argc, char** argv) {
     pi::PiConfig cfg;
     cfg.output_file = "pi_digits.txt";
-    bool gpu = false, per_call = false;
+    bool gpu = false, per_call = false, device_auto = false;
     size_t threshold = 10000;
     std::string dispatch = "threshold", model, log, final_step, convert, plans;
     for (int i = 1; i < argc; ++i) {
@@ -91,7 +92,11 @@ int main(int argc, char** argv) {
         else if (a == "--plans") plans = next();
         else if (a == "--device-levels") {
             cfg.device_levels = true;
-            if (i + 1 < argc && argv[i + 1][0] != '-') cfg.device_from_limbs = std::stoul(argv[++i]);
+            if (i + 1 < argc && argv[i + 1][0] != '-') {
+                std::string v = argv[++i];
+                if (v == "auto") device_auto = true;
+                else cfg.device_from_limbs = std::stoul(v);
+            }
         }
         else if (a == "--verbose" || a == "-v") cfg.verbose = true;
         else if (a == "--help" || a == "-h") { usage(); return 0; }
@@ -146,9 +151,12 @@ int main(int argc, char** argv) {
                 throw std::runtime_error("unknown dispatcher " + dispatch);
             }
             if (!log.empty() && !hip->open_log(log)) throw std::runtime_error("cannot open log " + log);
+            if (device_auto) cfg.device_switch = &hip->dispatcher();
             mode = std::string("HIP NTT, ") + hip->device_name() + ", " + hip->dispatcher().name() +
                    " dispatch, " + (cfg.level_batches ? (per_call ? "levels/per-call" : "levels/batched") : "recursive") +
-                   (cfg.device_levels ? ", device-resident levels" : "") + ", final " + final_step + ", conversion " + convert;
+                   (cfg.device_levels ? (device_auto ? ", device-resident levels (dispatcher switch)" : ", device-resident levels")
+                                      : "") +
+                   ", final " + final_step + ", conversion " + convert;
             m = std::move(h);
         } else {
             m = std::make_unique<pi::GmpMultiplier>();

@@ -23,15 +23,18 @@ std::array<float, 4> MlpModel::features(const ProductShape& s) {
             s.square ? 1.f : 0.f};
 }
 
+// Operand sizes must lie in the trained range; a batch larger than any
+// trained one is priced as the largest (a product's share of a batch stops
+// shrinking once the batch fills the device).
 bool MlpModel::in_range(const std::array<float, 4>& f) const {
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < 2; ++j)
         if (f[j] < lo[j] || f[j] > hi[j]) return false;
-    return true;
+    return f[2] >= lo[2];
 }
 
 std::array<float, 4> MlpModel::standardise(const std::array<float, 4>& f) const {
     std::array<float, 4> x;
-    for (int j = 0; j < 4; ++j) x[j] = (f[j] - mean[j]) / scale[j];
+    for (int j = 0; j < 4; ++j) x[j] = ((j == 2 ? std::min(f[j], hi[j]) : f[j]) - mean[j]) / scale[j];
     return x;
 }
 
@@ -124,9 +127,12 @@ static MlpModel load_or_throw(const std::string& path) {
 MlpDispatcher::MlpDispatcher(const std::string& model_path, const Dispatcher* fallback, unsigned cpu_threads)
     : MlpDispatcher(load_or_throw(model_path), fallback, cpu_threads) {}
 
+// Predicted GPU output for the shape: resident operands use output 2 when present.
+static inline int gpu_output(const MlpModel& m, const ProductShape& s) { return (s.resident && m.outputs > 2) ? 2 : 1; }
+
 Path MlpDispatcher::decide(const ProductShape& s, const float* y) const {
     const float cpu = y[0] - (s.batch > 1 ? log_threads_ : 0.f);
-    return y[1] < cpu ? Path::Gpu : Path::Cpu;
+    return y[gpu_output(model_, s)] < cpu ? Path::Gpu : Path::Cpu;
 }
 
 Path MlpDispatcher::choose(const ProductShape& s) const {
@@ -174,9 +180,9 @@ void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) con
         }
         out[i] = fallback_->choose(s[i]);  // outside the trained range: the rule decides, cost unknown
     }
-    std::sort(free_idx.begin(), free_idx.end(), [&](size_t a, size_t b) {
-        return (y[a * K] - y[a * K + 1]) > (y[b * K] - y[b * K + 1]);
-    });
+    auto g = [&](size_t i) { return y[i * K + gpu_output(model_, s[i])]; };
+    std::sort(free_idx.begin(), free_idx.end(),
+              [&](size_t a, size_t b) { return (y[a * K] - g(a)) > (y[b * K] - g(b)); });
     // A host product runs on one thread: the host share takes at least its
     // longest product (suffix maxima of the ranking).
     const size_t nf = free_idx.size();
@@ -188,7 +194,7 @@ void MlpDispatcher::choose_batch(const ProductShape* s, size_t n, Path* out) con
     size_t best_k = 0;
     for (size_t k = 0; k < nf; ++k) {
         const size_t i = free_idx[k];
-        dev += std::exp((double)y[i * K + 1]);
+        dev += std::exp((double)g(i));
         host -= std::exp((double)y[i * K]);
         const double t = std::max({dev, host / threads, longest[k + 1]});
         if (t < best) {

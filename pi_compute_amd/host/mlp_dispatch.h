@@ -9,7 +9,10 @@
 // Inputs (standardised with the model's mean/scale):
 //   log2(larger operand limbs), log2(smaller operand limbs), log2(batch), square
 // Outputs: predicted ln(milliseconds) per product, output 0 = host GMP,
-// output 1 = GPU engine (a device batch's time shared out by transform work).
+// output 1 = GPU engine with host operands (a device batch's time, staging
+// included, shared out by transform work), output 2 (models with three) =
+// GPU engine on device-resident operands (no staging); a resident shape is
+// priced with output 2 when the model has it.
 // A single product goes to the path with the lower prediction.  A batch is
 // split so that its device share and its host share (run on `cpu_threads`
 // threads at the same time) finish together as early as the predictions allow.

@@ -22,6 +22,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     if (cfg.verbose) std::cout << "  terms: " << terms << std::endl;
 
     BinarySplitting bs(mult_, cfg.num_threads);
+    bs.set_switch_dispatcher(cfg.device_switch);
     BSResult r;
     if (cfg.level_batches && cfg.device_levels) {
         try {

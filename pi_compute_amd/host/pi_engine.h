@@ -19,6 +19,7 @@
 #include <string>
 
 #include "binary_splitting.h"
+#include "dispatch.h"
 #include "multiplier.h"
 
 namespace pi {
@@ -31,6 +32,7 @@ struct PiConfig {
     bool level_batches = true;  // level-synchronous tree with one batch per level
     bool device_levels = false;  // upper levels with P, Q, R resident on the GPU (level_batches only)
     size_t device_from_limbs = 0;  // first device level: smallest operand at least this many limbs
+    const Dispatcher* device_switch = nullptr;  // if set, it picks the first device level instead
     enum class FinalStep { Mpf, Newton } final_step = FinalStep::Mpf;
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
 };

@@ -4,8 +4,10 @@
 //   pi_train_mlp --out MODEL --log FILE [--log FILE ...] [--steps 4000]
 //                [--lr 0.003] [--seed 1] [--holdout 0.2] [--threads T]
 //
-// Lines: {"an":..,"bn":..,"batch":..,"square":0|1,"path":"cpu"|"gpu","ms":..}
-// Target: ln(ms) of the path the line was measured on (output 0 cpu, 1 gpu).
+// Lines: {"an":..,"bn":..,"batch":..,"square":0|1,"path":"cpu"|"gpu"|"gpu_dev","ms":..}
+// Target: ln(ms) of the path the line was measured on (output 0 cpu, 1 gpu,
+// 2 gpu with device-resident operands; the model has 3 outputs when the log
+// holds gpu_dev lines, else 2).
 // Prints one JSON summary: sample counts, train/holdout loss, and on holdout
 // shapes measured on both paths the summed time of the model's choices, of the
 // threshold rule's choices and of the best choice.
@@ -38,14 +40,17 @@ struct Sample {
 bool parse(const std::string& line, Sample& s) {
     unsigned long long an, bn, batch;
     int sq;
-    char path[8];
+    char path[16];
     double ms;
-    if (std::sscanf(line.c_str(), "{\"an\":%llu,\"bn\":%llu,\"batch\":%llu,\"square\":%d,\"path\":\"%3[a-z]\",\"ms\":%lf}",
+    if (std::sscanf(line.c_str(), "{\"an\":%llu,\"bn\":%llu,\"batch\":%llu,\"square\":%d,\"path\":\"%15[a-z_]\",\"ms\":%lf}",
                     &an, &bn, &batch, &sq, path, &ms) != 6)
         return false;
     if (an == 0 || bn == 0 || !(ms > 0)) return false;
     s.shape = ProductShape{(size_t)an, (size_t)bn, (size_t)std::max(1ull, batch), sq != 0};
-    s.path = std::strcmp(path, "gpu") == 0 ? 1 : 0;
+    if (std::strcmp(path, "cpu") == 0) s.path = 0;
+    else if (std::strcmp(path, "gpu") == 0) s.path = 1;
+    else if (std::strcmp(path, "gpu_dev") == 0) s.path = 2;
+    else return false;
     s.ms = ms;
     return true;
 }
@@ -107,7 +112,9 @@ int main(int argc, char** argv) {
     }
 
     MlpModel model;
-    model.outputs = 2;
+    int K = 2;
+    for (const auto& smp : all) K = std::max(K, smp.path + 1);
+    model.outputs = K;
     const size_t n = train.size();
     std::vector<std::array<float, 4>> feats(n);
     for (size_t i = 0; i < n; ++i) feats[i] = MlpModel::features(train[i].shape);
@@ -128,8 +135,8 @@ int main(int argc, char** argv) {
     }
     std::vector<float> x(n * 4), t(n);
     std::vector<int> p(n);
-    double tmean[2] = {0, 0};
-    size_t tcount[2] = {0, 0};
+    double tmean[3] = {0, 0, 0};
+    size_t tcount[3] = {0, 0, 0};
     for (size_t i = 0; i < n; ++i) {
         auto z = model.standardise(feats[i]);
         std::copy(z.begin(), z.end(), &x[i * 4]);
@@ -140,37 +147,37 @@ int main(int argc, char** argv) {
     }
     // He-uniform initial weights from the seeded stream, output biases at each
     // path's mean target.
-    model.params.assign(gpu_mlp_param_count(2), 0.f);
+    model.params.assign(gpu_mlp_param_count(K), 0.f);
     const int H = PINTT_MLP_HIDDEN;
     auto fill = [&](size_t off, size_t count, int fan_in) {
         float lim = std::sqrt(6.f / fan_in);
         for (size_t i = 0; i < count; ++i) model.params[off + i] = (float)((2 * g.uniform() - 1) * lim);
     };
-    const size_t oW1 = 0, oW2 = H * 4 + H, oW3 = oW2 + H * H + H, oB3 = oW3 + 2 * H;
+    const size_t oW1 = 0, oW2 = H * 4 + H, oW3 = oW2 + H * H + H, oB3 = oW3 + (size_t)K * H;
     fill(oW1, H * 4, 4);
     fill(oW2, H * H, H);
-    fill(oW3, 2 * H, H);
-    for (int k = 0; k < 2; ++k) model.params[oB3 + k] = tcount[k] ? (float)(tmean[k] / tcount[k]) : 0.f;
+    fill(oW3, (size_t)K * H, H);
+    for (int k = 0; k < K; ++k) model.params[oB3 + k] = tcount[k] ? (float)(tmean[k] / tcount[k]) : 0.f;
 
     std::vector<float> hist(steps);
-    int rc = gpu_mlp_train(model.params.data(), 2, x.data(), p.data(), t.data(), n, steps, lr, hist.data());
+    int rc = gpu_mlp_train(model.params.data(), K, x.data(), p.data(), t.data(), n, steps, lr, hist.data());
     if (rc != PINTT_OK) {
         std::cerr << "gpu_mlp_train failed: " << gpu_mlp_last_error() << std::endl;
         return 1;
     }
     float final_loss = 0;
     std::vector<float> grad(model.params.size());
-    gpu_mlp_gradient(model.params.data(), 2, x.data(), p.data(), t.data(), n, grad.data(), &final_loss);
+    gpu_mlp_gradient(model.params.data(), K, x.data(), p.data(), t.data(), n, grad.data(), &final_loss);
 
     // Holdout: loss, and decision quality on shapes measured on both paths.
     double test_loss = 0;
     ThresholdDispatcher heur(10000);
     MlpDispatcher mlp(model, &heur, threads);
-    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<double, 2>> pairs;
-    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<int, 2>> pcount;
+    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<double, 3>> pairs;
+    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<int, 3>> pcount;
     for (const auto& s : test) {
         auto z = model.standardise(MlpModel::features(s.shape));
-        float y[2];
+        float y[PINTT_MLP_MAX_OUTPUTS];
         model.forward(z.data(), y);
         double e = y[s.path] - std::log(s.ms);
         test_loss += 0.5 * e * e;
@@ -181,29 +188,35 @@ int main(int argc, char** argv) {
         pc[s.path]++;
     }
     test_loss /= std::max<size_t>(1, test.size());
-    double t_mlp = 0, t_heur = 0, t_best = 0;
-    size_t npairs = 0, agree = 0;
+    // Host staging (path 1) and device-resident operands (path 2) are judged
+    // separately, each on the shapes measured on the host and on that path.
+    double t_mlp[3] = {0, 0, 0}, t_heur[3] = {0, 0, 0}, t_best[3] = {0, 0, 0};
+    size_t npairs[3] = {0, 0, 0}, agree[3] = {0, 0, 0};
     for (auto& kv : pairs) {
         auto& pc = pcount[kv.first];
-        if (!pc[0] || !pc[1]) continue;
-        ProductShape sh{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first)};
-        // Host time per product as seen by a batch: divided by the threads sharing it.
-        double cost[2] = {kv.second[0] / pc[0] / (sh.batch > 1 ? threads : 1), kv.second[1] / pc[1]};
-        int cm = mlp.choose(sh) == Path::Gpu, ch = heur.choose(sh) == Path::Gpu;
-        t_mlp += cost[cm];
-        t_heur += cost[ch];
-        t_best += std::min(cost[0], cost[1]);
-        agree += cm == ch;
-        ++npairs;
+        for (int gp = 1; gp <= 2; ++gp) {
+            if (!pc[0] || !pc[gp]) continue;
+            ProductShape sh{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first),
+                            gp == 2};
+            // Host time per product as seen by a batch: divided by the threads sharing it.
+            double cost[2] = {kv.second[0] / pc[0] / (sh.batch > 1 ? threads : 1), kv.second[gp] / pc[gp]};
+            int cm = mlp.choose(sh) == Path::Gpu, ch = heur.choose(sh) == Path::Gpu;
+            t_mlp[gp] += cost[cm];
+            t_heur[gp] += cost[ch];
+            t_best[gp] += std::min(cost[0], cost[1]);
+            agree[gp] += cm == ch;
+            ++npairs[gp];
+        }
     }
     if (!model.save(out)) {
         std::cerr << "cannot write " << out << std::endl;
         return 1;
     }
-    std::printf("{\"samples\":%zu,\"train\":%zu,\"holdout\":%zu,\"steps\":%d,\"loss_first\":%.5f,"
-                "\"loss_last\":%.5f,\"holdout_loss\":%.5f,\"holdout_pairs\":%zu,\"agree_with_threshold\":%zu,"
-                "\"ms_mlp\":%.3f,\"ms_threshold\":%.3f,\"ms_best\":%.3f}\n",
-                all.size(), train.size(), test.size(), steps, steps ? hist[0] : 0.f, final_loss, test_loss, npairs,
-                agree, t_mlp, t_heur, t_best);
+    std::printf("{\"samples\":%zu,\"train\":%zu,\"holdout\":%zu,\"outputs\":%d,\"steps\":%d,\"loss_first\":%.5f,"
+                "\"loss_last\":%.5f,\"holdout_loss\":%.5f,"
+                "\"staged\":{\"pairs\":%zu,\"agree_with_threshold\":%zu,\"ms_mlp\":%.3f,\"ms_threshold\":%.3f,\"ms_best\":%.3f},"
+                "\"resident\":{\"pairs\":%zu,\"agree_with_threshold\":%zu,\"ms_mlp\":%.3f,\"ms_threshold\":%.3f,\"ms_best\":%.3f}}\n",
+                all.size(), train.size(), test.size(), K, steps, steps ? hist[0] : 0.f, final_loss, test_loss,
+                npairs[1], agree[1], t_mlp[1], t_heur[1], t_best[1], npairs[2], agree[2], t_mlp[2], t_heur[2], t_best[2]);
     return 0;
 }
