@@ -321,18 +321,22 @@ def _pi_run(args, out: Path) -> dict:
 
 def pi_leg(digits: int) -> dict:
     """Chudnovsky pi at `digits`: the GMP-only run (the reference's algorithm,
-    our host build) against GPU runs with level batches + threshold dispatch,
-    level batches + learned dispatch, and per-call dispatch; every digit file
-    must equal the GMP-only file byte for byte."""
+    our host build) against GPU runs: per-call threshold dispatch; level
+    batches with threshold or learned dispatch; and device-resident upper
+    levels whose starting level the threshold rule or the learned model picks.
+    Every digit file must equal the GMP-only file byte for byte."""
     exe = str(ROOT / "pi_compute_amd" / "bin" / "pi_compute_hip")
     model = ROOT / "pi_compute_amd" / "data" / "cost_model.txt"
     runs = {
         "gmp_only": [exe, "--digits", str(digits)],
-        "gpu_levels_threshold": [exe, "--digits", str(digits), "--gpu"],
         "gpu_per_call_threshold": [exe, "--digits", str(digits), "--gpu", "--per-call"],
+        "gpu_levels_threshold": [exe, "--digits", str(digits), "--gpu"],
+        "gpu_device_levels_threshold": [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto"],
     }
     if model.exists():
-        runs["gpu_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", "--dispatch", "mlp", "--model", str(model)]
+        mlp = ["--dispatch", "mlp", "--model", str(model)]
+        runs["gpu_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", *mlp]
+        runs["gpu_device_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto", *mlp]
     out = {"digits": digits, "runs": {}}
     with tempfile.TemporaryDirectory(prefix="pi_bench_") as d:
         ref_bytes = None

@@ -54,9 +54,11 @@ def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
     assert got == gmp_digits[digits]
 
 
-def test_pi_gpu_learned_dispatch_matches_gmp_run(tmp_path, gmp_digits):
+@pytest.mark.parametrize("device", [False, True], ids=["host-levels", "device-levels-auto"])
+def test_pi_gpu_learned_dispatch_matches_gmp_run(tmp_path, gmp_digits, device):
     if not MODEL.exists():
         pytest.skip("no trained cost model in the tree")
+    extra = ["--device-levels", "auto"] if device else []
     got = _run([str(BIN / "pi_compute_hip"), "--digits", "1000000", "--gpu", "--dispatch", "mlp", "--model",
-                str(MODEL)], tmp_path / "mlp.txt")
+                str(MODEL), *extra], tmp_path / "mlp.txt")
     assert got == gmp_digits[1000000]
