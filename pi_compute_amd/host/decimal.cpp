@@ -2,6 +2,7 @@
 #include "decimal.h"
 
 #include <algorithm>
+#include <chrono>
 
 #include <cstring>
 #include <future>
@@ -108,7 +109,16 @@ struct Node {
 
 }  // namespace
 
-std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t leaf_digits) {
+std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t leaf_digits, DecimalStats* stats) {
+    using Clock = std::chrono::steady_clock;
+    DecimalStats st;
+    auto lap = [](Clock::time_point& t) {
+        const auto now = Clock::now();
+        const double s = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return s;
+    };
+    auto t = Clock::now();
     if (mpz_sgn(n) == 0) return "0";
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     leaf_digits = std::max<size_t>(leaf_digits, kLeafDigits);
@@ -128,6 +138,7 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
             m.square(&pw[j], &pw[j - 1]);
     }
 
+    st.powers_s += lap(t);
     std::vector<std::unique_ptr<Node>> nodes;
     nodes.emplace_back(new Node);
     mpz_abs(nodes[0]->x, n);
@@ -144,7 +155,9 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
         if (split.empty()) continue;
         mpz_srcptr P = &pw[j];
         const size_t np = mpz_sizeinbase(P, 2);
+        lap(t);
         newton_reciprocal(inv, P, np, m);  // ~ 2^(2 np) / P, within 4
+        st.reciprocals_s += lap(t);
         const size_t sx = np > 64 ? np - 64 : 0;
         const size_t ns = split.size();
         std::vector<__mpz_struct> hi(ns), q(ns), qp(ns);
@@ -162,6 +175,7 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
             jobs[s] = MulJob{&qp[s], &q[s], P};
         }
         m.multiply_batch(jobs.data(), ns, threads);
+        st.products_s += lap(t);
         // Corrections, then the node becomes its high part and a low node joins.
         std::vector<std::unique_ptr<Node>> lows(ns);
         struct Ctx {
@@ -197,6 +211,7 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
             },
             &ctx);
         for (auto& lo : lows) nodes.push_back(std::move(lo));
+        st.corrections_s += lap(t);
         for (size_t s = 0; s < ns; ++s) {
             mpz_clear(&hi[s]);
             mpz_clear(&q[s]);
@@ -229,6 +244,8 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
         &lctx, cost.data());
     size_t z = out.find_first_not_of('0');
     if (z != 0) out.erase(0, z);
+    st.leaves_s += lap(t);
+    if (stats) *stats = st;
     return out;
 }
 

@@ -25,6 +25,10 @@ std::string to_decimal(mpz_srcptr n, unsigned threads = 0);
 // x - q * 10^(2^j), both products of a level as one multiply_batch, then at
 // most a few +-1 corrections on the host.  Below leaf_digits the host
 // conversion above takes over, one subtree per thread.
-std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads = 0, size_t leaf_digits = 1 << 15);
+struct DecimalStats {
+    double powers_s = 0, reciprocals_s = 0, products_s = 0, corrections_s = 0, leaves_s = 0;
+};
+std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads = 0, size_t leaf_digits = 1 << 15,
+                       DecimalStats* stats = nullptr);
 
 }  // namespace pi

@@ -17,6 +17,7 @@
 //                             multiplier (default: newton with --gpu)
 //   --convert host|multiplier decimal conversion (default: multiplier with --gpu)
 //   --plans FILE              tile plans: loaded if FILE exists, saved after the run
+//   --convert-leaf D          multiplier conversion: host conversion below D digits (32768)
 //   --device-levels [L|auto]  with --gpu --tree levels: P, Q, R stay on the GPU from
 //                             the first level whose operands have >= L limbs (default 0),
 //                             or (auto) whose products the dispatcher sends to the GPU
@@ -90,6 +91,7 @@ int main(int argc, char** argv) {
         else if (a == "--final") final_step = next();
         else if (a == "--convert") convert = next();
         else if (a == "--plans") plans = next();
+        else if (a == "--convert-leaf") cfg.conversion_leaf_digits = std::stoul(next());
         else if (a == "--device-levels") {
             cfg.device_levels = true;
             if (i + 1 < argc && argv[i + 1][0] != '-') {

@@ -51,8 +51,9 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
     mpz_t pi_int;
     mpz_init(pi_int);
+    NewtonStats ns;
     if (cfg.final_step == PiConfig::FinalStep::Newton && mpz_sgn(r.R) > 0) {
-        pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, bits + 64, mult_);
+        pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, bits + 64, mult_, &ns);
         mpz_realloc2(r.Q, 0);
         mpz_realloc2(r.R, 0);
     } else {
@@ -85,8 +86,10 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const double fin_s = since(t1);
 
     const auto t2 = Clock::now();
-    std::string d = cfg.conversion == PiConfig::Conversion::Multiplier ? to_decimal(pi_int, mult_, cfg.num_threads)
-                                                                      : to_decimal(pi_int, cfg.num_threads);
+    DecimalStats ds;
+    std::string d = cfg.conversion == PiConfig::Conversion::Multiplier
+                        ? to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds)
+                        : to_decimal(pi_int, cfg.num_threads);
     mpz_clear(pi_int);
     std::string out;
     out.reserve(cfg.digits + 2);
@@ -102,12 +105,29 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
         std::fclose(f);
         if (w != out.size()) throw std::runtime_error("short write: " + cfg.output_file);
     }
-    if (cfg.verbose)
+    if (cfg.verbose) {
         std::cout << "  final step: " << fin_s << " s, decimal conversion: " << conv_s << " s" << std::endl;
+        if (cfg.final_step == PiConfig::FinalStep::Newton)
+            std::cout << "    final step: inverse square root " << ns.inv_sqrt_s << " s, 10^D " << ns.pow10_s
+                      << " s, reciprocal " << ns.reciprocal_s << " s, products " << ns.products_s << " s" << std::endl;
+        if (cfg.conversion == PiConfig::Conversion::Multiplier)
+            std::cout << "    conversion: powers " << ds.powers_s << " s, reciprocals " << ds.reciprocals_s
+                      << " s, quotient products " << ds.products_s << " s, corrections " << ds.corrections_s
+                      << " s, leaves " << ds.leaves_s << " s" << std::endl;
+    }
     return PiResult{std::move(out), since(t0), terms, bs_s, fin_s, conv_s};
 }
 
-void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t w, Multiplier& m) {
+void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t w, Multiplier& m,
+                      NewtonStats* stats) {
+    NewtonStats st;
+    auto clock = Clock::now();
+    auto lap = [&clock]() {
+        const auto now = Clock::now();
+        const double s = std::chrono::duration<double>(now - clock).count();
+        clock = now;
+        return s;
+    };
     // 426880 * sqrt(10005) * Q * 10^D / R
     //   = 10005 * (426880 Q) * (2^w / sqrt(10005)) * 10^D * (2^(nr+w) / Rt) / 2^(...)
     // with every operand cut to its top w bits and the exponents of 2 tracked.
@@ -117,14 +137,19 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
     mpz_mul_ui(a, Q, 426880);
     ex += (long)truncate_top(a, a, w);
     newton_inv_sqrt_ui(y, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
+    st.inv_sqrt_s = lap();
     m.multiply(t, a, y);
     ex += (long)truncate_top(t, t, w) - (long)w;
+    st.products_s += lap();
     pow_ui(a, 10, digits, m);
+    st.pow10_s = lap();
     m.multiply(y, t, a);
     ex += (long)truncate_top(y, y, w);
     const size_t sr = truncate_top(t, R, w);
     const size_t nr = mpz_sizeinbase(t, 2);
+    st.products_s += lap();
     newton_reciprocal(inv, t, w, m);  // 1/R ~ inv / 2^(nr + w + sr)
+    st.reciprocal_s = lap();
     ex -= (long)(nr + w + sr);
     m.multiply(a, y, inv);
     mpz_mul_ui(a, a, 10005);
@@ -133,6 +158,8 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
     else
         mpz_fdiv_q_2exp(out, a, (mp_bitcnt_t)(-ex));
     mpz_clears(a, y, t, inv, NULL);
+    st.products_s += lap();
+    if (stats) *stats = st;
 }
 
 std::string PiEngine::compute_digits(size_t digits) {

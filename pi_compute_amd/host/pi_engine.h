@@ -35,6 +35,7 @@ struct PiConfig {
     const Dispatcher* device_switch = nullptr;  // if set, it picks the first device level instead
     enum class FinalStep { Mpf, Newton } final_step = FinalStep::Mpf;
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
+    size_t conversion_leaf_digits = 1 << 15;  // Conversion::Multiplier: host conversion below this width
 };
 
 struct PiResult {
@@ -46,7 +47,11 @@ struct PiResult {
 
 // floor(426880 * sqrt(10005) * Q * 10^digits / R) with `bits` of working
 // precision, products through m (R > 0).  Used by FinalStep::Newton.
-void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t bits, Multiplier& m);
+struct NewtonStats {
+    double inv_sqrt_s = 0, pow10_s = 0, reciprocal_s = 0, products_s = 0;
+};
+void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t bits, Multiplier& m,
+                      NewtonStats* stats = nullptr);
 
 class PiEngine {
 public:
