@@ -988,6 +988,17 @@ int gpu_ntt_graph_launch(void* graph, void* stream) {
     return PINTT_OK;
 }
 
+// End of the device allocation holding p (p itself when unknown: no merging).
+static const uint64_t* allocation_end(const uint64_t* p) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<uint64_t*>(p)) != hipSuccess || !base) {
+        (void)hipGetLastError();
+        return p;
+    }
+    return reinterpret_cast<const uint64_t*>(static_cast<const char*>(base) + size);
+}
+
 int gpu_ntt_dev_alloc(size_t limbs, uint64_t** d_ptr) {
     if (!d_ptr || limbs == 0) return fail(PINTT_ERR_ARGS, "bad arguments");
     std::lock_guard<std::mutex> lock(g_mu);
@@ -1029,11 +1040,12 @@ int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* 
         off += limbs[i];
     }
     parallel_copy(gather, total);
-    // One DMA per run of destinations that follow each other in device memory.
+    // One DMA per run of destinations that follow each other inside one allocation.
     off = 0;
     for (size_t i = 0; i < count;) {
+        const uint64_t* end = allocation_end(d_dst[i]);
         size_t j = i + 1, run = limbs[i];
-        while (j < count && d_dst[j] == d_dst[j - 1] + limbs[j - 1]) run += limbs[j++];
+        while (j < count && d_dst[j] == d_dst[j - 1] + limbs[j - 1] && d_dst[j] + limbs[j] <= end) run += limbs[j++];
         if (run)
             PINTT_HIP(hipMemcpyAsync(d_dst[i], g.pin_in + off, run * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
         off += run;
@@ -1056,8 +1068,9 @@ int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const
     scatter.reserve(count);
     size_t off = 0;
     for (size_t i = 0; i < count;) {
+        const uint64_t* end = allocation_end(d_src[i]);
         size_t j = i + 1, run = limbs[i];
-        while (j < count && d_src[j] == d_src[j - 1] + limbs[j - 1]) run += limbs[j++];
+        while (j < count && d_src[j] == d_src[j - 1] + limbs[j - 1] && d_src[j] + limbs[j] <= end) run += limbs[j++];
         if (run)
             PINTT_HIP(hipMemcpyAsync(g.pin_out + off, d_src[i], run * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
         for (size_t k = i; k < j; ++k) {
