@@ -20,9 +20,13 @@
 #include <stdlib.h>
 #include <string.h>
 
-static const uint32_t P[3] = {2013265921u, 1811939329u, 2113929217u};
-static const uint32_t G[3] = {31u, 13u, 5u};
-static const int TWO_ADIC[3] = {27, 26, 25};
+/* Prime index i: set i / 3, prime i % 3.  Set 0 (wide): transforms up to 2^25;
+ * set 1 (narrow, every p < 2^30): transforms up to 2^22. */
+static const uint32_t P[6] = {2013265921u, 1811939329u, 2113929217u, 998244353u, 985661441u, 943718401u};
+static const uint32_t G[6] = {31u, 13u, 5u, 3u, 3u, 7u};
+static const int TWO_ADIC[6] = {27, 26, 25, 23, 22, 22};
+
+int orc_set_for(int log2n) { return log2n <= ORC_NARROW_MAX_LOG2N ? 1 : 0; }
 
 uint32_t orc_prime(int i) { return P[i]; }
 uint32_t orc_generator(int i) { return G[i]; }
@@ -131,8 +135,8 @@ void orc_ntt_gs_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi) {
 }
 
 /* Garner: x = y0 + p0*y1 + p0*p1*y2 with y_i in [0, p_i). */
-void orc_crt3(const uint32_t r[3], uint32_t out[3]) {
-    uint32_t p0 = P[0], p1 = P[1], p2 = P[2];
+void orc_crt3_set(int set, const uint32_t r[3], uint32_t out[3]) {
+    uint32_t p0 = P[3 * set], p1 = P[3 * set + 1], p2 = P[3 * set + 2];
     uint32_t y0 = r[0] % p0;
     uint32_t inv_p0_m1 = orc_invmod(p0 % p1, p1);
     uint32_t inv_p0_m2 = orc_invmod(p0 % p2, p2);
@@ -145,6 +149,77 @@ void orc_crt3(const uint32_t r[3], uint32_t out[3]) {
     out[0] = (uint32_t)x;
     out[1] = (uint32_t)(x >> 32);
     out[2] = (uint32_t)(x >> 64);
+}
+
+void orc_crt3(const uint32_t r[3], uint32_t out[3]) { orc_crt3_set(0, r, out); }
+
+/* ---- Lazy (Harvey) butterflies of the narrow set ------------------------- */
+
+uint32_t orc_shoup_quotient(uint32_t w, uint32_t p) { return (uint32_t)(((uint64_t)w << 32) / p); }
+
+static uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+uint32_t orc_narrow_load(uint32_t c, uint32_t p) { return min_u32(c, c - 4u * p); }
+
+void orc_narrow_ct(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p) {
+    const uint32_t p2 = 2u * p, wq = orc_shoup_quotient(w, p);
+    const uint32_t x = min_u32(*a, *a - p2);
+    const uint32_t q = (uint32_t)(((uint64_t)*b * wq) >> 32);
+    const uint32_t tn = q * p - *b * w; /* -(w b - q p) mod 2^32 */
+    *a = x - tn;
+    *b = x + tn + p2;
+}
+
+void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p) {
+    const uint32_t p2 = 2u * p, wq = orc_shoup_quotient(winv, p);
+    const uint32_t u = *a, v = *b;
+    const uint32_t s = u + v;
+    const uint32_t d = u - v + p2;
+    const uint32_t q = (uint32_t)(((uint64_t)d * wq) >> 32);
+    *a = min_u32(s, s - p2);
+    *b = d * winv - q * p;
+}
+
+uint32_t orc_narrow_pointwise(uint32_t x, uint32_t y, uint32_t p) {
+    const uint32_t p2 = 2u * p;
+    uint32_t pinv = p; /* -p^-1 mod 2^32 by Newton's iteration */
+    for (int i = 0; i < 5; ++i) pinv *= 2u - p * pinv;
+    pinv = 0u - pinv;
+    x = min_u32(x, x - p2);
+    y = min_u32(y, y - p2);
+    const uint64_t t = (uint64_t)x * y;
+    const uint32_t m = (uint32_t)t * pinv;
+    return (uint32_t)((t + (uint64_t)m * p) >> 32);
+}
+
+void orc_narrow_ct_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi) {
+    uint32_t p = P[prime];
+    size_t n = (size_t)1 << log2n;
+    uint32_t* z = (uint32_t*)malloc(n * sizeof(uint32_t));
+    orc_zeta_table(prime, log2n, z, NULL);
+    for (int s = s_lo; s < s_hi; ++s) {
+        size_t half = n >> (s + 1);
+        for (size_t b = 0; b < ((size_t)1 << s); ++b)
+            for (size_t j = 0; j < half; ++j)
+                orc_narrow_ct(&x[b * 2 * half + j], &x[b * 2 * half + j + half], z[((size_t)1 << s) + b], p);
+    }
+    free(z);
+}
+
+void orc_narrow_gs_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi) {
+    uint32_t p = P[prime];
+    size_t n = (size_t)1 << log2n;
+    uint32_t* z = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* zi = (uint32_t*)malloc(n * sizeof(uint32_t));
+    orc_zeta_table(prime, log2n, z, zi);
+    for (int s = s_hi - 1; s >= s_lo; --s) {
+        size_t half = n >> (s + 1);
+        for (size_t b = 0; b < ((size_t)1 << s); ++b)
+            for (size_t j = 0; j < half; ++j)
+                orc_narrow_gs(&x[b * 2 * half + j], &x[b * 2 * half + j + half], zi[((size_t)1 << s) + b], p);
+    }
+    free(z);
+    free(zi);
 }
 
 size_t orc_coeff_count(const uint64_t* a, size_t n) {
@@ -169,17 +244,18 @@ void orc_conv_residues(const uint64_t* a, size_t an, const uint64_t* b, size_t b
                        int log2n, uint32_t* res) {
     size_t n = (size_t)1 << log2n;
     uint32_t* fb = (uint32_t*)malloc(n * sizeof(uint32_t));
+    const int base = 3 * orc_set_for(log2n);
     for (int i = 0; i < 3; ++i) {
-        uint32_t p = P[i];
+        uint32_t p = P[base + i];
         uint32_t* fa = res + (size_t)i * n;
         for (size_t k = 0; k < n; ++k) {
             fa[k] = coeff(a, an, k) % p;
             fb[k] = coeff(b, bn, k) % p;
         }
-        orc_ntt_ct_layers(fa, log2n, i, 0, log2n);
-        orc_ntt_ct_layers(fb, log2n, i, 0, log2n);
+        orc_ntt_ct_layers(fa, log2n, base + i, 0, log2n);
+        orc_ntt_ct_layers(fb, log2n, base + i, 0, log2n);
         for (size_t k = 0; k < n; ++k) fa[k] = orc_mulmod(fa[k], fb[k], p);
-        orc_ntt_gs_layers(fa, log2n, i, 0, log2n);
+        orc_ntt_gs_layers(fa, log2n, base + i, 0, log2n);
         uint32_t ninv = orc_invmod((uint32_t)(n % p), p);
         for (size_t k = 0; k < n; ++k) fa[k] = orc_mulmod(fa[k], ninv, p);
     }
@@ -203,7 +279,7 @@ int orc_ntt_mul(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint
          * accumulator with the carry shifted down each step does exactly that. */
         if (j < n) {
             uint32_t r[3] = {res[j], res[n + j], res[2 * n + j]}, x[3];
-            orc_crt3(r, x);
+            orc_crt3_set(orc_set_for(log2n), r, x);
             acc += ((unsigned __int128)x[2] << 64) | ((unsigned __int128)x[1] << 32) | x[0];
         }
         w[j] = (uint32_t)acc;

@@ -11,10 +11,13 @@
  * The reference uses base-2^15 digits, 64-bit '%' arithmetic, primes of 2-adic
  * order 23/23/24 and a host CRT (src/gpu/int_ntt.cu:226-245,352-377).  The
  * restatement follows the product the engine computes instead: 32-bit
- * coefficients, primes 15*2^27+1, 27*2^26+1, 63*2^25+1 (transform length up
- * to 2^25), a Cooley-Tukey forward transform (natural order in, bit-reversed
- * out) and a Gentleman-Sande inverse, Garner CRT to an exact 96-bit value and
- * a sequential carry.  The whole product is pinned against GMP mpz_mul
+ * coefficients; two prime sets, the wide set 15*2^27+1, 27*2^26+1, 63*2^25+1
+ * for transforms of 2^23 .. 2^25 points and the narrow set 119*2^23+1,
+ * 235*2^22+1, 225*2^22+1 (all below 2^30) up to 2^22 points; a Cooley-Tukey
+ * forward transform (natural order in, bit-reversed out) and a Gentleman-Sande
+ * inverse, Garner CRT to an exact 96-bit value and a sequential carry.  The
+ * narrow set's lazy butterflies (values in [0, 4p) forward, [0, 2p) inverse)
+ * are restated separately, for range and congruence checks.  The whole product is pinned against GMP mpz_mul
  * (GmpMultiplier, src/arithmetic/gmp_multiplier.cpp:10-19).
  */
 #ifndef PINTT_ORACLE_H
@@ -29,8 +32,10 @@ extern "C" {
 
 #define ORC_NPRIMES 3
 #define ORC_MAX_LOG2N 25
+#define ORC_NARROW_MAX_LOG2N 22
 
-/* p_i, generator g_i of Z_p^*, and the 2-adic order of p_i - 1. */
+/* Prime index i in 0..5: set i / 3 (0 wide, 1 narrow), prime i % 3.
+ * p_i, generator g_i of Z_p^*, and the 2-adic order of p_i - 1. */
 uint32_t orc_prime(int i);
 uint32_t orc_generator(int i);
 int orc_two_adicity(int i);
@@ -56,9 +61,28 @@ void orc_ntt_ct_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi);
  * inverse returns N * x). */
 void orc_ntt_gs_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi);
 
-/* Garner CRT of residues (r0 mod p0, r1 mod p1, r2 mod p2) to the unique
- * x < p0*p1*p2, written as three little-endian 32-bit words. */
+/* Prime set serving a transform of 2^log2n points (1: narrow, 0: wide). */
+int orc_set_for(int log2n);
+
+/* Garner CRT of residues (r0 mod p0, r1 mod p1, r2 mod p2) of a set to the
+ * unique x < p0*p1*p2, written as three little-endian 32-bit words.
+ * orc_crt3: the wide set. */
+void orc_crt3_set(int set, const uint32_t r[3], uint32_t out[3]);
 void orc_crt3(const uint32_t r[3], uint32_t out[3]);
+
+/* Lazy butterflies of a narrow prime p < 2^30 with Shoup's quotient
+ * floor(w 2^32 / p).  ct: a, b in [0, 4p) -> (a + w b, a - w b) mod p, both in
+ * [0, 4p).  gs: a, b in [0, 2p) -> (a + b, (a - b) winv) mod p, both in
+ * [0, 2p).  pointwise: x, y in [0, 4p) -> x y 2^-32 mod p in [0, 2p).
+ * load: any 32-bit c -> c mod p in [0, 4p).  The layer functions run whole
+ * layers as orc_ntt_ct_layers / orc_ntt_gs_layers do, with unreduced values. */
+uint32_t orc_shoup_quotient(uint32_t w, uint32_t p);
+void orc_narrow_ct(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p);
+void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p);
+uint32_t orc_narrow_pointwise(uint32_t x, uint32_t y, uint32_t p);
+uint32_t orc_narrow_load(uint32_t c, uint32_t p);
+void orc_narrow_ct_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi);
+void orc_narrow_gs_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi);
 
 /* Number of 32-bit coefficients of an n-limb magnitude whose top limb is
  * non-zero (2n, or 2n-1 if the top limb's high half is zero). */
@@ -72,7 +96,8 @@ int orc_transform_log2(const uint64_t* a, size_t an, const uint64_t* b, size_t b
 int orc_ntt_mul(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out);
 
 /* Convolution residues, for intermediate-buffer checks: res[i*N + k] =
- * (sum_j a_j b_{k-j}) mod p_i with 32-bit coefficients, N = 2^log2n. */
+ * (sum_j a_j b_{k-j}) mod p_i with 32-bit coefficients, N = 2^log2n, over the
+ * primes of the set that serves N. */
 void orc_conv_residues(const uint64_t* a, size_t an, const uint64_t* b, size_t bn,
                        int log2n, uint32_t* res);
 

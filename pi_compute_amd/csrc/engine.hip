@@ -47,9 +47,9 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;  // end of the last batch
     hipStream_t last = nullptr; // stream that batch ran on
-    int tw_log2 = 0;
-    uint2* tw_fwd[kPrimes] = {};
-    uint2* tw_inv[kPrimes] = {};
+    bool tw_ready[kSets] = {};
+    uint2* tw_fwd[kSets][kPrimes] = {};
+    uint2* tw_inv[kSets][kPrimes] = {};
     size_t arena_words = 0;
     uint32_t* arena = nullptr;
     CarryGlobal* glob = nullptr;
@@ -137,7 +137,8 @@ uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((uint64_
 uint2 shoup(uint32_t w, uint32_t p) { return make_uint2(w, (uint32_t)(((uint64_t)w << 32) / p)); }
 
 CrtConst crt_constants(int log2n) {
-    const uint32_t p0 = kP[0], p1 = kP[1], p2 = kP[2];
+    const int set = set_for(log2n);
+    const uint32_t p0 = kPrimeSet[set][0], p1 = kPrimeSet[set][1], p2 = kPrimeSet[set][2];
     auto scale = [&](uint32_t p) {  // 2^32 / N mod p
         return mulmod((uint32_t)((1ull << 32) % p), invmod((uint32_t)((1ull << log2n) % p), p), p);
     };
@@ -198,18 +199,20 @@ int grow(T*& ptr, size_t count) {
     return PINTT_OK;
 }
 
-// The tables are built once at the full 2^25 entries (1.5 GiB for the three
-// primes, forward and inverse) so that captured graphs never see them move.
+// The tables of a prime set are built once, at the set's longest transform
+// (narrow set: 2^22 entries, 192 MiB for the three primes forward and inverse;
+// wide set: 2^25 entries, 1.5 GiB), when a length it serves is first seen, so
+// that captured graphs never see them move.
 int ensure_twiddles(int log2n) {
-    if (log2n <= g.tw_log2) return PINTT_OK;
-    int tl = kMaxLog2N;
-    size_t cnt = (size_t)1 << tl;
+    const int set = set_for(log2n);
+    if (g.tw_ready[set]) return PINTT_OK;
+    size_t cnt = (size_t)1 << kSetMaxLog2N[set];
     for (int i = 0; i < kPrimes; ++i) {
-        PINTT_TRY(grow(g.tw_fwd[i], cnt));
-        PINTT_TRY(grow(g.tw_inv[i], cnt));
-        PINTT_HIP(launch_build_twiddles(g.tw_fwd[i], g.tw_inv[i], i, tl, g.stream));
+        PINTT_TRY(grow(g.tw_fwd[set][i], cnt));
+        PINTT_TRY(grow(g.tw_inv[set][i], cnt));
+        PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], set, i, g.stream));
     }
-    g.tw_log2 = tl;
+    g.tw_ready[set] = true;
     return PINTT_OK;
 }
 
@@ -256,11 +259,11 @@ int log2_for(size_t an, size_t bn) {
     return k > kMaxLog2N ? -1 : k;
 }
 
-Twiddles twiddles() {
+Twiddles twiddles(int set) {
     Twiddles t;
     for (int i = 0; i < kPrimes; ++i) {
-        t.fwd[i] = g.tw_fwd[i];
-        t.inv[i] = g.tw_inv[i];
+        t.fwd[i] = g.tw_fwd[set][i];
+        t.inv[i] = g.tw_inv[set][i];
     }
     return t;
 }
@@ -377,7 +380,7 @@ struct BatchPlan {
     std::vector<BucketLayout> buckets;
     std::vector<Plan> plans;
     size_t bytes = 0, arena_words = 0, total_groups = 0, total_blocks = 0;
-    int max_log2n = 0;
+    int min_log2n = kMaxLog2N + 1, max_log2n = 0;
 };
 
 // Groups, buckets, descriptor layout and resource sizes of a batch.  May tune
@@ -416,7 +419,7 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
             b.max_m = std::max(b.max_m, gr.r.square ? 0u : gr.m);
             nsq += gr.r.square ? 1 : 0;
         }
-        b.pair = g.pair_rows && pair_case_available(pl.log2c, pl.log2rows);
+        b.pair = g.pair_rows && pair_case_available(set_for(pl.log2n), pl.log2c, pl.log2rows);
         bytes = align_up<FwdJob>(bytes);
         b.fwd_off = bytes;
         bytes += b.nfwd * sizeof(FwdJob);
@@ -455,6 +458,7 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
         bp->arena_words = std::max(bp->arena_words, slots * kPrimes * ((size_t)1 << b.log2n));
         bp->total_groups += b.groups.size();
         bp->total_blocks += b.nblocks;
+        bp->min_log2n = std::min(bp->min_log2n, b.log2n);
         bp->max_log2n = std::max(bp->max_log2n, b.log2n);
     }
     bp->bytes = bytes;
@@ -549,7 +553,6 @@ bool fits_inline(const BatchPlan& bp) {
 // device descriptors).
 int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned char* H, hipStream_t s,
                      bool timed) {
-    const Twiddles tw = twiddles();
     auto list = [&](auto* tag, size_t off, uint32_t n) {
         using T = std::remove_pointer_t<decltype(tag)>;
         if (H && n <= (uint32_t)kInlineJobs) return inline_jobs<T>(reinterpret_cast<const T*>(H + off), n);
@@ -560,6 +563,7 @@ int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned
     for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
         const BucketLayout& b = bp.buckets[bi];
         const Plan& pl = bp.plans[bi];
+        const Twiddles tw = twiddles(set_for(b.log2n));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
         for (uint32_t k = 0; k < b.nfwd; k += kMaxJobs) {
             const uint32_t c = std::min(kMaxJobs, b.nfwd - k);
@@ -592,6 +596,7 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
     if (reqs.empty()) return PINTT_OK;
     BatchPlan bp;
     PINTT_TRY(plan_batch(reqs, &bp));
+    PINTT_TRY(ensure_twiddles(bp.min_log2n));
     PINTT_TRY(ensure_twiddles(bp.max_log2n));
     PINTT_TRY(ensure_arena(bp.arena_words));
     PINTT_TRY(ensure_carry_state(bp.total_groups, bp.total_blocks));
@@ -646,6 +651,7 @@ void free_graph(BatchGraph* bg) {
 int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
     BatchPlan bp;
     PINTT_TRY(plan_batch(reqs, &bp));
+    PINTT_TRY(ensure_twiddles(bp.min_log2n));
     PINTT_TRY(ensure_twiddles(bp.max_log2n));
     BatchGraph* bg = new BatchGraph;
     auto cleanup = [&](int rc) {
@@ -884,10 +890,11 @@ void gpu_ntt_shutdown(void) {
     if (!g.ready) return;
     (void)hipSetDevice(g.device);
     (void)hipStreamSynchronize(g.stream);
-    for (int i = 0; i < kPrimes; ++i) {
-        (void)hipFree(g.tw_fwd[i]);
-        (void)hipFree(g.tw_inv[i]);
-    }
+    for (int set = 0; set < kSets; ++set)
+        for (int i = 0; i < kPrimes; ++i) {
+            (void)hipFree(g.tw_fwd[set][i]);
+            (void)hipFree(g.tw_inv[set][i]);
+        }
     (void)hipFree(g.arena);
     (void)hipFree(g.glob);
     (void)hipFree(g.tickets);

@@ -1,13 +1,17 @@
-// modarith.h - 32-bit modular arithmetic for the three NTT primes (gfx950).
+// modarith.h - 32-bit modular arithmetic for the NTT primes (gfx950).
 //
-// All three primes are below 2^31, so 2p < 2^32 and every sum of two reduced
-// residues fits a 32-bit lane.  Twiddle products use Shoup's precomputed
-// quotient (w' = floor(w * 2^32 / p)): one high and one low 32-bit product plus
-// a fused q*p subtraction, result in [0, 2p) for any 32-bit x, then one
-// conditional subtract.  The pointwise product of two variable residues uses
-// Montgomery reduction with R = 2^32 (the stray 2^-32 is folded into the CRT
-// constants).  Reductions are written as min(x, x - p) on unsigned values:
-// x - p wraps above x exactly when x < p.
+// Two sets of three primes.  The wide set (primes just below 2^31, 2-adic
+// order >= 25) serves transforms up to 2^25 points; every residue is kept fully
+// reduced, in [0, p).  The narrow set (primes below 2^30, 2-adic order >= 22)
+// serves transforms up to 2^22 points with Harvey's lazy butterflies: 4p still
+// fits a 32-bit lane, so forward values live in [0, 4p) and inverse values in
+// [0, 2p), and a butterfly needs one conditional subtraction instead of three.
+// Twiddle products use Shoup's precomputed quotient (w' = floor(w * 2^32 / p)):
+// one high and two low 32-bit products give x * w - q * p in [0, 2p) for any
+// 32-bit x.  The pointwise product of two variable residues uses Montgomery
+// reduction with R = 2^32 (the stray 2^-32 is folded into the CRT constants).
+// Reductions are written as min(x, x - c) on unsigned values: x - c wraps
+// above x exactly when x < c.
 //
 // Replaces the reference's 64-bit '%' arithmetic (src/gpu/int_ntt.cu:47-51).
 #pragma once
@@ -18,12 +22,19 @@
 namespace pintt {
 
 constexpr int kPrimes = 3;
-constexpr int kMaxLog2N = 25;  // minimum 2-adic order of p - 1 over the set
+constexpr int kMaxLog2N = 25;        // longest transform (wide set)
+constexpr int kNarrowMaxLog2N = 22;  // longest transform of the narrow set
 
-// p = c * 2^k + 1.
-constexpr uint32_t kP[kPrimes] = {2013265921u /* 15*2^27+1 */, 1811939329u /* 27*2^26+1 */,
-                                  2113929217u /* 63*2^25+1 */};
-constexpr uint32_t kGen[kPrimes] = {31u, 13u, 5u};
+// Prime sets: p = c * 2^k + 1.
+constexpr int kSetWide = 0, kSetNarrow = 1, kSets = 2;
+constexpr uint32_t kPrimeSet[kSets][kPrimes] = {
+    {2013265921u /* 15*2^27+1 */, 1811939329u /* 27*2^26+1 */, 2113929217u /* 63*2^25+1 */},
+    {998244353u /* 119*2^23+1 */, 985661441u /* 235*2^22+1 */, 943718401u /* 225*2^22+1 */}};
+constexpr uint32_t kGenSet[kSets][kPrimes] = {{31u, 13u, 5u}, {3u, 3u, 7u}};
+constexpr int kSetMaxLog2N[kSets] = {kMaxLog2N, kNarrowMaxLog2N};
+
+// The set that serves a transform of 2^log2n points.
+__host__ __device__ constexpr int set_for(int log2n) { return log2n <= kNarrowMaxLog2N ? kSetNarrow : kSetWide; }
 
 __host__ __device__ constexpr uint32_t neg_inv_mod_2_32(uint32_t p) {
     // Newton iteration for p^-1 mod 2^32, negated.
@@ -57,26 +68,75 @@ __device__ __forceinline__ uint32_t mul_mont(uint32_t a, uint32_t b, uint32_t p,
     return min(u, u - p);
 }
 
-// Reduce an arbitrary 32-bit coefficient (< 3p for every prime of the set).
-__device__ __forceinline__ uint32_t reduce_u32(uint32_t c, uint32_t p) {
-    c = min(c, c - 2u * p);
-    return min(c, c - p);
-}
+// Butterflies and the per-set steps around them.  The twiddle z is (w, w').
+template <int SET>
+struct Arith;
 
-// Cooley-Tukey butterfly: (a, b) -> (a + z*b, a - z*b).
-__device__ __forceinline__ void ct_bfly(uint32_t& a, uint32_t& b, uint2 z, uint32_t p) {
-    uint32_t t = mul_shoup(b, z.x, z.y, p);
-    uint32_t u = a;
-    a = add_mod(u, t, p);
-    b = sub_mod(u, t, p);
-}
+// Wide set: every value in [0, p).
+template <>
+struct Arith<kSetWide> {
+    // A 32-bit coefficient (< 3p for every wide prime) to [0, p).
+    __device__ static __forceinline__ uint32_t load_coeff(uint32_t c, uint32_t p) {
+        c = min(c, c - 2u * p);
+        return min(c, c - p);
+    }
+    // Cooley-Tukey: (a, b) -> (a + z*b, a - z*b).
+    __device__ static __forceinline__ void ct(uint32_t& a, uint32_t& b, uint2 z, uint32_t p) {
+        uint32_t t = mul_shoup(b, z.x, z.y, p);
+        uint32_t u = a;
+        a = add_mod(u, t, p);
+        b = sub_mod(u, t, p);
+    }
+    // Gentleman-Sande: (a, b) -> (a + b, (a - b) * zinv).  a - b + p lies in
+    // [1, 2p), inside the Shoup input range, so it needs no reduction first.
+    __device__ static __forceinline__ void gs(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p) {
+        uint32_t u = a, v = b;
+        a = add_mod(u, v, p);
+        b = mul_shoup(u + p - v, zi.x, zi.y, p);
+    }
+    // Forward outputs (in [0, p)) to the inverse input range, multiplied.
+    __device__ static __forceinline__ uint32_t pointwise(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) {
+        return mul_mont(x, y, p, pinv);
+    }
+};
 
-// Gentleman-Sande butterfly: (a, b) -> (a + b, (a - b) * zinv).  a - b + p lies
-// in [1, 2p), inside the Shoup input range, so it needs no reduction first.
-__device__ __forceinline__ void gs_bfly(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p) {
-    uint32_t u = a, v = b;
-    a = add_mod(u, v, p);
-    b = mul_shoup(u + p - v, zi.x, zi.y, p);
-}
+// Narrow set (p < 2^30): forward values in [0, 4p), inverse values in [0, 2p).
+template <>
+struct Arith<kSetNarrow> {
+    // Any 32-bit coefficient to [0, 4p): 2^32 < 8p, so one step suffices.
+    __device__ static __forceinline__ uint32_t load_coeff(uint32_t c, uint32_t p) { return min(c, c - 4u * p); }
+    // a, b in [0, 4p).  x = a reduced to [0, 2p); r = z*b - q*p in [0, 2p) is
+    // kept negated (tn = -r mod 2^32) so that both outputs take one add:
+    // x + r in [0, 4p) and x - r + 2p in (0, 4p).
+    __device__ static __forceinline__ void ct(uint32_t& a, uint32_t& b, uint2 z, uint32_t p) {
+        const uint32_t p2 = 2u * p;
+        const uint32_t x = min(a, a - p2);
+        const uint32_t q = __umulhi(b, z.y);
+        const uint32_t tn = q * p - b * z.x;
+        a = x - tn;
+        b = x + tn + p2;
+    }
+    // a, b in [0, 2p): a + b in [0, 4p) reduced to [0, 2p); (a - b + 2p) in
+    // (0, 4p) times zinv by Shoup, in [0, 2p).
+    __device__ static __forceinline__ void gs(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p) {
+        const uint32_t p2 = 2u * p;
+        const uint32_t u = a, v = b;
+        const uint32_t s = u + v;
+        a = min(s, s - p2);
+        const uint32_t d = u - v + p2;
+        const uint32_t q = __umulhi(d, zi.y);
+        b = d * zi.x - q * p;
+    }
+    // x, y in [0, 4p) reduced to [0, 2p); their product is below 4p^2 < 2^32 p,
+    // so the Montgomery result (t + m p) / 2^32 lies in [0, 2p) unreduced.
+    __device__ static __forceinline__ uint32_t pointwise(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) {
+        const uint32_t p2 = 2u * p;
+        x = min(x, x - p2);
+        y = min(y, y - p2);
+        const uint64_t t = (uint64_t)x * y;
+        const uint32_t m = (uint32_t)t * pinv;
+        return (uint32_t)((t + (uint64_t)m * p) >> 32);
+    }
+};
 
 }  // namespace pintt

@@ -76,9 +76,11 @@ __device__ __forceinline__ uint32_t block_scan_fn(uint32_t f, uint32_t* lds_wave
 }
 
 // Garner CRT of one position: the exact value x < p0 p1 p2 < 2^93 as 3 words.
+// Residues may be any 32-bit values (each is first multiplied by Shoup).
+template <int SET>
 __device__ __forceinline__ void crt_words(uint32_t r0, uint32_t r1, uint32_t r2, const CrtConst& cc,
                                           uint32_t& x0, uint32_t& x1, uint32_t& x2) {
-    constexpr uint32_t p0 = kP[0], p1 = kP[1], p2 = kP[2];
+    constexpr uint32_t p0 = kPrimeSet[SET][0], p1 = kPrimeSet[SET][1], p2 = kPrimeSet[SET][2];
     uint32_t y0 = mul_shoup(r0, cc.k0.x, cc.k0.y, p0);
     uint32_t y1 = sub_mod(mul_shoup(r1, cc.k1.x, cc.k1.y, p1), mul_shoup(y0, cc.c1.x, cc.c1.y, p1), p1);
     uint32_t y2 = sub_mod(mul_shoup(r2, cc.k2.x, cc.k2.y, p2), mul_shoup(y0, cc.c20.x, cc.c20.y, p2), p2);
@@ -158,22 +160,24 @@ __device__ __forceinline__ void load_chunk(ChunkRes<WT>& c, global_ptr<const uin
 
 // Adds the chunk product's word contributions X0[k] + X1[k-1] + X2[k-2] to the
 // thread's word sums s[0 .. WT).
-template <int WT>
+template <int WT, int SET>
 __device__ __forceinline__ void add_chunk_words(uint64_t* s, const ChunkRes<WT>& c, const CrtConst& cc) {
     uint32_t xm1[3], xm2[3];  // x at positions k-1, k-2
-    crt_words(c.h[0][0], c.h[0][1], c.h[0][2], cc, xm2[0], xm2[1], xm2[2]);
-    crt_words(c.h[1][0], c.h[1][1], c.h[1][2], cc, xm1[0], xm1[1], xm1[2]);
+    crt_words<SET>(c.h[0][0], c.h[0][1], c.h[0][2], cc, xm2[0], xm2[1], xm2[2]);
+    crt_words<SET>(c.h[1][0], c.h[1][1], c.h[1][2], cc, xm1[0], xm1[1], xm1[2]);
 #pragma unroll
     for (int e = 0; e < WT; ++e) {
         uint32_t x0, x1, x2;
-        crt_words(c.r[0][e], c.r[1][e], c.r[2][e], cc, x0, x1, x2);
+        crt_words<SET>(c.r[0][e], c.r[1][e], c.r[2][e], cc, x0, x1, x2);
         s[e] += (uint64_t)x0 + xm1[1] + xm2[2];
         xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
         xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
     }
 }
 
-// grid = total carry blocks of the launch; jobs sorted by blk0.
+// grid = total carry blocks of the launch; jobs sorted by blk0.  SET: the
+// prime set the residues belong to.
+template <int SET>
 __global__ void __launch_bounds__(kCarryThreads)
     k_crt_carry(const JobList<CarryJob> jobs, uint32_t njobs, int log2n, CrtConst cc,
                 CarryGlobal* __restrict__ glob) {
@@ -227,8 +231,8 @@ __global__ void __launch_bounds__(kCarryThreads)
             if (two)
                 load_chunk<WT>(c1, as_global(job.res[i + 1]), n, job.len[i + 1],
                                (long long)j0 - (long long)job.off[i + 1]);
-            if (c0.any) add_chunk_words<WT>(s, c0, cc);
-            if (two && c1.any) add_chunk_words<WT>(s, c1, cc);
+            if (c0.any) add_chunk_words<WT, SET>(s, c0, cc);
+            if (two && c1.any) add_chunk_words<WT, SET>(s, c1, cc);
         }
     }
 #pragma unroll
@@ -333,9 +337,10 @@ __device__ __forceinline__ uint32_t powmod_dev(uint32_t b, uint32_t e, uint32_t 
     return (uint32_t)r;
 }
 
-// Z[i], i = 2^s + b: w_{2^(s+1)}^bitrev_s(b) = w25^(bitrev_s(b) * 2^(24 - s)).
+// Z[i], i = 2^s + b: w_{2^(s+1)}^bitrev_s(b) = wK^(bitrev_s(b) * 2^(K - 1 - s)),
+// wK a primitive 2^K-th root of unity (K: the set's longest transform).
 __global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ inv, uint32_t p,
-                                 uint32_t w25, uint32_t count) {
+                                 uint32_t wk, int log2k, uint32_t count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     uint32_t ef = 0;
@@ -343,36 +348,42 @@ __global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ in
         int s = 31 - __clz(i);
         uint32_t b = i - (1u << s);
         uint32_t rev = __brev(b) >> (32 - s);
-        ef = rev << (24 - s);
+        ef = rev << (log2k - 1 - s);
     }
-    uint32_t ei = (ef == 0) ? 0u : ((1u << 25) - ef);
-    uint32_t z = (i == 0) ? 1u : powmod_dev(w25, ef, p);
-    uint32_t zi = (i == 0) ? 1u : powmod_dev(w25, ei, p);
+    uint32_t ei = (ef == 0) ? 0u : ((1u << log2k) - ef);
+    uint32_t z = (i == 0) ? 1u : powmod_dev(wk, ef, p);
+    uint32_t zi = (i == 0) ? 1u : powmod_dev(wk, ei, p);
     fwd[i] = make_uint2(z, (uint32_t)(((uint64_t)z << 32) / p));
     inv[i] = make_uint2(zi, (uint32_t)(((uint64_t)zi << 32) / p));
 }
 
 }  // namespace
 
-hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s) {
-    const uint32_t p = kP[prime];
-    // w25: a primitive 2^25-th root of unity, g^((p-1)/2^25).
-    uint64_t r = 1, x = kGen[prime];
-    uint64_t e = (uint64_t)(p - 1) >> 25;
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int set, int prime, hipStream_t s) {
+    const uint32_t p = kPrimeSet[set][prime];
+    const int log2k = kSetMaxLog2N[set];
+    // wK = g^((p-1)/2^K).
+    uint64_t r = 1, x = kGenSet[set][prime];
+    uint64_t e = (uint64_t)(p - 1) >> log2k;
     while (e) {
         if (e & 1) r = (r * x) % p;
         x = (x * x) % p;
         e >>= 1;
     }
-    uint32_t count = 1u << log2n;
+    uint32_t count = 1u << log2k;
     hipLaunchKernelGGL(k_build_twiddles, dim3((count + 255) / 256), dim3(256), 0, s, fwd, inv, p,
-                       (uint32_t)r, count);
+                       (uint32_t)r, log2k, count);
     return hipGetLastError();
 }
 
 hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
                             const CrtConst& cc, CarryGlobal* glob, hipStream_t s) {
-    hipLaunchKernelGGL(k_crt_carry, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc, glob);
+    if (set_for(log2n) == kSetNarrow)
+        hipLaunchKernelGGL(k_crt_carry<kSetNarrow>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n,
+                           cc, glob);
+    else
+        hipLaunchKernelGGL(k_crt_carry<kSetWide>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc,
+                           glob);
     return hipGetLastError();
 }
 

@@ -13,8 +13,8 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bx, uint32_t nb) {
 }
 
 // Pass 1: forward column layers, fused with limb unpacking and reduction.
-// grid = (column tiles, prime, job).
-template <int LOGR, int LOGW>
+// grid = (column tiles, prime, job).  SET: the prime set of the transform length.
+template <int LOGR, int LOGW, int SET>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     k_col_fwd(const JobList<FwdJob> jobs, int log2c, Twiddles tw) {
     using TL = Tile<LOGR, LOGW, true>;
@@ -22,7 +22,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     __shared__ uint2 zt[TL::L];  // the R - 1 twiddles of the column layers (entry 0 unused)
     const FwdJob job = PINTT_JOB(FwdJob, jobs, blockIdx.z);
     const int prime = blockIdx.y;
-    const uint32_t p = prime_of(prime);
+    const uint32_t p = prime_of<SET>(prime);
     global_ptr<uint32_t> dst = as_global(job.dst) + ((size_t)prime << (LOGR + log2c));
     global_ptr<const uint32_t> src = as_global(job.src);
     const TwTable z = PINTT_FWD_TABLE(tw, prime);
@@ -36,11 +36,11 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
         uint32_t k = (r << log2c) + c0 + w;
         uint32_t v = k < job.ncoef ? src[k] : 0u;
-        lds[TL::addr(w, r)] = reduce_u32(v, p);
+        lds[TL::addr(w, r)] = Arith<SET>::load_coeff(v, p);
     }
     __syncthreads();
     uint32_t* const tiles[1] = {lds};
-    ct_run<TL, 0, 1>(tiles, zt, 0, 0, p);
+    ct_run<SET, TL, 0, 1>(tiles, zt, 0, 0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 // Pass 3: inverse column layers, one prime of one chunk product per workgroup,
 // in place.  grid = (column tiles, prime, job).  It also arms the carry pass
 // that follows on the stream: a new epoch, fresh tickets for every group.
-template <int LOGR, int LOGW>
+template <int LOGR, int LOGW, int SET>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     k_col_inv(const JobList<InvJob> jobs, int log2c, Twiddles tw, CarryGlobal* __restrict__ glob) {
     using TL = Tile<LOGR, LOGW, true>;
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
         if (blockIdx.z == 0) glob->epoch = glob->epoch + 1;
     }
     const int prime = blockIdx.y;
-    const uint32_t p = prime_of(prime);
+    const uint32_t p = prime_of<SET>(prime);
     global_ptr<uint32_t> buf = as_global(job.buf) + ((size_t)prime << (LOGR + log2c));
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
     const TwTable z = PINTT_INV_TABLE(tw, prime);
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     }
     __syncthreads();
     uint32_t* const tiles[1] = {lds};
-    gs_run<TL, 0, 1>(tiles, zt, 0, 0, p);
+    gs_run<SET, TL, 0, 1>(tiles, zt, 0, 0, p);
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
@@ -88,12 +88,19 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 
 }  // namespace
 
-#define PINTT_COL_CASES(X) X(5, 5) X(5, 6) X(6, 4) X(6, 5) X(6, 6) X(7, 3) X(7, 4) X(7, 5) X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(9, 3) X(9, 4) X(9, 5) X(10, 3) X(10, 4) X(11, 3)
+// Tile shapes per prime set.  The wide set only serves 2^23 .. 2^25 points,
+// where rows of at most 2^14 leave 2^9 .. 2^11 rows.
+#define PINTT_COL_CASES_NARROW(X) X(5, 5) X(5, 6) X(6, 4) X(6, 5) X(6, 6) X(7, 3) X(7, 4) X(7, 5) X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(9, 3) X(9, 4) X(9, 5) X(10, 3) X(10, 4) X(11, 3)
+#define PINTT_COL_CASES_WIDE(X) X(9, 3) X(9, 4) X(9, 5) X(10, 3) X(10, 4) X(11, 3)
 
-bool col_case_available(int log2r, int log2w) {
+bool col_case_available(int set, int log2r, int log2w) {
 #define PINTT_CASE(LR, LW) \
     if (log2r == LR && log2w == LW) return true;
-    PINTT_COL_CASES(PINTT_CASE)
+    if (set == kSetNarrow) {
+        PINTT_COL_CASES_NARROW(PINTT_CASE)
+    } else {
+        PINTT_COL_CASES_WIDE(PINTT_CASE)
+    }
 #undef PINTT_CASE
     return false;
 }
@@ -101,28 +108,44 @@ bool col_case_available(int log2r, int log2w) {
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
-#define PINTT_CASE(LR, LW)                                                                          \
-    if (pl.log2r == LR && pl.log2w == LW) {                                                         \
-        hipLaunchKernelGGL((k_col_fwd<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs,    \
-                           pl.log2c, tw);                                                           \
-        return hipGetLastError();                                                                   \
+#define PINTT_CASE_SET(LR, LW, SET)                                                                  \
+    if (pl.log2r == LR && pl.log2w == LW) {                                                          \
+        hipLaunchKernelGGL((k_col_fwd<LR, LW, SET>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs, \
+                           pl.log2c, tw);                                                            \
+        return hipGetLastError();                                                                    \
     }
-    PINTT_COL_CASES(PINTT_CASE)
-#undef PINTT_CASE
+#define PINTT_CASE_N(LR, LW) PINTT_CASE_SET(LR, LW, kSetNarrow)
+#define PINTT_CASE_W(LR, LW) PINTT_CASE_SET(LR, LW, kSetWide)
+    if (set_for(pl.log2n) == kSetNarrow) {
+        PINTT_COL_CASES_NARROW(PINTT_CASE_N)
+    } else {
+        PINTT_COL_CASES_WIDE(PINTT_CASE_W)
+    }
+#undef PINTT_CASE_W
+#undef PINTT_CASE_N
+#undef PINTT_CASE_SET
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, CarryGlobal* glob,
                                   const Twiddles& tw, hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
-#define PINTT_CASE(LR, LW)                                                                          \
-    if (pl.log2r == LR && pl.log2w == LW) {                                                         \
-        hipLaunchKernelGGL((k_col_inv<LR, LW>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs,    \
-                           pl.log2c, tw, glob);                                                     \
-        return hipGetLastError();                                                                   \
+#define PINTT_CASE_SET(LR, LW, SET)                                                                  \
+    if (pl.log2r == LR && pl.log2w == LW) {                                                          \
+        hipLaunchKernelGGL((k_col_inv<LR, LW, SET>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs, \
+                           pl.log2c, tw, glob);                                                      \
+        return hipGetLastError();                                                                    \
     }
-    PINTT_COL_CASES(PINTT_CASE)
-#undef PINTT_CASE
+#define PINTT_CASE_N(LR, LW) PINTT_CASE_SET(LR, LW, kSetNarrow)
+#define PINTT_CASE_W(LR, LW) PINTT_CASE_SET(LR, LW, kSetWide)
+    if (set_for(pl.log2n) == kSetNarrow) {
+        PINTT_COL_CASES_NARROW(PINTT_CASE_N)
+    } else {
+        PINTT_COL_CASES_WIDE(PINTT_CASE_W)
+    }
+#undef PINTT_CASE_W
+#undef PINTT_CASE_N
+#undef PINTT_CASE_SET
     return hipErrorInvalidValue;
 }
 

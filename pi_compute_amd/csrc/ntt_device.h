@@ -10,8 +10,9 @@
 #include "ntt_kernels.h"
 
 namespace pintt {
+template <int SET>
 __device__ __forceinline__ uint32_t prime_of(int i) {
-    return i == 0 ? kP[0] : (i == 1 ? kP[1] : kP[2]);
+    return i == 0 ? kPrimeSet[SET][0] : (i == 1 ? kPrimeSet[SET][1] : kPrimeSet[SET][2]);
 }
 
 // Pointers read from job descriptors are generic to the compiler, which then
@@ -89,11 +90,11 @@ struct Stage {
 };
 
 // Forward (Cooley-Tukey) layers [S0, S0+K) on NB tiles (1..3) that share
-// their twiddles.  Twiddle of local layer S0+l, sub-block `sub` of the group:
-// global layer sb + S0 + l, global block ((row0 + seq) << (S0 + l)) +
-// (bblk << l) + sub, where row0 is the block index of sequence 0 at layer sb
-// (0 for columns, the row for rows).
-template <class TL, int S0, int K, int NB, class ZP>
+// their twiddles, with the butterflies of prime set SET.  Twiddle of local
+// layer S0+l, sub-block `sub` of the group: global layer sb + S0 + l, global
+// block ((row0 + seq) << (S0 + l)) + (bblk << l) + sub, where row0 is the
+// block index of sequence 0 at layer sb (0 for columns, the row for rows).
+template <int SET, class TL, int S0, int K, int NB, class ZP>
 __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
@@ -115,7 +116,7 @@ __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb,
 #pragma unroll
                 for (int i = 0; i < h; ++i)
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) ct_bfly(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
+                    for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
             }
         }
 #pragma unroll
@@ -126,7 +127,7 @@ __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb,
 }
 
 // Inverse (Gentleman-Sande) layers S0+K-1 down to S0 on NB tiles (1..2).
-template <class TL, int S0, int K, int NB, class ZP>
+template <int SET, class TL, int S0, int K, int NB, class ZP>
 __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
@@ -148,7 +149,7 @@ __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb
 #pragma unroll
                 for (int i = 0; i < h; ++i)
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) gs_bfly(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
+                    for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
             }
         }
 #pragma unroll
@@ -158,22 +159,22 @@ __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb
     }
 }
 
-template <class TL, int S0, int NB, class ZP>
+template <int SET, class TL, int S0, int NB, class ZP>
 __device__ __forceinline__ void ct_run(uint32_t* const (&t)[NB], ZP Z, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
-        ct_stage<TL, S0, K, NB>(t, Z, sb, row0, p);
+        ct_stage<SET, TL, S0, K, NB>(t, Z, sb, row0, p);
         __syncthreads();
-        ct_run<TL, S0 + K, NB>(t, Z, sb, row0, p);
+        ct_run<SET, TL, S0 + K, NB>(t, Z, sb, row0, p);
     }
 }
 
-template <class TL, int S0, int NB, class ZP>
+template <int SET, class TL, int S0, int NB, class ZP>
 __device__ __forceinline__ void gs_run(uint32_t* const (&t)[NB], ZP Zi, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
-        gs_run<TL, S0 + K, NB>(t, Zi, sb, row0, p);
-        gs_stage<TL, S0, K, NB>(t, Zi, sb, row0, p);
+        gs_run<SET, TL, S0 + K, NB>(t, Zi, sb, row0, p);
+        gs_stage<SET, TL, S0, K, NB>(t, Zi, sb, row0, p);
         __syncthreads();
     }
 }

@@ -134,16 +134,18 @@ __host__ inline Plan make_plan(int log2n) {
     return pl;
 }
 
-// Tile shapes compiled into the library (ntt_cols.hip / ntt_rows.hip).
-bool col_case_available(int log2r, int log2w);
-bool row_case_available(int log2c, int log2rows);
+// Tile shapes compiled into the library per prime set (ntt_cols.hip / ntt_rows.hip).
+bool col_case_available(int set, int log2r, int log2w);
+bool row_case_available(int set, int log2c, int log2rows);
 // Pair row kernels hold three tiles in LDS: rows of at most 2^13 words per workgroup.
-inline bool pair_case_available(int log2c, int log2rows) { return log2c + log2rows <= 13; }
+bool pair_case_available(int set, int log2c, int log2rows);
 
-// A plan with an explicit split, checked against the compiled tile shapes.
+// A plan with an explicit split, checked against the tile shapes compiled for
+// the prime set of its length.
 __host__ inline bool plan_valid(const Plan& pl) {
+    const int set = set_for(pl.log2n);
     return pl.log2r + pl.log2c == pl.log2n && pl.log2w <= pl.log2c && pl.log2rows <= pl.log2r &&
-           col_case_available(pl.log2r, pl.log2w) && row_case_available(pl.log2c, pl.log2rows);
+           col_case_available(set, pl.log2r, pl.log2w) && row_case_available(set, pl.log2c, pl.log2rows);
 }
 
 // A launch's job list: up to kInlineJobs jobs travel inside the kernel
@@ -188,7 +190,8 @@ inline JobList<T> inline_jobs(const T* host, uint32_t n) {
 
 // Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip); counts are
 // host values.
-hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int prime, int log2n, hipStream_t s);
+// Tables of 2^kSetMaxLog2N[set] entries for one prime of a set.
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int set, int prime, hipStream_t s);
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,

@@ -7,11 +7,12 @@ namespace {
 
 // Pass 2, per row tile: forward row layers of the chunk(s) (and of b unless
 // they are already done), Montgomery pointwise products, inverse row layers,
-// written back in place.  grid = (row tiles, prime, job).
+// written back in place.  grid = (row tiles, prime, job).  SET: the prime set
+// of the transform length.
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
 // ~250 registers on the paired transforms and halves occupancy.  The pair
 // kind (three tiles in flight) gets two waves and 256 registers instead.
-template <int LOGC, int LOGROWS, int KIND>
+template <int LOGC, int LOGROWS, int KIND, int SET>
 __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute__((amdgpu_waves_per_eu(KIND == kRowPair ? 2 : 3)))
     k_row_conv(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
     using TL = Tile<LOGC, LOGROWS, false>;
@@ -21,7 +22,7 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     __shared__ uint32_t ldsC[kPair ? TL::WORDS : 1];
     const RowJob job = PINTT_JOB(RowJob, jobs, blockIdx.z);
     const int prime = blockIdx.y;
-    const uint32_t p = prime_of(prime);
+    const uint32_t p = prime_of<SET>(prime);
     const uint32_t pinv = neg_inv_mod_2_32(p);
     const size_t base = ((size_t)prime << (log2r + LOGC)) + ((size_t)blockIdx.x << (LOGROWS + LOGC));
     const uint32_t row0 = blockIdx.x << LOGROWS;
@@ -39,13 +40,13 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     const auto zf = PINTT_FWD_TABLE(tw, prime);
     if constexpr (kPair) {
         uint32_t* const t[3] = {ldsA, ldsB, ldsC};
-        ct_run<TL, 0, 3>(t, zf, log2r, row0, p);
+        ct_run<SET, TL, 0, 3>(t, zf, log2r, row0, p);
     } else if constexpr (KIND == kRowTransformB) {
         uint32_t* const t[2] = {ldsA, ldsB};
-        ct_run<TL, 0, 2>(t, zf, log2r, row0, p);
+        ct_run<SET, TL, 0, 2>(t, zf, log2r, row0, p);
     } else {
         uint32_t* const t[1] = {ldsA};
-        ct_run<TL, 0, 1>(t, zf, log2r, row0, p);
+        ct_run<SET, TL, 0, 1>(t, zf, log2r, row0, p);
     }
     if constexpr (KIND == kRowTransformB || kPair) {
         if (job.write_b) {
@@ -61,17 +62,17 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         uint32_t idx = pad(e * TL::NT + threadIdx.x);
         uint32_t x = ldsA[idx];
         uint32_t y = KIND == kRowSquare ? x : ldsB[idx];
-        ldsA[idx] = mul_mont(x, y, p, pinv);
-        if constexpr (kPair) ldsC[idx] = mul_mont(ldsC[idx], y, p, pinv);
+        ldsA[idx] = Arith<SET>::pointwise(x, y, p, pinv);
+        if constexpr (kPair) ldsC[idx] = Arith<SET>::pointwise(ldsC[idx], y, p, pinv);
     }
     __syncthreads();
     const auto zi = PINTT_INV_TABLE(tw, prime);
     if constexpr (kPair) {
         uint32_t* const t[2] = {ldsA, ldsC};
-        gs_run<TL, 0, 2>(t, zi, log2r, row0, p);
+        gs_run<SET, TL, 0, 2>(t, zi, log2r, row0, p);
     } else {
         uint32_t* const t[1] = {ldsA};
-        gs_run<TL, 0, 1>(t, zi, log2r, row0, p);
+        gs_run<SET, TL, 0, 1>(t, zi, log2r, row0, p);
     }
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
@@ -83,23 +84,43 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
 
 }  // namespace
 
-#define PINTT_ROW_CASES(X) \
+// Tile shapes per prime set.  The wide set only serves 2^23 .. 2^25 points,
+// rows of 2^12 .. 2^14 (columns hold at most 2^11 rows).
+#define PINTT_ROW_CASES_NARROW(X) \
     X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(6, 5) X(6, 6) X(6, 7) X(7, 4) \
     X(7, 5) X(7, 6) X(8, 3) X(8, 4) X(8, 5) X(9, 2) X(9, 3) X(9, 4) \
     X(10, 1) X(10, 2) X(10, 3) X(11, 0) X(11, 1) X(11, 2) X(12, 0) X(12, 1) \
     X(13, 0) X(14, 0)
+#define PINTT_ROW_CASES_WIDE(X) X(12, 0) X(12, 1) X(13, 0) X(14, 0)
 
 // Pair kernels: the row cases with at most 2^13 words per workgroup.
-#define PINTT_PAIR_CASES(X) \
+#define PINTT_PAIR_CASES_NARROW(X) \
     X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(6, 5) X(6, 6) X(6, 7) X(7, 4) \
     X(7, 5) X(7, 6) X(8, 3) X(8, 4) X(8, 5) X(9, 2) X(9, 3) X(9, 4) \
     X(10, 1) X(10, 2) X(10, 3) X(11, 0) X(11, 1) X(11, 2) X(12, 0) X(12, 1) \
     X(13, 0)
+#define PINTT_PAIR_CASES_WIDE(X) X(12, 0) X(12, 1) X(13, 0)
 
-bool row_case_available(int log2c, int log2rows) {
+bool row_case_available(int set, int log2c, int log2rows) {
 #define PINTT_CASE(LC, LROWS) \
     if (log2c == LC && log2rows == LROWS) return true;
-    PINTT_ROW_CASES(PINTT_CASE)
+    if (set == kSetNarrow) {
+        PINTT_ROW_CASES_NARROW(PINTT_CASE)
+    } else {
+        PINTT_ROW_CASES_WIDE(PINTT_CASE)
+    }
+#undef PINTT_CASE
+    return false;
+}
+
+bool pair_case_available(int set, int log2c, int log2rows) {
+#define PINTT_CASE(LC, LROWS) \
+    if (log2c == LC && log2rows == LROWS) return true;
+    if (set == kSetNarrow) {
+        PINTT_PAIR_CASES_NARROW(PINTT_CASE)
+    } else {
+        PINTT_PAIR_CASES_WIDE(PINTT_CASE)
+    }
 #undef PINTT_CASE
     return false;
 }
@@ -107,26 +128,39 @@ bool row_case_available(int log2c, int log2rows) {
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,
                                   const Twiddles& tw, hipStream_t s) {
     dim3 grid(1u << (pl.log2r - pl.log2rows), kPrimes, njobs);
-#define PINTT_LAUNCH(LC, LROWS, K)                                                                \
-    hipLaunchKernelGGL((k_row_conv<LC, LROWS, K>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s, \
+#define PINTT_LAUNCH(LC, LROWS, K, SET)                                                                \
+    hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s, \
                        jobs, pl.log2r, tw)
-#define PINTT_CASE(LC, LROWS)                                                   \
-    if (pl.log2c == LC && pl.log2rows == LROWS) {                               \
-        if (kind == kRowSquare) PINTT_LAUNCH(LC, LROWS, kRowSquare);            \
-        else if (kind == kRowTransformB) PINTT_LAUNCH(LC, LROWS, kRowTransformB); \
-        else if (kind == kRowReuseB) PINTT_LAUNCH(LC, LROWS, kRowReuseB);       \
-        else return hipErrorInvalidValue;                                       \
-        return hipGetLastError();                                               \
+#define PINTT_CASE_SET(LC, LROWS, SET)                                                \
+    if (pl.log2c == LC && pl.log2rows == LROWS) {                                     \
+        if (kind == kRowSquare) PINTT_LAUNCH(LC, LROWS, kRowSquare, SET);             \
+        else if (kind == kRowTransformB) PINTT_LAUNCH(LC, LROWS, kRowTransformB, SET); \
+        else if (kind == kRowReuseB) PINTT_LAUNCH(LC, LROWS, kRowReuseB, SET);       \
+        else return hipErrorInvalidValue;                                             \
+        return hipGetLastError();                                                     \
     }
-#define PINTT_PAIR_CASE(LC, LROWS)                                              \
-    if (kind == kRowPair && pl.log2c == LC && pl.log2rows == LROWS) {          \
-        PINTT_LAUNCH(LC, LROWS, kRowPair);                                      \
-        return hipGetLastError();                                               \
+#define PINTT_PAIR_CASE_SET(LC, LROWS, SET)                                 \
+    if (kind == kRowPair && pl.log2c == LC && pl.log2rows == LROWS) {      \
+        PINTT_LAUNCH(LC, LROWS, kRowPair, SET);                             \
+        return hipGetLastError();                                           \
     }
-    PINTT_PAIR_CASES(PINTT_PAIR_CASE)
-    PINTT_ROW_CASES(PINTT_CASE)
-#undef PINTT_PAIR_CASE
-#undef PINTT_CASE
+#define PINTT_CASE_N(LC, LROWS) PINTT_CASE_SET(LC, LROWS, kSetNarrow)
+#define PINTT_CASE_W(LC, LROWS) PINTT_CASE_SET(LC, LROWS, kSetWide)
+#define PINTT_PAIR_CASE_N(LC, LROWS) PINTT_PAIR_CASE_SET(LC, LROWS, kSetNarrow)
+#define PINTT_PAIR_CASE_W(LC, LROWS) PINTT_PAIR_CASE_SET(LC, LROWS, kSetWide)
+    if (set_for(pl.log2n) == kSetNarrow) {
+        PINTT_PAIR_CASES_NARROW(PINTT_PAIR_CASE_N)
+        PINTT_ROW_CASES_NARROW(PINTT_CASE_N)
+    } else {
+        PINTT_PAIR_CASES_WIDE(PINTT_PAIR_CASE_W)
+        PINTT_ROW_CASES_WIDE(PINTT_CASE_W)
+    }
+#undef PINTT_PAIR_CASE_W
+#undef PINTT_PAIR_CASE_N
+#undef PINTT_CASE_W
+#undef PINTT_CASE_N
+#undef PINTT_PAIR_CASE_SET
+#undef PINTT_CASE_SET
 #undef PINTT_LAUNCH
     return hipErrorInvalidValue;
 }

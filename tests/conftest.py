@@ -33,6 +33,19 @@ class Oracle:
         for f in (L.orc_ntt_ct_layers, L.orc_ntt_gs_layers):
             f.argtypes = [_u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_crt3.argtypes = [_u32p, _u32p]
+        L.orc_crt3_set.argtypes = [ctypes.c_int, _u32p, _u32p]
+        L.orc_set_for.argtypes = [ctypes.c_int]
+        L.orc_shoup_quotient.restype = ctypes.c_uint32
+        L.orc_shoup_quotient.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        _pu32 = ctypes.POINTER(ctypes.c_uint32)
+        for f in (L.orc_narrow_ct, L.orc_narrow_gs):
+            f.argtypes = [_pu32, _pu32, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_narrow_pointwise.restype = ctypes.c_uint32
+        L.orc_narrow_pointwise.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_narrow_load.restype = ctypes.c_uint32
+        L.orc_narrow_load.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        for f in (L.orc_narrow_ct_layers, L.orc_narrow_gs_layers):
+            f.argtypes = [_u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_ntt_mul.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
         L.orc_gmp_mul.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
         L.orc_conv_residues.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, ctypes.c_int, _u32p]
@@ -130,11 +143,28 @@ class Oracle:
         self.L.orc_mlp_adam(w.ctypes.data_as(_f64p), g.ctypes.data_as(_f64p), m.ctypes.data_as(_f64p),
                             v.ctypes.data_as(_f64p), w.size, t, lr)
 
-    def crt3(self, r):
+    def crt3(self, r, prime_set=0):
         r = np.ascontiguousarray(r, np.uint32)
         out = np.zeros(3, np.uint32)
-        self.L.orc_crt3(self._p32(r), self._p32(out))
+        self.L.orc_crt3_set(prime_set, self._p32(r), self._p32(out))
         return int(out[0]) | (int(out[1]) << 32) | (int(out[2]) << 64)
+
+    # --- lazy butterflies of the narrow prime set ---
+    def narrow_ct(self, a, b, w, p):
+        x, y = ctypes.c_uint32(a), ctypes.c_uint32(b)
+        self.L.orc_narrow_ct(ctypes.byref(x), ctypes.byref(y), w, p)
+        return x.value, y.value
+
+    def narrow_gs(self, a, b, winv, p):
+        x, y = ctypes.c_uint32(a), ctypes.c_uint32(b)
+        self.L.orc_narrow_gs(ctypes.byref(x), ctypes.byref(y), winv, p)
+        return x.value, y.value
+
+    def narrow_layers(self, x, log2n, prime, lo, hi, inverse=False):
+        x = np.ascontiguousarray(x, np.uint32).copy()
+        f = self.L.orc_narrow_gs_layers if inverse else self.L.orc_narrow_ct_layers
+        f(self._p32(x), log2n, prime, lo, hi)
+        return x
 
 
 @pytest.fixture(scope="session")

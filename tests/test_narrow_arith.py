@@ -1,0 +1,106 @@
+"""Pins the lazy (Harvey) arithmetic of the narrow prime set in the oracle.
+
+Transforms of up to 2^22 points run over three primes below 2^30, so 4p fits
+a 32-bit word: forward values may stay in [0, 4p) and inverse values in
+[0, 2p), and each butterfly needs one conditional subtraction.  The checks
+here use only the definitions (congruences computed with Python integers,
+ranges) and the strict restatement, which the textbook DFT pins:
+  * every butterfly output lies in its range and is congruent to the exact
+    butterfly, on edge values of the input ranges and on random values;
+  * whole lazy transforms (columns then rows, and inverse) agree with the
+    strict transforms modulo p and stay in range;
+  * inputs outside the contract are caught by the same checks.
+"""
+import random
+
+import numpy as np
+import pytest
+
+NARROW = (3, 4, 5)  # prime indices of the narrow set
+
+
+def _edges(p, top):
+    vals = {0, 1, 2, p - 1, p, p + 1, 2 * p - 1, 2 * p, 2 * p + 1, 3 * p, top - 2, top - 1}
+    return sorted(v for v in vals if 0 <= v < top)
+
+
+def _twiddles(oracle, prime, rng):
+    p = oracle.prime(prime)
+    z, zi = oracle.zeta(prime, 8)
+    return [1, 2, p - 1, p - 2, int(z[3]), int(zi[3]), int(z[200])] + [rng.randrange(p) for _ in range(6)]
+
+
+@pytest.mark.parametrize("prime", NARROW)
+def test_ct_butterfly_range_and_congruence(oracle, prime):
+    p = oracle.prime(prime)
+    rng = random.Random(prime)
+    ws = _twiddles(oracle, prime, rng)
+    cases = [(a, b, w) for a in _edges(p, 4 * p) for b in _edges(p, 4 * p) for w in ws]
+    cases += [(rng.randrange(4 * p), rng.randrange(4 * p), rng.randrange(p)) for _ in range(20000)]
+    for a, b, w in cases:
+        A, B = oracle.narrow_ct(a, b, w, p)
+        assert A < 4 * p and B < 4 * p, (a, b, w)
+        assert (A - a - w * b) % p == 0 and (B - a + w * b) % p == 0, (a, b, w)
+
+
+@pytest.mark.parametrize("prime", NARROW)
+def test_gs_butterfly_range_and_congruence(oracle, prime):
+    p = oracle.prime(prime)
+    rng = random.Random(10 + prime)
+    ws = _twiddles(oracle, prime, rng)
+    cases = [(a, b, w) for a in _edges(p, 2 * p) for b in _edges(p, 2 * p) for w in ws]
+    cases += [(rng.randrange(2 * p), rng.randrange(2 * p), rng.randrange(p)) for _ in range(20000)]
+    for a, b, w in cases:
+        A, B = oracle.narrow_gs(a, b, w, p)
+        assert A < 2 * p and B < 2 * p, (a, b, w)
+        assert (A - a - b) % p == 0 and (B - (a - b) * w) % p == 0, (a, b, w)
+
+
+@pytest.mark.parametrize("prime", NARROW)
+def test_pointwise_and_load(oracle, prime):
+    p = oracle.prime(prime)
+    L = oracle.L
+    r_inv = pow(2**32, p - 2, p)
+    rng = random.Random(20 + prime)
+    pairs = [(x, y) for x in _edges(p, 4 * p) for y in _edges(p, 4 * p)]
+    pairs += [(rng.randrange(4 * p), rng.randrange(4 * p)) for _ in range(20000)]
+    for x, y in pairs:
+        r = L.orc_narrow_pointwise(x, y, p)
+        assert r < 2 * p and (r - x * y * r_inv) % p == 0, (x, y)
+    for c in _edges(p, 2**32) + [4 * p - 1, 4 * p, 4 * p + 1, 2**32 - 1] + [rng.randrange(2**32) for _ in range(5000)]:
+        v = L.orc_narrow_load(c, p)
+        assert v < 4 * p and (v - c) % p == 0, c
+
+
+@pytest.mark.parametrize("prime", NARROW)
+@pytest.mark.parametrize("log2n,split", [(1, 1), (5, 2), (10, 4), (12, 5)])
+def test_lazy_transforms_match_strict(oracle, prime, log2n, split):
+    p = oracle.prime(prime)
+    rng = np.random.default_rng(prime * 100 + log2n)
+    x = rng.integers(0, 4 * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
+    strict = oracle.ct_layers((x.astype(np.uint64) % p).astype(np.uint32), log2n, prime, 0, log2n)
+    # column layers, then row layers: the four-step split of the engine
+    cols = oracle.narrow_layers(x, log2n, prime, 0, split)
+    assert int(cols.max()) < 4 * p
+    lazy = oracle.narrow_layers(cols, log2n, prime, split, log2n)
+    assert int(lazy.max()) < 4 * p
+    assert np.array_equal(lazy.astype(np.uint64) % p, strict.astype(np.uint64))
+
+    y = rng.integers(0, 2 * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
+    strict_inv = oracle.gs_layers((y.astype(np.uint64) % p).astype(np.uint32), log2n, prime, 0, log2n)
+    rows = oracle.narrow_layers(y, log2n, prime, split, log2n, inverse=True)
+    assert int(rows.max()) < 2 * p
+    lazy_inv = oracle.narrow_layers(rows, log2n, prime, 0, split, inverse=True)
+    assert int(lazy_inv.max()) < 2 * p
+    assert np.array_equal(lazy_inv.astype(np.uint64) % p, strict_inv.astype(np.uint64))
+
+
+def test_out_of_contract_inputs_are_caught(oracle):
+    """The range checks have teeth: forward inputs at or above 4p break them."""
+    p = oracle.prime(3)
+    bad = 0
+    for a in range(4 * p, 2**32, 7919 * 4096):
+        A, B = oracle.narrow_ct(a, 0, 1, p)
+        if not (A < 4 * p and B < 4 * p and (A - a) % p == 0 and (B - a) % p == 0):
+            bad += 1
+    assert bad > 0

@@ -23,7 +23,8 @@ import pytest
 import pi_compute_amd as pc
 
 ROOT = Path(__file__).resolve().parent.parent
-FACTORS = {0: (2, 3, 5), 1: (2, 3), 2: (2, 3, 7)}  # prime factors of p - 1
+# Prime factors of p - 1; primes 0-2 form the wide set, 3-5 the narrow set.
+FACTORS = {0: (2, 3, 5), 1: (2, 3), 2: (2, 3, 7), 3: (2, 7, 17), 4: (2, 5, 47), 5: (2, 3, 5)}
 
 
 def _is_prime(n):
@@ -49,12 +50,16 @@ def _is_prime(n):
     return True
 
 
-@pytest.mark.parametrize("i", range(3))
+@pytest.mark.parametrize("i", range(6))
 def test_prime_set(oracle, i):
     L = oracle.L
     p, g, k = L.orc_prime(i), L.orc_generator(i), L.orc_two_adicity(i)
     assert _is_prime(p) and p < 2**31 and 2 * p < 2**32
-    assert (p - 1) % (1 << k) == 0 and ((p - 1) >> k) % 2 == 1 and k >= 25
+    assert (p - 1) % (1 << k) == 0 and ((p - 1) >> k) % 2 == 1
+    if i < 3:
+        assert k >= 25
+    else:  # narrow set: lazy butterflies need 4p below 2^32
+        assert k >= 22 and p < 2**30 and 4 * p < 2**32
     # g generates Z_p^*: g^((p-1)/q) != 1 for every prime q | p - 1
     m = p - 1
     for q in FACTORS[i]:
@@ -66,21 +71,25 @@ def test_prime_set(oracle, i):
 
 
 def test_crt_modulus_bounds_the_convolution(oracle):
+    # N * (2^32 - 1)^2 < M at each set's longest transform: every coefficient is
+    # reconstructed exactly
     M = oracle.prime(0) * oracle.prime(1) * oracle.prime(2)
-    # N * (2^32 - 1)^2 < M for N = 2^25: every coefficient is reconstructed exactly
     assert (1 << 25) * (2**32 - 1) ** 2 < M
+    M = oracle.prime(3) * oracle.prime(4) * oracle.prime(5)
+    assert (1 << 22) * (2**32 - 1) ** 2 < M
+    assert [oracle.L.orc_set_for(k) for k in (1, 22, 23, 25)] == [1, 1, 0, 0]
 
 
-@pytest.mark.parametrize("i", range(3))
+@pytest.mark.parametrize("i", range(6))
 def test_roots_of_unity(oracle, i):
     p = oracle.prime(i)
-    for k in (1, 2, 10, 25):
+    for k in (1, 2, 10, 25 if i < 3 else 22):
         w = oracle.L.orc_root(i, k)
         assert pow(w, 1 << k, p) == 1
         assert pow(w, 1 << (k - 1), p) == p - 1  # primitive
 
 
-@pytest.mark.parametrize("i", range(3))
+@pytest.mark.parametrize("i", range(6))
 def test_twiddle_tree(oracle, i):
     p, n = oracle.prime(i), 10
     z, zi = oracle.zeta(i, n)
@@ -101,7 +110,7 @@ def _bitrev(x, bits):
     return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
 
 
-@pytest.mark.parametrize("i", range(3))
+@pytest.mark.parametrize("i", range(6))
 @pytest.mark.parametrize("log2n", [1, 2, 3, 6, 9])
 def test_forward_is_dft_in_bitreversed_order(oracle, i, log2n):
     p, n = oracle.prime(i), 1 << log2n
@@ -115,7 +124,7 @@ def test_forward_is_dft_in_bitreversed_order(oracle, i, log2n):
 
 
 def test_delta_transforms_to_ones(oracle):
-    for i in range(3):
+    for i in range(6):
         x = np.zeros(64, np.uint32)
         x[0] = 1
         assert np.all(oracle.ct_layers(x, 6, i, 0, 6) == 1)
@@ -135,7 +144,7 @@ def test_split_layers_compose(oracle):
 
 def test_convolution_theorem_vs_schoolbook(oracle):
     rng = random.Random(11)
-    for i in range(3):
+    for i in range(6):
         p = oracle.prime(i)
         n = 32
         a = [rng.randrange(p) for _ in range(12)] + [0] * 20
@@ -149,13 +158,14 @@ def test_convolution_theorem_vs_schoolbook(oracle):
         assert [int(v) * ninv % p for v in c] == school
 
 
-def test_crt_against_integer_reconstruction(oracle):
-    P = [oracle.prime(i) for i in range(3)]
+@pytest.mark.parametrize("prime_set", [0, 1])
+def test_crt_against_integer_reconstruction(oracle, prime_set):
+    P = [oracle.prime(3 * prime_set + i) for i in range(3)]
     M = P[0] * P[1] * P[2]
     rng = random.Random(5)
     xs = [0, 1, M - 1, M // 2, 2**89 - 1, (1 << 64) + 12345] + [rng.randrange(M) for _ in range(300)]
     for x in xs:
-        assert oracle.crt3([x % P[0], x % P[1], x % P[2]]) == x
+        assert oracle.crt3([x % P[0], x % P[1], x % P[2]], prime_set) == x
 
 
 def _py_mul(a, b):
