@@ -113,8 +113,8 @@ int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs);  /* 
 int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
                              const size_t* yn, const int* subtract, uint64_t* const* d_out);
 
-/* Wait for all enqueued products; PINTT_ERR_HIP if the device reported a
- * failed carry look-back. */
+/* Wait for all enqueued products (PINTT_ERR_HIP if the stream reports an
+ * error). */
 int gpu_ntt_synchronize(void);
 
 /* Very unbalanced operands: the longer one may be cut into up to 4 chunks that

@@ -1,17 +1,17 @@
 // engine.hip - host side of the C ABI (include/pintt_gpu.h).
 //
-// One device context per process: a stream, the twiddle tables (one per
-// prime, grown to the largest transform seen), a residue arena, the carry
-// look-back state, descriptor buffers and staging buffers for host operands.
+// One device context per process: a stream, the twiddle tables (per prime of
+// each prime set, built when the set is first needed), a residue arena, the
+// carry maps, descriptor buffers and staging buffers for host operands.
 //
 // Every call is a batch (a single product is a batch of one):
 //   1. each product becomes a group: the longer operand is cut into m chunks
 //      when that makes the transforms cheaper (unbalanced operands), all
 //      sharing the one transform of the shorter operand;
 //   2. groups are bucketed by transform length N;
-//   3. per bucket, four launches (forward columns, row convolution, inverse
-//      columns, CRT + carry) process every group of the bucket, driven by job
-//      descriptors uploaded once per batch.
+//   3. per bucket, five launches (forward columns, row convolution, inverse
+//      columns, CRT + carry, carry fixup) process every group of the bucket,
+//      driven by job descriptors uploaded once per batch.
 // Calls are serialised by a mutex; work enqueued on a caller's stream is
 // ordered behind the previous batch with an event.
 //
@@ -52,11 +52,8 @@ struct Engine {
     uint2* tw_inv[kSets][kPrimes] = {};
     size_t arena_words = 0;
     uint32_t* arena = nullptr;
-    CarryGlobal* glob = nullptr;
-    size_t ticket_cap = 0;
-    unsigned long long* tickets = nullptr;
-    size_t status_cap = 0;
-    uint64_t* status = nullptr;
+    size_t maps_cap = 0;
+    uint32_t* maps = nullptr;  // carry maps, one per carry workgroup
     // Ring of descriptor buffers, so the host can prepare a few batches ahead:
     // slot k is reused only after its previous upload has completed.
     static constexpr int kDescRing = 8;
@@ -174,8 +171,6 @@ int ensure_init(int device) {
     PINTT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
     for (auto& e : g.desc_done) PINTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : g.ev) PINTT_HIP(hipEventCreate(&e));
-    PINTT_HIP(hipMalloc(reinterpret_cast<void**>(&g.glob), sizeof(CarryGlobal)));
-    PINTT_HIP(hipMemset(g.glob, 0, sizeof(CarryGlobal)));
     const char* at = std::getenv("PINTT_AUTOTUNE");
     g.autotune = !(at && at[0] == '0');
     const char* sp = std::getenv("PINTT_SPLIT");
@@ -223,17 +218,12 @@ int ensure_arena(size_t words) {
     return PINTT_OK;
 }
 
-// Look-back state: zero when allocated; stale words never match a later epoch.
-int ensure_carry_state(size_t groups, size_t blocks) {
-    if (groups > g.ticket_cap) {
-        PINTT_TRY(grow(g.tickets, groups));
-        PINTT_HIP(hipMemset(g.tickets, 0, groups * sizeof(unsigned long long)));
-        g.ticket_cap = groups;
-    }
-    if (blocks > g.status_cap) {
-        PINTT_TRY(grow(g.status, blocks));
-        PINTT_HIP(hipMemset(g.status, 0, blocks * sizeof(uint64_t)));
-        g.status_cap = blocks;
+// Carry maps: every map a fixup reads is written earlier in the same launch
+// pair, so the buffer needs no clearing.
+int ensure_carry_state(size_t blocks) {
+    if (blocks > g.maps_cap) {
+        PINTT_TRY(grow(g.maps, blocks));
+        g.maps_cap = blocks;
     }
     return PINTT_OK;
 }
@@ -465,11 +455,10 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
     return PINTT_OK;
 }
 
-// Write the descriptors of `bp` into H, pointing at the given arena, tickets
-// and status words.
-void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, unsigned long long* tickets,
-                      uint64_t* status) {
-    size_t ticket_next = 0, status_next = 0;
+// Write the descriptors of `bp` into H, pointing at the given arena and carry
+// maps.
+void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, uint32_t* maps) {
+    size_t maps_next = 0;
     for (const auto& b : bp.buckets) {
         const size_t n = (size_t)1 << b.log2n;
         const size_t slot = kPrimes * n;
@@ -491,7 +480,6 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, un
             const Request& r = gr.r;
             CarryJob car{};
             uint32_t* bslot = r.square ? nullptr : arena + slot * (next_slot + gr.m);
-            unsigned long long* ticket = tickets + ticket_next++;
             const size_t first = next_slot;
             const bool pair = b.pair && !r.square && gr.m >= 2;
             for (uint32_t i = 0; i < gr.m; ++i) {
@@ -514,7 +502,7 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, un
                     const size_t li = launch_of(kRowReuseB, i);
                     rj[li][rfill[li]++] = RowJob{dst, bslot, nullptr, 0};
                 }
-                ij[inv++] = InvJob{dst, i == 0 ? ticket : nullptr};
+                ij[inv++] = InvJob{dst};
                 car.res[i] = dst;
                 car.off[i] = (uint32_t)(2 * lo);
                 car.len[i] = (uint32_t)std::min(n, 2 * limbs + 2 * r.bn - 1);
@@ -527,11 +515,10 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, un
             car.blk0 = blk0;
             car.out = reinterpret_cast<uint32_t*>(r.out);
             car.nwords = 2 * (r.an + r.bn);
-            car.ticket = ticket;
-            car.status = status + status_next;
+            car.maps = maps + maps_next;
             cj[k] = car;
             blk0 += (uint32_t)gr.nblk;
-            status_next += gr.nblk;
+            maps_next += gr.nblk;
         }
     }
 }
@@ -580,12 +567,12 @@ int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned
         for (uint32_t k = 0; k < b.ninv; k += kMaxJobs) {
             const uint32_t c = std::min(kMaxJobs, b.ninv - k);
             PINTT_HIP(launch_inverse_columns(pl, list((InvJob*)nullptr, b.inv_off + k * sizeof(InvJob), c), c,
-                                             g.glob, tw, s));
+                                             tw, s));
         }
         if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
         const uint32_t ng = (uint32_t)b.groups.size();
         PINTT_HIP(launch_crt_carry(list((CarryJob*)nullptr, b.carry_off, ng), ng, b.nblocks, b.log2n,
-                                   crt_constants(b.log2n), g.glob, s));
+                                   crt_constants(b.log2n), s));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
     }
     return PINTT_OK;
@@ -599,19 +586,19 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
     PINTT_TRY(ensure_twiddles(bp.min_log2n));
     PINTT_TRY(ensure_twiddles(bp.max_log2n));
     PINTT_TRY(ensure_arena(bp.arena_words));
-    PINTT_TRY(ensure_carry_state(bp.total_groups, bp.total_blocks));
+    PINTT_TRY(ensure_carry_state(bp.total_blocks));
     if (fits_inline(bp)) {
         // Every job list rides in kernel arguments: nothing to upload.
         std::vector<unsigned char> H(bp.bytes);
-        fill_descriptors(bp, H.data(), g.arena, g.tickets, g.status);
-        // Ordered after the previous batch (arena, tickets and epoch are shared).
+        fill_descriptors(bp, H.data(), g.arena, g.maps);
+        // Ordered after the previous batch (arena and carry maps are shared).
         if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
         PINTT_TRY(enqueue_launches(bp, nullptr, H.data(), s, g.timing));
     } else {
         const int ring = g.desc_next;
         g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
         PINTT_TRY(ensure_descriptors(ring, bp.bytes));
-        fill_descriptors(bp, g.desc_host[ring], g.arena, g.tickets, g.status);
+        fill_descriptors(bp, g.desc_host[ring], g.arena, g.maps);
         PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bp.bytes, hipMemcpyHostToDevice, g.stream));
         PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
         // A caller stream first waits for the upload, which itself is ordered
@@ -632,8 +619,7 @@ struct BatchGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t* arena = nullptr;
-    unsigned long long* tickets = nullptr;
-    uint64_t* status = nullptr;
+    uint32_t* maps = nullptr;
     unsigned char* desc = nullptr;
     size_t nodes = 0;
 };
@@ -642,8 +628,7 @@ void free_graph(BatchGraph* bg) {
     if (bg->exec) (void)hipGraphExecDestroy(bg->exec);
     if (bg->graph) (void)hipGraphDestroy(bg->graph);
     (void)hipFree(bg->arena);
-    (void)hipFree(bg->tickets);
-    (void)hipFree(bg->status);
+    (void)hipFree(bg->maps);
     (void)hipFree(bg->desc);
     delete bg;
 }
@@ -659,15 +644,11 @@ int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
         return rc;
     };
     if (hipMalloc(reinterpret_cast<void**>(&bg->arena), std::max<size_t>(bp.arena_words, 1) * 4) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&bg->tickets), bp.total_groups * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&bg->status), std::max<size_t>(bp.total_blocks, 1) * 8) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&bg->maps), std::max<size_t>(bp.total_blocks, 1) * 4) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&bg->desc), bp.bytes) != hipSuccess)
         return cleanup(fail(PINTT_ERR_NO_MEMORY, "graph buffers"));
-    if (hipMemset(bg->tickets, 0, bp.total_groups * sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(bg->status, 0, std::max<size_t>(bp.total_blocks, 1) * 8) != hipSuccess)
-        return cleanup(fail(PINTT_ERR_HIP, "graph state init"));
     std::vector<unsigned char> H(bp.bytes);
-    fill_descriptors(bp, H.data(), bg->arena, bg->tickets, bg->status);
+    fill_descriptors(bp, H.data(), bg->arena, bg->maps);
     if (hipMemcpy(bg->desc, H.data(), bp.bytes, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(PINTT_ERR_HIP, "graph descriptor upload"));
     // Capture the launches on a private stream (nothing else runs on it).
@@ -810,11 +791,8 @@ void parallel_copy(const std::vector<Piece>& pieces, size_t total) {
     for (auto& t : th) t.join();
 }
 
-int check_device_error() {
-    uint32_t err = 0;
-    PINTT_HIP(hipMemcpyAsync(&err, &g.glob->error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
+int wait_stream() {
     PINTT_HIP(hipStreamSynchronize(g.stream));
-    if (err) return fail(PINTT_ERR_HIP, "carry look-back gave up (device state corrupted)");
     return PINTT_OK;
 }
 
@@ -868,7 +846,7 @@ int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const u
     PINTT_HIP(hipMemcpyAsync(g.io_in, g.pin_in, total_in * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
     PINTT_TRY(enqueue_batch(reqs, g.stream));
     PINTT_HIP(hipMemcpyAsync(g.pin_out, g.io_out, total_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
-    PINTT_TRY(check_device_error());
+    PINTT_TRY(wait_stream());
     parallel_copy(scatter, total_out);
     return PINTT_OK;
 }
@@ -896,9 +874,7 @@ void gpu_ntt_shutdown(void) {
             (void)hipFree(g.tw_inv[set][i]);
         }
     (void)hipFree(g.arena);
-    (void)hipFree(g.glob);
-    (void)hipFree(g.tickets);
-    (void)hipFree(g.status);
+    (void)hipFree(g.maps);
     for (int k = 0; k < Engine::kDescRing; ++k) {
         (void)hipFree(g.desc_dev[k]);
         if (g.desc_host[k]) (void)hipHostFree(g.desc_host[k]);
@@ -986,7 +962,7 @@ int gpu_ntt_graph_launch(void* graph, void* stream) {
     PINTT_TRY(ensure_init(-1));
     BatchGraph* bg = static_cast<BatchGraph*>(graph);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
-    // Serialised with every other product: all share the carry epoch.
+    // Serialised with every other batch on the device.
     if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
     PINTT_HIP(hipGraphLaunch(bg->exec, s));
     PINTT_HIP(hipEventRecord(g.done, s));
@@ -1086,7 +1062,7 @@ int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const
         }
         i = j;
     }
-    PINTT_TRY(check_device_error());
+    PINTT_TRY(wait_stream());
     parallel_copy(scatter, total);
     return PINTT_OK;
 }
@@ -1171,7 +1147,7 @@ void gpu_ntt_graph_destroy(void* graph) {
 int gpu_ntt_synchronize(void) {
     std::lock_guard<std::mutex> lock(g_mu);
     if (!g.ready) return PINTT_OK;
-    return check_device_error();
+    return wait_stream();
 }
 
 int gpu_ntt_transform_log2(size_t an, size_t bn) {

@@ -4,7 +4,7 @@
 namespace pintt {
 namespace {
 
-// Pass 4: CRT and carry propagation in one pass, per group.
+// Pass 4: CRT and carry propagation, per group, in two kernels.
 //
 // Word j of a group's product before carries is the sum, over its chunk
 // products i, of X0_i[k] + X1_i[k-1] + X2_i[k-2] with k = j - off_i, where
@@ -15,13 +15,19 @@ namespace {
 // carry-out for carry-in 0, and a larger carry-in adds exactly one more carry
 // when it overflows the run's first word while every later word is all ones
 // (t = 8: never).  These step maps are closed under composition and compose
-// with a few compares: a wave-shuffle scan inside
-// the block, then a decoupled look-back across the group's blocks (numbered in
-// the order they start, from the group's ticket counter, so a block only waits
-// for blocks that already run and publish their aggregate before waiting).
-// A status word carries epoch, flag and map in one 64-bit atomic, so relaxed
-// atomics suffice: release/acquire would add L2 write-backs and invalidations
-// on every publication and look-back round.
+// with a few compares.
+//
+// k_crt_carry: a workgroup owns kCarryWordsPerBlock words of one group.  It
+// reads the residues of every chunk product that reaches them with coalesced
+// 16-byte loads, writes their CRT values to LDS (three words per position),
+// and each thread then sums the 16 consecutive words it owns.  The threads'
+// carry maps are scanned across the workgroup; the words are written as if the
+// workgroup's carry-in were 0 and its own map goes to the group's map array.
+// k_carry_fixup: one wave per workgroup of the first kernel composes the maps
+// of the group's earlier workgroups into the true carry-in and adds it at the
+// workgroup's first word, rippling through words that are all ones, never past
+// the workgroup's last word (what spills out is already counted in the next
+// workgroup's carry-in).  No workgroup ever waits for another.
 
 // Map encoding: b in bits 0-2, t in bits 3-6; bit 7 marks the identity.
 constexpr uint32_t kFnIdentity = 1u << 7;
@@ -37,13 +43,6 @@ __device__ __forceinline__ uint32_t fn_compose(uint32_t later, uint32_t earlier)
     const uint32_t lo = fb + (gb >= ft ? 1u : 0u), hi = fb + (gb + 1 >= ft ? 1u : 0u);
     const uint32_t r = fn_make(lo, (gt >= 8 || lo == hi) ? 8u : gt);
     return (earlier & kFnIdentity) ? later : ((later & kFnIdentity) ? earlier : r);
-}
-
-constexpr uint64_t kFlagAggregate = 1, kFlagPrefix = 2;
-constexpr uint32_t kLookbackSpinLimit = 1u << 26;
-
-__device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint64_t flag, uint32_t fn) {
-    return ((uint64_t)epoch << 32) | (flag << 24) | fn;
 }
 
 // Inclusive scan of carry maps over the block in thread order.  Returns the
@@ -94,150 +93,119 @@ __device__ __forceinline__ void crt_words(uint32_t r0, uint32_t r1, uint32_t r2,
     x2 = (uint32_t)(hi >> 32);
 }
 
-// Ordered composition over the block of one map per thread, thread 0 being the
-// latest: returns f_0 o f_1 o ... o f_255 to every thread.
-__device__ __forceinline__ uint32_t block_compose_desc(uint32_t g, uint32_t* lds_waves) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Index of the group that owns carry workgroup `blk` (jobs sorted by blk0).
+__device__ __forceinline__ uint32_t find_job(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t blk) {
+    uint32_t lo = 0;  // last job with blk0 <= blk
+    if (jobs.ninl) {
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t o = __shfl_down(g, off, 64);
-        if (lane + off < 64) g = fn_compose(g, o);
-    }
-    if (lane == 0) lds_waves[wave] = g;
-    __syncthreads();
-    uint32_t r = kFnIdentity;
-#pragma unroll
-    for (int w = kCarryThreads / 64 - 1; w >= 0; --w) r = fn_compose(lds_waves[w], r);
-    __syncthreads();
-    return r;
-}
-
-// Residues of one chunk product that a thread needs: positions k0-2, k0-1 (the
-// words they spill into) and k0 .. k0+WT-1, three primes each; zero outside the
-// chunk product (residues 0 give x = 0).
-template <int WT>
-struct ChunkRes {
-    uint32_t h[2][3];
-    uint32_t r[3][WT];
-    bool any;  // some position in range
-};
-
-// Loads only, so that every chunk's loads are in flight before any CRT.
-// k0: position of the thread's first word inside the chunk product.
-template <int WT>
-__device__ __forceinline__ void load_chunk(ChunkRes<WT>& c, global_ptr<const uint32_t> res, size_t n, uint32_t len,
-                                           long long k0) {
-    c.any = !(k0 + WT <= 0 || k0 - 2 >= (long long)len);
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-        const long long k = k0 - 2 + d;
-        const bool in = c.any && k >= 0 && k < (long long)len;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) c.h[d][q] = in ? res[q * n + k] : 0u;
-    }
-    const bool full = c.any && k0 >= 0 && k0 + WT <= (long long)len;
-    if (full) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int v = 0; v < WT / 4; ++v) {
-                const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k0 + 4 * v);
-                c.r[q][4 * v] = x.x;
-                c.r[q][4 * v + 1] = x.y;
-                c.r[q][4 * v + 2] = x.z;
-                c.r[q][4 * v + 3] = x.w;
-            }
+        for (int i = 1; i < kInlineJobs; ++i)
+            if ((uint32_t)i < jobs.ninl && jobs.inl[i].blk0 <= blk) lo = i;
     } else {
-#pragma unroll
-        for (int e = 0; e < WT; ++e) {
-            const long long k = k0 + e;
-            const bool in = c.any && k >= 0 && k < (long long)len;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) c.r[q][e] = in ? res[q * n + k] : 0u;
+        uint32_t hi = njobs - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (jobs.dev[mid].blk0 <= blk) lo = mid;
+            else hi = mid - 1;
         }
     }
+    return lo;
 }
 
-// Adds the chunk product's word contributions X0[k] + X1[k-1] + X2[k-2] to the
-// thread's word sums s[0 .. WT).
-template <int WT, int SET>
-__device__ __forceinline__ void add_chunk_words(uint64_t* s, const ChunkRes<WT>& c, const CrtConst& cc) {
-    uint32_t xm1[3], xm2[3];  // x at positions k-1, k-2
-    crt_words<SET>(c.h[0][0], c.h[0][1], c.h[0][2], cc, xm2[0], xm2[1], xm2[2]);
-    crt_words<SET>(c.h[1][0], c.h[1][1], c.h[1][2], cc, xm1[0], xm1[1], xm1[2]);
-#pragma unroll
-    for (int e = 0; e < WT; ++e) {
-        uint32_t x0, x1, x2;
-        crt_words<SET>(c.r[0][e], c.r[1][e], c.r[2][e], cc, x0, x1, x2);
-        s[e] += (uint64_t)x0 + xm1[1] + xm2[2];
-        xm2[0] = xm1[0]; xm2[1] = xm1[1]; xm2[2] = xm1[2];
-        xm1[0] = x0; xm1[1] = x1; xm1[2] = x2;
-    }
+constexpr int kCT = kCarryThreads;
+constexpr int kWT = kCarryWordsPerThread;
+constexpr int kWB = kCarryWordsPerBlock;
+static_assert(kWB == 16 * kCT, "each thread loads four groups of four positions");
+// LDS slot of word j of the workgroup, j in [-2, kWB): the two words before the
+// workgroup first, then one pad word after every 16 so that the threads'
+// 16-word runs start on different banks.
+__device__ __forceinline__ uint32_t xslot(int j) {
+    const uint32_t i = (uint32_t)(j + 2);
+    return i + (i >> 4);
 }
+constexpr int kXWords = (kWB + 2) + ((kWB + 2) >> 4) + 1;
 
-// grid = total carry blocks of the launch; jobs sorted by blk0.  SET: the
-// prime set the residues belong to.
+// grid = carry workgroups of the launch; jobs sorted by blk0.  SET: the prime
+// set the residues belong to.
 template <int SET>
 __global__ void __launch_bounds__(kCarryThreads)
-    k_crt_carry(const JobList<CarryJob> jobs, uint32_t njobs, int log2n, CrtConst cc,
-                CarryGlobal* __restrict__ glob) {
-    constexpr int WB = kCarryWordsPerBlock;
-    constexpr int WT = kCarryWordsPerThread;
-    __shared__ uint32_t waves[kCarryThreads / 64];
-    __shared__ uint32_t sh_blk, sh_epoch, sh_job, sh_first_prefix, sh_first_wait;
-    if (threadIdx.x == 0) {
-        uint32_t lo = 0;  // last job with blk0 <= blockIdx.x
-        if (jobs.ninl) {
-#pragma unroll
-            for (int i = 1; i < kInlineJobs; ++i)
-                if ((uint32_t)i < jobs.ninl && jobs.inl[i].blk0 <= blockIdx.x) lo = i;
-        } else {
-            uint32_t hi = njobs - 1;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (jobs.dev[mid].blk0 <= blockIdx.x) lo = mid;
-                else hi = mid - 1;
-            }
-        }
-        sh_job = lo;
-        unsigned long long* ticket = jobs.ninl ? nullptr : jobs.dev[lo].ticket;
-        if (jobs.ninl) {
-#pragma unroll
-            for (int i = 0; i < kInlineJobs; ++i)
-                if ((uint32_t)i == lo) ticket = jobs.inl[i].ticket;
-        }
-        sh_blk = (uint32_t)__hip_atomic_fetch_add(as_global(ticket), 1ull, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        sh_epoch = __hip_atomic_load(&glob->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    k_crt_carry(const JobList<CarryJob> jobs, uint32_t njobs, int log2n, CrtConst cc) {
+    __shared__ uint32_t X[3][kXWords];
+    __shared__ uint32_t waves[kCT / 64];
+    __shared__ uint32_t sh_job;
+    if (threadIdx.x == 0) sh_job = find_job(jobs, njobs, blockIdx.x);
     __syncthreads();
     const CarryJob job = PINTT_JOB(CarryJob, jobs, sh_job);
-    const uint32_t blk = sh_blk, epoch = sh_epoch;
+    const uint32_t lb = blockIdx.x - job.blk0;  // workgroup within the group
+    const long long j0 = (long long)lb * kWB;   // its first word
     const size_t n = (size_t)1 << log2n;
-    const uint64_t j0 = (uint64_t)blk * WB + (uint64_t)WT * threadIdx.x;  // this thread's first word
-    const global_ptr<uint64_t> status = as_global(job.status);
+    const int t = threadIdx.x;
 
-    uint64_t s[WT];
+    uint64_t s[kWT];
 #pragma unroll
-    for (int e = 0; e < WT; ++e) s[e] = 0;
-    // Chunk products two at a time: both loads issued, then both CRTs.
+    for (int e = 0; e < kWT; ++e) s[e] = 0;
     // Constant chunk indices keep the job (a kernel-argument copy) in registers.
 #pragma unroll
-    for (uint32_t i = 0; i < (uint32_t)kMaxChunks; i += 2) {
-        if (i < job.m) {
-            ChunkRes<WT> c0, c1;
-            load_chunk<WT>(c0, as_global(job.res[i]), n, job.len[i], (long long)j0 - (long long)job.off[i]);
-            const bool two = i + 1 < job.m;
-            if (two)
-                load_chunk<WT>(c1, as_global(job.res[i + 1]), n, job.len[i + 1],
-                               (long long)j0 - (long long)job.off[i + 1]);
-            if (c0.any) add_chunk_words<WT, SET>(s, c0, cc);
-            if (two && c1.any) add_chunk_words<WT, SET>(s, c1, cc);
+    for (uint32_t i = 0; i < (uint32_t)kMaxChunks; ++i) {
+        if (i >= job.m) break;
+        const long long k0 = j0 - (long long)job.off[i];  // chunk position of word 0
+        const long long len = job.len[i];
+        if (k0 + kWB <= 0 || k0 - 2 >= len) continue;  // no position of this chunk reaches the words
+        const global_ptr<const uint32_t> res = as_global(job.res[i]);
+        // Positions k0 + 4t + 1024v + u (v, u < 4): all loads first, then the CRTs.
+        uint32_t r[4][3][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long long k = k0 + 4 * t + 4 * kCT * v;
+            if (k >= 0 && k + 4 <= len) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k);
+                    r[v][q][0] = x.x;
+                    r[v][q][1] = x.y;
+                    r[v][q][2] = x.z;
+                    r[v][q][3] = x.w;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool in = k + u >= 0 && k + u < len;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) r[v][q][u] = in ? res[q * n + k + u] : 0u;
+                }
+            }
         }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                uint32_t x0, x1, x2;
+                crt_words<SET>(r[v][0][u], r[v][1][u], r[v][2][u], cc, x0, x1, x2);
+                const uint32_t sl = xslot(4 * t + 4 * kCT * v + u);
+                X[0][sl] = x0;
+                X[1][sl] = x1;
+                X[2][sl] = x2;
+            }
+        if (t < 2) {  // the two positions before the workgroup's first word
+            const long long k = k0 - 2 + t;
+            const bool in = k >= 0 && k < len;
+            uint32_t x0, x1, x2;
+            crt_words<SET>(in ? res[k] : 0u, in ? res[n + k] : 0u, in ? res[2 * n + k] : 0u, cc, x0, x1, x2);
+            const uint32_t sl = xslot(t - 2);
+            X[0][sl] = x0;
+            X[1][sl] = x1;
+            X[2][sl] = x2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kWT; ++e) {
+            const int jj = kWT * t + e;
+            s[e] += (uint64_t)X[0][xslot(jj)] + X[1][xslot(jj - 1)] + X[2][xslot(jj - 2)];
+        }
+        __syncthreads();
     }
 #pragma unroll
-    for (int e = 0; e < WT; ++e)
-        if (j0 + e >= job.nwords) s[e] = 0;
+    for (int e = 0; e < kWT; ++e)
+        if ((uint64_t)(j0 + kWT * t + e) >= job.nwords) s[e] = 0;
 
     // Carry map of this thread's run (computed with carry-in 0).
     uint32_t f;
@@ -246,7 +214,7 @@ __global__ void __launch_bounds__(kCarryThreads)
         uint32_t w0 = 0;
         bool ones = true;
 #pragma unroll
-        for (int e = 0; e < WT; ++e) {
+        for (int e = 0; e < kWT; ++e) {
             uint64_t v = s[e] + c;
             uint32_t o = (uint32_t)v;
             c = v >> 32;
@@ -255,78 +223,81 @@ __global__ void __launch_bounds__(kCarryThreads)
         }
         // smallest carry-in that overflows word 0: 2^32 - w0 (8 stands for never)
         const uint32_t room = 0u - w0;  // 2^32 - w0 mod 2^32; 0 when w0 == 0
-        const uint32_t t = (ones && w0 != 0 && room <= 7u) ? room : 8u;
-        f = fn_make((uint32_t)c, t);
+        const uint32_t tt = (ones && w0 != 0 && room <= 7u) ? room : 8u;
+        f = fn_make((uint32_t)c, tt);
     }
     uint32_t agg;
-    const uint32_t excl = block_scan_fn<kCarryThreads>(f, waves, &agg);
-    __syncthreads();  // block_scan_fn's lds_waves are reused below
+    const uint32_t excl = block_scan_fn<kCT>(f, waves, &agg);
+    if (t == 0) as_global(job.maps)[lb] = agg;
 
-    // Look-back: each round, thread t reads the status of block hi - t; the maps
-    // up to the nearest published inclusive prefix are composed in order.
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&status[blk], status_word(epoch, blk == 0 ? kFlagPrefix : kFlagAggregate, agg),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t before = kFnIdentity;  // composition of every earlier block's map
-    long long hi = (long long)blk - 1;
-    uint32_t spins = 0;
-    while (hi >= 0) {
-        const long long idx = hi - (long long)threadIdx.x;
-        uint32_t flag = (uint32_t)kFlagPrefix, fm = kFnIdentity;  // before block 0: identity
-        if (idx >= 0) {
-            uint64_t v = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            flag = ((uint32_t)(v >> 32) == epoch) ? (uint32_t)((v >> 24) & 0xFF) : 0u;
-            fm = (uint32_t)(v & 0xFFFFFF);
-        }
-        if (threadIdx.x == 0) {
-            sh_first_prefix = kCarryThreads;
-            sh_first_wait = kCarryThreads;
-        }
-        __syncthreads();
-        if (flag == (uint32_t)kFlagPrefix) atomicMin(&sh_first_prefix, threadIdx.x);
-        if (flag == 0) atomicMin(&sh_first_wait, threadIdx.x);
-        __syncthreads();
-        const uint32_t first_prefix = sh_first_prefix, first_wait = sh_first_wait;
-        __syncthreads();
-        if (first_wait <= first_prefix && first_wait < kCarryThreads) {
-            if (++spins >= kLookbackSpinLimit) {  // cannot happen while blocks make progress
-                if (threadIdx.x == 0) glob->error = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        const uint32_t g = threadIdx.x <= first_prefix ? fm : kFnIdentity;
-        before = fn_compose(before, block_compose_desc(g, waves));
-        if (first_prefix < kCarryThreads) break;
-        hi -= kCarryThreads;
-    }
-    if (threadIdx.x == 0 && blk > 0)
-        __hip_atomic_store(&status[blk], status_word(epoch, kFlagPrefix, fn_compose(agg, before)),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-    // Final words with the thread's true carry-in.
-    uint64_t c = fn_apply(excl, fn_apply(before, 0));
-    uint32_t w[WT];
+    // Words with the workgroup's carry-in taken as 0, through LDS so that the
+    // stores are coalesced.
+    {
+        uint64_t c = fn_apply(excl, 0);
 #pragma unroll
-    for (int e = 0; e < WT; ++e) {
-        uint64_t v = s[e] + c;
-        w[e] = (uint32_t)v;
-        c = v >> 32;
+        for (int e = 0; e < kWT; ++e) {
+            const uint64_t v = s[e] + c;
+            X[0][xslot(kWT * t + e)] = (uint32_t)v;
+            c = v >> 32;
+        }
     }
+    __syncthreads();
     const global_ptr<uint32_t> out = as_global(job.out);
-    if (j0 + WT <= job.nwords) {
 #pragma unroll
-        for (int q = 0; q < WT / 4; ++q)
-            *(global_ptr<u32x4>)(out + j0 + 4 * q) = u32x4{w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]};
-    } else {
+    for (int v = 0; v < 4; ++v) {
+        const int jj = 4 * t + 4 * kCT * v;
+        const uint64_t j = (uint64_t)j0 + jj;
+        if (j + 4 <= job.nwords) {
+            *(global_ptr<u32x4>)(out + j) =
+                u32x4{X[0][xslot(jj)], X[0][xslot(jj + 1)], X[0][xslot(jj + 2)], X[0][xslot(jj + 3)]};
+        } else {
 #pragma unroll
-        for (int e = 0; e < WT; ++e)
-            if (j0 + e < job.nwords) out[j0 + e] = w[e];
+            for (int u = 0; u < 4; ++u)
+                if (j + u < job.nwords) out[j + u] = X[0][xslot(jj + u)];
+        }
     }
 }
 
+// One wave per carry workgroup: its true carry-in, rippled into its words.
+__global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs, uint32_t njobs) {
+    const uint32_t lane = threadIdx.x;
+    const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
+    const uint32_t lb = blockIdx.x - job.blk0;
+    if (lb == 0) return;
+    // Maps of workgroups 0 .. lb-1 in order, a contiguous share per lane.
+    const global_ptr<const uint32_t> maps = as_global(job.maps);
+    const uint32_t per = (lb + 63) / 64;
+    const uint32_t lo = min(lane * per, lb), hi = min(lo + per, lb);
+    uint32_t acc = kFnIdentity;
+    for (uint32_t i = lo; i < hi; ++i) acc = fn_compose(maps[i], acc);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {  // lane 0 ends with lane 63's o ... o lane 0's
+        const uint32_t other = __shfl_down(acc, off, 64);
+        if (lane + off < 64) acc = fn_compose(other, acc);
+    }
+    uint32_t c = fn_apply(__shfl(acc, 0, 64), 0);
+    const global_ptr<uint32_t> out = as_global(job.out);
+    const uint64_t j0 = (uint64_t)lb * kWB;
+    const uint64_t jend = min(j0 + (uint64_t)kWB, job.nwords);
+    for (uint64_t j = j0; c != 0 && j < jend; j += 64) {
+        const uint64_t jw = j + lane;
+        const bool in = jw < jend;
+        const uint32_t w = in ? out[jw] : 0u;
+        // Word j takes c; a carry out of it runs through the all-ones words after it.
+        const uint32_t w0 = __shfl(w, 0, 64) + c;
+        const bool over = w0 < c;
+        const uint64_t stop = __ballot(in && lane > 0 && w != 0xFFFFFFFFu);
+        const uint32_t first = stop ? (uint32_t)__builtin_ctzll(stop) : 64u;  // first word the carry ends in
+        if (lane == 0) out[jw] = w0;
+        if (over && in && lane > 0 && lane <= first) out[jw] = lane < first ? 0u : w + 1u;
+        c = (over && first == 64u) ? 1u : 0u;
+    }
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
+namespace {
 __device__ __forceinline__ uint32_t powmod_dev(uint32_t b, uint32_t e, uint32_t p) {
     uint64_t r = 1, x = b;
     while (e) {
@@ -377,13 +348,15 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int set, int prime, hip
 }
 
 hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
-                            const CrtConst& cc, CarryGlobal* glob, hipStream_t s) {
+                            const CrtConst& cc, hipStream_t s) {
     if (set_for(log2n) == kSetNarrow)
         hipLaunchKernelGGL(k_crt_carry<kSetNarrow>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n,
-                           cc, glob);
+                           cc);
     else
-        hipLaunchKernelGGL(k_crt_carry<kSetWide>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc,
-                           glob);
+        hipLaunchKernelGGL(k_crt_carry<kSetWide>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_carry_fixup, dim3(nblocks), dim3(64), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 

@@ -1,8 +1,7 @@
 // ntt_cols.hip - column passes of the four-step NTT (gfx950).
 //   k_col_fwd  limbs -> residues (unpack + reduce), log2(R) forward layers per
 //              column, every operand chunk of a bucket in one launch
-//   k_col_inv  log2(R) inverse layers per column, in place; also arms the
-//              CRT/carry pass that follows
+//   k_col_inv  log2(R) inverse layers per column, in place
 #include "ntt_device.h"
 
 namespace pintt {
@@ -50,19 +49,14 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 }
 
 // Pass 3: inverse column layers, one prime of one chunk product per workgroup,
-// in place.  grid = (column tiles, prime, job).  It also arms the carry pass
-// that follows on the stream: a new epoch, fresh tickets for every group.
+// in place.  grid = (column tiles, prime, job).
 template <int LOGR, int LOGW, int SET>
 __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
-    k_col_inv(const JobList<InvJob> jobs, int log2c, Twiddles tw, CarryGlobal* __restrict__ glob) {
+    k_col_inv(const JobList<InvJob> jobs, int log2c, Twiddles tw) {
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
     __shared__ uint2 zt[TL::L];  // inverse twiddles of the column layers
     const InvJob job = PINTT_JOB(InvJob, jobs, blockIdx.z);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        if (job.arm) *job.arm = 0;
-        if (blockIdx.z == 0) glob->epoch = glob->epoch + 1;
-    }
     const int prime = blockIdx.y;
     const uint32_t p = prime_of<SET>(prime);
     global_ptr<uint32_t> buf = as_global(job.buf) + ((size_t)prime << (LOGR + log2c));
@@ -127,13 +121,13 @@ hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, u
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, CarryGlobal* glob,
-                                  const Twiddles& tw, hipStream_t s) {
+hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, const Twiddles& tw,
+                                  hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
 #define PINTT_CASE_SET(LR, LW, SET)                                                                  \
     if (pl.log2r == LR && pl.log2w == LW) {                                                          \
         hipLaunchKernelGGL((k_col_inv<LR, LW, SET>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs, \
-                           pl.log2c, tw, glob);                                                      \
+                           pl.log2c, tw);                                                            \
         return hipGetLastError();                                                                    \
     }
 #define PINTT_CASE_N(LR, LW) PINTT_CASE_SET(LR, LW, kSetNarrow)

@@ -46,7 +46,7 @@ struct CrtConst {
 
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
-constexpr int kCarryThreads = 512;
+constexpr int kCarryThreads = 256;
 constexpr int kCarryWordsPerThread = 16;
 constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
 constexpr int kMaxChunks = 4;  // pieces the longer operand of one product may be cut into
@@ -80,7 +80,6 @@ struct RowJob {
 
 struct InvJob {             // inverse column pass of one chunk product, in place
     uint32_t* buf;
-    unsigned long long* arm;  // the group's carry ticket (reset here), or null
 };
 
 struct CarryJob {           // CRT + carry of one group
@@ -91,16 +90,7 @@ struct CarryJob {           // CRT + carry of one group
     uint32_t blk0;                    // first carry block of the group in this launch
     uint32_t* out;                    // product words
     uint64_t nwords;
-    unsigned long long* ticket;       // block tickets of the group
-    uint64_t* status;                 // one look-back word per carry block of the group
-};
-
-// Device-wide state of the carry pass: `epoch` tags look-back status words and
-// is advanced on the device by every inverse column launch (so a captured graph
-// can be replayed); `error` is set if a look-back ever gave up.
-struct CarryGlobal {
-    uint32_t epoch;
-    uint32_t error;
+    uint32_t* maps;                   // one carry map per carry workgroup of the group
 };
 
 // Row/column split and tile shapes for a transform of length 2^n.
@@ -196,10 +186,9 @@ hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, u
                                   hipStream_t s);
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,
                                   const Twiddles& tw, hipStream_t s);
-// Also advances glob->epoch and resets the tickets the jobs name.
-hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, CarryGlobal* glob,
-                                  const Twiddles& tw, hipStream_t s);
+hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, const Twiddles& tw,
+                                  hipStream_t s);
 hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
-                            const CrtConst& cc, CarryGlobal* glob, hipStream_t s);
+                            const CrtConst& cc, hipStream_t s);
 
 }  // namespace pintt
