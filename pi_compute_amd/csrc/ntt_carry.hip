@@ -264,18 +264,26 @@ __global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs
     const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
     const uint32_t lb = blockIdx.x - job.blk0;
     if (lb == 0) return;
-    // Maps of workgroups 0 .. lb-1 in order, a contiguous share per lane.
+    // Maps of workgroups 0 .. lb-1 in order, 1024 per round: 16 consecutive
+    // maps per lane, all loaded before any is composed.
     const global_ptr<const uint32_t> maps = as_global(job.maps);
-    const uint32_t per = (lb + 63) / 64;
-    const uint32_t lo = min(lane * per, lb), hi = min(lo + per, lb);
-    uint32_t acc = kFnIdentity;
-    for (uint32_t i = lo; i < hi; ++i) acc = fn_compose(maps[i], acc);
+    uint32_t total = kFnIdentity;  // composition of the maps of the rounds so far
+    for (uint32_t base = 0; base < lb; base += 1024) {
+        const uint32_t lo = base + 16 * lane;
+        uint32_t m[16];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {  // lane 0 ends with lane 63's o ... o lane 0's
-        const uint32_t other = __shfl_down(acc, off, 64);
-        if (lane + off < 64) acc = fn_compose(other, acc);
+        for (int i = 0; i < 16; ++i) m[i] = lo + i < lb ? maps[lo + i] : kFnIdentity;
+        uint32_t acc = kFnIdentity;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc = fn_compose(m[i], acc);
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {  // lane 0 ends with lane 63's o ... o lane 0's
+            const uint32_t other = __shfl_down(acc, off, 64);
+            if (lane + off < 64) acc = fn_compose(other, acc);
+        }
+        total = fn_compose(__shfl(acc, 0, 64), total);
     }
-    uint32_t c = fn_apply(__shfl(acc, 0, 64), 0);
+    uint32_t c = fn_apply(total, 0);
     const global_ptr<uint32_t> out = as_global(job.out);
     const uint64_t j0 = (uint64_t)lb * kWB;
     const uint64_t jend = min(j0 + (uint64_t)kWB, job.nwords);
