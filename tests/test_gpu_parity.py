@@ -172,10 +172,11 @@ def test_every_compiled_tile_plan(engine, oracle, log2n):
     want = oracle.gmp_mul(a, b)
     want_sq = oracle.gmp_mul(b, b) if log2n <= 20 else None
     # A product cut into two chunks whose chunk products have length 2^log2n:
-    # runs the paired row kernels (two chunks sharing b) of the plan.
+    # runs the paired row kernels (two chunks sharing b) of the plan, over the
+    # narrow prime set up to 2^22 and the wide set at 2^23.
     h, bn2 = total // 2, total // 3
     a2, b2 = pc.operands(2000 + log2n, 64 * 2 * h, 64 * bn2)
-    want2 = oracle.gmp_mul(a2, b2) if log2n <= 22 else None
+    want2 = oracle.gmp_mul(a2, b2) if log2n <= 23 else None
     saved = engine.get_plan(log2n)
     try:
         for (r, w, rows) in plans:
@@ -243,6 +244,20 @@ def test_batch_mixed_sizes_and_squares(engine, oracle):
     outs = engine.mul_batch(pairs)
     for i, ((a, b), got) in enumerate(zip(pairs, outs)):
         assert np.array_equal(got, oracle.gmp_mul(a, b)), f"batch member {i}"
+
+
+def test_batch_spans_both_prime_sets(engine, oracle):
+    """One batch with transforms of 2^12 .. 2^22 points (narrow primes) and of
+    2^23 points (wide primes): both twiddle sets and CRT constant sets in one call."""
+    pairs = [pc.operands(31, 64 * 700, 64 * 900), pc.operands(32, 64 * (3 << 19), 64 * ((1 << 19) + 5)),
+             pc.operands(33, 64 * (1 << 20), 64 * (1 << 20) - 64)]
+    pairs.append((pairs[0][0], pairs[0][0]))
+    want_lengths = [12, 23, 22, 12]
+    for (a, b), k in zip(pairs, want_lengths):
+        assert engine.transform_log2(a.size, b.size) == k
+    outs = engine.mul_batch(pairs)
+    for i, ((a, b), got) in enumerate(zip(pairs, outs)):
+        assert np.array_equal(got, oracle.gmp_mul(a, b)), f"batch member {i} (2^{want_lengths[i]})"
 
 
 def test_batch_device_pointers(engine, oracle):
