@@ -20,9 +20,10 @@ namespace {
 // k_crt_carry: a workgroup owns kCarryWordsPerBlock words of one group.  It
 // reads the residues of every chunk product that reaches them with coalesced
 // 16-byte loads, writes their CRT values to LDS (three words per position),
-// and each thread then sums the 16 consecutive words it owns.  The threads'
-// carry maps are scanned across the workgroup; the words are written as if the
-// workgroup's carry-in were 0 and its own map goes to the group's map array.
+// and each thread then sums the kCarryWordsPerThread consecutive words it
+// owns.  The threads' carry maps are scanned across the workgroup; the words
+// are written as if the workgroup's carry-in were 0, and its own map goes to
+// the group's map array.
 // k_carry_fixup: one wave per workgroup of the first kernel composes the maps
 // of the group's earlier workgroups into the true carry-in and adds it at the
 // workgroup's first word, rippling through words that are all ones, never past
@@ -114,15 +115,16 @@ __device__ __forceinline__ uint32_t find_job(const JobList<CarryJob>& jobs, uint
 constexpr int kCT = kCarryThreads;
 constexpr int kWT = kCarryWordsPerThread;
 constexpr int kWB = kCarryWordsPerBlock;
-static_assert(kWB == 16 * kCT, "each thread loads four groups of four positions");
+constexpr int kGroups = kWT / 4;  // groups of four consecutive positions per thread
+static_assert(kWT % 4 == 0 && kWB == kWT * kCT, "each thread loads kWT / 4 groups of four positions");
 // LDS slot of word j of the workgroup, j in [-2, kWB): the two words before the
-// workgroup first, then one pad word after every 16 so that the threads'
-// 16-word runs start on different banks.
+// workgroup first, then one pad word after every kWT so that the threads'
+// kWT-word runs start on different banks.
 __device__ __forceinline__ uint32_t xslot(int j) {
     const uint32_t i = (uint32_t)(j + 2);
-    return i + (i >> 4);
+    return i + i / kWT;
 }
-constexpr int kXWords = (kWB + 2) + ((kWB + 2) >> 4) + 1;
+constexpr int kXWords = (kWB + 2) + (kWB + 2) / kWT + 1;
 
 // grid = carry workgroups of the launch; jobs sorted by blk0.  SET: the prime
 // set the residues belong to.
@@ -151,10 +153,10 @@ __global__ void __launch_bounds__(kCarryThreads)
         const long long len = job.len[i];
         if (k0 + kWB <= 0 || k0 - 2 >= len) continue;  // no position of this chunk reaches the words
         const global_ptr<const uint32_t> res = as_global(job.res[i]);
-        // Positions k0 + 4t + 1024v + u (v, u < 4): all loads first, then the CRTs.
-        uint32_t r[4][3][4];
+        // Positions k0 + 4t + 4 kCT v + u (v < kGroups, u < 4): all loads first, then the CRTs.
+        uint32_t r[kGroups][3][4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < kGroups; ++v) {
             const long long k = k0 + 4 * t + 4 * kCT * v;
             if (k >= 0 && k + 4 <= len) {
 #pragma unroll
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(kCarryThreads)
             }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int v = 0; v < kGroups; ++v)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 uint32_t x0, x1, x2;
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(kCarryThreads)
     __syncthreads();
     const global_ptr<uint32_t> out = as_global(job.out);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < kGroups; ++v) {
         const int jj = 4 * t + 4 * kCT * v;
         const uint64_t j = (uint64_t)j0 + jj;
         if (j + 4 <= job.nwords) {

@@ -46,8 +46,8 @@ struct CrtConst {
 
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
-constexpr int kCarryThreads = 256;
-constexpr int kCarryWordsPerThread = 16;
+constexpr int kCarryThreads = 512;
+constexpr int kCarryWordsPerThread = 8;
 constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
 constexpr int kMaxChunks = 4;  // pieces the longer operand of one product may be cut into
 
