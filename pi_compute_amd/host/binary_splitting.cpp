@@ -97,7 +97,12 @@ void BinarySplitting::merge(BSResult& l, BSResult& r, BSResult& out, bool need_p
 }
 
 std::vector<BSResult> BinarySplitting::leaves(unsigned long a, unsigned long b) {
-    // Bottom: subtrees of leaf_span terms on host threads.
+    return leaves(a, b, threads_, &leaf_s_);
+}
+
+// Bottom: subtrees of leaf_span terms on host threads.
+std::vector<BSResult> BinarySplitting::leaves(unsigned long a, unsigned long b, unsigned threads,
+                                              double* seconds) const {
     const auto t0 = std::chrono::steady_clock::now();
     const unsigned long span = leaf_span_;
     const size_t nleaf = (size_t)((b - a + span - 1) / span);
@@ -107,14 +112,14 @@ std::vector<BSResult> BinarySplitting::leaves(unsigned long a, unsigned long b) 
         std::vector<BSResult>* nodes;
     } lctx{a, b, span, &nodes};
     parallel_for(
-        nleaf, threads_,
+        nleaf, threads,
         [](void* p, size_t i) {
             auto* c = static_cast<LeafCtx*>(p);
             unsigned long lo = c->a + i * c->span, hi = std::min(c->b, lo + c->span);
             compute_small(lo, hi, (*c->nodes)[i]);
         },
         &lctx);
-    leaf_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return nodes;
 }
 

@@ -51,6 +51,12 @@ public:
     // std::runtime_error if the device fails (the caller can rerun on the host).
     BSResult compute_device(unsigned long a, unsigned long b, bool need_top_p = true, size_t device_from_limbs = 0);
 
+    // compute_device cuts [a, b) into this many segments of whole leaf spans
+    // (at least 64 leaf spans each): the host subtrees of segment i+1 are
+    // computed on threads - 1 host threads while the device reduces segment i,
+    // and the segment roots merge on the device.  1: one tree.
+    void set_segments(unsigned s) { segments_ = s ? s : 1; }
+
     // With a dispatcher set, compute_device switches to device-resident values
     // at the first level whose products the dispatcher sends to the GPU (by
     // transform work, at least half), instead of by device_from_limbs.
@@ -69,20 +75,23 @@ public:
         double seconds;
     };
     const std::vector<Level>& levels() const { return levels_; }
-    double leaf_seconds() const { return leaf_s_; }      // host subtrees of the last compute*
+    double leaf_seconds() const { return leaf_s_; }      // host subtrees of the last compute* (summed)
+    double leaf_wait_seconds() const { return leaf_wait_s_; }  // compute_device: waited for them
     double upload_seconds() const { return upload_s_; }  // compute_device: host -> device switch
 
 private:
     void merge(BSResult& left, BSResult& right, BSResult& out, bool need_p);
     std::vector<BSResult> leaves(unsigned long a, unsigned long b);
+    std::vector<BSResult> leaves(unsigned long a, unsigned long b, unsigned threads, double* seconds) const;
     void host_level(std::vector<BSResult>& nodes, bool need_top_p);
     BSResult recurse(unsigned long a, unsigned long b, int depth);
 
     Multiplier& mult_;
     unsigned threads_;
     unsigned long leaf_span_ = 256;
+    unsigned segments_ = 8;
     std::vector<Level> levels_;
-    double leaf_s_ = 0, upload_s_ = 0;
+    double leaf_s_ = 0, leaf_wait_s_ = 0, upload_s_ = 0;
     const Dispatcher* switch_by_ = nullptr;
     bool level_prefers_device(const std::vector<BSResult>& nodes, bool need_top_p) const;
 };

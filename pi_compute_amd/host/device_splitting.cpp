@@ -1,6 +1,12 @@
 // device_splitting.cpp - BinarySplitting::compute_device: the upper tree
 // levels with P, Q and R resident in device memory (see binary_splitting.h).
 //
+// The term range is cut into segments of whole leaf spans.  While the device
+// reduces segment i (the calling thread drives it), the host subtrees of
+// segment i+1 are computed on the other host threads; the segment roots then
+// merge on the device.  P, Q and R of a range do not depend on the tree
+// shape, so the result is the same as one tree's.
+//
 // Per level: the level's results (P, Q, R of every merge, odd node copied up)
 // go to one buffer and the temporaries t = Qr*Rl, u = Pl*Rr to another; the
 // products run as one gpu_ntt_mul_batch_device, then R = t +- u as one
@@ -16,11 +22,13 @@
 #include <gmp.h>
 
 #include <algorithm>
-#include <cmath>
-#include <utility>
 #include <chrono>
+#include <cmath>
+#include <future>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "binary_splitting.h"
@@ -73,79 +81,93 @@ struct Arena {
 
 size_t bytes_of(mpz_srcptr x) { return std::max<size_t>(mpz_size(x), 1); }
 
-}  // namespace
-
-BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs) {
-    using Clock = std::chrono::steady_clock;
-    levels_.clear();
-    upload_s_ = 0;
-    std::vector<BSResult> nodes = leaves(a, b);
-    // Host levels while the operands are small (or the dispatcher keeps them on the host).
-    while (nodes.size() > 1) {
-        if (switch_by_) {
-            if (level_prefers_device(nodes, need_top_p)) break;
-        } else {
-            size_t smallest = SIZE_MAX;
-            for (const auto& n : nodes) smallest = std::min({smallest, mpz_size(n.P), mpz_size(n.Q), mpz_size(n.R)});
-            if (smallest >= device_from_limbs) break;
-        }
-        host_level(nodes, need_top_p);
-    }
-    if (nodes.size() == 1) return std::move(nodes[0]);
-
-    // Upload the level: magnitudes into one arena, signs kept here.
-    const auto tu = Clock::now();
-    size_t total = 0;
-    for (const auto& n : nodes) total += bytes_of(n.P) + bytes_of(n.Q) + bytes_of(n.R);
-    // Three buffers reused across levels: this level's values, the next
-    // level's, and the temporaries t and u.
-    Arena buf[3];
-    Arena* cur = &buf[0];
-    Arena* nxt = &buf[1];
-    Arena& tmp = buf[2];
-    cur->reset(total);
-    std::vector<DevNode> dn(nodes.size());
-    {
+// The device-resident part of one tree: the nodes of its current level and
+// three arenas (this level's values, the next level's, the temporaries t and
+// u), reused level after level.
+class DeviceTree {
+public:
+    // Magnitudes of the host nodes into one arena, signs kept here.
+    void upload(std::vector<BSResult>& nodes) {
+        size_t total = 0;
+        for (const auto& n : nodes) total += bytes_of(n.P) + bytes_of(n.Q) + bytes_of(n.R);
+        cur_->reset(total);
+        dn_.assign(nodes.size(), DevNode{});
         std::vector<uint64_t*> dst;
         std::vector<const uint64_t*> src;
         std::vector<size_t> len;
         std::vector<uint64_t> zero(1, 0);
         auto put = [&](mpz_srcptr x, DevVal& v) {
             v.n = bytes_of(x);
-            v.d = cur->take(v.n);
+            v.d = cur_->take(v.n);
             v.sign = mpz_sgn(x) < 0 ? -1 : 1;
             dst.push_back(v.d);
             src.push_back(mpz_size(x) ? reinterpret_cast<const uint64_t*>(mpz_limbs_read(x)) : zero.data());
             len.push_back(v.n);
         };
         for (size_t i = 0; i < nodes.size(); ++i) {
-            put(nodes[i].P, dn[i].P);
-            put(nodes[i].Q, dn[i].Q);
-            put(nodes[i].R, dn[i].R);
+            put(nodes[i].P, dn_[i].P);
+            put(nodes[i].Q, dn_[i].Q);
+            put(nodes[i].R, dn_[i].R);
         }
         check(gpu_ntt_upload(dst.size(), dst.data(), src.data(), len.data()), "gpu_ntt_upload");
+        nodes.clear();
     }
-    nodes.clear();
-    upload_s_ = std::chrono::duration<double>(Clock::now() - tu).count();
 
-    while (dn.size() > 1) {
+    // Nodes that live in other trees' arenas (which must outlive this tree's
+    // first level).
+    void adopt(std::vector<DevNode> nodes) { dn_ = std::move(nodes); }
+
+    // Levels until one node remains.
+    void reduce(bool need_top_p, std::vector<BinarySplitting::Level>& levels);
+
+    const DevNode& root() const { return dn_.front(); }
+
+    // The root into host integers.
+    BSResult download(bool need_p) const {
+        BSResult res;
+        auto get = [&](const DevVal& v, mpz_ptr x) {
+            uint64_t* p = reinterpret_cast<uint64_t*>(mpz_limbs_write(x, (mp_size_t)v.n));
+            const uint64_t* src = v.d;
+            size_t n = v.n;
+            check(gpu_ntt_download(1, &p, &src, &n), "gpu_ntt_download");
+            while (n && !p[n - 1]) --n;
+            mpz_limbs_finish(x, (mp_size_t)n);
+            if (v.sign < 0) mpz_neg(x, x);
+        };
+        if (need_p && root().P.d) get(root().P, res.P);
+        get(root().Q, res.Q);
+        get(root().R, res.R);
+        return res;
+    }
+
+private:
+    Arena buf_[3];
+    Arena* cur_ = &buf_[0];
+    Arena* nxt_ = &buf_[1];
+    Arena& tmp_ = buf_[2];
+    std::vector<DevNode> dn_;
+};
+
+void DeviceTree::reduce(bool need_top_p, std::vector<BinarySplitting::Level>& levels) {
+    using Clock = std::chrono::steady_clock;
+    while (dn_.size() > 1) {
         auto t0 = Clock::now();
-        const size_t pairs = dn.size() / 2;
-        const bool top = dn.size() == 2;
+        const size_t pairs = dn_.size() / 2;
+        const bool top = dn_.size() == 2;
         const bool want_p = !(top && !need_top_p);
-        std::vector<DevNode> next(pairs + (dn.size() & 1));
+        std::vector<DevNode> next(pairs + (dn_.size() & 1));
         size_t need = 0, tmp_need = 0, max_limbs = 0;
         for (size_t k = 0; k < pairs; ++k) {
-            const DevNode &l = dn[2 * k], &r = dn[2 * k + 1];
+            const DevNode &l = dn_[2 * k], &r = dn_[2 * k + 1];
             const size_t tn = r.Q.n + l.R.n, un = l.P.n + r.R.n;
             const bool same = l.R.sign == l.P.sign * r.R.sign;
             tmp_need += tn + un;
             need += (want_p ? l.P.n + r.P.n : 0) + l.Q.n + r.Q.n + std::max(tn, un) + (same ? 1 : 0);
             max_limbs = std::max({max_limbs, l.Q.n, r.Q.n, l.R.n, r.R.n});
         }
-        if (dn.size() & 1) need += dn.back().P.n + dn.back().Q.n + dn.back().R.n;
-        nxt->reset(need);
-        tmp.reset(tmp_need);
+        if (dn_.size() & 1) need += dn_.back().P.n + dn_.back().Q.n + dn_.back().R.n;
+        nxt_->reset(need);
+        tmp_.reset(tmp_need);
         // Products.
         std::vector<const uint64_t*> A, B;
         std::vector<size_t> AN, BN;
@@ -162,11 +184,11 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
             O.push_back(out.d);
         };
         for (size_t k = 0; k < pairs; ++k) {
-            const DevNode &l = dn[2 * k], &r = dn[2 * k + 1];
-            job(r.Q, l.R, T[k], tmp);
-            job(l.P, r.R, U[k], tmp);
-            if (want_p) job(l.P, r.P, next[k].P, *nxt);
-            job(l.Q, r.Q, next[k].Q, *nxt);
+            const DevNode &l = dn_[2 * k], &r = dn_[2 * k + 1];
+            job(r.Q, l.R, T[k], tmp_);
+            job(l.P, r.R, U[k], tmp_);
+            if (want_p) job(l.P, r.P, next[k].P, *nxt_);
+            job(l.Q, r.Q, next[k].Q, *nxt_);
         }
         check(gpu_ntt_mul_batch_device(O.size(), A.data(), AN.data(), B.data(), BN.data(), O.data(), nullptr),
               "gpu_ntt_mul_batch_device");
@@ -179,7 +201,7 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
             const bool same = T[k].sign == U[k].sign;
             DevVal& R = next[k].R;
             R.n = std::max(T[k].n, U[k].n) + (same ? 1 : 0);
-            R.d = nxt->take(R.n);
+            R.d = nxt_->take(R.n);
             R.sign = T[k].sign;
             X.push_back(T[k].d);
             XN.push_back(T[k].n);
@@ -223,40 +245,92 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
         } else {
             check(rc, "gpu_ntt_add_batch_device");
         }
-        if (dn.size() & 1) {
-            const DevNode& o = dn.back();
+        if (dn_.size() & 1) {
+            const DevNode& o = dn_.back();
             DevNode& m = next.back();
             for (auto pr : {std::make_pair(&o.P, &m.P), std::make_pair(&o.Q, &m.Q), std::make_pair(&o.R, &m.R)}) {
                 pr.second->n = pr.first->n;
                 pr.second->sign = pr.first->sign;
-                pr.second->d = nxt->take(pr.first->n);
+                pr.second->d = nxt_->take(pr.first->n);
                 check(gpu_ntt_dev_copy(pr.second->d, pr.first->d, pr.first->n), "gpu_ntt_dev_copy");
             }
         }
         check(gpu_ntt_synchronize(), "gpu_ntt_synchronize");
-        std::swap(cur, nxt);  // the old level's buffer is reused two levels on
-        dn.swap(next);
-        levels_.push_back(Level{pairs, O.size(), max_limbs,
-                                std::chrono::duration<double>(Clock::now() - t0).count()});
+        std::swap(cur_, nxt_);  // the old level's buffer is reused two levels on
+        dn_.swap(next);
+        levels.push_back(BinarySplitting::Level{pairs, O.size(), max_limbs,
+                                                std::chrono::duration<double>(Clock::now() - t0).count()});
     }
+}
 
-    // Top: download.
-    BSResult res;
-    auto get = [&](const DevVal& v, mpz_ptr x) {
-        uint64_t* p = reinterpret_cast<uint64_t*>(mpz_limbs_write(x, (mp_size_t)v.n));
-        const uint64_t* src = v.d;
-        size_t n = v.n;
-        check(gpu_ntt_download(1, &p, &src, &n), "gpu_ntt_download");
-        while (n && !p[n - 1]) --n;
-        mpz_limbs_finish(x, (mp_size_t)n);
-        if (v.sign < 0) mpz_neg(x, x);
+}  // namespace
+
+BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs) {
+    using Clock = std::chrono::steady_clock;
+    levels_.clear();
+    leaf_s_ = leaf_wait_s_ = upload_s_ = 0;
+    // Segments of whole leaf spans, at least 64 leaves each.
+    const unsigned long span = leaf_span_;
+    const unsigned long nleaf = (b - a + span - 1) / span;
+    const unsigned long nseg = std::max(1ul, std::min<unsigned long>(segments_, nleaf / 64));
+    std::vector<unsigned long> cut(nseg + 1);
+    for (unsigned long i = 0; i < nseg; ++i) cut[i] = a + (nleaf * i / nseg) * span;
+    cut[nseg] = b;
+    const unsigned bg_threads = threads_ > 1 ? threads_ - 1 : 1;
+
+    // Host levels of one segment while the operands are small (or the
+    // dispatcher keeps them on the host).
+    auto host_levels = [&](std::vector<BSResult>& nodes, bool top_p) {
+        while (nodes.size() > 1) {
+            if (switch_by_) {
+                if (level_prefers_device(nodes, top_p)) break;
+            } else {
+                size_t smallest = SIZE_MAX;
+                for (const auto& n : nodes)
+                    smallest = std::min({smallest, mpz_size(n.P), mpz_size(n.Q), mpz_size(n.R)});
+                if (smallest >= device_from_limbs) break;
+            }
+            host_level(nodes, top_p);
+        }
     };
-    if (need_top_p || dn[0].P.d) {
-        if (dn[0].P.d) get(dn[0].P, res.P);
+
+    std::vector<std::unique_ptr<DeviceTree>> trees;
+    std::vector<DevNode> roots;
+    auto tw = Clock::now();
+    double secs = 0;
+    std::vector<BSResult> nodes = leaves(cut[0], cut[1], threads_, &secs);
+    leaf_s_ += secs;
+    leaf_wait_s_ += std::chrono::duration<double>(Clock::now() - tw).count();
+    for (unsigned long i = 0; i < nseg; ++i) {
+        // The next segment's host subtrees on the other host threads, meanwhile.
+        std::future<std::vector<BSResult>> later;
+        double later_secs = 0;
+        if (i + 1 < nseg)
+            later = std::async(std::launch::async, [this, &cut, i, bg_threads, &later_secs]() {
+                return leaves(cut[i + 1], cut[i + 2], bg_threads, &later_secs);
+            });
+        const bool top_p = nseg == 1 ? need_top_p : true;
+        host_levels(nodes, top_p);
+        if (nseg == 1 && nodes.size() == 1) return std::move(nodes[0]);
+        const auto tu = Clock::now();
+        trees.push_back(std::make_unique<DeviceTree>());
+        trees.back()->upload(nodes);
+        upload_s_ += std::chrono::duration<double>(Clock::now() - tu).count();
+        trees.back()->reduce(top_p, levels_);
+        roots.push_back(trees.back()->root());
+        if (i + 1 < nseg) {
+            tw = Clock::now();
+            nodes = later.get();
+            leaf_s_ += later_secs;
+            leaf_wait_s_ += std::chrono::duration<double>(Clock::now() - tw).count();
+        }
     }
-    get(dn[0].Q, res.Q);
-    get(dn[0].R, res.R);
-    return res;
+    if (nseg == 1) return trees[0]->download(need_top_p);
+    // The segment roots merge on the device.
+    DeviceTree top;
+    top.adopt(std::move(roots));
+    top.reduce(need_top_p, levels_);
+    return top.download(need_top_p);
 }
 
 bool BinarySplitting::level_prefers_device(const std::vector<BSResult>& nodes, bool need_top_p) const {

@@ -18,6 +18,8 @@
 //   --convert host|multiplier decimal conversion (default: multiplier with --gpu)
 //   --plans FILE              tile plans: loaded if FILE exists, saved after the run
 //   --convert-leaf D          multiplier conversion: host conversion below D digits (32768)
+//   --bs-segments S           device levels: S segments, host subtrees of the next one computed
+//                             while the device reduces the current one (8; 1 = one tree)
 //   --device-levels [L|auto]  with --gpu --tree levels: P, Q, R stay on the GPU from
 //                             the first level whose operands have >= L limbs (default 0),
 //                             or (auto) whose products the dispatcher sends to the GPU
@@ -53,7 +55,8 @@ void usage() {
     std::cout << "usage: pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]\n"
                  "                      [--threads T] [--tree levels|recursive] [--per-call]\n"
                  "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
-                 "                      [--final mpf|newton] [--convert host|multiplier] [--device-levels [L]]\n"
+                 "                      [--final mpf|newton] [--convert host|multiplier] [--convert-leaf D]\n"
+                 "                      [--device-levels [LIMBS|auto]] [--bs-segments S] [--plans FILE]\n"
                  "                      [--verbose]\n";
 }
 
@@ -92,6 +95,7 @@ int main(int argc, char** argv) {
         else if (a == "--convert") convert = next();
         else if (a == "--plans") plans = next();
         else if (a == "--convert-leaf") cfg.conversion_leaf_digits = std::stoul(next());
+        else if (a == "--bs-segments") cfg.bs_segments = (unsigned)std::stoul(next());
         else if (a == "--device-levels") {
             cfg.device_levels = true;
             if (i + 1 < argc && argv[i + 1][0] != '-') {

@@ -23,6 +23,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
 
     BinarySplitting bs(mult_, cfg.num_threads);
     bs.set_switch_dispatcher(cfg.device_switch);
+    bs.set_segments(cfg.bs_segments);
     BSResult r;
     if (cfg.level_batches && cfg.device_levels) {
         try {
@@ -37,8 +38,9 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     }
     const double bs_s = since(t0);
     if (cfg.verbose) {
-        std::cout << "  binary splitting: " << bs_s << " s (host subtrees " << bs.leaf_seconds() << " s, upload "
-                  << bs.upload_seconds() << " s)" << std::endl;
+        std::cout << "  binary splitting: " << bs_s << " s (host subtrees " << bs.leaf_seconds() << " s";
+        if (cfg.level_batches && cfg.device_levels) std::cout << ", waited for " << bs.leaf_wait_seconds() << " s";
+        std::cout << ", upload " << bs.upload_seconds() << " s)" << std::endl;
         for (size_t i = 0; i < bs.levels().size(); ++i) {
             const auto& L = bs.levels()[i];
             std::cout << "    level " << i << ": " << L.products << " products, max " << L.max_limbs

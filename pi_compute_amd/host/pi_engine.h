@@ -36,6 +36,7 @@ struct PiConfig {
     enum class FinalStep { Mpf, Newton } final_step = FinalStep::Mpf;
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
     size_t conversion_leaf_digits = 1 << 15;  // Conversion::Multiplier: host conversion below this width
+    unsigned bs_segments = 8;                  // device levels: segments pipelined with their host subtrees
 };
 
 struct PiResult {

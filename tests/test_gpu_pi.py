@@ -44,9 +44,12 @@ def gmp_digits(tmp_path_factory):
         ["--tree", "levels", "--gpu-threshold", "64", "--convert", "multiplier"],
         ["--tree", "levels", "--device-levels"],
         ["--tree", "levels", "--device-levels", "2000"],
+        ["--tree", "levels", "--device-levels", "--bs-segments", "3"],
+        ["--tree", "levels", "--device-levels", "--bs-segments", "1"],
     ],
     ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64", "levels-mpf-host",
-         "levels-t64-convert", "device-levels", "device-levels-2000"],
+         "levels-t64-convert", "device-levels", "device-levels-2000", "device-levels-3-segments",
+         "device-levels-one-tree"],
 )
 def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
     digits = 1000000
