@@ -3,7 +3,9 @@
 
 #include <chrono>
 #include <cstdio>
+#include <future>
 #include <iostream>
+#include <memory>
 #include <stdexcept>
 
 #include "decimal.h"
@@ -20,6 +22,18 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const size_t precision = cfg.digits + 100;
     const unsigned long terms = BinarySplitting::terms_needed(precision);
     if (cfg.verbose) std::cout << "  terms: " << terms << std::endl;
+
+    // Working precision of the Newton final step (bits + 64 below).
+    const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
+    const bool newton = cfg.final_step == PiConfig::FinalStep::Newton;
+    std::unique_ptr<FinalConstants> pre;
+    std::future<void> pre_done;
+    if (newton && cfg.overlap_constants) {
+        pre = std::make_unique<FinalConstants>();
+        FinalConstants* c = pre.get();
+        pre_done = std::async(std::launch::async,
+                              [this, c, &cfg, bits]() { final_constants(*c, cfg.digits, bits + 64, mult_); });
+    }
 
     BinarySplitting bs(mult_, cfg.num_threads);
     bs.set_switch_dispatcher(cfg.device_switch);
@@ -50,12 +64,13 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
 
     const auto t1 = Clock::now();
     mpz_realloc2(r.P, 0);
-    const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
+    if (pre_done.valid()) pre_done.get();  // rethrows what the constants' thread threw
+    const double pre_wait_s = since(t1);
     mpz_t pi_int;
     mpz_init(pi_int);
     NewtonStats ns;
-    if (cfg.final_step == PiConfig::FinalStep::Newton && mpz_sgn(r.R) > 0) {
-        pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, bits + 64, mult_, &ns);
+    if (newton && mpz_sgn(r.R) > 0) {
+        pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, bits + 64, mult_, &ns, pre.get());
         mpz_realloc2(r.Q, 0);
         mpz_realloc2(r.R, 0);
     } else {
@@ -109,9 +124,12 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     }
     if (cfg.verbose) {
         std::cout << "  final step: " << fin_s << " s, decimal conversion: " << conv_s << " s" << std::endl;
-        if (cfg.final_step == PiConfig::FinalStep::Newton)
+        if (newton) {
             std::cout << "    final step: inverse square root " << ns.inv_sqrt_s << " s, 10^D " << ns.pow10_s
-                      << " s, reciprocal " << ns.reciprocal_s << " s, products " << ns.products_s << " s" << std::endl;
+                      << " s, reciprocal " << ns.reciprocal_s << " s, products " << ns.products_s << " s";
+            if (pre) std::cout << " (square root and 10^D beside binary splitting, waited " << pre_wait_s << " s)";
+            std::cout << std::endl;
+        }
         if (cfg.conversion == PiConfig::Conversion::Multiplier)
             std::cout << "    conversion: powers " << ds.powers_s << " s, reciprocals " << ds.reciprocals_s
                       << " s, quotient products " << ds.products_s << " s, corrections " << ds.corrections_s
@@ -120,8 +138,20 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     return PiResult{std::move(out), since(t0), terms, bs_s, fin_s, conv_s};
 }
 
+FinalConstants::FinalConstants() { mpz_inits(inv_sqrt, pow10, NULL); }
+FinalConstants::~FinalConstants() { mpz_clears(inv_sqrt, pow10, NULL); }
+
+void final_constants(FinalConstants& c, size_t digits, size_t w, Multiplier& m) {
+    auto t = Clock::now();
+    newton_inv_sqrt_ui(c.inv_sqrt, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
+    c.inv_sqrt_s = since(t);
+    t = Clock::now();
+    pow_ui(c.pow10, 10, digits, m);
+    c.pow10_s = since(t);
+}
+
 void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t w, Multiplier& m,
-                      NewtonStats* stats) {
+                      NewtonStats* stats, const FinalConstants* pre) {
     NewtonStats st;
     auto clock = Clock::now();
     auto lap = [&clock]() {
@@ -138,14 +168,23 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
     long ex = 0;  // value = integer * 2^ex
     mpz_mul_ui(a, Q, 426880);
     ex += (long)truncate_top(a, a, w);
-    newton_inv_sqrt_ui(y, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
-    st.inv_sqrt_s = lap();
-    m.multiply(t, a, y);
+    if (pre) {
+        st.inv_sqrt_s = pre->inv_sqrt_s;
+    } else {
+        newton_inv_sqrt_ui(y, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
+        st.inv_sqrt_s = lap();
+    }
+    m.multiply(t, a, pre ? pre->inv_sqrt : y);
     ex += (long)truncate_top(t, t, w) - (long)w;
     st.products_s += lap();
-    pow_ui(a, 10, digits, m);
-    st.pow10_s = lap();
-    m.multiply(y, t, a);
+    if (pre) {
+        m.multiply(y, t, pre->pow10);
+        st.pow10_s = pre->pow10_s;
+    } else {
+        pow_ui(a, 10, digits, m);
+        st.pow10_s = lap();
+        m.multiply(y, t, a);
+    }
     ex += (long)truncate_top(y, y, w);
     const size_t sr = truncate_top(t, R, w);
     const size_t nr = mpz_sizeinbase(t, 2);

@@ -37,6 +37,9 @@ struct PiConfig {
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
     size_t conversion_leaf_digits = 1 << 15;  // Conversion::Multiplier: host conversion below this width
     unsigned bs_segments = 8;                  // device levels: segments pipelined with their host subtrees
+    // FinalStep::Newton: sqrt(10005) and 10^D depend only on the precision, so
+    // they are computed on another thread while binary splitting runs.
+    bool overlap_constants = true;
 };
 
 struct PiResult {
@@ -46,13 +49,26 @@ struct PiResult {
     double bs_seconds, final_seconds, conv_seconds;
 };
 
+// The factors of the Newton final step that do not depend on Q and R:
+// y = 2^(2 bits) / sqrt(10005) to `bits` bits (newton_inv_sqrt_ui) and 10^digits.
+struct FinalConstants {
+    mpz_t inv_sqrt, pow10;
+    double inv_sqrt_s = 0, pow10_s = 0;
+    FinalConstants();
+    ~FinalConstants();
+    FinalConstants(const FinalConstants&) = delete;
+    FinalConstants& operator=(const FinalConstants&) = delete;
+};
+void final_constants(FinalConstants& c, size_t digits, size_t bits, Multiplier& m);
+
 // floor(426880 * sqrt(10005) * Q * 10^digits / R) with `bits` of working
-// precision, products through m (R > 0).  Used by FinalStep::Newton.
+// precision, products through m (R > 0).  Used by FinalStep::Newton.  With
+// `pre`, its constants (computed for the same digits and bits) are used.
 struct NewtonStats {
     double inv_sqrt_s = 0, pow10_s = 0, reciprocal_s = 0, products_s = 0;
 };
 void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t bits, Multiplier& m,
-                      NewtonStats* stats = nullptr);
+                      NewtonStats* stats = nullptr, const FinalConstants* pre = nullptr);
 
 class PiEngine {
 public:

@@ -255,8 +255,11 @@ int main(int argc, char** argv) {
             a.digits = b.digits = digits;
             b.final_step = pi::PiConfig::FinalStep::Newton;
             b.conversion = pi::PiConfig::Conversion::Multiplier;
-            check(engine.compute(a).digits == engine.compute(b).digits,
-                  "NewtonFinalStepMatchesMpf D=" + std::to_string(digits));
+            const std::string mpf = engine.compute(a).digits;
+            check(mpf == engine.compute(b).digits, "NewtonFinalStepMatchesMpf D=" + std::to_string(digits));
+            // sqrt(10005) and 10^D on another thread beside binary splitting, or after it
+            b.overlap_constants = !b.overlap_constants;
+            check(mpf == engine.compute(b).digits, "NewtonConstantsEitherOrder D=" + std::to_string(digits));
         }
     }
 

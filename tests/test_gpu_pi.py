@@ -46,10 +46,11 @@ def gmp_digits(tmp_path_factory):
         ["--tree", "levels", "--device-levels", "2000"],
         ["--tree", "levels", "--device-levels", "--bs-segments", "3"],
         ["--tree", "levels", "--device-levels", "--bs-segments", "1"],
+        ["--tree", "levels", "--device-levels", "--no-overlap"],
     ],
     ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64", "levels-mpf-host",
          "levels-t64-convert", "device-levels", "device-levels-2000", "device-levels-3-segments",
-         "device-levels-one-tree"],
+         "device-levels-one-tree", "device-levels-serial-constants"],
 )
 def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
     digits = 1000000
