@@ -109,7 +109,39 @@ struct Node {
 
 }  // namespace
 
-std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t leaf_digits, DecimalStats* stats) {
+DecimalPowers::~DecimalPowers() {
+    for (auto& x : pw) mpz_clear(&x);
+    for (auto& x : inv) mpz_clear(&x);
+}
+
+void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits) {
+    using Clock = std::chrono::steady_clock;
+    auto t = Clock::now();
+    leaf_digits = std::max<size_t>(leaf_digits, kLeafDigits);
+    d.width = width;
+    d.leaf_digits = leaf_digits;
+    int k = 0;
+    while (((size_t)2 << k) < width) ++k;  // 2^k < width <= 2^(k+1)
+    // Powers 10^(2^j), j <= k, by squaring through m.
+    d.pw.resize(k + 1);
+    d.inv.resize(k + 1);
+    for (int j = 0; j <= k; ++j) {
+        mpz_init(&d.pw[j]);
+        mpz_init(&d.inv[j]);
+        if (j == 0)
+            mpz_set_ui(&d.pw[j], 10);
+        else
+            m.square(&d.pw[j], &d.pw[j - 1]);
+    }
+    d.powers_s = std::chrono::duration<double>(Clock::now() - t).count();
+    t = Clock::now();
+    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j)
+        newton_reciprocal(&d.inv[j], &d.pw[j], mpz_sizeinbase(&d.pw[j], 2), m);  // ~ 2^(2 np) / P, within 4
+    d.reciprocals_s = std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t leaf_digits, DecimalStats* stats,
+                       const DecimalPowers* pre) {
     using Clock = std::chrono::steady_clock;
     DecimalStats st;
     auto lap = [](Clock::time_point& t) {
@@ -126,38 +158,34 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
     if (width <= 2 * leaf_digits) return to_decimal(n, threads);
     std::string out(width, '0');
 
-    int k = 0;
-    while (((size_t)2 << k) < width) ++k;  // 2^k < width <= 2^(k+1)
-    // Powers 10^(2^j), j <= k, by squaring through m.
-    std::vector<__mpz_struct> pw(k + 1);
-    for (int j = 0; j <= k; ++j) {
-        mpz_init(&pw[j]);
-        if (j == 0)
-            mpz_set_ui(&pw[j], 10);
-        else
-            m.square(&pw[j], &pw[j - 1]);
+    // The powers depend on the width only through k (2^k < width <= 2^(k+1)).
+    int kw = 0;
+    while (((size_t)2 << kw) < width) ++kw;
+    DecimalPowers own;
+    if (!pre || (int)pre->pw.size() != kw + 1 || pre->leaf_digits != leaf_digits) {
+        decimal_powers(own, width, m, leaf_digits);
+        pre = &own;
     }
-
-    st.powers_s += lap(t);
+    const int k = (int)pre->pw.size() - 1;
+    st.powers_s = pre->powers_s;
+    st.reciprocals_s = pre->reciprocals_s;
+    lap(t);
     std::vector<std::unique_ptr<Node>> nodes;
     nodes.emplace_back(new Node);
     mpz_abs(nodes[0]->x, n);
     nodes[0]->offset = 0;
     nodes[0]->width = width;
 
-    mpz_t inv;
-    mpz_init(inv);
     for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
         const size_t low = (size_t)1 << j;
         std::vector<size_t> split;
         for (size_t i = 0; i < nodes.size(); ++i)
             if (nodes[i]->width > low) split.push_back(i);
         if (split.empty()) continue;
-        mpz_srcptr P = &pw[j];
+        mpz_srcptr P = &pre->pw[j];
+        mpz_srcptr inv = &pre->inv[j];
         const size_t np = mpz_sizeinbase(P, 2);
         lap(t);
-        newton_reciprocal(inv, P, np, m);  // ~ 2^(2 np) / P, within 4
-        st.reciprocals_s += lap(t);
         const size_t sx = np > 64 ? np - 64 : 0;
         const size_t ns = split.size();
         std::vector<__mpz_struct> hi(ns), q(ns), qp(ns);
@@ -218,8 +246,6 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
             mpz_clear(&qp[s]);
         }
     }
-    mpz_clear(inv);
-    for (auto& p : pw) mpz_clear(&p);
 
     // Leaves: host conversion, one node per task (largest first).
     Powers leaf_pw(2 * leaf_digits);  // shared read-only by every task

@@ -12,6 +12,7 @@
 
 #include <cstddef>
 #include <string>
+#include <vector>
 
 #include "multiplier.h"
 
@@ -28,7 +29,26 @@ std::string to_decimal(mpz_srcptr n, unsigned threads = 0);
 struct DecimalStats {
     double powers_s = 0, reciprocals_s = 0, products_s = 0, corrections_s = 0, leaves_s = 0;
 };
+
+// The powers 10^(2^j) and their Newton reciprocals that the conversion of a
+// number of `width` digits uses: they depend only on width and leaf_digits, so
+// a caller that knows the width early can build them beforehand (for pi,
+// beside binary splitting).
+struct DecimalPowers {
+    size_t width = 0, leaf_digits = 0;
+    std::vector<__mpz_struct> pw;   // 10^(2^j), j <= k with 2^k < width <= 2^(k+1)
+    std::vector<__mpz_struct> inv;  // ~2^(2 bits(pw[j])) / pw[j] where 2^j >= leaf_digits, else 0
+    double powers_s = 0, reciprocals_s = 0;
+    DecimalPowers() = default;
+    ~DecimalPowers();
+    DecimalPowers(const DecimalPowers&) = delete;
+    DecimalPowers& operator=(const DecimalPowers&) = delete;
+};
+void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits = 1 << 15);
+
+// `pre`: powers built for a width with the same k and for the same leaf_digits
+// (others are ignored).
 std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads = 0, size_t leaf_digits = 1 << 15,
-                       DecimalStats* stats = nullptr);
+                       DecimalStats* stats = nullptr, const DecimalPowers* pre = nullptr);
 
 }  // namespace pi

@@ -7,6 +7,7 @@
 #include <iostream>
 #include <memory>
 #include <stdexcept>
+#include <thread>
 
 #include "decimal.h"
 #include "newton.h"
@@ -26,14 +27,50 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     // Working precision of the Newton final step (bits + 64 below).
     const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
     const bool newton = cfg.final_step == PiConfig::FinalStep::Newton;
+    const bool conv_mult = cfg.conversion == PiConfig::Conversion::Multiplier;
+    // What does not depend on P, Q, R, built on one more thread beside binary
+    // splitting: the final step's constants, then the conversion's powers of
+    // ten and their reciprocals (pi * 10^D has D + 1 digits).
     std::unique_ptr<FinalConstants> pre;
-    std::future<void> pre_done;
-    if (newton && cfg.overlap_constants) {
-        pre = std::make_unique<FinalConstants>();
-        FinalConstants* c = pre.get();
-        pre_done = std::async(std::launch::async,
-                              [this, c, &cfg, bits]() { final_constants(*c, cfg.digits, bits + 64, mult_); });
+    std::unique_ptr<DecimalPowers> dpow;
+    std::promise<void> pre_set, dpow_set;
+    std::future<void> pre_done, dpow_done;
+    std::thread beside;
+    if (cfg.overlap_constants && (newton || conv_mult)) {
+        if (newton) {
+            pre = std::make_unique<FinalConstants>();
+            pre_done = pre_set.get_future();
+        }
+        if (conv_mult) {
+            dpow = std::make_unique<DecimalPowers>();
+            dpow_done = dpow_set.get_future();
+        }
+        beside = std::thread([this, &cfg, bits, &pre, &dpow, &pre_set, &dpow_set]() {
+            if (pre) {
+                try {
+                    final_constants(*pre, cfg.digits, bits + 64, mult_);
+                    pre_set.set_value();
+                } catch (...) {
+                    pre_set.set_exception(std::current_exception());
+                }
+            }
+            if (dpow) {
+                try {
+                    decimal_powers(*dpow, cfg.digits + 1, mult_, cfg.conversion_leaf_digits);
+                    dpow_set.set_value();
+                } catch (...) {
+                    dpow_set.set_exception(std::current_exception());
+                }
+            }
+        });
     }
+    // The thread is joined on every way out of this function.
+    struct Joiner {
+        std::thread& t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } joiner{beside};
 
     BinarySplitting bs(mult_, cfg.num_threads);
     bs.set_switch_dispatcher(cfg.device_switch);
@@ -103,10 +140,11 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const double fin_s = since(t1);
 
     const auto t2 = Clock::now();
+    if (dpow_done.valid()) dpow_done.get();
+    const double dpow_wait_s = since(t2);
     DecimalStats ds;
-    std::string d = cfg.conversion == PiConfig::Conversion::Multiplier
-                        ? to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds)
-                        : to_decimal(pi_int, cfg.num_threads);
+    std::string d = conv_mult ? to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get())
+                              : to_decimal(pi_int, cfg.num_threads);
     mpz_clear(pi_int);
     std::string out;
     out.reserve(cfg.digits + 2);
@@ -130,10 +168,13 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             if (pre) std::cout << " (square root and 10^D beside binary splitting, waited " << pre_wait_s << " s)";
             std::cout << std::endl;
         }
-        if (cfg.conversion == PiConfig::Conversion::Multiplier)
+        if (conv_mult) {
             std::cout << "    conversion: powers " << ds.powers_s << " s, reciprocals " << ds.reciprocals_s
                       << " s, quotient products " << ds.products_s << " s, corrections " << ds.corrections_s
-                      << " s, leaves " << ds.leaves_s << " s" << std::endl;
+                      << " s, leaves " << ds.leaves_s << " s";
+            if (dpow) std::cout << " (powers and reciprocals beside binary splitting, waited " << dpow_wait_s << " s)";
+            std::cout << std::endl;
+        }
     }
     return PiResult{std::move(out), since(t0), terms, bs_s, fin_s, conv_s};
 }

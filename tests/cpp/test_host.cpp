@@ -225,8 +225,23 @@ int main(int argc, char** argv) {
                 free(s);
             }
         }
+        // Powers built beforehand (for a width with the same k, or another k,
+        // which is then ignored) give the same digits.
+        bool pre_ok = true;
+        for (size_t nb : {100000ul, 654321ul}) {
+            pi::draw_operand(x, g, nb);
+            char* s = mpz_get_str(nullptr, 10, x);
+            const size_t w = std::strlen(s);
+            for (size_t pw : {w, w - 1, 2 * w, w / 3}) {
+                pi::DecimalPowers pre;
+                pi::decimal_powers(pre, pw, gmp, 4096);
+                pre_ok = pre_ok && pi::to_decimal(x, gmp, 4, 4096, nullptr, &pre) == s;
+            }
+            free(s);
+        }
         mpz_clear(x);
         check(ok, "DecimalWithNewtonQuotients");
+        check(pre_ok, "DecimalWithPowersBuiltBeforehand");
     }
 
     // Pi digits.
