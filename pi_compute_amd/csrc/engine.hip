@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -65,12 +66,6 @@ struct Engine {
     size_t io_cap = 0;  // limbs
     uint64_t* io_in = nullptr;
     uint64_t* io_out = nullptr;
-    // Pinned host staging for host operands: gathered by host threads, one
-    // DMA each way per batch (pageable copies go through the runtime's own
-    // small staging buffers, one call per operand).
-    size_t pin_cap = 0;  // limbs
-    uint64_t* pin_in = nullptr;
-    uint64_t* pin_out = nullptr;
     // Batched add/subtract: job descriptors (pinned + device), per-block maps.
     size_t add_job_cap = 0;
     AddJob* add_jobs_host = nullptr;
@@ -97,6 +92,25 @@ struct Engine {
 
 Engine g;
 std::mutex g_mu;
+
+// Pinned host staging for host operands: gathered by host threads, one DMA
+// each way per call (pageable copies go through the runtime's own small
+// staging buffers, one call per operand).  A caller holds a slot from its
+// gather to its scatter but the engine lock only while it enqueues device
+// work, so one caller's host copies overlap another caller's device work.
+// The device-side staging (io_in, io_out) is shared: every use is ordered on
+// the library stream.
+struct Staging {
+    size_t cap = 0;  // limbs
+    uint64_t* in = nullptr;
+    uint64_t* out = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+};
+constexpr int kStagingSlots = 2;
+Staging g_stage[kStagingSlots];
+std::mutex g_stage_mu;
+std::condition_variable g_stage_cv;
 
 int fail(int code, const std::string& what) {
     t_error = what;
@@ -747,19 +761,70 @@ int ensure_io(size_t limbs) {
     return PINTT_OK;
 }
 
-int ensure_pinned(size_t limbs) {
-    if (limbs <= g.pin_cap) return PINTT_OK;
-    if (g.pin_in) (void)hipHostFree(g.pin_in);
-    if (g.pin_out) (void)hipHostFree(g.pin_out);
-    g.pin_in = g.pin_out = nullptr;
-    g.pin_cap = 0;
-    const size_t cap = std::max(limbs, (size_t)1 << 20) + (limbs >> 3);
-    if (hipHostMalloc((void**)&g.pin_in, cap * sizeof(uint64_t)) != hipSuccess ||
-        hipHostMalloc((void**)&g.pin_out, cap * sizeof(uint64_t)) != hipSuccess)
-        return fail(PINTT_ERR_NO_MEMORY, "hipHostMalloc of the staging buffers failed");
-    g.pin_cap = cap;
-    return PINTT_OK;
-}
+// A staging slot for one caller, returned (after its last transfer is
+// complete) when the holder goes out of scope.  Needs an initialised engine.
+class StagingSlot {
+public:
+    StagingSlot() = default;
+    StagingSlot(const StagingSlot&) = delete;
+    StagingSlot& operator=(const StagingSlot&) = delete;
+    ~StagingSlot() {
+        if (!s_) return;
+        if (pending_) (void)hipEventSynchronize(s_->done);
+        {
+            std::lock_guard<std::mutex> lock(g_stage_mu);
+            s_->busy = false;
+        }
+        g_stage_cv.notify_one();
+    }
+    int acquire(size_t limbs) {
+        {
+            std::unique_lock<std::mutex> lock(g_stage_mu);
+            g_stage_cv.wait(lock, [] {
+                for (const Staging& x : g_stage)
+                    if (!x.busy) return true;
+                return false;
+            });
+            for (Staging& x : g_stage)
+                if (!x.busy) {
+                    s_ = &x;
+                    break;
+                }
+            s_->busy = true;
+        }
+        PINTT_HIP(hipSetDevice(g.device));
+        if (!s_->done) PINTT_HIP(hipEventCreateWithFlags(&s_->done, hipEventDisableTiming));
+        if (limbs <= s_->cap) return PINTT_OK;
+        if (s_->in) (void)hipHostFree(s_->in);
+        if (s_->out) (void)hipHostFree(s_->out);
+        s_->in = s_->out = nullptr;
+        s_->cap = 0;
+        const size_t cap = std::max(limbs, (size_t)1 << 20) + (limbs >> 3);
+        if (hipHostMalloc((void**)&s_->in, cap * sizeof(uint64_t)) != hipSuccess ||
+            hipHostMalloc((void**)&s_->out, cap * sizeof(uint64_t)) != hipSuccess)
+            return fail(PINTT_ERR_NO_MEMORY, "hipHostMalloc of the staging buffers failed");
+        s_->cap = cap;
+        return PINTT_OK;
+    }
+    uint64_t* in() const { return s_->in; }
+    uint64_t* out() const { return s_->out; }
+    // On the library stream after the slot's transfers (engine lock held).
+    int record() {
+        PINTT_HIP(hipEventRecord(s_->done, g.stream));
+        pending_ = true;
+        return PINTT_OK;
+    }
+    // Until those transfers are complete (engine lock not held).
+    int wait() {
+        pending_ = false;
+        PINTT_HIP(hipEventSynchronize(s_->done));
+        return PINTT_OK;
+    }
+
+private:
+    Staging* s_ = nullptr;
+    bool pending_ = false;
+};
 
 // Copies pieces (dst, src, limbs) with up to `threads` host threads, each
 // taking an equal share of the total bytes (pieces are cut where needed).
@@ -815,38 +880,47 @@ int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const u
         total_in += an[i] + (sq ? 0 : bn[i]);
         total_out += an[i] + bn[i];
     }
-    std::lock_guard<std::mutex> lock(g_mu);
-    PINTT_TRY(ensure_init(-1));
-    PINTT_HIP(hipSetDevice(g.device));
-    PINTT_TRY(ensure_io(std::max(total_in, total_out)));
-    PINTT_TRY(ensure_pinned(std::max(total_in, total_out)));
-    // The previous batch may still read pin_in / write pin_out: it ended with
-    // a synchronize, so the buffers are free here.
-    std::vector<Request> reqs(count);
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_TRY(ensure_init(-1));
+    }
+    StagingSlot st;
+    PINTT_TRY(st.acquire(std::max(total_in, total_out)));
+    // Operand and product offsets inside the staging (host and device alike).
+    std::vector<size_t> a_off(count), b_off(count), o_off(count);
     std::vector<Piece> gather, scatter;
     gather.reserve(2 * count);
     scatter.reserve(count);
     size_t in_off = 0, out_off = 0;
     for (size_t i = 0; i < count; ++i) {
-        bool sq = a[i] == b[i] && an[i] == bn[i];
-        uint64_t* da = g.io_in + in_off;
-        gather.push_back(Piece{g.pin_in + in_off, a[i], an[i]});
+        const bool sq = a[i] == b[i] && an[i] == bn[i];
+        a_off[i] = b_off[i] = in_off;
+        gather.push_back(Piece{st.in() + in_off, a[i], an[i]});
         in_off += an[i];
-        uint64_t* db = da;
         if (!sq) {
-            db = g.io_in + in_off;
-            gather.push_back(Piece{g.pin_in + in_off, b[i], bn[i]});
+            b_off[i] = in_off;
+            gather.push_back(Piece{st.in() + in_off, b[i], bn[i]});
             in_off += bn[i];
         }
-        reqs[i] = Request{da, an[i], db, bn[i], g.io_out + out_off, sq};
-        scatter.push_back(Piece{out[i], g.pin_out + out_off, an[i] + bn[i]});
+        o_off[i] = out_off;
+        scatter.push_back(Piece{out[i], st.out() + out_off, an[i] + bn[i]});
         out_off += an[i] + bn[i];
     }
     parallel_copy(gather, total_in);
-    PINTT_HIP(hipMemcpyAsync(g.io_in, g.pin_in, total_in * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
-    PINTT_TRY(enqueue_batch(reqs, g.stream));
-    PINTT_HIP(hipMemcpyAsync(g.pin_out, g.io_out, total_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
-    PINTT_TRY(wait_stream());
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_HIP(hipSetDevice(g.device));
+        PINTT_TRY(ensure_io(std::max(total_in, total_out)));
+        std::vector<Request> reqs(count);
+        for (size_t i = 0; i < count; ++i)
+            reqs[i] = Request{g.io_in + a_off[i], an[i], g.io_in + b_off[i], bn[i], g.io_out + o_off[i],
+                              a[i] == b[i] && an[i] == bn[i]};
+        PINTT_HIP(hipMemcpyAsync(g.io_in, st.in(), total_in * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
+        PINTT_TRY(enqueue_batch(reqs, g.stream));
+        PINTT_HIP(hipMemcpyAsync(st.out(), g.io_out, total_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+        PINTT_TRY(st.record());
+    }
+    PINTT_TRY(st.wait());
     parallel_copy(scatter, total_out);
     return PINTT_OK;
 }
@@ -882,8 +956,12 @@ void gpu_ntt_shutdown(void) {
     }
     (void)hipFree(g.io_in);
     (void)hipFree(g.io_out);
-    if (g.pin_in) (void)hipHostFree(g.pin_in);
-    if (g.pin_out) (void)hipHostFree(g.pin_out);
+    for (Staging& x : g_stage) {
+        if (x.in) (void)hipHostFree(x.in);
+        if (x.out) (void)hipHostFree(x.out);
+        if (x.done) (void)hipEventDestroy(x.done);
+        x = Staging{};
+    }
     if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
     (void)hipFree(g.add_jobs_dev);
     (void)hipFree(g.add_maps);
@@ -1010,32 +1088,39 @@ int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* 
     if (!d_dst || !h_src || !limbs) return fail(PINTT_ERR_ARGS, "null pointer");
     size_t total = 0;
     for (size_t i = 0; i < count; ++i) total += limbs[i];
-    std::lock_guard<std::mutex> lock(g_mu);
-    PINTT_TRY(ensure_init(-1));
-    PINTT_HIP(hipSetDevice(g.device));
-    PINTT_HIP(hipStreamSynchronize(g.stream));
-    PINTT_TRY(ensure_pinned(total));
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_TRY(ensure_init(-1));
+    }
+    StagingSlot st;
+    PINTT_TRY(st.acquire(total));
     std::vector<Piece> gather;
     gather.reserve(count);
     size_t off = 0;
     for (size_t i = 0; i < count; ++i) {
-        gather.push_back(Piece{g.pin_in + off, h_src[i], limbs[i]});
+        gather.push_back(Piece{st.in() + off, h_src[i], limbs[i]});
         off += limbs[i];
     }
     parallel_copy(gather, total);
-    // One DMA per run of destinations that follow each other inside one allocation.
-    off = 0;
-    for (size_t i = 0; i < count;) {
-        const uint64_t* end = allocation_end(d_dst[i]);
-        size_t j = i + 1, run = limbs[i];
-        while (j < count && d_dst[j] == d_dst[j - 1] + limbs[j - 1] && d_dst[j] + limbs[j] <= end) run += limbs[j++];
-        if (run)
-            PINTT_HIP(hipMemcpyAsync(d_dst[i], g.pin_in + off, run * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
-        off += run;
-        i = j;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_HIP(hipSetDevice(g.device));
+        // One DMA per run of destinations that follow each other inside one allocation.
+        off = 0;
+        for (size_t i = 0; i < count;) {
+            const uint64_t* end = allocation_end(d_dst[i]);
+            size_t j = i + 1, run = limbs[i];
+            while (j < count && d_dst[j] == d_dst[j - 1] + limbs[j - 1] && d_dst[j] + limbs[j] <= end)
+                run += limbs[j++];
+            if (run)
+                PINTT_HIP(hipMemcpyAsync(d_dst[i], st.in() + off, run * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                         g.stream));
+            off += run;
+            i = j;
+        }
+        PINTT_TRY(st.record());
     }
-    PINTT_HIP(hipStreamSynchronize(g.stream));
-    return PINTT_OK;
+    return st.wait();
 }
 
 int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const* d_src, const size_t* limbs) {
@@ -1043,26 +1128,35 @@ int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const
     if (!h_dst || !d_src || !limbs) return fail(PINTT_ERR_ARGS, "null pointer");
     size_t total = 0;
     for (size_t i = 0; i < count; ++i) total += limbs[i];
-    std::lock_guard<std::mutex> lock(g_mu);
-    PINTT_TRY(ensure_init(-1));
-    PINTT_HIP(hipSetDevice(g.device));
-    PINTT_TRY(ensure_pinned(total));
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_TRY(ensure_init(-1));
+    }
+    StagingSlot st;
+    PINTT_TRY(st.acquire(total));
     std::vector<Piece> scatter;
     scatter.reserve(count);
-    size_t off = 0;
-    for (size_t i = 0; i < count;) {
-        const uint64_t* end = allocation_end(d_src[i]);
-        size_t j = i + 1, run = limbs[i];
-        while (j < count && d_src[j] == d_src[j - 1] + limbs[j - 1] && d_src[j] + limbs[j] <= end) run += limbs[j++];
-        if (run)
-            PINTT_HIP(hipMemcpyAsync(g.pin_out + off, d_src[i], run * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
-        for (size_t k = i; k < j; ++k) {
-            scatter.push_back(Piece{h_dst[k], g.pin_out + off, limbs[k]});
-            off += limbs[k];
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_HIP(hipSetDevice(g.device));
+        size_t off = 0;
+        for (size_t i = 0; i < count;) {
+            const uint64_t* end = allocation_end(d_src[i]);
+            size_t j = i + 1, run = limbs[i];
+            while (j < count && d_src[j] == d_src[j - 1] + limbs[j - 1] && d_src[j] + limbs[j] <= end)
+                run += limbs[j++];
+            if (run)
+                PINTT_HIP(hipMemcpyAsync(st.out() + off, d_src[i], run * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                         g.stream));
+            for (size_t k = i; k < j; ++k) {
+                scatter.push_back(Piece{h_dst[k], st.out() + off, limbs[k]});
+                off += limbs[k];
+            }
+            i = j;
         }
-        i = j;
+        PINTT_TRY(st.record());
     }
-    PINTT_TRY(wait_stream());
+    PINTT_TRY(st.wait());
     parallel_copy(scatter, total);
     return PINTT_OK;
 }
