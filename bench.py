@@ -380,7 +380,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(products shard across GPUs as independent calls; see DESIGN.md)", file=sys.stderr)
+              "(a tree level's products would shard across GPUs as independent calls)", file=sys.stderr)
         return 2
 
     lib = pc.load_library()

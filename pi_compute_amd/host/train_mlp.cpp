@@ -188,7 +188,7 @@ int main(int argc, char** argv) {
         pc[s.path]++;
     }
     test_loss /= std::max<size_t>(1, test.size());
-    // Host staging (path 1) and device-resident operands (path 2) are judged
+    // Host staging (path 1) and device-resident operands (path 2) are scored
     // separately, each on the shapes measured on the host and on that path.
     double t_mlp[3] = {0, 0, 0}, t_heur[3] = {0, 0, 0}, t_best[3] = {0, 0, 0};
     size_t npairs[3] = {0, 0, 0}, agree[3] = {0, 0, 0};
