@@ -236,14 +236,28 @@ def batch_replay(stream, cpu_budget: float, reps: int):
     out = dout.to_array()
     graph_nodes = graph.nodes
     graph.close()
+    # Algorithmic bytes of the batch: the per-product formula summed, with N from
+    # the 32-bit coefficient counts of each product.
+    def coeffs(x):
+        return 2 * x.size - (1 if int(x[-1]) >> 32 == 0 else 0)
+
+    b_alg = 0
+    for a, b, sq in prods:
+        log2n = max(1, (coeffs(a) + coeffs(b) - 2).bit_length())
+        b_alg += survey_bytes(a.size, b.size, log2n, sq)
+
+    def rate(ms):
+        return {"ms": round(ms, 3), "mults_per_s": round(n / ms * 1e3, 1),
+                "roofline_frac": round(b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
     res = {
         "workload": "4096 products, bits lognormal(median 2^18, sigma 0.6) clipped [2^12, 2^24], "
                     "every third a square, splitmix64 seed 7",
         "products": n, "limbs_in": int(total_in), "limbs_out": int(total_out),
-        "per_call": {"ms": round(ms_calls, 3), "mults_per_s": round(n / ms_calls * 1e3, 1)},
-        "batch_call": {"ms": round(ms_batch, 3), "mults_per_s": round(n / ms_batch * 1e3, 1)},
-        "graph_replay": {"ms": round(ms_graph, 3), "mults_per_s": round(n / ms_graph * 1e3, 1),
-                         "graph_nodes": int(graph_nodes)},
+        "algorithmic_bytes": int(b_alg),
+        "per_call": rate(ms_calls),
+        "batch_call": rate(ms_batch),
+        "graph_replay": {**rate(ms_graph), "graph_nodes": int(graph_nodes)},
     }
     if cpu_budget > 0:
         res["cpu_baseline"] = batch_cpu_baseline(prods, out, o_off, cpu_budget)
