@@ -125,7 +125,7 @@ std::vector<BSResult> BinarySplitting::leaves(unsigned long a, unsigned long b, 
 
 // One level: adjacent pairs merge (all products as one multiply_batch); an odd
 // last node moves up unchanged.
-void BinarySplitting::host_level(std::vector<BSResult>& nodes, bool need_top_p) {
+void BinarySplitting::host_level(std::vector<BSResult>& nodes, bool need_top_p, std::vector<Level>& levels) {
     using Clock = std::chrono::steady_clock;
     auto t0 = Clock::now();
     const size_t pairs = nodes.size() / 2;
@@ -152,7 +152,7 @@ void BinarySplitting::host_level(std::vector<BSResult>& nodes, bool need_top_p) 
         mpz_clear(&tmp[k]);
     }
     if (nodes.size() & 1) next.back() = std::move(nodes.back());
-    levels_.push_back(Level{pairs, jobs.size(), max_limbs,
+    levels.push_back(Level{pairs, jobs.size(), max_limbs,
                             std::chrono::duration<double>(Clock::now() - t0).count()});
     nodes.swap(next);
 }
@@ -160,7 +160,7 @@ void BinarySplitting::host_level(std::vector<BSResult>& nodes, bool need_top_p) 
 BSResult BinarySplitting::compute(unsigned long a, unsigned long b, bool need_top_p) {
     levels_.clear();
     std::vector<BSResult> nodes = leaves(a, b);
-    while (nodes.size() > 1) host_level(nodes, need_top_p);
+    while (nodes.size() > 1) host_level(nodes, need_top_p, levels_);
     return std::move(nodes[0]);
 }
 

@@ -52,9 +52,9 @@ public:
     BSResult compute_device(unsigned long a, unsigned long b, bool need_top_p = true, size_t device_from_limbs = 0);
 
     // compute_device cuts [a, b) into this many segments of whole leaf spans
-    // (at least 64 leaf spans each): the host subtrees of segment i+1 are
-    // computed on threads - 1 host threads while the device reduces segment i,
-    // and the segment roots merge on the device.  1: one tree.
+    // (at least 64 leaf spans each): segment i+1 is prepared (host subtrees on
+    // threads - 1 host threads, any host levels, upload) while the device
+    // reduces segment i, and the segment roots merge on the device.  1: one tree.
     void set_segments(unsigned s) { segments_ = s ? s : 1; }
 
     // With a dispatcher set, compute_device switches to device-resident values
@@ -76,14 +76,15 @@ public:
     };
     const std::vector<Level>& levels() const { return levels_; }
     double leaf_seconds() const { return leaf_s_; }      // host subtrees of the last compute* (summed)
-    double leaf_wait_seconds() const { return leaf_wait_s_; }  // compute_device: waited for them
+    double leaf_wait_seconds() const { return leaf_wait_s_; }  // compute_device: waited for prepared segments
     double upload_seconds() const { return upload_s_; }  // compute_device: host -> device switch
 
 private:
     void merge(BSResult& left, BSResult& right, BSResult& out, bool need_p);
     std::vector<BSResult> leaves(unsigned long a, unsigned long b);
     std::vector<BSResult> leaves(unsigned long a, unsigned long b, unsigned threads, double* seconds) const;
-    void host_level(std::vector<BSResult>& nodes, bool need_top_p);
+    // One level on the host (Multiplier::multiply_batch); its record goes to `levels`.
+    void host_level(std::vector<BSResult>& nodes, bool need_top_p, std::vector<Level>& levels);
     BSResult recurse(unsigned long a, unsigned long b, int depth);
 
     Multiplier& mult_;

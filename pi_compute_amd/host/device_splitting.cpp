@@ -2,9 +2,9 @@
 // levels with P, Q and R resident in device memory (see binary_splitting.h).
 //
 // The term range is cut into segments of whole leaf spans.  While the device
-// reduces segment i (the calling thread drives it), the host subtrees of
-// segment i+1 are computed on the other host threads; the segment roots then
-// merge on the device.  P, Q and R of a range do not depend on the tree
+// reduces segment i (the calling thread drives it), segment i+1 is prepared on
+// the other host threads (host subtrees, host levels, upload); the segment
+// roots then merge on the device.  P, Q and R of a range do not depend on the tree
 // shape, so the result is the same as one tree's.
 //
 // Per level: the level's results (P, Q, R of every merge, odd node copied up)
@@ -278,9 +278,20 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
     cut[nseg] = b;
     const unsigned bg_threads = threads_ > 1 ? threads_ - 1 : 1;
 
-    // Host levels of one segment while the operands are small (or the
-    // dispatcher keeps them on the host).
-    auto host_levels = [&](std::vector<BSResult>& nodes, bool top_p) {
+    // One segment from its terms to device-resident values, on any thread:
+    // host subtrees, host levels while the operands are small (or the
+    // dispatcher keeps them on the host), upload.
+    struct Segment {
+        std::unique_ptr<DeviceTree> tree;
+        BSResult root;         // the whole range reduced on the host (one segment only)
+        bool on_host = false;
+        std::vector<Level> levels;
+        double leaf_s = 0, upload_s = 0;
+    };
+    auto prepare = [this, &cut, nseg, need_top_p, device_from_limbs](unsigned long i, unsigned threads) {
+        Segment seg;
+        const bool top_p = nseg == 1 ? need_top_p : true;
+        std::vector<BSResult> nodes = leaves(cut[i], cut[i + 1], threads, &seg.leaf_s);
         while (nodes.size() > 1) {
             if (switch_by_) {
                 if (level_prefers_device(nodes, top_p)) break;
@@ -290,38 +301,39 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
                     smallest = std::min({smallest, mpz_size(n.P), mpz_size(n.Q), mpz_size(n.R)});
                 if (smallest >= device_from_limbs) break;
             }
-            host_level(nodes, top_p);
+            host_level(nodes, top_p, seg.levels);
         }
+        if (nodes.size() == 1 && nseg == 1) {
+            seg.root = std::move(nodes[0]);
+            seg.on_host = true;
+            return seg;
+        }
+        const auto tu = Clock::now();
+        seg.tree = std::make_unique<DeviceTree>();
+        seg.tree->upload(nodes);
+        seg.upload_s = std::chrono::duration<double>(Clock::now() - tu).count();
+        return seg;
     };
 
     std::vector<std::unique_ptr<DeviceTree>> trees;
     std::vector<DevNode> roots;
     auto tw = Clock::now();
-    double secs = 0;
-    std::vector<BSResult> nodes = leaves(cut[0], cut[1], threads_, &secs);
-    leaf_s_ += secs;
+    Segment seg = prepare(0, threads_);
     leaf_wait_s_ += std::chrono::duration<double>(Clock::now() - tw).count();
     for (unsigned long i = 0; i < nseg; ++i) {
-        // The next segment's host subtrees on the other host threads, meanwhile.
-        std::future<std::vector<BSResult>> later;
-        double later_secs = 0;
-        if (i + 1 < nseg)
-            later = std::async(std::launch::async, [this, &cut, i, bg_threads, &later_secs]() {
-                return leaves(cut[i + 1], cut[i + 2], bg_threads, &later_secs);
-            });
-        const bool top_p = nseg == 1 ? need_top_p : true;
-        host_levels(nodes, top_p);
-        if (nseg == 1 && nodes.size() == 1) return std::move(nodes[0]);
-        const auto tu = Clock::now();
-        trees.push_back(std::make_unique<DeviceTree>());
-        trees.back()->upload(nodes);
-        upload_s_ += std::chrono::duration<double>(Clock::now() - tu).count();
-        trees.back()->reduce(top_p, levels_);
-        roots.push_back(trees.back()->root());
+        // The next segment is prepared on the other host threads meanwhile.
+        std::future<Segment> later;
+        if (i + 1 < nseg) later = std::async(std::launch::async, prepare, i + 1, bg_threads);
+        levels_.insert(levels_.end(), seg.levels.begin(), seg.levels.end());
+        leaf_s_ += seg.leaf_s;
+        upload_s_ += seg.upload_s;
+        if (seg.on_host) return std::move(seg.root);
+        seg.tree->reduce(nseg == 1 ? need_top_p : true, levels_);
+        roots.push_back(seg.tree->root());
+        trees.push_back(std::move(seg.tree));
         if (i + 1 < nseg) {
             tw = Clock::now();
-            nodes = later.get();
-            leaf_s_ += later_secs;
+            seg = later.get();
             leaf_wait_s_ += std::chrono::duration<double>(Clock::now() - tw).count();
         }
     }
