@@ -45,11 +45,29 @@ unsigned long BinarySplitting::terms_needed(size_t digits) {
     return static_cast<unsigned long>(std::ceil(static_cast<double>(digits) / 14.1816474627)) + 1;
 }
 
+namespace {
+static_assert(GMP_LIMB_BITS == 64, "64-bit limbs");
+void set_u128(mpz_ptr x, unsigned __int128 v) {
+    mp_limb_t* d = mpz_limbs_write(x, 2);
+    d[0] = (mp_limb_t)v;
+    d[1] = (mp_limb_t)(v >> 64);
+    mpz_limbs_finish(x, d[1] ? 2 : (d[0] ? 1 : 0));
+}
+}  // namespace
+
 void BinarySplitting::base_case(unsigned long k, BSResult& r) {
     if (k == 0) {
         mpz_set_ui(r.P, 1);
         mpz_set_ui(r.Q, 1);
         mpz_set_ui(r.R, 13591409);
+        return;
+    }
+    if (k < (1ul << 24)) {
+        // (6k-5)(2k-1) < 2^52, P < 72 k^3 < 2^79, Q < 2^72 * 2^54: all fit 128 bits.
+        set_u128(r.P, (unsigned __int128)((6 * k - 5) * (2 * k - 1)) * (6 * k - 1));
+        mpz_neg(r.P, r.P);
+        set_u128(r.Q, (unsigned __int128)(k * k) * k * 10939058860032000ul);
+        mpz_mul_ui(r.R, r.P, 545140134ul * k + 13591409ul);
         return;
     }
     mpz_set_ui(r.P, 6 * k - 5);
@@ -66,23 +84,62 @@ void BinarySplitting::base_case(unsigned long k, BSResult& r) {
     mpz_mul(r.R, r.R, r.P);
 }
 
+namespace {
+// l covers the terms just before r's: P = Pl Pr, Q = Ql Qr, R = Rl Qr + Pl Rr.
+void merge_into(BSResult& l, const BSResult& r, mpz_ptr t) {
+    mpz_mul(t, l.R, r.Q);
+    mpz_mul(l.R, l.P, r.R);
+    mpz_add(l.R, l.R, t);
+    mpz_mul(l.P, l.P, r.P);
+    mpz_mul(l.Q, l.Q, r.Q);
+}
+
+// Per-thread scratch of compute_small, kept across calls so that its GMP
+// buffers are allocated once per thread rather than once per merge.
+struct SmallScratch {
+    std::vector<BSResult> stack;
+    std::vector<unsigned long> count;
+    mpz_t t;
+    SmallScratch() { mpz_init(t); }
+    ~SmallScratch() { mpz_clear(t); }
+};
+}  // namespace
+
+// Bottom-up over [a, b): a stack of partial results whose term counts are
+// decreasing powers of two (a binary counter); two equal neighbours merge, and
+// the rest merges right to left at the end.  The tree differs from the
+// recursive halving only in shape, which P, Q and R do not depend on.
 void BinarySplitting::compute_small(unsigned long a, unsigned long b, BSResult& r) {
-    if (b - a == 1) {
-        base_case(a, r);
+    if (b <= a) {
+        mpz_set_ui(r.P, 1);
+        mpz_set_ui(r.Q, 1);
+        mpz_set_ui(r.R, 0);
         return;
     }
-    unsigned long m = a + (b - a) / 2;
-    BSResult L, Rr;
-    compute_small(a, m, L);
-    compute_small(m, b, Rr);
-    mpz_t t;
-    mpz_init(t);
-    mpz_mul(t, Rr.Q, L.R);
-    mpz_mul(r.R, L.P, Rr.R);
-    mpz_add(r.R, r.R, t);
-    mpz_mul(r.P, L.P, Rr.P);
-    mpz_mul(r.Q, L.Q, Rr.Q);
-    mpz_clear(t);
+    thread_local SmallScratch sc;
+    size_t top = 0;
+    for (unsigned long k = a; k < b; ++k) {
+        if (sc.stack.size() <= top) {
+            sc.stack.emplace_back();
+            sc.count.push_back(0);
+        }
+        base_case(k, sc.stack[top]);
+        sc.count[top] = 1;
+        ++top;
+        while (top >= 2 && sc.count[top - 2] == sc.count[top - 1]) {
+            merge_into(sc.stack[top - 2], sc.stack[top - 1], sc.t);
+            sc.count[top - 2] *= 2;
+            --top;
+        }
+    }
+    while (top >= 2) {
+        merge_into(sc.stack[top - 2], sc.stack[top - 1], sc.t);
+        sc.count[top - 2] += sc.count[top - 1];
+        --top;
+    }
+    mpz_swap(r.P, sc.stack[0].P);
+    mpz_swap(r.Q, sc.stack[0].Q);
+    mpz_swap(r.R, sc.stack[0].R);
 }
 
 void BinarySplitting::merge(BSResult& l, BSResult& r, BSResult& out, bool need_p) {

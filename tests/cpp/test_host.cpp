@@ -88,6 +88,30 @@ int main(int argc, char** argv) {
         pi::BSResult r2 = bs.compute(2, 3);
         mpz_set_str(q, "87512470880256000", 10);
         check(mpz_cmp_si(r2.P, -231) == 0 && mpz_cmp(r2.Q, q) == 0, "BaseCaseTerm2");
+        // The 128-bit path (k < 2^24) and the mpz path agree with mpz arithmetic at
+        // large k on both sides of the switch.
+        {
+            bool ok = true;
+            mpz_t p, qq, rr;
+            mpz_inits(p, qq, rr, NULL);
+            for (unsigned long k : {1000ul, 7051375ul, (1ul << 24) - 1, 1ul << 24, (1ul << 24) + 7, 1ul << 30}) {
+                pi::BSResult t;
+                pi::BinarySplitting::base_case(k, t);
+                mpz_set_ui(p, 6 * k - 5);
+                mpz_mul_ui(p, p, 2 * k - 1);
+                mpz_mul_ui(p, p, 6 * k - 1);
+                mpz_neg(p, p);
+                mpz_ui_pow_ui(qq, k, 3);
+                mpz_mul_ui(qq, qq, 10939058860032000ul);
+                mpz_set_ui(rr, 545140134ul);
+                mpz_mul_ui(rr, rr, k);
+                mpz_add_ui(rr, rr, 13591409ul);
+                mpz_mul(rr, rr, p);
+                ok = ok && mpz_cmp(t.P, p) == 0 && mpz_cmp(t.Q, qq) == 0 && mpz_cmp(t.R, rr) == 0;
+            }
+            mpz_clears(p, qq, rr, NULL);
+            check(ok, "BaseCaseLargeTerms");
+        }
         // [0,2): P = P0 P1, Q = Q0 Q1, R = Q1 R0 + P0 R1
         bs.set_leaf_span(1);
         pi::BSResult m = bs.compute(0, 2);
