@@ -347,6 +347,28 @@ def from_int(x: int, nlimbs: int | None = None) -> np.ndarray:
 
 # ---- device-resident limb arrays (binary splitting's upper levels) ----
 
+def upload(pairs) -> None:
+    """Host limbs into device memory in one call: [(device address, numpy uint64 array)]."""
+    lib = load_library()
+    srcs = [np.ascontiguousarray(x, np.uint64) for _, x in pairs]
+    n = len(pairs)
+    dst = (_u64p * n)(*[ctypes.cast(d, _u64p) for d, _ in pairs])
+    src = (_u64p * n)(*[_ptr(x) for x in srcs])
+    _check(lib, lib.gpu_ntt_upload(n, dst, src, (ctypes.c_size_t * n)(*[x.size for x in srcs])))
+
+
+def download(pairs) -> None:
+    """Device memory into host limbs in one call: [(numpy uint64 array, written in place; device address)]."""
+    lib = load_library()
+    for x, _ in pairs:
+        if x.dtype != np.uint64 or not x.flags.c_contiguous:
+            raise ValueError("download needs contiguous uint64 arrays")
+    n = len(pairs)
+    dst = (_u64p * n)(*[_ptr(x) for x, _ in pairs])
+    src = (_u64p * n)(*[ctypes.cast(d, _u64p) for _, d in pairs])
+    _check(lib, lib.gpu_ntt_download(n, dst, src, (ctypes.c_size_t * n)(*[x.size for x, _ in pairs])))
+
+
 def add_batch(pairs, subtract) -> list:
     """Device add/subtract of magnitudes: [(x, y)] numpy uint64 limb arrays,
     subtract[i] -> x - y (needs x >= y).  Uploads, runs one batch, downloads."""

@@ -392,3 +392,49 @@ def test_plan_file_round_trip(engine, tmp_path):
     assert (after["log2r"], after["log2w"], after["log2rows"]) == (
         before[12]["log2r"], before[12]["log2w"], before[12]["log2rows"])
     assert lib.gpu_ntt_load_plans(str(tmp_path / "missing.txt").encode()) == 5
+
+
+def test_concurrent_callers(engine, oracle):
+    """Host threads calling the engine at once (products, uploads, downloads):
+    each call's staging is its own while the device work is serialised, and
+    every result equals mpz_mul's."""
+    import threading
+
+    from pi_compute_amd import hiprt
+
+    sizes = [(50000, 30000), (1 << 17, 1 << 15), (3000, 2999), (1 << 19, 1 << 18), (777, 12345)]
+    cases = [[pc.operands(9000 + 10 * t + i, 64 * an, 64 * bn) for i, (an, bn) in enumerate(sizes)]
+             for t in range(4)]
+    wants = [[oracle.gmp_mul(a, b) for a, b in c] for c in cases]
+    got = [[None] * len(sizes) for _ in cases]
+    moved = [None] * len(cases)
+    errors = []
+
+    def worker(t):
+        try:
+            for i, (a, b) in enumerate(cases[t]):
+                got[t][i] = engine.mul(a, b)
+            # an upload and a download of this thread's own device buffer
+            x = cases[t][0][0]
+            buf = hiprt.DeviceBuffer(8 * x.size)
+            try:
+                back = np.zeros_like(x)
+                engine.upload([(buf.ptr, x)])
+                engine.download([(back, buf.ptr)])
+                moved[t] = back
+            finally:
+                buf.close()
+        except Exception as e:  # reported below, on the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(len(cases))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=240)
+    assert not any(th.is_alive() for th in threads), "a caller did not finish"
+    assert not errors, errors
+    for t in range(len(cases)):
+        for i in range(len(sizes)):
+            assert np.array_equal(got[t][i], wants[t][i]), f"thread {t} product {i}"
+        assert np.array_equal(moved[t], cases[t][0][0]), f"thread {t} upload/download"
