@@ -29,6 +29,7 @@
 #include <cstring>
 #include <condition_variable>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -868,7 +869,7 @@ bool overlaps(const uint64_t* x, size_t xn, const uint64_t* y, size_t yn) { retu
 int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const uint64_t* const* b,
                const size_t* bn, uint64_t* const* out) {
     if (!a || !an || !b || !bn || !out) return fail(PINTT_ERR_ARGS, "null pointer");
-    size_t total_in = 0, total_out = 0;
+    size_t total_out = 0;
     for (size_t i = 0; i < count; ++i) {
         if (!a[i] || !b[i] || !out[i] || an[i] == 0 || bn[i] == 0)
             return fail(PINTT_ERR_ARGS, "null pointer or zero length");
@@ -876,9 +877,29 @@ int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const u
             return fail(PINTT_ERR_ARGS, "output overlaps an operand");
         if (an[i] + bn[i] > ((size_t)1 << (kMaxLog2N - 1)))
             return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
-        bool sq = a[i] == b[i] && an[i] == bn[i];
-        total_in += an[i] + (sq ? 0 : bn[i]);
         total_out += an[i] + bn[i];
+    }
+    // Operand offsets inside the staging (host and device alike).  An operand
+    // that several products share (same address and length, e.g. one power
+    // of ten times many numbers) is staged and copied once.
+    std::vector<size_t> a_off(count), b_off(count), o_off(count);
+    std::vector<std::pair<const uint64_t*, size_t>> staged_src;  // (address, limbs) in staging order
+    std::unordered_map<const uint64_t*, std::vector<size_t>> seen;  // address -> indices into staged
+    std::vector<size_t> staged_off;
+    size_t total_in = 0;
+    auto stage = [&](const uint64_t* x, size_t n) {
+        auto& v = seen[x];
+        for (size_t k : v)
+            if (staged_src[k].second == n) return staged_off[k];
+        v.push_back(staged_src.size());
+        staged_src.emplace_back(x, n);
+        staged_off.push_back(total_in);
+        total_in += n;
+        return staged_off.back();
+    };
+    for (size_t i = 0; i < count; ++i) {
+        a_off[i] = stage(a[i], an[i]);
+        b_off[i] = stage(b[i], bn[i]);
     }
     {
         std::lock_guard<std::mutex> lock(g_mu);
@@ -886,22 +907,13 @@ int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const u
     }
     StagingSlot st;
     PINTT_TRY(st.acquire(std::max(total_in, total_out)));
-    // Operand and product offsets inside the staging (host and device alike).
-    std::vector<size_t> a_off(count), b_off(count), o_off(count);
     std::vector<Piece> gather, scatter;
-    gather.reserve(2 * count);
+    gather.reserve(staged_src.size());
     scatter.reserve(count);
-    size_t in_off = 0, out_off = 0;
+    for (size_t k = 0; k < staged_src.size(); ++k)
+        gather.push_back(Piece{st.in() + staged_off[k], staged_src[k].first, staged_src[k].second});
+    size_t out_off = 0;
     for (size_t i = 0; i < count; ++i) {
-        const bool sq = a[i] == b[i] && an[i] == bn[i];
-        a_off[i] = b_off[i] = in_off;
-        gather.push_back(Piece{st.in() + in_off, a[i], an[i]});
-        in_off += an[i];
-        if (!sq) {
-            b_off[i] = in_off;
-            gather.push_back(Piece{st.in() + in_off, b[i], bn[i]});
-            in_off += bn[i];
-        }
         o_off[i] = out_off;
         scatter.push_back(Piece{out[i], st.out() + out_off, an[i] + bn[i]});
         out_off += an[i] + bn[i];
