@@ -20,6 +20,7 @@
 //   --convert-leaf D          multiplier conversion: host conversion below D digits (32768)
 //   --bs-segments S           device levels: S segments, host subtrees of the next one computed
 //                             while the device reduces the current one (8; 1 = one tree)
+//   --bs-leaf T               terms per host subtree at the bottom of the tree (256)
 //   --no-overlap              Newton final step: compute sqrt(10005) and 10^D after binary
 //                             splitting instead of beside it
 //   --device-levels [L|auto]  with --gpu --tree levels: P, Q, R stay on the GPU from
@@ -58,7 +59,7 @@ void usage() {
                  "                      [--threads T] [--tree levels|recursive] [--per-call]\n"
                  "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
                  "                      [--final mpf|newton] [--convert host|multiplier] [--convert-leaf D]\n"
-                 "                      [--device-levels [LIMBS|auto]] [--bs-segments S] [--no-overlap]\n"
+                 "                      [--device-levels [LIMBS|auto]] [--bs-segments S] [--bs-leaf T] [--no-overlap]\n"
                  "                      [--plans FILE]\n"
                  "                      [--verbose]\n";
 }
@@ -100,6 +101,7 @@ int main(int argc, char** argv) {
         else if (a == "--convert-leaf") cfg.conversion_leaf_digits = std::stoul(next());
         else if (a == "--bs-segments") cfg.bs_segments = (unsigned)std::stoul(next());
         else if (a == "--no-overlap") cfg.overlap_constants = false;
+        else if (a == "--bs-leaf") cfg.bs_leaf_span = std::stoul(next());
         else if (a == "--device-levels") {
             cfg.device_levels = true;
             if (i + 1 < argc && argv[i + 1][0] != '-') {

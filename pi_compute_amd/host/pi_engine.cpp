@@ -75,6 +75,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     BinarySplitting bs(mult_, cfg.num_threads);
     bs.set_switch_dispatcher(cfg.device_switch);
     bs.set_segments(cfg.bs_segments);
+    bs.set_leaf_span(cfg.bs_leaf_span);
     BSResult r;
     if (cfg.level_batches && cfg.device_levels) {
         try {
@@ -204,8 +205,19 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
     // 426880 * sqrt(10005) * Q * 10^D / R
     //   = 10005 * (426880 Q) * (2^w / sqrt(10005)) * 10^D * (2^(nr+w) / Rt) / 2^(...)
     // with every operand cut to its top w bits and the exponents of 2 tracked.
-    mpz_t a, y, t, inv;
-    mpz_inits(a, y, t, inv, NULL);
+    // The reciprocal of R runs on another thread while the numerator's
+    // products run here; they meet in the last product.
+    mpz_t a, y, t, inv, rt;
+    mpz_inits(a, y, t, inv, rt, NULL);
+    size_t sr = 0, nr = 0;
+    double recip_s = 0;
+    auto recip = std::async(std::launch::async, [&]() {
+        const auto t0 = Clock::now();
+        sr = truncate_top(rt, R, w);
+        nr = mpz_sizeinbase(rt, 2);
+        newton_reciprocal(inv, rt, w, m);  // 1/R ~ inv / 2^(nr + w + sr)
+        recip_s = since(t0);
+    });
     long ex = 0;  // value = integer * 2^ex
     mpz_mul_ui(a, Q, 426880);
     ex += (long)truncate_top(a, a, w);
@@ -227,11 +239,10 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
         m.multiply(y, t, a);
     }
     ex += (long)truncate_top(y, y, w);
-    const size_t sr = truncate_top(t, R, w);
-    const size_t nr = mpz_sizeinbase(t, 2);
     st.products_s += lap();
-    newton_reciprocal(inv, t, w, m);  // 1/R ~ inv / 2^(nr + w + sr)
-    st.reciprocal_s = lap();
+    recip.get();  // rethrows what the reciprocal's thread threw
+    st.reciprocal_s = recip_s;
+    lap();
     ex -= (long)(nr + w + sr);
     m.multiply(a, y, inv);
     mpz_mul_ui(a, a, 10005);
@@ -239,7 +250,7 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
         mpz_mul_2exp(out, a, (mp_bitcnt_t)ex);
     else
         mpz_fdiv_q_2exp(out, a, (mp_bitcnt_t)(-ex));
-    mpz_clears(a, y, t, inv, NULL);
+    mpz_clears(a, y, t, inv, rt, NULL);
     st.products_s += lap();
     if (stats) *stats = st;
 }

@@ -37,6 +37,7 @@ struct PiConfig {
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
     size_t conversion_leaf_digits = 1 << 15;  // Conversion::Multiplier: host conversion below this width
     unsigned bs_segments = 8;                  // device levels: segments pipelined with their host subtrees
+    unsigned long bs_leaf_span = 256;          // terms per host subtree at the bottom of the tree
     // sqrt(10005) and 10^D (FinalStep::Newton) and the conversion's powers of
     // ten and their reciprocals (Conversion::Multiplier) depend only on D, so
     // they are computed on another thread while binary splitting runs.
