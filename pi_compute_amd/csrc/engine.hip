@@ -50,6 +50,7 @@ struct Engine {
     hipEvent_t done = nullptr;  // end of the last batch
     hipStream_t last = nullptr; // stream that batch ran on
     bool tw_ready[kSets] = {};
+    uint32_t* tw_pow = nullptr;  // powers of the root while tables are built
     uint2* tw_fwd[kSets][kPrimes] = {};
     uint2* tw_inv[kSets][kPrimes] = {};
     size_t arena_words = 0;
@@ -217,10 +218,11 @@ int ensure_twiddles(int log2n) {
     const int set = set_for(log2n);
     if (g.tw_ready[set]) return PINTT_OK;
     size_t cnt = (size_t)1 << kSetMaxLog2N[set];
+    if (!g.tw_pow) PINTT_HIP(hipMalloc(reinterpret_cast<void**>(&g.tw_pow), kPowScratch * sizeof(uint32_t)));
     for (int i = 0; i < kPrimes; ++i) {
         PINTT_TRY(grow(g.tw_fwd[set][i], cnt));
         PINTT_TRY(grow(g.tw_inv[set][i], cnt));
-        PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], set, i, g.stream));
+        PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], g.tw_pow, set, i, g.stream));
     }
     g.tw_ready[set] = true;
     return PINTT_OK;
@@ -961,6 +963,7 @@ void gpu_ntt_shutdown(void) {
         }
     (void)hipFree(g.arena);
     (void)hipFree(g.maps);
+    (void)hipFree(g.tw_pow);
     for (int k = 0; k < Engine::kDescRing; ++k) {
         (void)hipFree(g.desc_dev[k]);
         if (g.desc_host[k]) (void)hipHostFree(g.desc_host[k]);

@@ -318,10 +318,20 @@ __device__ __forceinline__ uint32_t powmod_dev(uint32_t b, uint32_t e, uint32_t 
     return (uint32_t)r;
 }
 
+// Powers of wK in two tables, lo[i] = wK^i (i < 2^12) and hi[i] = wK^(i 2^12)
+// (i <= 2^(K-12)): every exponent e <= 2^K is hi[e >> 12] * lo[e & 4095].
+__global__ void k_power_tables(uint32_t* __restrict__ lo, uint32_t* __restrict__ hi, uint32_t p, uint32_t wk,
+                               uint32_t nhi) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kPowLo) lo[i] = powmod_dev(wk, i, p);
+    if (i < nhi) hi[i] = powmod_dev(wk, i << kPowLoBits, p);
+}
+
 // Z[i], i = 2^s + b: w_{2^(s+1)}^bitrev_s(b) = wK^(bitrev_s(b) * 2^(K - 1 - s)),
-// wK a primitive 2^K-th root of unity (K: the set's longest transform).
-__global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ inv, uint32_t p,
-                                 uint32_t wk, int log2k, uint32_t count) {
+// wK a primitive 2^K-th root of unity (K: the set's longest transform); the
+// inverse table holds wK^(2^K - e).
+__global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ inv, const uint32_t* __restrict__ lo,
+                                 const uint32_t* __restrict__ hi, uint32_t p, int log2k, uint32_t count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     uint32_t ef = 0;
@@ -332,15 +342,18 @@ __global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ in
         ef = rev << (log2k - 1 - s);
     }
     uint32_t ei = (ef == 0) ? 0u : ((1u << log2k) - ef);
-    uint32_t z = (i == 0) ? 1u : powmod_dev(wk, ef, p);
-    uint32_t zi = (i == 0) ? 1u : powmod_dev(wk, ei, p);
+    auto power = [&](uint32_t e) {
+        return (uint32_t)((uint64_t)hi[e >> kPowLoBits] * lo[e & (kPowLo - 1)] % p);
+    };
+    uint32_t z = (i == 0) ? 1u : power(ef);
+    uint32_t zi = (i == 0) ? 1u : power(ei);
     fwd[i] = make_uint2(z, (uint32_t)(((uint64_t)z << 32) / p));
     inv[i] = make_uint2(zi, (uint32_t)(((uint64_t)zi << 32) / p));
 }
 
 }  // namespace
 
-hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int set, int prime, hipStream_t s) {
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* pow_scratch, int set, int prime, hipStream_t s) {
     const uint32_t p = kPrimeSet[set][prime];
     const int log2k = kSetMaxLog2N[set];
     // wK = g^((p-1)/2^K).
@@ -351,9 +364,16 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int set, int prime, hip
         x = (x * x) % p;
         e >>= 1;
     }
+    const uint32_t nhi = (1u << (log2k - kPowLoBits)) + 1;
+    uint32_t* lo = pow_scratch;
+    uint32_t* hi = pow_scratch + kPowLo;
+    const uint32_t nt = nhi > kPowLo ? nhi : kPowLo;
+    hipLaunchKernelGGL(k_power_tables, dim3((nt + 255) / 256), dim3(256), 0, s, lo, hi, p, (uint32_t)r, nhi);
+    const hipError_t e1 = hipGetLastError();
+    if (e1 != hipSuccess) return e1;
     uint32_t count = 1u << log2k;
-    hipLaunchKernelGGL(k_build_twiddles, dim3((count + 255) / 256), dim3(256), 0, s, fwd, inv, p,
-                       (uint32_t)r, log2k, count);
+    hipLaunchKernelGGL(k_build_twiddles, dim3((count + 255) / 256), dim3(256), 0, s, fwd, inv, lo, hi, p, log2k,
+                       count);
     return hipGetLastError();
 }
 

@@ -180,8 +180,12 @@ inline JobList<T> inline_jobs(const T* host, uint32_t n) {
 
 // Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip); counts are
 // host values.
-// Tables of 2^kSetMaxLog2N[set] entries for one prime of a set.
-hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, int set, int prime, hipStream_t s);
+// Tables of 2^kSetMaxLog2N[set] entries for one prime of a set.  pow_scratch:
+// device room for kPowScratch words (powers of the root, used on the stream).
+constexpr int kPowLoBits = 12;
+constexpr uint32_t kPowLo = 1u << kPowLoBits;
+constexpr uint32_t kPowScratch = kPowLo + (1u << (kMaxLog2N - kPowLoBits)) + 1;
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* pow_scratch, int set, int prime, hipStream_t s);
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,
