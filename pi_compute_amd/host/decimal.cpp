@@ -98,14 +98,45 @@ std::string to_decimal(mpz_srcptr n, unsigned threads) {
 
 namespace {
 
+// A node of the level-by-level conversion: the value is the `size` limbs at
+// `limbs`, which `own` holds (or the caller's number, for the root), so a
+// quotient taken as the top limbs of a product needs no copy.
 struct Node {
-    mpz_t x;
-    size_t offset, width;  // digits [offset, offset + width) of the output; x < 10^width
-    Node() { mpz_init(x); }
-    ~Node() { mpz_clear(x); }
+    mpz_t own;
+    const mp_limb_t* limbs = nullptr;
+    size_t size = 0;
+    __mpz_struct view;
+    size_t offset, width;  // digits [offset, offset + width) of the output; value < 10^width
+    Node() { mpz_init(own); }
+    ~Node() { mpz_clear(own); }
     Node(const Node&) = delete;
     Node& operator=(const Node&) = delete;
+    void take(mpz_t x) {  // the value becomes x (x is left with the old storage)
+        mpz_swap(own, x);
+        limbs = mpz_limbs_read(own);
+        size = mpz_size(own);
+    }
+    void take_top(mpz_t x, size_t skip) {  // the value becomes x >> (GMP_NUMB_BITS * skip), x >= 0
+        mpz_swap(own, x);
+        const size_t n = mpz_size(own);
+        limbs = mpz_limbs_read(own) + std::min(skip, n);
+        size = n > skip ? n - skip : 0;
+    }
+    mpz_srcptr x() { return mpz_roinit_n(&view, limbs, (mp_size_t)size); }
 };
+
+// One split by P = 10^(2^j) of bit length np: the quotient estimate is
+// ((x >> sx) * inv) >> (e2 - sx) with inv ~ 2^e2 / P.  Both shifts are whole
+// limbs, so the shifted operands are views of the numbers they come from.
+// With x < P^2, |inv - 2^e2/P| <= 4 and 2^sx < P / 2^63 the estimate is
+// within 6 of floor(x / P).
+struct SplitShifts {
+    size_t sx, e2;
+};
+SplitShifts split_shifts(size_t np) {
+    const size_t L = GMP_NUMB_BITS;
+    return SplitShifts{(np > 64 ? np - 64 : 0) / L * L, (2 * np + L - 1) / L * L};
+}
 
 }  // namespace
 
@@ -135,8 +166,10 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
     }
     d.powers_s = std::chrono::duration<double>(Clock::now() - t).count();
     t = Clock::now();
-    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j)
-        newton_reciprocal(&d.inv[j], &d.pw[j], mpz_sizeinbase(&d.pw[j], 2), m);  // ~ 2^(2 np) / P, within 4
+    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
+        const size_t np = mpz_sizeinbase(&d.pw[j], 2);
+        newton_reciprocal(&d.inv[j], &d.pw[j], split_shifts(np).e2 - np, m);  // ~ 2^e2 / P, within 4
+    }
     d.reciprocals_s = std::chrono::duration<double>(Clock::now() - t).count();
 }
 
@@ -172,7 +205,8 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
     lap(t);
     std::vector<std::unique_ptr<Node>> nodes;
     nodes.emplace_back(new Node);
-    mpz_abs(nodes[0]->x, n);
+    nodes[0]->limbs = mpz_limbs_read(n);  // |n|
+    nodes[0]->size = mpz_size(n);
     nodes[0]->offset = 0;
     nodes[0]->width = width;
 
@@ -184,23 +218,26 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
         if (split.empty()) continue;
         mpz_srcptr P = &pre->pw[j];
         mpz_srcptr inv = &pre->inv[j];
-        const size_t np = mpz_sizeinbase(P, 2);
+        const SplitShifts sh = split_shifts(mpz_sizeinbase(P, 2));
+        const size_t qskip = (sh.e2 - sh.sx) / GMP_NUMB_BITS;
         lap(t);
-        const size_t sx = np > 64 ? np - 64 : 0;
         const size_t ns = split.size();
-        std::vector<__mpz_struct> hi(ns), q(ns), qp(ns);
-        for (size_t s = 0; s < ns; ++s) {
-            mpz_init(&hi[s]);
-            mpz_init(&q[s]);
-            mpz_init(&qp[s]);
-            mpz_tdiv_q_2exp(&hi[s], nodes[split[s]]->x, sx);
-        }
+        std::vector<__mpz_struct> hi(ns), qv(ns), qfull(ns), qp(ns);
         std::vector<MulJob> jobs(ns);
-        for (size_t s = 0; s < ns; ++s) jobs[s] = MulJob{&q[s], &hi[s], inv};
+        for (size_t s = 0; s < ns; ++s) {
+            const Node& nd = *nodes[split[s]];
+            const size_t skip = std::min(sh.sx / GMP_NUMB_BITS, nd.size);
+            mpz_roinit_n(&hi[s], nd.limbs + skip, (mp_size_t)(nd.size - skip));  // x >> sx
+            mpz_init(&qfull[s]);
+            mpz_init(&qp[s]);
+            jobs[s] = MulJob{&qfull[s], &hi[s], inv};
+        }
         m.multiply_batch(jobs.data(), ns, threads);
         for (size_t s = 0; s < ns; ++s) {
-            mpz_fdiv_q_2exp(&q[s], &q[s], 2 * np - sx);
-            jobs[s] = MulJob{&qp[s], &q[s], P};
+            const size_t n = mpz_size(&qfull[s]);
+            const size_t skip = std::min(qskip, n);
+            mpz_roinit_n(&qv[s], mpz_limbs_read(&qfull[s]) + skip, (mp_size_t)(n - skip));  // the estimate
+            jobs[s] = MulJob{&qp[s], &qv[s], P};
         }
         m.multiply_batch(jobs.data(), ns, threads);
         st.products_s += lap(t);
@@ -209,40 +246,55 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
         struct Ctx {
             std::vector<std::unique_ptr<Node>>* nodes;
             const std::vector<size_t>* split;
-            std::vector<__mpz_struct>* q;
+            std::vector<__mpz_struct>* qv;
+            std::vector<__mpz_struct>* qfull;
             std::vector<__mpz_struct>* qp;
             std::vector<std::unique_ptr<Node>>* lows;
             mpz_srcptr P;
-            size_t low;
-        } ctx{&nodes, &split, &q, &qp, &lows, P, low};
+            size_t low, qskip;
+        } ctx{&nodes, &split, &qv, &qfull, &qp, &lows, P, low, qskip};
         parallel_for(
             ns, threads,
             [](void* p, size_t s) {
                 auto* c = static_cast<Ctx*>(p);
                 Node& nd = *(*c->nodes)[(*c->split)[s]];
-                mpz_ptr qq = &(*c->q)[s];
                 auto* lo = new Node;
-                mpz_sub(lo->x, nd.x, &(*c->qp)[s]);  // remainder estimate
-                while (mpz_sgn(lo->x) < 0) {
-                    mpz_add(lo->x, lo->x, c->P);
-                    mpz_sub_ui(qq, qq, 1);
+                mpz_t r;
+                mpz_init(r);
+                mpz_sub(r, nd.x(), &(*c->qp)[s]);  // remainder of the estimate
+                long delta = 0;                    // quotient = estimate + delta
+                while (mpz_sgn(r) < 0) {
+                    mpz_add(r, r, c->P);
+                    --delta;
                 }
-                while (mpz_cmp(lo->x, c->P) >= 0) {
-                    mpz_sub(lo->x, lo->x, c->P);
-                    mpz_add_ui(qq, qq, 1);
+                while (mpz_cmp(r, c->P) >= 0) {
+                    mpz_sub(r, r, c->P);
+                    ++delta;
                 }
+                lo->take(r);
+                mpz_clear(r);
                 lo->width = c->low;
                 lo->offset = nd.offset + nd.width - c->low;
                 nd.width -= c->low;
-                mpz_swap(nd.x, qq);
+                if (delta == 0) {
+                    nd.take_top(&(*c->qfull)[s], c->qskip);
+                } else {
+                    mpz_t q;
+                    mpz_init(q);
+                    if (delta > 0)
+                        mpz_add_ui(q, &(*c->qv)[s], (unsigned long)delta);
+                    else
+                        mpz_sub_ui(q, &(*c->qv)[s], (unsigned long)-delta);
+                    nd.take(q);
+                    mpz_clear(q);
+                }
                 (*c->lows)[s].reset(lo);
             },
             &ctx);
         for (auto& lo : lows) nodes.push_back(std::move(lo));
         st.corrections_s += lap(t);
         for (size_t s = 0; s < ns; ++s) {
-            mpz_clear(&hi[s]);
-            mpz_clear(&q[s]);
+            mpz_clear(&qfull[s]);
             mpz_clear(&qp[s]);
         }
     }
@@ -262,9 +314,9 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
             auto* c = static_cast<LeafCtx*>(p);
             Node& nd = *(*c->nodes)[i];
             if (nd.width <= kLeafDigits) {
-                leaf(nd.x, &(*c->out)[nd.offset], nd.width);
+                leaf(nd.x(), &(*c->out)[nd.offset], nd.width);
             } else {
-                convert(nd.x, &(*c->out)[nd.offset], nd.width, *c->pw, 0);
+                convert(nd.x(), &(*c->out)[nd.offset], nd.width, *c->pw, 0);
             }
         },
         &lctx, cost.data());

@@ -37,7 +37,8 @@ struct DecimalStats {
 struct DecimalPowers {
     size_t width = 0, leaf_digits = 0;
     std::vector<__mpz_struct> pw;   // 10^(2^j), j <= k with 2^k < width <= 2^(k+1)
-    std::vector<__mpz_struct> inv;  // ~2^(2 bits(pw[j])) / pw[j] where 2^j >= leaf_digits, else 0
+    // ~2^e / pw[j], e = 2 bits(pw[j]) rounded up to whole limbs, where 2^j >= leaf_digits; else 0
+    std::vector<__mpz_struct> inv;
     double powers_s = 0, reciprocals_s = 0;
     DecimalPowers() = default;
     ~DecimalPowers();

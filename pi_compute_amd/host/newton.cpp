@@ -32,13 +32,42 @@ void fshift(mpz_t t, mpz_srcptr x, long s) {
         mpz_mul_2exp(t, x, (mp_bitcnt_t)(-s));
 }
 
+// e = 2^K - p for p > 0.  When |2^K - p| < 2^(L-1) (L < K: the usual case in
+// a Newton step, p being a product that approximates 2^K) the bits of p from
+// L-1 up to K are all ones or all zeros up to bit K, which two scans confirm,
+// and e follows from the low L bits of p; otherwise the full difference.
+void pow2_minus(mpz_t e, mpz_srcptr p, size_t K, size_t L) {
+    if (L < K && mpz_sgn(p) > 0) {
+        const size_t n = mpz_sizeinbase(p, 2);
+        if (n == K + 1 && mpz_scan1(p, L - 1) == K) {  // p = 2^K + (p mod 2^(L-1))
+            mpz_tdiv_r_2exp(e, p, L);
+            mpz_neg(e, e);
+            return;
+        }
+        if (n == K && mpz_scan0(p, L - 1) >= K) {  // p = 2^K - 2^L + (p mod 2^L)
+            mpz_t neg;
+            mpz_roinit_n(neg, mpz_limbs_read(p), -(mp_size_t)mpz_size(p));
+            mpz_fdiv_r_2exp(e, neg, L);  // 2^L - (p mod 2^L)
+            return;
+        }
+    }
+    mpz_set_ui(e, 1);
+    mpz_mul_2exp(e, e, K);
+    mpz_sub(e, e, p);
+}
+
 }  // namespace
 
 size_t truncate_top(mpz_t y, mpz_srcptr x, size_t bits) {
     const size_t n = mpz_sgn(x) ? mpz_sizeinbase(x, 2) : 0;
     const size_t s = n > bits ? n - bits : 0;
-    mpz_abs(y, x);
-    if (s) mpz_tdiv_q_2exp(y, y, s);
+    // Shift straight from x (truncation is symmetric in the sign), so only
+    // the kept bits are copied.
+    if (s)
+        mpz_tdiv_q_2exp(y, x, s);
+    else
+        mpz_set(y, x);
+    mpz_abs(y, y);
     return s;
 }
 
@@ -61,10 +90,8 @@ void newton_reciprocal(mpz_t y, mpz_srcptr d, size_t b, Multiplier& m) {
         const size_t c = *it;
         const size_t sd = truncate_top(dt, d, c + kGuard);
         const size_t K = (n - sd) + cur;
-        m.multiply(p, dt, y);  // ~ 2^K
-        mpz_set_ui(e, 1);
-        mpz_mul_2exp(e, e, K);
-        mpz_sub(e, e, p);  // signed
+        m.multiply(p, dt, y);               // ~ 2^K
+        pow2_minus(e, p, K, K - cur + 16);  // signed; |e| < 2^(K - cur + 4)
         const int sign = mpz_sgn(e);
         const size_t se = truncate_top(e, e, c - cur + kGuard);
         if (sign < 0) mpz_neg(e, e);
@@ -92,10 +119,8 @@ void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m) {
     for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
         const size_t c = *it;
         m.square(p, y);
-        mpz_mul_ui(p, p, a);  // ~ 2^(2 cur)
-        mpz_set_ui(e, 1);
-        mpz_mul_2exp(e, e, 2 * cur);
-        mpz_sub(e, e, p);
+        mpz_mul_ui(p, p, a);                  // ~ 2^(2 cur)
+        pow2_minus(e, p, 2 * cur, cur + 16);  // signed; |e| < 2^(cur + 5)
         const int sign = mpz_sgn(e);
         const size_t se = truncate_top(e, e, c - cur + kGuard);
         if (sign < 0) mpz_neg(e, e);
