@@ -32,12 +32,13 @@ void fshift(mpz_t t, mpz_srcptr x, long s) {
         mpz_mul_2exp(t, x, (mp_bitcnt_t)(-s));
 }
 
-// e = 2^K - p for p > 0.  When |2^K - p| < 2^(L-1) (L < K: the usual case in
-// a Newton step, p being a product that approximates 2^K) the bits of p from
-// L-1 up to K are all ones or all zeros up to bit K, which two scans confirm,
-// and e follows from the low L bits of p; otherwise the full difference.
+}  // namespace
+
+// When |2^K - p| < 2^(L-1) and L < K, the bits of p from L-1 up to K-1 are
+// all ones (p < 2^K) or all zeros below a top bit K (p >= 2^K), which a scan
+// confirms, and e follows from the low L bits of p.
 void pow2_minus(mpz_t e, mpz_srcptr p, size_t K, size_t L) {
-    if (L < K && mpz_sgn(p) > 0) {
+    if (L >= 1 && L < K && mpz_sgn(p) > 0) {
         const size_t n = mpz_sizeinbase(p, 2);
         if (n == K + 1 && mpz_scan1(p, L - 1) == K) {  // p = 2^K + (p mod 2^(L-1))
             mpz_tdiv_r_2exp(e, p, L);
@@ -55,8 +56,6 @@ void pow2_minus(mpz_t e, mpz_srcptr p, size_t K, size_t L) {
     mpz_mul_2exp(e, e, K);
     mpz_sub(e, e, p);
 }
-
-}  // namespace
 
 size_t truncate_top(mpz_t y, mpz_srcptr x, size_t bits) {
     const size_t n = mpz_sgn(x) ? mpz_sizeinbase(x, 2) : 0;

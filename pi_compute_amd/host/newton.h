@@ -27,6 +27,11 @@ void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m);
 // y = base^e exactly; the squarings go through m.
 void pow_ui(mpz_t y, unsigned long base, unsigned long e, Multiplier& m);
 
+// e = 2^K - p (signed), p >= 0.  `L` bounds the expected size, |2^K - p| <
+// 2^(L-1) (a Newton step's product approximates 2^K): then only the low L bits
+// of p are read and written; any p gives the exact difference.
+void pow2_minus(mpz_t e, mpz_srcptr p, size_t K, size_t L);
+
 // y = |x| >> s with s chosen so that y keeps the top `bits` bits of |x|
 // (s = 0 when |x| is shorter); returns s.  y may alias x.
 size_t truncate_top(mpz_t y, mpz_srcptr x, size_t bits);

@@ -224,6 +224,51 @@ int main(int argc, char** argv) {
             ok = ok && mpz_cmp(y, want) == 0;
         }
         check(ok, "PowerBySquaring");
+        // 2^K - p near 2^K on both sides (read from the low L bits) and far
+        // from it (the full difference), at and around the bound 2^(L-1).
+        ok = true;
+        mpz_t p, e, delta;
+        mpz_inits(p, e, delta, NULL);
+        for (size_t K : {64ul, 100ul, 1000ul, 5001ul})
+            for (size_t L : {0ul, 1ul, 2ul, 17ul, K / 2, K - 1, K, K + 5}) {
+                std::vector<long> shifts = {-1, 0, 1};
+                for (long sh : shifts)
+                    for (int off : {-1, 0, 1})
+                        for (int side : {-1, 1}) {
+                            // delta = 2^(L - 1 + sh) + off (clamped at 0)
+                            mpz_set_ui(delta, 0);
+                            const long ex = (long)L - 1 + sh;
+                            if (ex >= 0) mpz_setbit(delta, (mp_bitcnt_t)ex);
+                            if (off < 0 && mpz_sgn(delta) > 0) mpz_sub_ui(delta, delta, 1);
+                            if (off > 0) mpz_add_ui(delta, delta, 1);
+                            mpz_set_ui(p, 0);
+                            mpz_setbit(p, K);
+                            if (side < 0) {
+                                if (mpz_cmp(delta, p) > 0) continue;
+                                mpz_sub(p, p, delta);
+                            } else {
+                                mpz_add(p, p, delta);
+                            }
+                            pi::pow2_minus(e, p, K, L);
+                            mpz_set_ui(want, 0);
+                            mpz_setbit(want, K);
+                            mpz_sub(want, want, p);
+                            ok = ok && mpz_cmp(e, want) == 0;
+                        }
+                for (size_t nb : {1ul, K / 3, K, K + 1, 2 * K}) {
+                    pi::draw_operand(p, g, nb);
+                    pi::pow2_minus(e, p, K, L);
+                    mpz_set_ui(want, 0);
+                    mpz_setbit(want, K);
+                    mpz_sub(want, want, p);
+                    ok = ok && mpz_cmp(e, want) == 0;
+                }
+                mpz_set_ui(p, 0);
+                pi::pow2_minus(e, p, K, L);
+                ok = ok && mpz_sizeinbase(e, 2) == K + 1 && mpz_scan1(e, 0) == K;
+            }
+        mpz_clears(p, e, delta, NULL);
+        check(ok, "PowerOfTwoMinusProduct");
         mpz_clears(d, y, want, t, NULL);
     }
 
