@@ -113,6 +113,15 @@ int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs);  /* 
 int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
                              const size_t* yn, const int* subtract, uint64_t* const* d_out);
 
+/* Same, but a subtraction with x < y is no error: negative[i] is set to 1
+ * and out[i] holds x - y + 2^(64 w), w = max(xn[i], yn[i]); 0 otherwise.
+ * Lets a caller try "x - y if not negative" on a whole batch (the remainder
+ * corrections of a device-resident decimal conversion).  No reference
+ * counterpart. */
+int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, const size_t* xn,
+                                   const uint64_t* const* d_y, const size_t* yn, const int* subtract,
+                                   uint64_t* const* d_out, int* negative);
+
 /* Wait for all enqueued products (PINTT_ERR_HIP if the stream reports an
  * error). */
 int gpu_ntt_synchronize(void);

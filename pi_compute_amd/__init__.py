@@ -99,6 +99,12 @@ ABI = {
         [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p)],
     ),
+    "gpu_ntt_add_batch_device_flags": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
+         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p),
+         ctypes.POINTER(ctypes.c_int)],
+    ),
     # include/pintt_mlp.h
     "gpu_mlp_param_count": (ctypes.c_size_t, [ctypes.c_int]),
     "gpu_mlp_forward": (
@@ -369,9 +375,11 @@ def download(pairs) -> None:
     _check(lib, lib.gpu_ntt_download(n, dst, src, (ctypes.c_size_t * n)(*[x.size for x, _ in pairs])))
 
 
-def add_batch(pairs, subtract) -> list:
+def add_batch(pairs, subtract, flags=False):
     """Device add/subtract of magnitudes: [(x, y)] numpy uint64 limb arrays,
-    subtract[i] -> x - y (needs x >= y).  Uploads, runs one batch, downloads."""
+    subtract[i] -> x - y (needs x >= y).  Uploads, runs one batch, downloads.
+    flags=True: x < y is allowed (the result wraps modulo 2^(64 w)) and the
+    call returns (outputs, negative flags)."""
     lib = load_library()
     n = len(pairs)
     xs = [np.ascontiguousarray(x, np.uint64) for x, _ in pairs]
@@ -391,11 +399,17 @@ def add_batch(pairs, subtract) -> list:
         sz = lambda v: (ctypes.c_size_t * len(v))(*v)
         _check(lib, lib.gpu_ntt_upload(n, arr(dx), arr([_ptr(x) for x in xs]), sz([x.size for x in xs])))
         _check(lib, lib.gpu_ntt_upload(n, arr(dy), arr([_ptr(y) for y in ys]), sz([y.size for y in ys])))
-        _check(lib, lib.gpu_ntt_add_batch_device(n, arr(dx), sz([x.size for x in xs]), arr(dy), sz([y.size for y in ys]),
-                                                 (ctypes.c_int * n)(*[1 if s else 0 for s in subtract]), arr(do)))
+        subs = (ctypes.c_int * n)(*[1 if s else 0 for s in subtract])
+        neg = (ctypes.c_int * n)()
+        if flags:
+            _check(lib, lib.gpu_ntt_add_batch_device_flags(n, arr(dx), sz([x.size for x in xs]), arr(dy),
+                                                           sz([y.size for y in ys]), subs, arr(do), neg))
+        else:
+            _check(lib, lib.gpu_ntt_add_batch_device(n, arr(dx), sz([x.size for x in xs]), arr(dy),
+                                                     sz([y.size for y in ys]), subs, arr(do)))
         outs = [np.zeros(w, np.uint64) for w in widths]
         _check(lib, lib.gpu_ntt_download(n, arr([_ptr(o) for o in outs]), arr(do), sz(widths)))
-        return outs
+        return (outs, [bool(v) for v in neg]) if flags else outs
     finally:
         for p in dev:
             lib.gpu_ntt_dev_free(p)

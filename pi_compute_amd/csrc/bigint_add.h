@@ -25,8 +25,9 @@ struct AddJob {
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 
 // jobs: device array sorted by blk0; maps, carry_in: nblocks bytes each;
-// error: set to nonzero when a subtraction's x < y.
+// error: set to nonzero when a subtraction's x < y; status[i] (njobs words,
+// all written): 1 when job i is such a subtraction, else 0.
 hipError_t launch_add_batch(const AddJob* jobs, uint32_t njobs, uint32_t nblocks, uint8_t* maps, uint8_t* carry_in,
-                            uint32_t* error, hipStream_t s);
+                            uint32_t* error, uint32_t* status, hipStream_t s);
 
 }  // namespace pintt

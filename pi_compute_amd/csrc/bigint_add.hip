@@ -2,8 +2,9 @@
 // (gfx950): the signed add R = Qr*Rl + Pl*Rr of binary-splitting merges, kept
 // on the device between tree levels.
 //
-// With w = max(xn, yn): out = |x| + |y| (w + 1 limbs) or |x| - |y| (w limbs,
-// needs |x| >= |y|; lengths may include zero top limbs).  Subtraction is x + ~y + 1 over the width.  A run of limbs maps
+// With w = max(xn, yn): out = |x| + |y| (w + 1 limbs) or |x| - |y| (w limbs;
+// when |x| < |y| it is |x| - |y| + 2^(64 w) and the job is flagged negative;
+// lengths may include zero top limbs).  Subtraction is x + ~y + 1 over the width.  A run of limbs maps
 // an incoming carry c in {0, 1} to its carry-out: two bits (f(0), f(1)).
 // Three launches per batch:
 //   k_add_maps   each block of 4096 limbs: per-thread maps, block composition
@@ -104,7 +105,8 @@ __global__ void __launch_bounds__(kAT) k_add_maps(const AddJob* __restrict__ job
 
 // One workgroup per job: carry-in of each of its blocks, carry-out of the job.
 __global__ void __launch_bounds__(kAT) k_add_scan(const AddJob* __restrict__ jobs, const uint8_t* __restrict__ maps,
-                                                  uint8_t* __restrict__ carry_in, uint32_t* __restrict__ error) {
+                                                  uint8_t* __restrict__ carry_in, uint32_t* __restrict__ error,
+                                                  uint32_t* __restrict__ status) {
     __shared__ uint32_t lds[kAT / 64];
     const AddJob j = jobs[blockIdx.x];
     uint32_t c = j.sub ? 1u : 0u;
@@ -117,11 +119,10 @@ __global__ void __launch_bounds__(kAT) k_add_scan(const AddJob* __restrict__ job
         c = (total >> c) & 1u;
     }
     if (threadIdx.x == 0) {
-        if (j.sub) {
-            if (c != 1u) atomicOr(error, 1u);  // |x| < |y|
-        } else {
-            as_global(j.out)[width_of(j)] = c;
-        }
+        const uint32_t negative = (j.sub && c != 1u) ? 1u : 0u;  // |x| < |y|
+        if (negative) atomicOr(error, 1u);
+        if (!j.sub) as_global(j.out)[width_of(j)] = c;
+        status[blockIdx.x] = negative;
     }
 }
 
@@ -162,11 +163,11 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
 }
 
 hipError_t launch_add_batch(const AddJob* jobs, uint32_t njobs, uint32_t nblocks, uint8_t* maps, uint8_t* carry_in,
-                            uint32_t* error, hipStream_t s) {
+                            uint32_t* error, uint32_t* status, hipStream_t s) {
     hipLaunchKernelGGL(k_add_maps, dim3(nblocks), dim3(kAT), 0, s, jobs, njobs, maps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_add_scan, dim3(njobs), dim3(kAT), 0, s, jobs, maps, carry_in, error);
+    hipLaunchKernelGGL(k_add_scan, dim3(njobs), dim3(kAT), 0, s, jobs, maps, carry_in, error, status);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_add_apply, dim3(nblocks), dim3(kAT), 0, s, jobs, njobs, carry_in);

@@ -72,6 +72,7 @@ struct Engine {
     size_t add_job_cap = 0;
     AddJob* add_jobs_host = nullptr;
     AddJob* add_jobs_dev = nullptr;
+    uint32_t* add_status = nullptr;  // per job: 1 when a subtraction went negative
     size_t add_blk_cap = 0;
     uint8_t* add_maps = nullptr;
     uint8_t* add_carry = nullptr;
@@ -979,6 +980,7 @@ void gpu_ntt_shutdown(void) {
     }
     if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
     (void)hipFree(g.add_jobs_dev);
+    (void)hipFree(g.add_status);
     (void)hipFree(g.add_maps);
     (void)hipFree(g.add_carry);
     (void)hipFree(g.add_error);
@@ -1185,8 +1187,11 @@ int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs) {
     return PINTT_OK;
 }
 
-int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
-                             const size_t* yn, const int* subtract, uint64_t* const* d_out) {
+namespace {
+// negative == nullptr: a negative subtraction is an error; else negative[i]
+// receives its flag.
+int add_batch(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
+              const size_t* yn, const int* subtract, uint64_t* const* d_out, int* negative) {
     if (count == 0) return PINTT_OK;
     if (!d_x || !xn || !d_y || !yn || !subtract || !d_out) return fail(PINTT_ERR_ARGS, "null pointer");
     std::vector<AddJob> jobs(count);
@@ -1208,12 +1213,15 @@ int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const siz
     if (count > g.add_job_cap) {
         if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
         (void)hipFree(g.add_jobs_dev);
+        (void)hipFree(g.add_status);
         g.add_jobs_host = nullptr;
         g.add_jobs_dev = nullptr;
+        g.add_status = nullptr;
         g.add_job_cap = 0;
         const size_t cap = std::max<size_t>(count, 1024);
         if (hipHostMalloc(reinterpret_cast<void**>(&g.add_jobs_host), cap * sizeof(AddJob)) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void**>(&g.add_jobs_dev), cap * sizeof(AddJob)) != hipSuccess)
+            hipMalloc(reinterpret_cast<void**>(&g.add_jobs_dev), cap * sizeof(AddJob)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&g.add_status), cap * sizeof(uint32_t)) != hipSuccess)
             return fail(PINTT_ERR_NO_MEMORY, "add descriptors");
         g.add_job_cap = cap;
     }
@@ -1235,12 +1243,33 @@ int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const siz
     // A fill kernel, not hipMemsetAsync: no memset node if a caller ever captures this.
     PINTT_HIP(launch_fill_u32(g.add_error, 0u, 1, g.stream));
     PINTT_HIP(launch_add_batch(g.add_jobs_dev, (uint32_t)count, (uint32_t)blocks, g.add_maps, g.add_carry, g.add_error,
-                               g.stream));
+                               g.add_status, g.stream));
     uint32_t err = 0;
     PINTT_HIP(hipMemcpyAsync(&err, g.add_error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
+    std::vector<uint32_t> status(negative ? count : 0);
+    if (negative)
+        PINTT_HIP(hipMemcpyAsync(status.data(), g.add_status, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 g.stream));
     PINTT_HIP(hipStreamSynchronize(g.stream));
-    if (err) return fail(PINTT_ERR_ARGS, "subtraction with |x| < |y|");
+    if (negative) {
+        for (size_t i = 0; i < count; ++i) negative[i] = status[i] ? 1 : 0;
+    } else if (err) {
+        return fail(PINTT_ERR_ARGS, "subtraction with |x| < |y|");
+    }
     return PINTT_OK;
+}
+}  // namespace
+
+int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
+                             const size_t* yn, const int* subtract, uint64_t* const* d_out) {
+    return add_batch(count, d_x, xn, d_y, yn, subtract, d_out, nullptr);
+}
+
+int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, const size_t* xn,
+                                   const uint64_t* const* d_y, const size_t* yn, const int* subtract,
+                                   uint64_t* const* d_out, int* negative) {
+    if (count && !negative) return fail(PINTT_ERR_ARGS, "null pointer");
+    return add_batch(count, d_x, xn, d_y, yn, subtract, d_out, negative);
 }
 
 size_t gpu_ntt_graph_nodes(void* graph) { return graph ? static_cast<BatchGraph*>(graph)->nodes : 0; }

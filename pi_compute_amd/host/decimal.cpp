@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "newton.h"
+#include "pintt_gpu.h"
 
 namespace pi {
 namespace {
@@ -126,16 +127,389 @@ struct Node {
 };
 
 // One split by P = 10^(2^j) of bit length np: the quotient estimate is
-// ((x >> sx) * inv) >> (e2 - sx) with inv ~ 2^e2 / P.  Both shifts are whole
-// limbs, so the shifted operands are views of the numbers they come from.
-// With x < P^2, |inv - 2^e2/P| <= 4 and 2^sx < P / 2^63 the estimate is
-// within 6 of floor(x / P).
+// ((x >> sx) * inv) >> (e2 - sx) with 2^e2/P - 9 <= inv <= 2^e2/P - 1 (the
+// Newton reciprocal, within 4, minus 5).  Both shifts are whole limbs, so the
+// shifted operands are views of the numbers they come from.  With x < P^2,
+// 2^sx <= P / 2^63 and 2^e2 >= 2^64 P^2 the estimate is floor(x / P) or one
+// less: it never exceeds x / P, and it falls short by at most 2^-63 +
+// 9 x / 2^e2 + 1 < 2.
 struct SplitShifts {
     size_t sx, e2;
 };
 SplitShifts split_shifts(size_t np) {
     const size_t L = GMP_NUMB_BITS;
-    return SplitShifts{(np > 64 ? np - 64 : 0) / L * L, (2 * np + L - 1) / L * L};
+    return SplitShifts{(np > 64 ? np - 64 : 0) / L * L, (2 * np + 64 + L - 1) / L * L};
+}
+
+// The conversion's levels on the host: both product batches of a level
+// through m, corrections on host threads.
+void split_on_host(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t leaf_digits, Multiplier& m,
+                   unsigned threads, std::vector<std::unique_ptr<Node>>& nodes, DecimalStats& st) {
+    using Clock = std::chrono::steady_clock;
+    auto t = Clock::now();
+    auto lap = [](Clock::time_point& t) {
+        const auto now = Clock::now();
+        const double s = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return s;
+    };
+    const int k = (int)pre.pw.size() - 1;
+    nodes.clear();
+    nodes.emplace_back(new Node);
+    nodes[0]->limbs = mpz_limbs_read(n);  // |n|
+    nodes[0]->size = mpz_size(n);
+    nodes[0]->offset = 0;
+    nodes[0]->width = width;
+
+    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
+        const size_t low = (size_t)1 << j;
+        std::vector<size_t> split;
+        for (size_t i = 0; i < nodes.size(); ++i)
+            if (nodes[i]->width > low) split.push_back(i);
+        if (split.empty()) continue;
+        mpz_srcptr P = &pre.pw[j];
+        mpz_srcptr inv = &pre.inv[j];
+        const SplitShifts sh = split_shifts(mpz_sizeinbase(P, 2));
+        const size_t qskip = (sh.e2 - sh.sx) / GMP_NUMB_BITS;
+        lap(t);
+        const size_t ns = split.size();
+        std::vector<__mpz_struct> hi(ns), qv(ns), qfull(ns), qp(ns);
+        std::vector<MulJob> jobs(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            const Node& nd = *nodes[split[s]];
+            const size_t skip = std::min(sh.sx / GMP_NUMB_BITS, nd.size);
+            mpz_roinit_n(&hi[s], nd.limbs + skip, (mp_size_t)(nd.size - skip));  // x >> sx
+            mpz_init(&qfull[s]);
+            mpz_init(&qp[s]);
+            jobs[s] = MulJob{&qfull[s], &hi[s], inv};
+        }
+        m.multiply_batch(jobs.data(), ns, threads);
+        for (size_t s = 0; s < ns; ++s) {
+            const size_t n = mpz_size(&qfull[s]);
+            const size_t skip = std::min(qskip, n);
+            mpz_roinit_n(&qv[s], mpz_limbs_read(&qfull[s]) + skip, (mp_size_t)(n - skip));  // the estimate
+            jobs[s] = MulJob{&qp[s], &qv[s], P};
+        }
+        m.multiply_batch(jobs.data(), ns, threads);
+        st.products_s += lap(t);
+        // Corrections, then the node becomes its high part and a low node joins.
+        std::vector<std::unique_ptr<Node>> lows(ns);
+        std::vector<long> deltas(ns, 0);
+        struct Ctx {
+            std::vector<std::unique_ptr<Node>>* nodes;
+            const std::vector<size_t>* split;
+            std::vector<__mpz_struct>* qv;
+            std::vector<__mpz_struct>* qfull;
+            std::vector<__mpz_struct>* qp;
+            std::vector<std::unique_ptr<Node>>* lows;
+            std::vector<long>* delta;
+            mpz_srcptr P;
+            size_t low, qskip;
+        } ctx{&nodes, &split, &qv, &qfull, &qp, &lows, &deltas, P, low, qskip};
+        parallel_for(
+            ns, threads,
+            [](void* p, size_t s) {
+                auto* c = static_cast<Ctx*>(p);
+                Node& nd = *(*c->nodes)[(*c->split)[s]];
+                auto* lo = new Node;
+                mpz_t r;
+                mpz_init(r);
+                mpz_sub(r, nd.x(), &(*c->qp)[s]);  // remainder of the estimate
+                long delta = 0;                    // quotient = estimate + delta
+                while (mpz_sgn(r) < 0) {
+                    mpz_add(r, r, c->P);
+                    --delta;
+                }
+                while (mpz_cmp(r, c->P) >= 0) {
+                    mpz_sub(r, r, c->P);
+                    ++delta;
+                }
+                (*c->delta)[s] = delta;
+                lo->take(r);
+                mpz_clear(r);
+                lo->width = c->low;
+                lo->offset = nd.offset + nd.width - c->low;
+                nd.width -= c->low;
+                if (delta == 0) {
+                    nd.take_top(&(*c->qfull)[s], c->qskip);
+                } else {
+                    mpz_t q;
+                    mpz_init(q);
+                    if (delta > 0)
+                        mpz_add_ui(q, &(*c->qv)[s], (unsigned long)delta);
+                    else
+                        mpz_sub_ui(q, &(*c->qv)[s], (unsigned long)-delta);
+                    nd.take(q);
+                    mpz_clear(q);
+                }
+                (*c->lows)[s].reset(lo);
+            },
+            &ctx);
+        for (auto& lo : lows) nodes.push_back(std::move(lo));
+        for (long d : deltas) {
+            st.min_correction = std::min(st.min_correction, d);
+            st.max_correction = std::max(st.max_correction, d);
+        }
+        st.corrections_s += lap(t);
+        for (size_t s = 0; s < ns; ++s) {
+            mpz_clear(&qfull[s]);
+            mpz_clear(&qp[s]);
+        }
+    }
+
+}
+
+void dcheck(int rc, const char* what) {
+    if (rc != PINTT_OK) throw std::runtime_error(std::string(what) + ": " + gpu_ntt_last_error());
+}
+
+// Device allocations of one conversion, freed together at the end.
+class DevBuffers {
+public:
+    DevBuffers() = default;
+    ~DevBuffers() {
+        for (uint64_t* p : blocks_) (void)gpu_ntt_dev_free(p);
+    }
+    DevBuffers(const DevBuffers&) = delete;
+    DevBuffers& operator=(const DevBuffers&) = delete;
+    uint64_t* alloc(size_t limbs) {
+        uint64_t* p = nullptr;
+        dcheck(gpu_ntt_dev_alloc(std::max<size_t>(limbs, 1), &p), "gpu_ntt_dev_alloc");
+        blocks_.push_back(p);
+        return p;
+    }
+
+private:
+    std::vector<uint64_t*> blocks_;
+};
+
+// A device number: n >= 1 limbs, the top ones possibly zero.
+struct DevNum {
+    uint64_t* d = nullptr;
+    size_t n = 0;
+};
+
+// Limbs that hold every value below 10^digits.
+size_t limbs_for_digits(size_t digits) { return (size_t)((double)digits * 3.3219280948873623 / 64.0) + 2; }
+
+// Each nonzero src[i] into dst[i] (mpz_size(src[i]) limbs).
+void upload(const std::vector<mpz_srcptr>& src, const std::vector<DevNum>& dst) {
+    std::vector<uint64_t*> d;
+    std::vector<const uint64_t*> h;
+    std::vector<size_t> n;
+    for (size_t i = 0; i < src.size(); ++i) {
+        const size_t sz = mpz_size(src[i]);
+        if (!sz) throw std::runtime_error("device conversion: zero operand");
+        d.push_back(dst[i].d);
+        h.push_back(reinterpret_cast<const uint64_t*>(mpz_limbs_read(src[i])));
+        n.push_back(sz);
+    }
+    if (!d.empty()) dcheck(gpu_ntt_upload(d.size(), d.data(), h.data(), n.data()), "gpu_ntt_upload");
+}
+
+struct DevSplitNode {
+    DevNum x;
+    size_t offset, width;
+};
+
+// The conversion's levels on device-resident numbers: each level is two
+// product batches (estimate, estimate * P), a flagged subtraction for the
+// remainder x - q*P (never negative: the estimate never exceeds x / P), a
+// flagged subtraction of P (the estimate is floor(x / P) or one less) and one
+// that confirms the remainder is below P, and an increment of the quotients
+// that were one short.  Returns the leaves on the host.
+void device_split(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t leaf_digits,
+                  std::vector<std::unique_ptr<Node>>& out, DecimalStats& st) {
+    using Clock = std::chrono::steady_clock;
+    auto t = Clock::now();
+    auto lap = [&t]() {
+        const auto now = Clock::now();
+        const double s = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return s;
+    };
+    static_assert(sizeof(mp_limb_t) == sizeof(uint64_t), "64-bit limbs");
+    DevBuffers mem;
+    const int k = (int)pre.pw.size() - 1;
+    int jmin = k + 1;
+    while (jmin > 0 && ((size_t)1 << (jmin - 1)) >= leaf_digits) --jmin;
+    // Powers, reciprocals, the constant 1 (an all-zero limb follows it) and |n|.
+    size_t total = 2;
+    for (int j = jmin; j <= k; ++j) total += mpz_size(&pre.pw[j]) + mpz_size(&pre.inv[j]);
+    const size_t nn = std::max<size_t>(mpz_size(n), 1);
+    total += nn;
+    uint64_t* base = mem.alloc(total);
+    std::vector<DevNum> P(k + 1), inv(k + 1);
+    std::vector<mpz_srcptr> src;
+    std::vector<DevNum> dst;
+    size_t used = 0;
+    auto carve = [&](size_t limbs) {
+        DevNum v{base + used, std::max<size_t>(limbs, 1)};
+        used += v.n;
+        return v;
+    };
+    for (int j = jmin; j <= k; ++j) {
+        P[j] = carve(mpz_size(&pre.pw[j]));
+        inv[j] = carve(mpz_size(&pre.inv[j]));
+        src.push_back(&pre.pw[j]);
+        dst.push_back(P[j]);
+        src.push_back(&pre.inv[j]);
+        dst.push_back(inv[j]);
+    }
+    DevNum one = carve(2);
+    DevNum root = carve(nn);
+    {
+        const uint64_t o[2] = {1, 0};
+        const uint64_t* op = o;
+        size_t on = 2;
+        dcheck(gpu_ntt_upload(1, &one.d, &op, &on), "gpu_ntt_upload");
+    }
+    src.push_back(n);
+    dst.push_back(root);
+    upload(src, dst);
+    DevNum zero{one.d + 1, 1};
+
+    std::vector<DevSplitNode> nodes{{root, 0, width}};
+    for (int j = k; j >= jmin; --j) {
+        const size_t low = (size_t)1 << j;
+        std::vector<size_t> split;
+        for (size_t i = 0; i < nodes.size(); ++i)
+            if (nodes[i].width > low) split.push_back(i);
+        if (split.empty()) continue;
+        const size_t ns = split.size();
+        const SplitShifts sh = split_shifts(mpz_sizeinbase(&pre.pw[j], 2));
+        const size_t skip = sh.sx / GMP_NUMB_BITS, qskip = (sh.e2 - sh.sx) / GMP_NUMB_BITS;
+        const DevNum Pj = P[j], Ij = inv[j];
+        // Operands and room for the level: estimates, products, remainders (two
+        // candidates each) and incremented quotients.
+        std::vector<DevNum> hi(ns), qv(ns);
+        std::vector<size_t> qf_n(ns), qp_n(ns), r_n(ns), qq_n(ns);
+        size_t room = 0;
+        for (size_t s = 0; s < ns; ++s) {
+            const DevNum x = nodes[split[s]].x;
+            hi[s] = x.n > skip ? DevNum{x.d + skip, x.n - skip} : zero;
+            qf_n[s] = hi[s].n + Ij.n;
+            const size_t qvn = qf_n[s] > qskip ? qf_n[s] - qskip : 1;
+            qp_n[s] = qvn + Pj.n;
+            r_n[s] = std::max(x.n, qp_n[s]);
+            qq_n[s] = qvn + 1;
+            room += qf_n[s] + qp_n[s] + 2 * r_n[s] + qq_n[s];
+        }
+        uint64_t* lv = mem.alloc(room);
+        size_t at = 0;
+        auto take = [&](size_t limbs) {
+            uint64_t* p = lv + at;
+            at += limbs;
+            return p;
+        };
+        std::vector<DevNum> qf(ns), qp(ns), r(ns), r1(ns), qq(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            qf[s] = {take(qf_n[s]), qf_n[s]};
+            qp[s] = {take(qp_n[s]), qp_n[s]};
+            r[s] = {take(r_n[s]), r_n[s]};
+            r1[s] = {take(r_n[s]), r_n[s]};
+            qq[s] = {take(qq_n[s]), qq_n[s]};
+            qv[s] = qf[s].n > qskip ? DevNum{qf[s].d + qskip, qf[s].n - qskip} : zero;
+        }
+        // q = (hi * inv) >> (e2 - sx), then q * P.
+        std::vector<const uint64_t*> A(ns), B(ns), X(ns), Y(ns);
+        std::vector<uint64_t*> O(ns);
+        std::vector<size_t> AN(ns), BN(ns), XN(ns), YN(ns);
+        std::vector<int> SUB(ns, 1), NEG(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            A[s] = hi[s].d, AN[s] = hi[s].n, B[s] = Ij.d, BN[s] = Ij.n, O[s] = qf[s].d;
+        }
+        dcheck(gpu_ntt_mul_batch_device(ns, A.data(), AN.data(), B.data(), BN.data(), O.data(), nullptr),
+               "gpu_ntt_mul_batch_device");
+        for (size_t s = 0; s < ns; ++s) {
+            A[s] = qv[s].d, AN[s] = qv[s].n, B[s] = Pj.d, BN[s] = Pj.n, O[s] = qp[s].d;
+        }
+        dcheck(gpu_ntt_mul_batch_device(ns, A.data(), AN.data(), B.data(), BN.data(), O.data(), nullptr),
+               "gpu_ntt_mul_batch_device");
+        // r = x - q P >= 0.
+        for (size_t s = 0; s < ns; ++s) {
+            const DevNum x = nodes[split[s]].x;
+            X[s] = x.d, XN[s] = x.n, Y[s] = qp[s].d, YN[s] = qp[s].n, O[s] = r[s].d;
+        }
+        dcheck(gpu_ntt_add_batch_device_flags(ns, X.data(), XN.data(), Y.data(), YN.data(), SUB.data(), O.data(),
+                                              NEG.data()),
+               "gpu_ntt_add_batch_device_flags");
+        st.products_s += lap();
+        for (size_t s = 0; s < ns; ++s)
+            if (NEG[s]) throw std::runtime_error("device conversion: quotient estimate above x / P");
+        // r1 = r - P: where not negative the quotient was one short.
+        for (size_t s = 0; s < ns; ++s) {
+            X[s] = r[s].d, XN[s] = r[s].n, Y[s] = Pj.d, YN[s] = Pj.n, O[s] = r1[s].d;
+        }
+        dcheck(gpu_ntt_add_batch_device_flags(ns, X.data(), XN.data(), Y.data(), YN.data(), SUB.data(), O.data(),
+                                              NEG.data()),
+               "gpu_ntt_add_batch_device_flags");
+        std::vector<size_t> short_by_one;
+        for (size_t s = 0; s < ns; ++s)
+            if (!NEG[s]) short_by_one.push_back(s);
+        if (!short_by_one.empty()) st.max_correction = 1;
+        if (!short_by_one.empty()) {
+            // Those remainders must now be below P (r1 - P negative; the
+            // difference lands in r, no longer needed), and q + 1 replaces q.
+            const size_t m = short_by_one.size();
+            std::vector<const uint64_t*> X2(m), Y2(m);
+            std::vector<uint64_t*> O2(m);
+            std::vector<size_t> XN2(m), YN2(m);
+            std::vector<int> SUB2(m, 1), NEG2(m);
+            for (size_t i = 0; i < m; ++i) {
+                const size_t s = short_by_one[i];
+                X2[i] = r1[s].d, XN2[i] = r1[s].n, Y2[i] = Pj.d, YN2[i] = Pj.n, O2[i] = r[s].d;
+            }
+            dcheck(gpu_ntt_add_batch_device_flags(m, X2.data(), XN2.data(), Y2.data(), YN2.data(), SUB2.data(),
+                                                  O2.data(), NEG2.data()),
+                   "gpu_ntt_add_batch_device_flags");
+            for (size_t i = 0; i < m; ++i)
+                if (!NEG2[i]) throw std::runtime_error("device conversion: quotient estimate two or more short");
+            std::vector<int> ADD(m, 0);
+            for (size_t i = 0; i < m; ++i) {
+                const size_t s = short_by_one[i];
+                X2[i] = qv[s].d, XN2[i] = qv[s].n, Y2[i] = one.d, YN2[i] = 1, O2[i] = qq[s].d;
+            }
+            dcheck(gpu_ntt_add_batch_device(m, X2.data(), XN2.data(), Y2.data(), YN2.data(), ADD.data(), O2.data()),
+                   "gpu_ntt_add_batch_device");
+        }
+        // The node keeps the quotient (width - low digits), a new node takes the
+        // remainder (below P: its limbs).
+        std::vector<bool> incremented(ns, false);
+        for (size_t s : short_by_one) incremented[s] = true;
+        for (size_t s = 0; s < ns; ++s) {
+            DevSplitNode& nd = nodes[split[s]];
+            DevNum rem = incremented[s] ? r1[s] : r[s];
+            rem.n = std::min(rem.n, Pj.n);
+            DevNum q = incremented[s] ? qq[s] : qv[s];
+            nd.width -= low;
+            q.n = std::min(q.n, limbs_for_digits(nd.width));
+            nd.x = q;
+            nodes.push_back(DevSplitNode{rem, nd.offset + nd.width, low});
+        }
+        st.corrections_s += lap();
+    }
+    // Leaves back to the host.
+    std::vector<uint64_t*> H(nodes.size());
+    std::vector<const uint64_t*> D(nodes.size());
+    std::vector<size_t> N(nodes.size());
+    out.clear();
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        auto* nd = new Node;
+        out.emplace_back(nd);
+        H[i] = reinterpret_cast<uint64_t*>(mpz_limbs_write(nd->own, (mp_size_t)nodes[i].x.n));
+        D[i] = nodes[i].x.d;
+        N[i] = nodes[i].x.n;
+        nd->offset = nodes[i].offset;
+        nd->width = nodes[i].width;
+    }
+    dcheck(gpu_ntt_download(nodes.size(), H.data(), D.data(), N.data()), "gpu_ntt_download");
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        mpz_limbs_finish(out[i]->own, (mp_size_t)N[i]);
+        out[i]->limbs = mpz_limbs_read(out[i]->own);
+        out[i]->size = mpz_size(out[i]->own);
+    }
+    st.leaves_s += lap();
 }
 
 }  // namespace
@@ -169,12 +543,13 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
     for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
         const size_t np = mpz_sizeinbase(&d.pw[j], 2);
         newton_reciprocal(&d.inv[j], &d.pw[j], split_shifts(np).e2 - np, m);  // ~ 2^e2 / P, within 4
+        mpz_sub_ui(&d.inv[j], &d.inv[j], 5);                                    // below 2^e2 / P
     }
     d.reciprocals_s = std::chrono::duration<double>(Clock::now() - t).count();
 }
 
 std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t leaf_digits, DecimalStats* stats,
-                       const DecimalPowers* pre) {
+                       const DecimalPowers* pre, bool device) {
     using Clock = std::chrono::steady_clock;
     DecimalStats st;
     auto lap = [](Clock::time_point& t) {
@@ -199,104 +574,16 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
         decimal_powers(own, width, m, leaf_digits);
         pre = &own;
     }
-    const int k = (int)pre->pw.size() - 1;
     st.powers_s = pre->powers_s;
     st.reciprocals_s = pre->reciprocals_s;
     lap(t);
     std::vector<std::unique_ptr<Node>> nodes;
-    nodes.emplace_back(new Node);
-    nodes[0]->limbs = mpz_limbs_read(n);  // |n|
-    nodes[0]->size = mpz_size(n);
-    nodes[0]->offset = 0;
-    nodes[0]->width = width;
-
-    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
-        const size_t low = (size_t)1 << j;
-        std::vector<size_t> split;
-        for (size_t i = 0; i < nodes.size(); ++i)
-            if (nodes[i]->width > low) split.push_back(i);
-        if (split.empty()) continue;
-        mpz_srcptr P = &pre->pw[j];
-        mpz_srcptr inv = &pre->inv[j];
-        const SplitShifts sh = split_shifts(mpz_sizeinbase(P, 2));
-        const size_t qskip = (sh.e2 - sh.sx) / GMP_NUMB_BITS;
+    if (device) {
+        device_split(n, width, *pre, leaf_digits, nodes, st);
         lap(t);
-        const size_t ns = split.size();
-        std::vector<__mpz_struct> hi(ns), qv(ns), qfull(ns), qp(ns);
-        std::vector<MulJob> jobs(ns);
-        for (size_t s = 0; s < ns; ++s) {
-            const Node& nd = *nodes[split[s]];
-            const size_t skip = std::min(sh.sx / GMP_NUMB_BITS, nd.size);
-            mpz_roinit_n(&hi[s], nd.limbs + skip, (mp_size_t)(nd.size - skip));  // x >> sx
-            mpz_init(&qfull[s]);
-            mpz_init(&qp[s]);
-            jobs[s] = MulJob{&qfull[s], &hi[s], inv};
-        }
-        m.multiply_batch(jobs.data(), ns, threads);
-        for (size_t s = 0; s < ns; ++s) {
-            const size_t n = mpz_size(&qfull[s]);
-            const size_t skip = std::min(qskip, n);
-            mpz_roinit_n(&qv[s], mpz_limbs_read(&qfull[s]) + skip, (mp_size_t)(n - skip));  // the estimate
-            jobs[s] = MulJob{&qp[s], &qv[s], P};
-        }
-        m.multiply_batch(jobs.data(), ns, threads);
-        st.products_s += lap(t);
-        // Corrections, then the node becomes its high part and a low node joins.
-        std::vector<std::unique_ptr<Node>> lows(ns);
-        struct Ctx {
-            std::vector<std::unique_ptr<Node>>* nodes;
-            const std::vector<size_t>* split;
-            std::vector<__mpz_struct>* qv;
-            std::vector<__mpz_struct>* qfull;
-            std::vector<__mpz_struct>* qp;
-            std::vector<std::unique_ptr<Node>>* lows;
-            mpz_srcptr P;
-            size_t low, qskip;
-        } ctx{&nodes, &split, &qv, &qfull, &qp, &lows, P, low, qskip};
-        parallel_for(
-            ns, threads,
-            [](void* p, size_t s) {
-                auto* c = static_cast<Ctx*>(p);
-                Node& nd = *(*c->nodes)[(*c->split)[s]];
-                auto* lo = new Node;
-                mpz_t r;
-                mpz_init(r);
-                mpz_sub(r, nd.x(), &(*c->qp)[s]);  // remainder of the estimate
-                long delta = 0;                    // quotient = estimate + delta
-                while (mpz_sgn(r) < 0) {
-                    mpz_add(r, r, c->P);
-                    --delta;
-                }
-                while (mpz_cmp(r, c->P) >= 0) {
-                    mpz_sub(r, r, c->P);
-                    ++delta;
-                }
-                lo->take(r);
-                mpz_clear(r);
-                lo->width = c->low;
-                lo->offset = nd.offset + nd.width - c->low;
-                nd.width -= c->low;
-                if (delta == 0) {
-                    nd.take_top(&(*c->qfull)[s], c->qskip);
-                } else {
-                    mpz_t q;
-                    mpz_init(q);
-                    if (delta > 0)
-                        mpz_add_ui(q, &(*c->qv)[s], (unsigned long)delta);
-                    else
-                        mpz_sub_ui(q, &(*c->qv)[s], (unsigned long)-delta);
-                    nd.take(q);
-                    mpz_clear(q);
-                }
-                (*c->lows)[s].reset(lo);
-            },
-            &ctx);
-        for (auto& lo : lows) nodes.push_back(std::move(lo));
-        st.corrections_s += lap(t);
-        for (size_t s = 0; s < ns; ++s) {
-            mpz_clear(&qfull[s]);
-            mpz_clear(&qp[s]);
-        }
+    } else {
+        split_on_host(n, width, *pre, leaf_digits, m, threads, nodes, st);
+        lap(t);
     }
 
     // Leaves: host conversion, one node per task (largest first).

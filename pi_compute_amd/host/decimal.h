@@ -28,6 +28,7 @@ std::string to_decimal(mpz_srcptr n, unsigned threads = 0);
 // conversion above takes over, one subtree per thread.
 struct DecimalStats {
     double powers_s = 0, reciprocals_s = 0, products_s = 0, corrections_s = 0, leaves_s = 0;
+    long min_correction = 0, max_correction = 0;  // quotient = estimate + correction, over all splits
 };
 
 // The powers 10^(2^j) and their Newton reciprocals that the conversion of a
@@ -37,7 +38,8 @@ struct DecimalStats {
 struct DecimalPowers {
     size_t width = 0, leaf_digits = 0;
     std::vector<__mpz_struct> pw;   // 10^(2^j), j <= k with 2^k < width <= 2^(k+1)
-    // ~2^e / pw[j], e = 2 bits(pw[j]) rounded up to whole limbs, where 2^j >= leaf_digits; else 0
+    // 2^e / pw[j] - 9 .. 2^e / pw[j] - 1, e = 2 bits(pw[j]) + 64 rounded up to
+    // whole limbs, where 2^j >= leaf_digits; else 0
     std::vector<__mpz_struct> inv;
     double powers_s = 0, reciprocals_s = 0;
     DecimalPowers() = default;
@@ -48,8 +50,12 @@ struct DecimalPowers {
 void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits = 1 << 15);
 
 // `pre`: powers built for a width with the same k and for the same leaf_digits
-// (others are ignored).
+// (others are ignored).  `device`: the levels run on device-resident numbers
+// through the GPU engine's device entry points (m then only builds missing
+// powers): n, the powers and the reciprocals are uploaded once, each level is
+// two device product batches and flagged device subtractions, and only the
+// leaves come back.  Throws std::runtime_error if the device fails.
 std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads = 0, size_t leaf_digits = 1 << 15,
-                       DecimalStats* stats = nullptr, const DecimalPowers* pre = nullptr);
+                       DecimalStats* stats = nullptr, const DecimalPowers* pre = nullptr, bool device = false);
 
 }  // namespace pi

@@ -144,8 +144,20 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     if (dpow_done.valid()) dpow_done.get();
     const double dpow_wait_s = since(t2);
     DecimalStats ds;
-    std::string d = conv_mult ? to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get())
-                              : to_decimal(pi_int, cfg.num_threads);
+    const bool conv_device = conv_mult && cfg.level_batches && cfg.device_levels && cfg.device_conversion;
+    std::string d;
+    if (conv_device) {
+        try {
+            d = to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get(), true);
+        } catch (const std::exception& e) {
+            std::cerr << "  device conversion failed (" << e.what() << "); host levels instead" << std::endl;
+            ds = DecimalStats{};
+            d = to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get());
+        }
+    } else {
+        d = conv_mult ? to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get())
+                      : to_decimal(pi_int, cfg.num_threads);
+    }
     mpz_clear(pi_int);
     std::string out;
     out.reserve(cfg.digits + 2);
@@ -170,7 +182,8 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             std::cout << std::endl;
         }
         if (conv_mult) {
-            std::cout << "    conversion: powers " << ds.powers_s << " s, reciprocals " << ds.reciprocals_s
+            std::cout << "    conversion" << (conv_device ? " (device-resident levels)" : "") << ": powers "
+                      << ds.powers_s << " s, reciprocals " << ds.reciprocals_s
                       << " s, quotient products " << ds.products_s << " s, corrections " << ds.corrections_s
                       << " s, leaves " << ds.leaves_s << " s";
             if (dpow) std::cout << " (powers and reciprocals beside binary splitting, waited " << dpow_wait_s << " s)";

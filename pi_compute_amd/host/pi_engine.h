@@ -42,6 +42,9 @@ struct PiConfig {
     // ten and their reciprocals (Conversion::Multiplier) depend only on D, so
     // they are computed on another thread while binary splitting runs.
     bool overlap_constants = true;
+    // Conversion::Multiplier with device-resident levels: the conversion's
+    // levels also run on device-resident numbers (decimal.h), leaves on the host.
+    bool device_conversion = true;
 };
 
 struct PiResult {

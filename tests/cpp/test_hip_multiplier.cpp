@@ -5,6 +5,7 @@
 #include <gmp.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -13,7 +14,9 @@
 #include "binary_splitting.h"
 #include "dispatch.h"
 #include "gmp_multiplier.h"
+#include "decimal.h"
 #include "hip_ntt_multiplier.h"
+#include "operands.h"
 
 namespace {
 
@@ -212,6 +215,57 @@ int main() {
                       "DeviceResidentLevels b=" + std::to_string(b) + " from=" + std::to_string(from));
             }
         }
+    }
+
+    // Decimal conversion with device-resident levels (decimal.h) against
+    // mpz_get_str: random sizes, 10^k and its neighbours, and numbers whose
+    // remainders sit at the ends of their range.
+    {
+        bool ok = true, est_ok = true;
+        std::string err;
+        pi::SplitMix64 g(91);
+        mpz_t x, p;
+        mpz_inits(x, p, NULL);
+        auto conv = [&](unsigned threads) {
+            char* s = mpz_get_str(nullptr, 10, x);
+            pi::DecimalStats st;
+            try {
+                ok = ok && pi::to_decimal(x, gpu, threads, 4096, &st, nullptr, true) == s;
+                est_ok = est_ok && st.min_correction == 0 && st.max_correction <= 1;
+            } catch (const std::exception& ex) {
+                ok = false;
+                err = ex.what();
+            }
+            std::free(s);
+        };
+        for (size_t nb : {30000ul, 100000ul, 654321ul, 2000000ul, 20000000ul}) {
+            pi::draw_operand(x, g, nb);
+            conv(4);
+        }
+        for (unsigned long k : {8193ul, 65536ul, 100000ul, 131072ul})
+            for (int d : {-1, 0, 1}) {
+                mpz_ui_pow_ui(x, 10, k);
+                if (d < 0) mpz_sub_ui(x, x, 1);
+                if (d > 0) mpz_add_ui(x, x, 1);
+                conv(3);
+            }
+        mpz_ui_pow_ui(p, 10, 8192);
+        for (unsigned long mlt : {1ul, 7ul, 99999ul})
+            for (int d : {-1, 0, 1}) {
+                mpz_ui_pow_ui(x, 10, 50000);
+                mpz_sub_ui(x, x, 1);
+                mpz_tdiv_q(x, x, p);
+                mpz_mul_ui(x, x, mlt);
+                mpz_tdiv_r(x, x, p);
+                mpz_mul(x, x, p);
+                mpz_mul(x, x, p);
+                if (d < 0) mpz_sub_ui(x, x, 1);
+                if (d > 0) mpz_add_ui(x, x, 1);
+                conv(2);
+            }
+        mpz_clears(x, p, NULL);
+        check(ok, "DecimalDeviceLevelsMatchMpzGetStr" + (err.empty() ? std::string() : " (" + err + ")"));
+        check(est_ok, "DecimalDeviceLevelsEstimateWithinOne");
     }
 
     mpz_clears(a, b, r, e, NULL);

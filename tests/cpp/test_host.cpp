@@ -278,10 +278,18 @@ int main(int argc, char** argv) {
         bool ok = true;
         mpz_t x;
         mpz_init(x);
+        // Every quotient estimate is the quotient or one less (the device
+        // levels rely on it).
+        bool est_ok = true;
+        auto est = [&est_ok](const pi::DecimalStats& st) {
+            est_ok = est_ok && st.min_correction >= 0 && st.max_correction <= 1;
+        };
+        pi::DecimalStats st;
         for (size_t nb : {20000ul, 100000ul, 654321ul, 2000000ul}) {
             pi::draw_operand(x, g, nb);
             char* s = mpz_get_str(nullptr, 10, x);
-            ok = ok && pi::to_decimal(x, gmp, 4, 4096) == s;
+            ok = ok && pi::to_decimal(x, gmp, 4, 4096, &st) == s;
+            est(st);
             free(s);
         }
         for (unsigned long k : {8192ul, 65536ul, 100000ul, 131072ul}) {
@@ -290,10 +298,37 @@ int main(int argc, char** argv) {
                 if (d < 0) mpz_sub_ui(x, x, 1);
                 if (d > 0) mpz_add_ui(x, x, 1);
                 char* s = mpz_get_str(nullptr, 10, x);
-                ok = ok && pi::to_decimal(x, gmp, 3, 4096) == s;
+                ok = ok && pi::to_decimal(x, gmp, 3, 4096, &st) == s;
+                est(st);
                 free(s);
             }
         }
+        // Multiples of a split power, one below and above them, and runs of
+        // zeros and nines inside: remainders at the ends of their range.
+        for (unsigned long k : {16384ul, 50000ul}) {
+            mpz_t p;
+            mpz_init(p);
+            mpz_ui_pow_ui(p, 10, 8192);
+            for (unsigned long mlt : {1ul, 7ul, 99999ul}) {
+                for (int d : {-1, 0, 1}) {
+                    mpz_ui_pow_ui(x, 10, k);
+                    mpz_sub_ui(x, x, 1);
+                    mpz_tdiv_q(x, x, p);
+                    mpz_mul_ui(x, x, mlt);
+                    mpz_tdiv_r(x, x, p);  // below 10^8192
+                    mpz_mul(x, x, p);
+                    mpz_mul(x, x, p);
+                    if (d < 0) mpz_sub_ui(x, x, 1);
+                    if (d > 0) mpz_add_ui(x, x, 1);
+                    char* s = mpz_get_str(nullptr, 10, x);
+                    ok = ok && pi::to_decimal(x, gmp, 3, 4096, &st) == s;
+                    est(st);
+                    free(s);
+                }
+            }
+            mpz_clear(p);
+        }
+        check(est_ok, "DecimalQuotientEstimateWithinOne");
         // Powers built beforehand (for a width with the same k, or another k,
         // which is then ignored) give the same digits.
         bool pre_ok = true;

@@ -378,6 +378,38 @@ def test_device_subtract_negative_is_reported(engine):
     assert ei.value.status == 5
 
 
+@pytest.mark.parametrize("seed", range(2))
+def test_device_subtract_flags(engine, seed):
+    """The flagged form: each subtraction's sign is reported per job, a
+    negative result wraps modulo 2^(64 w), adds and non-negative subtractions
+    match the unflagged form, and the flags of one batch do not leak into the
+    next call."""
+    rng = np.random.default_rng(100 + seed)
+    pairs, sub = [], []
+    for xn in (1, 3, 4095, 4096, 4097, 9000):
+        for yn in (1, xn, xn + 2):
+            x = rng.integers(0, 2**64, xn, dtype=np.uint64)
+            y = rng.integers(0, 2**64, yn, dtype=np.uint64)
+            pairs += [(x, y), (y, x), (x, y)]
+            sub += [True, True, False]
+    k = 5000  # equal operands; a borrow running through every limb
+    ones = np.full(k, np.uint64(2**64 - 1))
+    pairs += [(ones, ones), (_int_limbs(1 << (64 * (k - 1)), k), np.array([1], np.uint64)),
+              (np.array([1], np.uint64), _int_limbs(1 << (64 * (k - 1)), k))]
+    sub += [True, True, True]
+    got, neg = pc.add_batch(pairs, sub, flags=True)
+    for g, n, (x, y), s in zip(got, neg, pairs, sub):
+        X, Y = pc.to_int(x), pc.to_int(y)
+        w = max(x.size, y.size)
+        if s:
+            assert n == (X < Y), (x.size, y.size)
+            assert pc.to_int(g) == (X - Y) % (1 << (64 * w)), (x.size, y.size)
+        else:
+            assert not n and pc.to_int(g) == X + Y
+    ok, neg2 = pc.add_batch([pairs[0], pairs[0]], [False, False], flags=True)
+    assert neg2 == [False, False]
+
+
 def test_plan_file_round_trip(engine, tmp_path):
     lib = pc.load_library()
     engine.autotune(12, 13)
