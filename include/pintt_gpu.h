@@ -122,6 +122,18 @@ int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, con
                                    const uint64_t* const* d_y, const size_t* yn, const int* subtract,
                                    uint64_t* const* d_out, int* negative);
 
+/* Decimal digits of device numbers: leaf i (xn[i] limbs at d_x[i], value
+ * below 10^width[i], width[i] <= PINTT_DECIMAL_LEAF_MAX_DIGITS, xn[i] <=
+ * PINTT_DECIMAL_LEAF_MAX_LIMBS) is written as exactly width[i] ASCII digits,
+ * left-padded with '0', at d_out + offset[i] (inside d_out's out_len bytes).
+ * A value not below 10^width[i] loses its top digits.  Waits for completion.
+ * The leaves of the device-resident decimal conversion: the reference
+ * converts with mpz_get_str on the host (src/io/base_converter.cpp). */
+#define PINTT_DECIMAL_LEAF_MAX_DIGITS 4096
+#define PINTT_DECIMAL_LEAF_MAX_LIMBS 214
+int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const size_t* width,
+                                  const size_t* offset, uint8_t* d_out, size_t out_len);
+
 /* Wait for all enqueued products (PINTT_ERR_HIP if the stream reports an
  * error). */
 int gpu_ntt_synchronize(void);

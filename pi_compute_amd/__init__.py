@@ -105,6 +105,11 @@ ABI = {
          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p),
          ctypes.POINTER(ctypes.c_int)],
     ),
+    "gpu_ntt_decimal_leaves_device": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+         ctypes.POINTER(ctypes.c_size_t), ctypes.c_void_p, ctypes.c_size_t],
+    ),
     # include/pintt_mlp.h
     "gpu_mlp_param_count": (ctypes.c_size_t, [ctypes.c_int]),
     "gpu_mlp_forward": (
@@ -373,6 +378,39 @@ def download(pairs) -> None:
     dst = (_u64p * n)(*[_ptr(x) for x, _ in pairs])
     src = (_u64p * n)(*[ctypes.cast(d, _u64p) for _, d in pairs])
     _check(lib, lib.gpu_ntt_download(n, dst, src, (ctypes.c_size_t * n)(*[x.size for x, _ in pairs])))
+
+
+def decimal_leaves(leaves) -> bytes:
+    """Device decimal digits: leaves = [(x, width)], x a numpy uint64 limb array
+    (value below 10^width, width <= 4096).  Uploads, converts all leaves into
+    one buffer (leaf i at the sum of the earlier widths), downloads it."""
+    lib = load_library()
+    n = len(leaves)
+    xs = [np.ascontiguousarray(x, np.uint64) for x, _ in leaves]
+    widths = [int(w) for _, w in leaves]
+    offsets = list(np.cumsum([0] + widths[:-1])) if n else []
+    total = sum(widths)
+    words = (total + 7) // 8
+    dev = []
+    try:
+        def alloc(k):
+            p = _u64p()
+            _check(lib, lib.gpu_ntt_dev_alloc(max(k, 1), ctypes.byref(p)))
+            dev.append(p)
+            return p
+        dx = [alloc(x.size) for x in xs]
+        dout = alloc(words)
+        arr = lambda ps: (_u64p * len(ps))(*ps)
+        sz = lambda v: (ctypes.c_size_t * len(v))(*[int(t) for t in v])
+        _check(lib, lib.gpu_ntt_upload(n, arr(dx), arr([_ptr(x) for x in xs]), sz([x.size for x in xs])))
+        _check(lib, lib.gpu_ntt_decimal_leaves_device(n, arr(dx), sz([x.size for x in xs]), sz(widths), sz(offsets),
+                                                      ctypes.cast(dout, ctypes.c_void_p), total))
+        host = np.zeros(max(words, 1), np.uint64)
+        _check(lib, lib.gpu_ntt_download(1, arr([_ptr(host)]), arr([dout]), sz([words])))
+        return host.tobytes()[:total]
+    finally:
+        for p in dev:
+            lib.gpu_ntt_dev_free(p)
 
 
 def add_batch(pairs, subtract, flags=False):

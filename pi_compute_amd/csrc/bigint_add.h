@@ -30,4 +30,18 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 hipError_t launch_add_batch(const AddJob* jobs, uint32_t njobs, uint32_t nblocks, uint8_t* maps, uint8_t* carry_in,
                             uint32_t* error, uint32_t* status, hipStream_t s);
 
+// Decimal leaves (decimal_leaves.hip): job i writes exactly width ASCII
+// digits of x (xn limbs, x < 10^width, width <= kLeafMaxDigits) at
+// out + offset, left-padded with '0'.
+constexpr int kLeafThreads = 64;
+constexpr int kLeafMaxDigits = 4096;
+constexpr int kLeafMaxLimbs = 214;  // ceil(4096 log2(10) / 64) + 1
+
+struct LeafJob {
+    const uint64_t* x;
+    uint64_t xn, width, offset;
+};
+
+hipError_t launch_decimal_leaves(const LeafJob* jobs, uint32_t njobs, uint8_t* out, hipStream_t s);
+
 }  // namespace pintt

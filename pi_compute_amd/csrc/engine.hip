@@ -38,6 +38,10 @@
 #include "bigint_add.h"
 #include "ntt_kernels.h"
 
+static_assert(pintt::kLeafMaxDigits == PINTT_DECIMAL_LEAF_MAX_DIGITS &&
+                  pintt::kLeafMaxLimbs == PINTT_DECIMAL_LEAF_MAX_LIMBS,
+              "decimal leaf limits");
+
 namespace pintt {
 namespace {
 
@@ -77,6 +81,10 @@ struct Engine {
     uint8_t* add_maps = nullptr;
     uint8_t* add_carry = nullptr;
     uint32_t* add_error = nullptr;
+    // Decimal leaves: job descriptors (pinned + device).
+    size_t leaf_job_cap = 0;
+    LeafJob* leaf_jobs_host = nullptr;
+    LeafJob* leaf_jobs_dev = nullptr;
     bool timing = false;
     hipEvent_t ev[5] = {};
     bool have_times = false;
@@ -984,6 +992,8 @@ void gpu_ntt_shutdown(void) {
     (void)hipFree(g.add_maps);
     (void)hipFree(g.add_carry);
     (void)hipFree(g.add_error);
+    if (g.leaf_jobs_host) (void)hipHostFree(g.leaf_jobs_host);
+    (void)hipFree(g.leaf_jobs_dev);
     (void)hipFree(g.tune_in);
     (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);
@@ -1270,6 +1280,42 @@ int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, con
                                    uint64_t* const* d_out, int* negative) {
     if (count && !negative) return fail(PINTT_ERR_ARGS, "null pointer");
     return add_batch(count, d_x, xn, d_y, yn, subtract, d_out, negative);
+}
+
+int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const size_t* width,
+                                  const size_t* offset, uint8_t* d_out, size_t out_len) {
+    if (count == 0) return PINTT_OK;
+    if (!d_x || !xn || !width || !offset || !d_out) return fail(PINTT_ERR_ARGS, "null pointer");
+    if (count > 0xFFFFFFFFull) return fail(PINTT_ERR_TOO_LARGE, "too many leaves");
+    std::vector<LeafJob> jobs(count);
+    for (size_t i = 0; i < count; ++i) {
+        if (!d_x[i] || xn[i] == 0 || xn[i] > (size_t)kLeafMaxLimbs || width[i] == 0 ||
+            width[i] > (size_t)kLeafMaxDigits || offset[i] > out_len || width[i] > out_len - offset[i])
+            return fail(PINTT_ERR_ARGS, "leaf: null pointer, length or width out of range");
+        jobs[i] = LeafJob{d_x[i], xn[i], width[i], offset[i]};
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    PINTT_HIP(hipStreamSynchronize(g.stream));  // the descriptor buffers below may be reused
+    if (count > g.leaf_job_cap) {
+        if (g.leaf_jobs_host) (void)hipHostFree(g.leaf_jobs_host);
+        (void)hipFree(g.leaf_jobs_dev);
+        g.leaf_jobs_host = nullptr;
+        g.leaf_jobs_dev = nullptr;
+        g.leaf_job_cap = 0;
+        const size_t cap = std::max<size_t>(count, 1024);
+        if (hipHostMalloc(reinterpret_cast<void**>(&g.leaf_jobs_host), cap * sizeof(LeafJob)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&g.leaf_jobs_dev), cap * sizeof(LeafJob)) != hipSuccess)
+            return fail(PINTT_ERR_NO_MEMORY, "leaf descriptors");
+        g.leaf_job_cap = cap;
+    }
+    std::memcpy(g.leaf_jobs_host, jobs.data(), count * sizeof(LeafJob));
+    PINTT_HIP(hipMemcpyAsync(g.leaf_jobs_dev, g.leaf_jobs_host, count * sizeof(LeafJob), hipMemcpyHostToDevice,
+                             g.stream));
+    PINTT_HIP(launch_decimal_leaves(g.leaf_jobs_dev, (uint32_t)count, d_out, g.stream));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    return PINTT_OK;
 }
 
 size_t gpu_ntt_graph_nodes(void* graph) { return graph ? static_cast<BatchGraph*>(graph)->nodes : 0; }

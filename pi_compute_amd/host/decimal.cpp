@@ -312,14 +312,16 @@ struct DevSplitNode {
     size_t offset, width;
 };
 
-// The conversion's levels on device-resident numbers: each level is two
-// product batches (estimate, estimate * P), a flagged subtraction for the
-// remainder x - q*P (never negative: the estimate never exceeds x / P), a
-// flagged subtraction of P (the estimate is floor(x / P) or one less) and one
-// that confirms the remainder is below P, and an increment of the quotients
-// that were one short.  Returns the leaves on the host.
-void device_split(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t leaf_digits,
-                  std::vector<std::unique_ptr<Node>>& out, DecimalStats& st) {
+// The whole conversion on device-resident numbers, into out (width
+// characters).  Each level is two product batches (estimate, estimate * P), a
+// flagged subtraction for the remainder x - q*P (never negative: the estimate
+// never exceeds x / P), a flagged subtraction of P (the estimate is
+// floor(x / P) or one less) and one that confirms the remainder is below P,
+// and an increment of the quotients that were one short.  The leaves (at most
+// PINTT_DECIMAL_LEAF_MAX_DIGITS digits: leaf_digits must not be larger) are
+// written as digits by the device and come back in one copy.
+void device_convert(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t leaf_digits, std::string& out,
+                    DecimalStats& st) {
     using Clock = std::chrono::steady_clock;
     auto t = Clock::now();
     auto lap = [&t]() {
@@ -489,26 +491,28 @@ void device_split(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t l
         }
         st.corrections_s += lap();
     }
-    // Leaves back to the host.
-    std::vector<uint64_t*> H(nodes.size());
-    std::vector<const uint64_t*> D(nodes.size());
-    std::vector<size_t> N(nodes.size());
-    out.clear();
-    for (size_t i = 0; i < nodes.size(); ++i) {
-        auto* nd = new Node;
-        out.emplace_back(nd);
-        H[i] = reinterpret_cast<uint64_t*>(mpz_limbs_write(nd->own, (mp_size_t)nodes[i].x.n));
-        D[i] = nodes[i].x.d;
-        N[i] = nodes[i].x.n;
-        nd->offset = nodes[i].offset;
-        nd->width = nodes[i].width;
+    // Leaves: digits written on the device, then one copy into out.
+    const size_t cnt = nodes.size();
+    std::vector<const uint64_t*> LX(cnt);
+    std::vector<size_t> LN(cnt), LW(cnt), LO(cnt);
+    for (size_t i = 0; i < cnt; ++i) {
+        if (nodes[i].width > (size_t)PINTT_DECIMAL_LEAF_MAX_DIGITS)
+            throw std::runtime_error("device conversion: leaf wider than the device leaves take");
+        LX[i] = nodes[i].x.d;
+        LN[i] = nodes[i].x.n;
+        LW[i] = nodes[i].width;
+        LO[i] = nodes[i].offset;
     }
-    dcheck(gpu_ntt_download(nodes.size(), H.data(), D.data(), N.data()), "gpu_ntt_download");
-    for (size_t i = 0; i < nodes.size(); ++i) {
-        mpz_limbs_finish(out[i]->own, (mp_size_t)N[i]);
-        out[i]->limbs = mpz_limbs_read(out[i]->own);
-        out[i]->size = mpz_size(out[i]->own);
-    }
+    const size_t words = (width + 7) / 8;
+    uint64_t* digits = mem.alloc(words);
+    dcheck(gpu_ntt_decimal_leaves_device(cnt, LX.data(), LN.data(), LW.data(), LO.data(),
+                                         reinterpret_cast<uint8_t*>(digits), width),
+           "gpu_ntt_decimal_leaves_device");
+    out.resize(words * 8);
+    uint64_t* h = reinterpret_cast<uint64_t*>(&out[0]);
+    const uint64_t* dd = digits;
+    dcheck(gpu_ntt_download(1, &h, &dd, &words), "gpu_ntt_download");
+    out.resize(width);
     st.leaves_s += lap();
 }
 
@@ -562,9 +566,10 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
     if (mpz_sgn(n) == 0) return "0";
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     leaf_digits = std::max<size_t>(leaf_digits, kLeafDigits);
+    if (device) leaf_digits = std::min<size_t>(leaf_digits, PINTT_DECIMAL_LEAF_MAX_DIGITS);
     const size_t width = mpz_sizeinbase(n, 10);
     if (width <= 2 * leaf_digits) return to_decimal(n, threads);
-    std::string out(width, '0');
+    std::string out;
 
     // The powers depend on the width only through k (2^k < width <= 2^(k+1)).
     int kw = 0;
@@ -577,14 +582,17 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
     st.powers_s = pre->powers_s;
     st.reciprocals_s = pre->reciprocals_s;
     lap(t);
-    std::vector<std::unique_ptr<Node>> nodes;
     if (device) {
-        device_split(n, width, *pre, leaf_digits, nodes, st);
-        lap(t);
-    } else {
-        split_on_host(n, width, *pre, leaf_digits, m, threads, nodes, st);
-        lap(t);
+        device_convert(n, width, *pre, leaf_digits, out, st);
+        size_t z = out.find_first_not_of('0');
+        if (z != 0) out.erase(0, z);
+        if (stats) *stats = st;
+        return out;
     }
+    out.assign(width, '0');
+    std::vector<std::unique_ptr<Node>> nodes;
+    split_on_host(n, width, *pre, leaf_digits, m, threads, nodes, st);
+    lap(t);
 
     // Leaves: host conversion, one node per task (largest first).
     Powers leaf_pw(2 * leaf_digits);  // shared read-only by every task

@@ -17,7 +17,8 @@
 //                             multiplier (default: newton with --gpu)
 //   --convert host|multiplier decimal conversion (default: multiplier with --gpu)
 //   --plans FILE              tile plans: loaded if FILE exists, saved after the run
-//   --convert-leaf D          multiplier conversion: host conversion below D digits (32768)
+//   --convert-leaf D          multiplier conversion: host conversion below D digits (32768;
+//                             device-resident conversion levels: device leaves of at most 4096)
 //   --bs-segments S           device levels: S segments, host subtrees of the next one computed
 //                             while the device reduces the current one (8; 1 = one tree)
 //   --bs-leaf T               terms per host subtree at the bottom of the tree (256)

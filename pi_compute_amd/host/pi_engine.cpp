@@ -11,6 +11,7 @@
 
 #include "decimal.h"
 #include "newton.h"
+#include "pintt_gpu.h"
 
 namespace pi {
 
@@ -28,6 +29,11 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
     const bool newton = cfg.final_step == PiConfig::FinalStep::Newton;
     const bool conv_mult = cfg.conversion == PiConfig::Conversion::Multiplier;
+    // Device-resident conversion levels end in device leaves, which take at most
+    // PINTT_DECIMAL_LEAF_MAX_DIGITS digits: the powers are built for that leaf.
+    const bool conv_device = conv_mult && cfg.level_batches && cfg.device_levels && cfg.device_conversion;
+    const size_t conv_leaf = conv_device ? std::min<size_t>(cfg.conversion_leaf_digits, PINTT_DECIMAL_LEAF_MAX_DIGITS)
+                                         : cfg.conversion_leaf_digits;
     // What does not depend on P, Q, R, built on one more thread beside binary
     // splitting: the final step's constants, then the conversion's powers of
     // ten and their reciprocals (pi * 10^D has D + 1 digits).
@@ -45,7 +51,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             dpow = std::make_unique<DecimalPowers>();
             dpow_done = dpow_set.get_future();
         }
-        beside = std::thread([this, &cfg, bits, &pre, &dpow, &pre_set, &dpow_set]() {
+        beside = std::thread([this, &cfg, bits, conv_leaf, &pre, &dpow, &pre_set, &dpow_set]() {
             if (pre) {
                 try {
                     final_constants(*pre, cfg.digits, bits + 64, mult_);
@@ -56,7 +62,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             }
             if (dpow) {
                 try {
-                    decimal_powers(*dpow, cfg.digits + 1, mult_, cfg.conversion_leaf_digits);
+                    decimal_powers(*dpow, cfg.digits + 1, mult_, conv_leaf);
                     dpow_set.set_value();
                 } catch (...) {
                     dpow_set.set_exception(std::current_exception());
@@ -144,18 +150,17 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     if (dpow_done.valid()) dpow_done.get();
     const double dpow_wait_s = since(t2);
     DecimalStats ds;
-    const bool conv_device = conv_mult && cfg.level_batches && cfg.device_levels && cfg.device_conversion;
     std::string d;
     if (conv_device) {
         try {
-            d = to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get(), true);
+            d = to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get(), true);
         } catch (const std::exception& e) {
             std::cerr << "  device conversion failed (" << e.what() << "); host levels instead" << std::endl;
             ds = DecimalStats{};
-            d = to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get());
+            d = to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get());
         }
     } else {
-        d = conv_mult ? to_decimal(pi_int, mult_, cfg.num_threads, cfg.conversion_leaf_digits, &ds, dpow.get())
+        d = conv_mult ? to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get())
                       : to_decimal(pi_int, cfg.num_threads);
     }
     mpz_clear(pi_int);

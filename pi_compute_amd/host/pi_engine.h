@@ -35,7 +35,9 @@ struct PiConfig {
     const Dispatcher* device_switch = nullptr;  // if set, it picks the first device level instead
     enum class FinalStep { Mpf, Newton } final_step = FinalStep::Mpf;
     enum class Conversion { Host, Multiplier } conversion = Conversion::Host;
-    size_t conversion_leaf_digits = 1 << 15;  // Conversion::Multiplier: host conversion below this width
+    // Conversion::Multiplier: host conversion below this width (device-resident
+    // conversion levels: at most 4096, the widest leaf the device converts)
+    size_t conversion_leaf_digits = 1 << 15;
     unsigned bs_segments = 8;                  // device levels: segments pipelined with their host subtrees
     unsigned long bs_leaf_span = 256;          // terms per host subtree at the bottom of the tree
     // sqrt(10005) and 10^D (FinalStep::Newton) and the conversion's powers of

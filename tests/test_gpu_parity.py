@@ -410,6 +410,33 @@ def test_device_subtract_flags(engine, seed):
     assert neg2 == [False, False]
 
 
+def test_device_decimal_leaves(engine):
+    """Device decimal leaves against Python's str(): widths around the 19-digit
+    chunks up to the 4096-digit limit, zero, 10^w - 1, 10^(w-1), random values,
+    zero top limbs up to the longest operand, hundreds of leaves in one call
+    (several workgroups), and the limits rejected on the host."""
+    import random
+    rnd = random.Random(11)
+    leaves, want = [], []
+    for w in (1, 2, 9, 10, 18, 19, 20, 37, 38, 39, 57, 100, 1000, 4095, 4096):
+        limbs = min(214, int(w * 3.3219280948873623 / 64) + 2)
+        vals = [0, 10**w - 1, 10 ** (w - 1), rnd.randrange(10**w), rnd.randrange(10 ** min(w, 19))]
+        for v in vals:
+            leaves.append((_int_limbs(v, limbs), w))
+            want.append(str(v).zfill(w))
+    for _ in range(400):
+        w = rnd.choice([4096, 4096, rnd.randrange(1, 4097)])
+        v = rnd.randrange(10**w)
+        leaves.append((_int_limbs(v, 214), w))
+        want.append(str(v).zfill(w))
+    got = pc.decimal_leaves(leaves)
+    assert got.decode() == "".join(want)
+    with pytest.raises(pc.PinttError):
+        pc.decimal_leaves([(np.zeros(1, np.uint64), 4097)])
+    with pytest.raises(pc.PinttError):
+        pc.decimal_leaves([(np.zeros(215, np.uint64), 10)])
+
+
 def test_plan_file_round_trip(engine, tmp_path):
     lib = pc.load_library()
     engine.autotune(12, 13)
