@@ -10,8 +10,8 @@
 #include <thread>
 #include <vector>
 
+#include "device_ops.h"
 #include "newton.h"
-#include "pintt_gpu.h"
 
 namespace pi {
 namespace {
@@ -259,35 +259,9 @@ void split_on_host(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t 
 
 }
 
-void dcheck(int rc, const char* what) {
-    if (rc != PINTT_OK) throw std::runtime_error(std::string(what) + ": " + gpu_ntt_last_error());
-}
-
-// Device allocations of one conversion, freed together at the end.
-class DevBuffers {
-public:
-    DevBuffers() = default;
-    ~DevBuffers() {
-        for (uint64_t* p : blocks_) (void)gpu_ntt_dev_free(p);
-    }
-    DevBuffers(const DevBuffers&) = delete;
-    DevBuffers& operator=(const DevBuffers&) = delete;
-    uint64_t* alloc(size_t limbs) {
-        uint64_t* p = nullptr;
-        dcheck(gpu_ntt_dev_alloc(std::max<size_t>(limbs, 1), &p), "gpu_ntt_dev_alloc");
-        blocks_.push_back(p);
-        return p;
-    }
-
-private:
-    std::vector<uint64_t*> blocks_;
-};
-
-// A device number: n >= 1 limbs, the top ones possibly zero.
-struct DevNum {
-    uint64_t* d = nullptr;
-    size_t n = 0;
-};
+void dcheck(int rc, const char* what) { dev::check(rc, what); }
+using DevBuffers = dev::Buffers;
+using DevNum = dev::Num;
 
 // Limbs that hold every value below 10^digits.
 size_t limbs_for_digits(size_t digits) { return (size_t)((double)digits * 3.3219280948873623 / 64.0) + 2; }

@@ -1,0 +1,249 @@
+// device_newton.cpp - see pi_engine.h (pi_scaled_newton_device).
+//
+// The final step's numbers stay in device memory: every truncation keeps
+// whole limbs (a pointer offset), products are gpu_ntt_mul_batch_device
+// calls, additions and subtractions gpu_ntt_add_batch_device(_flags), and the
+// host only reads a few limbs to learn signs.
+//
+// Reciprocal (beta = 2^64): Y ~ beta^(l + c) / D_l at precision c limbs,
+// D_l the top l = min(Ld, c + G) limbs of D.  One step to c' <= 2c - 2:
+//   D' = top l' = min(Ld, c' + G) limbs of D, K = l' + c, p = D' Y ~ beta^K,
+//   E = beta^K - p (|E| < beta^(K - c + 2) = beta^m), E_t = the limbs of |E|
+//   from m - (c' - c + G) up to m, term = (Y E_t) >> (K - (m - (c' - c + G))
+//   - (c' - c)) limbs, Y' = Y beta^(c' - c) +- term.
+// With G = 3 guard limbs each truncation costs under one unit of Y', and
+// the step's own error (the square of the relative error of Y) is below one
+// as well: Y ends within a few units of beta^(Ld + B) / D.  The same step on
+// host integers, checked against the exact quotient, is the host test
+// NewtonLimbsWithinFew (newton_reciprocal_limbs in newton.h).
+#include <gmp.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <utility>
+#include <vector>
+
+#include "device_ops.h"
+#include "newton.h"
+#include "pi_engine.h"
+
+namespace pi {
+namespace {
+
+using dev::Num;
+
+class Dev {
+public:
+    explicit Dev(size_t limbs) : base_(mem_.alloc(limbs)), cap_(limbs) {
+        const uint64_t c[2] = {0, 1};
+        zero = take(1);
+        one = take(1);
+        const uint64_t* src[2] = {&c[0], &c[1]};
+        uint64_t* dst[2] = {zero.d, one.d};
+        const size_t n[2] = {1, 1};
+        dev::check(gpu_ntt_upload(2, dst, src, n), "gpu_ntt_upload");
+    }
+    Num take(size_t n) {
+        n = std::max<size_t>(n, 1);
+        if (used_ + n > cap_) throw std::runtime_error("device final step: arena overflow");
+        Num r{base_ + used_, n};
+        used_ += n;
+        return r;
+    }
+    Num upload(mpz_srcptr x, size_t from_limb, size_t limbs) {
+        Num r = take(limbs);
+        const uint64_t* h = reinterpret_cast<const uint64_t*>(mpz_limbs_read(x)) + from_limb;
+        dev::check(gpu_ntt_upload(1, &r.d, &h, &limbs), "gpu_ntt_upload");
+        return r;
+    }
+    Num mul(Num a, Num b) {
+        Num r = take(a.n + b.n);
+        const uint64_t* A = a.d;
+        const uint64_t* B = b.d;
+        dev::check(gpu_ntt_mul_batch_device(1, &A, &a.n, &B, &b.n, &r.d, nullptr), "gpu_ntt_mul_batch_device");
+        return r;
+    }
+    // Limbs [from, from + count) of x (zero past its length).
+    std::vector<uint64_t> fetch(Num x, size_t from, size_t count) {
+        std::vector<uint64_t> h(count, 0);
+        if (from < x.n) {
+            size_t k = std::min(count, x.n - from);
+            uint64_t* hp = h.data();
+            const uint64_t* s = x.d + from;
+            dev::check(gpu_ntt_download(1, &hp, &s, &k), "gpu_ntt_download");
+        }
+        return h;
+    }
+    // out = x + y (max + 1 limbs) or x - y (max limbs); returns whether a
+    // subtraction went negative (the result is then x - y + beta^max).
+    bool addsub(uint64_t* out, Num x, Num y, bool sub) {
+        const uint64_t* X = x.d;
+        const uint64_t* Y = y.d;
+        const int s = sub ? 1 : 0;
+        int neg = 0;
+        dev::check(gpu_ntt_add_batch_device_flags(1, &X, &x.n, &Y, &y.n, &s, &out, &neg),
+                   "gpu_ntt_add_batch_device_flags");
+        return neg != 0;
+    }
+    Num zero, one;
+
+private:
+    dev::Buffers mem_;
+    uint64_t* base_;
+    size_t cap_, used_ = 0;
+};
+
+constexpr size_t kGuard = 3;
+constexpr size_t kExactLimbs = 16;
+
+// Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
+// and the same number on the host (its top limbs start the iteration).
+Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B) {
+    const size_t Ld = D.n;
+    if (mpz_size(Dh) != Ld) throw std::runtime_error("device reciprocal: host and device operands differ");
+    std::vector<size_t> sched;
+    for (size_t c = B; c > kExactLimbs; c = (c + 3) / 2) sched.push_back(c);
+    size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
+    Num Y;
+    {
+        const size_t l = std::min(Ld, c + kGuard);
+        mpz_t dl, t;
+        mpz_roinit_n(dl, mpz_limbs_read(Dh) + (Ld - l), (mp_size_t)l);
+        mpz_init_set_ui(t, 1);
+        mpz_mul_2exp(t, t, 64 * (l + c));
+        mpz_tdiv_q(t, t, dl);
+        Y = g.upload(t, 0, mpz_size(t));
+        mpz_clear(t);
+    }
+    for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
+        const size_t cp = *it, s = cp - c;
+        const size_t l = std::min(Ld, cp + kGuard);
+        const Num Dp{D.d + (Ld - l), l};
+        const size_t K = l + c;
+        const Num p = g.mul(Dp, Y);  // ~ beta^K, p.n >= K + 1
+        const std::vector<uint64_t> top = g.fetch(p, K - 1, p.n - K + 1);
+        bool above = false;
+        for (size_t i = 2; i < top.size(); ++i) above = above || top[i] != 0;
+        Num E;  // |beta^K - p|, K limbs
+        bool positive;
+        if (!above && top[1] == 1) {  // p >= beta^K
+            positive = false;
+            E = Num{p.d, K};
+        } else if (!above && top[1] == 0 && top[0] == ~0ull) {  // p < beta^K
+            positive = true;
+            E = g.take(K);
+            (void)g.addsub(E.d, g.zero, Num{p.d, K}, true);  // beta^K - p
+        } else {
+            throw std::runtime_error("device reciprocal: product far from beta^K");
+        }
+        const size_t m = K - c + 2;
+        if (m < K) {
+            const std::vector<uint64_t> z = g.fetch(E, m, std::min<size_t>(2, K - m));
+            for (uint64_t v : z)
+                if (v) throw std::runtime_error("device reciprocal: correction above its bound");
+        }
+        const size_t T = s + kGuard;
+        const size_t lo = m > T ? m - T : 0;
+        const Num Et{E.d + lo, std::min(m, K) - lo};
+        const Num p2 = g.mul(Y, Et);
+        const size_t sh = K - lo - s;
+        if (p2.n <= sh + s) throw std::runtime_error("device reciprocal: correction shorter than expected");
+        const Num term{p2.d + sh, p2.n - sh};
+        const Num hi{term.d + s, term.n - s};
+        const size_t hn = std::max(Y.n, hi.n) + 1;
+        Num Yn = g.take(s + hn);
+        if (positive) {
+            dev::check(gpu_ntt_dev_copy(Yn.d, term.d, s), "gpu_ntt_dev_copy");
+            (void)g.addsub(Yn.d + s, Y, hi, false);
+        } else {
+            // Y beta^s - term: the low limbs are -term mod beta^s, with a borrow
+            // into the high part when they are not all zero.
+            const bool borrow = g.addsub(Yn.d, g.zero, Num{term.d, s}, true);
+            if (borrow) {
+                Num t = g.take(std::max(Y.n, hi.n));
+                if (g.addsub(t.d, Y, hi, true) || g.addsub(Yn.d + s, t, g.one, true))
+                    throw std::runtime_error("device reciprocal: negative estimate");
+            } else if (g.addsub(Yn.d + s, Y, hi, true)) {
+                throw std::runtime_error("device reciprocal: negative estimate");
+            }
+            Yn.n = s + std::max(Y.n, hi.n);
+        }
+        Yn.n = std::min(Yn.n, cp + 2);  // Y' <= beta^(c' + 1) plus a few units
+        Y = Yn;
+        c = cp;
+    }
+    return Y;
+}
+
+// The top `limbs` limbs of x: (first limb, count).
+std::pair<size_t, size_t> top_limbs(mpz_srcptr x, size_t limbs) {
+    const size_t n = mpz_size(x);
+    return n > limbs ? std::make_pair(n - limbs, limbs) : std::make_pair((size_t)0, n);
+}
+
+}  // namespace
+
+void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B) {
+    if (mpz_sgn(d) <= 0) throw std::runtime_error("device reciprocal: divisor must be positive");
+    const size_t Ld = mpz_size(d);
+    Dev g(8 * (Ld + B) + 64);
+    const Num D = g.upload(d, 0, Ld);
+    const Num Y = reciprocal(g, D, d, B);
+    uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(y, (mp_size_t)Y.n));
+    const uint64_t* src = Y.d;
+    size_t n = Y.n;
+    dev::check(gpu_ntt_download(1, &h, &src, &n), "gpu_ntt_download");
+    mpz_limbs_finish(y, (mp_size_t)Y.n);
+}
+
+void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, const FinalConstants& pre) {
+    if (w % 64 != 0) throw std::runtime_error("device final step: precision not a whole number of limbs");
+    if (mpz_sgn(Q) <= 0 || mpz_sgn(R) <= 0 || mpz_sgn(pre.inv_sqrt) <= 0 || mpz_sgn(pre.pow10) <= 0)
+        throw std::runtime_error("device final step: operands must be positive");
+    const size_t W = w / 64 + 2;  // working limbs
+    Dev g(40 * W + 64);
+    long ex = 0;  // value = integer * beta^ex, before the 2^w of the square root's scale
+    // 426880 * sqrt(10005) = 426880 * 10005 * y / 2^w, y = pre.inv_sqrt
+    const auto q = top_limbs(Q, W), y = top_limbs(pre.inv_sqrt, W), t10 = top_limbs(pre.pow10, W),
+               r = top_limbs(R, W);
+    const Num Qd = g.upload(Q, q.first, q.second);
+    const Num yd = g.upload(pre.inv_sqrt, y.first, y.second);
+    const Num pd = g.upload(pre.pow10, t10.first, t10.second);
+    const Num Rd = g.upload(R, r.first, r.second);
+    ex += (long)(q.first + y.first + t10.first);
+    auto cut = [&](Num x) {  // the top W limbs of x; the exponent follows
+        if (x.n <= W) return x;
+        ex += (long)(x.n - W);
+        return Num{x.d + (x.n - W), W};
+    };
+    Num k = g.take(1);
+    {
+        const uint64_t v = 426880ull * 10005ull;
+        const uint64_t* h = &v;
+        size_t one = 1;
+        dev::check(gpu_ntt_upload(1, &k.d, &h, &one), "gpu_ntt_upload");
+    }
+    Num t = cut(g.mul(Qd, yd));
+    t = cut(g.mul(t, pd));
+    t = cut(g.mul(t, k));
+    mpz_t rh;
+    mpz_roinit_n(rh, mpz_limbs_read(R) + r.first, (mp_size_t)r.second);
+    const Num inv = reciprocal(g, Rd, rh, W);  // 1/R ~ inv / beta^(r.second + W + r.first)
+    const Num z = g.mul(t, inv);
+    ex -= (long)(r.second + W + r.first) + (long)(w / 64);
+    // floor(z * beta^ex): ex < 0 drops limbs.
+    if (ex >= 0) throw std::runtime_error("device final step: unexpected scale");
+    const size_t drop = (size_t)(-ex);
+    if (drop >= z.n) {
+        mpz_set_ui(out, 0);
+        return;
+    }
+    const size_t n = z.n - drop;
+    uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(out, (mp_size_t)n));
+    const uint64_t* src = z.d + drop;
+    size_t nn = n;
+    dev::check(gpu_ntt_download(1, &h, &src, &nn), "gpu_ntt_download");
+    mpz_limbs_finish(out, (mp_size_t)n);
+}
+
+}  // namespace pi

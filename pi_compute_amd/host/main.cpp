@@ -24,6 +24,8 @@
 //   --bs-leaf T               terms per host subtree at the bottom of the tree (256)
 //   --no-overlap              Newton final step: compute sqrt(10005) and 10^D after binary
 //                             splitting instead of beside it
+//   --host-final-step         with --device-levels: the Newton final step on host numbers
+//                             (products staged through PCIe) instead of device-resident ones
 //   --host-conversion-levels  with --device-levels: the multiplier conversion's levels on
 //                             host numbers (products staged through PCIe) instead of on
 //                             device-resident ones
@@ -64,7 +66,7 @@ void usage() {
                  "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
                  "                      [--final mpf|newton] [--convert host|multiplier] [--convert-leaf D]\n"
                  "                      [--device-levels [LIMBS|auto]] [--bs-segments S] [--bs-leaf T] [--no-overlap]\n"
-                 "                      [--host-conversion-levels]\n"
+                 "                      [--host-final-step] [--host-conversion-levels]\n"
                  "                      [--plans FILE]\n"
                  "                      [--verbose]\n";
 }
@@ -107,6 +109,7 @@ int main(int argc, char** argv) {
         else if (a == "--bs-segments") cfg.bs_segments = (unsigned)std::stoul(next());
         else if (a == "--no-overlap") cfg.overlap_constants = false;
         else if (a == "--host-conversion-levels") cfg.device_conversion = false;
+        else if (a == "--host-final-step") cfg.device_final = false;
         else if (a == "--bs-leaf") cfg.bs_leaf_span = std::stoul(next());
         else if (a == "--device-levels") {
             cfg.device_levels = true;

@@ -10,6 +10,8 @@
 // cuts (2^-(c'+g-1) each) plus one unit from the floor.
 #include "newton.h"
 
+#include <algorithm>
+#include <stdexcept>
 #include <vector>
 
 namespace pi {
@@ -102,6 +104,52 @@ void newton_reciprocal(mpz_t y, mpz_srcptr d, size_t b, Multiplier& m) {
         cur = c;
     }
     mpz_clears(t, dt, e, p, NULL);
+}
+
+void newton_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B) {
+    constexpr size_t G = 3, kExact = 16;
+    const size_t Ld = mpz_size(d);
+    std::vector<size_t> sched;
+    for (size_t c = B; c > kExact; c = (c + 3) / 2) sched.push_back(c);
+    size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
+    mpz_t dp, p, e, t;
+    mpz_inits(dp, p, e, t, NULL);
+    auto top = [&](size_t l) {  // the top l limbs of d
+        mpz_tdiv_q_2exp(dp, d, 64 * (Ld - l));
+    };
+    {
+        const size_t l = std::min(Ld, c + G);
+        top(l);
+        mpz_set_ui(t, 1);
+        mpz_mul_2exp(t, t, 64 * (l + c));
+        mpz_tdiv_q(y, t, dp);
+    }
+    for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
+        const size_t cp = *it, s = cp - c;
+        const size_t l = std::min(Ld, cp + G);
+        top(l);
+        const size_t K = l + c;
+        mpz_mul(p, dp, y);
+        mpz_set_ui(e, 1);
+        mpz_mul_2exp(e, e, 64 * K);
+        mpz_sub(e, e, p);
+        const int sign = mpz_sgn(e);
+        mpz_abs(e, e);
+        const size_t m = K - c + 2;
+        if (mpz_size(e) > m) throw std::runtime_error("newton_reciprocal_limbs: correction above its bound");
+        const size_t T = s + G;
+        const size_t lo = m > T ? m - T : 0;
+        mpz_tdiv_q_2exp(t, e, 64 * lo);
+        mpz_mul(p, y, t);
+        mpz_tdiv_q_2exp(p, p, 64 * (K - lo - s));
+        mpz_mul_2exp(y, y, 64 * s);
+        if (sign >= 0)
+            mpz_add(y, y, p);
+        else
+            mpz_sub(y, y, p);
+        c = cp;
+    }
+    mpz_clears(dp, p, e, t, NULL);
 }
 
 void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m) {

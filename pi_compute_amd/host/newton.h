@@ -21,6 +21,12 @@ namespace pi {
 // y ~ 2^(n+b) / d, n = bit length of d > 0; |y - 2^(n+b)/d| <= 4.
 void newton_reciprocal(mpz_t y, mpz_srcptr d, size_t b, Multiplier& m);
 
+// The same quotient in whole limbs (beta = 2^64): y ~ beta^(Ld + B) / d with
+// Ld = limbs of d > 0, every truncation a whole number of limbs (3 guard
+// limbs), exact products on the host.  The host statement of the device
+// reciprocal in the final step (pi_scaled_newton_device).
+void newton_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
+
 // y ~ 2^b / sqrt(a), a > 0; |y - 2^b/sqrt(a)| <= 4.
 void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m);
 

@@ -29,6 +29,10 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const mp_bitcnt_t bits = static_cast<mp_bitcnt_t>(precision * 3.3219281) + 64;
     const bool newton = cfg.final_step == PiConfig::FinalStep::Newton;
     const bool conv_mult = cfg.conversion == PiConfig::Conversion::Multiplier;
+    // With device-resident levels the Newton final step also runs on device
+    // numbers, cut to whole limbs: its precision is rounded up to a limb.
+    const bool final_device = newton && cfg.level_batches && cfg.device_levels && cfg.device_final;
+    const size_t fin_w = final_device ? (bits + 64 + 63) / 64 * 64 : bits + 64;
     // Device-resident conversion levels end in device leaves, which take at most
     // PINTT_DECIMAL_LEAF_MAX_DIGITS digits: the powers are built for that leaf.
     const bool conv_device = conv_mult && cfg.level_batches && cfg.device_levels && cfg.device_conversion;
@@ -51,10 +55,10 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             dpow = std::make_unique<DecimalPowers>();
             dpow_done = dpow_set.get_future();
         }
-        beside = std::thread([this, &cfg, bits, conv_leaf, &pre, &dpow, &pre_set, &dpow_set]() {
+        beside = std::thread([this, &cfg, fin_w, conv_leaf, &pre, &dpow, &pre_set, &dpow_set]() {
             if (pre) {
                 try {
-                    final_constants(*pre, cfg.digits, bits + 64, mult_);
+                    final_constants(*pre, cfg.digits, fin_w, mult_);
                     pre_set.set_value();
                 } catch (...) {
                     pre_set.set_exception(std::current_exception());
@@ -113,8 +117,23 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     mpz_t pi_int;
     mpz_init(pi_int);
     NewtonStats ns;
+    bool fin_on_device = false;
     if (newton && mpz_sgn(r.R) > 0) {
-        pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, bits + 64, mult_, &ns, pre.get());
+        if (final_device) {
+            try {
+                const FinalConstants* c = pre.get();
+                FinalConstants own;
+                if (!c) {
+                    final_constants(own, cfg.digits, fin_w, mult_);
+                    c = &own;
+                }
+                pi_scaled_newton_device(pi_int, r.Q, r.R, fin_w, *c);
+                fin_on_device = true;
+            } catch (const std::exception& e) {
+                std::cerr << "  device final step failed (" << e.what() << "); host final step instead" << std::endl;
+            }
+        }
+        if (!fin_on_device) pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, fin_w, mult_, &ns, pre.get());
         mpz_realloc2(r.Q, 0);
         mpz_realloc2(r.R, 0);
     } else {
@@ -180,7 +199,9 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     }
     if (cfg.verbose) {
         std::cout << "  final step: " << fin_s << " s, decimal conversion: " << conv_s << " s" << std::endl;
-        if (newton) {
+        if (fin_on_device) {
+            std::cout << "    final step on device-resident numbers" << std::endl;
+        } else if (newton) {
             std::cout << "    final step: inverse square root " << ns.inv_sqrt_s << " s, 10^D " << ns.pow10_s
                       << " s, reciprocal " << ns.reciprocal_s << " s, products " << ns.products_s << " s";
             if (pre) std::cout << " (square root and 10^D beside binary splitting, waited " << pre_wait_s << " s)";

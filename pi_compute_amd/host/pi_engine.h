@@ -47,6 +47,9 @@ struct PiConfig {
     // Conversion::Multiplier with device-resident levels: the conversion's
     // levels also run on device-resident numbers (decimal.h), leaves on the host.
     bool device_conversion = true;
+    // FinalStep::Newton with device-resident levels: the final step on
+    // device-resident numbers too (pi_scaled_newton_device).
+    bool device_final = true;
 };
 
 struct PiResult {
@@ -76,6 +79,18 @@ struct NewtonStats {
 };
 void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size_t bits, Multiplier& m,
                       NewtonStats* stats = nullptr, const FinalConstants* pre = nullptr);
+
+// Same quotient with every operand in device memory (the GPU engine's device
+// entry points): Q, R and the constants are uploaded once, cut to w/64 + 2
+// limbs, the reciprocal of R runs as whole-limb Newton steps (newton.h,
+// newton_reciprocal_limbs) and only the result comes back.  w must be a
+// multiple of 64 and pre computed with it.  Throws std::runtime_error if the
+// device fails or a check on the way does.
+void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, const FinalConstants& pre);
+
+// The device reciprocal alone (d uploaded, y downloaded): the same integer as
+// newton_reciprocal_limbs(y, d, B), step for step.
+void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
 
 class PiEngine {
 public:

@@ -17,6 +17,8 @@
 #include "decimal.h"
 #include "hip_ntt_multiplier.h"
 #include "operands.h"
+#include "pi_engine.h"
+#include "newton.h"
 
 namespace {
 
@@ -266,6 +268,71 @@ int main() {
         mpz_clears(x, p, NULL);
         check(ok, "DecimalDeviceLevelsMatchMpzGetStr" + (err.empty() ? std::string() : " (" + err + ")"));
         check(est_ok, "DecimalDeviceLevelsEstimateWithinOne");
+    }
+
+    // Whole-limb reciprocal on the device: the same integer as its host
+    // statement, step for step, for divisors shorter and longer than the
+    // precision, 2^(64 L - 1) and all-ones divisors.
+    {
+        bool ok = true;
+        std::string err;
+        pi::SplitMix64 g(5);
+        mpz_t d, yd, yh;
+        mpz_inits(d, yd, yh, NULL);
+        const size_t cases[][2] = {{1, 40}, {3, 17}, {50, 400}, {700, 300}, {2000, 5000}, {40000, 40000},
+                                   {100000, 60000}};
+        for (int kind = 0; kind < 3; ++kind)
+            for (const auto& cs : cases) {
+                const size_t Ld = cs[0], B = cs[1];
+                if (kind == 0) {
+                    pi::draw_operand(d, g, 64 * Ld);
+                    mpz_setbit(d, 64 * Ld - 1);
+                } else if (kind == 1) {
+                    mpz_set_ui(d, 0);
+                    mpz_setbit(d, 64 * Ld - 1);
+                } else {
+                    mpz_set_ui(d, 0);
+                    mpz_setbit(d, 64 * Ld);
+                    mpz_sub_ui(d, d, 1);
+                }
+                try {
+                    pi::device_reciprocal_limbs(yd, d, B);
+                    pi::newton_reciprocal_limbs(yh, d, B);
+                    ok = ok && mpz_cmp(yd, yh) == 0;
+                } catch (const std::exception& ex) {
+                    ok = false;
+                    err = ex.what();
+                }
+            }
+        mpz_clears(d, yd, yh, NULL);
+        check(ok, "DeviceReciprocalMatchesHostStatement" + (err.empty() ? std::string() : " (" + err + ")"));
+    }
+    // The final step on device numbers against the host final step.
+    {
+        bool ok = true;
+        std::string err;
+        for (size_t digits : {20000ul, 150001ul}) {
+            const size_t prec = digits + 100;
+            const unsigned long terms = pi::BinarySplitting::terms_needed(prec);
+            pi::BSResult bs;
+            pi::BinarySplitting::compute_small(0, terms, bs);
+            const size_t bits = (size_t)(prec * 3.3219281) + 64;
+            const size_t w = (bits + 64 + 63) / 64 * 64;
+            pi::FinalConstants c;
+            pi::final_constants(c, digits, w, gpu);
+            mpz_t hd, hh;
+            mpz_inits(hd, hh, NULL);
+            try {
+                pi::pi_scaled_newton_device(hd, bs.Q, bs.R, w, c);
+                pi::pi_scaled_newton(hh, bs.Q, bs.R, digits, w, gpu, nullptr, &c);
+                ok = ok && mpz_cmp(hd, hh) == 0;
+            } catch (const std::exception& ex) {
+                ok = false;
+                err = ex.what();
+            }
+            mpz_clears(hd, hh, NULL);
+        }
+        check(ok, "DeviceFinalStepMatchesHost" + (err.empty() ? std::string() : " (" + err + ")"));
     }
 
     mpz_clears(a, b, r, e, NULL);

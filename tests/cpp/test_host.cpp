@@ -217,6 +217,32 @@ int main(int argc, char** argv) {
                 ok = ok && mpz_cmp_ui(t, 4) <= 0;
             }
         check(ok, "NewtonInverseSqrtWithin4");
+        // Whole-limb reciprocal (the device final step's algorithm): random
+        // divisors, 2^(64 L - 1) and all-ones divisors, precisions below and far
+        // above the divisor's length.
+        ok = true;
+        for (int it = 0; it < 600; ++it) {
+            const size_t Ld = 1 + (size_t)(it * 37) % 260, B = 1 + (size_t)(it * 53) % 700;
+            if (it % 3 == 0) {
+                pi::draw_operand(d, g, 64 * Ld);
+                mpz_setbit(d, 64 * Ld - 1);
+            } else if (it % 3 == 1) {
+                mpz_set_ui(d, 0);
+                mpz_setbit(d, 64 * Ld - 1);
+            } else {
+                mpz_set_ui(d, 0);
+                mpz_setbit(d, 64 * Ld);
+                mpz_sub_ui(d, d, 1);
+            }
+            pi::newton_reciprocal_limbs(y, d, B);
+            mpz_set_ui(t, 0);
+            mpz_setbit(t, 64 * (Ld + B));
+            mpz_tdiv_q(want, t, d);
+            mpz_sub(t, y, want);
+            mpz_abs(t, t);
+            ok = ok && mpz_cmp_ui(t, 4) <= 0;
+        }
+        check(ok, "NewtonLimbsWithinFew");
         ok = true;
         for (unsigned long e : {0ul, 1ul, 2ul, 77ul, 1000ul, 123457ul}) {
             pi::pow_ui(y, 10, e, gmp);

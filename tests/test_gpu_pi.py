@@ -1,9 +1,9 @@
 """Pi digit files through the GPU multiplier equal the GMP-only run, byte for
 byte, in every dispatch mode (level batches, per-call, depth-first), with the
 Newton final step and the Newton-quotient decimal conversion (the defaults
-with --gpu; with device-resident levels the conversion's levels run on the
-device too, or on the host with --host-conversion-levels) and with the
-reference's mpf final step."""
+with --gpu; with device-resident levels the final step and the conversion
+run on device numbers too, or on the host with --host-final-step and
+--host-conversion-levels) and with the reference's mpf final step."""
 import subprocess
 from pathlib import Path
 
@@ -50,12 +50,13 @@ def gmp_digits(tmp_path_factory):
         ["--tree", "levels", "--device-levels", "--bs-segments", "1"],
         ["--tree", "levels", "--device-levels", "--no-overlap"],
         ["--tree", "levels", "--device-levels", "--host-conversion-levels"],
+        ["--tree", "levels", "--device-levels", "--host-final-step"],
         ["--tree", "levels", "--device-levels", "--convert-leaf", "4096"],
     ],
     ids=["levels", "levels-per-call", "recursive", "levels-t64", "recursive-t64", "levels-mpf-host",
          "levels-t64-convert", "device-levels", "device-levels-2000", "device-levels-3-segments",
          "device-levels-one-tree", "device-levels-serial-constants", "device-levels-host-conversion",
-         "device-levels-conversion-leaf-4096"],
+         "device-levels-host-final-step", "device-levels-conversion-leaf-4096"],
 )
 def test_pi_gpu_matches_gmp_run(tmp_path, gmp_digits, mode):
     digits = 1000000
