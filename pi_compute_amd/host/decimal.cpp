@@ -497,7 +497,7 @@ DecimalPowers::~DecimalPowers() {
     for (auto& x : inv) mpz_clear(&x);
 }
 
-void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits) {
+void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits, bool device) {
     using Clock = std::chrono::steady_clock;
     auto t = Clock::now();
     leaf_digits = std::max<size_t>(leaf_digits, kLeafDigits);
@@ -520,8 +520,12 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
     t = Clock::now();
     for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
         const size_t np = mpz_sizeinbase(&d.pw[j], 2);
-        newton_reciprocal(&d.inv[j], &d.pw[j], split_shifts(np).e2 - np, m);  // ~ 2^e2 / P, within 4
-        mpz_sub_ui(&d.inv[j], &d.inv[j], 5);                                    // below 2^e2 / P
+        const size_t e2 = split_shifts(np).e2;
+        if (device)  // whole limbs: e2 / 64 = limbs(P) + B
+            device_reciprocal_limbs(&d.inv[j], &d.pw[j], e2 / 64 - mpz_size(&d.pw[j]));
+        else
+            newton_reciprocal(&d.inv[j], &d.pw[j], e2 - np, m);  // ~ 2^e2 / P, within 4
+        mpz_sub_ui(&d.inv[j], &d.inv[j], 5);                     // below 2^e2 / P
     }
     d.reciprocals_s = std::chrono::duration<double>(Clock::now() - t).count();
 }

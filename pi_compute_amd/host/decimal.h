@@ -47,7 +47,10 @@ struct DecimalPowers {
     DecimalPowers(const DecimalPowers&) = delete;
     DecimalPowers& operator=(const DecimalPowers&) = delete;
 };
-void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits = 1 << 15);
+// device: the reciprocals as whole-limb Newton steps on device numbers
+// (newton.h, device_reciprocal_limbs); the powers go through m either way.
+void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits = 1 << 15,
+                    bool device = false);
 
 // `pre`: powers built for a width with the same k and for the same leaf_digits
 // (others are ignored).  `device`: the levels run on device-resident numbers

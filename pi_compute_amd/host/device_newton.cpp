@@ -94,7 +94,8 @@ private:
 };
 
 constexpr size_t kGuard = 3;
-constexpr size_t kExactLimbs = 16;
+constexpr size_t kExactLimbs = 16;   // the host mirror's exact start (newton.cpp)
+constexpr size_t kHostLimbs = 16384;
 
 // Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
 // and the same number on the host (its top limbs start the iteration).
@@ -103,15 +104,19 @@ Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B) {
     if (mpz_size(Dh) != Ld) throw std::runtime_error("device reciprocal: host and device operands differ");
     std::vector<size_t> sched;
     for (size_t c = B; c > kExactLimbs; c = (c + 3) / 2) sched.push_back(c);
+    // The steps up to kHostLimbs run on the host (newton_reciprocal_limbs to
+    // that precision is the same integer the device steps would reach; small
+    // products are cheaper there than device round trips).
     size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
+    while (!sched.empty() && sched.back() <= kHostLimbs) {
+        c = sched.back();
+        sched.pop_back();
+    }
     Num Y;
     {
-        const size_t l = std::min(Ld, c + kGuard);
-        mpz_t dl, t;
-        mpz_roinit_n(dl, mpz_limbs_read(Dh) + (Ld - l), (mp_size_t)l);
-        mpz_init_set_ui(t, 1);
-        mpz_mul_2exp(t, t, 64 * (l + c));
-        mpz_tdiv_q(t, t, dl);
+        mpz_t t;
+        mpz_init(t);
+        newton_reciprocal_limbs(t, Dh, c);
         Y = g.upload(t, 0, mpz_size(t));
         mpz_clear(t);
     }

@@ -27,6 +27,12 @@ void newton_reciprocal(mpz_t y, mpz_srcptr d, size_t b, Multiplier& m);
 // reciprocal in the final step (pi_scaled_newton_device).
 void newton_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
 
+// The same steps on device numbers through the GPU engine's device entry
+// points (device_newton.cpp): d uploaded, y downloaded, the same integer as
+// newton_reciprocal_limbs(y, d, B).  Throws std::runtime_error if the device
+// fails.
+void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
+
 // y ~ 2^b / sqrt(a), a > 0; |y - 2^b/sqrt(a)| <= 4.
 void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m);
 

@@ -55,7 +55,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             dpow = std::make_unique<DecimalPowers>();
             dpow_done = dpow_set.get_future();
         }
-        beside = std::thread([this, &cfg, fin_w, conv_leaf, &pre, &dpow, &pre_set, &dpow_set]() {
+        beside = std::thread([this, &cfg, fin_w, conv_leaf, conv_device, &pre, &dpow, &pre_set, &dpow_set]() {
             if (pre) {
                 try {
                     final_constants(*pre, cfg.digits, fin_w, mult_);
@@ -66,7 +66,15 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             }
             if (dpow) {
                 try {
-                    decimal_powers(*dpow, cfg.digits + 1, mult_, conv_leaf);
+                    try {
+                        decimal_powers(*dpow, cfg.digits + 1, mult_, conv_leaf, conv_device);
+                    } catch (const std::runtime_error& e) {
+                        if (!conv_device) throw;
+                        std::cerr << "  device reciprocals failed (" << e.what() << "); host reciprocals instead"
+                                  << std::endl;
+                        dpow = std::make_unique<DecimalPowers>();
+                        decimal_powers(*dpow, cfg.digits + 1, mult_, conv_leaf, false);
+                    }
                     dpow_set.set_value();
                 } catch (...) {
                     dpow_set.set_exception(std::current_exception());

@@ -88,10 +88,6 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
 // device fails or a check on the way does.
 void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, const FinalConstants& pre);
 
-// The device reciprocal alone (d uploaded, y downloaded): the same integer as
-// newton_reciprocal_limbs(y, d, B), step for step.
-void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
-
 class PiEngine {
 public:
     explicit PiEngine(Multiplier& m) : mult_(m) {}
