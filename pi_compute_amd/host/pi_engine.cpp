@@ -38,14 +38,14 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     const bool conv_device = conv_mult && cfg.level_batches && cfg.device_levels && cfg.device_conversion;
     const size_t conv_leaf = conv_device ? std::min<size_t>(cfg.conversion_leaf_digits, PINTT_DECIMAL_LEAF_MAX_DIGITS)
                                          : cfg.conversion_leaf_digits;
-    // What does not depend on P, Q, R, built on one more thread beside binary
-    // splitting: the final step's constants, then the conversion's powers of
+    // What does not depend on P, Q, R, built beside binary splitting on two
+    // more threads: the final step's constants, and the conversion's powers of
     // ten and their reciprocals (pi * 10^D has D + 1 digits).
     std::unique_ptr<FinalConstants> pre;
     std::unique_ptr<DecimalPowers> dpow;
     std::promise<void> pre_set, dpow_set;
     std::future<void> pre_done, dpow_done;
-    std::thread beside;
+    std::thread beside, beside2;
     if (cfg.overlap_constants && (newton || conv_mult)) {
         if (newton) {
             pre = std::make_unique<FinalConstants>();
@@ -55,15 +55,16 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             dpow = std::make_unique<DecimalPowers>();
             dpow_done = dpow_set.get_future();
         }
-        beside = std::thread([this, &cfg, fin_w, conv_leaf, conv_device, &pre, &dpow, &pre_set, &dpow_set]() {
-            if (pre) {
+        if (pre)
+            beside = std::thread([this, &cfg, fin_w, &pre, &pre_set]() {
                 try {
                     final_constants(*pre, cfg.digits, fin_w, mult_);
                     pre_set.set_value();
                 } catch (...) {
                     pre_set.set_exception(std::current_exception());
                 }
-            }
+            });
+        beside2 = std::thread([this, &cfg, conv_leaf, conv_device, &dpow, &dpow_set]() {
             if (dpow) {
                 try {
                     try {
@@ -82,13 +83,13 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             }
         });
     }
-    // The thread is joined on every way out of this function.
+    // The threads are joined on every way out of this function.
     struct Joiner {
         std::thread& t;
         ~Joiner() {
             if (t.joinable()) t.join();
         }
-    } joiner{beside};
+    } joiner{beside}, joiner2{beside2};
 
     BinarySplitting bs(mult_, cfg.num_threads);
     bs.set_switch_dispatcher(cfg.device_switch);
