@@ -86,6 +86,7 @@ struct Engine {
     LeafJob* leaf_jobs_host = nullptr;
     LeafJob* leaf_jobs_dev = nullptr;
     bool timing = false;
+    bool school = true;  // small products to the schoolbook kernel (off while tuning)
     hipEvent_t ev[5] = {};
     bool have_times = false;
     // Tile plan per transform length: tuned on the device at first use
@@ -395,9 +396,17 @@ struct BatchPlan {
     std::vector<Group> groups;
     std::vector<BucketLayout> buckets;
     std::vector<Plan> plans;
+    std::vector<size_t> school;  // requests small enough for the schoolbook kernel
+    size_t school_off = 0;
     size_t bytes = 0, arena_words = 0, total_groups = 0, total_blocks = 0;
     int min_log2n = kMaxLog2N + 1, max_log2n = 0;
 };
+
+// Products whose operands both fit the schoolbook kernel run there (unless
+// the tuner is timing transforms).
+bool school_eligible(const Request& r) {
+    return g.school && r.an <= (size_t)kSchoolMaxLimbs && r.bn <= (size_t)kSchoolMaxLimbs;
+}
 
 // Groups, buckets, descriptor layout and resource sizes of a batch.  May tune
 // tile plans for lengths seen for the first time (which waits on the device).
@@ -407,6 +416,10 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
     const std::vector<Group>& groups = bp->groups;
     std::vector<BucketLayout>& buckets = bp->buckets;
     for (size_t i = 0; i < groups.size(); ++i) {
+        if (school_eligible(reqs[i])) {
+            bp->school.push_back(i);
+            continue;
+        }
         auto it = std::find_if(buckets.begin(), buckets.end(),
                                [&](const BucketLayout& b) { return b.log2n == groups[i].log2n; });
         if (it == buckets.end()) {
@@ -477,6 +490,9 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
         bp->min_log2n = std::min(bp->min_log2n, b.log2n);
         bp->max_log2n = std::max(bp->max_log2n, b.log2n);
     }
+    bytes = align_up<SchoolJob>(bytes);
+    bp->school_off = bytes;
+    bytes += bp->school.size() * sizeof(SchoolJob);
     bp->bytes = bytes;
     return PINTT_OK;
 }
@@ -484,6 +500,11 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
 // Write the descriptors of `bp` into H, pointing at the given arena and carry
 // maps.
 void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, uint32_t* maps) {
+    SchoolJob* sj = reinterpret_cast<SchoolJob*>(H + bp.school_off);
+    for (size_t k = 0; k < bp.school.size(); ++k) {
+        const Request& r = bp.groups[bp.school[k]].r;
+        sj[k] = SchoolJob{r.a, r.square ? r.a : r.b, r.out, (uint32_t)r.an, (uint32_t)(r.square ? r.an : r.bn)};
+    }
     size_t maps_next = 0;
     for (const auto& b : bp.buckets) {
         const size_t n = (size_t)1 << b.log2n;
@@ -552,6 +573,7 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, ui
 // The kernel launches of a batch whose descriptors sit at device address D.
 // True when every launch of the plan can carry its jobs in kernel arguments.
 bool fits_inline(const BatchPlan& bp) {
+    if (bp.school.size() > (size_t)kInlineJobs) return false;
     for (const BucketLayout& b : bp.buckets) {
         if (b.nfwd > (uint32_t)kInlineJobs || b.ninv > (uint32_t)kInlineJobs || b.groups.size() > (size_t)kInlineJobs)
             return false;
@@ -571,6 +593,10 @@ int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned
         if (H && n <= (uint32_t)kInlineJobs) return inline_jobs<T>(reinterpret_cast<const T*>(H + off), n);
         return device_jobs<T>(reinterpret_cast<const T*>(D + off));
     };
+    if (!bp.school.empty()) {
+        const uint32_t ns = (uint32_t)bp.school.size();
+        PINTT_HIP(launch_schoolbook(list((SchoolJob*)nullptr, bp.school_off, ns), ns, s));
+    }
     // Grid z carries the job index: launches of more than 65535 jobs go in pieces.
     constexpr uint32_t kMaxJobs = 65535;
     for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
@@ -609,8 +635,10 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
     if (reqs.empty()) return PINTT_OK;
     BatchPlan bp;
     PINTT_TRY(plan_batch(reqs, &bp));
-    PINTT_TRY(ensure_twiddles(bp.min_log2n));
-    PINTT_TRY(ensure_twiddles(bp.max_log2n));
+    if (!bp.buckets.empty()) {
+        PINTT_TRY(ensure_twiddles(bp.min_log2n));
+        PINTT_TRY(ensure_twiddles(bp.max_log2n));
+    }
     PINTT_TRY(ensure_arena(bp.arena_words));
     PINTT_TRY(ensure_carry_state(bp.total_blocks));
     if (fits_inline(bp)) {
@@ -662,8 +690,10 @@ void free_graph(BatchGraph* bg) {
 int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
     BatchPlan bp;
     PINTT_TRY(plan_batch(reqs, &bp));
-    PINTT_TRY(ensure_twiddles(bp.min_log2n));
-    PINTT_TRY(ensure_twiddles(bp.max_log2n));
+    if (!bp.buckets.empty()) {
+        PINTT_TRY(ensure_twiddles(bp.min_log2n));
+        PINTT_TRY(ensure_twiddles(bp.max_log2n));
+    }
     BatchGraph* bg = new BatchGraph;
     auto cleanup = [&](int rc) {
         free_graph(bg);
@@ -716,6 +746,7 @@ int tune(int log2n) {
     const bool saved_timing = g.timing;
     g.split_mode = 0;
     g.timing = false;
+    g.school = false;
     Plan best = make_plan(log2n);
     float best_ms = 1e30f;
     const std::vector<Request> one{Request{g.tune_in, an, g.tune_in + an, bn, g.tune_out, false}};
@@ -745,6 +776,7 @@ int tune(int log2n) {
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     g.split_mode = saved_split;
+    g.school = true;
     g.timing = saved_timing;
     g.plan[log2n] = best;
     g.plan_ms[log2n] = best_ms < 1e29f ? best_ms : 0.f;

@@ -178,8 +178,19 @@ inline JobList<T> inline_jobs(const T* host, uint32_t n) {
         r_;                                                                          \
     })
 
-// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip); counts are
-// host values.
+// A small product of a batch (schoolbook.hip): out (an + bn limbs) = a * b,
+// an, bn <= kSchoolMaxLimbs; a == b squares.
+constexpr int kSchoolThreads = 256;
+constexpr int kSchoolMaxLimbs = 512;
+struct SchoolJob {
+    const uint64_t* a;
+    const uint64_t* b;
+    uint64_t* out;
+    uint32_t an, bn;
+};
+
+// Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip,
+// schoolbook.hip); counts are host values.
 // Tables of 2^kSetMaxLog2N[set] entries for one prime of a set.  pow_scratch:
 // device room for kPowScratch words (powers of the root, used on the stream).
 constexpr int kPowLoBits = 12;
@@ -192,6 +203,7 @@ hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, u
                                   const Twiddles& tw, hipStream_t s);
 hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
+hipError_t launch_schoolbook(const JobList<SchoolJob>& jobs, uint32_t njobs, hipStream_t s);
 hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
                             const CrtConst& cc, hipStream_t s);
 

@@ -246,6 +246,36 @@ def test_batch_mixed_sizes_and_squares(engine, oracle):
         assert np.array_equal(got, oracle.gmp_mul(a, b)), f"batch member {i}"
 
 
+@pytest.mark.parametrize("an,bn", [(1, 1), (1, 512), (512, 1), (2, 300), (255, 256), (511, 512), (512, 512),
+                                   (512, 513), (513, 513)])
+def test_schoolbook_edges(engine, oracle, an, bn):
+    """Products with both operands of at most 512 limbs run on the schoolbook
+    kernel, larger ones on the transforms: both sides of the border, random
+    limbs and all-ones operands (the largest column sums and carries)."""
+    a, b = pc.operands(900 + an + bn, 64 * an, 64 * bn)
+    _check(engine, oracle, a, b, f"{an}x{bn}")
+    ones_a = np.full(an, np.uint64(2**64 - 1))
+    ones_b = np.full(bn, np.uint64(2**64 - 1))
+    _check(engine, oracle, ones_a, ones_b, f"all-ones {an}x{bn}")
+    assert np.array_equal(engine.sqr(ones_a), oracle.gmp_mul(ones_a, ones_a))
+
+
+def test_schoolbook_in_mixed_batches(engine, oracle):
+    """Hundreds of small products (schoolbook) in one batch with transform
+    products, squares among them, through the host and the device entry points."""
+    rng = np.random.default_rng(515)
+    pairs = []
+    for i in range(300):
+        an = int(rng.integers(1, 513))
+        bn = int(rng.integers(1, 513))
+        a, b = pc.operands(7000 + i, 64 * an, 64 * bn)
+        pairs.append((a, a) if i % 4 == 3 else (a, b))
+    pairs += [pc.operands(61, 64 * 5000, 64 * 700), pc.operands(62, 64 * 513, 64 * 2)]
+    outs = engine.mul_batch(pairs)
+    for i, ((a, b), got) in enumerate(zip(pairs, outs)):
+        assert np.array_equal(got, oracle.gmp_mul(a, b)), f"batch member {i} ({a.size}x{b.size})"
+
+
 def test_batch_spans_both_prime_sets(engine, oracle):
     """One batch with transforms of 2^12 .. 2^22 points (narrow primes) and of
     2^23 points (wide primes): both twiddle sets and CRT constant sets in one call."""
