@@ -18,8 +18,10 @@
 #include <gmp.h>
 
 #include <cstddef>
+#include <memory>
 #include <vector>
 
+#include "device_ops.h"
 #include "dispatch.h"
 #include "multiplier.h"
 
@@ -34,6 +36,8 @@ struct BSResult {
     BSResult(BSResult&& o) noexcept;
     BSResult& operator=(BSResult&& o) noexcept;
 };
+
+struct DeviceQR;
 
 class BinarySplitting {
 public:
@@ -50,6 +54,10 @@ public:
     // the top.  Every such product runs on the GPU engine.  Throws
     // std::runtime_error if the device fails (the caller can rerun on the host).
     BSResult compute_device(unsigned long a, unsigned long b, bool need_top_p = true, size_t device_from_limbs = 0);
+
+    // The same without the top P, leaving Q and R in device memory (no copy to
+    // the host): for a final step that also runs on the device.
+    std::unique_ptr<DeviceQR> compute_device_qr(unsigned long a, unsigned long b, size_t device_from_limbs = 0);
 
     // compute_device cuts [a, b) into this many segments of whole leaf spans
     // (at least 64 leaf spans each): segment i+1 is prepared (host subtrees on
@@ -80,6 +88,8 @@ public:
     double upload_seconds() const { return upload_s_; }  // compute_device: host -> device switch
 
 private:
+    template <class F>
+    BSResult run_segments(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs, F&& at_root);
     void merge(BSResult& left, BSResult& right, BSResult& out, bool need_p);
     std::vector<BSResult> leaves(unsigned long a, unsigned long b);
     std::vector<BSResult> leaves(unsigned long a, unsigned long b, unsigned threads, double* seconds) const;
@@ -95,6 +105,15 @@ private:
     double leaf_s_ = 0, leaf_wait_s_ = 0, upload_s_ = 0;
     const Dispatcher* switch_by_ = nullptr;
     bool level_prefers_device(const std::vector<BSResult>& nodes, bool need_top_p) const;
+};
+
+// Q and R of a range kept in device memory (compute_device_qr): magnitudes,
+// exact lengths, the sign of R apart (Q > 0).
+struct DeviceQR {
+    dev::Buffers mem;
+    dev::Num Q, R;
+    int r_sign = 1;
+    void download(BSResult& r) const;  // Q and R into host integers
 };
 
 }  // namespace pi

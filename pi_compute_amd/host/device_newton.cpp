@@ -98,10 +98,10 @@ constexpr size_t kExactLimbs = 16;   // the host mirror's exact start (newton.cp
 constexpr size_t kHostLimbs = 16384;
 
 // Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
-// and the same number on the host (its top limbs start the iteration).
+// and its top limbs on the host (at least min(Ld, kHostLimbs + kGuard + 1)
+// of them, or all): the steps up to kHostLimbs read no others.
 Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B) {
     const size_t Ld = D.n;
-    if (mpz_size(Dh) != Ld) throw std::runtime_error("device reciprocal: host and device operands differ");
     std::vector<size_t> sched;
     for (size_t c = B; c > kExactLimbs; c = (c + 3) / 2) sched.push_back(c);
     // The steps up to kHostLimbs run on the host (newton_reciprocal_limbs to
@@ -114,9 +114,15 @@ Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B) {
     }
     Num Y;
     {
-        mpz_t t;
+        const size_t need = std::min(Ld, c + kGuard);
+        const size_t have = mpz_size(Dh);
+        if (have < need || have > Ld) throw std::runtime_error("device reciprocal: host top limbs do not fit");
+        // The top `need` limbs (the same integer the whole divisor gives the
+        // host steps: each reads at most c + kGuard top limbs).
+        mpz_t top, t;
+        mpz_roinit_n(top, mpz_limbs_read(Dh) + (have - need), (mp_size_t)need);
         mpz_init(t);
-        newton_reciprocal_limbs(t, Dh, c);
+        newton_reciprocal_limbs(t, need == Ld ? Dh : top, c);
         Y = g.upload(t, 0, mpz_size(t));
         mpz_clear(t);
     }
@@ -201,21 +207,19 @@ void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B) {
     mpz_limbs_finish(y, (mp_size_t)Y.n);
 }
 
-void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, const FinalConstants& pre) {
-    if (w % 64 != 0) throw std::runtime_error("device final step: precision not a whole number of limbs");
-    if (mpz_sgn(Q) <= 0 || mpz_sgn(R) <= 0 || mpz_sgn(pre.inv_sqrt) <= 0 || mpz_sgn(pre.pow10) <= 0)
-        throw std::runtime_error("device final step: operands must be positive");
+namespace {
+
+// The final step from Q (its top limbs on the device, q_first limbs below
+// them dropped), R likewise, and the top limbs of that R on the host.
+void final_on_device(Dev& g, Num Qd, size_t q_first, Num Rd, size_t r_first, mpz_srcptr r_top, size_t w,
+                     const FinalConstants& pre, mpz_t out) {
     const size_t W = w / 64 + 2;  // working limbs
-    Dev g(40 * W + 64);
     long ex = 0;  // value = integer * beta^ex, before the 2^w of the square root's scale
     // 426880 * sqrt(10005) = 426880 * 10005 * y / 2^w, y = pre.inv_sqrt
-    const auto q = top_limbs(Q, W), y = top_limbs(pre.inv_sqrt, W), t10 = top_limbs(pre.pow10, W),
-               r = top_limbs(R, W);
-    const Num Qd = g.upload(Q, q.first, q.second);
+    const auto y = top_limbs(pre.inv_sqrt, W), t10 = top_limbs(pre.pow10, W);
     const Num yd = g.upload(pre.inv_sqrt, y.first, y.second);
     const Num pd = g.upload(pre.pow10, t10.first, t10.second);
-    const Num Rd = g.upload(R, r.first, r.second);
-    ex += (long)(q.first + y.first + t10.first);
+    ex += (long)(q_first + y.first + t10.first);
     auto cut = [&](Num x) {  // the top W limbs of x; the exponent follows
         if (x.n <= W) return x;
         ex += (long)(x.n - W);
@@ -231,11 +235,9 @@ void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, co
     Num t = cut(g.mul(Qd, yd));
     t = cut(g.mul(t, pd));
     t = cut(g.mul(t, k));
-    mpz_t rh;
-    mpz_roinit_n(rh, mpz_limbs_read(R) + r.first, (mp_size_t)r.second);
-    const Num inv = reciprocal(g, Rd, rh, W);  // 1/R ~ inv / beta^(r.second + W + r.first)
+    const Num inv = reciprocal(g, Rd, r_top, W);  // 1/R ~ inv / beta^(Rd.n + W + r_first)
     const Num z = g.mul(t, inv);
-    ex -= (long)(r.second + W + r.first) + (long)(w / 64);
+    ex -= (long)(Rd.n + W + r_first) + (long)(w / 64);
     // floor(z * beta^ex): ex < 0 drops limbs.
     if (ex >= 0) throw std::runtime_error("device final step: unexpected scale");
     const size_t drop = (size_t)(-ex);
@@ -249,6 +251,45 @@ void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, co
     size_t nn = n;
     dev::check(gpu_ntt_download(1, &h, &src, &nn), "gpu_ntt_download");
     mpz_limbs_finish(out, (mp_size_t)n);
+}
+
+void check_final_args(size_t w, const FinalConstants& pre) {
+    if (w % 64 != 0) throw std::runtime_error("device final step: precision not a whole number of limbs");
+    if (mpz_sgn(pre.inv_sqrt) <= 0 || mpz_sgn(pre.pow10) <= 0)
+        throw std::runtime_error("device final step: constants must be positive");
+}
+
+}  // namespace
+
+void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, const FinalConstants& pre) {
+    check_final_args(w, pre);
+    if (mpz_sgn(Q) <= 0 || mpz_sgn(R) <= 0) throw std::runtime_error("device final step: operands must be positive");
+    const size_t W = w / 64 + 2;
+    Dev g(40 * W + 64);
+    const auto q = top_limbs(Q, W), r = top_limbs(R, W);
+    const Num Qd = g.upload(Q, q.first, q.second);
+    const Num Rd = g.upload(R, r.first, r.second);
+    mpz_t rh;
+    mpz_roinit_n(rh, mpz_limbs_read(R) + r.first, (mp_size_t)r.second);
+    final_on_device(g, Qd, q.first, Rd, r.first, rh, w, pre, out);
+}
+
+void pi_scaled_newton_device(mpz_t out, const DeviceQR& qr, size_t w, const FinalConstants& pre) {
+    check_final_args(w, pre);
+    if (qr.r_sign < 0) throw std::runtime_error("device final step: R must be positive");
+    const size_t W = w / 64 + 2;
+    Dev g(36 * W + 64);
+    // Views of the top W limbs (Q and R carry exact lengths: top limbs nonzero).
+    const size_t qf = qr.Q.n > W ? qr.Q.n - W : 0, rf = qr.R.n > W ? qr.R.n - W : 0;
+    const Num Qd{qr.Q.d + qf, qr.Q.n - qf};
+    const Num Rd{qr.R.d + rf, qr.R.n - rf};
+    // The top limbs of R that the host steps of the reciprocal read.
+    const size_t k = std::min(Rd.n, kHostLimbs + kGuard + 1);
+    const std::vector<uint64_t> top = g.fetch(Rd, Rd.n - k, k);
+    if (top.back() == 0) throw std::runtime_error("device final step: R has a zero top limb");
+    mpz_t rh;
+    mpz_roinit_n(rh, reinterpret_cast<const mp_limb_t*>(top.data()), (mp_size_t)k);
+    final_on_device(g, Qd, qf, Rd, rf, rh, w, pre, out);
 }
 
 }  // namespace pi

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "binary_splitting.h"
+#include "device_ops.h"
 #include "pintt_gpu.h"
 
 namespace pi {
@@ -265,7 +266,12 @@ void DeviceTree::reduce(bool need_top_p, std::vector<BinarySplitting::Level>& le
 
 }  // namespace
 
-BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs) {
+// The segment pipeline shared by compute_device and compute_device_qr:
+// at_root(root node) runs while every device tree is alive and gives the
+// result; a range reduced on the host alone is returned as it is.
+template <class F>
+BSResult BinarySplitting::run_segments(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs,
+                                       F&& at_root) {
     using Clock = std::chrono::steady_clock;
     levels_.clear();
     leaf_s_ = leaf_wait_s_ = upload_s_ = 0;
@@ -337,12 +343,81 @@ BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool 
             leaf_wait_s_ += std::chrono::duration<double>(Clock::now() - tw).count();
         }
     }
-    if (nseg == 1) return trees[0]->download(need_top_p);
+    if (nseg == 1) return at_root(*trees[0]);
     // The segment roots merge on the device.
     DeviceTree top;
     top.adopt(std::move(roots));
     top.reduce(need_top_p, levels_);
-    return top.download(need_top_p);
+    return at_root(top);
+}
+
+BSResult BinarySplitting::compute_device(unsigned long a, unsigned long b, bool need_top_p, size_t device_from_limbs) {
+    return run_segments(a, b, need_top_p, device_from_limbs,
+                        [need_top_p](const DeviceTree& t) { return t.download(need_top_p); });
+}
+
+void DeviceQR::download(BSResult& r) const {
+    auto get = [](const dev::Num& v, int sign, mpz_ptr x) {
+        uint64_t* p = reinterpret_cast<uint64_t*>(mpz_limbs_write(x, (mp_size_t)v.n));
+        const uint64_t* src = v.d;
+        size_t n = v.n;
+        check(gpu_ntt_download(1, &p, &src, &n), "gpu_ntt_download");
+        while (n && !p[n - 1]) --n;
+        mpz_limbs_finish(x, (mp_size_t)n);
+        if (sign < 0) mpz_neg(x, x);
+    };
+    get(Q, 1, r.Q);
+    get(R, r_sign, r.R);
+}
+
+std::unique_ptr<DeviceQR> BinarySplitting::compute_device_qr(unsigned long a, unsigned long b,
+                                                             size_t device_from_limbs) {
+    auto out = std::make_unique<DeviceQR>();
+    // Q and R of the root into buffers of their own (their exact lengths), so
+    // the trees can go.
+    auto keep = [&out](const DevVal& v, dev::Num& dst) {
+        // The length is an upper bound: drop zero top limbs, reading the top
+        // in windows of 64 limbs.
+        size_t n = v.n;
+        for (bool more = true; more && n > 1;) {
+            const size_t k = std::min<size_t>(n, 64);
+            std::vector<uint64_t> h(k);
+            uint64_t* hp = h.data();
+            const uint64_t* src = v.d + (n - k);
+            size_t kk = k;
+            check(gpu_ntt_download(1, &hp, &src, &kk), "gpu_ntt_download");
+            size_t i = k;
+            while (i > 0 && h[i - 1] == 0 && n > 1) {
+                --i;
+                --n;
+            }
+            more = i == 0;  // the whole window was zero
+        }
+        dst.n = n;
+        dst.d = out->mem.alloc(n);
+        check(gpu_ntt_dev_copy(dst.d, v.d, n), "gpu_ntt_dev_copy");
+    };
+    BSResult host = run_segments(a, b, false, device_from_limbs, [&](const DeviceTree& t) {
+        keep(t.root().Q, out->Q);
+        keep(t.root().R, out->R);
+        out->r_sign = t.root().R.sign;
+        check(gpu_ntt_synchronize(), "gpu_ntt_synchronize");
+        return BSResult{};
+    });
+    if (!out->Q.d) {
+        // The range was reduced on the host alone: upload its Q and R.
+        auto put = [&out](mpz_srcptr x, dev::Num& dst) {
+            dst.n = std::max<size_t>(mpz_size(x), 1);
+            dst.d = out->mem.alloc(dst.n);
+            const uint64_t zero = 0;
+            const uint64_t* src = mpz_size(x) ? reinterpret_cast<const uint64_t*>(mpz_limbs_read(x)) : &zero;
+            check(gpu_ntt_upload(1, &dst.d, &src, &dst.n), "gpu_ntt_upload");
+        };
+        put(host.Q, out->Q);
+        put(host.R, out->R);
+        out->r_sign = mpz_sgn(host.R) < 0 ? -1 : 1;
+    }
+    return out;
 }
 
 bool BinarySplitting::level_prefers_device(const std::vector<BSResult>& nodes, bool need_top_p) const {

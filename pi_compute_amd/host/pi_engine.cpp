@@ -96,12 +96,17 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     bs.set_segments(cfg.bs_segments);
     bs.set_leaf_span(cfg.bs_leaf_span);
     BSResult r;
+    std::unique_ptr<DeviceQR> qr;  // Q and R left on the device for the device final step
     if (cfg.level_batches && cfg.device_levels) {
         try {
-            r = bs.compute_device(0, terms, false, cfg.device_from_limbs);
+            if (final_device)
+                qr = bs.compute_device_qr(0, terms, cfg.device_from_limbs);
+            else
+                r = bs.compute_device(0, terms, false, cfg.device_from_limbs);
         } catch (const std::exception& e) {
             // Device trouble: the same tree on the host path, exact either way.
             std::cerr << "  device-resident levels failed (" << e.what() << "); host levels instead" << std::endl;
+            qr.reset();
             r = bs.compute(0, terms, false);
         }
     } else {
@@ -127,7 +132,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     mpz_init(pi_int);
     NewtonStats ns;
     bool fin_on_device = false;
-    if (newton && mpz_sgn(r.R) > 0) {
+    if (newton && (qr ? qr->r_sign > 0 : mpz_sgn(r.R) > 0)) {
         if (final_device) {
             try {
                 const FinalConstants* c = pre.get();
@@ -136,16 +141,25 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
                     final_constants(own, cfg.digits, fin_w, mult_);
                     c = &own;
                 }
-                pi_scaled_newton_device(pi_int, r.Q, r.R, fin_w, *c);
+                if (qr)
+                    pi_scaled_newton_device(pi_int, *qr, fin_w, *c);
+                else
+                    pi_scaled_newton_device(pi_int, r.Q, r.R, fin_w, *c);
                 fin_on_device = true;
             } catch (const std::exception& e) {
                 std::cerr << "  device final step failed (" << e.what() << "); host final step instead" << std::endl;
             }
         }
-        if (!fin_on_device) pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, fin_w, mult_, &ns, pre.get());
+        if (!fin_on_device) {
+            if (qr) qr->download(r);
+            pi_scaled_newton(pi_int, r.Q, r.R, cfg.digits, fin_w, mult_, &ns, pre.get());
+        }
+        qr.reset();
         mpz_realloc2(r.Q, 0);
         mpz_realloc2(r.R, 0);
     } else {
+        if (qr) qr->download(r);
+        qr.reset();
         mpf_t s, num, den, x;
         mpf_init2(s, bits);
         mpf_init2(num, bits);

@@ -87,6 +87,9 @@ void pi_scaled_newton(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t digits, size
 // multiple of 64 and pre computed with it.  Throws std::runtime_error if the
 // device fails or a check on the way does.
 void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, const FinalConstants& pre);
+// The same from Q and R already in device memory (binary_splitting.h,
+// compute_device_qr): nothing of them crosses PCIe.
+void pi_scaled_newton_device(mpz_t out, const DeviceQR& qr, size_t w, const FinalConstants& pre);
 
 class PiEngine {
 public:

@@ -215,6 +215,11 @@ int main() {
                 pi::BSResult dq = bs.compute_device(0, b, false, from);
                 check(eq(dv, w) && mpz_cmp(dq.Q, w.Q) == 0 && mpz_cmp(dq.R, w.R) == 0,
                       "DeviceResidentLevels b=" + std::to_string(b) + " from=" + std::to_string(from));
+                // Q and R left on the device, then read back.
+                pi::BSResult kq;
+                bs.compute_device_qr(0, b, from)->download(kq);
+                check(mpz_cmp(kq.Q, w.Q) == 0 && mpz_cmp(kq.R, w.R) == 0,
+                      "DeviceResidentQR b=" + std::to_string(b) + " from=" + std::to_string(from));
             }
         }
     }
@@ -320,17 +325,19 @@ int main() {
             const size_t w = (bits + 64 + 63) / 64 * 64;
             pi::FinalConstants c;
             pi::final_constants(c, digits, w, gpu);
-            mpz_t hd, hh;
-            mpz_inits(hd, hh, NULL);
+            mpz_t hd, hh, hq;
+            mpz_inits(hd, hh, hq, NULL);
             try {
                 pi::pi_scaled_newton_device(hd, bs.Q, bs.R, w, c);
                 pi::pi_scaled_newton(hh, bs.Q, bs.R, digits, w, gpu, nullptr, &c);
-                ok = ok && mpz_cmp(hd, hh) == 0;
+                pi::BinarySplitting bsd(gpu, 4);
+                pi::pi_scaled_newton_device(hq, *bsd.compute_device_qr(0, terms), w, c);
+                ok = ok && mpz_cmp(hd, hh) == 0 && mpz_cmp(hq, hh) == 0;
             } catch (const std::exception& ex) {
                 ok = false;
                 err = ex.what();
             }
-            mpz_clears(hd, hh, NULL);
+            mpz_clears(hd, hh, hq, NULL);
         }
         check(ok, "DeviceFinalStepMatchesHost" + (err.empty() ? std::string() : " (" + err + ")"));
     }
