@@ -21,7 +21,8 @@
 //                             device-resident conversion levels: device leaves of at most 4096)
 //   --bs-segments S           device levels: S segments, host subtrees of the next one computed
 //                             while the device reduces the current one (8; 1 = one tree)
-//   --bs-leaf T               terms per host subtree at the bottom of the tree (256)
+//   --bs-leaf T               terms per host subtree at the bottom of the tree (128 with
+//                             device levels, else 256)
 //   --no-overlap              Newton final step: compute sqrt(10005) and 10^D after binary
 //                             splitting instead of beside it
 //   --host-final-step         with --device-levels: the Newton final step on host numbers

@@ -94,7 +94,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     BinarySplitting bs(mult_, cfg.num_threads);
     bs.set_switch_dispatcher(cfg.device_switch);
     bs.set_segments(cfg.bs_segments);
-    bs.set_leaf_span(cfg.bs_leaf_span);
+    bs.set_leaf_span(cfg.bs_leaf_span ? cfg.bs_leaf_span : (cfg.level_batches && cfg.device_levels ? 128 : 256));
     BSResult r;
     std::unique_ptr<DeviceQR> qr;  // Q and R left on the device for the device final step
     if (cfg.level_batches && cfg.device_levels) {

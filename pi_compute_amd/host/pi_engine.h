@@ -39,7 +39,9 @@ struct PiConfig {
     // conversion levels: at most 4096, the widest leaf the device converts)
     size_t conversion_leaf_digits = 1 << 15;
     unsigned bs_segments = 8;                  // device levels: segments pipelined with their host subtrees
-    unsigned long bs_leaf_span = 256;          // terms per host subtree at the bottom of the tree
+    // Terms per host subtree at the bottom of the tree; 0: 128 with device-resident
+    // levels (the device's small-product kernel takes the next level), else 256.
+    unsigned long bs_leaf_span = 0;
     // sqrt(10005) and 10^D (FinalStep::Newton) and the conversion's powers of
     // ten and their reciprocals (Conversion::Multiplier) depend only on D, so
     // they are computed on another thread while binary splitting runs.
