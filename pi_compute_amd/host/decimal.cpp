@@ -294,8 +294,8 @@ struct DevSplitNode {
 // and an increment of the quotients that were one short.  The leaves (at most
 // PINTT_DECIMAL_LEAF_MAX_DIGITS digits: leaf_digits must not be larger) are
 // written as digits by the device and come back in one copy.
-void device_convert(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t leaf_digits, std::string& out,
-                    DecimalStats& st) {
+void device_convert(const DevNum* xdev, mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t leaf_digits,
+                    std::string& out, DecimalStats& st) {
     using Clock = std::chrono::steady_clock;
     auto t = Clock::now();
     auto lap = [&t]() {
@@ -309,13 +309,16 @@ void device_convert(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t
     const int k = (int)pre.pw.size() - 1;
     int jmin = k + 1;
     while (jmin > 0 && ((size_t)1 << (jmin - 1)) >= leaf_digits) --jmin;
-    // Powers, reciprocals, the constant 1 (an all-zero limb follows it) and |n|.
+    // Powers and reciprocals (uploaded here unless the caller's copies are on
+    // the device already), the constant 1 (an all-zero limb follows it) and n.
+    std::vector<DevNum> P(k + 1), inv(k + 1);
+    const bool have = pre.dev && pre.dev->jmin <= jmin && (int)pre.dev->P.size() == k + 1;
     size_t total = 2;
-    for (int j = jmin; j <= k; ++j) total += mpz_size(&pre.pw[j]) + mpz_size(&pre.inv[j]);
-    const size_t nn = std::max<size_t>(mpz_size(n), 1);
+    if (!have)
+        for (int j = jmin; j <= k; ++j) total += mpz_size(&pre.pw[j]) + mpz_size(&pre.inv[j]);
+    const size_t nn = xdev ? 0 : std::max<size_t>(mpz_size(n), 1);
     total += nn;
     uint64_t* base = mem.alloc(total);
-    std::vector<DevNum> P(k + 1), inv(k + 1);
     std::vector<mpz_srcptr> src;
     std::vector<DevNum> dst;
     size_t used = 0;
@@ -325,6 +328,11 @@ void device_convert(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t
         return v;
     };
     for (int j = jmin; j <= k; ++j) {
+        if (have) {
+            P[j] = pre.dev->P[j];
+            inv[j] = pre.dev->inv[j];
+            continue;
+        }
         P[j] = carve(mpz_size(&pre.pw[j]));
         inv[j] = carve(mpz_size(&pre.inv[j]));
         src.push_back(&pre.pw[j]);
@@ -333,16 +341,20 @@ void device_convert(mpz_srcptr n, size_t width, const DecimalPowers& pre, size_t
         dst.push_back(inv[j]);
     }
     DevNum one = carve(2);
-    DevNum root = carve(nn);
+    DevNum root = xdev ? *xdev : carve(nn);
     {
         const uint64_t o[2] = {1, 0};
         const uint64_t* op = o;
         size_t on = 2;
         dcheck(gpu_ntt_upload(1, &one.d, &op, &on), "gpu_ntt_upload");
     }
-    src.push_back(n);
-    dst.push_back(root);
-    upload(src, dst);
+    if (!xdev) {
+        src.push_back(n);
+        dst.push_back(root);
+    }
+    if (!src.empty()) upload(src, dst);
+    // The root's length: at most the limbs its width allows.
+    root.n = std::min(root.n, limbs_for_digits(width));
     DevNum zero{one.d + 1, 1};
 
     std::vector<DevSplitNode> nodes{{root, 0, width}};
@@ -497,6 +509,29 @@ DecimalPowers::~DecimalPowers() {
     for (auto& x : inv) mpz_clear(&x);
 }
 
+DevicePowers::DevicePowers(const DecimalPowers& d, int jmin_) : jmin(jmin_) {
+    const int k = (int)d.pw.size() - 1;
+    P.resize(k + 1);
+    inv.resize(k + 1);
+    size_t total = 0;
+    for (int j = jmin; j <= k; ++j) total += mpz_size(&d.pw[j]) + mpz_size(&d.inv[j]);
+    uint64_t* base = mem.alloc(total);
+    std::vector<mpz_srcptr> src;
+    std::vector<DevNum> dst;
+    size_t used = 0;
+    for (int j = jmin; j <= k; ++j) {
+        P[j] = DevNum{base + used, mpz_size(&d.pw[j])};
+        used += P[j].n;
+        inv[j] = DevNum{base + used, mpz_size(&d.inv[j])};
+        used += inv[j].n;
+        src.push_back(&d.pw[j]);
+        dst.push_back(P[j]);
+        src.push_back(&d.inv[j]);
+        dst.push_back(inv[j]);
+    }
+    upload(src, dst);
+}
+
 void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits, bool device) {
     using Clock = std::chrono::steady_clock;
     auto t = Clock::now();
@@ -526,6 +561,11 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
         else
             newton_reciprocal(&d.inv[j], &d.pw[j], e2 - np, m);  // ~ 2^e2 / P, within 4
         mpz_sub_ui(&d.inv[j], &d.inv[j], 5);                     // below 2^e2 / P
+    }
+    if (device) {
+        int jmin = k + 1;
+        while (jmin > 0 && ((size_t)1 << (jmin - 1)) >= leaf_digits) --jmin;
+        d.dev = std::make_unique<DevicePowers>(d, jmin);
     }
     d.reciprocals_s = std::chrono::duration<double>(Clock::now() - t).count();
 }
@@ -561,7 +601,7 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
     st.reciprocals_s = pre->reciprocals_s;
     lap(t);
     if (device) {
-        device_convert(n, width, *pre, leaf_digits, out, st);
+        device_convert(nullptr, n, width, *pre, leaf_digits, out, st);
         size_t z = out.find_first_not_of('0');
         if (z != 0) out.erase(0, z);
         if (stats) *stats = st;
@@ -596,6 +636,62 @@ std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads, size_t lea
     size_t z = out.find_first_not_of('0');
     if (z != 0) out.erase(0, z);
     st.leaves_s += lap(t);
+    if (stats) *stats = st;
+    return out;
+}
+
+std::string to_decimal_device(const uint64_t* x, size_t xn, unsigned threads, size_t leaf_digits,
+                              DecimalStats* stats, const DecimalPowers& pre) {
+    DecimalStats st;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    leaf_digits = std::min<size_t>(std::max<size_t>(leaf_digits, kLeafDigits), PINTT_DECIMAL_LEAF_MAX_DIGITS);
+    // The true length and top limb, from the top in windows of 64 limbs.
+    size_t n = xn;
+    uint64_t top = 0;
+    while (n > 0 && !top) {
+        const size_t k = std::min<size_t>(n, 64);
+        std::vector<uint64_t> h(k);
+        uint64_t* hp = h.data();
+        const uint64_t* src = x + (n - k);
+        size_t kk = k;
+        dcheck(gpu_ntt_download(1, &hp, &src, &kk), "gpu_ntt_download");
+        size_t i = k;
+        while (i > 0 && h[i - 1] == 0) --i;
+        if (i) {
+            n -= k - i;
+            top = h[i - 1];
+        } else {
+            n -= k;
+        }
+    }
+    if (n == 0) return "0";
+    size_t bits = 64 * (n - 1);
+    for (uint64_t t = top; t; t >>= 1) ++bits;
+    // x < 2^bits, so its digits are at most floor(bits log10(2)) + 1.
+    const size_t width = (size_t)((double)bits * 0.30102999566398120) + 1;
+    if (width <= 2 * leaf_digits) {
+        mpz_t h;
+        mpz_init(h);
+        uint64_t* hp = reinterpret_cast<uint64_t*>(mpz_limbs_write(h, (mp_size_t)n));
+        size_t nn = n;
+        dcheck(gpu_ntt_download(1, &hp, &x, &nn), "gpu_ntt_download");
+        mpz_limbs_finish(h, (mp_size_t)n);
+        std::string s = to_decimal(h, threads);
+        mpz_clear(h);
+        return s;
+    }
+    int kw = 0;
+    while (((size_t)2 << kw) < width) ++kw;
+    if ((int)pre.pw.size() != kw + 1 || pre.leaf_digits != leaf_digits)
+        throw std::runtime_error("device conversion: powers built for another width or leaf");
+    st.powers_s = pre.powers_s;
+    st.reciprocals_s = pre.reciprocals_s;
+    std::string out;
+    const DevNum root{const_cast<uint64_t*>(x), n};
+    device_convert(&root, nullptr, width, pre, leaf_digits, out, st);
+    size_t z = out.find_first_not_of('0');
+    if (z == std::string::npos) return "0";
+    if (z != 0) out.erase(0, z);
     if (stats) *stats = st;
     return out;
 }

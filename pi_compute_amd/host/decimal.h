@@ -11,9 +11,11 @@
 #include <gmp.h>
 
 #include <cstddef>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "device_ops.h"
 #include "multiplier.h"
 
 namespace pi {
@@ -35,6 +37,17 @@ struct DecimalStats {
 // number of `width` digits uses: they depend only on width and leaf_digits, so
 // a caller that knows the width early can build them beforehand (for pi,
 // beside binary splitting).
+struct DecimalPowers;
+
+// Device copies of the powers 10^(2^j) and reciprocals with 2^j >= the leaf
+// width (j >= jmin), made once by decimal_powers(..., device = true).
+struct DevicePowers {
+    dev::Buffers mem;
+    std::vector<dev::Num> P, inv;  // indexed by j
+    int jmin;
+    DevicePowers(const DecimalPowers& d, int jmin);
+};
+
 struct DecimalPowers {
     size_t width = 0, leaf_digits = 0;
     std::vector<__mpz_struct> pw;   // 10^(2^j), j <= k with 2^k < width <= 2^(k+1)
@@ -42,13 +55,15 @@ struct DecimalPowers {
     // whole limbs, where 2^j >= leaf_digits; else 0
     std::vector<__mpz_struct> inv;
     double powers_s = 0, reciprocals_s = 0;
+    std::unique_ptr<DevicePowers> dev;  // device copies (device = true)
     DecimalPowers() = default;
     ~DecimalPowers();
     DecimalPowers(const DecimalPowers&) = delete;
     DecimalPowers& operator=(const DecimalPowers&) = delete;
 };
 // device: the reciprocals as whole-limb Newton steps on device numbers
-// (newton.h, device_reciprocal_limbs); the powers go through m either way.
+// (newton.h, device_reciprocal_limbs), and device copies of what the device
+// conversion reads; the powers go through m either way.
 void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits = 1 << 15,
                     bool device = false);
 
@@ -60,5 +75,13 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
 // leaves come back.  Throws std::runtime_error if the device fails.
 std::string to_decimal(mpz_srcptr n, Multiplier& m, unsigned threads = 0, size_t leaf_digits = 1 << 15,
                        DecimalStats* stats = nullptr, const DecimalPowers* pre = nullptr, bool device = false);
+
+// The same digits from a non-negative number already in device memory (xn
+// limbs at x, top limbs may be zero), every level and the leaves on the
+// device: nothing of it crosses PCIe but the digits.  pre must hold the
+// powers for its width and leaf_digits (at most 4,096).  Throws
+// std::runtime_error if they do not or the device fails.
+std::string to_decimal_device(const uint64_t* x, size_t xn, unsigned threads, size_t leaf_digits,
+                              DecimalStats* stats, const DecimalPowers& pre);
 
 }  // namespace pi

@@ -212,7 +212,7 @@ namespace {
 // The final step from Q (its top limbs on the device, q_first limbs below
 // them dropped), R likewise, and the top limbs of that R on the host.
 void final_on_device(Dev& g, Num Qd, size_t q_first, Num Rd, size_t r_first, mpz_srcptr r_top, size_t w,
-                     const FinalConstants& pre, mpz_t out) {
+                     const FinalConstants& pre, mpz_ptr out, DeviceInt* dout) {
     const size_t W = w / 64 + 2;  // working limbs
     long ex = 0;  // value = integer * beta^ex, before the 2^w of the square root's scale
     // 426880 * sqrt(10005) = 426880 * 10005 * y / 2^w, y = pre.inv_sqrt
@@ -241,11 +241,15 @@ void final_on_device(Dev& g, Num Qd, size_t q_first, Num Rd, size_t r_first, mpz
     // floor(z * beta^ex): ex < 0 drops limbs.
     if (ex >= 0) throw std::runtime_error("device final step: unexpected scale");
     const size_t drop = (size_t)(-ex);
-    if (drop >= z.n) {
-        mpz_set_ui(out, 0);
+    if (drop >= z.n) throw std::runtime_error("device final step: empty result");
+    const size_t n = z.n - drop;
+    if (dout) {  // into a buffer of its own (the arena goes with g)
+        dout->x.n = n;
+        dout->x.d = dout->mem.alloc(n);
+        dev::check(gpu_ntt_dev_copy(dout->x.d, z.d + drop, n), "gpu_ntt_dev_copy");
+        dev::check(gpu_ntt_synchronize(), "gpu_ntt_synchronize");
         return;
     }
-    const size_t n = z.n - drop;
     uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(out, (mp_size_t)n));
     const uint64_t* src = z.d + drop;
     size_t nn = n;
@@ -271,10 +275,11 @@ void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, co
     const Num Rd = g.upload(R, r.first, r.second);
     mpz_t rh;
     mpz_roinit_n(rh, mpz_limbs_read(R) + r.first, (mp_size_t)r.second);
-    final_on_device(g, Qd, q.first, Rd, r.first, rh, w, pre, out);
+    final_on_device(g, Qd, q.first, Rd, r.first, rh, w, pre, out, nullptr);
 }
 
-void pi_scaled_newton_device(mpz_t out, const DeviceQR& qr, size_t w, const FinalConstants& pre) {
+namespace {
+void final_from_qr(const DeviceQR& qr, size_t w, const FinalConstants& pre, mpz_ptr out, DeviceInt* dout) {
     check_final_args(w, pre);
     if (qr.r_sign < 0) throw std::runtime_error("device final step: R must be positive");
     const size_t W = w / 64 + 2;
@@ -289,7 +294,16 @@ void pi_scaled_newton_device(mpz_t out, const DeviceQR& qr, size_t w, const Fina
     if (top.back() == 0) throw std::runtime_error("device final step: R has a zero top limb");
     mpz_t rh;
     mpz_roinit_n(rh, reinterpret_cast<const mp_limb_t*>(top.data()), (mp_size_t)k);
-    final_on_device(g, Qd, qf, Rd, rf, rh, w, pre, out);
+    final_on_device(g, Qd, qf, Rd, rf, rh, w, pre, out, dout);
+}
+}  // namespace
+
+void pi_scaled_newton_device(mpz_t out, const DeviceQR& qr, size_t w, const FinalConstants& pre) {
+    final_from_qr(qr, w, pre, out, nullptr);
+}
+
+void pi_scaled_newton_device(DeviceInt& out, const DeviceQR& qr, size_t w, const FinalConstants& pre) {
+    final_from_qr(qr, w, pre, nullptr, &out);
 }
 
 }  // namespace pi

@@ -132,6 +132,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     mpz_init(pi_int);
     NewtonStats ns;
     bool fin_on_device = false;
+    std::unique_ptr<DeviceInt> pi_dev;  // the final step's result left on the device for the conversion
     if (newton && (qr ? qr->r_sign > 0 : mpz_sgn(r.R) > 0)) {
         if (final_device) {
             try {
@@ -141,12 +142,17 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
                     final_constants(own, cfg.digits, fin_w, mult_);
                     c = &own;
                 }
-                if (qr)
+                if (qr && conv_device) {
+                    pi_dev = std::make_unique<DeviceInt>();
+                    pi_scaled_newton_device(*pi_dev, *qr, fin_w, *c);
+                } else if (qr) {
                     pi_scaled_newton_device(pi_int, *qr, fin_w, *c);
-                else
+                } else {
                     pi_scaled_newton_device(pi_int, r.Q, r.R, fin_w, *c);
+                }
                 fin_on_device = true;
             } catch (const std::exception& e) {
+                pi_dev.reset();
                 std::cerr << "  device final step failed (" << e.what() << "); host final step instead" << std::endl;
             }
         }
@@ -195,12 +201,32 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     std::string d;
     if (conv_device) {
         try {
-            d = to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get(), true);
+            if (pi_dev) {
+                DecimalPowers own;
+                const DecimalPowers* dp = dpow.get();
+                if (!dp) {
+                    decimal_powers(own, cfg.digits + 1, mult_, conv_leaf, true);
+                    dp = &own;
+                }
+                d = to_decimal_device(pi_dev->x.d, pi_dev->x.n, cfg.num_threads, conv_leaf, &ds, *dp);
+            } else {
+                d = to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get(), true);
+            }
         } catch (const std::exception& e) {
             std::cerr << "  device conversion failed (" << e.what() << "); host levels instead" << std::endl;
+            if (pi_dev) {  // the value back to the host first
+                const size_t n = pi_dev->x.n;
+                uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(pi_int, (mp_size_t)n));
+                const uint64_t* src = pi_dev->x.d;
+                size_t nn = n;
+                if (gpu_ntt_download(1, &h, &src, &nn) != PINTT_OK) throw;
+                mpz_limbs_finish(pi_int, (mp_size_t)n);
+                pi_dev.reset();
+            }
             ds = DecimalStats{};
             d = to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get());
         }
+        pi_dev.reset();
     } else {
         d = conv_mult ? to_decimal(pi_int, mult_, cfg.num_threads, conv_leaf, &ds, dpow.get())
                       : to_decimal(pi_int, cfg.num_threads);

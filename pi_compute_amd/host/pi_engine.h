@@ -92,6 +92,12 @@ void pi_scaled_newton_device(mpz_t out, mpz_srcptr Q, mpz_srcptr R, size_t w, co
 // The same from Q and R already in device memory (binary_splitting.h,
 // compute_device_qr): nothing of them crosses PCIe.
 void pi_scaled_newton_device(mpz_t out, const DeviceQR& qr, size_t w, const FinalConstants& pre);
+// ... and the result left in device memory too (for to_decimal_device).
+struct DeviceInt {
+    dev::Buffers mem;
+    dev::Num x;
+};
+void pi_scaled_newton_device(DeviceInt& out, const DeviceQR& qr, size_t w, const FinalConstants& pre);
 
 class PiEngine {
 public:

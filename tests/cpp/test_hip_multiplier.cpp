@@ -270,8 +270,37 @@ int main() {
                 if (d > 0) mpz_add_ui(x, x, 1);
                 conv(2);
             }
+        // From a number already in device memory (zero top limbs included),
+        // with powers built beforehand for its width, and below the level
+        // threshold (converted on the host).
+        bool dev_ok = true;
+        for (size_t nb : {3000ul, 100000ul, 654321ul, 3000000ul}) {
+            pi::draw_operand(x, g, nb);
+            char* s = mpz_get_str(nullptr, 10, x);
+            const size_t w = std::strlen(s);
+            try {
+                pi::DecimalPowers pw;
+                pi::decimal_powers(pw, w, gpu, 4096, true);
+                const size_t n = mpz_size(x) + 5;
+                uint64_t* d = nullptr;
+                if (gpu_ntt_dev_alloc(n, &d) != 0) throw std::runtime_error("alloc");
+                std::vector<uint64_t> h(n, 0);
+                std::memcpy(h.data(), mpz_limbs_read(x), mpz_size(x) * 8);
+                const uint64_t* hp = h.data();
+                size_t nn = n;
+                if (gpu_ntt_upload(1, &d, &hp, &nn) != 0) throw std::runtime_error("upload");
+                const std::string got = pi::to_decimal_device(d, n, 4, 4096, nullptr, pw);
+                (void)gpu_ntt_dev_free(d);
+                dev_ok = dev_ok && got == s;
+            } catch (const std::exception& ex) {
+                dev_ok = false;
+                err = ex.what();
+            }
+            std::free(s);
+        }
         mpz_clears(x, p, NULL);
         check(ok, "DecimalDeviceLevelsMatchMpzGetStr" + (err.empty() ? std::string() : " (" + err + ")"));
+        check(dev_ok, "DecimalFromDeviceNumberMatchesMpzGetStr" + (err.empty() ? std::string() : " (" + err + ")"));
         check(est_ok, "DecimalDeviceLevelsEstimateWithinOne");
     }
 
