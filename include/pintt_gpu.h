@@ -113,6 +113,15 @@ int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs);  /* 
 int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const uint64_t* const* d_y,
                              const size_t* yn, const int* subtract, uint64_t* const* d_out);
 
+/* Same batch enqueued without waiting (ordered with products on the library
+ * stream): a subtraction with x < y sets the device word *d_negative to 1
+ * (it is never cleared here; the caller zeroes and reads it) and leaves that
+ * output undefined.  Lets a caller enqueue a whole stretch of binary-splitting
+ * levels and check once.  No reference counterpart. */
+int gpu_ntt_add_batch_device_async(size_t count, const uint64_t* const* d_x, const size_t* xn,
+                                   const uint64_t* const* d_y, const size_t* yn, const int* subtract,
+                                   uint64_t* const* d_out, uint32_t* d_negative);
+
 /* Same, but a subtraction with x < y is no error: negative[i] is set to 1
  * and out[i] holds x - y + 2^(64 w), w = max(xn[i], yn[i]); 0 otherwise.
  * Lets a caller try "x - y if not negative" on a whole batch (the remainder

@@ -99,6 +99,11 @@ ABI = {
         [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p)],
     ),
+    "gpu_ntt_add_batch_device_async": (
+        ctypes.c_int,
+        [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
+         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p), ctypes.c_void_p],
+    ),
     "gpu_ntt_add_batch_device_flags": (
         ctypes.c_int,
         [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
@@ -413,11 +418,13 @@ def decimal_leaves(leaves) -> bytes:
             lib.gpu_ntt_dev_free(p)
 
 
-def add_batch(pairs, subtract, flags=False):
+def add_batch(pairs, subtract, flags=False, enqueue=False):
     """Device add/subtract of magnitudes: [(x, y)] numpy uint64 limb arrays,
     subtract[i] -> x - y (needs x >= y).  Uploads, runs one batch, downloads.
     flags=True: x < y is allowed (the result wraps modulo 2^(64 w)) and the
-    call returns (outputs, negative flags)."""
+    call returns (outputs, negative flags).  enqueue=True: the batch goes
+    through the enqueue-only entry point and the call returns (outputs,
+    whether any subtraction went negative)."""
     lib = load_library()
     n = len(pairs)
     xs = [np.ascontiguousarray(x, np.uint64) for x, _ in pairs]
@@ -439,6 +446,18 @@ def add_batch(pairs, subtract, flags=False):
         _check(lib, lib.gpu_ntt_upload(n, arr(dy), arr([_ptr(y) for y in ys]), sz([y.size for y in ys])))
         subs = (ctypes.c_int * n)(*[1 if s else 0 for s in subtract])
         neg = (ctypes.c_int * n)()
+        if enqueue:
+            flag = alloc(1)
+            zero = np.zeros(1, np.uint64)
+            _check(lib, lib.gpu_ntt_upload(1, arr([flag]), arr([_ptr(zero)]), sz([1])))
+            _check(lib, lib.gpu_ntt_add_batch_device_async(n, arr(dx), sz([x.size for x in xs]), arr(dy),
+                                                           sz([y.size for y in ys]), subs, arr(do),
+                                                           ctypes.cast(flag, ctypes.c_void_p)))
+            got = np.zeros(1, np.uint64)
+            _check(lib, lib.gpu_ntt_download(1, arr([_ptr(got)]), arr([flag]), sz([1])))
+            outs = [np.zeros(w, np.uint64) for w in widths]
+            _check(lib, lib.gpu_ntt_download(n, arr([_ptr(o) for o in outs]), arr(do), sz(widths)))
+            return outs, bool(got[0])
         if flags:
             _check(lib, lib.gpu_ntt_add_batch_device_flags(n, arr(dx), sz([x.size for x in xs]), arr(dy),
                                                            sz([y.size for y in ys]), subs, arr(do), neg))

@@ -1307,6 +1307,71 @@ int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const siz
     return add_batch(count, d_x, xn, d_y, yn, subtract, d_out, nullptr);
 }
 
+int gpu_ntt_add_batch_device_async(size_t count, const uint64_t* const* d_x, const size_t* xn,
+                                   const uint64_t* const* d_y, const size_t* yn, const int* subtract,
+                                   uint64_t* const* d_out, uint32_t* d_negative) {
+    if (count == 0) return PINTT_OK;
+    if (!d_x || !xn || !d_y || !yn || !subtract || !d_out || !d_negative) return fail(PINTT_ERR_ARGS, "null pointer");
+    std::vector<AddJob> jobs(count);
+    size_t blocks = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (!d_x[i] || !d_y[i] || !d_out[i] || xn[i] == 0 || yn[i] == 0)
+            return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+        const size_t w = std::max(xn[i], yn[i]);
+        const size_t nb = (w + kAddLimbsPerBlock - 1) / kAddLimbsPerBlock;
+        if (blocks + nb > 0xFFFFFFFFull) return fail(PINTT_ERR_TOO_LARGE, "too many add blocks");
+        jobs[i] = AddJob{d_x[i], d_y[i], d_out[i], xn[i], yn[i], subtract[i] ? 1u : 0u, (uint32_t)blocks,
+                         (uint32_t)nb, 0u};
+        blocks += nb;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    // Device scratch that earlier batches may still be using grows only after
+    // the stream drains.
+    if (count > g.add_job_cap || blocks > g.add_blk_cap) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        if (count > g.add_job_cap) {
+            (void)hipFree(g.add_status);
+            g.add_status = nullptr;
+            const size_t cap = std::max<size_t>(count, 1024);
+            if (hipMalloc(reinterpret_cast<void**>(&g.add_status), cap * sizeof(uint32_t)) != hipSuccess)
+                return fail(PINTT_ERR_NO_MEMORY, "add status");
+            if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
+            (void)hipFree(g.add_jobs_dev);
+            g.add_jobs_host = nullptr;
+            g.add_jobs_dev = nullptr;
+            if (hipHostMalloc(reinterpret_cast<void**>(&g.add_jobs_host), cap * sizeof(AddJob)) != hipSuccess ||
+                hipMalloc(reinterpret_cast<void**>(&g.add_jobs_dev), cap * sizeof(AddJob)) != hipSuccess)
+                return fail(PINTT_ERR_NO_MEMORY, "add descriptors");
+            g.add_job_cap = cap;
+        }
+        if (blocks > g.add_blk_cap) {
+            (void)hipFree(g.add_maps);
+            (void)hipFree(g.add_carry);
+            g.add_maps = g.add_carry = nullptr;
+            g.add_blk_cap = 0;
+            const size_t cap = std::max<size_t>(blocks, 1 << 16);
+            if (hipMalloc(reinterpret_cast<void**>(&g.add_maps), cap) != hipSuccess ||
+                hipMalloc(reinterpret_cast<void**>(&g.add_carry), cap) != hipSuccess)
+                return fail(PINTT_ERR_NO_MEMORY, "add block state");
+            g.add_blk_cap = cap;
+        }
+    }
+    // Descriptors through the ring that product batches use (its slot waits
+    // only for its own previous upload).
+    const int ring = g.desc_next;
+    g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
+    const size_t bytes = count * sizeof(AddJob);
+    PINTT_TRY(ensure_descriptors(ring, bytes));
+    std::memcpy(g.desc_host[ring], jobs.data(), bytes);
+    PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bytes, hipMemcpyHostToDevice, g.stream));
+    PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
+    PINTT_HIP(launch_add_batch(reinterpret_cast<const AddJob*>(g.desc_dev[ring]), (uint32_t)count, (uint32_t)blocks,
+                               g.add_maps, g.add_carry, d_negative, g.add_status, g.stream));
+    return PINTT_OK;
+}
+
 int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, const size_t* xn,
                                    const uint64_t* const* d_y, const size_t* yn, const int* subtract,
                                    uint64_t* const* d_out, int* negative) {

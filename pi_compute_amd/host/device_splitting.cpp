@@ -17,8 +17,9 @@
 // Signs: Q > 0; P(k) < 0 for every leaf k >= 1; R(a, b) has the sign of its
 // first term (the series alternates with terms shrinking ~1.5e14-fold), so
 // when t and u differ in sign |t| > |u| and R = sign(t) * (|t| - |u|).  The
-// device add reports a subtraction that would go negative; then the level
-// is redone here on the host from downloaded t and u.
+// levels are enqueued without waits; a subtraction that would go negative
+// sets a device flag, read once a tree is reduced, and then the caller
+// computes the range on the host instead.
 #include <gmp.h>
 
 #include <algorithm>
@@ -141,12 +142,43 @@ public:
         return res;
     }
 
+    // Throws if a level's subtraction went negative (waits for the levels).
+    void check_negative() const {
+        if (!flag_) return;
+        uint64_t v = 0;
+        uint64_t* h = &v;
+        const uint64_t* src = flag_;
+        size_t one = 1;
+        check(gpu_ntt_download(1, &h, &src, &one), "gpu_ntt_download");
+        if (v) throw std::runtime_error("device-resident levels: a subtraction for R went negative");
+    }
+
+    DeviceTree() = default;
+    ~DeviceTree() {
+        if (flag_) (void)gpu_ntt_dev_free(flag_);
+    }
+    DeviceTree(const DeviceTree&) = delete;
+    DeviceTree& operator=(const DeviceTree&) = delete;
+
 private:
+    // A device word the level adds set when a subtraction goes negative.
+    uint32_t* negative_flag() {
+        if (!flag_) {
+            check(gpu_ntt_dev_alloc(1, &flag_), "gpu_ntt_dev_alloc");
+            const uint64_t zero = 0;
+            const uint64_t* src = &zero;
+            size_t one = 1;
+            check(gpu_ntt_upload(1, &flag_, &src, &one), "gpu_ntt_upload");
+        }
+        return reinterpret_cast<uint32_t*>(flag_);
+    }
+
     Arena buf_[3];
     Arena* cur_ = &buf_[0];
     Arena* nxt_ = &buf_[1];
     Arena& tmp_ = buf_[2];
     std::vector<DevNode> dn_;
+    uint64_t* flag_ = nullptr;
 };
 
 void DeviceTree::reduce(bool need_top_p, std::vector<BinarySplitting::Level>& levels) {
@@ -211,41 +243,12 @@ void DeviceTree::reduce(bool need_top_p, std::vector<BinarySplitting::Level>& le
             SUB.push_back(same ? 0 : 1);
             RO.push_back(R.d);
         }
-        int rc = gpu_ntt_add_batch_device(pairs, X.data(), XN.data(), Y.data(), YN.data(), SUB.data(), RO.data());
-        if (rc == PINTT_ERR_ARGS) {
-            // Some |t| < |u|: redo every R of this level on the host.
-            for (size_t k = 0; k < pairs; ++k) {
-                mpz_t t, u;
-                mpz_inits(t, u, NULL);
-                uint64_t* tp = reinterpret_cast<uint64_t*>(mpz_limbs_write(t, T[k].n));
-                uint64_t* up = reinterpret_cast<uint64_t*>(mpz_limbs_write(u, U[k].n));
-                const uint64_t* src[2] = {T[k].d, U[k].d};
-                uint64_t* dst[2] = {tp, up};
-                size_t len[2] = {T[k].n, U[k].n};
-                check(gpu_ntt_download(2, dst, src, len), "gpu_ntt_download");
-                size_t tn = T[k].n, un = U[k].n;
-                while (tn && !tp[tn - 1]) --tn;
-                while (un && !up[un - 1]) --un;
-                mpz_limbs_finish(t, (mp_size_t)tn);
-                mpz_limbs_finish(u, (mp_size_t)un);
-                if (T[k].sign < 0) mpz_neg(t, t);
-                if (U[k].sign < 0) mpz_neg(u, u);
-                mpz_add(t, t, u);
-                DevVal& R = next[k].R;
-                R.sign = mpz_sgn(t) < 0 ? -1 : 1;
-                std::vector<uint64_t> mag(R.n, 0);
-                size_t cnt = 0;
-                mpz_export(mag.data(), &cnt, -1, 8, 0, 0, t);
-                if (cnt > R.n) throw std::runtime_error("R larger than its bound");
-                const uint64_t* ms = mag.data();
-                uint64_t* rd = R.d;
-                size_t rn = R.n;
-                check(gpu_ntt_upload(1, &rd, &ms, &rn), "gpu_ntt_upload");
-                mpz_clears(t, u, NULL);
-            }
-        } else {
-            check(rc, "gpu_ntt_add_batch_device");
-        }
+        // Enqueued without a wait: a subtraction that went negative (|t| < |u|,
+        // which the sign rule above excludes) sets the tree's flag, read once
+        // at the end.
+        check(gpu_ntt_add_batch_device_async(pairs, X.data(), XN.data(), Y.data(), YN.data(), SUB.data(), RO.data(),
+                                             negative_flag()),
+              "gpu_ntt_add_batch_device_async");
         if (dn_.size() & 1) {
             const DevNode& o = dn_.back();
             DevNode& m = next.back();
@@ -256,8 +259,7 @@ void DeviceTree::reduce(bool need_top_p, std::vector<BinarySplitting::Level>& le
                 check(gpu_ntt_dev_copy(pr.second->d, pr.first->d, pr.first->n), "gpu_ntt_dev_copy");
             }
         }
-        check(gpu_ntt_synchronize(), "gpu_ntt_synchronize");
-        std::swap(cur_, nxt_);  // the old level's buffer is reused two levels on
+        std::swap(cur_, nxt_);  // the old level's buffer is reused two levels on (stream order)
         dn_.swap(next);
         levels.push_back(BinarySplitting::Level{pairs, O.size(), max_limbs,
                                                 std::chrono::duration<double>(Clock::now() - t0).count()});
@@ -343,11 +345,16 @@ BSResult BinarySplitting::run_segments(unsigned long a, unsigned long b, bool ne
             leaf_wait_s_ += std::chrono::duration<double>(Clock::now() - tw).count();
         }
     }
-    if (nseg == 1) return at_root(*trees[0]);
+    if (nseg == 1) {
+        trees[0]->check_negative();
+        return at_root(*trees[0]);
+    }
     // The segment roots merge on the device.
     DeviceTree top;
     top.adopt(std::move(roots));
     top.reduce(need_top_p, levels_);
+    for (const auto& t : trees) t->check_negative();
+    top.check_negative();
     return at_root(top);
 }
 

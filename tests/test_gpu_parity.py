@@ -440,6 +440,25 @@ def test_device_subtract_flags(engine, seed):
     assert neg2 == [False, False]
 
 
+def test_device_add_enqueued(engine):
+    """The enqueue-only add: the same sums and differences as the waiting
+    form, and the device word set only when a subtraction went negative."""
+    rng = np.random.default_rng(31)
+    pairs, sub = [], []
+    for xn in (1, 4095, 4097, 20000):
+        x = rng.integers(0, 2**64, xn, dtype=np.uint64)
+        y = rng.integers(0, 2**64, max(1, xn // 2), dtype=np.uint64)
+        pairs += [(x, y), (x, y)]
+        sub += [False, True]
+    got, negative = pc.add_batch(pairs, sub, enqueue=True)
+    assert not negative
+    for g, (x, y), s in zip(got, pairs, sub):
+        X, Y = pc.to_int(x), pc.to_int(y)
+        assert pc.to_int(g) == (X - Y if s else X + Y)
+    _, negative = pc.add_batch([(np.array([5], np.uint64), np.array([7], np.uint64))], [True], enqueue=True)
+    assert negative
+
+
 def test_device_decimal_leaves(engine):
     """Device decimal leaves against Python's str(): widths around the 19-digit
     chunks up to the 4096-digit limit, zero, 10^w - 1, 10^(w-1), random values,
