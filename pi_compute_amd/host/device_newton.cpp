@@ -35,13 +35,14 @@ using dev::Num;
 class Dev {
 public:
     explicit Dev(size_t limbs) : base_(mem_.alloc(limbs)), cap_(limbs) {
-        const uint64_t c[2] = {0, 1};
+        const uint64_t c[3] = {0, 1, 1ull << 63};
         zero = take(1);
         one = take(1);
-        const uint64_t* src[2] = {&c[0], &c[1]};
-        uint64_t* dst[2] = {zero.d, one.d};
-        const size_t n[2] = {1, 1};
-        dev::check(gpu_ntt_upload(2, dst, src, n), "gpu_ntt_upload");
+        half = take(1);
+        const uint64_t* src[3] = {&c[0], &c[1], &c[2]};
+        uint64_t* dst[3] = {zero.d, one.d, half.d};
+        const size_t n[3] = {1, 1, 1};
+        dev::check(gpu_ntt_upload(3, dst, src, n), "gpu_ntt_upload");
     }
     Num take(size_t n) {
         n = std::max<size_t>(n, 1);
@@ -85,7 +86,7 @@ public:
                    "gpu_ntt_add_batch_device_flags");
         return neg != 0;
     }
-    Num zero, one;
+    Num zero, one, half;  // 0, 1 and 2^63 (beta / 2)
 
 private:
     dev::Buffers mem_;
@@ -96,6 +97,66 @@ private:
 constexpr size_t kGuard = 3;
 constexpr size_t kExactLimbs = 16;   // the host mirror's exact start (newton.cpp)
 constexpr size_t kHostLimbs = 16384;
+
+// One Newton step from precision c to cp limbs, given p ~ beta^K on the
+// device (p.n >= K + 1): E = beta^K - p, E_t = the limbs of |E| from m - T to
+// m (m = K - c + 2, T = cp - c + kGuard), Y' = Y beta^(cp - c) +- (Y E_t) >>
+// (K - (m - T) - (cp - c)) limbs.  half: the correction is halved (inverse
+// square root), as E_t times 2^63 with one more limb dropped.
+Num newton_step(Dev& g, Num Y, Num p, size_t K, size_t c, size_t cp, bool half) {
+    const size_t s = cp - c;
+    if (p.n < K + 1) throw std::runtime_error("device Newton step: product shorter than expected");
+    const std::vector<uint64_t> top = g.fetch(p, K - 1, p.n - K + 1);
+    bool above = false;
+    for (size_t i = 2; i < top.size(); ++i) above = above || top[i] != 0;
+    Num E;  // |beta^K - p|, K limbs
+    bool positive;
+    if (!above && top[1] == 1) {  // p >= beta^K
+        positive = false;
+        E = Num{p.d, K};
+    } else if (!above && top[1] == 0 && top[0] == ~0ull) {  // p < beta^K
+        positive = true;
+        E = g.take(K);
+        (void)g.addsub(E.d, g.zero, Num{p.d, K}, true);  // beta^K - p
+    } else {
+        throw std::runtime_error("device Newton step: product far from beta^K");
+    }
+    const size_t m = K - c + 2;
+    if (m < K) {
+        const std::vector<uint64_t> z = g.fetch(E, m, std::min<size_t>(2, K - m));
+        for (uint64_t v : z)
+            if (v) throw std::runtime_error("device Newton step: correction above its bound");
+    }
+    const size_t T = s + kGuard;
+    const size_t lo = m > T ? m - T : 0;
+    Num Et{E.d + lo, std::min(m, K) - lo};
+    if (half) Et = g.mul(Et, g.half);
+    const Num p2 = g.mul(Y, Et);
+    const size_t sh = K - lo - s + (half ? 1 : 0);
+    if (p2.n <= sh + s) throw std::runtime_error("device Newton step: correction shorter than expected");
+    const Num term{p2.d + sh, p2.n - sh};
+    const Num hi{term.d + s, term.n - s};
+    const size_t hn = std::max(Y.n, hi.n) + 1;
+    Num Yn = g.take(s + hn);
+    if (positive) {
+        dev::check(gpu_ntt_dev_copy(Yn.d, term.d, s), "gpu_ntt_dev_copy");
+        (void)g.addsub(Yn.d + s, Y, hi, false);
+    } else {
+        // Y beta^s - term: the low limbs are -term mod beta^s, with a borrow
+        // into the high part when they are not all zero.
+        const bool borrow = g.addsub(Yn.d, g.zero, Num{term.d, s}, true);
+        if (borrow) {
+            Num t = g.take(std::max(Y.n, hi.n));
+            if (g.addsub(t.d, Y, hi, true) || g.addsub(Yn.d + s, t, g.one, true))
+                throw std::runtime_error("device Newton step: negative estimate");
+        } else if (g.addsub(Yn.d + s, Y, hi, true)) {
+            throw std::runtime_error("device Newton step: negative estimate");
+        }
+        Yn.n = s + std::max(Y.n, hi.n);
+    }
+    Yn.n = std::min(Yn.n, cp + 2);  // Y' <= beta^(cp + 1) plus a few units
+    return Yn;
+}
 
 // Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
 // and its top limbs on the host (at least min(Ld, kHostLimbs + kGuard + 1)
@@ -127,60 +188,10 @@ Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B) {
         mpz_clear(t);
     }
     for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
-        const size_t cp = *it, s = cp - c;
+        const size_t cp = *it;
         const size_t l = std::min(Ld, cp + kGuard);
         const Num Dp{D.d + (Ld - l), l};
-        const size_t K = l + c;
-        const Num p = g.mul(Dp, Y);  // ~ beta^K, p.n >= K + 1
-        const std::vector<uint64_t> top = g.fetch(p, K - 1, p.n - K + 1);
-        bool above = false;
-        for (size_t i = 2; i < top.size(); ++i) above = above || top[i] != 0;
-        Num E;  // |beta^K - p|, K limbs
-        bool positive;
-        if (!above && top[1] == 1) {  // p >= beta^K
-            positive = false;
-            E = Num{p.d, K};
-        } else if (!above && top[1] == 0 && top[0] == ~0ull) {  // p < beta^K
-            positive = true;
-            E = g.take(K);
-            (void)g.addsub(E.d, g.zero, Num{p.d, K}, true);  // beta^K - p
-        } else {
-            throw std::runtime_error("device reciprocal: product far from beta^K");
-        }
-        const size_t m = K - c + 2;
-        if (m < K) {
-            const std::vector<uint64_t> z = g.fetch(E, m, std::min<size_t>(2, K - m));
-            for (uint64_t v : z)
-                if (v) throw std::runtime_error("device reciprocal: correction above its bound");
-        }
-        const size_t T = s + kGuard;
-        const size_t lo = m > T ? m - T : 0;
-        const Num Et{E.d + lo, std::min(m, K) - lo};
-        const Num p2 = g.mul(Y, Et);
-        const size_t sh = K - lo - s;
-        if (p2.n <= sh + s) throw std::runtime_error("device reciprocal: correction shorter than expected");
-        const Num term{p2.d + sh, p2.n - sh};
-        const Num hi{term.d + s, term.n - s};
-        const size_t hn = std::max(Y.n, hi.n) + 1;
-        Num Yn = g.take(s + hn);
-        if (positive) {
-            dev::check(gpu_ntt_dev_copy(Yn.d, term.d, s), "gpu_ntt_dev_copy");
-            (void)g.addsub(Yn.d + s, Y, hi, false);
-        } else {
-            // Y beta^s - term: the low limbs are -term mod beta^s, with a borrow
-            // into the high part when they are not all zero.
-            const bool borrow = g.addsub(Yn.d, g.zero, Num{term.d, s}, true);
-            if (borrow) {
-                Num t = g.take(std::max(Y.n, hi.n));
-                if (g.addsub(t.d, Y, hi, true) || g.addsub(Yn.d + s, t, g.one, true))
-                    throw std::runtime_error("device reciprocal: negative estimate");
-            } else if (g.addsub(Yn.d + s, Y, hi, true)) {
-                throw std::runtime_error("device reciprocal: negative estimate");
-            }
-            Yn.n = s + std::max(Y.n, hi.n);
-        }
-        Yn.n = std::min(Yn.n, cp + 2);  // Y' <= beta^(c' + 1) plus a few units
-        Y = Yn;
+        Y = newton_step(g, Y, g.mul(Dp, Y), l + c, c, cp, false);  // D' Y ~ beta^K, K = l + c
         c = cp;
     }
     return Y;
@@ -193,6 +204,40 @@ std::pair<size_t, size_t> top_limbs(mpz_srcptr x, size_t limbs) {
 }
 
 }  // namespace
+
+void device_inv_sqrt_limbs(mpz_t y, unsigned long a, size_t B) {
+    if (a == 0) throw std::runtime_error("device inverse square root of 0");
+    std::vector<size_t> sched;
+    for (size_t c = B; c > kExactLimbs; c = (c + 3) / 2) sched.push_back(c);
+    size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
+    while (!sched.empty() && sched.back() <= kHostLimbs) {
+        c = sched.back();
+        sched.pop_back();
+    }
+    Dev g(16 * (B + 4) + 64);
+    Num Y, ka = g.take(1);
+    {
+        const uint64_t av = a;
+        const uint64_t* h = &av;
+        size_t one = 1;
+        dev::check(gpu_ntt_upload(1, &ka.d, &h, &one), "gpu_ntt_upload");
+        mpz_t t;
+        mpz_init(t);
+        newton_inv_sqrt_limbs(t, a, c);  // the host steps up to kHostLimbs
+        Y = g.upload(t, 0, mpz_size(t));
+        mpz_clear(t);
+    }
+    for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
+        const size_t cp = *it;
+        Y = newton_step(g, Y, g.mul(g.mul(Y, Y), ka), 2 * c, c, cp, true);  // a Y^2 ~ beta^(2c)
+        c = cp;
+    }
+    uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(y, (mp_size_t)Y.n));
+    const uint64_t* src = Y.d;
+    size_t n = Y.n;
+    dev::check(gpu_ntt_download(1, &h, &src, &n), "gpu_ntt_download");
+    mpz_limbs_finish(y, (mp_size_t)Y.n);
+}
 
 void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B) {
     if (mpz_sgn(d) <= 0) throw std::runtime_error("device reciprocal: divisor must be positive");

@@ -152,6 +152,44 @@ void newton_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B) {
     mpz_clears(dp, p, e, t, NULL);
 }
 
+void newton_inv_sqrt_limbs(mpz_t y, unsigned long a, size_t B) {
+    constexpr size_t G = 3, kExact = 16;
+    std::vector<size_t> sched;
+    for (size_t c = B; c > kExact; c = (c + 3) / 2) sched.push_back(c);
+    size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
+    mpz_t t, p, e;
+    mpz_inits(t, p, e, NULL);
+    mpz_set_ui(t, 1);  // start: floor(sqrt(floor(beta^(2c) / a)))
+    mpz_mul_2exp(t, t, 128 * c);
+    mpz_tdiv_q_ui(t, t, a);
+    mpz_sqrt(y, t);
+    for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
+        const size_t cp = *it, s = cp - c;
+        mpz_mul(p, y, y);
+        mpz_mul_ui(p, p, a);  // ~ beta^(2c)
+        mpz_set_ui(e, 1);
+        mpz_mul_2exp(e, e, 128 * c);
+        mpz_sub(e, e, p);
+        const int sign = mpz_sgn(e);
+        mpz_abs(e, e);
+        const size_t m = c + 2;
+        if (mpz_size(e) > m) throw std::runtime_error("newton_inv_sqrt_limbs: correction above its bound");
+        const size_t T = s + G;
+        const size_t lo = m > T ? m - T : 0;
+        mpz_tdiv_q_2exp(t, e, 64 * lo);
+        mpz_mul_2exp(t, t, 63);  // the halving of the step, as beta / 2
+        mpz_mul(p, y, t);
+        mpz_tdiv_q_2exp(p, p, 64 * (2 * c - lo - s + 1));
+        mpz_mul_2exp(y, y, 64 * s);
+        if (sign >= 0)
+            mpz_add(y, y, p);
+        else
+            mpz_sub(y, y, p);
+        c = cp;
+    }
+    mpz_clears(t, p, e, NULL);
+}
+
 void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m) {
     auto sched = schedule(b);
     const size_t c0 = sched.empty() ? b : sched.back() / 2 + 16;

@@ -33,6 +33,13 @@ void newton_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
 // fails.
 void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
 
+// y ~ beta^B / sqrt(a) in whole limbs, a > 0 (the step: E = beta^(2c) - a y^2,
+// y' = y beta^(c'-c) + y E_t 2^63 >> limbs, 2^63 being the step's halving as
+// beta / 2), exact products on the host; and the same steps on device numbers
+// (device_newton.cpp), the same integer.
+void newton_inv_sqrt_limbs(mpz_t y, unsigned long a, size_t B);
+void device_inv_sqrt_limbs(mpz_t y, unsigned long a, size_t B);
+
 // y ~ 2^b / sqrt(a), a > 0; |y - 2^b/sqrt(a)| <= 4.
 void newton_inv_sqrt_ui(mpz_t y, unsigned long a, size_t b, Multiplier& m);
 

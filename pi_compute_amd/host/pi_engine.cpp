@@ -56,9 +56,17 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
             dpow_done = dpow_set.get_future();
         }
         if (pre)
-            beside = std::thread([this, &cfg, fin_w, &pre, &pre_set]() {
+            beside = std::thread([this, &cfg, fin_w, final_device, &pre, &pre_set]() {
                 try {
-                    final_constants(*pre, cfg.digits, fin_w, mult_);
+                    try {
+                        final_constants(*pre, cfg.digits, fin_w, mult_, final_device);
+                    } catch (const std::runtime_error& e) {
+                        if (!final_device) throw;
+                        std::cerr << "  device inverse square root failed (" << e.what() << "); host instead"
+                                  << std::endl;
+                        pre = std::make_unique<FinalConstants>();
+                        final_constants(*pre, cfg.digits, fin_w, mult_, false);
+                    }
                     pre_set.set_value();
                 } catch (...) {
                     pre_set.set_exception(std::current_exception());
@@ -271,9 +279,12 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
 FinalConstants::FinalConstants() { mpz_inits(inv_sqrt, pow10, NULL); }
 FinalConstants::~FinalConstants() { mpz_clears(inv_sqrt, pow10, NULL); }
 
-void final_constants(FinalConstants& c, size_t digits, size_t w, Multiplier& m) {
+void final_constants(FinalConstants& c, size_t digits, size_t w, Multiplier& m, bool device) {
     auto t = Clock::now();
-    newton_inv_sqrt_ui(c.inv_sqrt, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
+    if (device && w % 64 == 0)
+        device_inv_sqrt_limbs(c.inv_sqrt, 10005, w / 64);  // the same 2^w / sqrt(10005), within 4
+    else
+        newton_inv_sqrt_ui(c.inv_sqrt, 10005, w, m);  // sqrt(10005) = 10005 * y / 2^w
     c.inv_sqrt_s = since(t);
     t = Clock::now();
     pow_ui(c.pow10, 10, digits, m);

@@ -71,7 +71,9 @@ struct FinalConstants {
     FinalConstants(const FinalConstants&) = delete;
     FinalConstants& operator=(const FinalConstants&) = delete;
 };
-void final_constants(FinalConstants& c, size_t digits, size_t bits, Multiplier& m);
+// device (bits a multiple of 64): the inverse square root as whole-limb
+// device Newton steps (newton.h, device_inv_sqrt_limbs).
+void final_constants(FinalConstants& c, size_t digits, size_t bits, Multiplier& m, bool device = false);
 
 // floor(426880 * sqrt(10005) * Q * 10^digits / R) with `bits` of working
 // precision, products through m (R > 0).  Used by FinalStep::Newton.  With

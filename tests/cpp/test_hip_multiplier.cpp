@@ -338,6 +338,17 @@ int main() {
                     err = ex.what();
                 }
             }
+        for (unsigned long a : {2ul, 10005ul, 999999937ul})
+            for (size_t B : {10ul, 300ul, 20000ul, 70000ul}) {
+                try {
+                    pi::device_inv_sqrt_limbs(yd, a, B);
+                    pi::newton_inv_sqrt_limbs(yh, a, B);
+                    ok = ok && mpz_cmp(yd, yh) == 0;
+                } catch (const std::exception& ex) {
+                    ok = false;
+                    err = ex.what();
+                }
+            }
         mpz_clears(d, yd, yh, NULL);
         check(ok, "DeviceReciprocalMatchesHostStatement" + (err.empty() ? std::string() : " (" + err + ")"));
     }

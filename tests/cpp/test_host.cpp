@@ -243,6 +243,20 @@ int main(int argc, char** argv) {
             ok = ok && mpz_cmp_ui(t, 4) <= 0;
         }
         check(ok, "NewtonLimbsWithinFew");
+        // Whole-limb inverse square root (the device final step's constant).
+        ok = true;
+        for (unsigned long a : {2ul, 3ul, 10005ul, 999999937ul, 4294967291ul})
+            for (size_t B = 1; B < 700; B += 29) {
+                pi::newton_inv_sqrt_limbs(y, a, B);
+                mpz_set_ui(t, 0);
+                mpz_setbit(t, 128 * B);
+                mpz_tdiv_q_ui(t, t, a);
+                mpz_sqrt(want, t);
+                mpz_sub(t, y, want);
+                mpz_abs(t, t);
+                ok = ok && mpz_cmp_ui(t, 4) <= 0;
+            }
+        check(ok, "NewtonInverseSqrtLimbsWithinFew");
         ok = true;
         for (unsigned long e : {0ul, 1ul, 2ul, 77ul, 1000ul, 123457ul}) {
             pi::pow_ui(y, 10, e, gmp);
