@@ -1,9 +1,12 @@
-// device_newton.cpp - see pi_engine.h (pi_scaled_newton_device).
+// device_newton.cpp - whole-limb Newton iterations on device numbers: the
+// reciprocal and the inverse square root (newton.h), and the final step of pi
+// (pi_engine.h, pi_scaled_newton_device).
 //
-// The final step's numbers stay in device memory: every truncation keeps
-// whole limbs (a pointer offset), products are gpu_ntt_mul_batch_device
-// calls, additions and subtractions gpu_ntt_add_batch_device(_flags), and the
-// host only reads a few limbs to learn signs.
+// Every truncation keeps whole limbs (a pointer offset), products are
+// gpu_ntt_mul_batch_device calls, additions and subtractions
+// gpu_ntt_add_batch_device_flags, and the host only reads a few limbs to learn
+// signs.  Steps up to 16384 limbs run on the host (GMP is faster there than
+// device round trips); both sides compute the same integers.
 //
 // Reciprocal (beta = 2^64): Y ~ beta^(l + c) / D_l at precision c limbs,
 // D_l the top l = min(Ld, c + G) limbs of D.  One step to c' <= 2c - 2:
@@ -11,11 +14,15 @@
 //   E = beta^K - p (|E| < beta^(K - c + 2) = beta^m), E_t = the limbs of |E|
 //   from m - (c' - c + G) up to m, term = (Y E_t) >> (K - (m - (c' - c + G))
 //   - (c' - c)) limbs, Y' = Y beta^(c' - c) +- term.
+// Inverse square root: the same step with p = a Y^2 ~ beta^(2c) (K = 2c) and
+// the correction halved, as E_t times 2^63 = beta / 2 with one more limb
+// dropped.
 // With G = 3 guard limbs each truncation costs under one unit of Y', and
 // the step's own error (the square of the relative error of Y) is below one
-// as well: Y ends within a few units of beta^(Ld + B) / D.  The same step on
-// host integers, checked against the exact quotient, is the host test
-// NewtonLimbsWithinFew (newton_reciprocal_limbs in newton.h).
+// as well: Y ends within a few units of the exact value.  The same steps on
+// host integers (newton_reciprocal_limbs, newton_inv_sqrt_limbs) are checked
+// against exact quotients and square roots by the host tests, and the device
+// results against them, integer for integer, by the GPU tests.
 #include <gmp.h>
 
 #include <cstring>
