@@ -72,3 +72,17 @@ def test_pi_gpu_learned_dispatch_matches_gmp_run(tmp_path, gmp_digits, device):
     got = _run([str(BIN / "pi_compute_hip"), "--digits", "1000000", "--gpu", "--dispatch", "mlp", "--model",
                 str(MODEL), *extra], tmp_path / "mlp.txt")
     assert got == gmp_digits[1000000]
+
+
+@pytest.mark.parametrize("digits", [8193, 65537, 16777216])
+def test_pi_device_tail_at_power_of_two_widths(tmp_path, digits):
+    """The device-resident tail (final step, conversion levels and leaves)
+    where the digit count of pi * 10^D sits just past a power of two (the
+    number of conversion levels changes there): the reference's own GMP
+    program gives the expected file."""
+    if not REF_PI.exists():
+        pytest.skip("the reference program is not built in this tree")
+    want = _run([str(REF_PI), "--digits", str(digits)], tmp_path / "ref.txt")
+    got = _run([str(BIN / "pi_compute_hip"), "--digits", str(digits), "--gpu", "--tree", "levels", "--device-levels"],
+               tmp_path / "gpu.txt")
+    assert got == want
