@@ -143,6 +143,18 @@ int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, con
 int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const size_t* width,
                                   const size_t* offset, uint8_t* d_out, size_t out_len);
 
+/* The bottom of the Chudnovsky binary-splitting tree on the device: subtree i
+ * covers the terms [a + 64 i, a + 64 (i + 1)) (terms at or past b count as
+ * the identity P = Q = 1, R = 0), and d_out + 3 * PINTT_BS_LEAF_LIMBS * i
+ * receives the magnitudes of its P, Q and R, PINTT_BS_LEAF_LIMBS limbs each
+ * (zero-padded); lengths[3 i + 0..2] their exact limb counts, signs[2 i + 0..1]
+ * the signs of P and R (+1 or -1).  Requires a < b <= 2^24.  Waits.  Replaces
+ * the host subtrees of the level-synchronous tree (the reference's recursion,
+ * src/engine/binary_splitting.cpp:137-160, at the bottom of the tree). */
+#define PINTT_BS_LEAF_TERMS 64
+#define PINTT_BS_LEAF_LIMBS 255
+int gpu_ntt_bs_leaves(uint64_t a, uint64_t b, size_t count, uint64_t* d_out, uint32_t* lengths, int32_t* signs);
+
 /* Wait for all enqueued products (PINTT_ERR_HIP if the stream reports an
  * error). */
 int gpu_ntt_synchronize(void);

@@ -110,6 +110,11 @@ ABI = {
          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u64p),
          ctypes.POINTER(ctypes.c_int)],
     ),
+    "gpu_ntt_bs_leaves": (
+        ctypes.c_int,
+        [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t, _u64p, ctypes.POINTER(ctypes.c_uint32),
+         ctypes.POINTER(ctypes.c_int32)],
+    ),
     "gpu_ntt_decimal_leaves_device": (
         ctypes.c_int,
         [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),

@@ -44,4 +44,15 @@ struct LeafJob {
 
 hipError_t launch_decimal_leaves(const LeafJob* jobs, uint32_t njobs, uint8_t* out, hipStream_t s);
 
+// Binary-splitting subtrees (bs_leaves.hip): subtree i covers terms
+// [a + 64 i, a + 64 (i + 1)) with terms at or past b as the identity; out
+// holds its P, Q, R magnitudes (kBsLeafLimbs limbs each, zero-padded), lengths
+// their exact limb counts, signs those of P and R.  b < 2^24.
+constexpr int kBsLeafThreads = 256;
+constexpr int kBsLeafTerms = 64;
+constexpr int kBsLeafLevels = 6;
+constexpr int kBsLeafLimbs = 255;  // (4 << 6) - 1
+hipError_t launch_bs_leaves(uint64_t a, uint64_t b, uint32_t count, uint64_t* out, uint32_t* lengths, int32_t* signs,
+                            uint32_t* error, hipStream_t s);
+
 }  // namespace pintt

@@ -38,6 +38,8 @@
 #include "bigint_add.h"
 #include "ntt_kernels.h"
 
+static_assert(pintt::kBsLeafTerms == PINTT_BS_LEAF_TERMS && pintt::kBsLeafLimbs == PINTT_BS_LEAF_LIMBS,
+              "subtree shape");
 static_assert(pintt::kLeafMaxDigits == PINTT_DECIMAL_LEAF_MAX_DIGITS &&
                   pintt::kLeafMaxLimbs == PINTT_DECIMAL_LEAF_MAX_LIMBS,
               "decimal leaf limits");
@@ -81,6 +83,11 @@ struct Engine {
     uint8_t* add_maps = nullptr;
     uint8_t* add_carry = nullptr;
     uint32_t* add_error = nullptr;
+    // Binary-splitting subtrees: lengths, signs and the error word (device).
+    size_t bs_cap = 0;
+    uint32_t* bs_len = nullptr;
+    int32_t* bs_sign = nullptr;
+    uint32_t* bs_err = nullptr;
     // Decimal leaves: job descriptors (pinned + device).
     size_t leaf_job_cap = 0;
     LeafJob* leaf_jobs_host = nullptr;
@@ -1024,6 +1031,9 @@ void gpu_ntt_shutdown(void) {
     (void)hipFree(g.add_maps);
     (void)hipFree(g.add_carry);
     (void)hipFree(g.add_error);
+    (void)hipFree(g.bs_len);
+    (void)hipFree(g.bs_sign);
+    (void)hipFree(g.bs_err);
     if (g.leaf_jobs_host) (void)hipHostFree(g.leaf_jobs_host);
     (void)hipFree(g.leaf_jobs_dev);
     (void)hipFree(g.tune_in);
@@ -1377,6 +1387,39 @@ int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, con
                                    uint64_t* const* d_out, int* negative) {
     if (count && !negative) return fail(PINTT_ERR_ARGS, "null pointer");
     return add_batch(count, d_x, xn, d_y, yn, subtract, d_out, negative);
+}
+
+int gpu_ntt_bs_leaves(uint64_t a, uint64_t b, size_t count, uint64_t* d_out, uint32_t* lengths, int32_t* signs) {
+    if (count == 0) return PINTT_OK;
+    if (!d_out || !lengths || !signs) return fail(PINTT_ERR_ARGS, "null pointer");
+    if (b > (1ull << 24) || a >= b) return fail(PINTT_ERR_ARGS, "terms must satisfy a < b <= 2^24");
+    if (count > 0x7FFFFFFFull) return fail(PINTT_ERR_TOO_LARGE, "too many subtrees");
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    if (count > g.bs_cap) {
+        PINTT_HIP(hipStreamSynchronize(g.stream));
+        (void)hipFree(g.bs_len);
+        (void)hipFree(g.bs_sign);
+        g.bs_len = nullptr;
+        g.bs_sign = nullptr;
+        g.bs_cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&g.bs_len), count * 3 * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&g.bs_sign), count * 2 * sizeof(int32_t)) != hipSuccess)
+            return fail(PINTT_ERR_NO_MEMORY, "subtree lengths");
+        g.bs_cap = count;
+    }
+    if (!g.bs_err && hipMalloc(reinterpret_cast<void**>(&g.bs_err), sizeof(uint32_t)) != hipSuccess)
+        return fail(PINTT_ERR_NO_MEMORY, "subtree error word");
+    PINTT_HIP(launch_fill_u32(g.bs_err, 0u, 1, g.stream));
+    PINTT_HIP(launch_bs_leaves(a, b, (uint32_t)count, d_out, g.bs_len, g.bs_sign, g.bs_err, g.stream));
+    uint32_t err = 0;
+    PINTT_HIP(hipMemcpyAsync(lengths, g.bs_len, count * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+    PINTT_HIP(hipMemcpyAsync(signs, g.bs_sign, count * 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g.stream));
+    PINTT_HIP(hipMemcpyAsync(&err, g.bs_err, sizeof(err), hipMemcpyDeviceToHost, g.stream));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    if (err) return fail(PINTT_ERR_ARGS, "subtree: a subtraction for R went negative");
+    return PINTT_OK;
 }
 
 int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, const size_t* xn, const size_t* width,

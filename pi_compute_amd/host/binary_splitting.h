@@ -77,6 +77,11 @@ public:
 
     void set_leaf_span(unsigned long s) { leaf_span_ = s ? s : 1; }
 
+    // compute_device / compute_device_qr: the subtrees of the bottom of the
+    // tree computed on the device (gpu_ntt_bs_leaves, 64 terms each; ranges up
+    // to 2^24 terms) instead of on host threads.
+    void set_device_leaves(bool on) { device_leaves_ = on; }
+
     struct Level {
         size_t merges, products;
         size_t max_limbs;
@@ -101,6 +106,7 @@ private:
     unsigned threads_;
     unsigned long leaf_span_ = 256;
     unsigned segments_ = 8;
+    bool device_leaves_ = false;
     std::vector<Level> levels_;
     double leaf_s_ = 0, leaf_wait_s_ = 0, upload_s_ = 0;
     const Dispatcher* switch_by_ = nullptr;

@@ -115,6 +115,25 @@ public:
         nodes.clear();
     }
 
+    // Subtrees of PINTT_BS_LEAF_TERMS terms computed on the device
+    // (gpu_ntt_bs_leaves) as the first level, a < b <= 2^24.
+    void from_device_leaves(unsigned long a, unsigned long b) {
+        const size_t count = (b - a + PINTT_BS_LEAF_TERMS - 1) / PINTT_BS_LEAF_TERMS;
+        const size_t per = 3 * (size_t)PINTT_BS_LEAF_LIMBS;
+        cur_->reset(count * per);
+        uint64_t* d = cur_->take(count * per);
+        std::vector<uint32_t> len(3 * count);
+        std::vector<int32_t> sg(2 * count);
+        check(gpu_ntt_bs_leaves(a, b, count, d, len.data(), sg.data()), "gpu_ntt_bs_leaves");
+        dn_.assign(count, DevNode{});
+        for (size_t i = 0; i < count; ++i) {
+            uint64_t* base = d + i * per;
+            dn_[i].P = DevVal{base, len[3 * i], sg[2 * i]};
+            dn_[i].Q = DevVal{base + PINTT_BS_LEAF_LIMBS, len[3 * i + 1], 1};
+            dn_[i].R = DevVal{base + 2 * PINTT_BS_LEAF_LIMBS, len[3 * i + 2], sg[2 * i + 1]};
+        }
+    }
+
     // Nodes that live in other trees' arenas (which must outlive this tree's
     // first level).
     void adopt(std::vector<DevNode> nodes) { dn_ = std::move(nodes); }
@@ -277,8 +296,10 @@ BSResult BinarySplitting::run_segments(unsigned long a, unsigned long b, bool ne
     using Clock = std::chrono::steady_clock;
     levels_.clear();
     leaf_s_ = leaf_wait_s_ = upload_s_ = 0;
-    // Segments of whole leaf spans, at least 64 leaves each.
-    const unsigned long span = leaf_span_;
+    // Segments of whole leaf spans, at least 64 leaves each.  Device subtrees
+    // (ranges below 2^24 terms) have a fixed span.
+    const bool dev_leaves = device_leaves_ && b <= (1ul << 24) && b > a;
+    const unsigned long span = dev_leaves ? (unsigned long)PINTT_BS_LEAF_TERMS : leaf_span_;
     const unsigned long nleaf = (b - a + span - 1) / span;
     const unsigned long nseg = std::max(1ul, std::min<unsigned long>(segments_, nleaf / 64));
     std::vector<unsigned long> cut(nseg + 1);
@@ -296,8 +317,15 @@ BSResult BinarySplitting::run_segments(unsigned long a, unsigned long b, bool ne
         std::vector<Level> levels;
         double leaf_s = 0, upload_s = 0;
     };
-    auto prepare = [this, &cut, nseg, need_top_p, device_from_limbs](unsigned long i, unsigned threads) {
+    auto prepare = [this, &cut, nseg, need_top_p, device_from_limbs, dev_leaves](unsigned long i, unsigned threads) {
         Segment seg;
+        if (dev_leaves) {
+            const auto tu = std::chrono::steady_clock::now();
+            seg.tree = std::make_unique<DeviceTree>();
+            seg.tree->from_device_leaves(cut[i], cut[i + 1]);
+            seg.leaf_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tu).count();
+            return seg;
+        }
         const bool top_p = nseg == 1 ? need_top_p : true;
         std::vector<BSResult> nodes = leaves(cut[i], cut[i + 1], threads, &seg.leaf_s);
         while (nodes.size() > 1) {

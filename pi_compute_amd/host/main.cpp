@@ -25,6 +25,8 @@
 //                             device levels, else 256)
 //   --no-overlap              Newton final step: compute sqrt(10005) and 10^D after binary
 //                             splitting instead of beside it
+//   --host-subtrees           with --device-levels: the bottom subtrees of binary splitting on
+//                             host threads (GMP) instead of the device
 //   --host-final-step         with --device-levels: the Newton final step on host numbers
 //                             (products staged through PCIe) instead of device-resident ones
 //   --host-conversion-levels  with --device-levels: the multiplier conversion's levels on
@@ -67,7 +69,7 @@ void usage() {
                  "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
                  "                      [--final mpf|newton] [--convert host|multiplier] [--convert-leaf D]\n"
                  "                      [--device-levels [LIMBS|auto]] [--bs-segments S] [--bs-leaf T] [--no-overlap]\n"
-                 "                      [--host-final-step] [--host-conversion-levels]\n"
+                 "                      [--host-subtrees] [--host-final-step] [--host-conversion-levels]\n"
                  "                      [--plans FILE]\n"
                  "                      [--verbose]\n";
 }
@@ -111,6 +113,7 @@ int main(int argc, char** argv) {
         else if (a == "--no-overlap") cfg.overlap_constants = false;
         else if (a == "--host-conversion-levels") cfg.device_conversion = false;
         else if (a == "--host-final-step") cfg.device_final = false;
+        else if (a == "--host-subtrees") cfg.device_leaves = false;
         else if (a == "--bs-leaf") cfg.bs_leaf_span = std::stoul(next());
         else if (a == "--device-levels") {
             cfg.device_levels = true;

@@ -103,6 +103,7 @@ PiResult PiEngine::compute(const PiConfig& cfg) {
     bs.set_switch_dispatcher(cfg.device_switch);
     bs.set_segments(cfg.bs_segments);
     bs.set_leaf_span(cfg.bs_leaf_span ? cfg.bs_leaf_span : (cfg.level_batches && cfg.device_levels ? 128 : 256));
+    bs.set_device_leaves(cfg.level_batches && cfg.device_levels && cfg.device_leaves);
     BSResult r;
     std::unique_ptr<DeviceQR> qr;  // Q and R left on the device for the device final step
     if (cfg.level_batches && cfg.device_levels) {

@@ -52,6 +52,9 @@ struct PiConfig {
     // FinalStep::Newton with device-resident levels: the final step on
     // device-resident numbers too (pi_scaled_newton_device).
     bool device_final = true;
+    // Device-resident levels: the bottom subtrees of binary splitting on the
+    // device too (gpu_ntt_bs_leaves) instead of host threads.
+    bool device_leaves = true;
 };
 
 struct PiResult {

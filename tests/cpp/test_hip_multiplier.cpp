@@ -304,6 +304,75 @@ int main() {
         check(est_ok, "DecimalDeviceLevelsEstimateWithinOne");
     }
 
+    // Subtrees of 64 terms on the device against the host recursion, at the
+    // start of the series (leaf 0), mid-range, near 2^24 and with partial last
+    // subtrees (identity terms), then whole device-resident trees built on them.
+    {
+        bool ok = true;
+        std::string err;
+        const unsigned long starts[] = {0ul, 1ul, 1000ul, 7000000ul, (1ul << 24) - 300};
+        const unsigned long spans[] = {1ul, 63ul, 64ul, 65ul, 200ul, 4096ul};
+        for (unsigned long a0 : starts)
+            for (unsigned long n : spans) {
+                const unsigned long b0 = std::min(a0 + n, 1ul << 24);
+                const size_t count = (b0 - a0 + 63) / 64;
+                uint64_t* d = nullptr;
+                if (gpu_ntt_dev_alloc(count * 3 * PINTT_BS_LEAF_LIMBS, &d) != 0) {
+                    ok = false;
+                    err = "alloc";
+                    continue;
+                }
+                std::vector<uint32_t> len(3 * count);
+                std::vector<int32_t> sg(2 * count);
+                std::vector<uint64_t> h(count * 3 * PINTT_BS_LEAF_LIMBS);
+                const uint64_t* src = d;
+                uint64_t* hp = h.data();
+                size_t hn = h.size();
+                if (gpu_ntt_bs_leaves(a0, b0, count, d, len.data(), sg.data()) != 0 ||
+                    gpu_ntt_download(1, &hp, &src, &hn) != 0) {
+                    ok = false;
+                    err = gpu_ntt_last_error();
+                }
+                (void)gpu_ntt_dev_free(d);
+                for (size_t i = 0; i < count && ok; ++i) {
+                    pi::BSResult w;
+                    pi::BinarySplitting::compute_small(a0 + 64 * i, std::min(a0 + 64 * (i + 1), b0), w);
+                    mpz_srcptr want[3] = {w.P, w.Q, w.R};
+                    for (int v = 0; v < 3; ++v) {
+                        mpz_t got;
+                        mpz_roinit_n(got, reinterpret_cast<const mp_limb_t*>(h.data() + (3 * i + v) * PINTT_BS_LEAF_LIMBS),
+                                     (mp_size_t)len[3 * i + v]);
+                        const int sign = v == 1 ? 1 : sg[2 * i + (v == 0 ? 0 : 1)];
+                        mpz_t signed_got;
+                        mpz_init_set(signed_got, got);
+                        if (sign < 0) mpz_neg(signed_got, signed_got);
+                        ok = ok && mpz_cmp(signed_got, want[v]) == 0;
+                        mpz_clear(signed_got);
+                    }
+                    if (!ok) err = "subtree " + std::to_string(a0 + 64 * i) + " differs";
+                }
+            }
+        check(ok, "DeviceSubtreesMatchHost" + (err.empty() ? std::string() : " (" + err + ")"));
+        ok = true;
+        pi::HipNttMultiplier m(1);
+        pi::BinarySplitting bs(m, 4);
+        bs.set_device_leaves(true);
+        for (unsigned long b0 : {3ul, 65ul, 4097ul, 30000ul, 200001ul}) {
+            pi::BSResult w, got, kq;
+            pi::BinarySplitting::compute_small(0, b0, w);
+            try {
+                got = bs.compute_device(0, b0, true, 0);
+                bs.compute_device_qr(0, b0, 0)->download(kq);
+                ok = ok && mpz_cmp(got.P, w.P) == 0 && mpz_cmp(got.Q, w.Q) == 0 && mpz_cmp(got.R, w.R) == 0 &&
+                     mpz_cmp(kq.Q, w.Q) == 0 && mpz_cmp(kq.R, w.R) == 0;
+            } catch (const std::exception& ex) {
+                ok = false;
+                err = ex.what();
+            }
+        }
+        check(ok, "DeviceResidentTreeOnDeviceSubtrees" + (err.empty() ? std::string() : " (" + err + ")"));
+    }
+
     // Whole-limb reciprocal on the device: the same integer as its host
     // statement, step for step, for divisors shorter and longer than the
     // precision, 2^(64 L - 1) and all-ones divisors.
