@@ -42,10 +42,20 @@ __global__ void __launch_bounds__(kST) k_schoolbook(const JobList<SchoolJob> job
         S2[j] = s2;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // Column j plus the carry from below (under 2^73), limb j out.
+    // Carries by the first wave: lane l takes K >= 2 consecutive columns and
+    // carries them from zero; then the lanes' carry-outs (< 2^74) run through
+    // lane 0 in order (a carry-in overflows a lane of two or more limbs only if
+    // its limbs above the second are all ones), and each lane adds its own
+    // carry-in.
+    __shared__ uint64_t LC0[64], LC1[64], LR0[64], LR1[64], CI0[65], CI1[65];
+    __shared__ uint32_t LN[64], LONES[64];
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        const uint32_t K = nc > 128 ? (nc + 63) / 64 : 2;
+        const uint32_t j0 = lane * K < nc ? lane * K : nc, j1 = j0 + K < nc ? j0 + K : nc;
         uint64_t c0 = 0, c1 = 0;
-        for (uint32_t j = 0; j < nc; ++j) {
+        uint32_t ones = 1;
+        for (uint32_t j = j0; j < j1; ++j) {
             const uint64_t t0 = S0[j] + c0;
             const uint64_t k0 = t0 < c0 ? 1u : 0u;
             const uint64_t t1 = S1[j] + c1;
@@ -55,8 +65,48 @@ __global__ void __launch_bounds__(kST) k_schoolbook(const JobList<SchoolJob> job
             S0[j] = t0;
             c0 = u1;
             c1 = S2[j] + k1 + k2;
+            if (j >= j0 + 2 && t0 != ~0ull) ones = 0;
         }
-        S0[nc] = c0;  // the product is below 2^(64 (an + bn)): nothing above
+        LC0[lane] = c0;
+        LC1[lane] = c1;
+        LR0[lane] = j1 > j0 ? S0[j0] : 0;
+        LR1[lane] = j1 > j0 + 1 ? S0[j0 + 1] : 0;
+        LN[lane] = j1 - j0;
+        LONES[lane] = ones;
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            uint64_t i0 = 0, i1 = 0;  // carry into lane l
+            for (uint32_t l = 0; l < 64; ++l) {
+                CI0[l] = i0;
+                CI1[l] = i1;
+                const uint32_t n = LN[l];
+                if (n == 0) continue;  // nothing here: the carry passes on
+                const uint64_t s0 = LR0[l] + i0;
+                const uint64_t k = s0 < i0 ? 1u : 0u;
+                uint64_t add;  // what the carry-in adds to the lane's carry-out
+                if (n == 1) {
+                    add = i1 + k;  // below 2^11
+                } else {
+                    const uint64_t s1 = LR1[l] + i1;
+                    const uint64_t kk = (s1 < i1 ? 1u : 0u) + ((s1 + k) < s1 ? 1u : 0u);
+                    add = (LONES[l] && kk) ? 1u : 0u;
+                }
+                i0 = LC0[l] + add;
+                i1 = LC1[l] + (i0 < add ? 1u : 0u);
+            }
+            CI0[64] = i0;
+            CI1[64] = i1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint64_t a0 = CI0[lane], a1 = CI1[lane];  // the lane's carry-in, added to its limbs
+        for (uint32_t j = j0; j < j1 && (a0 | a1); ++j) {
+            const uint64_t t = S0[j] + a0;
+            const uint64_t k = t < a0 ? 1u : 0u;
+            S0[j] = t;
+            a0 = a1 + k;
+            a1 = 0;
+        }
+        if (lane == 0) S0[nc] = CI0[64];  // the product is below 2^(64 (an + bn)): nothing above
     }
     __syncthreads();
     const global_ptr<uint64_t> out = as_global(job.out);
