@@ -146,8 +146,9 @@ int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, cons
 /* The bottom of the Chudnovsky binary-splitting tree on the device: subtree i
  * covers the terms [a + 64 i, a + 64 (i + 1)) (terms at or past b count as
  * the identity P = Q = 1, R = 0), and d_out + 3 * PINTT_BS_LEAF_LIMBS * i
- * receives the magnitudes of its P, Q and R, PINTT_BS_LEAF_LIMBS limbs each
- * (zero-padded); lengths[3 i + 0..2] their exact limb counts, signs[2 i + 0..1]
+ * receives the magnitudes of its P, Q and R in slots of PINTT_BS_LEAF_LIMBS
+ * limbs; lengths[3 i + 0..2] their exact limb counts (limbs past them are
+ * unspecified), signs[2 i + 0..1]
  * the signs of P and R (+1 or -1).  Requires a < b <= 2^24.  Waits.  Replaces
  * the host subtrees of the level-synchronous tree (the reference's recursion,
  * src/engine/binary_splitting.cpp:137-160, at the bottom of the tree). */

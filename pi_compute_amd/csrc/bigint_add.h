@@ -46,7 +46,8 @@ hipError_t launch_decimal_leaves(const LeafJob* jobs, uint32_t njobs, uint8_t* o
 
 // Binary-splitting subtrees (bs_leaves.hip): subtree i covers terms
 // [a + 64 i, a + 64 (i + 1)) with terms at or past b as the identity; out
-// holds its P, Q, R magnitudes (kBsLeafLimbs limbs each, zero-padded), lengths
+// holds its P, Q, R magnitudes (kBsLeafLimbs limbs each, only the first
+// lengths[] of them meaningful), lengths
 // their exact limb counts, signs those of P and R.  b < 2^24.
 constexpr int kBsLeafThreads = 256;
 constexpr int kBsLeafTerms = 64;

@@ -6,8 +6,9 @@
 // Leaf k: P = -(6k-5)(2k-1)(6k-1), Q = k^3 * 10939058860032000,
 // R = P * (13591409 + 545140134 k); leaf 0: P = Q = 1, R = 13591409; terms
 // past the end of the range are the identity P = Q = 1, R = 0.  Merge:
-// P = Pl Pr, Q = Ql Qr, R = Qr Rl + Pl Rr.  Magnitudes in LDS with fixed
-// capacities (3 limbs at a leaf, 2c + 1 after each merge), signs apart: R has
+// P = Pl Pr, Q = Ql Qr, R = Qr Rl + Pl Rr.  Magnitudes in LDS slots of fixed
+// capacity (3 limbs at a leaf, 2c + 1 after each merge) with the limbs in use
+// tracked, so products cost what the operands hold; signs apart: R has
 // the sign of t = Qr Rl, and when t and u = Pl Rr differ in sign |t| >= |u|
 // (the series alternates with terms shrinking ~1.5e14-fold); a borrow out of
 // t - u would contradict that and is reported.  Each level: every column of
@@ -37,7 +38,9 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
                                                   uint32_t* __restrict__ lengths, int32_t* __restrict__ signs,
                                                   uint32_t* __restrict__ error) {
     __shared__ uint64_t nodes[2][kNodeWords];
-    __shared__ int sgn[2][2][kS];  // [buffer][P or R][node]
+    __shared__ int sgn[2][2][kS];             // [buffer][P or R][node]
+    __shared__ uint16_t len[2][3][kS];        // [buffer][P, Q or R][node]: limbs in use
+    __shared__ uint16_t plen[4 * kS / 2];     // limbs of each product of a level
     __shared__ uint64_t C0[kCols], C1[kCols], C2[kCols];
     __shared__ uint64_t prod[kProdWords];
     const int tid = threadIdx.x;
@@ -83,6 +86,12 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
         }
         sgn[0][0][tid] = sp;
         sgn[0][1][tid] = sr;
+        for (int v = 0; v < 3; ++v) {
+            const uint64_t* x = P + v * cap_of(0);
+            int n = cap_of(0);
+            while (n > 1 && x[n - 1] == 0) --n;
+            len[0][v][tid] = (uint16_t)n;
+        }
     }
     __syncthreads();
     int cur = 0;
@@ -91,23 +100,24 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
         const int M = (kS / 2) >> L;  // merges
         const int ncol = 2 * c - 1;   // columns of one product
         const uint64_t* nd = nodes[cur];
-        // Operands of product p of merge m: 0 Pl Pr, 1 Ql Qr, 2 Qr Rl, 3 Pl Rr.
-        auto opnd = [&](int m, int p, int side) -> const uint64_t* {
-            const uint64_t* l = nd + (2 * m) * 3 * c;
-            const uint64_t* r = nd + (2 * m + 1) * 3 * c;
-            switch (p) {
-                case 0: return side ? r : l;
-                case 1: return side ? r + c : l + c;
-                case 2: return side ? l + 2 * c : r + c;
-                default: return side ? r + 2 * c : l;
-            }
+        // Operands of product p of merge m: 0 Pl Pr, 1 Ql Qr, 2 Qr Rl, 3 Pl Rr
+        // (value v of node: 0 P, 1 Q, 2 R).
+        auto which = [](int p, int side, int& node, int& v) {
+            const int nl[4] = {0, 0, 1, 0}, vl[4] = {0, 1, 1, 0};  // side 0: node offset, value
+            const int nr[4] = {1, 1, 0, 1}, vr[4] = {0, 1, 2, 2};  // side 1
+            node = side ? nr[p] : nl[p];
+            v = side ? vr[p] : vl[p];
         };
         for (int idx = tid; idx < M * 4 * ncol; idx += kT) {
             const int m = idx / (4 * ncol), rem = idx % (4 * ncol), p = rem / ncol, j = rem % ncol;
-            const uint64_t* x = opnd(m, p, 0);
-            const uint64_t* y = opnd(m, p, 1);
-            const int lo = j >= c ? j - c + 1 : 0, hi = j < c ? j : c - 1;
+            int nx_, vx, ny_, vy;
+            which(p, 0, nx_, vx);
+            which(p, 1, ny_, vy);
+            const uint64_t* x = nd + (2 * m + nx_) * 3 * c + vx * c;
+            const uint64_t* y = nd + (2 * m + ny_) * 3 * c + vy * c;
+            const int lx = len[cur][vx][2 * m + nx_], ly = len[cur][vy][2 * m + ny_];
             uint64_t s0 = 0, s1 = 0, s2 = 0;
+            const int lo = j >= ly ? j - ly + 1 : 0, hi = j < lx ? j : lx - 1;
             for (int i = lo; i <= hi; ++i) {
                 uint64_t pl, ph;
                 mul64(x[i], y[j - i], pl, ph);
@@ -123,10 +133,15 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
             C2[idx] = s2;
         }
         __syncthreads();
-        for (int q = tid; q < M * 4; q += kT) {  // product q: 2c limbs
+        for (int q = tid; q < M * 4; q += kT) {  // product q: lx + ly <= 2c limbs
+            const int m = q / 4, p = q % 4;
+            int nx_, vx, ny_, vy;
+            which(p, 0, nx_, vx);
+            which(p, 1, ny_, vy);
+            const int n = len[cur][vx][2 * m + nx_] + len[cur][vy][2 * m + ny_] - 1;  // columns in use
             uint64_t c0 = 0, c1 = 0;
             uint64_t* o = prod + q * 2 * c;
-            for (int j = 0; j < ncol; ++j) {
+            for (int j = 0; j < n; ++j) {
                 const int idx = q * ncol + j;
                 const uint64_t t0 = C0[idx] + c0;
                 const uint64_t k0c = t0 < c0 ? 1u : 0u;
@@ -138,7 +153,8 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
                 c0 = u1;
                 c1 = C2[idx] + k1 + k2;
             }
-            o[ncol] = c0;  // the product is below 2^(64 * 2c)
+            o[n] = c0;  // the product is below 2^(64 (lx + ly))
+            plen[q] = (uint16_t)(n + 1);
         }
         __syncthreads();
         uint64_t* nx = nodes[cur ^ 1];
@@ -147,37 +163,46 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
             const uint64_t* qq = prod + (m * 4 + 1) * 2 * c;
             const uint64_t* t = prod + (m * 4 + 2) * 2 * c;
             const uint64_t* u = prod + (m * 4 + 3) * 2 * c;
+            const int lp = plen[m * 4 + 0], lq = plen[m * 4 + 1], lt = plen[m * 4 + 2], lu = plen[m * 4 + 3];
             uint64_t* P = nx + m * 3 * C;
             uint64_t* Q = P + C;
             uint64_t* R = Q + C;
-            for (int i = 0; i < 2 * c; ++i) {
-                P[i] = pp[i];
-                Q[i] = qq[i];
-            }
-            P[2 * c] = 0;
-            Q[2 * c] = 0;
-            const int st = sgn[cur][1][2 * m];                                 // sign of t = Qr Rl
-            const int su = sgn[cur][0][2 * m] * sgn[cur][1][2 * m + 1];       // sign of u = Pl Rr
-            uint64_t carry = 0;
+            for (int i = 0; i < lp; ++i) P[i] = pp[i];
+            for (int i = 0; i < lq; ++i) Q[i] = qq[i];
+            const int st = sgn[cur][1][2 * m];                            // sign of t = Qr Rl
+            const int su = sgn[cur][0][2 * m] * sgn[cur][1][2 * m + 1];  // sign of u = Pl Rr
+            const int lr = lt > lu ? lt : lu;
+            int nr;
             if (st == su) {
-                for (int i = 0; i < 2 * c; ++i) {
-                    const uint64_t s = t[i] + u[i];
+                uint64_t carry = 0;
+                for (int i = 0; i < lr; ++i) {
+                    const uint64_t a = i < lt ? t[i] : 0, b = i < lu ? u[i] : 0;
+                    const uint64_t s = a + b;
                     const uint64_t r = s + carry;
-                    carry = (s < t[i] || r < s) ? 1u : 0u;
+                    carry = (s < a || r < s) ? 1u : 0u;
                     R[i] = r;
                 }
-                R[2 * c] = carry;
+                R[lr] = carry;
+                nr = lr + 1;
             } else {
                 uint64_t borrow = 0;
-                for (int i = 0; i < 2 * c; ++i) {
-                    const uint64_t d = t[i] - u[i];
+                for (int i = 0; i < lr; ++i) {
+                    const uint64_t a = i < lt ? t[i] : 0, b = i < lu ? u[i] : 0;
+                    const uint64_t d = a - b;
                     const uint64_t r = d - borrow;
-                    borrow = (t[i] < u[i] || d < borrow) ? 1u : 0u;
+                    borrow = (a < b || d < borrow) ? 1u : 0u;
                     R[i] = r;
                 }
-                R[2 * c] = 0;
+                nr = lr;
                 if (borrow) atomicOr(error, 1u);  // |t| < |u|: the sign rule does not hold
             }
+            int np = lp, nq = lq;
+            while (np > 1 && P[np - 1] == 0) --np;
+            while (nq > 1 && Q[nq - 1] == 0) --nq;
+            while (nr > 1 && R[nr - 1] == 0) --nr;
+            len[cur ^ 1][0][m] = (uint16_t)np;
+            len[cur ^ 1][1][m] = (uint16_t)nq;
+            len[cur ^ 1][2][m] = (uint16_t)nr;
             sgn[cur ^ 1][0][m] = sgn[cur][0][2 * m] * sgn[cur][0][2 * m + 1];
             sgn[cur ^ 1][1][m] = st;
         }
@@ -189,11 +214,7 @@ __global__ void __launch_bounds__(kT) k_bs_leaves(uint64_t a, uint64_t b, uint64
     const uint64_t* nd = nodes[cur];
     uint64_t* o = out + (size_t)blockIdx.x * 3 * cf;
     for (int i = tid; i < 3 * cf; i += kT) o[i] = nd[i];
-    if (tid < 3) {
-        int n = cf;
-        while (n > 1 && nd[tid * cf + n - 1] == 0) --n;
-        lengths[blockIdx.x * 3 + tid] = (uint32_t)n;
-    }
+    if (tid < 3) lengths[blockIdx.x * 3 + tid] = len[cur][tid][0];
     if (tid == 0) {
         signs[blockIdx.x * 2 + 0] = sgn[cur][0][0];
         signs[blockIdx.x * 2 + 1] = sgn[cur][1][0];
