@@ -262,17 +262,23 @@ int ensure_carry_state(size_t blocks) {
     return PINTT_OK;
 }
 
-// Ring slot `k` with room for `bytes`, its previous upload finished.
+// Ring slot `k` with room for `bytes`, its previous upload finished.  A slot
+// that must grow makes every smaller slot grow with it, with half again as much
+// room: the stream drains once, not once per slot as batches get larger.
 int ensure_descriptors(int k, size_t bytes) {
     PINTT_HIP(hipEventSynchronize(g.desc_done[k]));
     if (bytes <= g.desc_cap[k]) return PINTT_OK;
     PINTT_HIP(hipStreamSynchronize(g.stream));
-    if (g.desc_host[k]) PINTT_HIP(hipHostFree(g.desc_host[k]));
-    g.desc_host[k] = nullptr;
-    size_t cap = std::max(bytes, (size_t)1 << 16);
-    PINTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&g.desc_host[k]), cap, hipHostMallocDefault));
-    PINTT_TRY(grow(g.desc_dev[k], cap));
-    g.desc_cap[k] = cap;
+    const size_t cap = std::max(bytes + bytes / 2, (size_t)1 << 16);
+    for (int j = 0; j < Engine::kDescRing; ++j) {
+        if (g.desc_cap[j] >= cap) continue;
+        if (g.desc_host[j]) PINTT_HIP(hipHostFree(g.desc_host[j]));
+        g.desc_host[j] = nullptr;
+        g.desc_cap[j] = 0;
+        PINTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&g.desc_host[j]), cap, hipHostMallocDefault));
+        PINTT_TRY(grow(g.desc_dev[j], cap));
+        g.desc_cap[j] = cap;
+    }
     return PINTT_OK;
 }
 
