@@ -59,6 +59,7 @@ struct Engine {
     uint32_t* tw_pow = nullptr;  // powers of the root while tables are built
     uint2* tw_fwd[kSets][kPrimes] = {};
     uint2* tw_inv[kSets][kPrimes] = {};
+    uint32_t* tw_row[kSets][kPrimes] = {};  // row factor tables
     size_t arena_words = 0;
     uint32_t* arena = nullptr;
     size_t maps_cap = 0;
@@ -239,7 +240,9 @@ int ensure_twiddles(int log2n) {
     for (int i = 0; i < kPrimes; ++i) {
         PINTT_TRY(grow(g.tw_fwd[set][i], cnt));
         PINTT_TRY(grow(g.tw_inv[set][i], cnt));
-        PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], g.tw_pow, set, i, g.stream));
+        PINTT_TRY(grow(g.tw_row[set][i], row_table_words(set)));
+        PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], g.tw_row[set][i], g.tw_pow, set, i,
+                                        g.stream));
     }
     g.tw_ready[set] = true;
     return PINTT_OK;
@@ -294,6 +297,7 @@ Twiddles twiddles(int set) {
     for (int i = 0; i < kPrimes; ++i) {
         t.fwd[i] = g.tw_fwd[set][i];
         t.inv[i] = g.tw_inv[set][i];
+        t.row[i] = g.tw_row[set][i];
     }
     return t;
 }
@@ -1014,6 +1018,7 @@ void gpu_ntt_shutdown(void) {
         for (int i = 0; i < kPrimes; ++i) {
             (void)hipFree(g.tw_fwd[set][i]);
             (void)hipFree(g.tw_inv[set][i]);
+            (void)hipFree(g.tw_row[set][i]);
         }
     (void)hipFree(g.arena);
     (void)hipFree(g.maps);

@@ -351,9 +351,22 @@ __global__ void k_build_twiddles(uint2* __restrict__ fwd, uint2* __restrict__ in
     inv[i] = make_uint2(zi, (uint32_t)(((uint64_t)zi << 32) / p));
 }
 
+// Row factor table: lo[i] = wK^i 2^32 mod p (i < 2^12), then hi[i] =
+// (wK^(i 2^12), floor(wK^(i 2^12) 2^32 / p)) for i < 2^(K-12).
+__global__ void k_row_factor_table(uint32_t* __restrict__ row, const uint32_t* __restrict__ lo,
+                                   const uint32_t* __restrict__ hi, uint32_t p, uint32_t nhi) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kPowLo) row[i] = (uint32_t)(((uint64_t)lo[i] << 32) % p);
+    if (i < nhi) {
+        uint2* h = reinterpret_cast<uint2*>(row + kPowLo);
+        h[i] = make_uint2(hi[i], (uint32_t)(((uint64_t)hi[i] << 32) / p));
+    }
+}
+
 }  // namespace
 
-hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* pow_scratch, int set, int prime, hipStream_t s) {
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* row, uint32_t* pow_scratch, int set, int prime,
+                                 hipStream_t s) {
     const uint32_t p = kPrimeSet[set][prime];
     const int log2k = kSetMaxLog2N[set];
     // wK = g^((p-1)/2^K).
@@ -371,6 +384,10 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* pow_scratch, 
     hipLaunchKernelGGL(k_power_tables, dim3((nt + 255) / 256), dim3(256), 0, s, lo, hi, p, (uint32_t)r, nhi);
     const hipError_t e1 = hipGetLastError();
     if (e1 != hipSuccess) return e1;
+    const uint32_t nrow = 1u << (log2k - kPowLoBits);
+    hipLaunchKernelGGL(k_row_factor_table, dim3((nt + 255) / 256), dim3(256), 0, s, row, lo, hi, p, nrow);
+    const hipError_t e2 = hipGetLastError();
+    if (e2 != hipSuccess) return e2;
     uint32_t count = 1u << log2k;
     hipLaunchKernelGGL(k_build_twiddles, dim3((count + 255) / 256), dim3(256), 0, s, fwd, inv, lo, hi, p, log2k,
                        count);

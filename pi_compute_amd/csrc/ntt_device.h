@@ -44,6 +44,28 @@ __device__ __forceinline__ uint2 tw_at(const uint2* z, uint32_t i) { return z[i]
 #define PINTT_FWD_TABLE(tw, prime) pick((tw).fwd[0], (tw).fwd[1], (tw).fwd[2], (prime))
 #define PINTT_INV_TABLE(tw, prime) pick((tw).inv[0], (tw).inv[1], (tw).inv[2], (prime))
 
+// Row factors.  Of a transform of 2^n points as 2^Lr rows of 2^Lc elements,
+// row r holds at global layer Lr + m the blocks r 2^m + j (j < 2^m), whose
+// twiddles factor as Z[2^(Lr+m) + r 2^m + j] = Z[2^m + j] * a_r^(2^(Lc-1-m)),
+// a_r = w_(2^n)^bitrev_Lr(r).  Layer m pairs elements 2^(Lc-1-m) apart, so
+// scaling element c of the row by a_r^c first makes the row's layers those of
+// block 0 (twiddles shared by every row: sb = row0 = 0 below), and the inverse
+// layers of block 0 followed by a scaling by a_r^-c undo them.  a_r^c = wK^e,
+// wK the set's 2^K-th root, e = bitrev_Lr(r) c 2^(K-n) < 2^K, comes from two
+// tables: lo[i] = wK^i 2^32 mod p (i < 2^12, Montgomery form) and, after it,
+// hi[i] = (wK^(i 2^12), its Shoup quotient); one Shoup product gives wK^e in
+// Montgomery form, ready for mul_mont.
+using RowTable = global_ptr<const uint32_t>;
+__device__ __forceinline__ RowTable pick_row(const uint32_t* t0, const uint32_t* t1, const uint32_t* t2, int prime) {
+    return (RowTable)(prime == 0 ? t0 : (prime == 1 ? t1 : t2));
+}
+__device__ __forceinline__ uint32_t row_factor(RowTable t, uint32_t e, uint32_t p) {
+    const uint32_t lo = t[e & (kPowLo - 1)];
+    const unsigned long long h = ((global_ptr<const unsigned long long>)(t + kPowLo))[e >> kPowLoBits];
+    return mul_shoup(lo, (uint32_t)h, (uint32_t)(h >> 32), p);
+}
+#define PINTT_ROW_TABLE(tw, prime) pick_row((tw).row[0], (tw).row[1], (tw).row[2], (prime))
+
 // One pad word after every 16 keeps radix-16 strides off a single LDS bank.
 __device__ __forceinline__ uint32_t pad(uint32_t x) { return x + (x >> 4); }
 

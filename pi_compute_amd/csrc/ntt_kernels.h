@@ -30,6 +30,7 @@ namespace pintt {
 struct Twiddles {
     const uint2* fwd[kPrimes];  // (w, floor(w 2^32 / p)) in Cooley-Tukey order
     const uint2* inv[kPrimes];  // inverses, same indexing
+    const uint32_t* row[kPrimes];  // row factor tables (lo, then hi pairs; ntt_device.h)
 };
 
 // Constants for Garner's CRT at one transform length N.  The residues coming
@@ -196,7 +197,12 @@ struct SchoolJob {
 constexpr int kPowLoBits = 12;
 constexpr uint32_t kPowLo = 1u << kPowLoBits;
 constexpr uint32_t kPowScratch = kPowLo + (1u << (kMaxLog2N - kPowLoBits)) + 1;
-hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* pow_scratch, int set, int prime, hipStream_t s);
+// Words of the row factor table of one prime of a set.
+__host__ __device__ constexpr uint32_t row_table_words(int set) {
+    return kPowLo + 2u * (1u << (kSetMaxLog2N[set] - kPowLoBits));
+}
+hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* row, uint32_t* pow_scratch, int set, int prime,
+                                 hipStream_t s);
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,

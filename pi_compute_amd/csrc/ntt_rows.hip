@@ -9,6 +9,15 @@ namespace {
 // they are already done), Montgomery pointwise products, inverse row layers,
 // written back in place.  grid = (row tiles, prime, job).  SET: the prime set
 // of the transform length.
+// Twiddles: a row's own twiddles are the table entries of its blocks (one
+// (w, w') pair per element and direction, read from HBM).  Workgroups of one
+// row that transform one tile (squares, later chunks against b's stored
+// rows) instead scale the row by a_r^c as it is loaded and by a_r^-c as it is
+// stored and run the layers of block 0, whose twiddles every workgroup shares
+// (row_factor): that took the rows of a 2^21-point square from 71.6 to 61.7 us
+// on MI355X, while the paired transforms (three tiles, two waves) and tiles
+// of several rows got slower with it.  Transformed rows of b, stored for
+// later chunks or read back, are never scaled.
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
 // ~250 registers on the paired transforms and halves occupancy.  The pair
 // kind (three tiles in flight) gets two waves and 256 registers instead.
@@ -17,6 +26,7 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     k_row_conv(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
     using TL = Tile<LOGC, LOGROWS, false>;
     constexpr bool kPair = KIND == kRowPair;
+    constexpr bool kScaled = LOGROWS == 0 && (KIND == kRowSquare || KIND == kRowReuseB);
     __shared__ uint32_t ldsA[TL::WORDS];
     __shared__ uint32_t ldsB[KIND == kRowSquare ? 1 : TL::WORDS];
     __shared__ uint32_t ldsC[kPair ? TL::WORDS : 1];
@@ -29,24 +39,49 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     global_ptr<uint32_t> a = as_global(job.a) + base;
     global_ptr<uint32_t> b = as_global(job.b) + base;
     global_ptr<uint32_t> a2 = as_global(job.a2) + base;
+    // Scaled rows: element e of a thread sits at c = e NT + tid, and
+    // a_r^c = a_r^tid * a_r^(e NT), the second factor the same for the whole
+    // workgroup.  Exponents bitrev_log2r(r) c 2^(K - n) stay below 2^K.
+    const RowTable rt = PINTT_ROW_TABLE(tw, prime);
+    constexpr uint32_t kMask = (1u << kSetMaxLog2N[SET]) - 1;
+    const uint32_t rbits = __brev(row0) >> (32 - log2r);
+    const int eshift = kSetMaxLog2N[SET] - (log2r + LOGC);
+    uint32_t ft_fwd = 0, ft_inv = 0;
+    if constexpr (kScaled) {
+        const uint32_t et = (rbits * threadIdx.x) << eshift;
+        ft_fwd = row_factor(rt, et, p);
+        ft_inv = row_factor(rt, (0u - et) & kMask, p);
+    }
+    auto factor = [&](int e, bool inverse) {
+        const uint32_t eu = (rbits * (uint32_t)(e * TL::NT)) << eshift;
+        const uint32_t fu = row_factor(rt, inverse ? (0u - eu) & kMask : eu, p);
+        return mul_mont(fu, inverse ? ft_inv : ft_fwd, p, pinv);
+    };
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
-        ldsA[pad(idx)] = a[idx];
+        if constexpr (kScaled) {
+            ldsA[pad(idx)] = mul_mont(a[idx], factor(e, false), p, pinv);
+        } else {
+            ldsA[pad(idx)] = a[idx];
+        }
         if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
         if constexpr (kPair) ldsC[pad(idx)] = a2[idx];
     }
     __syncthreads();
+    // Scaled rows run block 0's layers: sb = row0 = 0.
+    const int sb = kScaled ? 0 : log2r;
+    const uint32_t rb = kScaled ? 0u : row0;
     const auto zf = PINTT_FWD_TABLE(tw, prime);
     if constexpr (kPair) {
         uint32_t* const t[3] = {ldsA, ldsB, ldsC};
-        ct_run<SET, TL, 0, 3>(t, zf, log2r, row0, p);
+        ct_run<SET, TL, 0, 3>(t, zf, sb, rb, p);
     } else if constexpr (KIND == kRowTransformB) {
         uint32_t* const t[2] = {ldsA, ldsB};
-        ct_run<SET, TL, 0, 2>(t, zf, log2r, row0, p);
+        ct_run<SET, TL, 0, 2>(t, zf, sb, rb, p);
     } else {
         uint32_t* const t[1] = {ldsA};
-        ct_run<SET, TL, 0, 1>(t, zf, log2r, row0, p);
+        ct_run<SET, TL, 0, 1>(t, zf, sb, rb, p);
     }
     if constexpr (KIND == kRowTransformB || kPair) {
         if (job.write_b) {
@@ -69,15 +104,19 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     const auto zi = PINTT_INV_TABLE(tw, prime);
     if constexpr (kPair) {
         uint32_t* const t[2] = {ldsA, ldsC};
-        gs_run<SET, TL, 0, 2>(t, zi, log2r, row0, p);
+        gs_run<SET, TL, 0, 2>(t, zi, sb, rb, p);
     } else {
         uint32_t* const t[1] = {ldsA};
-        gs_run<SET, TL, 0, 1>(t, zi, log2r, row0, p);
+        gs_run<SET, TL, 0, 1>(t, zi, sb, rb, p);
     }
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
-        a[idx] = ldsA[pad(idx)];
+        if constexpr (kScaled) {
+            a[idx] = mul_mont(ldsA[pad(idx)], factor(e, true), p, pinv);
+        } else {
+            a[idx] = ldsA[pad(idx)];
+        }
         if constexpr (kPair) a2[idx] = ldsC[pad(idx)];
     }
 }
