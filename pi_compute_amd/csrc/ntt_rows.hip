@@ -10,23 +10,28 @@ namespace {
 // written back in place.  grid = (row tiles, prime, job).  SET: the prime set
 // of the transform length.
 // Twiddles: a row's own twiddles are the table entries of its blocks (one
-// (w, w') pair per element and direction, read from HBM).  Workgroups of one
-// row that transform one tile (squares, later chunks against b's stored
+// (w, w') pair per element and direction: N pairs per direction in all,
+// which outgrow the caches from 2^22 points on).  SCALED kernels (one row per
+// workgroup; squares, one-chunk products and later chunks against b's stored
 // rows) instead scale the row by a_r^c as it is loaded and by a_r^-c as it is
 // stored and run the layers of block 0, whose twiddles every workgroup shares
-// (row_factor): that took the rows of a 2^21-point square from 71.6 to 61.7 us
-// on MI355X, while the paired transforms (three tiles, two waves) and tiles
-// of several rows got slower with it.  Transformed rows of b, stored for
-// later chunks or read back, are never scaled.
+// (row_factor).  Measured on MI355X, row pass time of squares and one-chunk
+// products: 12-17% lower at 2^22 points, 44-48% lower at 2^25, unchanged at
+// 2^21, 15-30% higher at 2^17..2^20; the paired kernel (three tiles, two
+// waves) was slower with it at 2^21.  launch_row_convolution picks SCALED
+// from 2^22 points on.
+// Transformed rows of b, stored for later chunks or read back, are never
+// scaled.
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
 // ~250 registers on the paired transforms and halves occupancy.  The pair
 // kind (three tiles in flight) gets two waves and 256 registers instead.
-template <int LOGC, int LOGROWS, int KIND, int SET>
+template <int LOGC, int LOGROWS, int KIND, int SET, bool SCALED>
 __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute__((amdgpu_waves_per_eu(KIND == kRowPair ? 2 : 3)))
     k_row_conv(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
     using TL = Tile<LOGC, LOGROWS, false>;
     constexpr bool kPair = KIND == kRowPair;
-    constexpr bool kScaled = LOGROWS == 0 && (KIND == kRowSquare || KIND == kRowReuseB);
+    constexpr bool kScaled = SCALED;
+    static_assert(!SCALED || (LOGROWS == 0 && !kPair), "scaled rows: one row per workgroup, one or two tiles");
     __shared__ uint32_t ldsA[TL::WORDS];
     __shared__ uint32_t ldsB[KIND == kRowSquare ? 1 : TL::WORDS];
     __shared__ uint32_t ldsC[kPair ? TL::WORDS : 1];
@@ -61,11 +66,14 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         if constexpr (kScaled) {
-            ldsA[pad(idx)] = mul_mont(a[idx], factor(e, false), p, pinv);
+            const uint32_t f = factor(e, false);
+            ldsA[pad(idx)] = mul_mont(a[idx], f, p, pinv);
+            if constexpr (KIND == kRowTransformB) ldsB[pad(idx)] = mul_mont(b[idx], f, p, pinv);
+            if constexpr (KIND == kRowReuseB) ldsB[pad(idx)] = b[idx];
         } else {
             ldsA[pad(idx)] = a[idx];
+            if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
         }
-        if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
         if constexpr (kPair) ldsC[pad(idx)] = a2[idx];
     }
     __syncthreads();
@@ -121,6 +129,22 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     }
 }
 
+// Scaled rows from this transform length on.
+constexpr int kScaledRowsMinLog2N = 22;
+
+template <int LC, int LROWS, int K, int SET>
+void launch_rows(const Plan& pl, dim3 grid, const JobList<RowJob>& jobs, const Twiddles& tw, hipStream_t s) {
+    if constexpr (LROWS == 0 && K != kRowPair) {
+        if (pl.log2n >= kScaledRowsMinLog2N) {
+            hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET, true>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s,
+                               jobs, pl.log2r, tw);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET, false>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s, jobs,
+                       pl.log2r, tw);
+}
+
 }  // namespace
 
 // Tile shapes per prime set.  The wide set only serves 2^23 .. 2^25 points,
@@ -167,9 +191,7 @@ bool pair_case_available(int set, int log2c, int log2rows) {
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,
                                   const Twiddles& tw, hipStream_t s) {
     dim3 grid(1u << (pl.log2r - pl.log2rows), kPrimes, njobs);
-#define PINTT_LAUNCH(LC, LROWS, K, SET)                                                                \
-    hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s, \
-                       jobs, pl.log2r, tw)
+#define PINTT_LAUNCH(LC, LROWS, K, SET) launch_rows<LC, LROWS, K, SET>(pl, grid, jobs, tw, s)
 #define PINTT_CASE_SET(LC, LROWS, SET)                                                \
     if (pl.log2c == LC && pl.log2rows == LROWS) {                                     \
         if (kind == kRowSquare) PINTT_LAUNCH(LC, LROWS, kRowSquare, SET);             \

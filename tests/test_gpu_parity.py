@@ -219,6 +219,24 @@ def test_split_products(engine, oracle, chunks, an, bn):
     assert np.array_equal(engine.mul(a, b), want), "automatic choice"
 
 
+@pytest.mark.parametrize("chunks", [3, 4])
+@pytest.mark.parametrize("an,bn", [(2400000, 600000), (4800000, 1200000)])
+def test_split_products_scaled_rows(engine, oracle, chunks, an, bn):
+    """Chunks of 2^22 and 2^23 points: later chunks reuse b's stored rows in
+    the row kernels that scale each row (one row per workgroup)."""
+    a, b = pc.operands(an - bn, 64 * an, 64 * bn)
+    want = oracle.gmp_mul(a, b)
+    engine.set_split(chunks)
+    try:
+        assert engine.product_plan(an, bn)["chunks"] == chunks
+        assert engine.product_plan(an, bn)["log2n"] >= 22
+        assert np.array_equal(engine.mul(a, b), want), f"{chunks} chunks"
+    finally:
+        engine.set_split(1)
+    assert np.array_equal(engine.mul(a, b), want), "automatic choice"
+    assert np.array_equal(engine.sqr(b), oracle.gmp_mul(b, b)), "square"
+
+
 def test_split_all_ones_carry_chains(engine, oracle):
     # Overlapping chunk products whose sums ripple carries across chunk borders.
     a = np.full(50000, np.uint64(0xFFFFFFFFFFFFFFFF))
