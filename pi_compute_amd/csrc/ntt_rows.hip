@@ -17,9 +17,10 @@ namespace {
 // stored and run the layers of block 0, whose twiddles every workgroup shares
 // (row_factor).  Measured on MI355X, row pass time of squares and one-chunk
 // products: 12-17% lower at 2^22 points, 44-48% lower at 2^25, unchanged at
-// 2^21, 15-30% higher at 2^17..2^20; the paired kernel (three tiles, two
-// waves) was slower with it at 2^21.  launch_row_convolution picks SCALED
-// from 2^22 points on.
+// 2^21, 15-30% higher at 2^17..2^20.  The paired kernel (three tiles, two
+// waves): 4% higher at 2^21 and 2^24, 2% lower at 2^23, 8% lower at 2^22.
+// launch_row_convolution picks SCALED from 2^22 points on, for pairs at 2^22
+// only.
 // Transformed rows of b, stored for later chunks or read back, are never
 // scaled.
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
@@ -31,7 +32,7 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     using TL = Tile<LOGC, LOGROWS, false>;
     constexpr bool kPair = KIND == kRowPair;
     constexpr bool kScaled = SCALED;
-    static_assert(!SCALED || (LOGROWS == 0 && !kPair), "scaled rows: one row per workgroup, one or two tiles");
+    static_assert(!SCALED || LOGROWS == 0, "scaled rows: one row per workgroup");
     __shared__ uint32_t ldsA[TL::WORDS];
     __shared__ uint32_t ldsB[KIND == kRowSquare ? 1 : TL::WORDS];
     __shared__ uint32_t ldsC[kPair ? TL::WORDS : 1];
@@ -68,13 +69,14 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         if constexpr (kScaled) {
             const uint32_t f = factor(e, false);
             ldsA[pad(idx)] = mul_mont(a[idx], f, p, pinv);
-            if constexpr (KIND == kRowTransformB) ldsB[pad(idx)] = mul_mont(b[idx], f, p, pinv);
+            if constexpr (KIND == kRowTransformB || kPair) ldsB[pad(idx)] = mul_mont(b[idx], f, p, pinv);
             if constexpr (KIND == kRowReuseB) ldsB[pad(idx)] = b[idx];
+            if constexpr (kPair) ldsC[pad(idx)] = mul_mont(a2[idx], f, p, pinv);
         } else {
             ldsA[pad(idx)] = a[idx];
             if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
+            if constexpr (kPair) ldsC[pad(idx)] = a2[idx];
         }
-        if constexpr (kPair) ldsC[pad(idx)] = a2[idx];
     }
     __syncthreads();
     // Scaled rows run block 0's layers: sb = row0 = 0.
@@ -121,11 +123,13 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         if constexpr (kScaled) {
-            a[idx] = mul_mont(ldsA[pad(idx)], factor(e, true), p, pinv);
+            const uint32_t f = factor(e, true);
+            a[idx] = mul_mont(ldsA[pad(idx)], f, p, pinv);
+            if constexpr (kPair) a2[idx] = mul_mont(ldsC[pad(idx)], f, p, pinv);
         } else {
             a[idx] = ldsA[pad(idx)];
+            if constexpr (kPair) a2[idx] = ldsC[pad(idx)];
         }
-        if constexpr (kPair) a2[idx] = ldsC[pad(idx)];
     }
 }
 
@@ -134,8 +138,10 @@ constexpr int kScaledRowsMinLog2N = 22;
 
 template <int LC, int LROWS, int K, int SET>
 void launch_rows(const Plan& pl, dim3 grid, const JobList<RowJob>& jobs, const Twiddles& tw, hipStream_t s) {
-    if constexpr (LROWS == 0 && K != kRowPair) {
-        if (pl.log2n >= kScaledRowsMinLog2N) {
+    if constexpr (LROWS == 0) {
+        // Pairs only at 2^22 points, the narrow set's longest transform.
+        const bool scaled = pl.log2n >= kScaledRowsMinLog2N && (K != kRowPair || SET == kSetNarrow);
+        if (scaled) {
             hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET, true>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s,
                                jobs, pl.log2r, tw);
             return;
