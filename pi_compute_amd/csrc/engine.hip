@@ -55,6 +55,10 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;  // end of the last batch
     hipStream_t last = nullptr; // stream that batch ran on
+    // Work enqueued on the library stream outside product batches (twiddle
+    // builds, copies, additions) since a caller stream last waited for it.
+    bool lib_pending = false;
+    hipEvent_t lib_mark = nullptr;
     bool tw_ready[kSets] = {};
     uint32_t* tw_pow = nullptr;  // powers of the root while tables are built
     uint2* tw_fwd[kSets][kPrimes] = {};
@@ -203,6 +207,7 @@ int ensure_init(int device) {
                                              ", the kernels are built for gfx950 only");
     PINTT_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     PINTT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    PINTT_HIP(hipEventCreateWithFlags(&g.lib_mark, hipEventDisableTiming));
     for (auto& e : g.desc_done) PINTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : g.ev) PINTT_HIP(hipEventCreate(&e));
     const char* at = std::getenv("PINTT_AUTOTUNE");
@@ -213,6 +218,21 @@ int ensure_init(int device) {
     g.pair_rows = !(pr && pr[0] == '0');
     g.device = dev;
     g.ready = true;
+    return PINTT_OK;
+}
+
+// Device work enqueued on the library stream that products on other streams
+// must follow: the first use of the twiddle tables, device copies, enqueued
+// additions.  (Product batches on the library stream order themselves through
+// `done`.)
+void library_work_enqueued() { g.lib_pending = true; }
+
+// Make stream `s` wait for the library stream's pending work, if any.
+int order_after_library(hipStream_t s) {
+    if (s == g.stream || !g.lib_pending) return PINTT_OK;
+    PINTT_HIP(hipEventRecord(g.lib_mark, g.stream));
+    PINTT_HIP(hipStreamWaitEvent(s, g.lib_mark, 0));
+    g.lib_pending = false;
     return PINTT_OK;
 }
 
@@ -242,8 +262,9 @@ int ensure_twiddles(int log2n) {
         PINTT_TRY(grow(g.tw_inv[set][i], cnt));
         PINTT_TRY(grow(g.tw_row[set][i], row_table_words(set)));
         PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], g.tw_row[set][i], g.tw_pow, set, i,
-                                        g.stream));
+                                        kSetMaxLog2N[set], g.stream));
     }
+    library_work_enqueued();  // a product on a caller stream waits for the tables
     g.tw_ready[set] = true;
     return PINTT_OK;
 }
@@ -662,8 +683,10 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
         // Every job list rides in kernel arguments: nothing to upload.
         std::vector<unsigned char> H(bp.bytes);
         fill_descriptors(bp, H.data(), g.arena, g.maps);
-        // Ordered after the previous batch (arena and carry maps are shared).
+        // Ordered after the previous batch (arena and carry maps are shared)
+        // and after the library stream's own work (tables built at first use).
         if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+        PINTT_TRY(order_after_library(s));
         PINTT_TRY(enqueue_launches(bp, nullptr, H.data(), s, g.timing));
     } else {
         const int ring = g.desc_next;
@@ -673,8 +696,11 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
         PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bp.bytes, hipMemcpyHostToDevice, g.stream));
         PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
         // A caller stream first waits for the upload, which itself is ordered
-        // after the previous batch on the library stream.
-        if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
+        // after the previous batch and all other work on the library stream.
+        if (s != g.stream) {
+            PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
+            g.lib_pending = false;
+        }
         PINTT_TRY(enqueue_launches(bp, g.desc_dev[ring], g.desc_host[ring], s, g.timing));
     }
     if (g.timing) g.have_times = true;
@@ -1051,6 +1077,7 @@ void gpu_ntt_shutdown(void) {
     (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(g.done);
+    (void)hipEventDestroy(g.lib_mark);
     (void)hipStreamDestroy(g.stream);
     g = Engine{};
 }
@@ -1120,8 +1147,9 @@ int gpu_ntt_graph_launch(void* graph, void* stream) {
     PINTT_TRY(ensure_init(-1));
     BatchGraph* bg = static_cast<BatchGraph*>(graph);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
-    // Serialised with every other batch on the device.
+    // Serialised with every other batch on the device, after the tables.
     if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+    PINTT_TRY(order_after_library(s));
     PINTT_HIP(hipGraphLaunch(bg->exec, s));
     PINTT_HIP(hipEventRecord(g.done, s));
     if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
@@ -1247,6 +1275,7 @@ int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs) {
     std::lock_guard<std::mutex> lock(g_mu);
     PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipMemcpyAsync(d_dst, d_src, limbs * sizeof(uint64_t), hipMemcpyDeviceToDevice, g.stream));
+    library_work_enqueued();
     return PINTT_OK;
 }
 
@@ -1390,6 +1419,7 @@ int gpu_ntt_add_batch_device_async(size_t count, const uint64_t* const* d_x, con
     PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
     PINTT_HIP(launch_add_batch(reinterpret_cast<const AddJob*>(g.desc_dev[ring]), (uint32_t)count, (uint32_t)blocks,
                                g.add_maps, g.add_carry, d_negative, g.add_status, g.stream));
+    library_work_enqueued();
     return PINTT_OK;
 }
 

@@ -366,9 +366,10 @@ __global__ void k_row_factor_table(uint32_t* __restrict__ row, const uint32_t* _
 }  // namespace
 
 hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* row, uint32_t* pow_scratch, int set, int prime,
-                                 hipStream_t s) {
+                                 int log2count, hipStream_t s) {
     const uint32_t p = kPrimeSet[set][prime];
     const int log2k = kSetMaxLog2N[set];
+    if (log2count < 0 || log2count > log2k) return hipErrorInvalidValue;
     // wK = g^((p-1)/2^K).
     uint64_t r = 1, x = kGenSet[set][prime];
     uint64_t e = (uint64_t)(p - 1) >> log2k;
@@ -388,7 +389,7 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* row, uint32_t
     hipLaunchKernelGGL(k_row_factor_table, dim3((nt + 255) / 256), dim3(256), 0, s, row, lo, hi, p, nrow);
     const hipError_t e2 = hipGetLastError();
     if (e2 != hipSuccess) return e2;
-    uint32_t count = 1u << log2k;
+    uint32_t count = 1u << log2count;
     hipLaunchKernelGGL(k_build_twiddles, dim3((count + 255) / 256), dim3(256), 0, s, fwd, inv, lo, hi, p, log2k,
                        count);
     return hipGetLastError();

@@ -192,8 +192,11 @@ struct SchoolJob {
 
 // Launch entry points (ntt_cols.hip, ntt_rows.hip, ntt_carry.hip,
 // schoolbook.hip); counts are host values.
-// Tables of 2^kSetMaxLog2N[set] entries for one prime of a set.  pow_scratch:
-// device room for kPowScratch words (powers of the root, used on the stream).
+// Tables for one prime of a set: the first 2^log2count entries (log2count <=
+// kSetMaxLog2N[set]) of the forward and inverse tables, which are the whole
+// tables of every transform of at most 2^log2count points, and the full row
+// factor table.  pow_scratch: device room for kPowScratch words (powers of the
+// root, used on the stream).
 constexpr int kPowLoBits = 12;
 constexpr uint32_t kPowLo = 1u << kPowLoBits;
 constexpr uint32_t kPowScratch = kPowLo + (1u << (kMaxLog2N - kPowLoBits)) + 1;
@@ -202,7 +205,7 @@ __host__ __device__ constexpr uint32_t row_table_words(int set) {
     return kPowLo + 2u * (1u << (kSetMaxLog2N[set] - kPowLoBits));
 }
 hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* row, uint32_t* pow_scratch, int set, int prime,
-                                 hipStream_t s);
+                                 int log2count, hipStream_t s);
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s);
 hipError_t launch_row_convolution(const Plan& pl, const JobList<RowJob>& jobs, uint32_t njobs, RowKind kind,

@@ -5,10 +5,16 @@ Mirrors tests/test_gpu_ntt_multiplier.cpp of the reference (every check is
 operands at every transform length the engine plans (2^10 .. 2^25), squares,
 adversarial carry chains and the device-pointer entry point.
 """
+import subprocess
+import sys
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 import pi_compute_amd as pc
+
+ROOT = Path(__file__).resolve().parent.parent
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +75,22 @@ def test_largest_transform(engine, oracle):
     a, b = pc.operands(25, 64 * (7 << 20), 64 * ((1 << 20) + 1))
     assert engine.transform_log2(a.size, b.size) == 25
     _check(engine, oracle, a, b, "n=2^25")
+    # The square of 2^23 limbs: a 2^25-point square (wide primes, scaled rows).
+    s = pc.operands(2525, 64 * (1 << 23), 64)[0]
+    assert engine.transform_log2(s.size, s.size) == 25
+    assert np.array_equal(engine.sqr(s), oracle.gmp_mul(s, s)), "square n=2^25"
+
+
+@pytest.mark.parametrize("mode", ["product", "graph"])
+def test_first_product_on_caller_stream(mode):
+    """A fresh process with the shipped tile plans loaded (nothing tunes) makes
+    its first product a 2^23-point one on its own stream (or a graph replayed
+    there): the twiddle tables built for it on the library stream at that
+    moment must be complete before the product's kernels read them."""
+    r = subprocess.run([sys.executable, str(ROOT / "tests" / "first_use_case.py"), mode], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0 and "FIRST_USE_OK" in r.stdout, r.stdout + r.stderr
 
 
 @pytest.mark.parametrize("n", [1, 2, 7, 64, 1000, 65536])
@@ -170,7 +192,10 @@ def test_every_compiled_tile_plan(engine, oracle, log2n):
     a, b = pc.operands(1000 + log2n, 64 * ((2 * total) // 3), 64 * (total - (2 * total) // 3))
     assert engine.transform_log2(a.size, b.size) == log2n
     want = oracle.gmp_mul(a, b)
-    want_sq = oracle.gmp_mul(b, b) if log2n <= 20 else None
+    # b has a third of the limbs: its square runs at this length too (from
+    # 2^22 points on, the scaled one-row square kernels).
+    assert engine.transform_log2(b.size, b.size) == log2n
+    want_sq = oracle.gmp_mul(b, b)
     # A product cut into two chunks whose chunk products have length 2^log2n:
     # runs the paired row kernels (two chunks sharing b) of the plan, over the
     # narrow prime set up to 2^22 and the wide set at 2^23.
