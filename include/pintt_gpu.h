@@ -68,6 +68,24 @@ int gpu_ntt_sqr(const uint64_t* a, size_t an, uint64_t* out);
 int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
                        uint64_t* d_out, void* stream);
 
+/* Same product, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  It calls no hipMalloc, hipFree or other
+ * allocating routine: its residues, carry maps and twiddle tables live in one
+ * static device array of 72.1 MiB per device (set up by gpu_ntt_init or the
+ * first call of any entry point), which limits it to an + bn <=
+ * PINTT_NOALLOC_MAX_LIMBS (transforms of at most 2^PINTT_NOALLOC_MAX_LOG2N
+ * points); larger products return PINTT_ERR_TOO_LARGE.  Tile plans are the
+ * loaded or tuned ones (never tuned here).  The work is enqueued on `stream`
+ * (NULL = library stream) and not ordered with any other call: every call on
+ * a device shares that device's scratch, so calls that overlap in time on one
+ * device (other streams, other threads) race and must be ordered by the
+ * caller; calls on one stream are. */
+#define PINTT_NOALLOC_MAX_LOG2N 20
+#define PINTT_NOALLOC_MAX_LIMBS (1u << (PINTT_NOALLOC_MAX_LOG2N - 1))
+int gpu_ntt_mul_device_noalloc(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn,
+                               uint64_t* d_out, void* stream);
+
 /* Batches: count independent products out[i] = a[i] * b[i] in one call
  * (a[i] == b[i] with an[i] == bn[i] squares).  The engine buckets them by
  * transform length and runs each bucket as one launch per pass, so a level of

@@ -43,6 +43,10 @@ ABI = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p],
     ),
+    "gpu_ntt_mul_device_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     "gpu_ntt_transform_log2": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t]),
     "gpu_ntt_max_limbs": (ctypes.c_size_t, []),
     "gpu_ntt_product_plan": (

@@ -44,6 +44,36 @@ static_assert(pintt::kLeafMaxDigits == PINTT_DECIMAL_LEAF_MAX_DIGITS &&
                   pintt::kLeafMaxLimbs == PINTT_DECIMAL_LEAF_MAX_LIMBS,
               "decimal leaf limits");
 
+// Device memory of gpu_ntt_mul_device_noalloc, laid out in 32-bit words (each
+// region starts on a 256-byte boundary): forward and inverse twiddle tables
+// of the narrow prime set for transforms of up to 2^kNoAllocMaxLog2N points
+// (three primes, (w, w') pairs), the row factor tables, the power tables the
+// builder uses, a residue arena of two operand slots at the longest transform,
+// and the carry maps.  72.1 MiB; each device the library runs on holds it
+// until the process ends.
+namespace pintt {
+namespace {
+constexpr int kNoAllocMaxLog2N = PINTT_NOALLOC_MAX_LOG2N;
+constexpr size_t kNoAllocN = (size_t)1 << kNoAllocMaxLog2N;
+constexpr uint32_t kNoAllocMaxChunks = 3;  // every job list of a product fits the kernel arguments
+constexpr size_t words_up(size_t w) { return (w + 63) & ~(size_t)63; }
+constexpr size_t kNaTwWords = 2 * kNoAllocN;  // one table of one prime
+constexpr size_t kNaRowWords = words_up(row_table_words(kSetNarrow));
+constexpr size_t kNaFwd = 0;
+constexpr size_t kNaInv = kNaFwd + kPrimes * kNaTwWords;
+constexpr size_t kNaRow = kNaInv + kPrimes * kNaTwWords;
+constexpr size_t kNaPow = kNaRow + kPrimes * kNaRowWords;
+constexpr size_t kNaArena = kNaPow + words_up(kPowScratch);
+constexpr size_t kNaArenaWords = 2 * kPrimes * kNoAllocN;  // two slots; m <= 3 chunks of half the length fit too
+constexpr size_t kNaMaps = kNaArena + kNaArenaWords;
+constexpr size_t kNaMapWords = words_up(kNoAllocN / kCarryWordsPerBlock + 1);
+constexpr size_t kNaWords = kNaMaps + kNaMapWords;
+static_assert(PINTT_NOALLOC_MAX_LIMBS == ((size_t)1 << (kNoAllocMaxLog2N - 1)), "limit in the header");
+}  // namespace
+}  // namespace pintt
+
+static __device__ __align__(256) uint32_t scratch[pintt::kNaWords];
+
 namespace pintt {
 namespace {
 
@@ -112,6 +142,10 @@ struct Engine {
     size_t tune_cap = 0;
     uint64_t* tune_in = nullptr;
     uint64_t* tune_out = nullptr;
+    // This device's copy of `scratch` (its device address, from
+    // hipGetSymbolAddress) and the tables built in it at initialisation.
+    uint32_t* na_base = nullptr;
+    Twiddles na_tw{};
 };
 
 Engine g;
@@ -189,6 +223,28 @@ CrtConst crt_constants(int log2n) {
     return c;
 }
 
+// Once per device, with it current: the device address of `scratch` and the
+// twiddle tables of gpu_ntt_mul_device_noalloc, built in it and waited for.
+int setup_noalloc_scratch() {
+    uint32_t* base = nullptr;
+    PINTT_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&base), HIP_SYMBOL(scratch)));
+    if (!base) return fail(PINTT_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+    Twiddles tw{};
+    for (int i = 0; i < kPrimes; ++i) {
+        uint2* fwd = reinterpret_cast<uint2*>(base + kNaFwd + i * kNaTwWords);
+        uint2* inv = reinterpret_cast<uint2*>(base + kNaInv + i * kNaTwWords);
+        uint32_t* row = base + kNaRow + i * kNaRowWords;
+        PINTT_HIP(launch_build_twiddles(fwd, inv, row, base + kNaPow, kSetNarrow, i, kNoAllocMaxLog2N, g.stream));
+        tw.fwd[i] = fwd;
+        tw.inv[i] = inv;
+        tw.row[i] = row;
+    }
+    PINTT_HIP(hipStreamSynchronize(g.stream));
+    g.na_base = base;
+    g.na_tw = tw;
+    return PINTT_OK;
+}
+
 int ensure_init(int device) {
     if (g.ready) {
         if (device < 0 || device == g.device) return PINTT_OK;
@@ -217,6 +273,7 @@ int ensure_init(int device) {
     const char* pr = std::getenv("PINTT_PAIR");
     g.pair_rows = !(pr && pr[0] == '0');
     g.device = dev;
+    PINTT_TRY(setup_noalloc_scratch());
     g.ready = true;
     return PINTT_OK;
 }
@@ -359,6 +416,18 @@ struct Group {
 
 int plan_for(int log2n, Plan* out);
 
+// How a batch may be planned: whether tile plans may be tuned on the device
+// first (which allocates and waits), and the most chunks a product may be cut
+// into.
+struct PlanMode {
+    bool tune = true;
+    uint32_t max_chunks = kMaxChunks;
+};
+
+// The plan of a length without tuning: the tuned or loaded one, else the
+// heuristic.
+Plan plan_lookup(int log2n) { return g.have_plan[log2n] ? g.plan[log2n] : make_plan(log2n); }
+
 // Estimated cost of `transforms` transforms of length 2^k: a third of the
 // tuned time of a full product (three transforms) when known, else N log N.
 double transform_cost(int k, int transforms) {
@@ -367,7 +436,7 @@ double transform_cost(int k, int transforms) {
     return base * transforms;
 }
 
-int plan_group(const Request& req, Group* gr) {
+int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{}) {
     Request r = req;
     if (!r.square && r.an < r.bn) {
         std::swap(r.a, r.b);
@@ -379,18 +448,18 @@ int plan_group(const Request& req, Group* gr) {
     gr->log2n = log2_for(r.an, r.bn);
     if (gr->log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
     if (!r.square && g.split_mode) {
-        if (g.autotune && g.split_mode == 1 && !g.have_plan[gr->log2n]) {
+        if (mode.tune && g.autotune && g.split_mode == 1 && !g.have_plan[gr->log2n]) {
             Plan tmp;
             PINTT_TRY(plan_for(gr->log2n, &tmp));
         }
         double best = transform_cost(gr->log2n, 3);
-        for (uint32_t m = 2; m <= (uint32_t)kMaxChunks; ++m) {
+        for (uint32_t m = 2; m <= mode.max_chunks; ++m) {
             size_t h = (r.an + m - 1) / m;
             h = (h + 15) & ~(size_t)15;  // chunk offsets stay multiples of 32 coefficients
             if (h < r.bn || (m - 1) * h >= r.an) continue;  // at most two chunk products per word
             int k = log2_for(h, r.bn);
             if (k < 0 || k >= gr->log2n) continue;
-            if (g.autotune && g.split_mode == 1 && !g.have_plan[k]) {
+            if (mode.tune && g.autotune && g.split_mode == 1 && !g.have_plan[k]) {
                 Plan tmp;
                 PINTT_TRY(plan_for(k, &tmp));
             }
@@ -448,9 +517,9 @@ bool school_eligible(const Request& r) {
 
 // Groups, buckets, descriptor layout and resource sizes of a batch.  May tune
 // tile plans for lengths seen for the first time (which waits on the device).
-int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
+int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp, const PlanMode& mode = PlanMode{}) {
     bp->groups.resize(reqs.size());
-    for (size_t i = 0; i < reqs.size(); ++i) PINTT_TRY(plan_group(reqs[i], &bp->groups[i]));
+    for (size_t i = 0; i < reqs.size(); ++i) PINTT_TRY(plan_group(reqs[i], &bp->groups[i], mode));
     const std::vector<Group>& groups = bp->groups;
     std::vector<BucketLayout>& buckets = bp->buckets;
     for (size_t i = 0; i < groups.size(); ++i) {
@@ -469,7 +538,13 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp) {
     }
     // Tile plans first: the row launches depend on them.
     bp->plans.resize(buckets.size());
-    for (size_t bi = 0; bi < buckets.size(); ++bi) PINTT_TRY(plan_for(buckets[bi].log2n, &bp->plans[bi]));
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        if (mode.tune) {
+            PINTT_TRY(plan_for(buckets[bi].log2n, &bp->plans[bi]));
+        } else {
+            bp->plans[bi] = plan_lookup(buckets[bi].log2n);
+        }
+    }
     // Buckets run one after another on one stream and reuse the arena.
     size_t bytes = 0;
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
@@ -623,9 +698,9 @@ bool fits_inline(const BatchPlan& bp) {
 
 // D: device copy of the descriptors (may be null when H is given and every
 // launch fits inline); H: host copy (null inside captures, which always read
-// device descriptors).
+// device descriptors).  tw: twiddle tables to use instead of the library's.
 int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned char* H, hipStream_t s,
-                     bool timed) {
+                     bool timed, const Twiddles* tw_tables = nullptr) {
     auto list = [&](auto* tag, size_t off, uint32_t n) {
         using T = std::remove_pointer_t<decltype(tag)>;
         if (H && n <= (uint32_t)kInlineJobs) return inline_jobs<T>(reinterpret_cast<const T*>(H + off), n);
@@ -640,7 +715,7 @@ int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned
     for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
         const BucketLayout& b = bp.buckets[bi];
         const Plan& pl = bp.plans[bi];
-        const Twiddles tw = twiddles(set_for(b.log2n));
+        const Twiddles tw = tw_tables ? *tw_tables : twiddles(set_for(b.log2n));
         if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
         for (uint32_t k = 0; k < b.nfwd; k += kMaxJobs) {
             const uint32_t c = std::min(kMaxJobs, b.nfwd - k);
@@ -1118,6 +1193,34 @@ int gpu_ntt_mul_batch_device(size_t count, const uint64_t* const* d_a, const siz
 int gpu_ntt_mul_device(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
                        void* stream) {
     return gpu_ntt_mul_batch_device(1, &d_a, &an, &d_b, &bn, &d_out, stream);
+}
+
+int gpu_ntt_mul_device_noalloc(const uint64_t* d_a, size_t an, const uint64_t* d_b, size_t bn, uint64_t* d_out,
+                               void* stream) {
+    if (!d_a || !d_b || !d_out || an == 0 || bn == 0) return fail(PINTT_ERR_ARGS, "null pointer or zero length");
+    if (an + bn > (size_t)PINTT_NOALLOC_MAX_LIMBS)
+        return fail(PINTT_ERR_TOO_LARGE, "gpu_ntt_mul_device_noalloc: an + bn = " + std::to_string(an + bn) +
+                                             " limbs exceeds the " + std::to_string(PINTT_NOALLOC_MAX_LIMBS) +
+                                             "-limb limit of its static scratch (use gpu_ntt_mul_device)");
+    const std::vector<Request> one{Request{d_a, an, d_b, bn, d_out, d_a == d_b && an == bn}};
+    std::lock_guard<std::mutex> lock(g_mu);
+    PINTT_TRY(ensure_init(-1));
+    PINTT_HIP(hipSetDevice(g.device));
+    // Plans as loaded or tuned before, else the heuristic: never tuned here.
+    BatchPlan bp;
+    PlanMode mode;
+    mode.tune = false;
+    mode.max_chunks = kNoAllocMaxChunks;
+    PINTT_TRY(plan_batch(one, &bp, mode));
+    if (!fits_inline(bp) || bp.arena_words > kNaArenaWords || bp.total_blocks > kNaMapWords ||
+        (!bp.buckets.empty() && bp.max_log2n > kNoAllocMaxLog2N))
+        return fail(PINTT_ERR_TOO_LARGE, "gpu_ntt_mul_device_noalloc: the product's plan does not fit the scratch");
+    std::vector<unsigned char> H(bp.bytes);
+    fill_descriptors(bp, H.data(), g.na_base + kNaArena, g.na_base + kNaMaps);
+    // Every job list rides in the kernel arguments: no descriptor upload, no
+    // ordering with the library's own products (whose buffers are separate).
+    return enqueue_launches(bp, nullptr, H.data(), stream ? static_cast<hipStream_t>(stream) : g.stream, false,
+                            &g.na_tw);
 }
 
 int gpu_ntt_graph_create(size_t count, const uint64_t* const* d_a, const size_t* an, const uint64_t* const* d_b,
