@@ -83,12 +83,12 @@ struct Engine {
     bool ready = false;
     int device = -1;
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;  // end of the last batch
-    hipStream_t last = nullptr; // stream that batch ran on
-    // Work enqueued on the library stream outside product batches (twiddle
-    // builds, copies, additions) since a caller stream last waited for it.
-    bool lib_pending = false;
-    hipEvent_t lib_mark = nullptr;
+    // Streams are ordered lazily: `last` is the stream that received the most
+    // recent engine work and `done` is recorded there after it; work for
+    // another stream first waits on `done`.  Calls that stay on one stream
+    // cost one event record each and no cross-queue wait.
+    hipEvent_t done = nullptr;
+    hipStream_t last = nullptr;
     bool tw_ready[kSets] = {};
     uint32_t* tw_pow = nullptr;  // powers of the root while tables are built
     uint2* tw_fwd[kSets][kPrimes] = {};
@@ -263,7 +263,6 @@ int ensure_init(int device) {
                                              ", the kernels are built for gfx950 only");
     PINTT_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     PINTT_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
-    PINTT_HIP(hipEventCreateWithFlags(&g.lib_mark, hipEventDisableTiming));
     for (auto& e : g.desc_done) PINTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : g.ev) PINTT_HIP(hipEventCreate(&e));
     const char* at = std::getenv("PINTT_AUTOTUNE");
@@ -278,18 +277,24 @@ int ensure_init(int device) {
     return PINTT_OK;
 }
 
-// Device work enqueued on the library stream that products on other streams
-// must follow: the first use of the twiddle tables, device copies, enqueued
-// additions.  (Product batches on the library stream order themselves through
-// `done`.)
-void library_work_enqueued() { g.lib_pending = true; }
+// Engine work about to be enqueued on `s`: ordered after all earlier engine
+// work, on whichever stream it went.
+int begin_on(hipStream_t s) {
+    if (g.last && g.last != s) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
+    return PINTT_OK;
+}
 
-// Make stream `s` wait for the library stream's pending work, if any.
-int order_after_library(hipStream_t s) {
-    if (s == g.stream || !g.lib_pending) return PINTT_OK;
-    PINTT_HIP(hipEventRecord(g.lib_mark, g.stream));
-    PINTT_HIP(hipStreamWaitEvent(s, g.lib_mark, 0));
-    g.lib_pending = false;
+// Engine work just enqueued on `s` (ends the ordering chain there).
+int end_on(hipStream_t s) {
+    PINTT_HIP(hipEventRecord(g.done, s));
+    g.last = s;
+    return PINTT_OK;
+}
+
+// Until all engine work on every stream has completed.
+int drain() {
+    if (g.last) PINTT_HIP(hipEventSynchronize(g.done));
+    PINTT_HIP(hipStreamSynchronize(g.stream));
     return PINTT_OK;
 }
 
@@ -297,7 +302,7 @@ int order_after_library(hipStream_t s) {
 template <class T>
 int grow(T*& ptr, size_t count) {
     if (ptr) {
-        PINTT_HIP(hipStreamSynchronize(g.stream));
+        PINTT_TRY(drain());
         PINTT_HIP(hipFree(ptr));
         ptr = nullptr;
     }
@@ -318,10 +323,12 @@ int ensure_twiddles(int log2n) {
         PINTT_TRY(grow(g.tw_fwd[set][i], cnt));
         PINTT_TRY(grow(g.tw_inv[set][i], cnt));
         PINTT_TRY(grow(g.tw_row[set][i], row_table_words(set)));
+    }
+    PINTT_TRY(begin_on(g.stream));
+    for (int i = 0; i < kPrimes; ++i)
         PINTT_HIP(launch_build_twiddles(g.tw_fwd[set][i], g.tw_inv[set][i], g.tw_row[set][i], g.tw_pow, set, i,
                                         kSetMaxLog2N[set], g.stream));
-    }
-    library_work_enqueued();  // a product on a caller stream waits for the tables
+    PINTT_TRY(end_on(g.stream));  // products on any stream wait for the tables
     g.tw_ready[set] = true;
     return PINTT_OK;
 }
@@ -349,7 +356,7 @@ int ensure_carry_state(size_t blocks) {
 int ensure_descriptors(int k, size_t bytes) {
     PINTT_HIP(hipEventSynchronize(g.desc_done[k]));
     if (bytes <= g.desc_cap[k]) return PINTT_OK;
-    PINTT_HIP(hipStreamSynchronize(g.stream));
+    PINTT_TRY(drain());
     const size_t cap = std::max(bytes + bytes / 2, (size_t)1 << 16);
     for (int j = 0; j < Engine::kDescRing; ++j) {
         if (g.desc_cap[j] >= cap) continue;
@@ -758,31 +765,26 @@ int enqueue_batch(const std::vector<Request>& reqs, hipStream_t s) {
         // Every job list rides in kernel arguments: nothing to upload.
         std::vector<unsigned char> H(bp.bytes);
         fill_descriptors(bp, H.data(), g.arena, g.maps);
-        // Ordered after the previous batch (arena and carry maps are shared)
-        // and after the library stream's own work (tables built at first use).
-        if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
-        PINTT_TRY(order_after_library(s));
+        // Ordered after all earlier engine work (the arena and carry maps are
+        // shared; tables are built at first use).
+        PINTT_TRY(begin_on(s));
         PINTT_TRY(enqueue_launches(bp, nullptr, H.data(), s, g.timing));
     } else {
         const int ring = g.desc_next;
         g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
         PINTT_TRY(ensure_descriptors(ring, bp.bytes));
         fill_descriptors(bp, g.desc_host[ring], g.arena, g.maps);
+        // The upload follows all earlier engine work (the slot's previous
+        // readers among it); the launches follow the upload.
+        PINTT_TRY(begin_on(g.stream));
         PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bp.bytes, hipMemcpyHostToDevice, g.stream));
         PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
-        // A caller stream first waits for the upload, which itself is ordered
-        // after the previous batch and all other work on the library stream.
-        if (s != g.stream) {
-            PINTT_HIP(hipStreamWaitEvent(s, g.desc_done[ring], 0));
-            g.lib_pending = false;
-        }
+        PINTT_TRY(end_on(g.stream));
+        PINTT_TRY(begin_on(s));
         PINTT_TRY(enqueue_launches(bp, g.desc_dev[ring], g.desc_host[ring], s, g.timing));
     }
     if (g.timing) g.have_times = true;
-    PINTT_HIP(hipEventRecord(g.done, s));
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
-    g.last = s;
-    return PINTT_OK;
+    return end_on(s);
 }
 
 // A batch captured once into a HIP graph, with its own arena, carry state and
@@ -1018,10 +1020,7 @@ void parallel_copy(const std::vector<Piece>& pieces, size_t total) {
     for (auto& t : th) t.join();
 }
 
-int wait_stream() {
-    PINTT_HIP(hipStreamSynchronize(g.stream));
-    return PINTT_OK;
-}
+int wait_stream() { return drain(); }
 
 bool overlaps(const uint64_t* x, size_t xn, const uint64_t* y, size_t yn) { return x < y + yn && y < x + xn; }
 
@@ -1088,9 +1087,11 @@ int host_batch(size_t count, const uint64_t* const* a, const size_t* an, const u
         for (size_t i = 0; i < count; ++i)
             reqs[i] = Request{g.io_in + a_off[i], an[i], g.io_in + b_off[i], bn[i], g.io_out + o_off[i],
                               a[i] == b[i] && an[i] == bn[i]};
+        PINTT_TRY(begin_on(g.stream));
         PINTT_HIP(hipMemcpyAsync(g.io_in, st.in(), total_in * sizeof(uint64_t), hipMemcpyHostToDevice, g.stream));
         PINTT_TRY(enqueue_batch(reqs, g.stream));
         PINTT_HIP(hipMemcpyAsync(st.out(), g.io_out, total_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+        PINTT_TRY(end_on(g.stream));
         PINTT_TRY(st.record());
     }
     PINTT_TRY(st.wait());
@@ -1114,7 +1115,7 @@ void gpu_ntt_shutdown(void) {
     std::lock_guard<std::mutex> lock(g_mu);
     if (!g.ready) return;
     (void)hipSetDevice(g.device);
-    (void)hipStreamSynchronize(g.stream);
+    (void)drain();
     for (int set = 0; set < kSets; ++set)
         for (int i = 0; i < kPrimes; ++i) {
             (void)hipFree(g.tw_fwd[set][i]);
@@ -1152,7 +1153,6 @@ void gpu_ntt_shutdown(void) {
     (void)hipFree(g.tune_out);
     for (auto& e : g.ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(g.done);
-    (void)hipEventDestroy(g.lib_mark);
     (void)hipStreamDestroy(g.stream);
     g = Engine{};
 }
@@ -1237,7 +1237,7 @@ int gpu_ntt_graph_create(size_t count, const uint64_t* const* d_a, const size_t*
     std::lock_guard<std::mutex> lock(g_mu);
     PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipSetDevice(g.device));
-    PINTT_HIP(hipStreamSynchronize(g.stream));
+    PINTT_TRY(drain());
     BatchGraph* bg = nullptr;
     PINTT_TRY(create_graph(reqs, &bg));
     *graph = bg;
@@ -1251,13 +1251,9 @@ int gpu_ntt_graph_launch(void* graph, void* stream) {
     BatchGraph* bg = static_cast<BatchGraph*>(graph);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g.stream;
     // Serialised with every other batch on the device, after the tables.
-    if (s != g.last) PINTT_HIP(hipStreamWaitEvent(s, g.done, 0));
-    PINTT_TRY(order_after_library(s));
+    PINTT_TRY(begin_on(s));
     PINTT_HIP(hipGraphLaunch(bg->exec, s));
-    PINTT_HIP(hipEventRecord(g.done, s));
-    if (s != g.stream) PINTT_HIP(hipStreamWaitEvent(g.stream, g.done, 0));
-    g.last = s;
-    return PINTT_OK;
+    return end_on(s);
 }
 
 // End of the device allocation holding p (p itself when unknown: no merging).
@@ -1287,7 +1283,7 @@ int gpu_ntt_dev_free(uint64_t* d_ptr) {
     if (!d_ptr) return PINTT_OK;
     std::lock_guard<std::mutex> lock(g_mu);
     PINTT_TRY(ensure_init(-1));
-    PINTT_HIP(hipStreamSynchronize(g.stream));
+    PINTT_TRY(drain());
     PINTT_HIP(hipFree(d_ptr));
     return PINTT_OK;
 }
@@ -1316,6 +1312,7 @@ int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* 
     {
         std::lock_guard<std::mutex> lock(g_mu);
         PINTT_HIP(hipSetDevice(g.device));
+        PINTT_TRY(begin_on(g.stream));
         // One DMA per run of destinations that follow each other inside one allocation.
         off = 0;
         for (size_t i = 0; i < count;) {
@@ -1329,6 +1326,7 @@ int gpu_ntt_upload(size_t count, uint64_t* const* d_dst, const uint64_t* const* 
             off += run;
             i = j;
         }
+        PINTT_TRY(end_on(g.stream));
         PINTT_TRY(st.record());
     }
     return st.wait();
@@ -1350,6 +1348,7 @@ int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const
     {
         std::lock_guard<std::mutex> lock(g_mu);
         PINTT_HIP(hipSetDevice(g.device));
+        PINTT_TRY(begin_on(g.stream));
         size_t off = 0;
         for (size_t i = 0; i < count;) {
             const uint64_t* end = allocation_end(d_src[i]);
@@ -1365,6 +1364,7 @@ int gpu_ntt_download(size_t count, uint64_t* const* h_dst, const uint64_t* const
             }
             i = j;
         }
+        PINTT_TRY(end_on(g.stream));
         PINTT_TRY(st.record());
     }
     PINTT_TRY(st.wait());
@@ -1377,9 +1377,9 @@ int gpu_ntt_dev_copy(uint64_t* d_dst, const uint64_t* d_src, size_t limbs) {
     if (limbs == 0) return PINTT_OK;
     std::lock_guard<std::mutex> lock(g_mu);
     PINTT_TRY(ensure_init(-1));
+    PINTT_TRY(begin_on(g.stream));
     PINTT_HIP(hipMemcpyAsync(d_dst, d_src, limbs * sizeof(uint64_t), hipMemcpyDeviceToDevice, g.stream));
-    library_work_enqueued();
-    return PINTT_OK;
+    return end_on(g.stream);
 }
 
 namespace {
@@ -1404,7 +1404,7 @@ int add_batch(size_t count, const uint64_t* const* d_x, const size_t* xn, const 
     std::lock_guard<std::mutex> lock(g_mu);
     PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipSetDevice(g.device));
-    PINTT_HIP(hipStreamSynchronize(g.stream));  // the staging buffers below may be reused
+    PINTT_TRY(drain());  // the staging buffers below may be reused
     if (count > g.add_job_cap) {
         if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
         (void)hipFree(g.add_jobs_dev);
@@ -1434,6 +1434,7 @@ int add_batch(size_t count, const uint64_t* const* d_x, const size_t* xn, const 
     if (!g.add_error && hipMalloc(reinterpret_cast<void**>(&g.add_error), sizeof(uint32_t)) != hipSuccess)
         return fail(PINTT_ERR_NO_MEMORY, "add error flag");
     std::memcpy(g.add_jobs_host, jobs.data(), count * sizeof(AddJob));
+    PINTT_TRY(begin_on(g.stream));
     PINTT_HIP(hipMemcpyAsync(g.add_jobs_dev, g.add_jobs_host, count * sizeof(AddJob), hipMemcpyHostToDevice, g.stream));
     // A fill kernel, not hipMemsetAsync: no memset node if a caller ever captures this.
     PINTT_HIP(launch_fill_u32(g.add_error, 0u, 1, g.stream));
@@ -1445,6 +1446,7 @@ int add_batch(size_t count, const uint64_t* const* d_x, const size_t* xn, const 
     if (negative)
         PINTT_HIP(hipMemcpyAsync(status.data(), g.add_status, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                  g.stream));
+    PINTT_TRY(end_on(g.stream));
     PINTT_HIP(hipStreamSynchronize(g.stream));
     if (negative) {
         for (size_t i = 0; i < count; ++i) negative[i] = status[i] ? 1 : 0;
@@ -1483,7 +1485,7 @@ int gpu_ntt_add_batch_device_async(size_t count, const uint64_t* const* d_x, con
     // Device scratch that earlier batches may still be using grows only after
     // the stream drains.
     if (count > g.add_job_cap || blocks > g.add_blk_cap) {
-        PINTT_HIP(hipStreamSynchronize(g.stream));
+        PINTT_TRY(drain());
         if (count > g.add_job_cap) {
             (void)hipFree(g.add_status);
             g.add_status = nullptr;
@@ -1518,12 +1520,12 @@ int gpu_ntt_add_batch_device_async(size_t count, const uint64_t* const* d_x, con
     const size_t bytes = count * sizeof(AddJob);
     PINTT_TRY(ensure_descriptors(ring, bytes));
     std::memcpy(g.desc_host[ring], jobs.data(), bytes);
+    PINTT_TRY(begin_on(g.stream));
     PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bytes, hipMemcpyHostToDevice, g.stream));
     PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
     PINTT_HIP(launch_add_batch(reinterpret_cast<const AddJob*>(g.desc_dev[ring]), (uint32_t)count, (uint32_t)blocks,
                                g.add_maps, g.add_carry, d_negative, g.add_status, g.stream));
-    library_work_enqueued();
-    return PINTT_OK;
+    return end_on(g.stream);
 }
 
 int gpu_ntt_add_batch_device_flags(size_t count, const uint64_t* const* d_x, const size_t* xn,
@@ -1542,7 +1544,7 @@ int gpu_ntt_bs_leaves(uint64_t a, uint64_t b, size_t count, uint64_t* d_out, uin
     PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipSetDevice(g.device));
     if (count > g.bs_cap) {
-        PINTT_HIP(hipStreamSynchronize(g.stream));
+        PINTT_TRY(drain());
         (void)hipFree(g.bs_len);
         (void)hipFree(g.bs_sign);
         g.bs_len = nullptr;
@@ -1555,12 +1557,14 @@ int gpu_ntt_bs_leaves(uint64_t a, uint64_t b, size_t count, uint64_t* d_out, uin
     }
     if (!g.bs_err && hipMalloc(reinterpret_cast<void**>(&g.bs_err), sizeof(uint32_t)) != hipSuccess)
         return fail(PINTT_ERR_NO_MEMORY, "subtree error word");
+    PINTT_TRY(begin_on(g.stream));
     PINTT_HIP(launch_fill_u32(g.bs_err, 0u, 1, g.stream));
     PINTT_HIP(launch_bs_leaves(a, b, (uint32_t)count, d_out, g.bs_len, g.bs_sign, g.bs_err, g.stream));
     uint32_t err = 0;
     PINTT_HIP(hipMemcpyAsync(lengths, g.bs_len, count * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
     PINTT_HIP(hipMemcpyAsync(signs, g.bs_sign, count * 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g.stream));
     PINTT_HIP(hipMemcpyAsync(&err, g.bs_err, sizeof(err), hipMemcpyDeviceToHost, g.stream));
+    PINTT_TRY(end_on(g.stream));
     PINTT_HIP(hipStreamSynchronize(g.stream));
     if (err) return fail(PINTT_ERR_ARGS, "subtree: a subtraction for R went negative");
     return PINTT_OK;
@@ -1581,7 +1585,7 @@ int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, cons
     std::lock_guard<std::mutex> lock(g_mu);
     PINTT_TRY(ensure_init(-1));
     PINTT_HIP(hipSetDevice(g.device));
-    PINTT_HIP(hipStreamSynchronize(g.stream));  // the descriptor buffers below may be reused
+    PINTT_TRY(drain());  // the descriptor buffers below may be reused
     if (count > g.leaf_job_cap) {
         if (g.leaf_jobs_host) (void)hipHostFree(g.leaf_jobs_host);
         (void)hipFree(g.leaf_jobs_dev);
@@ -1595,9 +1599,11 @@ int gpu_ntt_decimal_leaves_device(size_t count, const uint64_t* const* d_x, cons
         g.leaf_job_cap = cap;
     }
     std::memcpy(g.leaf_jobs_host, jobs.data(), count * sizeof(LeafJob));
+    PINTT_TRY(begin_on(g.stream));
     PINTT_HIP(hipMemcpyAsync(g.leaf_jobs_dev, g.leaf_jobs_host, count * sizeof(LeafJob), hipMemcpyHostToDevice,
                              g.stream));
     PINTT_HIP(launch_decimal_leaves(g.leaf_jobs_dev, (uint32_t)count, d_out, g.stream));
+    PINTT_TRY(end_on(g.stream));
     PINTT_HIP(hipStreamSynchronize(g.stream));
     return PINTT_OK;
 }
@@ -1607,7 +1613,7 @@ size_t gpu_ntt_graph_nodes(void* graph) { return graph ? static_cast<BatchGraph*
 void gpu_ntt_graph_destroy(void* graph) {
     if (!graph) return;
     std::lock_guard<std::mutex> lock(g_mu);
-    if (g.ready) (void)hipStreamSynchronize(g.stream);
+    if (g.ready) (void)drain();
     (void)hipDeviceSynchronize();
     free_graph(static_cast<BatchGraph*>(graph));
 }
