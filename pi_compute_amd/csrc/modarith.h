@@ -68,6 +68,16 @@ __device__ __forceinline__ uint32_t mul_mont(uint32_t a, uint32_t b, uint32_t p,
     return min(u, u - p);
 }
 
+// x * f * 2^-32 mod p for any 32-bit x, f < p and fp = f * p^-1 mod 2^32: a
+// Montgomery product with a multiplier fixed for many x (one multiplication
+// gives its fp, where Shoup's quotient would need a division).  m = x fp makes
+// the low halves of x f and m p equal, so (x f - m p) / 2^32 is the difference
+// of the high halves, in (-p, p); the result, plus p, lies in (0, 2p).
+__device__ __forceinline__ uint32_t mul_fixed(uint32_t x, uint32_t f, uint32_t fp, uint32_t p) {
+    const uint32_t m = x * fp;
+    return __umulhi(x, f) - __umulhi(m, p) + p;
+}
+
 // Butterflies and the per-set steps around them.  The twiddle z is (w, w').
 template <int SET>
 struct Arith;
@@ -97,6 +107,11 @@ struct Arith<kSetWide> {
     // Forward outputs (in [0, p)) to the inverse input range, multiplied.
     __device__ static __forceinline__ uint32_t pointwise(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) {
         return mul_mont(x, y, p, pinv);
+    }
+    // x * f 2^-32 (mul_fixed), fully reduced.
+    __device__ static __forceinline__ uint32_t scale(uint32_t x, uint32_t f, uint32_t fp, uint32_t p) {
+        const uint32_t u = mul_fixed(x, f, fp, p);
+        return min(u, u - p);
     }
 };
 
@@ -136,6 +151,11 @@ struct Arith<kSetNarrow> {
         const uint64_t t = (uint64_t)x * y;
         const uint32_t m = (uint32_t)t * pinv;
         return (uint32_t)((t + (uint64_t)m * p) >> 32);
+    }
+    // x * f 2^-32 (mul_fixed) in (0, 2p): inside both the forward [0, 4p) and
+    // the inverse [0, 2p) input ranges.
+    __device__ static __forceinline__ uint32_t scale(uint32_t x, uint32_t f, uint32_t fp, uint32_t p) {
+        return mul_fixed(x, f, fp, p);
     }
 };
 

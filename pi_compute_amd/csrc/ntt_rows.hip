@@ -2,6 +2,8 @@
 // rows of the operands, Montgomery pointwise products, inverse layers, in place.
 #include "ntt_device.h"
 
+#include <cstdlib>
+
 namespace pintt {
 namespace {
 
@@ -12,15 +14,15 @@ namespace {
 // Twiddles: a row's own twiddles are the table entries of its blocks (one
 // (w, w') pair per element and direction: N pairs per direction in all,
 // which outgrow the caches from 2^22 points on).  SCALED kernels (one row per
-// workgroup; squares, one-chunk products and later chunks against b's stored
-// rows) instead scale the row by a_r^c as it is loaded and by a_r^-c as it is
-// stored and run the layers of block 0, whose twiddles every workgroup shares
-// (row_factor).  Measured on MI355X, row pass time of squares and one-chunk
-// products: 12-17% lower at 2^22 points, 44-48% lower at 2^25, unchanged at
-// 2^21, 15-30% higher at 2^17..2^20.  The paired kernel (three tiles, two
-// waves): 4% higher at 2^21 and 2^24, 2% lower at 2^23, 8% lower at 2^22.
-// launch_row_convolution picks SCALED from 2^22 points on, for pairs at 2^22
-// only.
+// workgroup) instead scale the row by a_r^c as it is loaded and by a_r^-c as
+// it is stored and run the layers of block 0, whose twiddles every workgroup
+// shares (row_factor); a scaling costs three multiplications per element and
+// direction (mul_fixed) plus one Shoup product per element position for the
+// factor.  Measured on MI355X against per-row twiddles: squares 5% faster at
+// 2^22 points, paired rows 2% faster at 2^21 and 7% faster at 2^20 (the k = 25
+// product), one-chunk rows 6% slower at 2^20.  launch_row_convolution picks
+// SCALED from 2^21 points on for squares, one-chunk products and later chunks,
+// and from 2^20 on for pairs.
 // Transformed rows of b, stored for later chunks or read back, are never
 // scaled.
 // Three waves per SIMD (at most 168 VGPRs): left alone, the compiler spends
@@ -46,32 +48,47 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     global_ptr<uint32_t> b = as_global(job.b) + base;
     global_ptr<uint32_t> a2 = as_global(job.a2) + base;
     // Scaled rows: element e of a thread sits at c = e NT + tid, and
-    // a_r^c = a_r^tid * a_r^(e NT), the second factor the same for the whole
-    // workgroup.  Exponents bitrev_log2r(r) c 2^(K - n) stay below 2^K.
+    // a_r^c = a_r^tid * a_r^(e NT).  Exponents bitrev_log2r(r) c 2^(K - n)
+    // stay below 2^K.  The second factor, beta_e, is the same for the whole
+    // workgroup: 32 threads put the 16 forward and 16 inverse ones in LDS as
+    // plain residues with their Shoup quotients.  The first, h = a_r^(+-tid)
+    // in Montgomery form, is the thread's own; h beta_e (one Shoup product)
+    // is then the Montgomery form f of a_r^(+-c), and an element is scaled
+    // by mul_fixed with f and f p^-1.
     const RowTable rt = PINTT_ROW_TABLE(tw, prime);
     constexpr uint32_t kMask = (1u << kSetMaxLog2N[SET]) - 1;
     const uint32_t rbits = __brev(row0) >> (32 - log2r);
     const int eshift = kSetMaxLog2N[SET] - (log2r + LOGC);
-    uint32_t ft_fwd = 0, ft_inv = 0;
+    const uint32_t pinv_pos = 0u - pinv;  // p^-1 mod 2^32
+    __shared__ uint2 beta[kScaled ? 2 * kElemsPerThread : 1];
+    uint32_t h_fwd = 0, h_inv = 0;
     if constexpr (kScaled) {
         const uint32_t et = (rbits * threadIdx.x) << eshift;
-        ft_fwd = row_factor(rt, et, p);
-        ft_inv = row_factor(rt, (0u - et) & kMask, p);
+        h_fwd = row_factor(rt, et, p);
+        h_inv = row_factor(rt, (0u - et) & kMask, p);
+        if (threadIdx.x < 2 * kElemsPerThread) {
+            const uint32_t e = threadIdx.x % kElemsPerThread;
+            const uint32_t eu = (rbits * (e * TL::NT)) << eshift;
+            const uint32_t bm = row_factor(rt, threadIdx.x < kElemsPerThread ? eu : (0u - eu) & kMask, p);
+            const uint32_t bv = mul_mont(bm, 1u, p, pinv);  // out of Montgomery form
+            beta[threadIdx.x] = make_uint2(bv, (uint32_t)(((uint64_t)bv << 32) / p));
+        }
+        __syncthreads();
     }
     auto factor = [&](int e, bool inverse) {
-        const uint32_t eu = (rbits * (uint32_t)(e * TL::NT)) << eshift;
-        const uint32_t fu = row_factor(rt, inverse ? (0u - eu) & kMask : eu, p);
-        return mul_mont(fu, inverse ? ft_inv : ft_fwd, p, pinv);
+        const uint2 bq = beta[(inverse ? kElemsPerThread : 0) + e];
+        const uint32_t f = mul_shoup(inverse ? h_inv : h_fwd, bq.x, bq.y, p);
+        return make_uint2(f, f * pinv_pos);
     };
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         if constexpr (kScaled) {
-            const uint32_t f = factor(e, false);
-            ldsA[pad(idx)] = mul_mont(a[idx], f, p, pinv);
-            if constexpr (KIND == kRowTransformB || kPair) ldsB[pad(idx)] = mul_mont(b[idx], f, p, pinv);
+            const uint2 f = factor(e, false);
+            ldsA[pad(idx)] = Arith<SET>::scale(a[idx], f.x, f.y, p);
+            if constexpr (KIND == kRowTransformB || kPair) ldsB[pad(idx)] = Arith<SET>::scale(b[idx], f.x, f.y, p);
             if constexpr (KIND == kRowReuseB) ldsB[pad(idx)] = b[idx];
-            if constexpr (kPair) ldsC[pad(idx)] = mul_mont(a2[idx], f, p, pinv);
+            if constexpr (kPair) ldsC[pad(idx)] = Arith<SET>::scale(a2[idx], f.x, f.y, p);
         } else {
             ldsA[pad(idx)] = a[idx];
             if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
@@ -123,9 +140,9 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         if constexpr (kScaled) {
-            const uint32_t f = factor(e, true);
-            a[idx] = mul_mont(ldsA[pad(idx)], f, p, pinv);
-            if constexpr (kPair) a2[idx] = mul_mont(ldsC[pad(idx)], f, p, pinv);
+            const uint2 f = factor(e, true);
+            a[idx] = Arith<SET>::scale(ldsA[pad(idx)], f.x, f.y, p);
+            if constexpr (kPair) a2[idx] = Arith<SET>::scale(ldsC[pad(idx)], f.x, f.y, p);
         } else {
             a[idx] = ldsA[pad(idx)];
             if constexpr (kPair) a2[idx] = ldsC[pad(idx)];
@@ -133,14 +150,21 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     }
 }
 
-// Scaled rows from this transform length on.
-constexpr int kScaledRowsMinLog2N = 22;
+// Scaled rows from this transform length on (PINTT_SCALED_MIN), for pairs
+// between PINTT_SCALED_PAIR_MIN and PINTT_SCALED_PAIR_MAX.
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+const int kScaledRowsMinLog2N = env_int("PINTT_SCALED_MIN", 21);
+const int kScaledPairMinLog2N = env_int("PINTT_SCALED_PAIR_MIN", 20);
+const int kScaledPairMaxLog2N = env_int("PINTT_SCALED_PAIR_MAX", kMaxLog2N);
 
 template <int LC, int LROWS, int K, int SET>
 void launch_rows(const Plan& pl, dim3 grid, const JobList<RowJob>& jobs, const Twiddles& tw, hipStream_t s) {
     if constexpr (LROWS == 0) {
-        // Pairs only at 2^22 points, the narrow set's longest transform.
-        const bool scaled = pl.log2n >= kScaledRowsMinLog2N && (K != kRowPair || SET == kSetNarrow);
+        const bool scaled = K == kRowPair ? pl.log2n >= kScaledPairMinLog2N && pl.log2n <= kScaledPairMaxLog2N
+                                          : pl.log2n >= kScaledRowsMinLog2N;
         if (scaled) {
             hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET, true>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s,
                                jobs, pl.log2r, tw);

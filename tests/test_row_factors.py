@@ -104,3 +104,72 @@ def test_row_factor_table_split(p, g, k):
         assert x * f * rinv % p == x * t.power(e) % p  # mul_mont(x, f) = x wK^e
         if e:
             assert t.power(e) * t.power((1 << k) - e) % p == 1  # the inverse exponent
+
+
+M32 = 1 << 32
+
+
+def mul_shoup(x: int, w: int, wq: int, p: int) -> int:
+    """modarith.h mul_shoup on 32-bit words: q = mulhi(x, w'), r = x w - q p."""
+    q = (x * wq) >> 32
+    r = (x * w - q * p) % M32
+    return min(r, (r - p) % M32)
+
+
+def mul_fixed(x: int, f: int, fp: int, p: int) -> int:
+    """modarith.h mul_fixed on 32-bit words: mulhi(x, f) - mulhi(x fp, p) + p."""
+    m = (x * fp) % M32
+    return (((x * f) >> 32) - ((m * p) >> 32) + p) % M32
+
+
+@pytest.mark.parametrize("p,g,k", PRIMES)
+def test_mul_fixed_is_a_montgomery_product_in_range(p, g, k):
+    rng = random.Random(p)
+    pinv = pow(p, -1, M32)
+    rinv = pow(M32, -1, p)
+    for _ in range(20000):
+        x, f = rng.randrange(M32), rng.randrange(p)
+        u = mul_fixed(x, f, f * pinv % M32, p)
+        assert 0 < u < 2 * p and u % p == x * f * rinv % p
+    for x in (0, 1, p - 1, 2 * p - 1, M32 - 1):
+        for f in (0, 1, p - 1):
+            u = mul_fixed(x, f, f * pinv % M32, p)
+            assert 0 < u < 2 * p and u % p == x * f * rinv % p
+
+
+@pytest.mark.parametrize("p,g,k", PRIMES)
+@pytest.mark.parametrize("lr,lc", [(9, 12), (10, 12), (6, 8)])
+def test_scaled_row_factor_per_element(p, g, k, lr, lc):
+    """The kernels' factor of element c = e NT + tid of row r (NT = 2^lc / 16
+    threads): h = a_r^(+-tid) in Montgomery form from the two tables; beta_e =
+    a_r^(+-e NT) as a plain residue with its Shoup quotient; f = Shoup(h,
+    beta_e), the Montgomery form of a_r^(+-c); then mul_fixed(x, f, f p^-1) is
+    x a_r^(+-c) mod p, in (0, 2p)."""
+    t = Tables(p, g, k)
+    n, nt = lr + lc, (1 << lc) // 16
+    r32, rinv, pinv = M32 % p, pow(M32, -1, p), pow(p, -1, M32)
+    mask = (1 << k) - 1
+    rng = random.Random(lr * 31 + lc)
+
+    def row_factor(e):  # lo[e & 4095] (Montgomery form) times hi[e >> 12] (Shoup)
+        lo = t.power(e & 4095) * r32 % p
+        hi = t.power(e >> 12 << 12)
+        return mul_shoup(lo, hi, (hi << 32) // p, p)
+
+    for r in (0, 1, (1 << lr) - 1, rng.randrange(1 << lr)):
+        rb = brev(r, lr)
+        for tid in (0, 1, nt - 1, rng.randrange(nt)):
+            et = (rb * tid) << (k - n)
+            for inverse in (False, True):
+                h = row_factor((-et) & mask if inverse else et)
+                for e in (0, 1, 15, rng.randrange(16)):
+                    eu = (rb * e * nt) << (k - n)
+                    bm = row_factor((-eu) & mask if inverse else eu)
+                    beta = bm * rinv % p  # mul_mont(bm, 1): out of Montgomery form
+                    f = mul_shoup(h, beta, (beta << 32) // p, p)
+                    c = e * nt + tid
+                    want_f = t.power(-((rb * c) << (k - n)) if inverse else (rb * c) << (k - n))
+                    assert f == want_f * r32 % p
+                    x = rng.randrange(M32)
+                    u = mul_fixed(x, f, f * pinv % M32, p)
+                    assert 0 < u < 2 * p and u % p == x * want_f % p
