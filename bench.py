@@ -11,9 +11,13 @@ engine itself uses, see pi_compute_amd/hiprt.py).
 
 Prints one JSON line: mults/s, the dominant pass's roofline (algorithmic bytes
 over its HIP-event time), the reference's own GmpMultiplier timed on this
-host's cores on a bounded sample as the CPU baseline, the tree-level replay
-batch (4,096 products: per call, one batch call, one graph replay), and the
-100M-digit pi leg (GMP-only run against the GPU runs, digit files compared).
+host's cores on a bounded sample as the CPU baseline, and beside them the other
+configurations: the 2^22 x 2^22-bit single product and square; the size sweep
+k = 16..28 (GPU rates and roofline per shape, GMP one-thread medians of 10 and
+all-core rates, the CPU/GPU crossover); the tree-level replay batch (4,096
+products: per call, one batch call, one graph replay); 10M-digit pi with one
+GPU call per product; and 100M-digit pi (the reference's own GMP-only program
+and GPU runs, digit files compared byte for byte).
 """
 from __future__ import annotations
 
@@ -119,65 +123,205 @@ def time_passes(stream, da, an, db, bn, dout, reps: int) -> dict:
     return {k: v / reps for k, v in acc.items()}
 
 
-def cpu_baseline(a: np.ndarray, b: np.ndarray, gpu_out: np.ndarray, budget_s: float):
-    """GmpMultiplier (mpz_mul) on this host: the reference's own source built
-    under oracle/_ref when present (kind "reference"), else the oracle's GMP
-    restatement (kind "port").  c threads run independent products."""
-    u64p = ctypes.POINTER(ctypes.c_uint64)
-    ref = ROOT / "oracle" / "_ref" / "libgmpmul_ref.so"
-    if ref.exists():
-        L = ctypes.CDLL(str(ref))
-        L.ref_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p, ctypes.c_int]
-        L.ref_gmp_time.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
-        L.ref_gmp_time.restype = ctypes.c_double
-        kind = "reference"
+def cpu_share() -> tuple:
+    """(threads the CPU legs use, host cores visible).  A one-GPU box allots 16
+    host cores to its GPU whatever the machine has; the CPU legs use that share
+    (or fewer where fewer are visible)."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, visible)), visible
 
-        def product(out):
-            L.ref_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
-                          out.ctypes.data_as(u64p), 0)
 
-        def timed(reps):
-            return L.ref_gmp_time(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size, 0, reps)
-    else:
-        L = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
-        L.orc_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p]
-        kind = "port"
+class Gmp:
+    """mpz_mul on this host: the reference's own GmpMultiplier
+    (src/arithmetic/gmp_multiplier.cpp:10-19) built from its sources under
+    oracle/_ref (kind "reference"), else the oracle's GMP call (kind "port")."""
 
-        def product(out):
-            L.orc_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
-                          out.ctypes.data_as(u64p))
+    def __init__(self):
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        self._p = lambda x: x.ctypes.data_as(u64p)
+        ref = ROOT / "oracle" / "_ref" / "libgmpmul_ref.so"
+        if ref.exists():
+            L = ctypes.CDLL(str(ref))
+            L.ref_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p, ctypes.c_int]
+            L.ref_gmp_time.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+            L.ref_gmp_time.restype = ctypes.c_double
+            L.ref_gmp_times.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
+            self.kind = "reference"
+        else:
+            L = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+            L.orc_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p]
+            self.kind = "port"
+        self.L = L
 
-        def timed(reps):
-            out = np.zeros(a.size + b.size, np.uint64)
+    def mul(self, a, b, square=False):
+        out = np.zeros(a.size + b.size, np.uint64)
+        if self.kind == "reference":
+            self.L.ref_gmp_mul(self._p(a), a.size, self._p(b), b.size, self._p(out), 1 if square else 0)
+        else:
+            self.L.orc_gmp_mul(self._p(a), a.size, self._p(b), b.size, self._p(out))
+        return out
+
+    def times(self, a, b, square, reps):
+        """Seconds of each of `reps` products, one at a time on this thread."""
+        if self.kind == "reference":
+            t = (ctypes.c_double * reps)()
+            self.L.ref_gmp_times(self._p(a), a.size, self._p(b), b.size, 1 if square else 0, reps, t)
+            return list(t)
+        out = []
+        for _ in range(reps):
             t0 = time.perf_counter()
-            for _ in range(reps):
-                product(out)
-            return (time.perf_counter() - t0) / reps
+            self.mul(a, b, square)
+            out.append(time.perf_counter() - t0)
+        return out
 
-    want = np.zeros(a.size + b.size, np.uint64)
-    t0 = time.perf_counter()
-    product(want)
-    single = time.perf_counter() - t0
-    exact = bool(np.array_equal(want, gpu_out))
-    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
-    reps = max(1, int(budget_s / max(single, 1e-6)))
-    per = [0.0] * cores
-    threads = [threading.Thread(target=lambda i=i: per.__setitem__(i, timed(reps))) for i in range(cores)]
-    t0 = time.perf_counter()
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    wall = time.perf_counter() - t0
-    value = cores * reps / wall
+    def time_loop(self, a, b, square, reps):
+        """Mean seconds per product over `reps` back-to-back products."""
+        if self.kind == "reference":
+            return self.L.ref_gmp_time(self._p(a), a.size, self._p(b), b.size, 1 if square else 0, reps)
+        return sum(self.times(a, b, square, reps)) / reps
+
+    def throughput(self, a, b, square, budget_s, single_s, threads):
+        """Products/s of `threads` host threads each running products of this
+        shape back to back for about budget_s (at least one each)."""
+        reps = max(1, int(budget_s / max(single_s, 1e-7)))
+        per = [0.0] * threads
+        th = [threading.Thread(target=lambda i=i: per.__setitem__(i, self.time_loop(a, b, square, reps)))
+              for i in range(threads)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return threads * reps / (time.perf_counter() - t0), reps
+
+
+def cpu_baseline(gmp: Gmp, a: np.ndarray, b: np.ndarray, gpu_out: np.ndarray, budget_s: float):
+    """The k = 26 multiply through GmpMultiplier on this host's CPU share:
+    one-thread latency (median of 3) and c threads of independent products."""
+    exact = bool(np.array_equal(gmp.mul(a, b), gpu_out))
+    single = float(np.median(gmp.times(a, b, False, 3)))
+    cores, visible = cpu_share()
+    value, reps = gmp.throughput(a, b, False, budget_s, single, cores)
     sample = (f"{cores} threads x {reps} products of the same k=26 multiply "
-              f"({a.size}x{b.size} limbs); one-thread latency {single * 1e3:.1f} ms")
-    return {"value": round(value, 4), "unit": "mults/s", "cores": cores, "kind": kind,
-            "sample": sample, "single_thread_ms": round(single * 1e3, 3),
+              f"({a.size}x{b.size} limbs); one-thread latency {single * 1e3:.1f} ms (median of 3)")
+    return {"value": round(value, 4), "unit": "mults/s", "cores": cores, "cores_visible": visible,
+            "kind": gmp.kind, "sample": sample, "single_thread_ms": round(single * 1e3, 3),
             "bit_exact_vs_gpu": exact}
 
 
-def batch_replay(stream, cpu_budget: float, reps: int):
+def device_rate(stream, a, b, square, min_ms=40.0):
+    """Device-resident products of a by b (or a^2) back to back on `stream`:
+    ms per product over enough products to take about min_ms, and the GPU
+    result of the last one."""
+    da = hiprt.DeviceBuffer.from_array(a)
+    db = da if square else hiprt.DeviceBuffer.from_array(b)
+    nb = b.size if not square else a.size
+    dout = hiprt.DeviceBuffer(8 * (a.size + nb))
+    try:
+        one = time_gpu(stream, da, a.size, db, nb, dout, 3, 2)
+        reps = int(min(2000, max(10, min_ms / max(one, 1e-4))))
+        ms = time_gpu(stream, da, a.size, db, nb, dout, reps, 2)
+        return ms, reps, dout.to_array()
+    finally:
+        for x in {id(da): da, id(db): db, id(dout): dout}.values():
+            x.close()
+
+
+def host_latency(a, b, square, reps=10):
+    """Median wall time of one product through the host-buffer ABI (staging
+    and PCIe included: what HipNttMultiplier pays per call)."""
+    t = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        if square:
+            pc.sqr(a)
+        else:
+            pc.mul(a, b)
+        t.append(time.perf_counter() - t0)
+    return float(np.median(t[1:]))
+
+
+def product_entry(stream, gmp: Gmp, a, b, square, cpu_budget, reps=10):
+    """One shape: device-resident GPU rate with its roofline, host-buffer GPU
+    latency, GMP one-thread median of `reps` and all-core throughput, and the
+    GPU product checked against GMP's."""
+    nb = a.size if square else b.size
+    log2n = pc.transform_log2(a.size, nb)
+    ms, n_gpu, got = device_rate(stream, a, b, square)
+    b_alg = survey_bytes(a.size, nb, log2n, square)
+    want = gmp.mul(a, a if square else b, square)
+    gmp_t = gmp.times(a, a if square else b, square, reps)
+    single = float(np.median(gmp_t))
+    cores, _ = cpu_share()
+    allcore, _ = gmp.throughput(a, a if square else b, square, cpu_budget, single, cores) if cpu_budget > 0 \
+        else (None, 0)
+    host_s = host_latency(a, b, square)
+    gbs = b_alg / (ms * 1e-3) / 1e9
+    e = {"limbs": [int(a.size), int(nb)], "log2n": log2n, "gpu_device_ms": round(ms, 5),
+         "gpu_device_mults_per_s": round(1e3 / ms, 1), "gpu_products_timed": n_gpu,
+         "algorithmic_bytes": int(b_alg), "achieved_GBs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+         "gpu_host_buffers_ms": round(host_s * 1e3, 4),
+         "gmp_one_thread_ms": round(single * 1e3, 4), "gmp_one_thread_reps": reps,
+         "bit_exact": bool(np.array_equal(got, want))}
+    if allcore is not None:
+        e["gmp_all_core_mults_per_s"] = round(allcore, 2)
+        e["gmp_cores"] = cores
+    if not square:
+        pp = pc.product_plan(a.size, b.size)
+        e["product_plan"] = [pp["log2n"], pp["chunks"]]
+    return e
+
+
+def config1_leg(stream, gmp: Gmp, cpu_budget: float):
+    """Two exactly-2^22-bit operands (65,536 limbs each, seed 1): a*b and a*a."""
+    a, b = pc.operands(1, 1 << 22, 1 << 22)
+    return {"workload": "single multiply and square, 2^22 x 2^22 bits (65,536 limbs each), splitmix64 seed 1",
+            "multiply": product_entry(stream, gmp, a, b, False, cpu_budget),
+            "square": product_entry(stream, gmp, a, a, True, cpu_budget)}
+
+
+def crossover(rows, op, gpu_key):
+    """Smallest k of the sweep from which the GPU beats one GMP thread at
+    every larger k (None if it never does, the first k if it always does)."""
+    ks = sorted(rows)
+    best = None
+    for k in reversed(ks):
+        if rows[k][op][gpu_key] < rows[k][op]["gmp_one_thread_ms"]:
+            best = k
+        else:
+            break
+    return best
+
+
+def sweep_leg(stream, gmp: Gmp, kmin: int, kmax: int, cpu_budget: float):
+    """The size sweep: a of exactly 2^k bits, b of floor(2^k / 3) bits, seed k;
+    a*b and a^2 at each k.  Per shape: GPU rates (device-resident and through
+    host buffers), roofline, GMP one-thread median of 10 and all-core rate."""
+    from pi_compute_amd import workloads
+
+    rows = {}
+    for k in range(kmin, kmax + 1):
+        print(f"sweep k={k}", file=sys.stderr, flush=True)
+        a, b = workloads.size_sweep(k)
+        rows[k] = {"multiply": product_entry(stream, gmp, a, b, False, cpu_budget),
+                   "square": product_entry(stream, gmp, a, a, True, cpu_budget)}
+    # The reference's rule: the GPU when the smaller operand has >= 10000 limbs
+    # (src/arithmetic/int_ntt_multiplier.cpp:56-59, default src/main.cpp:63).
+    rule = {k: ((1 << (k - 6)) >= 10000, -(-((1 << k) // 3) // 64) >= 10000) for k in rows}
+    out = {"workload": "a = 2^k bits, b = floor(2^k/3) bits, splitmix64 seed k; a*b and a^2",
+           "gmp_kind": gmp.kind,
+           "k": {str(k): v for k, v in rows.items()},
+           "crossover_k": {op: {"gpu_host_buffers_vs_gmp_one_thread": crossover(rows, op, "gpu_host_buffers_ms"),
+                                "gpu_device_vs_gmp_one_thread": crossover(rows, op, "gpu_device_ms")}
+                           for op in ("multiply", "square")},
+           "threshold_rule_gpu_from_k": {"multiply": min([k for k in rows if rule[k][1]], default=None),
+                                         "square": min([k for k in rows if rule[k][0]], default=None)},
+           "all_bit_exact": all(v[op]["bit_exact"] for v in rows.values() for op in v)}
+    return out
+
+
+def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int):
     """The tree-level replay batch: 4,096 independent products (lognormal bit
     lengths, median 2^18, every third a square), operands resident in HBM.
     Times one product per call, one batch call, and a captured-graph replay."""
@@ -260,35 +404,17 @@ def batch_replay(stream, cpu_budget: float, reps: int):
         "graph_replay": {**rate(ms_graph), "graph_nodes": int(graph_nodes)},
     }
     if cpu_budget > 0:
-        res["cpu_baseline"] = batch_cpu_baseline(prods, out, o_off, cpu_budget)
+        res["cpu_baseline"] = batch_cpu_baseline(gmp, prods, out, o_off, cpu_budget)
     din.close()
     dout.close()
     return res
 
 
-def batch_cpu_baseline(prods, out, o_off, budget_s: float):
+def batch_cpu_baseline(gmp: Gmp, prods, out, o_off, budget_s: float):
     """mpz_mul (the reference's GmpMultiplier when built) over the batch in
-    order on c host threads until the time budget runs out; every product it
-    reaches is also compared with the GPU's."""
-    u64p = ctypes.POINTER(ctypes.c_uint64)
-    ref = ROOT / "oracle" / "_ref" / "libgmpmul_ref.so"
-    if ref.exists():
-        L = ctypes.CDLL(str(ref))
-        L.ref_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p, ctypes.c_int]
-        kind = "reference"
-
-        def product(a, b, sq, dst):
-            L.ref_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
-                          dst.ctypes.data_as(u64p), 1 if sq else 0)
-    else:
-        L = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
-        L.orc_gmp_mul.argtypes = [u64p, ctypes.c_size_t, u64p, ctypes.c_size_t, u64p]
-        kind = "port"
-
-        def product(a, b, sq, dst):
-            L.orc_gmp_mul(a.ctypes.data_as(u64p), a.size, b.ctypes.data_as(u64p), b.size,
-                          dst.ctypes.data_as(u64p))
-    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+    order on the host's CPU share until the time budget runs out; every product
+    it reaches is also compared with the GPU's."""
+    cores, visible = cpu_share()
     lock = threading.Lock()
     state = {"next": 0, "done": 0, "exact": True}
     deadline = time.perf_counter() + budget_s
@@ -301,8 +427,7 @@ def batch_cpu_baseline(prods, out, o_off, budget_s: float):
                     return
                 state["next"] += 1
             a, b, sq = prods[i]
-            dst = np.zeros(a.size + b.size, np.uint64)
-            product(a, b, sq, dst)
+            dst = gmp.mul(a, b, sq)
             ok = np.array_equal(dst, out[o_off[i]:o_off[i] + dst.size])
             with lock:
                 state["done"] += 1
@@ -315,60 +440,108 @@ def batch_cpu_baseline(prods, out, o_off, budget_s: float):
     for t in threads:
         t.join()
     wall = time.perf_counter() - t0
-    return {"value": round(state["done"] / wall, 2), "unit": "mults/s", "cores": cores, "kind": kind,
+    return {"value": round(state["done"] / wall, 2), "unit": "mults/s", "cores": cores, "cores_visible": visible,
+            "kind": gmp.kind,
             "sample": f"first {state['done']} of {len(prods)} batch products on {cores} threads ({wall:.1f} s)",
             "bit_exact_vs_gpu": state["exact"], "products_checked": state["done"]}
 
 
+REF_PI = ROOT / "oracle" / "_ref" / "pi_compute_ref"
+
+
 def _pi_run(args, out: Path) -> dict:
-    """One pi_compute_hip run; returns its JSON summary line plus the wall time."""
+    """One pi program run; its JSON summary line (pi_compute_hip) or its
+    "Computed D digits of pi in S seconds" line (the reference program), plus
+    the wall time."""
     t0 = time.perf_counter()
     r = subprocess.run([*args, "--output", str(out)], capture_output=True, text=True, timeout=900)
     wall = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError(f"{' '.join(args)} failed: {r.stderr.strip()[-400:]}")
-    line = next((l for l in r.stdout.splitlines() if l.startswith("{")), "{}")
-    info = json.loads(line)
+    line = next((l for l in r.stdout.splitlines() if l.startswith("{")), None)
+    if line:
+        info = json.loads(line)
+    else:
+        m = next((l for l in r.stdout.splitlines() if l.startswith("Computed ")), "")
+        secs = m.split(" in ")[1].split()[0] if " in " in m else "nan"
+        info = {"seconds": float(secs)}
     info["wall_s"] = round(wall, 3)
     return info
 
 
-def pi_leg(digits: int) -> dict:
-    """Chudnovsky pi at `digits`: the GMP-only run (the reference's algorithm,
-    our host build) against GPU runs: per-call threshold dispatch; level
-    batches with threshold or learned dispatch; and device-resident upper
-    levels whose starting level the threshold rule or the learned model picks.
-    Every digit file must equal the GMP-only file byte for byte."""
-    exe = str(ROOT / "pi_compute_amd" / "bin" / "pi_compute_hip")
-    model = ROOT / "pi_compute_amd" / "data" / "cost_model.txt"
-    runs = {
-        "gmp_only": [exe, "--digits", str(digits)],
-        "gpu_per_call_threshold": [exe, "--digits", str(digits), "--gpu", "--per-call"],
-        "gpu_levels_threshold": [exe, "--digits", str(digits), "--gpu"],
-        "gpu_device_levels_threshold": [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto"],
-    }
-    if model.exists():
-        mlp = ["--dispatch", "mlp", "--model", str(model)]
-        runs["gpu_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", *mlp]
-        runs["gpu_device_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto", *mlp]
+def _pi_runs(digits: int, runs: dict, tag: str) -> dict:
+    """The runs in order; the first one's file (the reference program's when
+    built) is the one every other file must equal byte for byte."""
     out = {"digits": digits, "runs": {}}
     with tempfile.TemporaryDirectory(prefix="pi_bench_") as d:
-        ref_bytes = None
+        ref_bytes, ref_name = None, None
         for name, args in runs.items():
-            print(f"pi leg: {name}", file=sys.stderr, flush=True)
+            print(f"{tag}: {name}", file=sys.stderr, flush=True)
             path = Path(d) / f"{name}.txt"
             info = _pi_run(args, path)
             data = path.read_bytes()
             if ref_bytes is None:
-                ref_bytes = data
+                ref_bytes, ref_name = data, name
                 info["sha256"] = hashlib.sha256(data).hexdigest()
+                info["bytes"] = len(data)
             else:
-                info["identical_to_gmp_only"] = data == ref_bytes
+                info["identical"] = data == ref_bytes
             path.unlink()
             out["runs"][name] = {k: info[k] for k in info if k != "digits"}
-    g = out["runs"]["gmp_only"]["seconds"]
-    out["speedup_vs_gmp_only"] = {k: round(g / v["seconds"], 2) for k, v in out["runs"].items() if k != "gmp_only"}
-    out["all_identical"] = all(v.get("identical_to_gmp_only", True) for v in out["runs"].values())
+    out["identity_reference"] = ref_name
+    out["all_identical"] = all(v.get("identical", True) for v in out["runs"].values())
+    return out
+
+
+def pi_leg(digits: int) -> dict:
+    """Chudnovsky pi at `digits`: the reference's own GMP-only program (built
+    from its sources, when present) and our GMP-only build of the same
+    algorithm, against GPU runs: per-call threshold dispatch; level batches
+    with threshold or learned dispatch; and device-resident upper levels whose
+    starting level the threshold rule or the learned model picks.  Every digit
+    file must equal the reference program's byte for byte."""
+    exe = str(ROOT / "pi_compute_amd" / "bin" / "pi_compute_hip")
+    model = ROOT / "pi_compute_amd" / "data" / "cost_model.txt"
+    runs = {}
+    if REF_PI.exists():
+        runs["reference_program"] = [str(REF_PI), "--digits", str(digits)]
+    runs.update({
+        "gmp_only": [exe, "--digits", str(digits)],
+        "gpu_per_call_threshold": [exe, "--digits", str(digits), "--gpu", "--per-call"],
+        "gpu_levels_threshold": [exe, "--digits", str(digits), "--gpu"],
+        "gpu_device_levels_threshold": [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto"],
+    })
+    if model.exists():
+        mlp = ["--dispatch", "mlp", "--model", str(model)]
+        runs["gpu_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", *mlp]
+        runs["gpu_device_levels_mlp"] = [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto", *mlp]
+    out = _pi_runs(digits, runs, "pi leg")
+    for base in ("reference_program", "gmp_only"):
+        if base in out["runs"]:
+            g = out["runs"][base]["seconds"]
+            out[f"speedup_vs_{base}"] = {k: round(g / v["seconds"], 2) for k, v in out["runs"].items()
+                                         if k.startswith("gpu_")}
+    return out
+
+
+def config3_leg(digits: int = 10_000_000) -> dict:
+    """Chudnovsky pi at 10M digits with one GPU call per product above the
+    threshold (the reference's depth-first tree, its mpf final step and host
+    conversion), against the GMP-only runs; files byte-identical."""
+    exe = str(ROOT / "pi_compute_amd" / "bin" / "pi_compute_hip")
+    runs = {}
+    if REF_PI.exists():
+        runs["reference_program"] = [str(REF_PI), "--digits", str(digits)]
+    runs["gmp_only"] = [exe, "--digits", str(digits)]
+    runs["gpu_per_call"] = [exe, "--digits", str(digits), "--gpu", "--tree", "recursive", "--final", "mpf",
+                            "--convert", "host"]
+    runs["gpu_device_levels"] = [exe, "--digits", str(digits), "--gpu", "--device-levels", "auto"]
+    out = _pi_runs(digits, runs, "config 3")
+    out["workload"] = ("10M-digit Chudnovsky, 705,145 terms: one GPU call per product above the 10,000-limb "
+                       "threshold, depth-first tree, mpf final step and host conversion (gpu_per_call); "
+                       "the device-resident pipeline beside it")
+    base = out["runs"].get("reference_program", out["runs"]["gmp_only"])["seconds"]
+    out["speedup_vs_first"] = {k: round(base / v["seconds"], 2) for k, v in out["runs"].items() if k.startswith("gpu_")}
     return out
 
 
@@ -389,6 +562,13 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true", help="tune the tile plans instead of using the profiled ones")
     ap.add_argument("--pi-digits", type=int, default=100_000_000,
                     help="digits of the pi leg (GMP-only vs GPU runs of pi_compute_hip); 0 skips it")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 2^22 x 2^22-bit single-product leg")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the size sweep")
+    ap.add_argument("--sweep", default="16:28", metavar="KMIN:KMAX", help="k range of the size sweep")
+    ap.add_argument("--sweep-budget", type=float, default=0.4,
+                    help="seconds of all-core GMP work per sweep shape (0: one-thread medians only)")
+    ap.add_argument("--config3-digits", type=int, default=10_000_000,
+                    help="digits of the per-call pi leg (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -474,10 +654,25 @@ def main() -> int:
         "passes_ms_square": {k: round(v, 4) for k, v in passes_sq.items()},
         "pass_GBs": {k: round(pb[k] / (passes[k] * 1e-3) / 1e9, 1) for k in passes},
     }
+    print(f"k=26 multiply {1e3 / ms_mul:.1f}/s, square {1e3 / ms_sqr:.1f}/s", file=sys.stderr, flush=True)
+    gmp = Gmp()
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(a, b, gpu_out, args.cpu_budget)
+        result["cpu_baseline"] = cpu_baseline(gmp, a, b, gpu_out, args.cpu_budget)
+    for x in (da, db, dout, dsq):
+        x.close()
+    cpu_each = 0.0 if args.no_cpu else args.sweep_budget
+    if not args.no_config1:
+        print("config 1", file=sys.stderr, flush=True)
+        result["config1"] = config1_leg(stream, gmp, max(cpu_each, 1.0) if cpu_each else 0.0)
+    if not args.no_sweep:
+        kmin, kmax = (int(x) for x in args.sweep.split(":"))
+        result["size_sweep"] = sweep_leg(stream, gmp, kmin, kmax, cpu_each)
     if not args.no_batch:
-        result["batch_replay"] = batch_replay(stream, 0 if args.no_cpu else args.cpu_budget / 2, args.batch_reps)
+        print("batch replay", file=sys.stderr, flush=True)
+        result["batch_replay"] = batch_replay(stream, gmp, 0 if args.no_cpu else args.cpu_budget / 2,
+                                              args.batch_reps)
+    if args.config3_digits:
+        result["config3"] = config3_leg(args.config3_digits)
     if args.pi_digits:
         result["pi"] = pi_leg(args.pi_digits)
     print(json.dumps(result), flush=True)

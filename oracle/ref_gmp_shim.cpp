@@ -62,4 +62,27 @@ double ref_gmp_time(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, 
     return ((t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec)) / (reps > 0 ? reps : 1);
 }
 
+// Times `reps` products one at a time (each timed on its own, for medians);
+// seconds per product into times[0 .. reps).
+void ref_gmp_times(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, int square, int reps,
+                   double* times) {
+    mpz_t x, y, r;
+    mpz_init(x);
+    mpz_init(y);
+    mpz_init(r);
+    mpz_import(x, an, -1, sizeof(uint64_t), 0, 0, a);
+    mpz_import(y, bn, -1, sizeof(uint64_t), 0, 0, b);
+    for (int i = 0; i < reps; ++i) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        if (square) mult().square(r, x);
+        else mult().multiply(r, x, y);
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        times[i] = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    }
+    mpz_clear(x);
+    mpz_clear(y);
+    mpz_clear(r);
+}
+
 }  // extern "C"

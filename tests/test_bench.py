@@ -56,3 +56,43 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_crossover_is_the_first_k_from_which_the_gpu_always_wins():
+    def row(gpu, cpu):
+        return {"multiply": {"gpu_host_buffers_ms": gpu, "gmp_one_thread_ms": cpu}}
+
+    rows = {16: row(1.0, 0.5), 17: row(1.0, 2.0), 18: row(3.0, 2.0), 19: row(1.0, 4.0), 20: row(1.0, 9.0)}
+    assert bench.crossover(rows, "multiply", "gpu_host_buffers_ms") == 19
+    rows[18] = row(1.0, 2.0)
+    assert bench.crossover(rows, "multiply", "gpu_host_buffers_ms") == 17
+    assert bench.crossover({16: row(2.0, 1.0)}, "multiply", "gpu_host_buffers_ms") is None
+
+
+def test_gmp_helper_matches_the_oracle():
+    import numpy as np
+
+    import pi_compute_amd as pc
+
+    gmp = bench.Gmp()
+    a, b = pc.operands(3, 64 * 700, 64 * 300)
+    want = pc.to_int(a) * pc.to_int(b)
+    assert pc.to_int(gmp.mul(a, b)) == want
+    assert pc.to_int(gmp.mul(a, a, True)) == pc.to_int(a) ** 2
+    t = gmp.times(a, b, False, 5)
+    assert len(t) == 5 and all(x > 0 for x in t)
+    rate, reps = gmp.throughput(a, b, False, 0.05, float(np.median(t)), 2)
+    assert rate > 0 and reps >= 1
+
+
+def test_pi_runs_compare_files_and_read_both_programs(tmp_path):
+    exe = ROOT / "pi_compute_amd" / "bin" / "pi_compute_hip"
+    runs = {}
+    if bench.REF_PI.exists():
+        runs["reference_program"] = [str(bench.REF_PI), "--digits", "3000"]
+    runs["gmp_only"] = [str(exe), "--digits", "3000"]
+    runs["gmp_only_recursive"] = [str(exe), "--digits", "3000", "--tree", "recursive"]
+    out = bench._pi_runs(3000, runs, "test")
+    assert out["all_identical"] and out["identity_reference"] == next(iter(runs))
+    first = out["runs"][next(iter(runs))]
+    assert first["bytes"] == 3002 and first["seconds"] >= 0
