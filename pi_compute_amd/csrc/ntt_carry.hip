@@ -15,20 +15,25 @@ namespace {
 // carry-out for carry-in 0, and a larger carry-in adds exactly one more carry
 // when it overflows the run's first word while every later word is all ones
 // (t = 8: never).  These step maps are closed under composition and compose
-// with a few compares.
+// with a few compares.  A word's map is constant unless its sum's low half is
+// within 7 of 2^32, and a run's map is constant as soon as one of its words'
+// maps is.
 //
-// k_crt_carry: a workgroup owns kCarryWordsPerBlock words of one group.  It
-// reads the residues of every chunk product that reaches them with coalesced
-// 16-byte loads, writes their CRT values to LDS (three words per position),
-// and each thread then sums the kCarryWordsPerThread consecutive words it
-// owns.  The threads' carry maps are scanned across the workgroup; the words
-// are written as if the workgroup's carry-in were 0, and its own map goes to
-// the group's map array.
-// k_carry_fixup: one wave per workgroup of the first kernel composes the maps
-// of the group's earlier workgroups into the true carry-in and adds it at the
-// workgroup's first word, rippling through words that are all ones, never past
-// the workgroup's last word (what spills out is already counted in the next
-// workgroup's carry-in).  No workgroup ever waits for another.
+// k_crt_carry: a workgroup sums kCarryWindowWords + kCarryWordsPerBlock
+// consecutive words; each thread loads the residues of its own 8 positions of
+// every chunk product that reaches them (two 16-byte loads per plane), forms
+// their CRT values and adds them into its word sums; the terms that spill into
+// the next thread's first two words pass by lane shuffles (and through LDS
+// across waves).  The threads' carry maps are scanned across the workgroup.
+// The window's map is almost always constant: then it gives the block's exact
+// carry-in and the block's words are written final.  Otherwise they are
+// written as if the carry-in were 0 and the block's map is flagged.  Every
+// block stores its map (the effect of its own words) in the group's map array.
+// k_carry_fixup: one wave per block; a flagged block composes the maps of the
+// group's earlier blocks into its true carry-in and adds it at its first word,
+// rippling through words that are all ones, never past its last word (what
+// spills out is already counted in the next block's carry-in).  No workgroup
+// ever waits for another.
 
 // Map encoding: b in bits 0-2, t in bits 3-6; bit 7 marks the identity.
 constexpr uint32_t kFnIdentity = 1u << 7;
@@ -114,100 +119,114 @@ __device__ __forceinline__ uint32_t find_job(const JobList<CarryJob>& jobs, uint
 
 constexpr int kCT = kCarryThreads;
 constexpr int kWT = kCarryWordsPerThread;
-constexpr int kWB = kCarryWordsPerBlock;
-constexpr int kGroups = kWT / 4;  // groups of four consecutive positions per thread
-static_assert(kWT % 4 == 0 && kWB == kWT * kCT, "each thread loads kWT / 4 groups of four positions");
-// LDS slot of word j of the workgroup, j in [-2, kWB): the two words before the
-// workgroup first, then one pad word after every kWT so that the threads'
-// kWT-word runs start on different banks.
-__device__ __forceinline__ uint32_t xslot(int j) {
-    const uint32_t i = (uint32_t)(j + 2);
-    return i + i / kWT;
+constexpr int kWB = kCarryWordsPerBlock;                  // words a workgroup writes
+constexpr int kWinThreads = kCarryWindowWords / kWT;      // threads whose words form the window
+static_assert(kWT == 8 && kCarryWindowWords % kWT == 0 && kWinThreads <= 32 && kCT % 64 == 0, "carry layout");
+constexpr uint32_t kFnUnresolved = 1u << 8;  // map flag: the block's carry-in is not yet in its words
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d) {
+    const uint32_t lo = __shfl_up((uint32_t)v, d, 64), hi = __shfl_up((uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
 }
-constexpr int kXWords = (kWB + 2) + (kWB + 2) / kWT + 1;
 
 // grid = carry workgroups of the launch; jobs sorted by blk0.  SET: the prime
 // set the residues belong to.
 template <int SET>
 __global__ void __launch_bounds__(kCarryThreads)
     k_crt_carry(const JobList<CarryJob> jobs, uint32_t njobs, int log2n, CrtConst cc) {
-    __shared__ uint32_t X[3][kXWords];
     __shared__ uint32_t waves[kCT / 64];
-    __shared__ uint32_t sh_job;
+    __shared__ uint64_t edge[kCT / 64][2];  // a wave's last lane's spill terms, for the next wave's lane 0
+    __shared__ uint64_t head[2][2];         // the spill terms of the two positions before the first word
+    __shared__ uint32_t sh_job, sh_win;
     if (threadIdx.x == 0) sh_job = find_job(jobs, njobs, blockIdx.x);
     __syncthreads();
     const CarryJob job = PINTT_JOB(CarryJob, jobs, sh_job);
-    const uint32_t lb = blockIdx.x - job.blk0;  // workgroup within the group
-    const long long j0 = (long long)lb * kWB;   // its first word
+    const uint32_t lb = blockIdx.x - job.blk0;  // block within the group
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const long long first = (long long)lb * kWB - kCarryWindowWords;  // first word summed
+    const long long P = first + kWT * t;                               // the thread's first word
     const size_t n = (size_t)1 << log2n;
-    const int t = threadIdx.x;
 
     uint64_t s[kWT];
 #pragma unroll
     for (int e = 0; e < kWT; ++e) s[e] = 0;
+    uint64_t ea = 0, eb = 0;  // terms of the next thread's words 0 and 1
+    uint64_t ha = 0, hb = 0;  // threads 0, 1: terms of words `first` and `first` + 1
     // Constant chunk indices keep the job (a kernel-argument copy) in registers.
 #pragma unroll
     for (uint32_t i = 0; i < (uint32_t)kMaxChunks; ++i) {
         if (i >= job.m) break;
-        const long long k0 = j0 - (long long)job.off[i];  // chunk position of word 0
+        const long long k0 = first - (long long)job.off[i];  // chunk position of word `first`
         const long long len = job.len[i];
-        if (k0 + kWB <= 0 || k0 - 2 >= len) continue;  // no position of this chunk reaches the words
+        if (k0 + kCT * kWT <= 0 || k0 - 2 >= len) continue;  // no position of this chunk reaches the words
         const global_ptr<const uint32_t> res = as_global(job.res[i]);
-        // Positions k0 + 4t + 4 kCT v + u (v < kGroups, u < 4): all loads first, then the CRTs.
-        uint32_t r[kGroups][3][4];
+        const long long k = k0 + kWT * t;
+        uint32_t r[3][kWT];
+        if (k >= 0 && k + kWT <= len) {
 #pragma unroll
-        for (int v = 0; v < kGroups; ++v) {
-            const long long k = k0 + 4 * t + 4 * kCT * v;
-            if (k >= 0 && k + 4 <= len) {
+            for (int q = 0; q < 3; ++q)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k);
-                    r[v][q][0] = x.x;
-                    r[v][q][1] = x.y;
-                    r[v][q][2] = x.z;
-                    r[v][q][3] = x.w;
+                for (int h = 0; h < kWT; h += 4) {
+                    const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k + h);
+                    r[q][h] = x.x;
+                    r[q][h + 1] = x.y;
+                    r[q][h + 2] = x.z;
+                    r[q][h + 3] = x.w;
                 }
-            } else {
+        } else {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const bool in = k + u >= 0 && k + u < len;
+            for (int u = 0; u < kWT; ++u) {
+                const bool in = k + u >= 0 && k + u < len;
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) r[v][q][u] = in ? res[q * n + k + u] : 0u;
-                }
+                for (int q = 0; q < 3; ++q) r[q][u] = in ? res[q * n + k + u] : 0u;
             }
         }
 #pragma unroll
-        for (int v = 0; v < kGroups; ++v)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                uint32_t x0, x1, x2;
-                crt_words<SET>(r[v][0][u], r[v][1][u], r[v][2][u], cc, x0, x1, x2);
-                const uint32_t sl = xslot(4 * t + 4 * kCT * v + u);
-                X[0][sl] = x0;
-                X[1][sl] = x1;
-                X[2][sl] = x2;
-            }
-        if (t < 2) {  // the two positions before the workgroup's first word
-            const long long k = k0 - 2 + t;
-            const bool in = k >= 0 && k < len;
+        for (int u = 0; u < kWT; ++u) {
             uint32_t x0, x1, x2;
-            crt_words<SET>(in ? res[k] : 0u, in ? res[n + k] : 0u, in ? res[2 * n + k] : 0u, cc, x0, x1, x2);
-            const uint32_t sl = xslot(t - 2);
-            X[0][sl] = x0;
-            X[1][sl] = x1;
-            X[2][sl] = x2;
+            crt_words<SET>(r[0][u], r[1][u], r[2][u], cc, x0, x1, x2);
+            s[u] += x0;
+            if (u + 1 < kWT) s[u + 1] += x1;
+            else ea += x1;
+            if (u + 2 < kWT) s[u + 2] += x2;
+            else if (u + 2 == kWT) ea += x2;
+            else eb += x2;
         }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < kWT; ++e) {
-            const int jj = kWT * t + e;
-            s[e] += (uint64_t)X[0][xslot(jj)] + X[1][xslot(jj - 1)] + X[2][xslot(jj - 2)];
+        if (t < 2) {  // positions first - 2 and first - 1
+            const long long kk = k0 - 2 + t;
+            const bool in = kk >= 0 && kk < len;
+            uint32_t x0, x1, x2;
+            crt_words<SET>(in ? res[kk] : 0u, in ? res[n + kk] : 0u, in ? res[2 * n + kk] : 0u, cc, x0, x1, x2);
+            if (t == 0) {
+                ha += x2;
+            } else {
+                ha += x1;
+                hb += x2;
+            }
         }
-        __syncthreads();
     }
+    // Spill terms from the thread before: lane shuffles inside a wave, LDS
+    // across waves and, for the first thread, the two positions before.
+    const uint64_t pa = shfl_up_u64(ea, 1), pb = shfl_up_u64(eb, 1);
+    if (lane == 63) {
+        edge[wave][0] = ea;
+        edge[wave][1] = eb;
+    }
+    if (t < 2) {
+        head[t][0] = ha;
+        head[t][1] = hb;
+    }
+    __syncthreads();
+    uint64_t ia = pa, ib = pb;
+    if (lane == 0) {
+        ia = wave ? edge[wave - 1][0] : head[0][0] + head[1][0];
+        ib = wave ? edge[wave - 1][1] : head[0][1] + head[1][1];
+    }
+    s[0] += ia;
+    s[1] += ib;
 #pragma unroll
     for (int e = 0; e < kWT; ++e)
-        if ((uint64_t)(j0 + kWT * t + e) >= job.nwords) s[e] = 0;
+        if (P + e < 0 || (uint64_t)(P + e) >= job.nwords) s[e] = 0;
 
     // Carry map of this thread's run (computed with carry-in 0).
     uint32_t f;
@@ -228,47 +247,56 @@ __global__ void __launch_bounds__(kCarryThreads)
         const uint32_t tt = (ones && w0 != 0 && room <= 7u) ? room : 8u;
         f = fn_make((uint32_t)c, tt);
     }
+    // The window's map (threads 0 .. kWinThreads - 1, in wave 0).
+    uint32_t wm = f;
+#pragma unroll
+    for (int off = 1; off < kWinThreads; off <<= 1) {
+        const uint32_t other = __shfl_up(wm, off, 64);
+        if (lane >= off) wm = fn_compose(wm, other);
+    }
+    if (t == kWinThreads - 1) sh_win = wm;
+    // Maps of the block's own threads; the window counts as the identity.
     uint32_t agg;
-    const uint32_t excl = block_scan_fn<kCT>(f, waves, &agg);
-    if (t == 0) as_global(job.maps)[lb] = agg;
-
-    // Words with the workgroup's carry-in taken as 0, through LDS so that the
-    // stores are coalesced.
+    const bool in_window = t < kWinThreads;
+    const uint32_t excl = block_scan_fn<kCT>(in_window ? kFnIdentity : f, waves, &agg);
+    const uint32_t win = sh_win;
+    const bool resolved = ((win >> 3) & 15u) >= 8u;  // constant: the carry-in is the same for every carry into the window
+    const uint32_t cin = resolved ? (win & 7u) : 0u;
+    if (t == 0) as_global(job.maps)[lb] = agg | (resolved ? 0u : kFnUnresolved);
+    if (in_window) return;
+    uint32_t w[kWT];
     {
-        uint64_t c = fn_apply(excl, 0);
+        uint64_t c = fn_apply(excl, cin);
 #pragma unroll
         for (int e = 0; e < kWT; ++e) {
             const uint64_t v = s[e] + c;
-            X[0][xslot(kWT * t + e)] = (uint32_t)v;
+            w[e] = (uint32_t)v;
             c = v >> 32;
         }
     }
-    __syncthreads();
     const global_ptr<uint32_t> out = as_global(job.out);
+    const uint64_t j = (uint64_t)P;  // >= 0 outside the window
+    if (j + kWT <= job.nwords) {
+        *(global_ptr<u32x4>)(out + j) = u32x4{w[0], w[1], w[2], w[3]};
+        *(global_ptr<u32x4>)(out + j + 4) = u32x4{w[4], w[5], w[6], w[7]};
+    } else {
 #pragma unroll
-    for (int v = 0; v < kGroups; ++v) {
-        const int jj = 4 * t + 4 * kCT * v;
-        const uint64_t j = (uint64_t)j0 + jj;
-        if (j + 4 <= job.nwords) {
-            *(global_ptr<u32x4>)(out + j) =
-                u32x4{X[0][xslot(jj)], X[0][xslot(jj + 1)], X[0][xslot(jj + 2)], X[0][xslot(jj + 3)]};
-        } else {
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (j + u < job.nwords) out[j + u] = X[0][xslot(jj + u)];
-        }
+        for (int e = 0; e < kWT; ++e)
+            if (j + e < job.nwords) out[j + e] = w[e];
     }
 }
 
-// One wave per carry workgroup: its true carry-in, rippled into its words.
+// One wave per block of the first kernel: a flagged block's true carry-in,
+// rippled into its words.
 __global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs, uint32_t njobs) {
     const uint32_t lane = threadIdx.x;
     const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
     const uint32_t lb = blockIdx.x - job.blk0;
     if (lb == 0) return;
-    // Maps of workgroups 0 .. lb-1 in order, 1024 per round: 16 consecutive
-    // maps per lane, all loaded before any is composed.
     const global_ptr<const uint32_t> maps = as_global(job.maps);
+    if (!(maps[lb] & kFnUnresolved)) return;  // its window fixed its carry-in
+    // Maps of blocks 0 .. lb-1 in order, 1024 per round: 16 consecutive
+    // maps per lane, all loaded before any is composed.
     uint32_t total = kFnIdentity;  // composition of the maps of the rounds so far
     for (uint32_t base = 0; base < lb; base += 1024) {
         const uint32_t lo = base + 16 * lane;

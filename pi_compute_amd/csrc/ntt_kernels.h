@@ -47,9 +47,13 @@ struct CrtConst {
 
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
-constexpr int kCarryThreads = 512;
+constexpr int kCarryThreads = 256;
 constexpr int kCarryWordsPerThread = 8;
-constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread;
+// A carry workgroup sums kCarryThreads * kCarryWordsPerThread words: the
+// kCarryWindowWords before its block, whose sums almost always fix the block's
+// carry-in, then the kCarryWordsPerBlock words of the block, which it writes.
+constexpr int kCarryWindowWords = 32;
+constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread - kCarryWindowWords;
 constexpr int kMaxChunks = 4;  // pieces the longer operand of one product may be cut into
 
 // Work descriptors.  One launch of a pass processes a bucket of products that

@@ -262,6 +262,39 @@ def test_split_products_scaled_rows(engine, oracle, chunks, an, bn):
     assert np.array_equal(engine.sqr(b), oracle.gmp_mul(b, b)), "square"
 
 
+def _from_coefficients(c):
+    """64-bit limbs from 32-bit coefficients (little endian)."""
+    c = np.asarray(c, np.uint64)
+    if c.size % 2:
+        c = np.append(c, np.uint64(0))
+    return c[0::2] | (c[1::2] << np.uint64(32))
+
+
+@pytest.mark.parametrize("chunks", [1, 2, 3])
+def test_carry_through_unresolved_windows(engine, oracle, chunks):
+    """Word sums of exactly 2^32 - 1 over many carry blocks with a carry
+    entering them: a = (2^32 - 1)(1 + x^m + ... + x^(L-1)), b = 1 + x^m in
+    32-bit coefficients, so coefficients m+1 .. 2m-1 of a*b sum to 2^32 - 1 and
+    pass on the carry out of coefficient m.  No block there can fix its carry-in
+    from the words before it; the fixup kernel must supply it.  Also a * 1
+    (every word sum 2^32 - 1, no carry at all)."""
+    m, L = 20000, 50000
+    ca = np.zeros(L, np.uint64)
+    ca[0] = 0xFFFFFFFF
+    ca[m:] = 0xFFFFFFFF
+    cb = np.zeros(m + 1, np.uint64)
+    cb[0] = cb[m] = 1
+    a, b = _from_coefficients(ca), _from_coefficients(cb)
+    ones = np.full(30000, np.uint64(2**64 - 1))
+    one = np.array([1], np.uint64)
+    engine.set_split(chunks if chunks > 1 else 1)
+    try:
+        _check(engine, oracle, a, b, f"unresolved windows, {chunks} chunks")
+        _check(engine, oracle, ones, one, "all-ones times one")
+    finally:
+        engine.set_split(1)
+
+
 def test_split_all_ones_carry_chains(engine, oracle):
     # Overlapping chunk products whose sums ripple carries across chunk borders.
     a = np.full(50000, np.uint64(0xFFFFFFFFFFFFFFFF))
