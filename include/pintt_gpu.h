@@ -196,6 +196,23 @@ int gpu_ntt_transform_log2(size_t an, size_t bn);
  * reference sizes one transform per call (src/gpu/int_ntt.cu:305-314). */
 int gpu_ntt_product_plan(size_t an, size_t bn, int* log2n, int* chunks, size_t* chunk_limbs);
 
+/* Chunk counts priced by the caller (the learned dispatcher).  For an an x bn
+ * product (an >= bn, not a square) the function writes predicted costs of
+ * cutting the longer operand into m = 1 .. 4 chunks to cost[m - 1] and returns
+ * nonzero, or returns 0 to leave the choice to the engine's tuned per-length
+ * costs; the engine takes the cheapest cut its rules allow.  It is called
+ * with the engine lock held while a batch is planned and must not call into
+ * this library.  NULL restores the engine's own choice.  Forced splits
+ * (gpu_ntt_set_split with 0 or 2..4) take precedence.  Replaces the
+ * reference's one transform length per product (src/gpu/int_ntt.cu:303-314). */
+typedef int (*pintt_chunk_cost_fn)(size_t an, size_t bn, double* cost, void* ctx);
+void gpu_ntt_set_chunk_cost(pintt_chunk_cost_fn fn, void* ctx);
+
+/* Whether an an x bn product (an >= bn) can be cut into `chunks` pieces
+ * (1 = not cut): returns 1 and the transform length 2^*log2n it then uses,
+ * else 0.  Pure arithmetic: needs no device. */
+int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n);
+
 /* Largest an + bn accepted. */
 size_t gpu_ntt_max_limbs(void);
 

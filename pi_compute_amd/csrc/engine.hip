@@ -66,7 +66,7 @@ constexpr size_t kNaPow = kNaRow + kPrimes * kNaRowWords;
 constexpr size_t kNaArena = kNaPow + words_up(kPowScratch);
 constexpr size_t kNaArenaWords = 2 * kPrimes * kNoAllocN;  // two slots; m <= 3 chunks of half the length fit too
 constexpr size_t kNaMaps = kNaArena + kNaArenaWords;
-constexpr size_t kNaMapWords = words_up(kNoAllocN / kCarryWordsPerBlock + 1);
+constexpr size_t kNaMapWords = words_up(kNoAllocN / carry_block_words(kCarryMinThreads) + 1);
 constexpr size_t kNaWords = kNaMaps + kNaMapWords;
 static_assert(PINTT_NOALLOC_MAX_LIMBS == ((size_t)1 << (kNoAllocMaxLog2N - 1)), "limit in the header");
 }  // namespace
@@ -146,6 +146,9 @@ struct Engine {
     // hipGetSymbolAddress) and the tables built in it at initialisation.
     uint32_t* na_base = nullptr;
     Twiddles na_tw{};
+    // Caller's costs of chunk counts (gpu_ntt_set_chunk_cost), or null.
+    pintt_chunk_cost_fn chunk_cost = nullptr;
+    void* chunk_ctx = nullptr;
 };
 
 Engine g;
@@ -443,6 +446,28 @@ double transform_cost(int k, int transforms) {
     return base * transforms;
 }
 
+// The cut of an an x bn product (an >= bn) into m chunks: chunk limbs *h (a
+// multiple of 16 when m > 1, so chunk offsets stay multiples of 32
+// coefficients) and transform length 2^*k; false when the rules forbid it (a
+// chunk shorter than b, an empty last chunk, no shorter transform).
+bool chunk_shape(size_t an, size_t bn, uint32_t m, size_t* h, int* k) {
+    const int full = log2_for(an, bn);
+    if (full < 0 || m < 1 || m > (uint32_t)kMaxChunks) return false;
+    if (m == 1) {
+        *h = an;
+        *k = full;
+        return true;
+    }
+    size_t c = (an + m - 1) / m;
+    c = (c + 15) & ~(size_t)15;
+    if (c < bn || (m - 1) * c >= an) return false;  // at most two chunk products per word
+    const int kk = log2_for(c, bn);
+    if (kk < 0 || kk >= full) return false;
+    *h = c;
+    *k = kk;
+    return true;
+}
+
 int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{}) {
     Request r = req;
     if (!r.square && r.an < r.bn) {
@@ -454,6 +479,31 @@ int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{})
     gr->h = r.an;
     gr->log2n = log2_for(r.an, r.bn);
     if (gr->log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
+    if (!r.square && g.split_mode == 1 && g.chunk_cost) {
+        // The caller's prices, cheapest allowed cut first.
+        double cost[kMaxChunks];
+        if (g.chunk_cost(r.an, r.bn, cost, g.chunk_ctx)) {
+            double best = 0;
+            bool found = false;
+            for (uint32_t m = 1; m <= mode.max_chunks && m <= (uint32_t)kMaxChunks; ++m) {
+                size_t h;
+                int k;
+                if (!chunk_shape(r.an, r.bn, m, &h, &k) || !(cost[m - 1] == cost[m - 1])) continue;
+                if (!found || cost[m - 1] < best) {
+                    found = true;
+                    best = cost[m - 1];
+                    gr->m = m;
+                    gr->h = h;
+                    gr->log2n = k;
+                }
+            }
+            if (found) {
+                gr->nblk = (2 * (r.an + r.bn) + carry_block_words(carry_threads()) - 1) /
+                           carry_block_words(carry_threads());
+                return PINTT_OK;
+            }
+        }
+    }
     if (!r.square && g.split_mode) {
         if (mode.tune && g.autotune && g.split_mode == 1 && !g.have_plan[gr->log2n]) {
             Plan tmp;
@@ -461,11 +511,9 @@ int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{})
         }
         double best = transform_cost(gr->log2n, 3);
         for (uint32_t m = 2; m <= mode.max_chunks; ++m) {
-            size_t h = (r.an + m - 1) / m;
-            h = (h + 15) & ~(size_t)15;  // chunk offsets stay multiples of 32 coefficients
-            if (h < r.bn || (m - 1) * h >= r.an) continue;  // at most two chunk products per word
-            int k = log2_for(h, r.bn);
-            if (k < 0 || k >= gr->log2n) continue;
+            size_t h;
+            int k;
+            if (!chunk_shape(r.an, r.bn, m, &h, &k)) continue;
             if (mode.tune && g.autotune && g.split_mode == 1 && !g.have_plan[k]) {
                 Plan tmp;
                 PINTT_TRY(plan_for(k, &tmp));
@@ -480,7 +528,8 @@ int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{})
             }
         }
     }
-    gr->nblk = (2 * (r.an + r.bn) + kCarryWordsPerBlock - 1) / kCarryWordsPerBlock;
+    const size_t wb = carry_block_words(carry_threads());
+    gr->nblk = (2 * (r.an + r.bn) + wb - 1) / wb;
     return PINTT_OK;
 }
 
@@ -1154,7 +1203,11 @@ void gpu_ntt_shutdown(void) {
     for (auto& e : g.ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(g.done);
     (void)hipStreamDestroy(g.stream);
+    const pintt_chunk_cost_fn cf = g.chunk_cost;  // a caller's registration outlives the device state
+    void* const cc = g.chunk_ctx;
     g = Engine{};
+    g.chunk_cost = cf;
+    g.chunk_ctx = cc;
 }
 
 int gpu_ntt_mul(const uint64_t* a, size_t an, const uint64_t* b, size_t bn, uint64_t* out) {
@@ -1644,6 +1697,21 @@ int gpu_ntt_product_plan(size_t an, size_t bn, int* log2n, int* chunks, size_t* 
 }
 
 size_t gpu_ntt_max_limbs(void) { return ((size_t)1 << (kMaxLog2N - 1)); }
+
+void gpu_ntt_set_chunk_cost(pintt_chunk_cost_fn fn, void* ctx) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g.chunk_cost = fn;
+    g.chunk_ctx = ctx;
+}
+
+int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n) {
+    if (an < bn) std::swap(an, bn);
+    size_t h;
+    int k;
+    if (an == 0 || bn == 0 || chunks < 1 || !chunk_shape(an, bn, (uint32_t)chunks, &h, &k)) return 0;
+    if (log2n) *log2n = k;
+    return 1;
+}
 
 const char* gpu_ntt_last_error(void) { return t_error.c_str(); }
 

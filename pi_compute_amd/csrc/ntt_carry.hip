@@ -1,6 +1,8 @@
 // ntt_carry.hip - CRT + carry pass and the twiddle-table builder (gfx950).
 #include "ntt_device.h"
 
+#include <cstdlib>
+
 namespace pintt {
 namespace {
 
@@ -19,8 +21,8 @@ namespace {
 // within 7 of 2^32, and a run's map is constant as soon as one of its words'
 // maps is.
 //
-// k_crt_carry: a workgroup sums kCarryWindowWords + kCarryWordsPerBlock
-// consecutive words; each thread loads the residues of its own 8 positions of
+// k_crt_carry: a workgroup of CT threads sums 8 CT consecutive words (the
+// kCarryWindowWords of the window, then its block); each thread loads the residues of its own 8 positions of
 // every chunk product that reaches them (two 16-byte loads per plane), forms
 // their CRT values and adds them into its word sums; the terms that spill into
 // the next thread's first two words pass by lane shuffles (and through LDS
@@ -117,11 +119,9 @@ __device__ __forceinline__ uint32_t find_job(const JobList<CarryJob>& jobs, uint
     return lo;
 }
 
-constexpr int kCT = kCarryThreads;
 constexpr int kWT = kCarryWordsPerThread;
-constexpr int kWB = kCarryWordsPerBlock;                  // words a workgroup writes
 constexpr int kWinThreads = kCarryWindowWords / kWT;      // threads whose words form the window
-static_assert(kWT == 8 && kCarryWindowWords % kWT == 0 && kWinThreads <= 32 && kCT % 64 == 0, "carry layout");
+static_assert(kWT == 8 && kCarryWindowWords % kWT == 0 && kWinThreads <= 32, "carry layout");
 constexpr uint32_t kFnUnresolved = 1u << 8;  // map flag: the block's carry-in is not yet in its words
 
 __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d) {
@@ -130,10 +130,13 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d) {
 }
 
 // grid = carry workgroups of the launch; jobs sorted by blk0.  SET: the prime
-// set the residues belong to.
-template <int SET>
-__global__ void __launch_bounds__(kCarryThreads)
+// set the residues belong to; CT: threads per workgroup.
+template <int SET, int CT>
+__global__ void __launch_bounds__(CT)
     k_crt_carry(const JobList<CarryJob> jobs, uint32_t njobs, int log2n, CrtConst cc) {
+    constexpr int kCT = CT;
+    constexpr int kWB = carry_block_words(CT);  // words the workgroup writes
+    static_assert(kCT % 64 == 0, "whole waves");
     __shared__ uint32_t waves[kCT / 64];
     __shared__ uint64_t edge[kCT / 64][2];  // a wave's last lane's spill terms, for the next wave's lane 0
     __shared__ uint64_t head[2][2];         // the spill terms of the two positions before the first word
@@ -287,8 +290,10 @@ __global__ void __launch_bounds__(kCarryThreads)
 }
 
 // One wave per block of the first kernel: a flagged block's true carry-in,
-// rippled into its words.
+// rippled into its words (blocks of carry_block_words(CT) words).
+template <int CT>
 __global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs, uint32_t njobs) {
+    constexpr int kWB = carry_block_words(CT);
     const uint32_t lane = threadIdx.x;
     const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
     const uint32_t lb = blockIdx.x - job.blk0;
@@ -423,17 +428,35 @@ hipError_t launch_build_twiddles(uint2* fwd, uint2* inv, uint32_t* row, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
-                            const CrtConst& cc, hipStream_t s) {
+int carry_threads() {
+    static const int t = [] {
+        const char* e = std::getenv("PINTT_CARRY_THREADS");
+        const int v = e ? std::atoi(e) : 256;
+        return v == 64 || v == 128 ? v : 256;
+    }();
+    return t;
+}
+
+template <int CT>
+hipError_t launch_crt_carry_ct(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
+                               const CrtConst& cc, hipStream_t s) {
     if (set_for(log2n) == kSetNarrow)
-        hipLaunchKernelGGL(k_crt_carry<kSetNarrow>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n,
-                           cc);
+        hipLaunchKernelGGL((k_crt_carry<kSetNarrow, CT>), dim3(nblocks), dim3(CT), 0, s, jobs, njobs, log2n, cc);
     else
-        hipLaunchKernelGGL(k_crt_carry<kSetWide>, dim3(nblocks), dim3(kCarryThreads), 0, s, jobs, njobs, log2n, cc);
+        hipLaunchKernelGGL((k_crt_carry<kSetWide, CT>), dim3(nblocks), dim3(CT), 0, s, jobs, njobs, log2n, cc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_carry_fixup, dim3(nblocks), dim3(64), 0, s, jobs, njobs);
+    hipLaunchKernelGGL(k_carry_fixup<CT>, dim3(nblocks), dim3(64), 0, s, jobs, njobs);
     return hipGetLastError();
+}
+
+hipError_t launch_crt_carry(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t nblocks, int log2n,
+                            const CrtConst& cc, hipStream_t s) {
+    switch (carry_threads()) {
+        case 64: return launch_crt_carry_ct<64>(jobs, njobs, nblocks, log2n, cc, s);
+        case 128: return launch_crt_carry_ct<128>(jobs, njobs, nblocks, log2n, cc, s);
+        default: return launch_crt_carry_ct<256>(jobs, njobs, nblocks, log2n, cc, s);
+    }
 }
 
 }  // namespace pintt

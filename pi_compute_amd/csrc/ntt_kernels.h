@@ -47,13 +47,17 @@ struct CrtConst {
 
 constexpr int kElemsPerThread = 16;
 constexpr int kMinLog2N = 10;
-constexpr int kCarryThreads = 256;
 constexpr int kCarryWordsPerThread = 8;
-// A carry workgroup sums kCarryThreads * kCarryWordsPerThread words: the
-// kCarryWindowWords before its block, whose sums almost always fix the block's
-// carry-in, then the kCarryWordsPerBlock words of the block, which it writes.
+// A carry workgroup of T threads sums 8 T words: the kCarryWindowWords before
+// its block, whose sums almost always fix the block's carry-in, then the
+// carry_block_words(T) words of the block, which it writes.  T is
+// carry_threads() (64, 128 or 256; ntt_carry.hip).
 constexpr int kCarryWindowWords = 32;
-constexpr int kCarryWordsPerBlock = kCarryThreads * kCarryWordsPerThread - kCarryWindowWords;
+constexpr int kCarryMinThreads = 64;
+__host__ __device__ constexpr uint32_t carry_block_words(int threads) {
+    return (uint32_t)(threads * kCarryWordsPerThread - kCarryWindowWords);
+}
+int carry_threads();
 constexpr int kMaxChunks = 4;  // pieces the longer operand of one product may be cut into
 
 // Work descriptors.  One launch of a pass processes a bucket of products that

@@ -9,9 +9,14 @@
 //                   and once more with device-resident operands
 //   tree level      4096 products, lognormal bit lengths (median 2^18,
 //                   sigma 0.6, clipped to [2^12, 2^24]), every third a square
+//   chunk cuts      single device-resident products of 2^10 .. 2^22 limbs by
+//                   1/1 .. 1/16 as long, with the longer operand cut into each
+//                   allowed m = 1 .. 4 chunks (forced, path "gpu_dev_m<m>")
+//                   and as the engine picks it (path "gpu_dev")
 //
 //   pi_calibrate --out FILE [--kmin 8] [--kmax 28] [--reps 10] [--limit 48]
 //                [--threads T] [--cpu-budget SECONDS] [--quick]
+//                [--cut-step OCTAVES] [--cut-reps N] [--cuts-only]
 #include <gmp.h>
 
 #include <algorithm>
@@ -26,6 +31,7 @@
 #include "dispatch.h"
 #include "hip_ntt_multiplier.h"
 #include "operands.h"
+#include "pintt_gpu.h"
 
 using namespace pi;
 using Clock = std::chrono::steady_clock;
@@ -74,7 +80,9 @@ int main(int argc, char** argv) {
     size_t limit = 48;
     unsigned threads = 0;
     double cpu_budget = 3.0;  // seconds of estimated host work per configuration
-    bool quick = false;
+    bool quick = false, cuts_only = false;
+    double cut_step = 0.125;
+    int cut_reps = 3;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto val = [&]() { return std::string(i + 1 < argc ? argv[++i] : ""); };
@@ -86,6 +94,9 @@ int main(int argc, char** argv) {
         else if (a == "--threads") threads = std::stoul(val());
         else if (a == "--cpu-budget") cpu_budget = std::stod(val());
         else if (a == "--quick") quick = true;
+        else if (a == "--cut-step") cut_step = std::stod(val());
+        else if (a == "--cut-reps") cut_reps = std::stoi(val());
+        else if (a == "--cuts-only") cuts_only = true;
         else {
             std::cerr << "unknown option " << a << std::endl;
             return 2;
@@ -106,6 +117,7 @@ int main(int argc, char** argv) {
 
     // Size sweep, single calls.
     auto t0 = Clock::now();
+    if (cuts_only) kmax = kmin - 1;
     for (int k = kmin; k <= kmax; ++k) {
         SplitMix64 g(k);
         mpz_t a, b, r;
@@ -128,7 +140,7 @@ int main(int argc, char** argv) {
     }
 
     // Uniform batches.
-    for (int lg = 4; lg <= 20; ++lg) {
+    for (int lg = 4; lg <= (cuts_only ? 3 : 20); ++lg) {
         for (size_t B = 2; B <= 4096; B *= (quick ? 8 : 2)) {
             if ((B << lg) > ((size_t)1 << 24)) break;
             SplitMix64 g(1000 * lg + B);
@@ -155,7 +167,7 @@ int main(int argc, char** argv) {
     }
 
     // Tree-level replay.
-    {
+    if (!cuts_only) {
         const size_t n = 4096;
         SplitMix64 g(7);
         std::vector<double> u(4 * n);
@@ -180,6 +192,39 @@ int main(int argc, char** argv) {
         }
         m.multiply_batch_resident(jobs.data(), jobs.size());
         std::printf("tree level done (%.1f s)\n", seconds(t0));
+    }
+    // Chunk cuts: every allowed cut of single device-resident products.
+    {
+        m.set_log_limit(0);
+        m.set_dispatcher(&gpu);
+        const double ratios[] = {1, 1.5, 2, 3, 4, 6, 8, 12, 16};
+        int shapes = 0;
+        for (double la = 10; la <= 22.51; la += quick ? 2.0 : cut_step) {
+            for (double ratio : ratios) {
+                const size_t an = (size_t)std::llround(std::exp2(la));
+                const size_t bn = std::max<size_t>(1, (size_t)std::llround((double)an / ratio));
+                if (an + bn > gpu_ntt_max_limbs()) continue;
+                SplitMix64 g(7000 + 64 * shapes);
+                Batch bt(1);
+                draw_operand(&bt.a[0], g, 64 * an);
+                draw_operand(&bt.b[0], g, 64 * bn);
+                auto jobs = bt.jobs();
+                m.multiply_batch_resident(jobs.data(), 1);  // warm-up: plans and buffers for this shape
+                for (int c = 1; c <= 4; ++c) {
+                    int log2n = 0;
+                    if (!gpu_ntt_chunk_plan(an, bn, c, &log2n)) continue;
+                    gpu_ntt_set_split(c == 1 ? 0 : c);
+                    m.set_log_tag("_m" + std::to_string(c));
+                    for (int rep = 0; rep < (quick ? 1 : cut_reps); ++rep) m.multiply_batch_resident(jobs.data(), 1);
+                }
+                gpu_ntt_set_split(1);
+                m.set_log_tag("");
+                for (int rep = 0; rep < (quick ? 1 : cut_reps); ++rep) m.multiply_batch_resident(jobs.data(), 1);
+                ++shapes;
+            }
+            std::printf("chunk cuts up to 2^%.1f limbs done (%.1f s)\n", la, seconds(t0));
+            std::fflush(stdout);
+        }
     }
     m.close_log();
     m.print_stats();

@@ -29,6 +29,13 @@ public:
     virtual void choose_batch(const ProductShape* s, size_t n, Path* out) const {
         for (size_t i = 0; i < n; ++i) out[i] = choose(s[i]);
     }
+    // Predicted costs (any monotone scale) of a device product of an x bn
+    // limbs (an >= bn) with the longer operand cut into m = 1 .. 4 chunks, in
+    // cost[m - 1]; false leaves the cut to the engine's own tuned costs.
+    virtual bool chunk_costs(size_t an, size_t bn, double* cost) const {
+        (void)an, (void)bn, (void)cost;
+        return false;
+    }
     virtual const char* name() const = 0;
 };
 

@@ -28,9 +28,19 @@ HipNttMultiplier::HipNttMultiplier(size_t threshold, int device)
     : heuristic_(threshold), dispatcher_(&heuristic_), device_(device) {
     if (gpu_ntt_init(device) != PINTT_OK)
         throw std::runtime_error(std::string("HipNttMultiplier: ") + gpu_ntt_last_error());
+    // The engine asks the dispatcher for the cut of each product (a learned
+    // model prices it; the threshold rule leaves it to the engine).
+    gpu_ntt_set_chunk_cost(&HipNttMultiplier::chunk_cost, this);
 }
 
-HipNttMultiplier::~HipNttMultiplier() { close_log(); }
+HipNttMultiplier::~HipNttMultiplier() {
+    gpu_ntt_set_chunk_cost(nullptr, nullptr);
+    close_log();
+}
+
+int HipNttMultiplier::chunk_cost(size_t an, size_t bn, double* cost, void* self) {
+    return static_cast<HipNttMultiplier*>(self)->dispatcher_->chunk_costs(an, bn, cost) ? 1 : 0;
+}
 
 std::string HipNttMultiplier::device_name() const { return "gfx950 device " + std::to_string(device_); }
 
@@ -50,8 +60,8 @@ void HipNttMultiplier::close_log() {
 void HipNttMultiplier::log_line(size_t an, size_t bn, size_t batch, bool square, const char* path, double ms) {
     std::lock_guard<std::mutex> lock(log_mu_);
     if (!log_) return;
-    std::fprintf(log_, "{\"an\":%zu,\"bn\":%zu,\"batch\":%zu,\"square\":%d,\"path\":\"%s\",\"ms\":%.6f}\n", an, bn,
-                 batch, square ? 1 : 0, path, ms);
+    std::fprintf(log_, "{\"an\":%zu,\"bn\":%zu,\"batch\":%zu,\"square\":%d,\"path\":\"%s%s\",\"ms\":%.6f}\n", an,
+                 bn, batch, square ? 1 : 0, path, log_tag_.c_str(), ms);
 }
 
 bool HipNttMultiplier::log_pick(size_t k, size_t n) const {

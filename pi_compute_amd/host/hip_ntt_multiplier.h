@@ -61,6 +61,8 @@ public:
     void close_log();
     // At most this many evenly spaced lines per path and batch (0: all).
     void set_log_limit(size_t n) { log_limit_ = n; }
+    // Appended to the path name of logged lines (calibration of forced cuts).
+    void set_log_tag(const std::string& tag) { log_tag_ = tag; }
 
     struct Stats {
         std::atomic<uint64_t> gpu_calls{0};       // products on the device
@@ -79,6 +81,7 @@ private:
     void host_product(mpz_ptr result, mpz_srcptr a, mpz_srcptr b, size_t batch, bool log);
     bool log_pick(size_t k, size_t n) const;
     void log_line(size_t an, size_t bn, size_t batch, bool square, const char* path, double ms);
+    static int chunk_cost(size_t an, size_t bn, double* cost, void* self);
 
     ThresholdDispatcher heuristic_;
     const Dispatcher* dispatcher_;
@@ -86,6 +89,7 @@ private:
     bool batching_ = true;
     std::FILE* log_ = nullptr;
     size_t log_limit_ = 0;
+    std::string log_tag_;
     std::mutex log_mu_;
     mutable Stats stats_;
 };

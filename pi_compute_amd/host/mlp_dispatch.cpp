@@ -135,6 +135,21 @@ Path MlpDispatcher::decide(const ProductShape& s, const float* y) const {
     return y[gpu_output(model_, s)] < cpu ? Path::Gpu : Path::Cpu;
 }
 
+// Output of a device-resident product cut into m chunks (models with seven outputs).
+constexpr int kChunkOutput0 = 3;
+
+bool MlpDispatcher::chunk_costs(size_t an, size_t bn, double* cost) const {
+    if (model_.outputs < kChunkOutput0 + 4) return false;
+    ProductShape s{an, bn, 1, false, true};
+    const auto f = MlpModel::features(s);
+    if (!model_.in_range(f)) return false;
+    const auto x = model_.standardise(f);
+    float y[PINTT_MLP_MAX_OUTPUTS];
+    model_.forward(x.data(), y);
+    for (int m = 0; m < 4; ++m) cost[m] = y[kChunkOutput0 + m];
+    return true;
+}
+
 Path MlpDispatcher::choose(const ProductShape& s) const {
     const auto f = MlpModel::features(s);
     if (!model_.in_range(f)) return fallback_->choose(s);

@@ -12,7 +12,10 @@
 // output 1 = GPU engine with host operands (a device batch's time, staging
 // included, shared out by transform work), output 2 (models with three) =
 // GPU engine on device-resident operands (no staging); a resident shape is
-// priced with output 2 when the model has it.
+// priced with output 2 when the model has it.  Outputs 3 .. 6 (models with
+// seven) = one device-resident product with the longer operand cut into
+// m = 1 .. 4 chunks: the engine asks for them (chunk_costs) and takes the
+// cheapest cut it allows, in place of its tuned per-length costs.
 // A single product goes to the path with the lower prediction.  A batch is
 // split so that its device share and its host share (run on `cpu_threads`
 // threads at the same time) finish together as early as the predictions allow.
@@ -49,6 +52,7 @@ public:
 
     Path choose(const ProductShape& s) const override;
     void choose_batch(const ProductShape* s, size_t n, Path* out) const override;
+    bool chunk_costs(size_t an, size_t bn, double* cost) const override;
     const char* name() const override { return "mlp"; }
 
     const MlpModel& model() const { return model_; }

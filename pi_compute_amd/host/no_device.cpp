@@ -7,8 +7,9 @@
 // behind the ABI).  GMP-only runs never reach it; `pi_compute_hip --gpu`
 // fails at HipNttMultiplier's construction, like the reference's GPU
 // multipliers without a device (src/arithmetic/int_ntt_multiplier.cpp:26-38).
-// Two answers describe layouts rather than compute and stay exact: the
-// transform-length rule and the cost model's parameter count.
+// Three answers describe layouts rather than compute and stay exact: the
+// transform-length rule, the chunk-cut rule and the cost model's parameter
+// count.
 //
 // Reference counterpart: the PI_CUDA_ENABLED gate around every GPU source
 // (src/arithmetic/gpu_ntt_multiplier.h:14, src/gpu/int_ntt.cu:10).
@@ -88,6 +89,31 @@ int gpu_ntt_transform_log2(size_t an, size_t bn) {
 }
 int gpu_ntt_product_plan(size_t, size_t, int*, int*, size_t*) { return no_device(); }
 size_t gpu_ntt_max_limbs(void) { return (size_t)1 << 24; }
+void gpu_ntt_set_chunk_cost(pintt_chunk_cost_fn, void*) {}
+// The engine's cut rule (engine.hip chunk_shape): chunks of ceil(an / m)
+// limbs rounded up to 16, none shorter than b, the last not empty, and a
+// transform shorter than the uncut product's.
+int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n) {
+    if (an < bn) {
+        const size_t t = an;
+        an = bn;
+        bn = t;
+    }
+    const int full = gpu_ntt_transform_log2(an, bn);
+    if (full < 0 || chunks < 1 || chunks > 4) return 0;
+    if (chunks == 1) {
+        if (log2n) *log2n = full;
+        return 1;
+    }
+    const size_t m = (size_t)chunks;
+    size_t h = (an + m - 1) / m;
+    h = (h + 15) & ~(size_t)15;
+    if (h < bn || (m - 1) * h >= an) return 0;
+    const int k = gpu_ntt_transform_log2(h, bn);
+    if (k < 0 || k >= full) return 0;
+    if (log2n) *log2n = k;
+    return 1;
+}
 const char* gpu_ntt_last_error(void) { return kNoDevice; }
 int gpu_ntt_autotune(int, int) { return no_device(); }
 int gpu_ntt_get_plan(int, int*, int*, int*, float*) { return no_device(); }

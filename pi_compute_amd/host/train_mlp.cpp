@@ -3,14 +3,20 @@
 //
 //   pi_train_mlp --out MODEL --log FILE [--log FILE ...] [--steps 4000]
 //                [--lr 0.003] [--seed 1] [--holdout 0.2] [--threads T]
+//                [--holdout-out FILE]
 //
-// Lines: {"an":..,"bn":..,"batch":..,"square":0|1,"path":"cpu"|"gpu"|"gpu_dev","ms":..}
+// Lines: {"an":..,"bn":..,"batch":..,"square":0|1,"path":P,"ms":..}, P one of
+// "cpu", "gpu", "gpu_dev", "gpu_dev_m1" .. "gpu_dev_m4".
 // Target: ln(ms) of the path the line was measured on (output 0 cpu, 1 gpu,
-// 2 gpu with device-resident operands; the model has 3 outputs when the log
-// holds gpu_dev lines, else 2).
-// Prints one JSON summary: sample counts, train/holdout loss, and on holdout
+// 2 gpu with device-resident operands, 3 .. 6 one device-resident product
+// cut into 1 .. 4 chunks; the model has as many outputs as the highest path
+// in the logs needs).
+// Prints one JSON summary: sample counts, train/holdout loss; on holdout
 // shapes measured on both paths the summed time of the model's choices, of the
-// threshold rule's choices and of the best choice.
+// threshold rule's choices and of the best choice; on holdout shapes measured
+// with every allowed cut, the summed time of the model's cut, the engine's own
+// cut, the uncut product and the best cut.  --holdout-out writes the holdout
+// lines (the test slice).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -25,6 +31,7 @@
 #include "dispatch.h"
 #include "mlp_dispatch.h"
 #include "operands.h"
+#include "pintt_gpu.h"
 #include "pintt_mlp.h"
 
 using namespace pi;
@@ -35,6 +42,7 @@ struct Sample {
     ProductShape shape;
     int path;
     double ms;
+    std::string line;
 };
 
 bool parse(const std::string& line, Sample& s) {
@@ -42,7 +50,7 @@ bool parse(const std::string& line, Sample& s) {
     int sq;
     char path[16];
     double ms;
-    if (std::sscanf(line.c_str(), "{\"an\":%llu,\"bn\":%llu,\"batch\":%llu,\"square\":%d,\"path\":\"%15[a-z_]\",\"ms\":%lf}",
+    if (std::sscanf(line.c_str(), "{\"an\":%llu,\"bn\":%llu,\"batch\":%llu,\"square\":%d,\"path\":\"%15[a-z_0-9]\",\"ms\":%lf}",
                     &an, &bn, &batch, &sq, path, &ms) != 6)
         return false;
     if (an == 0 || bn == 0 || !(ms > 0)) return false;
@@ -50,15 +58,18 @@ bool parse(const std::string& line, Sample& s) {
     if (std::strcmp(path, "cpu") == 0) s.path = 0;
     else if (std::strcmp(path, "gpu") == 0) s.path = 1;
     else if (std::strcmp(path, "gpu_dev") == 0) s.path = 2;
+    else if (std::strncmp(path, "gpu_dev_m", 9) == 0 && path[9] >= '1' && path[9] <= '4' && !path[10])
+        s.path = 2 + (path[9] - '0');
     else return false;
     s.ms = ms;
+    s.line = line;
     return true;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string out;
+    std::string out, holdout_out;
     std::vector<std::string> logs;
     int steps = 4000;
     float lr = 3e-3f;
@@ -75,6 +86,7 @@ int main(int argc, char** argv) {
         else if (a == "--seed") seed = std::stoull(val());
         else if (a == "--holdout") holdout = std::stod(val());
         else if (a == "--threads") threads = std::stoul(val());
+        else if (a == "--holdout-out") holdout_out = val();
         else {
             std::cerr << "unknown option " << a << std::endl;
             return 2;
@@ -135,8 +147,8 @@ int main(int argc, char** argv) {
     }
     std::vector<float> x(n * 4), t(n);
     std::vector<int> p(n);
-    double tmean[3] = {0, 0, 0};
-    size_t tcount[3] = {0, 0, 0};
+    double tmean[PINTT_MLP_MAX_OUTPUTS] = {};
+    size_t tcount[PINTT_MLP_MAX_OUTPUTS] = {};
     for (size_t i = 0; i < n; ++i) {
         auto z = model.standardise(feats[i]);
         std::copy(z.begin(), z.end(), &x[i * 4]);
@@ -173,8 +185,8 @@ int main(int argc, char** argv) {
     double test_loss = 0;
     ThresholdDispatcher heur(10000);
     MlpDispatcher mlp(model, &heur, threads);
-    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<double, 3>> pairs;
-    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<int, 3>> pcount;
+    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<double, PINTT_MLP_MAX_OUTPUTS>> pairs;
+    std::map<std::tuple<size_t, size_t, size_t, bool>, std::array<int, PINTT_MLP_MAX_OUTPUTS>> pcount;
     for (const auto& s : test) {
         auto z = model.standardise(MlpModel::features(s.shape));
         float y[PINTT_MLP_MAX_OUTPUTS];
@@ -208,6 +220,43 @@ int main(int argc, char** argv) {
             ++npairs[gp];
         }
     }
+    // Cuts: single device-resident products measured with every allowed cut.
+    double c_mlp = 0, c_engine = 0, c_uncut = 0, c_best = 0;
+    size_t c_shapes = 0, c_split_chosen = 0;
+    if (K >= 7)
+        for (auto& kv : pairs) {
+            auto& pc = pcount[kv.first];
+            const size_t an = std::get<0>(kv.first), bn = std::get<1>(kv.first);
+            if (std::get<2>(kv.first) != 1 || std::get<3>(kv.first) || !pc[2] || !pc[3]) continue;
+            bool complete = true;
+            for (int c = 1; c <= 4; ++c)
+                if (gpu_ntt_chunk_plan(an, bn, c, nullptr) && !pc[2 + c]) complete = false;
+            if (!complete) continue;
+            double cost[4];
+            if (!mlp.chunk_costs(std::max(an, bn), std::min(an, bn), cost)) continue;
+            int pick = 1;
+            double best = 1e300;
+            for (int c = 1; c <= 4; ++c) {
+                if (!pc[2 + c]) continue;
+                const double t = kv.second[2 + c] / pc[2 + c];
+                best = std::min(best, t);
+                if (cost[c - 1] < cost[pick - 1] || !pc[2 + pick]) pick = c;
+            }
+            c_mlp += kv.second[2 + pick] / pc[2 + pick];
+            c_engine += kv.second[2] / pc[2];
+            c_uncut += kv.second[3] / pc[3];
+            c_best += best;
+            c_split_chosen += pick > 1;
+            ++c_shapes;
+        }
+    if (!holdout_out.empty()) {
+        std::ofstream h(holdout_out);
+        for (const auto& smp : test) h << smp.line << "\n";
+        if (!h) {
+            std::cerr << "cannot write " << holdout_out << std::endl;
+            return 1;
+        }
+    }
     if (!model.save(out)) {
         std::cerr << "cannot write " << out << std::endl;
         return 1;
@@ -215,8 +264,11 @@ int main(int argc, char** argv) {
     std::printf("{\"samples\":%zu,\"train\":%zu,\"holdout\":%zu,\"outputs\":%d,\"steps\":%d,\"loss_first\":%.5f,"
                 "\"loss_last\":%.5f,\"holdout_loss\":%.5f,"
                 "\"staged\":{\"pairs\":%zu,\"agree_with_threshold\":%zu,\"ms_mlp\":%.3f,\"ms_threshold\":%.3f,\"ms_best\":%.3f},"
-                "\"resident\":{\"pairs\":%zu,\"agree_with_threshold\":%zu,\"ms_mlp\":%.3f,\"ms_threshold\":%.3f,\"ms_best\":%.3f}}\n",
+                "\"resident\":{\"pairs\":%zu,\"agree_with_threshold\":%zu,\"ms_mlp\":%.3f,\"ms_threshold\":%.3f,\"ms_best\":%.3f},"
+                "\"cuts\":{\"shapes\":%zu,\"split_chosen\":%zu,\"ms_mlp\":%.3f,\"ms_engine\":%.3f,\"ms_uncut\":%.3f,"
+                "\"ms_best\":%.3f}}\n",
                 all.size(), train.size(), test.size(), K, steps, steps ? hist[0] : 0.f, final_loss, test_loss,
-                npairs[1], agree[1], t_mlp[1], t_heur[1], t_best[1], npairs[2], agree[2], t_mlp[2], t_heur[2], t_best[2]);
+                npairs[1], agree[1], t_mlp[1], t_heur[1], t_best[1], npairs[2], agree[2], t_mlp[2], t_heur[2], t_best[2],
+                c_shapes, c_split_chosen, c_mlp, c_engine, c_uncut, c_best);
     return 0;
 }

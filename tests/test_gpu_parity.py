@@ -295,6 +295,41 @@ def test_carry_through_unresolved_windows(engine, oracle, chunks):
         engine.set_split(1)
 
 
+def test_chunk_cost_hook(engine, oracle):
+    """A caller's chunk costs (gpu_ntt_set_chunk_cost, the learned dispatcher's
+    hook) decide the cut: each allowed m priced cheapest is the one planned,
+    and the product stays exact; a callback that declines leaves the engine's
+    own choice."""
+    import ctypes
+
+    lib = pc.load_library()
+    proto = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
+                             ctypes.c_void_p)
+    a, b = pc.operands(26, 1 << 26, (1 << 26) // 3)
+    want = oracle.gmp_mul(a, b)
+    own = engine.product_plan(a.size, b.size)
+    try:
+        for m in (1, 2, 3, 4):
+            if not lib.gpu_ntt_chunk_plan(a.size, b.size, m, None):
+                continue
+
+            def price(an, bn, cost, ctx, m=m):
+                for i in range(4):
+                    cost[i] = 0.0 if i + 1 == m else 1.0
+                return 1
+
+            cb = proto(price)
+            lib.gpu_ntt_set_chunk_cost(ctypes.cast(cb, ctypes.c_void_p), None)
+            assert engine.product_plan(a.size, b.size)["chunks"] == m
+            _check(engine, oracle, a, b, f"cut into {m} by the caller's costs")
+            assert np.array_equal(engine.mul(b, a), want)
+        cb = proto(lambda an, bn, cost, ctx: 0)
+        lib.gpu_ntt_set_chunk_cost(ctypes.cast(cb, ctypes.c_void_p), None)
+        assert engine.product_plan(a.size, b.size) == own
+    finally:
+        lib.gpu_ntt_set_chunk_cost(None, None)
+
+
 def test_split_all_ones_carry_chains(engine, oracle):
     # Overlapping chunk products whose sums ripple carries across chunk borders.
     a = np.full(50000, np.uint64(0xFFFFFFFFFFFFFFFF))
