@@ -209,9 +209,11 @@ typedef int (*pintt_chunk_cost_fn)(size_t an, size_t bn, double* cost, void* ctx
 void gpu_ntt_set_chunk_cost(pintt_chunk_cost_fn fn, void* ctx);
 
 /* Whether an an x bn product (an >= bn) can be cut into `chunks` pieces
- * (1 = not cut): returns 1 and the transform length 2^*log2n it then uses,
- * else 0.  Pure arithmetic: needs no device. */
-int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n);
+ * (1 = not cut): returns 1, the limbs of each chunk of the longer operand
+ * (*chunk_limbs, the last chunk may be shorter) and the transform length
+ * 2^*log2n it then uses, else 0.  Either pointer may be null.  Pure
+ * arithmetic: needs no device. */
+int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, size_t* chunk_limbs, int* log2n);
 
 /* Largest an + bn accepted. */
 size_t gpu_ntt_max_limbs(void);

@@ -4,12 +4,12 @@
  * (src/arithmetic/int_ntt_multiplier.cpp:56-59, src/arithmetic/gpu_ntt_multiplier.cpp:67-72,
  * default 10000 from src/main.cpp:63).  These entry points have no counterpart
  * there: they train and evaluate the small FP32 network that replaces that
- * threshold (4 inputs -> 32 ReLU -> 32 ReLU -> K outputs, one predicted
+ * threshold (8 inputs -> 32 ReLU -> 32 ReLU -> K outputs, one predicted
  * log-time per path).  The threshold rule stays the fallback.
  *
  * Parameter vector (float, this order, row-major, row = output unit):
- *   W1[32][4], b1[32], W2[32][32], b2[32], W3[K][32], b3[K]
- * Samples are sample-major: x[n][4], y[n][K].
+ *   W1[32][8], b1[32], W2[32][32], b2[32], W3[K][32], b3[K]
+ * Samples are sample-major: x[n][8], y[n][K].
  * Loss (training): L = 1/(2n) * sum_s (y[s][path[s]] - target[s])^2, i.e. each
  * sample supervises only the output of the path it was measured on.
  * All functions use the current HIP device, block until done, and return a
@@ -26,11 +26,11 @@
 extern "C" {
 #endif
 
-#define PINTT_MLP_INPUTS 4
+#define PINTT_MLP_INPUTS 8
 #define PINTT_MLP_HIDDEN 32
 #define PINTT_MLP_MAX_OUTPUTS 8
 
-/* 1216 + 33 * outputs, or 0 when outputs is outside [1, 8]. */
+/* 1344 + 33 * outputs, or 0 when outputs is outside [1, 8]. */
 size_t gpu_mlp_param_count(int outputs);
 
 /* Inference: y = f(x) for n samples. */

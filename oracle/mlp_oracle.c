@@ -13,7 +13,7 @@
 #include <math.h>
 #include <stddef.h>
 
-#define IN 4
+#define IN 8
 #define H 32
 
 static int oW1(void) { return 0; }
@@ -46,7 +46,7 @@ static double out(const double* w, int k, int K, const double* h2) {
     return s;
 }
 
-/* x[n][4] -> y[n][K] */
+/* x[n][8] -> y[n][K] */
 void orc_mlp_forward(const double* w, int K, const double* x, size_t n, double* y) {
     double z1[H], h1[H], z2[H], h2[H];
     for (size_t s = 0; s < n; ++s) {

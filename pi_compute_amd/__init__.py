@@ -56,7 +56,8 @@ ABI = {
     ),
     "gpu_ntt_last_error": (ctypes.c_char_p, []),
     "gpu_ntt_set_chunk_cost": (None, [ctypes.c_void_p, ctypes.c_void_p]),
-    "gpu_ntt_chunk_plan": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gpu_ntt_chunk_plan": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]),
     "gpu_ntt_set_timing": (None, [ctypes.c_int]),
     "gpu_ntt_pass_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "gpu_ntt_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),

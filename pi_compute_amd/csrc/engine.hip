@@ -479,7 +479,18 @@ int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{})
     gr->h = r.an;
     gr->log2n = log2_for(r.an, r.bn);
     if (gr->log2n < 0) return fail(PINTT_ERR_TOO_LARGE, "product exceeds the 2^25 transform limit");
-    if (!r.square && g.split_mode == 1 && g.chunk_cost) {
+    // Only products that some cut applies to are priced: most products of a
+    // tree level are near-balanced or too short for a shorter transform, and
+    // the caller's pricing runs on the host once per product.
+    auto cuttable = [&]() {
+        for (uint32_t m = 2; m <= mode.max_chunks && m <= (uint32_t)kMaxChunks; ++m) {
+            size_t h;
+            int k;
+            if (chunk_shape(r.an, r.bn, m, &h, &k)) return true;
+        }
+        return false;
+    };
+    if (!r.square && g.split_mode == 1 && g.chunk_cost && cuttable()) {
         // The caller's prices, cheapest allowed cut first.
         double cost[kMaxChunks];
         if (g.chunk_cost(r.an, r.bn, cost, g.chunk_ctx)) {
@@ -1704,11 +1715,12 @@ void gpu_ntt_set_chunk_cost(pintt_chunk_cost_fn fn, void* ctx) {
     g.chunk_ctx = ctx;
 }
 
-int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n) {
+int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, size_t* chunk_limbs, int* log2n) {
     if (an < bn) std::swap(an, bn);
     size_t h;
     int k;
     if (an == 0 || bn == 0 || chunks < 1 || !chunk_shape(an, bn, (uint32_t)chunks, &h, &k)) return 0;
+    if (chunk_limbs) *chunk_limbs = h;
     if (log2n) *log2n = k;
     return 1;
 }

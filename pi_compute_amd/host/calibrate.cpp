@@ -212,7 +212,7 @@ int main(int argc, char** argv) {
                 m.multiply_batch_resident(jobs.data(), 1);  // warm-up: plans and buffers for this shape
                 for (int c = 1; c <= 4; ++c) {
                     int log2n = 0;
-                    if (!gpu_ntt_chunk_plan(an, bn, c, &log2n)) continue;
+                    if (!gpu_ntt_chunk_plan(an, bn, c, nullptr, &log2n)) continue;
                     gpu_ntt_set_split(c == 1 ? 0 : c);
                     m.set_log_tag("_m" + std::to_string(c));
                     for (int rep = 0; rep < (quick ? 1 : cut_reps); ++rep) m.multiply_batch_resident(jobs.data(), 1);

@@ -30,12 +30,17 @@ HipNttMultiplier::HipNttMultiplier(size_t threshold, int device)
         throw std::runtime_error(std::string("HipNttMultiplier: ") + gpu_ntt_last_error());
     // The engine asks the dispatcher for the cut of each product (a learned
     // model prices it; the threshold rule leaves it to the engine).
-    gpu_ntt_set_chunk_cost(&HipNttMultiplier::chunk_cost, this);
+    set_dispatcher_cuts(true);
 }
 
 HipNttMultiplier::~HipNttMultiplier() {
     gpu_ntt_set_chunk_cost(nullptr, nullptr);
     close_log();
+}
+
+void HipNttMultiplier::set_dispatcher_cuts(bool on) {
+    if (on) gpu_ntt_set_chunk_cost(&HipNttMultiplier::chunk_cost, this);
+    else gpu_ntt_set_chunk_cost(nullptr, nullptr);
 }
 
 int HipNttMultiplier::chunk_cost(size_t an, size_t bn, double* cost, void* self) {

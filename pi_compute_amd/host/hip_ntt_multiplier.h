@@ -43,6 +43,10 @@ public:
     // Replace the path choice (not owned; nullptr restores the threshold rule).
     void set_dispatcher(const Dispatcher* d) { dispatcher_ = d ? d : &heuristic_; }
     const Dispatcher& dispatcher() const { return *dispatcher_; }
+    // When true (the default) the engine asks the dispatcher for the cost of
+    // each chunk count of a product (a learned model prices the cuts); when
+    // false the engine cuts by its own tuned per-length costs.
+    void set_dispatcher_cuts(bool on);
     // Every product on the GPU with operands uploaded first and results
     // downloaded after; only the device batch itself is timed and logged (path
     // "gpu_dev"): the cost of products whose operands already live on the

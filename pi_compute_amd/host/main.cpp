@@ -3,7 +3,7 @@
 //
 //   pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]
 //                  [--threads T] [--tree levels|recursive] [--per-call]
-//                  [--dispatch threshold|mlp] [--model FILE] [--log FILE]
+//                  [--dispatch threshold|mlp] [--model FILE] [--cuts dispatcher|engine] [--log FILE]
 //                  [--verbose]
 //
 //   --gpu            HipNttMultiplier (default: GmpMultiplier, the GMP-only run)
@@ -11,6 +11,8 @@
 //   --tree recursive depth-first tree of the reference, one multiply per product
 //   --per-call       with --tree levels: device products one engine call each
 //   --dispatch mlp   learned cost model from --model (threshold rule otherwise)
+//   --cuts engine    chunk counts of unbalanced products from the engine's tuned
+//                    costs (default "dispatcher": the learned model prices them)
 //   --log FILE       per-product timing lines (training data for the cost model)
 //   --final mpf|newton        final quotient and square root: GMP mpf (the
 //                             reference's sequence) or Newton through the
@@ -66,7 +68,8 @@ std::string data_dir() {
 void usage() {
     std::cout << "usage: pi_compute_hip --digits N [--output FILE] [--gpu] [--gpu-threshold L]\n"
                  "                      [--threads T] [--tree levels|recursive] [--per-call]\n"
-                 "                      [--dispatch threshold|mlp] [--model FILE] [--log FILE]\n"
+                 "                      [--dispatch threshold|mlp] [--model FILE] [--cuts dispatcher|engine]\n"
+                 "                      [--log FILE]\n"
                  "                      [--final mpf|newton] [--convert host|multiplier] [--convert-leaf D]\n"
                  "                      [--device-levels [LIMBS|auto]] [--bs-segments S] [--bs-leaf T] [--no-overlap]\n"
                  "                      [--host-subtrees] [--host-final-step] [--host-conversion-levels]\n"
@@ -81,7 +84,7 @@ int main(int argc, char** argv) {
     cfg.output_file = "pi_digits.txt";
     bool gpu = false, per_call = false, device_auto = false;
     size_t threshold = 10000;
-    std::string dispatch = "threshold", model, log, final_step, convert, plans;
+    std::string dispatch = "threshold", model, log, final_step, convert, plans, cuts = "dispatcher";
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -104,6 +107,7 @@ int main(int argc, char** argv) {
         } else if (a == "--per-call") per_call = true;
         else if (a == "--dispatch") dispatch = next();
         else if (a == "--model") model = next();
+        else if (a == "--cuts") cuts = next();
         else if (a == "--log") log = next();
         else if (a == "--final") final_step = next();
         else if (a == "--convert") convert = next();
@@ -175,6 +179,8 @@ int main(int argc, char** argv) {
             } else if (dispatch != "threshold") {
                 throw std::runtime_error("unknown dispatcher " + dispatch);
             }
+            if (cuts != "dispatcher" && cuts != "engine") throw std::runtime_error("unknown --cuts " + cuts);
+            hip->set_dispatcher_cuts(cuts == "dispatcher");
             if (!log.empty() && !hip->open_log(log)) throw std::runtime_error("cannot open log " + log);
             if (device_auto) cfg.device_switch = &hip->dispatcher();
             mode = std::string("HIP NTT, ") + hip->device_name() + ", " + hip->dispatcher().name() +

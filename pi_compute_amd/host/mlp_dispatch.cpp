@@ -9,6 +9,7 @@
 #include <stdexcept>
 #include <thread>
 
+#include "pintt_gpu.h"
 #include "pintt_mlp.h"
 
 namespace pi {
@@ -17,24 +18,34 @@ constexpr int IN = PINTT_MLP_INPUTS, H = PINTT_MLP_HIDDEN;
 constexpr int oB1 = H * IN, oW2 = oB1 + H, oB2 = oW2 + H * H, oW3 = oB2 + H;
 }  // namespace
 
-std::array<float, 4> MlpModel::features(const ProductShape& s) {
-    const double big = (double)(s.an > s.bn ? s.an : s.bn), small = (double)(s.an < s.bn ? s.an : s.bn);
-    return {(float)std::log2(big), (float)std::log2(small), (float)std::log2((double)(s.batch ? s.batch : 1)),
-            s.square ? 1.f : 0.f};
+MlpFeatures MlpModel::features(const ProductShape& s) {
+    const size_t big = std::max(s.an, s.bn), small = std::min(s.an, s.bn);
+    MlpFeatures f;
+    f[0] = (float)std::log2((double)big);
+    f[1] = (float)std::log2((double)small);
+    f[2] = (float)std::log2((double)(s.batch ? s.batch : 1));
+    f[3] = s.square ? 1.f : 0.f;
+    for (int m = 1; m <= 4; ++m) {
+        size_t h = 0;
+        int k = 0;
+        const bool ok = (m == 1 || !s.square) && gpu_ntt_chunk_plan(big, small, m, &h, &k);
+        f[3 + m] = ok ? (float)((double)k - std::log2((double)(2 * h + 2 * small - 1))) : -1.f;
+    }
+    return f;
 }
 
 // Operand sizes must lie in the trained range; a batch larger than any
 // trained one is priced as the largest (a product's share of a batch stops
 // shrinking once the batch fills the device).
-bool MlpModel::in_range(const std::array<float, 4>& f) const {
+bool MlpModel::in_range(const MlpFeatures& f) const {
     for (int j = 0; j < 2; ++j)
         if (f[j] < lo[j] || f[j] > hi[j]) return false;
     return f[2] >= lo[2];
 }
 
-std::array<float, 4> MlpModel::standardise(const std::array<float, 4>& f) const {
-    std::array<float, 4> x;
-    for (int j = 0; j < 4; ++j) x[j] = ((j == 2 ? std::min(f[j], hi[j]) : f[j]) - mean[j]) / scale[j];
+MlpFeatures MlpModel::standardise(const MlpFeatures& f) const {
+    MlpFeatures x;
+    for (int j = 0; j < IN; ++j) x[j] = ((j == 2 ? std::min(f[j], hi[j]) : f[j]) - mean[j]) / scale[j];
     return x;
 }
 
@@ -58,10 +69,11 @@ void MlpModel::forward(const float* x, float* y) const {
     }
 }
 
-// Text format:
-//   pintt-mlp 1
+// Text format (version 2: PINTT_MLP_INPUTS = 8 inputs; version 1 files had
+// four and are refused):
+//   pintt-mlp 2
 //   outputs K
-//   mean m0 m1 m2 m3 / scale ... / lo ... / hi ...
+//   mean m0 .. m7 / scale ... / lo ... / hi ...
 //   params N
 //   N floats
 bool MlpModel::load(const std::string& path, std::string* err) {
@@ -73,7 +85,8 @@ bool MlpModel::load(const std::string& path, std::string* err) {
     if (!f) return bad("cannot open");
     std::string tag, key;
     int version = 0;
-    if (!(f >> tag >> version) || tag != "pintt-mlp" || version != 1) return bad("not a pintt-mlp v1 file");
+    if (!(f >> tag >> version) || tag != "pintt-mlp") return bad("not a pintt-mlp file");
+    if (version != 2) return bad("pintt-mlp version 1 (four inputs): retrain with pi_train_mlp");
     if (!(f >> key >> outputs) || key != "outputs" || outputs < 2 || outputs > PINTT_MLP_MAX_OUTPUTS)
         return bad("bad outputs");
     for (auto* arr : {&mean, &scale, &lo, &hi}) {
@@ -94,9 +107,9 @@ bool MlpModel::load(const std::string& path, std::string* err) {
 bool MlpModel::save(const std::string& path) const {
     std::FILE* f = std::fopen(path.c_str(), "w");
     if (!f) return false;
-    std::fprintf(f, "pintt-mlp 1\noutputs %d\n", outputs);
+    std::fprintf(f, "pintt-mlp 2\noutputs %d\n", outputs);
     const char* names[4] = {"mean", "scale", "lo", "hi"};
-    const std::array<float, 4>* arrs[4] = {&mean, &scale, &lo, &hi};
+    const MlpFeatures* arrs[4] = {&mean, &scale, &lo, &hi};
     for (int a = 0; a < 4; ++a) {
         std::fprintf(f, "%s", names[a]);
         for (float v : *arrs[a]) std::fprintf(f, " %.9g", v);

@@ -7,7 +7,13 @@
 // Either path returns the exact product; only wall time depends on the choice.
 //
 // Inputs (standardised with the model's mean/scale):
-//   log2(larger operand limbs), log2(smaller operand limbs), log2(batch), square
+//   log2(larger operand limbs), log2(smaller operand limbs), log2(batch), square,
+//   and for m = 1 .. 4 the padding of the transform the product uses when its
+//   longer operand is cut into m chunks: log2(2^k / (2 h + 2 bn - 1)) for
+//   chunks of h limbs and transform length 2^k (gpu_ntt_chunk_plan), -1 where
+//   that cut is not allowed (squares are never cut).  Run time steps up each
+//   time a product crosses a power-of-two transform length; the padding
+//   inputs let a smooth network place those steps exactly.
 // Outputs: predicted ln(milliseconds) per product, output 0 = host GMP,
 // output 1 = GPU engine with host operands (a device batch's time, staging
 // included, shared out by transform work), output 2 (models with three) =
@@ -26,17 +32,20 @@
 #include <vector>
 
 #include "dispatch.h"
+#include "pintt_mlp.h"
 
 namespace pi {
 
+using MlpFeatures = std::array<float, PINTT_MLP_INPUTS>;
+
 struct MlpModel {
     int outputs = 2;
-    std::array<float, 4> mean{}, scale{}, lo{}, hi{};  // feature standardisation and trained range
-    std::vector<float> params;                         // layout of include/pintt_mlp.h
+    MlpFeatures mean{}, scale{}, lo{}, hi{};  // feature standardisation and trained range
+    std::vector<float> params;                // layout of include/pintt_mlp.h
 
-    static std::array<float, 4> features(const ProductShape& s);
-    bool in_range(const std::array<float, 4>& f) const;
-    std::array<float, 4> standardise(const std::array<float, 4>& f) const;
+    static MlpFeatures features(const ProductShape& s);
+    bool in_range(const MlpFeatures& f) const;
+    MlpFeatures standardise(const MlpFeatures& f) const;
     // Host forward pass, same arithmetic order as the device kernel.
     void forward(const float* x, float* y) const;
 

@@ -93,7 +93,7 @@ void gpu_ntt_set_chunk_cost(pintt_chunk_cost_fn, void*) {}
 // The engine's cut rule (engine.hip chunk_shape): chunks of ceil(an / m)
 // limbs rounded up to 16, none shorter than b, the last not empty, and a
 // transform shorter than the uncut product's.
-int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n) {
+int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, size_t* chunk_limbs, int* log2n) {
     if (an < bn) {
         const size_t t = an;
         an = bn;
@@ -102,6 +102,7 @@ int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n) {
     const int full = gpu_ntt_transform_log2(an, bn);
     if (full < 0 || chunks < 1 || chunks > 4) return 0;
     if (chunks == 1) {
+        if (chunk_limbs) *chunk_limbs = an;
         if (log2n) *log2n = full;
         return 1;
     }
@@ -111,6 +112,7 @@ int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, int* log2n) {
     if (h < bn || (m - 1) * h >= an) return 0;
     const int k = gpu_ntt_transform_log2(h, bn);
     if (k < 0 || k >= full) return 0;
+    if (chunk_limbs) *chunk_limbs = h;
     if (log2n) *log2n = k;
     return 1;
 }

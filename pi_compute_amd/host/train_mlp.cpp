@@ -128,9 +128,10 @@ int main(int argc, char** argv) {
     for (const auto& smp : all) K = std::max(K, smp.path + 1);
     model.outputs = K;
     const size_t n = train.size();
-    std::vector<std::array<float, 4>> feats(n);
+    constexpr int IN = PINTT_MLP_INPUTS;
+    std::vector<MlpFeatures> feats(n);
     for (size_t i = 0; i < n; ++i) feats[i] = MlpModel::features(train[i].shape);
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < IN; ++j) {
         double m = 0, v = 0, lo = 1e30, hi = -1e30;
         for (auto& f : feats) {
             m += f[j];
@@ -145,13 +146,13 @@ int main(int argc, char** argv) {
         model.lo[j] = (float)lo - 0.25f;
         model.hi[j] = (float)hi + 0.25f;
     }
-    std::vector<float> x(n * 4), t(n);
+    std::vector<float> x(n * IN), t(n);
     std::vector<int> p(n);
     double tmean[PINTT_MLP_MAX_OUTPUTS] = {};
     size_t tcount[PINTT_MLP_MAX_OUTPUTS] = {};
     for (size_t i = 0; i < n; ++i) {
         auto z = model.standardise(feats[i]);
-        std::copy(z.begin(), z.end(), &x[i * 4]);
+        std::copy(z.begin(), z.end(), &x[i * IN]);
         t[i] = (float)std::log(train[i].ms);
         p[i] = train[i].path;
         tmean[p[i]] += t[i];
@@ -165,8 +166,8 @@ int main(int argc, char** argv) {
         float lim = std::sqrt(6.f / fan_in);
         for (size_t i = 0; i < count; ++i) model.params[off + i] = (float)((2 * g.uniform() - 1) * lim);
     };
-    const size_t oW1 = 0, oW2 = H * 4 + H, oW3 = oW2 + H * H + H, oB3 = oW3 + (size_t)K * H;
-    fill(oW1, H * 4, 4);
+    const size_t oW1 = 0, oW2 = H * IN + H, oW3 = oW2 + H * H + H, oB3 = oW3 + (size_t)K * H;
+    fill(oW1, H * IN, IN);
     fill(oW2, H * H, H);
     fill(oW3, (size_t)K * H, H);
     for (int k = 0; k < K; ++k) model.params[oB3 + k] = tcount[k] ? (float)(tmean[k] / tcount[k]) : 0.f;
@@ -230,7 +231,7 @@ int main(int argc, char** argv) {
             if (std::get<2>(kv.first) != 1 || std::get<3>(kv.first) || !pc[2] || !pc[3]) continue;
             bool complete = true;
             for (int c = 1; c <= 4; ++c)
-                if (gpu_ntt_chunk_plan(an, bn, c, nullptr) && !pc[2 + c]) complete = false;
+                if (gpu_ntt_chunk_plan(an, bn, c, nullptr, nullptr) && !pc[2 + c]) complete = false;
             if (!complete) continue;
             double cost[4];
             if (!mlp.chunk_costs(std::max(an, bn), std::min(an, bn), cost)) continue;

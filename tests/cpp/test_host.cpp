@@ -430,10 +430,10 @@ int main(int argc, char** argv) {
               "ThresholdRule");
         pi::MlpModel m;
         m.outputs = 2;
-        m.mean = {10, 9, 3, 0.3f};
-        m.scale = {4, 4, 2, 0.5f};
-        m.lo = {3, 3, 0, 0};
-        m.hi = {25, 25, 12, 1};
+        m.mean = {10, 9, 3, 0.3f, 0.5f, 0.4f, 0.3f, 0.2f};
+        m.scale = {4, 4, 2, 0.5f, 0.3f, 0.6f, 0.7f, 0.8f};
+        m.lo = {3, 3, 0, 0, -1, -1, -1, -1};
+        m.hi = {25, 25, 12, 1, 11, 11, 11, 11};
         m.params.resize(gpu_mlp_param_count(2));
         pi::SplitMix64 g(5);
         for (auto& v : m.params) v = (float)(g.uniform() - 0.5);
@@ -446,7 +446,7 @@ int main(int argc, char** argv) {
         // Out-of-range shapes go to the fallback.
         pi::FixedDispatcher cpu(pi::Path::Cpu);
         pi::MlpModel big = m;
-        const int oB3 = 1216 + 32 * 2;
+        const int H = PINTT_MLP_HIDDEN, oB3 = H * PINTT_MLP_INPUTS + H + H * H + H + H * 2;
         for (auto& v : big.params) v = 0;
         big.params[oB3 + 0] = 5.f;  // host predicted slow, GPU fast
         big.params[oB3 + 1] = 0.f;
@@ -463,10 +463,10 @@ int main(int argc, char** argv) {
     {
         pi::MlpModel m;
         m.outputs = 2;
-        m.mean = {12, 10, 4, 0.3f};
-        m.scale = {4, 4, 3, 0.5f};
-        m.lo = {0, 0, 0, 0};
-        m.hi = {30, 30, 20, 1};
+        m.mean = {12, 10, 4, 0.3f, 0.5f, 0.4f, 0.3f, 0.2f};
+        m.scale = {4, 4, 3, 0.5f, 0.3f, 0.6f, 0.7f, 0.8f};
+        m.lo = {0, 0, 0, 0, -1, -1, -1, -1};
+        m.hi = {30, 30, 20, 1, 11, 11, 11, 11};
         m.params.resize(gpu_mlp_param_count(2));
         pi::SplitMix64 g(17);
         for (auto& v : m.params) v = (float)(0.6 * (g.uniform() - 0.5));

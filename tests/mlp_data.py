@@ -1,11 +1,12 @@
 """Seeded parameters and samples for the cost-model tests."""
 import numpy as np
 
-IN, H = 4, 32
+IN, H = 8, 32
+HEAD = H * IN + H + H * H + H  # parameters before the output layer (W3, b3)
 
 
 def param_count(k):
-    return H * IN + H + H * H + H + (H + 1) * k
+    return HEAD + (H + 1) * k
 
 
 def params(k, seed):

@@ -80,10 +80,10 @@ def test_missing_library_fails_loudly(tmp_path):
 
 def test_cost_model_abi_without_device():
     lib = pc.load_library()
-    assert lib.gpu_mlp_param_count(2) == 1216 + 66
+    assert lib.gpu_mlp_param_count(2) == 1344 + 66  # 8 -> 32 -> 32 -> 2
     assert lib.gpu_mlp_param_count(0) == 0 and lib.gpu_mlp_param_count(9) == 0
-    w = np.zeros(1282, np.float32)
-    x = np.zeros((3, 4), np.float32)
+    w = np.zeros(1410, np.float32)
+    x = np.zeros((3, 8), np.float32)
     y = np.zeros((3, 2), np.float32)
     f = pc._f32p
     assert lib.gpu_mlp_forward(w.ctypes.data_as(f), 9, x.ctypes.data_as(f), 3, y.ctypes.data_as(f)) == 5

@@ -29,7 +29,7 @@ from pathlib import Path
 import pytest
 
 import pi_compute_amd as pc
-from cost_model import features, forward, in_range, load_model
+from cost_model import chunk_plan, features, forward, in_range, load_model
 
 ROOT = Path(__file__).resolve().parent.parent
 MODEL = ROOT / "pi_compute_amd" / "data" / "cost_model.txt"
@@ -99,7 +99,7 @@ def test_cut_choice_beats_the_uncut_and_tuned_rules(model, shapes):
     for (an, bn, batch, square), d in shapes.items():
         if batch != 1 or square or "gpu_dev_m1" not in d or "gpu_dev" not in d:
             continue
-        allowed = [c for c in range(1, 5) if lib.gpu_ntt_chunk_plan(an, bn, c, None)]
+        allowed = [c for c in range(1, 5) if lib.gpu_ntt_chunk_plan(an, bn, c, None, None)]
         if any(f"gpu_dev_m{c}" not in d for c in allowed):
             continue
         y = forward(model, features(max(an, bn), min(an, bn), 1, 0))
@@ -117,40 +117,39 @@ def test_cut_choice_beats_the_uncut_and_tuned_rules(model, shapes):
     assert t_mlp < t_engine
 
 
-def _chunk_rule(an, bn, m):
-    """The cut rule restated: chunks of ceil(an / m) limbs rounded up to 16,
-    none shorter than b, no empty last chunk, a shorter transform than uncut."""
-    if an < bn:
-        an, bn = bn, an
-
-    def log2n(x, y):
-        need, k = 2 * x + 2 * y - 1, 10
-        while (1 << k) < need:
-            k += 1
-        return k if k <= 25 else -1
-
-    full = log2n(an, bn)
-    if full < 0 or not 1 <= m <= 4:
-        return None
-    if m == 1:
-        return full
-    h = -(-an // m)
-    h = (h + 15) & ~15
-    if h < bn or (m - 1) * h >= an:
-        return None
-    k = log2n(h, bn)
-    return k if 0 <= k < full else None
-
-
 def test_chunk_plan_rule():
+    """libpintt's cut rule (the one the engine plans with and the model's
+    padding features read) against the restatement in cost_model.chunk_plan."""
     lib = pc.load_library()
-    out = ctypes.c_int(0)
+    h, k = ctypes.c_size_t(0), ctypes.c_int(0)
     cases = [(an, bn) for an in (1, 15, 16, 1000, 4095, 65536, 349526, 1 << 20, 3 << 20, 1 << 23)
              for bn in (1, 7, 300, 5000, 65536, 349526, 1 << 20)]
     for an, bn in cases:
         for m in range(0, 6):
-            want = _chunk_rule(an, bn, m)
-            got = lib.gpu_ntt_chunk_plan(an, bn, m, ctypes.byref(out))
+            want = chunk_plan(an, bn, m)
+            got = lib.gpu_ntt_chunk_plan(an, bn, m, ctypes.byref(h), ctypes.byref(k))
             assert got == (want is not None), (an, bn, m)
             if want is not None:
-                assert out.value == want, (an, bn, m)
+                assert (h.value, k.value) == want, (an, bn, m)
+    # Known cuts of 2^20 x 2^18 limbs: uncut it needs 2^22 points (2.5 M
+    # coefficients); two chunks of 2^19 limbs need 2^21 (1.5 M); three of
+    # 349536 limbs (ceil(2^20 / 3) rounded up to 16) also 2^21; four of 2^18
+    # limbs, as long as b, need 2^20 (just under 1 M).
+    assert chunk_plan(1 << 20, 1 << 18, 1) == (1 << 20, 22)
+    assert chunk_plan(1 << 20, 1 << 18, 2) == (1 << 19, 21)
+    assert chunk_plan(1 << 20, 1 << 18, 3) == (349536, 21)
+    assert chunk_plan(1 << 20, 1 << 18, 4) == (1 << 18, 20)
+    assert chunk_plan(1 << 20, (1 << 18) + 1, 4) is None  # a chunk shorter than b
+
+
+def test_padding_features():
+    """The padding inputs of an exact power-of-two need are 0, of one word
+    above a power of two close to 1, and -1 for cuts that are not allowed."""
+    f = features(1 << 20, 1 << 18, 1, False)
+    assert f[4] == pytest.approx(22 - math.log2(2 * (1 << 20) + 2 * (1 << 18) - 1))
+    assert f[5] == pytest.approx(21 - math.log2(2 * (1 << 19) + 2 * (1 << 18) - 1))
+    sq = features(1 << 19, 1 << 19, 1, True)
+    assert sq[4] == pytest.approx(21 - math.log2((1 << 21) - 1), abs=1e-6) and abs(sq[4]) < 1e-6
+    assert list(sq[5:]) == [-1.0, -1.0, -1.0]
+    g = features(1 << 18, 1, 1, False)  # need 2^19 + 1: the transform of 2^20 is half empty
+    assert g[4] == pytest.approx(1.0, abs=1e-5)

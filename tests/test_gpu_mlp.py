@@ -70,7 +70,8 @@ def test_training_fits_a_cost_surface(engine):
     sq = rng.integers(0, 2, n).astype(np.float64)
     cpu = 1.45 * np.log(2.0) * (lb + ls) / 2 - 9 - 0.2 * sq
     gpu = np.log(0.05 + np.exp(0.8 * np.log(2.0) * (lb + ls) / 2 - 8) * 2.0 ** -lbat) - 0.1 * sq
-    x = np.stack([lb, ls, lbat, sq], 1)
+    noise = rng.uniform(0, 1, (n, 4))  # four inputs the surface does not depend on
+    x = np.concatenate([np.stack([lb, ls, lbat, sq], 1), noise], 1)
     mean, sd = x.mean(0), x.std(0)
     xs = ((x - mean) / sd).astype(np.float32)
     path = rng.integers(0, 2, n).astype(np.int32)

@@ -310,7 +310,7 @@ def test_chunk_cost_hook(engine, oracle):
     own = engine.product_plan(a.size, b.size)
     try:
         for m in (1, 2, 3, 4):
-            if not lib.gpu_ntt_chunk_plan(a.size, b.size, m, None):
+            if not lib.gpu_ntt_chunk_plan(a.size, b.size, m, None, None):
                 continue
 
             def price(an, bn, cost, ctx, m=m):

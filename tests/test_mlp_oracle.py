@@ -6,7 +6,7 @@ and a planted mistake."""
 import numpy as np
 import pytest
 
-from mlp_data import H, IN, param_count, params, samples
+from mlp_data import HEAD, H, IN, param_count, params, samples
 
 
 def _np_forward(w, k, x):
@@ -24,7 +24,7 @@ def _np_forward(w, k, x):
 
 @pytest.mark.parametrize("k", [1, 2, 5, 8])
 def test_param_count(oracle, k):
-    assert oracle.mlp_param_count(k) == param_count(k) == 1216 + 33 * k
+    assert oracle.mlp_param_count(k) == param_count(k) == HEAD + 33 * k
 
 
 @pytest.mark.parametrize("k", [2, 3])
@@ -43,7 +43,7 @@ def test_gradient_matches_finite_differences(oracle, k, seed):
     eps = 1e-6
     rng = np.random.default_rng(seed)
     # every bias and output weight, and a sample of the hidden weights
-    idx = list(range(H * IN, H * IN + H)) + list(range(1216, w.size)) + list(rng.choice(1216, 200, replace=False))
+    idx = list(range(H * IN, H * IN + H)) + list(range(HEAD, w.size)) + list(rng.choice(HEAD, 200, replace=False))
     for i in idx:
         wp, wm = w.copy(), w.copy()
         wp[i] += eps
@@ -55,15 +55,15 @@ def test_gradient_matches_finite_differences(oracle, k, seed):
 def test_loss_closed_form_with_zero_weights(oracle):
     k = 2
     w = np.zeros(param_count(k))
-    w[1216 + 32 * k:] = [0.5, -1.0]
+    w[HEAD + 32 * k:] = [0.5, -1.0]
     x, path, t = samples(30, k, 9)
     pred = np.where(path == 0, 0.5, -1.0)
     g, L = oracle.mlp_gradient(w, k, x, path, t)
     assert L == pytest.approx(0.5 * np.mean((pred - t) ** 2))
     # only output biases move: dL/db3[j] = mean over samples of path j of the error, scaled by 1/n
     for j in range(k):
-        assert g[1216 + 32 * k + j] == pytest.approx(np.sum((pred - t)[path == j]) / len(t))
-    assert np.all(g[:1216 + 32 * k] == 0)
+        assert g[HEAD + 32 * k + j] == pytest.approx(np.sum((pred - t)[path == j]) / len(t))
+    assert np.all(g[:HEAD + 32 * k] == 0)
 
 
 def test_adam_first_step_closed_form(oracle):
@@ -82,7 +82,7 @@ def test_planted_mistake_is_caught(oracle):
     w = params(k, 11)
     x, path, t = samples(20, k, 11)
     g, _ = oracle.mlp_gradient(w, k, x, path, t)
-    i = 1216 + 5
+    i = HEAD + 5
     g[i] *= 1.01
     wp, wm = w.copy(), w.copy()
     wp[i] += 1e-6
