@@ -2,7 +2,9 @@
 //   k_col_fwd  limbs -> residues (unpack + reduce), log2(R) forward layers per
 //              column, every operand chunk of a bucket in one launch
 //   k_col_inv  log2(R) inverse layers per column, in place
-#include "ntt_device.h"
+#include "ntt_regs.h"
+
+#include <cstdlib>
 
 namespace pintt {
 namespace {
@@ -80,6 +82,128 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     }
 }
 
+// Column passes with register-resident stages (ntt_regs.h) for tiles of 2^LR
+// rows (LR = 9, 10) by 8 columns, 2^(LR - 1) threads.  Row r of column w:
+// stage A runs layers 0..5 on r's top six bits for the groups g = (r_low, w)
+// (r_low = r's low LR - 6 bits; g = (wave << 4) | (lane & 15), w = g & 7);
+// stage B runs the last LR - 6 layers inside each lane on the groups
+// (r >> (LR - 6), w).  Global memory is read and written in the stage A entry
+// layout (forward) or the stage B layout (inverse) as whole 32-byte column
+// rows; LDS (word (r << 3) | w, padded by S words per 64) carries the one
+// exchange between the stages, at 32 distinct banks per half-wave both ways:
+// S = 1 for 2^10 rows, 2 for 2^9, and the stage B lane bits chosen per shape.
+template <int LR>
+struct RegCols {
+    static_assert(LR == 9 || LR == 10, "register column tiles: 2^9 or 2^10 rows");
+    static constexpr int NT = 1 << (LR - 1);
+    static constexpr int KB = LR - 6;  // stage B layers
+    static constexpr uint32_t S = LR == 10 ? 1u : 2u;
+    static constexpr int WORDS = (8 << LR) + (int)S * ((8 << LR) >> 6);
+    __device__ static __forceinline__ uint32_t lds_at(uint32_t r, uint32_t w) {
+        const uint32_t k = (r << 3) | w;
+        return k + S * (k >> 6);
+    }
+    // Stage A: row of entry-layout register i / post-layout register i.
+    __device__ static __forceinline__ uint32_t row_entry(int i, uint32_t lane, uint32_t rlow) {
+        return (entry_t(i, lane) << (LR - 6)) | rlow;
+    }
+    __device__ static __forceinline__ uint32_t row_post(int i, uint32_t lane, uint32_t rlow) {
+        return (post_t(i, lane) << (LR - 6)) | rlow;
+    }
+    // Stage B: the lane's rows are (hi << 4) | i; hi from the lane and wave.
+    __device__ static __forceinline__ uint32_t hi_bits(uint32_t lane, uint32_t wave) {
+        const uint32_t l3 = (lane >> 3) & 1u, l4 = (lane >> 4) & 1u, l5 = (lane >> 5) & 1u;
+        if constexpr (LR == 10)  // r4 = L5, r5 = wave0, r6 = L3, r7 = L4, r8, r9 = wave1, wave2
+            return l5 | ((wave & 1u) << 1) | (l3 << 2) | (l4 << 3) | ((wave >> 1) << 4);
+        else  // r4 = L5, r5 = L3, r6 = L4, r7, r8 = wave0, wave1 (r3 = register bit 3)
+            return l5 | (l3 << 1) | (l4 << 2) | (wave << 3);
+    }
+};
+
+// Stage B twiddles: group g of the lane (register bit 3 when KB = 3) at layer
+// 6 + l, block ((hi << (4 - KB)) | g) << l | blk.
+struct TwColB {
+    TwTable z;
+    uint32_t hi;
+    int KB;
+    __device__ __forceinline__ uint2 operator()(int g, int l, uint32_t blk) const {
+        const uint32_t outer = KB == 4 ? hi : ((hi << 1) | (uint32_t)g);
+        return tw_at(z, (1u << (6 + l)) + (outer << l) + blk);
+    }
+};
+struct TwColA {
+    TwTable z;
+    __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const { return tw_at(z, (1u << l) + blk); }
+};
+
+template <int LR, int SET>
+__global__ void __launch_bounds__(RegCols<LR>::NT) k_col_fwd_reg(const JobList<FwdJob> jobs, int log2c, Twiddles tw) {
+    using RC = RegCols<LR>;
+    __shared__ uint32_t lds[RC::WORDS];
+    const FwdJob job = PINTT_JOB(FwdJob, jobs, blockIdx.z);
+    const int prime = blockIdx.y;
+    const uint32_t p = prime_of<SET>(prime);
+    global_ptr<uint32_t> dst = as_global(job.dst) + ((size_t)prime << (LR + log2c));
+    global_ptr<const uint32_t> src = as_global(job.src);
+    const TwTable z = PINTT_FWD_TABLE(tw, prime);
+    const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << 3;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t g = (wave << 4) | (lane & 15u), w = g & 7u, rlow = g >> 3;
+    uint32_t v[1][16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t k = (RC::row_entry(i, lane, rlow) << log2c) + c0 + w;
+        v[0][i] = Arith<SET>::load_coeff(k < job.ncoef ? src[k] : 0u, p);
+    }
+    ct6<SET, 1>(v, TwColA{z}, p);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lds[RC::lds_at(RC::row_post(i, lane, rlow), w)] = v[0][i];
+    __syncthreads();
+    const uint32_t hi = RC::hi_bits(lane, wave), wb = lane & 7u;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at((hi << 4) | (uint32_t)i, wb)];
+    ct_lane<SET, RC::KB>(v[0], TwColB{z, hi, RC::KB}, p);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) dst[((((hi << 4) | (uint32_t)i)) << log2c) + c0 + wb] = v[0][i];
+}
+
+template <int LR, int SET>
+__global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<InvJob> jobs, int log2c, Twiddles tw) {
+    using RC = RegCols<LR>;
+    __shared__ uint32_t lds[RC::WORDS];
+    const InvJob job = PINTT_JOB(InvJob, jobs, blockIdx.z);
+    const int prime = blockIdx.y;
+    const uint32_t p = prime_of<SET>(prime);
+    global_ptr<uint32_t> buf = as_global(job.buf) + ((size_t)prime << (LR + log2c));
+    const TwTable z = PINTT_INV_TABLE(tw, prime);
+    const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << 3;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t hi = RC::hi_bits(lane, wave), wb = lane & 7u;
+    uint32_t v[1][16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[0][i] = buf[((((hi << 4) | (uint32_t)i)) << log2c) + c0 + wb];
+    gs_lane<SET, RC::KB>(v[0], TwColB{z, hi, RC::KB}, p);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lds[RC::lds_at((hi << 4) | (uint32_t)i, wb)] = v[0][i];
+    __syncthreads();
+    const uint32_t g = (wave << 4) | (lane & 15u), w = g & 7u, rlow = g >> 3;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at(RC::row_post(i, lane, rlow), w)];
+    gs6<SET, 1>(v, TwColA{z}, p);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) buf[(RC::row_entry(i, lane, rlow) << log2c) + c0 + w] = v[0][i];
+}
+
+// Column tiles of 2^9 and 2^10 rows by 8 columns go to the register kernels
+// unless PINTT_REG_COLS=0.
+bool reg_cols_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("PINTT_REG_COLS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 }  // namespace
 
 // Tile shapes per prime set.  The wide set only serves 2^23 .. 2^25 points,
@@ -102,6 +226,16 @@ bool col_case_available(int set, int log2r, int log2w) {
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
+    if (set_for(pl.log2n) == kSetNarrow && pl.log2w == 3 && reg_cols_enabled()) {
+        if (pl.log2r == 9) {
+            hipLaunchKernelGGL((k_col_fwd_reg<9, kSetNarrow>), grid, dim3(RegCols<9>::NT), 0, s, jobs, pl.log2c, tw);
+            return hipGetLastError();
+        }
+        if (pl.log2r == 10) {
+            hipLaunchKernelGGL((k_col_fwd_reg<10, kSetNarrow>), grid, dim3(RegCols<10>::NT), 0, s, jobs, pl.log2c, tw);
+            return hipGetLastError();
+        }
+    }
 #define PINTT_CASE_SET(LR, LW, SET)                                                                  \
     if (pl.log2r == LR && pl.log2w == LW) {                                                          \
         hipLaunchKernelGGL((k_col_fwd<LR, LW, SET>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs, \
@@ -124,6 +258,16 @@ hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, u
 hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
+    if (set_for(pl.log2n) == kSetNarrow && pl.log2w == 3 && reg_cols_enabled()) {
+        if (pl.log2r == 9) {
+            hipLaunchKernelGGL((k_col_inv_reg<9, kSetNarrow>), grid, dim3(RegCols<9>::NT), 0, s, jobs, pl.log2c, tw);
+            return hipGetLastError();
+        }
+        if (pl.log2r == 10) {
+            hipLaunchKernelGGL((k_col_inv_reg<10, kSetNarrow>), grid, dim3(RegCols<10>::NT), 0, s, jobs, pl.log2c, tw);
+            return hipGetLastError();
+        }
+    }
 #define PINTT_CASE_SET(LR, LW, SET)                                                                  \
     if (pl.log2r == LR && pl.log2w == LW) {                                                          \
         hipLaunchKernelGGL((k_col_inv<LR, LW, SET>), grid, dim3(Tile<LR, LW, true>::NT), 0, s, jobs, \

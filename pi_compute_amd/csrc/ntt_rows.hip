@@ -1,6 +1,6 @@
 // ntt_rows.hip - row pass of the four-step NTT (gfx950): forward layers on the
 // rows of the operands, Montgomery pointwise products, inverse layers, in place.
-#include "ntt_device.h"
+#include "ntt_regs.h"
 
 #include <cstdlib>
 
@@ -150,6 +150,186 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     }
 }
 
+// Twiddle of stage layer l, block blk, for a stage whose first layer is
+// global layer L0 and whose group sits in block `outer` there.
+struct TwStage {
+    TwTable z;
+    int L0;
+    uint32_t outer;
+    __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const {
+        return tw_at(z, (1u << (L0 + l)) + (outer << l) + blk);
+    }
+};
+
+// Row pass for rows of 2^12 points, one row per workgroup of 256 threads,
+// with register-resident six-layer stages (ntt_regs.h): stage A runs layers
+// 0..5 on the groups of 64 elements 64 apart (group j = element index mod 64),
+// stage B layers 6..11 on the 64 contiguous groups.  Positions in the row:
+//   stage A, entry layout: k = (i << 8) | lanepos, lanepos = (L5 L4 wave q)
+//   stage A, post layout:  k = (post_t(r) << 6) | j
+//   stage B, entry layout: k = (j << 6) | (i << 2) | (L5 L4)
+//   stage B, post layout:  k = (j << 6) | post_t(r)
+// with j = (wave << 4) | q, q = lane & 15.  Global loads and stores use the
+// stage A entry layout (16 consecutive words per 16 lanes); LDS, padded by two
+// words per 64 (address k + 2 (k >> 6)), is visited twice: after stage A and
+// after stage B's inverse.  Both LDS layouts hit 32 distinct banks per
+// half-wave.  The pointwise product is formed in registers between the
+// forward and the inverse stage B.  Transformed rows of b (kRowTransformB with
+// write_b, kRowReuseB, kRowPair with write_b) are stored in register order,
+// word (r << 8) | lanepos for register r of the stage B post layout: every
+// kernel of this shape reads and writes them the same way.
+// SCALED as in k_row_conv: rows scaled by a_r^(+-k) on load and store, block
+// 0's twiddles; otherwise the row's own twiddles.
+template <int KIND, int SET, bool SCALED>
+__global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
+    constexpr int LC = 12, NT = 256;
+    constexpr bool kPair = KIND == kRowPair;
+    constexpr bool kTransB = KIND == kRowTransformB;
+    constexpr int NA = kPair ? 2 : 1;                       // chunk tiles
+    constexpr int NL = NA + ((kPair || kTransB) ? 1 : 0);   // tiles through stage A forward
+    constexpr int TWORDS = (1 << LC) + 2 * ((1 << LC) >> 6);
+    __shared__ uint32_t lds[NL][TWORDS];
+    const RowJob job = PINTT_JOB(RowJob, jobs, blockIdx.z);
+    const int prime = blockIdx.y;
+    const uint32_t p = prime_of<SET>(prime);
+    const uint32_t pinv = neg_inv_mod_2_32(p);
+    const size_t base = ((size_t)prime << (log2r + LC)) + ((size_t)blockIdx.x << LC);
+    const uint32_t row0 = blockIdx.x;
+    global_ptr<uint32_t> ga[2] = {as_global(job.a) + base, as_global(job.a2) + base};
+    global_ptr<uint32_t> gb = as_global(job.b) + base;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t j = (wave << 4) | (lane & 15u);
+    const uint32_t lanepos = ((lane >> 4) << 6) | j;
+    const int sb = SCALED ? 0 : log2r;
+    const uint32_t rb = SCALED ? 0u : row0;
+    const TwTable zf = PINTT_FWD_TABLE(tw, prime);
+    const TwTable zi = PINTT_INV_TABLE(tw, prime);
+    const TwStage fA{zf, sb, rb}, fB{zf, sb + 6, (rb << 6) | j};
+    const TwStage iA{zi, sb, rb}, iB{zi, sb + 6, (rb << 6) | j};
+    auto lds_at = [](uint32_t k) { return k + 2u * (k >> 6); };
+
+    // Scaling factors (SCALED): element k = (i << 8) | lanepos is scaled by
+    // a_r^k = a_r^lanepos * a_r^(i << 8); the second factor is uniform and sits
+    // in LDS with its Shoup quotient, the first is the lane's own (Montgomery
+    // form).  As in k_row_conv.
+    __shared__ uint2 beta[SCALED ? 32 : 1];
+    uint32_t h_fwd = 0, h_inv = 0;
+    const uint32_t pinv_pos = 0u - pinv;
+    if constexpr (SCALED) {
+        const RowTable rt = PINTT_ROW_TABLE(tw, prime);
+        constexpr uint32_t kMask = (1u << kSetMaxLog2N[SET]) - 1;
+        const uint32_t rbits = __brev(row0) >> (32 - log2r);
+        const int eshift = kSetMaxLog2N[SET] - (log2r + LC);
+        const uint32_t et = (rbits * lanepos) << eshift;
+        h_fwd = row_factor(rt, et, p);
+        h_inv = row_factor(rt, (0u - et) & kMask, p);
+        if (threadIdx.x < 32) {
+            const uint32_t e = threadIdx.x & 15u;
+            const uint32_t eu = (rbits * (e * NT)) << eshift;
+            const uint32_t bm = row_factor(rt, threadIdx.x < 16 ? eu : (0u - eu) & kMask, p);
+            const uint32_t bv = mul_mont(bm, 1u, p, pinv);  // out of Montgomery form
+            beta[threadIdx.x] = make_uint2(bv, (uint32_t)(((uint64_t)bv << 32) / p));
+        }
+        __syncthreads();
+    }
+    auto factor = [&](int e, bool inverse) {
+        const uint2 bq = beta[(inverse ? 16 : 0) + e];
+        const uint32_t f = mul_shoup(inverse ? h_inv : h_fwd, bq.x, bq.y, p);
+        return make_uint2(f, f * pinv_pos);
+    };
+
+    // Stage A forward on every tile: chunk tiles first, then b.
+    {
+        uint32_t v[NL][16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t k = ((uint32_t)i << 8) | lanepos;
+#pragma unroll
+            for (int u = 0; u < NL; ++u) v[u][i] = u < NA ? ga[u][k] : gb[k];
+            if constexpr (SCALED) {
+                const uint2 f = factor(i, false);
+#pragma unroll
+                for (int u = 0; u < NL; ++u) v[u][i] = Arith<SET>::scale(v[u][i], f.x, f.y, p);
+            }
+        }
+        ct6<SET, NL>(v, fA, p);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t k = (post_t(r, lane) << 6) | j;
+#pragma unroll
+            for (int u = 0; u < NL; ++u) lds[u][lds_at(k)] = v[u][r];
+        }
+    }
+    __syncthreads();
+    // Stage B: b's transformed row (post layout), then each chunk: forward,
+    // pointwise, inverse, back to LDS in the stage B entry layout.
+    uint32_t vb[1][16];
+    if constexpr (KIND == kRowReuseB) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) vb[0][r] = gb[((uint32_t)r << 8) | lanepos];
+    } else if constexpr (kPair || kTransB) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) vb[0][i] = lds[NL - 1][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))];
+        ct6<SET, 1>(vb, fB, p);
+        if (job.write_b) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) gb[((uint32_t)r << 8) | lanepos] = vb[0][r];
+        }
+    }
+    {
+        uint32_t v[NA][16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+#pragma unroll
+            for (int u = 0; u < NA; ++u) v[u][i] = lds[u][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))];
+        ct6<SET, NA>(v, fB, p);
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+#pragma unroll
+            for (int u = 0; u < NA; ++u)
+                v[u][r] = Arith<SET>::pointwise(v[u][r], KIND == kRowSquare ? v[u][r] : vb[0][r], p, pinv);
+        gs6<SET, NA>(v, iB, p);
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+#pragma unroll
+            for (int u = 0; u < NA; ++u) lds[u][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))] = v[u][i];
+    }
+    __syncthreads();
+    // Stage A inverse, then the chunk products go back in place.
+    {
+        uint32_t v[NA][16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t k = (post_t(r, lane) << 6) | j;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) v[u][r] = lds[u][lds_at(k)];
+        }
+        gs6<SET, NA>(v, iA, p);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t k = ((uint32_t)i << 8) | lanepos;
+            if constexpr (SCALED) {
+                const uint2 f = factor(i, true);
+#pragma unroll
+                for (int u = 0; u < NA; ++u) ga[u][k] = Arith<SET>::scale(v[u][i], f.x, f.y, p);
+            } else {
+#pragma unroll
+                for (int u = 0; u < NA; ++u) ga[u][k] = v[u][i];
+            }
+        }
+    }
+}
+
+// Rows of 2^12 points, one per workgroup, go to k_row_reg12 unless
+// PINTT_REG_ROWS=0.
+bool reg_rows_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("PINTT_REG_ROWS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 // Scaled rows from this transform length on (PINTT_SCALED_MIN), for pairs
 // between PINTT_SCALED_PAIR_MIN and PINTT_SCALED_PAIR_MAX.
 int env_int(const char* name, int dflt) {
@@ -165,6 +345,15 @@ void launch_rows(const Plan& pl, dim3 grid, const JobList<RowJob>& jobs, const T
     if constexpr (LROWS == 0) {
         const bool scaled = K == kRowPair ? pl.log2n >= kScaledPairMinLog2N && pl.log2n <= kScaledPairMaxLog2N
                                           : pl.log2n >= kScaledRowsMinLog2N;
+        if constexpr (LC == 12) {
+            if (reg_rows_enabled()) {
+                if (scaled)
+                    hipLaunchKernelGGL((k_row_reg12<K, SET, true>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
+                else
+                    hipLaunchKernelGGL((k_row_reg12<K, SET, false>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
+                return;
+            }
+        }
         if (scaled) {
             hipLaunchKernelGGL((k_row_conv<LC, LROWS, K, SET, true>), grid, dim3(Tile<LC, LROWS, false>::NT), 0, s,
                                jobs, pl.log2r, tw);
