@@ -123,7 +123,7 @@ struct RegCols {
 // Stage B twiddles: group g of the lane (register bit 3 when KB = 3) at layer
 // 6 + l, block ((hi << (4 - KB)) | g) << l | blk.
 struct TwColB {
-    TwTable z;
+    ConstTable z;
     uint32_t hi;
     int KB;
     __device__ __forceinline__ uint2 operator()(int g, int l, uint32_t blk) const {
@@ -132,7 +132,7 @@ struct TwColB {
     }
 };
 struct TwColA {
-    TwTable z;
+    ConstTable z;
     __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const { return tw_at(z, (1u << l) + blk); }
 };
 
@@ -150,19 +150,31 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_fwd_reg(const JobList<F
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint32_t g = (wave << 4) | (lane & 15u), w = g & 7u, rlow = g >> 3;
     uint32_t v[1][16];
+    // Rows from R/2 on hold no coefficient when the operand fills at most half
+    // the transform (every operand of a product of two halves, every square):
+    // then layer 0 is a copy and the top half is not read.
+    if (job.ncoef <= (1u << (LR + log2c - 1))) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const uint32_t k = (RC::row_entry(i, lane, rlow) << log2c) + c0 + w;
-        v[0][i] = Arith<SET>::load_coeff(k < job.ncoef ? src[k] : 0u, p);
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t k = (RC::row_entry(i, lane, rlow) << log2c) + c0 + w;
+            v[0][i] = Arith<SET>::load_coeff(k < job.ncoef ? src[k] : 0u, p);
+        }
+        ct6<SET, 1, true>(v, TwColA{as_const(z)}, p);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t k = (RC::row_entry(i, lane, rlow) << log2c) + c0 + w;
+            v[0][i] = Arith<SET>::load_coeff(k < job.ncoef ? src[k] : 0u, p);
+        }
+        ct6<SET, 1>(v, TwColA{as_const(z)}, p);
     }
-    ct6<SET, 1>(v, TwColA{z}, p);
 #pragma unroll
     for (int i = 0; i < 16; ++i) lds[RC::lds_at(RC::row_post(i, lane, rlow), w)] = v[0][i];
     __syncthreads();
     const uint32_t hi = RC::hi_bits(lane, wave), wb = lane & 7u;
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at((hi << 4) | (uint32_t)i, wb)];
-    ct_lane<SET, RC::KB>(v[0], TwColB{z, hi, RC::KB}, p);
+    ct_lane<SET, RC::KB>(v[0], TwColB{as_const(z), hi, RC::KB}, p);
 #pragma unroll
     for (int i = 0; i < 16; ++i) dst[((((hi << 4) | (uint32_t)i)) << log2c) + c0 + wb] = v[0][i];
 }
@@ -182,14 +194,14 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<I
     uint32_t v[1][16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[0][i] = buf[((((hi << 4) | (uint32_t)i)) << log2c) + c0 + wb];
-    gs_lane<SET, RC::KB>(v[0], TwColB{z, hi, RC::KB}, p);
+    gs_lane<SET, RC::KB>(v[0], TwColB{as_const(z), hi, RC::KB}, p);
 #pragma unroll
     for (int i = 0; i < 16; ++i) lds[RC::lds_at((hi << 4) | (uint32_t)i, wb)] = v[0][i];
     __syncthreads();
     const uint32_t g = (wave << 4) | (lane & 15u), w = g & 7u, rlow = g >> 3;
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at(RC::row_post(i, lane, rlow), w)];
-    gs6<SET, 1>(v, TwColA{z}, p);
+    gs6<SET, 1>(v, TwColA{as_const(z)}, p);
 #pragma unroll
     for (int i = 0; i < 16; ++i) buf[(RC::row_entry(i, lane, rlow) << log2c) + c0 + w] = v[0][i];
 }

@@ -41,6 +41,15 @@ __device__ __forceinline__ uint2 tw_at(TwTable z, uint32_t i) {
     return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
 }
 __device__ __forceinline__ uint2 tw_at(const uint2* z, uint32_t i) { return z[i]; }
+// The same table read through the constant address space: the tables never
+// change while a kernel runs, so a wave-uniform index becomes a scalar load
+// (no vector registers held for twiddles shared by the whole wave).
+using ConstTable = __attribute__((address_space(4))) const unsigned long long*;
+__device__ __forceinline__ ConstTable as_const(TwTable z) { return (ConstTable)(const unsigned long long*)z; }
+__device__ __forceinline__ uint2 tw_at(ConstTable z, uint32_t i) {
+    const unsigned long long v = z[i];
+    return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+}
 #define PINTT_FWD_TABLE(tw, prime) pick((tw).fwd[0], (tw).fwd[1], (tw).fwd[2], (prime))
 #define PINTT_INV_TABLE(tw, prime) pick((tw).inv[0], (tw).inv[1], (tw).inv[2], (prime))
 

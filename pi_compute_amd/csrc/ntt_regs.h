@@ -49,11 +49,18 @@ __device__ __forceinline__ void lane_swaps(uint32_t (&v)[NB][16]) {
 }
 
 // Six forward layers on NB tiles sharing twiddles; entry layout in, post
-// layout out.
-template <int SET, int NB, class TW>
+// layout out.  ZT: the group's top half (t >= 32, registers 8..15) is zero,
+// so layer 0 only copies the bottom half up.
+template <int SET, int NB, bool ZT = false, class TW>
 __device__ __forceinline__ void ct6(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
+    if constexpr (ZT) {
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
+        for (int u = 0; u < NB; ++u)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[u][i + 8] = v[u][i];
+    }
+#pragma unroll
+    for (int l = ZT ? 1 : 0; l < 4; ++l) {
         const int h = 8 >> l;
 #pragma unroll
         for (int sub = 0; sub < (1 << l); ++sub) {

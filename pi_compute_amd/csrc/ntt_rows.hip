@@ -153,7 +153,7 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
 // Twiddle of stage layer l, block blk, for a stage whose first layer is
 // global layer L0 and whose group sits in block `outer` there.
 struct TwStage {
-    TwTable z;
+    ConstTable z;
     int L0;
     uint32_t outer;
     __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const {
@@ -204,8 +204,8 @@ __global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, i
     const uint32_t rb = SCALED ? 0u : row0;
     const TwTable zf = PINTT_FWD_TABLE(tw, prime);
     const TwTable zi = PINTT_INV_TABLE(tw, prime);
-    const TwStage fA{zf, sb, rb}, fB{zf, sb + 6, (rb << 6) | j};
-    const TwStage iA{zi, sb, rb}, iB{zi, sb + 6, (rb << 6) | j};
+    const TwStage fA{as_const(zf), sb, rb}, fB{as_const(zf), sb + 6, (rb << 6) | j};
+    const TwStage iA{as_const(zi), sb, rb}, iB{as_const(zi), sb + 6, (rb << 6) | j};
     auto lds_at = [](uint32_t k) { return k + 2u * (k >> 6); };
 
     // Scaling factors (SCALED): element k = (i << 8) | lanepos is scaled by
@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, i
         return make_uint2(f, f * pinv_pos);
     };
 
-    // Stage A forward on every tile: chunk tiles first, then b.
+    // Stage A forward on every tile at once (chunk tiles, then b): their loads
+    // are all in flight together and the tiles share each twiddle.
     {
         uint32_t v[NL][16];
 #pragma unroll
@@ -295,7 +296,7 @@ __global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, i
             for (int u = 0; u < NA; ++u) lds[u][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))] = v[u][i];
     }
     __syncthreads();
-    // Stage A inverse, then the chunk products go back in place.
+    // Stage A inverse on the chunk tiles together, then back in place.
     {
         uint32_t v[NA][16];
 #pragma unroll
