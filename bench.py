@@ -94,6 +94,19 @@ def survey_bytes(an: int, bn: int, log2n: int, square: bool) -> int:
     return 24 * an + 96 * n if square else 16 * (an + bn) + 144 * n
 
 
+def clock_warmup(stream, da, an, db, bn, dout, seconds: float) -> int:
+    """Untimed products for about `seconds` of wall time, so that the timed
+    region starts with the GPU at its working clocks (a few milliseconds of
+    warmup steps leave it ramping up).  Returns the number of products."""
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(16):
+            pc.mul_device(da.ptr, an, db.ptr, bn, dout.ptr, stream.handle)
+        stream.synchronize()
+        n += 16
+    return n
+
+
 def time_gpu(stream, da, an, db, bn, dout, steps: int, warmup: int) -> float:
     for _ in range(warmup):
         pc.mul_device(da.ptr, an, db.ptr, bn, dout.ptr, stream.handle)
@@ -569,6 +582,8 @@ def main() -> int:
                     help="seconds of all-core GMP work per sweep shape (0: one-thread medians only)")
     ap.add_argument("--config3-digits", type=int, default=10_000_000,
                     help="digits of the per-call pi leg (0 skips it)")
+    ap.add_argument("--clock-warmup", type=float, default=0.5, metavar="S",
+                    help="seconds of untimed products before the warmup steps (GPU clock ramp)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -609,6 +624,7 @@ def main() -> int:
         hiprt.check(hiprt.runtime().hipMemcpy(dst.ptr, src.ptr, nbytes, hiprt.D2D), "calibration copy")
         src.close()
         dst.close()
+    prewarm = clock_warmup(stream, da, an, db, bn, dout, args.clock_warmup)
     ms_mul = time_gpu(stream, da, an, db, bn, dout, args.steps, args.warmup)
     gpu_out = dout.to_array()
     ms_sqr = time_gpu(stream, da, an, da, an, dsq, args.steps, args.warmup)
@@ -638,6 +654,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (splitmix64 seed 26 operands, resident in HBM)",
+        "clock_warmup": {"seconds": args.clock_warmup, "products": prewarm,
+                         "note": "untimed products before the warmup steps, so the timed steps run at working clocks"},
         "config": {"workload": "exact multiply 2^26 x floor(2^26/3) bits",
                    "limbs": [an, bn],
                    "transform": f"{pp['chunks']} chunk product(s) at N=2^{pp['log2n']}, "
