@@ -139,7 +139,8 @@ struct Engine {
     float plan_ms[kMaxLog2N + 1] = {};
     int split_mode = 1;  // 0: never cut operands into chunks, 1: when cheaper, m >= 2: m chunks when valid
     bool pair_rows = true;  // first two chunks of a group in one row launch (PINTT_PAIR=0: off)
-    size_t tune_cap = 0;
+    size_t tune_cap = 0;      // limbs per tuning operand
+    size_t tune_out_cap = 0;  // limbs of tuning outputs
     uint64_t* tune_in = nullptr;
     uint64_t* tune_out = nullptr;
     // This device's copy of `scratch` (its device address, from
@@ -911,16 +912,24 @@ int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
 }
 
 // Time every compiled tile plan for length 2^log2n on synthetic operands
-// (an = bn = N/4 limbs, no chunking) and keep the fastest.  Runs on the
-// library stream and waits for it, so it must not happen inside a capture.
+// (an = bn = N/4 limbs, no chunking) and keep the fastest.  Below 2^22
+// points a single product is launch-bound and every plan measures the same,
+// so a plan is timed on a batch of 2^(22 - log2n) products (2^22 points of
+// transforms in all, the regime of tree levels and batch calls), all reading
+// the same operands and writing their own outputs.  Runs on the library
+// stream and waits for it, so it must not happen inside a capture.
 int tune(int log2n) {
     const size_t n = (size_t)1 << log2n;
     const size_t an = n / 4, bn = n / 4;
+    const size_t count = log2n >= 22 ? 1 : ((size_t)1 << (22 - log2n));
     if (g.tune_cap < an) {
         PINTT_TRY(grow(g.tune_in, 2 * an));
-        PINTT_TRY(grow(g.tune_out, 2 * an));
         PINTT_HIP(hipMemset(g.tune_in, 0xA7, 2 * an * sizeof(uint64_t)));
         g.tune_cap = an;
+    }
+    if (g.tune_out_cap < count * 2 * an) {
+        PINTT_TRY(grow(g.tune_out, count * 2 * an));
+        g.tune_out_cap = count * 2 * an;
     }
     const int saved_split = g.split_mode;
     const bool saved_timing = g.timing;
@@ -929,7 +938,9 @@ int tune(int log2n) {
     g.school = false;
     Plan best = make_plan(log2n);
     float best_ms = 1e30f;
-    const std::vector<Request> one{Request{g.tune_in, an, g.tune_in + an, bn, g.tune_out, false}};
+    std::vector<Request> one;
+    for (size_t i = 0; i < count; ++i)
+        one.push_back(Request{g.tune_in, an, g.tune_in + an, bn, g.tune_out + i * 2 * an, false});
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int rc = hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess
                  ? PINTT_OK

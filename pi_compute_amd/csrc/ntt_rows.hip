@@ -161,79 +161,134 @@ struct TwStage {
     }
 };
 
-// Row pass for rows of 2^12 points, one row per workgroup of 256 threads,
-// with register-resident six-layer stages (ntt_regs.h): stage A runs layers
-// 0..5 on the groups of 64 elements 64 apart (group j = element index mod 64),
-// stage B layers 6..11 on the 64 contiguous groups.  Positions in the row:
-//   stage A, entry layout: k = (i << 8) | lanepos, lanepos = (L5 L4 wave q)
-//   stage A, post layout:  k = (post_t(r) << 6) | j
-//   stage B, entry layout: k = (j << 6) | (i << 2) | (L5 L4)
-//   stage B, post layout:  k = (j << 6) | post_t(r)
-// with j = (wave << 4) | q, q = lane & 15.  Global loads and stores use the
-// stage A entry layout (16 consecutive words per 16 lanes); LDS, padded by two
-// words per 64 (address k + 2 (k >> 6)), is visited twice: after stage A and
-// after stage B's inverse.  Both LDS layouts hit 32 distinct banks per
-// half-wave.  The pointwise product is formed in registers between the
+// Geometry of the register row kernels: rows of 2^LC points (LC = 10, 11,
+// 12), 16 residues per thread, 256 threads: a row takes 2^(LC-4) threads (one,
+// two or four waves) and a workgroup 2^(12-LC) rows.  Stage A runs layers
+// 0..5 of a row on its 2^(LC-6) groups of 64 elements 2^(LC-6) apart, stage B
+// layers 6..LC-1 on its 64 groups of 2^(LC-6) contiguous elements: a six-layer
+// lane-swap stage (LC = 12), a five-layer one (LC = 11) or four layers inside
+// each lane (LC = 10).  Positions in a row, for the thread's row-local index
+// ell (wave w_r = ell >> 6 within the row), lane L and register i / r:
+//   j = (w_r << 4) | (L & 15)                 stage A group
+//   lanepos = ((L >> 4) << (LC - 6)) | j
+//   stage A entry: k = (i << (LC - 4)) | lanepos   (global loads and stores)
+//   stage A post:  k = (post_t<SB>(r) << (LC - 6)) | j
+//   stage B entry: k = (u << (LC - 6)) | t', with
+//     LC = 12: u = j,                    t' = (i << 2) | (L >> 4)
+//     LC = 11: u = (w_r << 5) | (L & 31), t' = (i << 1) | (L >> 5)
+//     LC = 10: u = L,                    t' = i
+// LDS, padded by PW words per 2^PSH, is visited twice (after stage A and after
+// stage B's inverse); SB and the padding are chosen so that both layouts hit
+// 32 distinct banks per half-wave (tests/test_reg_layouts.py checks them).
+template <int LC>
+struct RegRows {
+    static_assert(LC >= 10 && LC <= 12, "register rows of 2^10 .. 2^12 points");
+    static constexpr int GA = LC - 6;
+    static constexpr int LPR = 1 << (LC - 4);  // threads per row
+    static constexpr int RPW = 256 / LPR;      // rows per workgroup
+    static constexpr int SB = LC == 12 ? 1 : 2;
+    static constexpr int PSH = LC == 12 ? 6 : (LC == 11 ? 5 : 4);
+    static constexpr uint32_t PW = LC == 12 ? 2u : 1u;
+    static constexpr int WORDS = (1 << LC) + (int)PW * ((1 << LC) >> PSH);
+    __device__ static __forceinline__ uint32_t lds_at(uint32_t k) { return k + PW * (k >> PSH); }
+    // Stage B entry position of register i.
+    __device__ static __forceinline__ uint32_t b_entry(int i, uint32_t lane, uint32_t wr) {
+        if constexpr (LC == 12) return (((wr << 4) | (lane & 15u)) << 6) | ((uint32_t)i << 2) | (lane >> 4);
+        else if constexpr (LC == 11) return (((wr << 5) | (lane & 31u)) << 5) | ((uint32_t)i << 1) | (lane >> 5);
+        else return (lane << 4) | (uint32_t)i;
+    }
+    // Stage B group index u.
+    __device__ static __forceinline__ uint32_t b_group(uint32_t lane, uint32_t wr) {
+        if constexpr (LC == 12) return (wr << 4) | (lane & 15u);
+        else if constexpr (LC == 11) return (wr << 5) | (lane & 31u);
+        else return lane;
+    }
+};
+
+// Stage B of a register row kernel, forward or inverse, on NB tiles.
+template <int LC, int SET, int NB, bool INV, class TW>
+__device__ __forceinline__ void row_stage_b(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
+    if constexpr (LC == 12) {
+        if constexpr (INV) gs6<SET, NB, 1>(v, tw, p);
+        else ct6<SET, NB, false, 1>(v, tw, p);
+    } else if constexpr (LC == 11) {
+        if constexpr (INV) gs5<SET, NB>(v, tw, p);
+        else ct5<SET, NB>(v, tw, p);
+    } else {
+        auto tl = [&](int, int l, uint32_t blk) { return tw(l, blk); };
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            if constexpr (INV) gs_lane<SET, 4>(v[u], tl, p);
+            else ct_lane<SET, 4>(v[u], tl, p);
+        }
+    }
+}
+
+// Row pass with register-resident stages (ntt_regs.h) for rows of 2^LC
+// points (RegRows).  The pointwise product is formed in registers between the
 // forward and the inverse stage B.  Transformed rows of b (kRowTransformB with
 // write_b, kRowReuseB, kRowPair with write_b) are stored in register order,
-// word (r << 8) | lanepos for register r of the stage B post layout: every
+// word (r << (LC - 4)) | ell for register r of the stage B post layout: every
 // kernel of this shape reads and writes them the same way.
 // SCALED as in k_row_conv: rows scaled by a_r^(+-k) on load and store, block
 // 0's twiddles; otherwise the row's own twiddles.
-template <int KIND, int SET, bool SCALED>
-__global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
-    constexpr int LC = 12, NT = 256;
+template <int LC, int KIND, int SET, bool SCALED>
+__global__ void __launch_bounds__(256) k_row_reg(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
+    using RG = RegRows<LC>;
     constexpr bool kPair = KIND == kRowPair;
     constexpr bool kTransB = KIND == kRowTransformB;
     constexpr int NA = kPair ? 2 : 1;                       // chunk tiles
     constexpr int NL = NA + ((kPair || kTransB) ? 1 : 0);   // tiles through stage A forward
-    constexpr int TWORDS = (1 << LC) + 2 * ((1 << LC) >> 6);
-    __shared__ uint32_t lds[NL][TWORDS];
+    __shared__ uint32_t lds[NL][RG::RPW][RG::WORDS];
     const RowJob job = PINTT_JOB(RowJob, jobs, blockIdx.z);
     const int prime = blockIdx.y;
     const uint32_t p = prime_of<SET>(prime);
     const uint32_t pinv = neg_inv_mod_2_32(p);
-    const size_t base = ((size_t)prime << (log2r + LC)) + ((size_t)blockIdx.x << LC);
-    const uint32_t row0 = blockIdx.x;
+    const uint32_t rw = threadIdx.x >> (LC - 4);            // row within the workgroup
+    const uint32_t ell = threadIdx.x & (RG::LPR - 1);        // thread within the row
+    const uint32_t row0 = blockIdx.x * RG::RPW + rw;
+    const size_t base = ((size_t)prime << (log2r + LC)) + ((size_t)row0 << LC);
     global_ptr<uint32_t> ga[2] = {as_global(job.a) + base, as_global(job.a2) + base};
     global_ptr<uint32_t> gb = as_global(job.b) + base;
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    const uint32_t j = (wave << 4) | (lane & 15u);
-    const uint32_t lanepos = ((lane >> 4) << 6) | j;
+    const uint32_t lane = lane_id(), wr = ell >> 6;
+    const uint32_t j = (wr << 4) | (lane & 15u);
+    const uint32_t lanepos = ((lane >> 4) << RG::GA) | j;
     const int sb = SCALED ? 0 : log2r;
     const uint32_t rb = SCALED ? 0u : row0;
+    const uint32_t u = RG::b_group(lane, wr);
     const TwTable zf = PINTT_FWD_TABLE(tw, prime);
     const TwTable zi = PINTT_INV_TABLE(tw, prime);
-    const TwStage fA{as_const(zf), sb, rb}, fB{as_const(zf), sb + 6, (rb << 6) | j};
-    const TwStage iA{as_const(zi), sb, rb}, iB{as_const(zi), sb + 6, (rb << 6) | j};
-    auto lds_at = [](uint32_t k) { return k + 2u * (k >> 6); };
+    const TwStage fA{as_const(zf), sb, rb}, fB{as_const(zf), sb + 6, (rb << 6) | u};
+    const TwStage iA{as_const(zi), sb, rb}, iB{as_const(zi), sb + 6, (rb << 6) | u};
 
-    // Scaling factors (SCALED): element k = (i << 8) | lanepos is scaled by
-    // a_r^k = a_r^lanepos * a_r^(i << 8); the second factor is uniform and sits
-    // in LDS with its Shoup quotient, the first is the lane's own (Montgomery
-    // form).  As in k_row_conv.
-    __shared__ uint2 beta[SCALED ? 32 : 1];
+    // Scaling factors (SCALED): element k = (i << (LC - 4)) | lanepos of row
+    // r is scaled by a_r^k = a_r^lanepos * a_r^(i << (LC - 4)); the second
+    // factor is uniform over the row and sits in LDS with its Shoup quotient,
+    // the first is the thread's own (Montgomery form).  As in k_row_conv.
+    __shared__ uint2 beta[SCALED ? RG::RPW : 1][SCALED ? 32 : 1];
     uint32_t h_fwd = 0, h_inv = 0;
     const uint32_t pinv_pos = 0u - pinv;
     if constexpr (SCALED) {
         const RowTable rt = PINTT_ROW_TABLE(tw, prime);
         constexpr uint32_t kMask = (1u << kSetMaxLog2N[SET]) - 1;
-        const uint32_t rbits = __brev(row0) >> (32 - log2r);
         const int eshift = kSetMaxLog2N[SET] - (log2r + LC);
+        const uint32_t rbits = __brev(row0) >> (32 - log2r);
         const uint32_t et = (rbits * lanepos) << eshift;
         h_fwd = row_factor(rt, et, p);
         h_inv = row_factor(rt, (0u - et) & kMask, p);
-        if (threadIdx.x < 32) {
+        if (threadIdx.x < 32u * RG::RPW) {
+            const uint32_t br = threadIdx.x >> 5;  // the row whose factor this thread makes
+            const uint32_t rbr = __brev(blockIdx.x * RG::RPW + br) >> (32 - log2r);
             const uint32_t e = threadIdx.x & 15u;
-            const uint32_t eu = (rbits * (e * NT)) << eshift;
-            const uint32_t bm = row_factor(rt, threadIdx.x < 16 ? eu : (0u - eu) & kMask, p);
+            const uint32_t eu = (rbr * (e << (LC - 4))) << eshift;
+            const uint32_t bm = row_factor(rt, (threadIdx.x & 16u) ? (0u - eu) & kMask : eu, p);
             const uint32_t bv = mul_mont(bm, 1u, p, pinv);  // out of Montgomery form
-            beta[threadIdx.x] = make_uint2(bv, (uint32_t)(((uint64_t)bv << 32) / p));
+            beta[br][threadIdx.x & 31u] = make_uint2(bv, (uint32_t)(((uint64_t)bv << 32) / p));
         }
         __syncthreads();
     }
     auto factor = [&](int e, bool inverse) {
-        const uint2 bq = beta[(inverse ? 16 : 0) + e];
+        const uint2 bq = beta[SCALED ? rw : 0][(inverse ? 16 : 0) + e];
         const uint32_t f = mul_shoup(inverse ? h_inv : h_fwd, bq.x, bq.y, p);
         return make_uint2(f, f * pinv_pos);
     };
@@ -244,37 +299,37 @@ __global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, i
         uint32_t v[NL][16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const uint32_t k = ((uint32_t)i << 8) | lanepos;
+            const uint32_t k = ((uint32_t)i << (LC - 4)) | lanepos;
 #pragma unroll
-            for (int u = 0; u < NL; ++u) v[u][i] = u < NA ? ga[u][k] : gb[k];
+            for (int t = 0; t < NL; ++t) v[t][i] = t < NA ? ga[t][k] : gb[k];
             if constexpr (SCALED) {
                 const uint2 f = factor(i, false);
 #pragma unroll
-                for (int u = 0; u < NL; ++u) v[u][i] = Arith<SET>::scale(v[u][i], f.x, f.y, p);
+                for (int t = 0; t < NL; ++t) v[t][i] = Arith<SET>::scale(v[t][i], f.x, f.y, p);
             }
         }
-        ct6<SET, NL>(v, fA, p);
+        ct6<SET, NL, false, RG::SB>(v, fA, p);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const uint32_t k = (post_t(r, lane) << 6) | j;
+            const uint32_t k = (post_t<RG::SB>(r, lane) << RG::GA) | j;
 #pragma unroll
-            for (int u = 0; u < NL; ++u) lds[u][lds_at(k)] = v[u][r];
+            for (int t = 0; t < NL; ++t) lds[t][rw][RG::lds_at(k)] = v[t][r];
         }
     }
     __syncthreads();
-    // Stage B: b's transformed row (post layout), then each chunk: forward,
-    // pointwise, inverse, back to LDS in the stage B entry layout.
+    // Stage B: b's transformed row (post layout), then the chunk tiles:
+    // forward, pointwise, inverse, back to LDS in the stage B entry layout.
     uint32_t vb[1][16];
     if constexpr (KIND == kRowReuseB) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) vb[0][r] = gb[((uint32_t)r << 8) | lanepos];
+        for (int r = 0; r < 16; ++r) vb[0][r] = gb[((uint32_t)r << (LC - 4)) | ell];
     } else if constexpr (kPair || kTransB) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) vb[0][i] = lds[NL - 1][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))];
-        ct6<SET, 1>(vb, fB, p);
+        for (int i = 0; i < 16; ++i) vb[0][i] = lds[NL - 1][rw][RG::lds_at(RG::b_entry(i, lane, wr))];
+        row_stage_b<LC, SET, 1, false>(vb, fB, p);
         if (job.write_b) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) gb[((uint32_t)r << 8) | lanepos] = vb[0][r];
+            for (int r = 0; r < 16; ++r) gb[((uint32_t)r << (LC - 4)) | ell] = vb[0][r];
         }
     }
     {
@@ -282,18 +337,18 @@ __global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, i
 #pragma unroll
         for (int i = 0; i < 16; ++i)
 #pragma unroll
-            for (int u = 0; u < NA; ++u) v[u][i] = lds[u][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))];
-        ct6<SET, NA>(v, fB, p);
+            for (int t = 0; t < NA; ++t) v[t][i] = lds[t][rw][RG::lds_at(RG::b_entry(i, lane, wr))];
+        row_stage_b<LC, SET, NA, false>(v, fB, p);
 #pragma unroll
         for (int r = 0; r < 16; ++r)
 #pragma unroll
-            for (int u = 0; u < NA; ++u)
-                v[u][r] = Arith<SET>::pointwise(v[u][r], KIND == kRowSquare ? v[u][r] : vb[0][r], p, pinv);
-        gs6<SET, NA>(v, iB, p);
+            for (int t = 0; t < NA; ++t)
+                v[t][r] = Arith<SET>::pointwise(v[t][r], KIND == kRowSquare ? v[t][r] : vb[0][r], p, pinv);
+        row_stage_b<LC, SET, NA, true>(v, iB, p);
 #pragma unroll
         for (int i = 0; i < 16; ++i)
 #pragma unroll
-            for (int u = 0; u < NA; ++u) lds[u][lds_at((j << 6) | ((uint32_t)i << 2) | (lane >> 4))] = v[u][i];
+            for (int t = 0; t < NA; ++t) lds[t][rw][RG::lds_at(RG::b_entry(i, lane, wr))] = v[t][i];
     }
     __syncthreads();
     // Stage A inverse on the chunk tiles together, then back in place.
@@ -301,27 +356,27 @@ __global__ void __launch_bounds__(256) k_row_reg12(const JobList<RowJob> jobs, i
         uint32_t v[NA][16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const uint32_t k = (post_t(r, lane) << 6) | j;
+            const uint32_t k = (post_t<RG::SB>(r, lane) << RG::GA) | j;
 #pragma unroll
-            for (int u = 0; u < NA; ++u) v[u][r] = lds[u][lds_at(k)];
+            for (int t = 0; t < NA; ++t) v[t][r] = lds[t][rw][RG::lds_at(k)];
         }
-        gs6<SET, NA>(v, iA, p);
+        gs6<SET, NA, RG::SB>(v, iA, p);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const uint32_t k = ((uint32_t)i << 8) | lanepos;
+            const uint32_t k = ((uint32_t)i << (LC - 4)) | lanepos;
             if constexpr (SCALED) {
                 const uint2 f = factor(i, true);
 #pragma unroll
-                for (int u = 0; u < NA; ++u) ga[u][k] = Arith<SET>::scale(v[u][i], f.x, f.y, p);
+                for (int t = 0; t < NA; ++t) ga[t][k] = Arith<SET>::scale(v[t][i], f.x, f.y, p);
             } else {
 #pragma unroll
-                for (int u = 0; u < NA; ++u) ga[u][k] = v[u][i];
+                for (int t = 0; t < NA; ++t) ga[t][k] = v[t][i];
             }
         }
     }
 }
 
-// Rows of 2^12 points, one per workgroup, go to k_row_reg12 unless
+// Rows of 2^10 .. 2^12 points on the register kernels (k_row_reg) unless
 // PINTT_REG_ROWS=0.
 bool reg_rows_enabled() {
     static const bool on = [] {
@@ -343,15 +398,29 @@ const int kScaledPairMaxLog2N = env_int("PINTT_SCALED_PAIR_MAX", kMaxLog2N);
 
 template <int LC, int LROWS, int K, int SET>
 void launch_rows(const Plan& pl, dim3 grid, const JobList<RowJob>& jobs, const Twiddles& tw, hipStream_t s) {
+    // Rows of 2^10 (four per workgroup) and 2^11 points (two per workgroup):
+    // the register kernels, with the row's own twiddles below the scaling
+    // thresholds.
+    if constexpr ((LC == 10 && LROWS == 2) || (LC == 11 && LROWS == 1)) {
+        if (set_for(pl.log2n) == kSetNarrow && reg_rows_enabled()) {
+            const bool scaled = K == kRowPair ? pl.log2n >= kScaledPairMinLog2N && pl.log2n <= kScaledPairMaxLog2N
+                                              : pl.log2n >= kScaledRowsMinLog2N;
+            if (scaled)
+                hipLaunchKernelGGL((k_row_reg<LC, K, SET, true>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
+            else
+                hipLaunchKernelGGL((k_row_reg<LC, K, SET, false>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
+            return;
+        }
+    }
     if constexpr (LROWS == 0) {
         const bool scaled = K == kRowPair ? pl.log2n >= kScaledPairMinLog2N && pl.log2n <= kScaledPairMaxLog2N
                                           : pl.log2n >= kScaledRowsMinLog2N;
         if constexpr (LC == 12) {
             if (reg_rows_enabled()) {
                 if (scaled)
-                    hipLaunchKernelGGL((k_row_reg12<K, SET, true>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
+                    hipLaunchKernelGGL((k_row_reg<12, K, SET, true>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
                 else
-                    hipLaunchKernelGGL((k_row_reg12<K, SET, false>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
+                    hipLaunchKernelGGL((k_row_reg<12, K, SET, false>), grid, dim3(256), 0, s, jobs, pl.log2r, tw);
                 return;
             }
         }

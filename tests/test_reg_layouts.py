@@ -1,5 +1,5 @@
 """CPU model of the register-resident NTT stages (pi_compute_amd/csrc/ntt_regs.h)
-and of the lane layouts of k_row_reg12, k_col_fwd_reg and k_col_inv_reg.
+and of the lane layouts of k_row_reg (rows of 2^10, 2^11 and 2^12 points), k_col_fwd_reg and k_col_inv_reg.
 
 The model moves element labels through a wave exactly as the two gfx950 lane
 swaps do (v_permlane32_swap: lanes 32..63 of the first register trade with
@@ -22,9 +22,19 @@ def entry_t(i, lane):
     return (i << 2) | (lane >> 4)
 
 
-def post_t(r, lane):
+def post_t(r, lane, sb=1):
+    """Element of post-layout register r: v_permlane32_swap traded register bit
+    3 with lane bit 5, v_permlane16_swap register bit sb with lane bit 4."""
     i3, i2, i1, i0 = (r >> 3) & 1, (r >> 2) & 1, (r >> 1) & 1, r & 1
-    return (((lane >> 5) & 1) << 5) | (i2 << 4) | (((lane >> 4) & 1) << 3) | (i0 << 2) | (i3 << 1) | i1
+    l5, l4 = (lane >> 5) & 1, (lane >> 4) & 1
+    if sb == 1:
+        return (l5 << 5) | (i2 << 4) | (l4 << 3) | (i0 << 2) | (i3 << 1) | i1
+    return (l5 << 5) | (l4 << 4) | (i1 << 3) | (i0 << 2) | (i3 << 1) | i2
+
+
+def post5_t(r, lane):
+    """Five-layer stage: entry t = (i3 i2 i1 i0 L5), post t = (L5 i2 i1 i0 i3)."""
+    return (((lane >> 5) & 1) << 4) | ((r & 7) << 1) | ((r >> 3) & 1)
 
 
 def permlane32_swap(x, y):
@@ -43,14 +53,15 @@ def permlane16_swap(x, y):
     return nx, ny
 
 
-def lane_swaps(v):
+def lane_swaps(v, sb=1, with16=True):
     v = [c.copy() for c in v]
     for i in range(8):
         v[i], v[i | 8] = permlane32_swap(v[i], v[i | 8])
-    for i in range(16):
-        if i & 2:
-            continue
-        v[i], v[i | 2] = permlane16_swap(v[i], v[i | 2])
+    if with16:
+        for i in range(16):
+            if i & (1 << sb):
+                continue
+            v[i], v[i | (1 << sb)] = permlane16_swap(v[i], v[i | (1 << sb)])
     return v
 
 
@@ -67,46 +78,76 @@ def test_swap_model_matches_measured_lane_layout():
 def test_swaps_give_post_layout_and_undo_themselves():
     lanes = np.arange(64)
     v = [np.array([entry_t(i, l) for l in lanes]) for i in range(16)]
-    w = lane_swaps(v)
+    for sb in (1, 2):
+        w = lane_swaps(v, sb)
+        for r in range(16):
+            assert list(w[r]) == [post_t(r, l, sb) for l in lanes]
+        back = lane_swaps(w, sb)
+        for i in range(16):
+            assert list(back[i]) == list(v[i])
+    # Five-layer stage: groups on lanes l and l + 32, entry t = (i L5).
+    v5 = [np.array([(i << 1) | ((l >> 5) & 1) for l in lanes]) for i in range(16)]
+    w5 = lane_swaps(v5, with16=False)
     for r in range(16):
-        assert list(w[r]) == [post_t(r, l) for l in lanes]
-    back = lane_swaps(w)
-    for i in range(16):
-        assert list(back[i]) == list(v[i])
+        assert list(w5[r]) == [post5_t(r, l) for l in lanes]
 
 
-def ct6_pairs():
-    """(register a, register b, layer, block) of every butterfly of ct6, per lane,
-    in the post-swap part evaluated with the lane's own bits."""
+def ct6_pairs(sb):
+    """(layout, register a, register b, layer, block) of every butterfly of
+    ct6<SB = sb>, as the kernel forms them: layers 0..3 pair registers 8 >> l
+    apart (block = the top l register bits); layer 4 pairs r and r | 8 for r
+    with bits 3 and sb clear and their twin r | 2^sb, with the block
+    post_t(r) >> 2; layer 5 pairs r and r | 2^sb with the block post_t(r) >> 1."""
     out = []
     for l in range(4):
         h = 8 >> l
         for sub in range(1 << l):
             for x in range(h):
                 out.append(("entry", sub * 2 * h + x, sub * 2 * h + x + h, l, lambda lane, sub=sub: sub))
-    for i2, i0 in itertools.product((0, 1), repeat=2):
-        for i1 in (0, 1):
-            r = (i2 << 2) | (i1 << 1) | i0
-            out.append(("post", r, r | 8, 4,
-                        lambda lane, i2=i2, i0=i0: (((lane >> 5) & 1) << 3) | (i2 << 2) | (((lane >> 4) & 1) << 1) | i0))
-    for i3, i2, i0 in itertools.product((0, 1), repeat=3):
-        r = (i3 << 3) | (i2 << 2) | i0
-        out.append(("post", r, r | 2, 5,
-                    lambda lane, i3=i3, i2=i2, i0=i0:
-                    (((lane >> 5) & 1) << 4) | (i2 << 3) | (((lane >> 4) & 1) << 2) | (i0 << 1) | i3))
+    for r in range(16):
+        if r & 8 or r & (1 << sb):
+            continue
+        for q in (0, 1):
+            rr = r | (q << sb)
+            out.append(("post", rr, rr | 8, 4, lambda lane, r=r: post_t(r, lane, sb) >> 2))
+    for r in range(16):
+        if r & (1 << sb):
+            continue
+        out.append(("post", r, r | (1 << sb), 5, lambda lane, r=r: post_t(r, lane, sb) >> 1))
     return out
 
 
 def test_ct6_butterflies_follow_the_textbook_network():
     # In-place CT over a 64-element group: layer m pairs t and t + 2^(5-m)
     # (bit 5-m of t clear) with the twiddle of block t >> (6-m).
+    for sb in (1, 2):
+        pairs = ct6_pairs(sb)
+        assert len(pairs) == 6 * 8
+        for lane in range(64):
+            for kind, ra, rb, m, blk in pairs:
+                ta = entry_t(ra, lane) if kind == "entry" else post_t(ra, lane, sb)
+                tb = entry_t(rb, lane) if kind == "entry" else post_t(rb, lane, sb)
+                d = 1 << (5 - m)
+                assert ta & d == 0 and tb == ta + d, (sb, lane, ra, rb, m)
+                assert blk(lane) == ta >> (6 - m), (sb, lane, ra, m)
+
+
+def test_ct5_butterflies_follow_the_textbook_network():
+    # 32-element groups: layers 0..3 on registers (entry t = (i L5)), layer 4
+    # pairs r and r | 8 after the swap with the block (L5 i2 i1 i0).
     for lane in range(64):
-        for kind, ra, rb, m, blk in ct6_pairs():
-            f = entry_t if kind == "entry" else post_t
-            ta, tb = f(ra, lane), f(rb, lane)
-            d = 1 << (5 - m)
-            assert ta & d == 0 and tb == ta + d, (lane, ra, rb, m)
-            assert blk(lane) == ta >> (6 - m), (lane, ra, m)
+        l5 = (lane >> 5) & 1
+        for l in range(4):
+            h = 8 >> l
+            for sub in range(1 << l):
+                for x in range(h):
+                    ra, rb = sub * 2 * h + x, sub * 2 * h + x + h
+                    ta, tb = (ra << 1) | l5, (rb << 1) | l5
+                    d = 1 << (4 - l)
+                    assert ta & d == 0 and tb == ta + d and ta >> (5 - l) == sub
+        for r in range(8):
+            ta, tb = post5_t(r, lane), post5_t(r | 8, lane)
+            assert ta & 1 == 0 and tb == ta + 1 and ta >> 1 == (l5 << 3) | r
 
 
 def _banks_ok(addrs):
@@ -117,40 +158,74 @@ def _banks_ok(addrs):
     return True
 
 
-def test_row_reg12_layouts():
-    L = 4096
-    lds = lambda k: k + 2 * (k >> 6)
-    seen = {name: [] for name in ("a_entry", "a_post", "b_entry", "b_post", "rho")}
-    for wave in range(4):
-        for reg in range(16):
-            acc = {name: [] for name in seen}
-            for lane in range(64):
-                j = (wave << 4) | (lane & 15)
-                lanepos = ((lane >> 4) << 6) | j
-                acc["a_entry"].append((reg << 8) | lanepos)
-                acc["a_post"].append((post_t(reg, lane) << 6) | j)
-                acc["b_entry"].append((j << 6) | (reg << 2) | (lane >> 4))
-                acc["b_post"].append((j << 6) | post_t(reg, lane))
-                acc["rho"].append((reg << 8) | lanepos)
-            for name in seen:
-                seen[name] += acc[name]
-            # LDS: stage A post (write/read) and stage B entry (read/write).
-            assert _banks_ok([lds(k) for k in acc["a_post"]])
-            assert _banks_ok([lds(k) for k in acc["b_entry"]])
-            # Global: stage A entry and register-order b rows, 16-word runs.
-            for name in ("a_entry", "rho"):
-                ks = acc[name]
-                for g in range(4):
-                    run = ks[16 * g:16 * g + 16]
-                    assert run == list(range(run[0], run[0] + 16))
-    for name, ks in seen.items():
-        assert sorted(ks) == list(range(L)), name
-    # Stage A groups are positions j + 64 t; stage B groups 64 u + t'.
-    for wave, lane, reg in itertools.product(range(4), range(64), range(16)):
-        j = (wave << 4) | (lane & 15)
-        assert ((reg << 8) | ((lane >> 4) << 6) | j) == (entry_t(reg, lane) << 6) | j
-        assert ((j << 6) | (reg << 2) | (lane >> 4)) == (j << 6) | entry_t(reg, lane)
-    assert max(lds(k) for k in range(L)) < L + 2 * (L >> 6)
+ROW_GEOMETRY = {  # LC: (swap-B register bit, LDS pad shift, pad words) as in RegRows
+    10: (2, 4, 1),
+    11: (2, 5, 1),
+    12: (1, 6, 2),
+}
+
+
+def _row_positions(LC, ell, reg):
+    """Positions of register `reg` of row-local thread ell in each layout of
+    k_row_reg<LC> (RegRows in ntt_rows.hip)."""
+    sb = ROW_GEOMETRY[LC][0]
+    GA = LC - 6
+    lane, wr = ell & 63, ell >> 6
+    j = (wr << 4) | (lane & 15)
+    lanepos = ((lane >> 4) << GA) | j
+    if LC == 12:
+        u, tb = j, (reg << 2) | (lane >> 4)
+    elif LC == 11:
+        u, tb = (wr << 5) | (lane & 31), (reg << 1) | (lane >> 5)
+    else:
+        u, tb = lane, reg
+    if LC == 12:
+        tpost = post_t(reg, lane, 1)
+    elif LC == 11:
+        tpost = post5_t(reg, lane)
+    else:
+        tpost = reg
+    return {"a_entry": (reg << (LC - 4)) | lanepos, "a_post": (post_t(reg, lane, sb) << GA) | j,
+            "b_entry": (u << GA) | tb, "b_post": (u << GA) | tpost, "rho": (reg << (LC - 4)) | ell,
+            "a_group": j, "b_group": u}
+
+
+def test_row_reg_layouts():
+    for LC, (sb, psh, pw) in ROW_GEOMETRY.items():
+        L = 1 << LC
+        lds = lambda k: k + pw * (k >> psh)
+        names = ("a_entry", "a_post", "b_entry", "b_post", "rho")
+        seen = {name: [] for name in names}
+        for wr in range(L // 16 // 64):  # waves of one row
+            for reg in range(16):
+                acc = {name: [] for name in names}
+                for lane in range(64):
+                    pos = _row_positions(LC, (wr << 6) | lane, reg)
+                    for name in names:
+                        acc[name].append(pos[name])
+                for name in names:
+                    seen[name] += acc[name]
+                # LDS: stage A post (write/read) and stage B entry (read/write).
+                assert _banks_ok([lds(k) for k in acc["a_post"]]), (LC, "stage A post")
+                assert _banks_ok([lds(k) for k in acc["b_entry"]]), (LC, "stage B entry")
+                # Global: stage A entry and register-order b rows, 16-word runs.
+                for name in ("a_entry", "rho"):
+                    ks = acc[name]
+                    for g in range(4):
+                        run = ks[16 * g:16 * g + 16]
+                        assert run == list(range(run[0], run[0] + 16)), (LC, name)
+        for name, ks in seen.items():
+            assert sorted(ks) == list(range(L)), (LC, name)
+        # Groups: stage A's are the positions j + 2^(LC-6) t (t = the six layers'
+        # bits), stage B's the 2^(LC-6) positions 2^(LC-6) u + t'.
+        GA = LC - 6
+        for ell, reg in itertools.product(range(L // 16), range(16)):
+            pos = _row_positions(LC, ell, reg)
+            for key in ("a_entry", "a_post"):
+                assert pos[key] & ((1 << GA) - 1) == pos["a_group"]
+            for key in ("b_entry", "b_post"):
+                assert pos[key] >> GA == pos["b_group"]
+        assert max(lds(k) for k in range(L)) < L + pw * (L >> psh)
 
 
 def _col_model(LR):
