@@ -83,25 +83,28 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 }
 
 // Column passes with register-resident stages (ntt_regs.h) for tiles of 2^LR
-// rows (LR = 9, 10) by 8 columns, 2^(LR - 1) threads.  Row r of column w:
+// rows (LR = 9, 10, 11) by 8 columns, 2^(LR - 1) threads.  Row r of column w:
 // stage A runs layers 0..5 on r's top six bits for the groups g = (r_low, w)
 // (r_low = r's low LR - 6 bits; g = (wave << 4) | (lane & 15), w = g & 7);
-// stage B runs the last LR - 6 layers inside each lane on the groups
-// (r >> (LR - 6), w).  Global memory is read and written in the stage A entry
-// layout (forward) or the stage B layout (inverse) as whole 32-byte column
-// rows; LDS (word (r << 3) | w, padded by S words per 64) carries the one
-// exchange between the stages, at 32 distinct banks per half-wave both ways:
-// S = 1 for 2^10 rows, 2 for 2^9, and the stage B lane bits chosen per shape.
+// stage B runs the last LR - 6 layers on the groups (r >> (LR - 6), w): inside
+// each lane for 3 or 4 layers, as a five-layer lane-swap stage (lanes l and
+// l + 32, row bit 0 on lane bit 5) for LR = 11.  Global memory is read and
+// written in the stage A entry layout (forward) or the stage B layout
+// (inverse) as whole 32-byte column rows; LDS (word k = (r << 3) | w, padded
+// by S words per 2^PSH) carries the one exchange between the stages, at 32
+// distinct banks per half-wave both ways (padding and the lane bits of the
+// stage B groups chosen per shape; tests/test_reg_layouts.py checks them).
 template <int LR>
 struct RegCols {
-    static_assert(LR == 9 || LR == 10, "register column tiles: 2^9 or 2^10 rows");
+    static_assert(LR >= 9 && LR <= 11, "register column tiles: 2^9 .. 2^11 rows");
     static constexpr int NT = 1 << (LR - 1);
     static constexpr int KB = LR - 6;  // stage B layers
-    static constexpr uint32_t S = LR == 10 ? 1u : 2u;
-    static constexpr int WORDS = (8 << LR) + (int)S * ((8 << LR) >> 6);
+    static constexpr uint32_t S = LR == 9 ? 2u : 1u;
+    static constexpr int PSH = LR == 11 ? 7 : 6;
+    static constexpr int WORDS = (8 << LR) + (int)S * ((8 << LR) >> PSH);
     __device__ static __forceinline__ uint32_t lds_at(uint32_t r, uint32_t w) {
         const uint32_t k = (r << 3) | w;
-        return k + S * (k >> 6);
+        return k + S * (k >> PSH);
     }
     // Stage A: row of entry-layout register i / post-layout register i.
     __device__ static __forceinline__ uint32_t row_entry(int i, uint32_t lane, uint32_t rlow) {
@@ -110,31 +113,58 @@ struct RegCols {
     __device__ static __forceinline__ uint32_t row_post(int i, uint32_t lane, uint32_t rlow) {
         return (post_t(i, lane) << (LR - 6)) | rlow;
     }
-    // Stage B: the lane's rows are (hi << 4) | i; hi from the lane and wave.
+    // Stage B: the lane's group is hi = r >> 4 (LR = 9, 10; two groups per
+    // lane for LR = 9, told apart by register bit 3) or r >> 5 (LR = 11).
     __device__ static __forceinline__ uint32_t hi_bits(uint32_t lane, uint32_t wave) {
         const uint32_t l3 = (lane >> 3) & 1u, l4 = (lane >> 4) & 1u, l5 = (lane >> 5) & 1u;
-        if constexpr (LR == 10)  // r4 = L5, r5 = wave0, r6 = L3, r7 = L4, r8, r9 = wave1, wave2
+        if constexpr (LR == 11)  // r5, r6 = wave0, wave1; r7 = L3; r8 = L4; r9, r10 = wave2, wave3 (r0 = L5)
+            return (wave & 3u) | (l3 << 2) | (l4 << 3) | ((wave >> 2) << 4);
+        else if constexpr (LR == 10)  // r4 = L5, r5 = wave0, r6 = L3, r7 = L4, r8, r9 = wave1, wave2
             return l5 | ((wave & 1u) << 1) | (l3 << 2) | (l4 << 3) | ((wave >> 1) << 4);
         else  // r4 = L5, r5 = L3, r6 = L4, r7, r8 = wave0, wave1 (r3 = register bit 3)
             return l5 | (l3 << 1) | (l4 << 2) | (wave << 3);
     }
+    // Stage B row of register i, in the entry layout and (LR = 11) the post one.
+    __device__ static __forceinline__ uint32_t row_b(int i, uint32_t lane, uint32_t hi) {
+        if constexpr (LR == 11) return (hi << 5) | ((uint32_t)i << 1) | ((lane >> 5) & 1u);
+        else return (hi << 4) | (uint32_t)i;
+    }
+    __device__ static __forceinline__ uint32_t row_b_post(int i, uint32_t lane, uint32_t hi) {
+        if constexpr (LR == 11)
+            return (hi << 5) | (((lane >> 5) & 1u) << 4) | (((uint32_t)i & 7u) << 1) | ((uint32_t)i >> 3);
+        else return (hi << 4) | (uint32_t)i;
+    }
 };
 
-// Stage B twiddles: group g of the lane (register bit 3 when KB = 3) at layer
-// 6 + l, block ((hi << (4 - KB)) | g) << l | blk.
+// Stage B twiddles: at layer 6 + l, block (outer << l) | blk with outer the
+// group's block at layer 6: hi (LR = 10, 11) or (hi << 1) | g (LR = 9, g =
+// register bit 3).
 struct TwColB {
     ConstTable z;
     uint32_t hi;
     int KB;
     __device__ __forceinline__ uint2 operator()(int g, int l, uint32_t blk) const {
-        const uint32_t outer = KB == 4 ? hi : ((hi << 1) | (uint32_t)g);
+        const uint32_t outer = KB == 3 ? ((hi << 1) | (uint32_t)g) : hi;
         return tw_at(z, (1u << (6 + l)) + (outer << l) + blk);
     }
+    __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const { return (*this)(0, l, blk); }
 };
 struct TwColA {
     ConstTable z;
     __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const { return tw_at(z, (1u << l) + blk); }
 };
+
+// Stage B of a register column kernel on the lane's 16 residues.
+template <int LR, int SET, bool INV>
+__device__ __forceinline__ void col_stage_b(uint32_t (&v)[1][16], const TwColB& tw, uint32_t p) {
+    if constexpr (LR == 11) {
+        if constexpr (INV) gs5<SET, 1>(v, tw, p);
+        else ct5<SET, 1>(v, tw, p);
+    } else {
+        if constexpr (INV) gs_lane<SET, LR - 6>(v[0], tw, p);
+        else ct_lane<SET, LR - 6>(v[0], tw, p);
+    }
+}
 
 template <int LR, int SET>
 __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_fwd_reg(const JobList<FwdJob> jobs, int log2c, Twiddles tw) {
@@ -173,10 +203,10 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_fwd_reg(const JobList<F
     __syncthreads();
     const uint32_t hi = RC::hi_bits(lane, wave), wb = lane & 7u;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at((hi << 4) | (uint32_t)i, wb)];
-    ct_lane<SET, RC::KB>(v[0], TwColB{as_const(z), hi, RC::KB}, p);
+    for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at(RC::row_b(i, lane, hi), wb)];
+    col_stage_b<LR, SET, false>(v, TwColB{as_const(z), hi, RC::KB}, p);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) dst[((((hi << 4) | (uint32_t)i)) << log2c) + c0 + wb] = v[0][i];
+    for (int i = 0; i < 16; ++i) dst[(RC::row_b_post(i, lane, hi) << log2c) + c0 + wb] = v[0][i];
 }
 
 template <int LR, int SET>
@@ -193,10 +223,10 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<I
     const uint32_t hi = RC::hi_bits(lane, wave), wb = lane & 7u;
     uint32_t v[1][16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[0][i] = buf[((((hi << 4) | (uint32_t)i)) << log2c) + c0 + wb];
-    gs_lane<SET, RC::KB>(v[0], TwColB{as_const(z), hi, RC::KB}, p);
+    for (int i = 0; i < 16; ++i) v[0][i] = buf[(RC::row_b_post(i, lane, hi) << log2c) + c0 + wb];
+    col_stage_b<LR, SET, true>(v, TwColB{as_const(z), hi, RC::KB}, p);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) lds[RC::lds_at((hi << 4) | (uint32_t)i, wb)] = v[0][i];
+    for (int i = 0; i < 16; ++i) lds[RC::lds_at(RC::row_b(i, lane, hi), wb)] = v[0][i];
     __syncthreads();
     const uint32_t g = (wave << 4) | (lane & 15u), w = g & 7u, rlow = g >> 3;
 #pragma unroll
@@ -206,7 +236,7 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<I
     for (int i = 0; i < 16; ++i) buf[(RC::row_entry(i, lane, rlow) << log2c) + c0 + w] = v[0][i];
 }
 
-// Column tiles of 2^9 and 2^10 rows by 8 columns go to the register kernels
+// Column tiles of 2^9 .. 2^11 rows by 8 columns go to the register kernels
 // unless PINTT_REG_COLS=0.
 bool reg_cols_enabled() {
     static const bool on = [] {
@@ -238,15 +268,22 @@ bool col_case_available(int set, int log2r, int log2w) {
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
-    if (set_for(pl.log2n) == kSetNarrow && pl.log2w == 3 && reg_cols_enabled()) {
-        if (pl.log2r == 9) {
-            hipLaunchKernelGGL((k_col_fwd_reg<9, kSetNarrow>), grid, dim3(RegCols<9>::NT), 0, s, jobs, pl.log2c, tw);
-            return hipGetLastError();
+    if (pl.log2w == 3 && pl.log2r >= 9 && pl.log2r <= 11 && reg_cols_enabled()) {
+#define PINTT_REG_COL(LR, SET)                                                                                   \
+    if (pl.log2r == LR) {                                                                                         \
+        hipLaunchKernelGGL((k_col_fwd_reg<LR, SET>), grid, dim3(RegCols<LR>::NT), 0, s, jobs, pl.log2c, tw); \
+        return hipGetLastError();                                                                                \
+    }
+        if (set_for(pl.log2n) == kSetNarrow) {
+            PINTT_REG_COL(9, kSetNarrow)
+            PINTT_REG_COL(10, kSetNarrow)
+            PINTT_REG_COL(11, kSetNarrow)
+        } else {
+            PINTT_REG_COL(9, kSetWide)
+            PINTT_REG_COL(10, kSetWide)
+            PINTT_REG_COL(11, kSetWide)
         }
-        if (pl.log2r == 10) {
-            hipLaunchKernelGGL((k_col_fwd_reg<10, kSetNarrow>), grid, dim3(RegCols<10>::NT), 0, s, jobs, pl.log2c, tw);
-            return hipGetLastError();
-        }
+#undef PINTT_REG_COL
     }
 #define PINTT_CASE_SET(LR, LW, SET)                                                                  \
     if (pl.log2r == LR && pl.log2w == LW) {                                                          \
@@ -270,15 +307,22 @@ hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, u
 hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
-    if (set_for(pl.log2n) == kSetNarrow && pl.log2w == 3 && reg_cols_enabled()) {
-        if (pl.log2r == 9) {
-            hipLaunchKernelGGL((k_col_inv_reg<9, kSetNarrow>), grid, dim3(RegCols<9>::NT), 0, s, jobs, pl.log2c, tw);
-            return hipGetLastError();
+    if (pl.log2w == 3 && pl.log2r >= 9 && pl.log2r <= 11 && reg_cols_enabled()) {
+#define PINTT_REG_COL(LR, SET)                                                                                   \
+    if (pl.log2r == LR) {                                                                                         \
+        hipLaunchKernelGGL((k_col_inv_reg<LR, SET>), grid, dim3(RegCols<LR>::NT), 0, s, jobs, pl.log2c, tw); \
+        return hipGetLastError();                                                                                \
+    }
+        if (set_for(pl.log2n) == kSetNarrow) {
+            PINTT_REG_COL(9, kSetNarrow)
+            PINTT_REG_COL(10, kSetNarrow)
+            PINTT_REG_COL(11, kSetNarrow)
+        } else {
+            PINTT_REG_COL(9, kSetWide)
+            PINTT_REG_COL(10, kSetWide)
+            PINTT_REG_COL(11, kSetWide)
         }
-        if (pl.log2r == 10) {
-            hipLaunchKernelGGL((k_col_inv_reg<10, kSetNarrow>), grid, dim3(RegCols<10>::NT), 0, s, jobs, pl.log2c, tw);
-            return hipGetLastError();
-        }
+#undef PINTT_REG_COL
     }
 #define PINTT_CASE_SET(LR, LW, SET)                                                                  \
     if (pl.log2r == LR && pl.log2w == LW) {                                                          \

@@ -229,51 +229,66 @@ def test_row_reg_layouts():
 
 
 def _col_model(LR):
-    S = 1 if LR == 10 else 2
+    """LDS address, stage B group bits (hi) and stage B rows of the register
+    column kernels (RegCols in ntt_cols.hip)."""
+    S, PSH = {9: (2, 6), 10: (1, 6), 11: (1, 7)}[LR]
     NT = 1 << (LR - 1)
-    lds = lambda r, w: ((r << 3) | w) + S * (((r << 3) | w) >> 6)
+    lds = lambda r, w: ((r << 3) | w) + S * (((r << 3) | w) >> PSH)
 
     def hi_bits(lane, wave):
         l3, l4, l5 = (lane >> 3) & 1, (lane >> 4) & 1, (lane >> 5) & 1
+        if LR == 11:
+            return (wave & 3) | (l3 << 2) | (l4 << 3) | ((wave >> 2) << 4)
         if LR == 10:
             return l5 | ((wave & 1) << 1) | (l3 << 2) | (l4 << 3) | ((wave >> 1) << 4)
         return l5 | (l3 << 1) | (l4 << 2) | (wave << 3)
 
-    return S, NT, lds, hi_bits
+    def row_b(i, lane, hi, post=False):
+        if LR == 11:
+            return (hi << 5) | (post5_t(i, lane) if post else (i << 1) | ((lane >> 5) & 1))
+        return (hi << 4) | i
+
+    return S, NT, lds, hi_bits, row_b
 
 
 def test_col_reg_layouts():
-    for LR in (9, 10):
-        S, NT, lds, hi_bits = _col_model(LR)
+    for LR in (9, 10, 11):
+        S, NT, lds, hi_bits, row_b = _col_model(LR)
         R = 1 << LR
-        a_entry, a_post, b_rows = [], [], []
+        a_entry, a_post, b_rows, b_post = [], [], [], []
         for wave in range(NT // 64):
             for reg in range(16):
-                ae, ap, br = [], [], []
+                ae, ap, br, bp = [], [], [], []
                 for lane in range(64):
                     g = (wave << 4) | (lane & 15)
                     w, rlow = g & 7, g >> 3
                     ae.append(((entry_t(reg, lane) << (LR - 6)) | rlow, w))
                     ap.append(((post_t(reg, lane) << (LR - 6)) | rlow, w))
-                    br.append(((hi_bits(lane, wave) << 4) | reg, lane & 7))
+                    hi = hi_bits(lane, wave)
+                    br.append((row_b(reg, lane, hi), lane & 7))
+                    bp.append((row_b(reg, lane, hi, True), lane & 7))
                 assert _banks_ok([lds(r, w) for r, w in ap]), (LR, "stage A post")
                 assert _banks_ok([lds(r, w) for r, w in br]), (LR, "stage B")
                 # Global rows: 8 consecutive columns of one row per 8 lanes.
-                for seq in (ae, br):
+                for seq in (ae, bp):
                     for g8 in range(8):
                         run = seq[8 * g8:8 * g8 + 8]
                         assert len({r for r, _ in run}) == 1 and [w for _, w in run] == list(range(8))
                 a_entry += ae
                 a_post += ap
                 b_rows += br
+                b_post += bp
         full = sorted((r, w) for r in range(R) for w in range(8))
-        for name, seq in (("a_entry", a_entry), ("a_post", a_post), ("b", b_rows)):
+        for name, seq in (("a_entry", a_entry), ("a_post", a_post), ("b", b_rows), ("b_post", b_post)):
             assert sorted(seq) == full, (LR, name)
-        # Stage B groups: a lane's registers are consecutive rows of one column,
-        # 16 rows (LR = 10) or two runs of 8 (LR = 9, register bit 3 = row bit 3).
-        for wave, lane in itertools.product(range(NT // 64), range(64)):
-            rows = [(hi_bits(lane, wave) << 4) | reg for reg in range(16)]
-            assert rows == list(range(rows[0], rows[0] + 16))
+        # Stage B groups: the rows sharing r >> (LR - 6) in one column, 2^(LR-6)
+        # of them, on one lane (LR = 10), half a lane (LR = 9, register bit 3)
+        # or lanes l and l + 32 (LR = 11).
+        groups = {}
+        for wave, lane, reg in itertools.product(range(NT // 64), range(64), range(16)):
+            r = row_b(reg, lane, hi_bits(lane, wave))
+            groups.setdefault((r >> (LR - 6), lane & 7), set()).add(r)
+        assert len(groups) == 64 * 8 and all(len(v) == 1 << (LR - 6) for v in groups.values())
         # Stage A groups: the 64 rows sharing r's low LR - 6 bits, in column w.
         for wave, lane in itertools.product(range(NT // 64), range(64)):
             g = (wave << 4) | (lane & 15)
