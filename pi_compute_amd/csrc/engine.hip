@@ -579,22 +579,36 @@ struct BatchPlan {
 
 // Products whose operands both fit the schoolbook kernel run there (unless
 // the tuner is timing transforms).
+size_t school_max_limbs() {
+    static const size_t m = [] {
+        const char* e = std::getenv("PINTT_SCHOOL_MAX");
+        const long v = e ? std::atol(e) : kSchoolMaxLimbs;
+        return (size_t)std::max(0L, std::min(v, (long)kSchoolMaxLimbs));
+    }();
+    return m;
+}
 bool school_eligible(const Request& r) {
-    return g.school && r.an <= (size_t)kSchoolMaxLimbs && r.bn <= (size_t)kSchoolMaxLimbs;
+    return g.school && r.an <= school_max_limbs() && r.bn <= school_max_limbs();
 }
 
 // Groups, buckets, descriptor layout and resource sizes of a batch.  May tune
 // tile plans for lengths seen for the first time (which waits on the device).
 int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp, const PlanMode& mode = PlanMode{}) {
     bp->groups.resize(reqs.size());
-    for (size_t i = 0; i < reqs.size(); ++i) PINTT_TRY(plan_group(reqs[i], &bp->groups[i], mode));
+    // Schoolbook products need no transform plan: only their request is kept
+    // (a tree level's first batches hold tens of thousands of them).
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        if (school_eligible(reqs[i])) {
+            bp->groups[i].r = reqs[i];
+            bp->school.push_back(i);
+        } else {
+            PINTT_TRY(plan_group(reqs[i], &bp->groups[i], mode));
+        }
+    }
     const std::vector<Group>& groups = bp->groups;
     std::vector<BucketLayout>& buckets = bp->buckets;
     for (size_t i = 0; i < groups.size(); ++i) {
-        if (school_eligible(reqs[i])) {
-            bp->school.push_back(i);
-            continue;
-        }
+        if (school_eligible(reqs[i])) continue;
         auto it = std::find_if(buckets.begin(), buckets.end(),
                                [&](const BucketLayout& b) { return b.log2n == groups[i].log2n; });
         if (it == buckets.end()) {

@@ -289,85 +289,51 @@ __global__ void __launch_bounds__(CT)
     }
 }
 
-// The carry maps and first block of job z, for a lane's own z (a select chain
-// over the inline jobs: indexing the kernel argument with a per-lane value
-// would copy it to scratch).
-__device__ __forceinline__ void job_maps(const JobList<CarryJob>& jobs, uint32_t z, const uint32_t*& maps,
-                                         uint32_t& blk0) {
-    if (jobs.ninl) {
-        maps = jobs.inl[0].maps;
-        blk0 = jobs.inl[0].blk0;
-#pragma unroll
-        for (int i = 1; i < kInlineJobs; ++i)
-            if (z == (uint32_t)i) {
-                maps = jobs.inl[i].maps;
-                blk0 = jobs.inl[i].blk0;
-            }
-    } else {
-        maps = jobs.dev[z].maps;
-        blk0 = jobs.dev[z].blk0;
-    }
-}
-
-// One lane per block of the first kernel, 64 blocks per wave: the wave takes
-// the flagged blocks among its 64 one at a time (almost always none) and
-// ripples each one's true carry-in into its words (blocks of
-// carry_block_words(CT) words).
+// One wave per block of the first kernel: a flagged block's true carry-in,
+// rippled into its words (blocks of carry_block_words(CT) words).
 template <int CT>
-__global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs, uint32_t njobs, uint32_t nblocks) {
+__global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs, uint32_t njobs) {
     constexpr int kWB = carry_block_words(CT);
     const uint32_t lane = threadIdx.x;
-    bool mine = false;
-    {
-        const uint32_t blk = blockIdx.x * 64u + lane;
-        if (blk < nblocks) {
-            const uint32_t* mp;
-            uint32_t blk0;
-            job_maps(jobs, find_job(jobs, njobs, blk), mp, blk0);
-            const uint32_t lb = blk - blk0;
-            mine = lb != 0 && (as_global(mp)[lb] & kFnUnresolved);  // else its window fixed its carry-in
+    const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
+    const uint32_t lb = blockIdx.x - job.blk0;
+    if (lb == 0) return;
+    const global_ptr<const uint32_t> maps = as_global(job.maps);
+    if (!(maps[lb] & kFnUnresolved)) return;  // its window fixed its carry-in
+    // Maps of blocks 0 .. lb-1 in order, 1024 per round: 16 consecutive
+    // maps per lane, all loaded before any is composed.
+    uint32_t total = kFnIdentity;  // composition of the maps of the rounds so far
+    for (uint32_t base = 0; base < lb; base += 1024) {
+        const uint32_t lo = base + 16 * lane;
+        uint32_t m[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[i] = lo + i < lb ? maps[lo + i] : kFnIdentity;
+        uint32_t acc = kFnIdentity;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc = fn_compose(m[i], acc);
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {  // lane 0 ends with lane 63's o ... o lane 0's
+            const uint32_t other = __shfl_down(acc, off, 64);
+            if (lane + off < 64) acc = fn_compose(other, acc);
         }
+        total = fn_compose(__shfl(acc, 0, 64), total);
     }
-    for (uint64_t todo = __ballot(mine); todo != 0; todo &= todo - 1) {
-        const uint32_t blk = blockIdx.x * 64u + (uint32_t)__builtin_ctzll(todo);
-        const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blk));
-        const uint32_t lb = blk - job.blk0;
-        const global_ptr<const uint32_t> maps = as_global(job.maps);
-        // Maps of blocks 0 .. lb-1 in order, 1024 per round: 16 consecutive
-        // maps per lane, all loaded before any is composed.
-        uint32_t total = kFnIdentity;  // composition of the maps of the rounds so far
-        for (uint32_t base = 0; base < lb; base += 1024) {
-            const uint32_t lo = base + 16 * lane;
-            uint32_t m[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) m[i] = lo + i < lb ? maps[lo + i] : kFnIdentity;
-            uint32_t acc = kFnIdentity;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) acc = fn_compose(m[i], acc);
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {  // lane 0 ends with lane 63's o ... o lane 0's
-                const uint32_t other = __shfl_down(acc, off, 64);
-                if (lane + off < 64) acc = fn_compose(other, acc);
-            }
-            total = fn_compose(__shfl(acc, 0, 64), total);
-        }
-        uint32_t c = fn_apply(total, 0);
-        const global_ptr<uint32_t> out = as_global(job.out);
-        const uint64_t j0 = (uint64_t)lb * kWB;
-        const uint64_t jend = min(j0 + (uint64_t)kWB, job.nwords);
-        for (uint64_t j = j0; c != 0 && j < jend; j += 64) {
-            const uint64_t jw = j + lane;
-            const bool in = jw < jend;
-            const uint32_t w = in ? out[jw] : 0u;
-            // Word j takes c; a carry out of it runs through the all-ones words after it.
-            const uint32_t w0 = __shfl(w, 0, 64) + c;
-            const bool over = w0 < c;
-            const uint64_t stop = __ballot(in && lane > 0 && w != 0xFFFFFFFFu);
-            const uint32_t first = stop ? (uint32_t)__builtin_ctzll(stop) : 64u;  // first word the carry ends in
-            if (lane == 0) out[jw] = w0;
-            if (over && in && lane > 0 && lane <= first) out[jw] = lane < first ? 0u : w + 1u;
-            c = (over && first == 64u) ? 1u : 0u;
-        }
+    uint32_t c = fn_apply(total, 0);
+    const global_ptr<uint32_t> out = as_global(job.out);
+    const uint64_t j0 = (uint64_t)lb * kWB;
+    const uint64_t jend = min(j0 + (uint64_t)kWB, job.nwords);
+    for (uint64_t j = j0; c != 0 && j < jend; j += 64) {
+        const uint64_t jw = j + lane;
+        const bool in = jw < jend;
+        const uint32_t w = in ? out[jw] : 0u;
+        // Word j takes c; a carry out of it runs through the all-ones words after it.
+        const uint32_t w0 = __shfl(w, 0, 64) + c;
+        const bool over = w0 < c;
+        const uint64_t stop = __ballot(in && lane > 0 && w != 0xFFFFFFFFu);
+        const uint32_t first = stop ? (uint32_t)__builtin_ctzll(stop) : 64u;  // first word the carry ends in
+        if (lane == 0) out[jw] = w0;
+        if (over && in && lane > 0 && lane <= first) out[jw] = lane < first ? 0u : w + 1u;
+        c = (over && first == 64u) ? 1u : 0u;
     }
 }
 
@@ -480,7 +446,7 @@ hipError_t launch_crt_carry_ct(const JobList<CarryJob>& jobs, uint32_t njobs, ui
         hipLaunchKernelGGL((k_crt_carry<kSetWide, CT>), dim3(nblocks), dim3(CT), 0, s, jobs, njobs, log2n, cc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_carry_fixup<CT>, dim3((nblocks + 63) / 64), dim3(64), 0, s, jobs, njobs, nblocks);
+    hipLaunchKernelGGL(k_carry_fixup<CT>, dim3(nblocks), dim3(64), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 
