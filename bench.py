@@ -659,7 +659,7 @@ def main() -> int:
         "config": {"workload": "exact multiply 2^26 x floor(2^26/3) bits",
                    "limbs": [an, bn],
                    "transform": f"{pp['chunks']} chunk product(s) at N=2^{pp['log2n']}, "
-                                f"3 primes < 2^{30 if pp['log2n'] <= 22 else 31}",
+                                f"3 primes < 2^{29 if pp['log2n'] <= 22 else 31}",
                    "tile_plans": plans, "product_plan": pp, "parallelism": "single GPU"},
         "roofline": roof,
         "product_roofline": {"algorithmic_bytes": b_alg,

@@ -21,10 +21,10 @@
 #include <string.h>
 
 /* Prime index i: set i / 3, prime i % 3.  Set 0 (wide): transforms up to 2^25;
- * set 1 (narrow, every p < 2^30): transforms up to 2^22. */
-static const uint32_t P[6] = {2013265921u, 1811939329u, 2113929217u, 998244353u, 985661441u, 943718401u};
-static const uint32_t G[6] = {31u, 13u, 5u, 3u, 3u, 7u};
-static const int TWO_ADIC[6] = {27, 26, 25, 23, 22, 22};
+ * set 1 (narrow, every p < 2^29, so 8p < 2^32): transforms up to 2^22. */
+static const uint32_t P[6] = {2013265921u, 1811939329u, 2113929217u, 469762049u, 415236097u, 377487361u};
+static const uint32_t G[6] = {31u, 13u, 5u, 3u, 5u, 7u};
+static const int TWO_ADIC[6] = {27, 26, 25, 26, 22, 23};
 
 int orc_set_for(int log2n) { return log2n <= ORC_NARROW_MAX_LOG2N ? 1 : 0; }
 
@@ -159,16 +159,22 @@ uint32_t orc_shoup_quotient(uint32_t w, uint32_t p) { return (uint32_t)(((uint64
 
 static uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
-uint32_t orc_narrow_load(uint32_t c, uint32_t p) { return min_u32(c, c - 4u * p); }
+uint32_t orc_narrow_load(uint32_t c, uint32_t p) { return min_u32(c, c - 8u * p); }
+
+/* x plus and minus Shoup's w b (-(w b - q p) mod 2^32 kept negated). */
+static void narrow_ct_core(uint32_t x, uint32_t* a, uint32_t* b, uint32_t w, uint32_t p) {
+    const uint32_t wq = orc_shoup_quotient(w, p);
+    const uint32_t q = (uint32_t)(((uint64_t)*b * wq) >> 32);
+    const uint32_t tn = q * p - *b * w;
+    *a = x - tn;
+    *b = x + tn + 2u * p;
+}
 
 void orc_narrow_ct(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p) {
-    const uint32_t p2 = 2u * p, wq = orc_shoup_quotient(w, p);
-    const uint32_t x = min_u32(*a, *a - p2);
-    const uint32_t q = (uint32_t)(((uint64_t)*b * wq) >> 32);
-    const uint32_t tn = q * p - *b * w; /* -(w b - q p) mod 2^32 */
-    *a = x - tn;
-    *b = x + tn + p2;
+    narrow_ct_core(min_u32(*a, *a - 4u * p), a, b, w, p);
 }
+
+void orc_narrow_ct_lazy(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p) { narrow_ct_core(*a, a, b, w, p); }
 
 void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p) {
     const uint32_t p2 = 2u * p, wq = orc_shoup_quotient(winv, p);
@@ -181,11 +187,13 @@ void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p) {
 }
 
 uint32_t orc_narrow_pointwise(uint32_t x, uint32_t y, uint32_t p) {
-    const uint32_t p2 = 2u * p;
+    const uint32_t p2 = 2u * p, p4 = 4u * p;
     uint32_t pinv = p; /* -p^-1 mod 2^32 by Newton's iteration */
     for (int i = 0; i < 5; ++i) pinv *= 2u - p * pinv;
     pinv = 0u - pinv;
+    x = min_u32(x, x - p4);
     x = min_u32(x, x - p2);
+    y = min_u32(y, y - p4);
     y = min_u32(y, y - p2);
     const uint64_t t = (uint64_t)x * y;
     const uint32_t m = (uint32_t)t * pinv;
@@ -197,11 +205,18 @@ void orc_narrow_ct_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi)
     size_t n = (size_t)1 << log2n;
     uint32_t* z = (uint32_t*)malloc(n * sizeof(uint32_t));
     orc_zeta_table(prime, log2n, z, NULL);
+    int bound = 8; /* inputs below 8p; a layer skips the reduction while every value is below 6p */
     for (int s = s_lo; s < s_hi; ++s) {
         size_t half = n >> (s + 1);
+        const int lazy = bound <= 6;
         for (size_t b = 0; b < ((size_t)1 << s); ++b)
-            for (size_t j = 0; j < half; ++j)
-                orc_narrow_ct(&x[b * 2 * half + j], &x[b * 2 * half + j + half], z[((size_t)1 << s) + b], p);
+            for (size_t j = 0; j < half; ++j) {
+                uint32_t* u = &x[b * 2 * half + j];
+                uint32_t* v = &x[b * 2 * half + j + half];
+                if (lazy) orc_narrow_ct_lazy(u, v, z[((size_t)1 << s) + b], p);
+                else orc_narrow_ct(u, v, z[((size_t)1 << s) + b], p);
+            }
+        bound = lazy ? bound + 2 : 6;
     }
     free(z);
 }

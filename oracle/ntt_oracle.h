@@ -12,11 +12,11 @@
  * order 23/23/24 and a host CRT (src/gpu/int_ntt.cu:226-245,352-377).  The
  * restatement follows the product the engine computes instead: 32-bit
  * coefficients; two prime sets, the wide set 15*2^27+1, 27*2^26+1, 63*2^25+1
- * for transforms of 2^23 .. 2^25 points and the narrow set 119*2^23+1,
- * 235*2^22+1, 225*2^22+1 (all below 2^30) up to 2^22 points; a Cooley-Tukey
+ * for transforms of 2^23 .. 2^25 points and the narrow set 7*2^26+1,
+ * 99*2^22+1, 45*2^23+1 (all below 2^29) up to 2^22 points; a Cooley-Tukey
  * forward transform (natural order in, bit-reversed out) and a Gentleman-Sande
  * inverse, Garner CRT to an exact 96-bit value and a sequential carry.  The
- * narrow set's lazy butterflies (values in [0, 4p) forward, [0, 2p) inverse)
+ * narrow set's lazy butterflies (values in [0, 8p) forward, [0, 2p) inverse)
  * are restated separately, for range and congruence checks.  The whole product is pinned against GMP mpz_mul
  * (GmpMultiplier, src/arithmetic/gmp_multiplier.cpp:10-19).
  */
@@ -70,14 +70,19 @@ int orc_set_for(int log2n);
 void orc_crt3_set(int set, const uint32_t r[3], uint32_t out[3]);
 void orc_crt3(const uint32_t r[3], uint32_t out[3]);
 
-/* Lazy butterflies of a narrow prime p < 2^30 with Shoup's quotient
- * floor(w 2^32 / p).  ct: a, b in [0, 4p) -> (a + w b, a - w b) mod p, both in
- * [0, 4p).  gs: a, b in [0, 2p) -> (a + b, (a - b) winv) mod p, both in
- * [0, 2p).  pointwise: x, y in [0, 4p) -> x y 2^-32 mod p in [0, 2p).
- * load: any 32-bit c -> c mod p in [0, 4p).  The layer functions run whole
- * layers as orc_ntt_ct_layers / orc_ntt_gs_layers do, with unreduced values. */
+/* Lazy butterflies of a narrow prime p < 2^29 with Shoup's quotient
+ * floor(w 2^32 / p).  ct: a in [0, 8p), any b -> (a + w b, a - w b) mod p,
+ * both in [0, 6p) (a reduced to [0, 4p) first).  ct_lazy: a in [0, 6p), any b
+ * -> the same congruences, both in [0, 8p) (no reduction).  gs: a, b in
+ * [0, 2p) -> (a + b, (a - b) winv) mod p, both in [0, 2p).  pointwise: x, y in
+ * [0, 8p) -> x y 2^-32 mod p in [0, 2p).  load: any 32-bit c -> c mod p in
+ * [0, 8p).  The layer functions run whole layers as orc_ntt_ct_layers /
+ * orc_ntt_gs_layers do, with unreduced values; the forward ones start below
+ * 8p and skip the reduction while every value is below 6p (every other layer),
+ * as the device's register stages do. */
 uint32_t orc_shoup_quotient(uint32_t w, uint32_t p);
 void orc_narrow_ct(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p);
+void orc_narrow_ct_lazy(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p);
 void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p);
 uint32_t orc_narrow_pointwise(uint32_t x, uint32_t y, uint32_t p);
 uint32_t orc_narrow_load(uint32_t c, uint32_t p);

@@ -2,10 +2,13 @@
 //
 // Two sets of three primes.  The wide set (primes just below 2^31, 2-adic
 // order >= 25) serves transforms up to 2^25 points; every residue is kept fully
-// reduced, in [0, p).  The narrow set (primes below 2^30, 2-adic order >= 22)
-// serves transforms up to 2^22 points with Harvey's lazy butterflies: 4p still
-// fits a 32-bit lane, so forward values live in [0, 4p) and inverse values in
-// [0, 2p), and a butterfly needs one conditional subtraction instead of three.
+// reduced, in [0, p).  The narrow set (primes below 2^29, 2-adic order >= 22)
+// serves transforms up to 2^22 points with Harvey's lazy butterflies: 8p still
+// fits a 32-bit lane, so forward values live in [0, 8p) -- a forward butterfly
+// reduces its first input only when it may exceed 6p, every other layer -- and
+// inverse values in [0, 2p).  Their product, 2^85.93, exceeds every
+// coefficient of a product the set serves: a coefficient sums at most
+// min(la, lb) <= 2^21 terms below 2^64 when la + lb - 1 <= 2^22.
 // Twiddle products use Shoup's precomputed quotient (w' = floor(w * 2^32 / p)):
 // one high and two low 32-bit products give x * w - q * p in [0, 2p) for any
 // 32-bit x.  The pointwise product of two variable residues uses Montgomery
@@ -29,8 +32,8 @@ constexpr int kNarrowMaxLog2N = 22;  // longest transform of the narrow set
 constexpr int kSetWide = 0, kSetNarrow = 1, kSets = 2;
 constexpr uint32_t kPrimeSet[kSets][kPrimes] = {
     {2013265921u /* 15*2^27+1 */, 1811939329u /* 27*2^26+1 */, 2113929217u /* 63*2^25+1 */},
-    {998244353u /* 119*2^23+1 */, 985661441u /* 235*2^22+1 */, 943718401u /* 225*2^22+1 */}};
-constexpr uint32_t kGenSet[kSets][kPrimes] = {{31u, 13u, 5u}, {3u, 3u, 7u}};
+    {469762049u /* 7*2^26+1 */, 415236097u /* 99*2^22+1 */, 377487361u /* 45*2^23+1 */}};
+constexpr uint32_t kGenSet[kSets][kPrimes] = {{31u, 13u, 5u}, {3u, 5u, 7u}};
 constexpr int kSetMaxLog2N[kSets] = {kMaxLog2N, kNarrowMaxLog2N};
 
 // The set that serves a transform of 2^log2n points.
@@ -97,6 +100,10 @@ struct Arith<kSetWide> {
         a = add_mod(u, t, p);
         b = sub_mod(u, t, p);
     }
+    // Forward butterfly with a bound check the wide set never skips.
+    __device__ static __forceinline__ void ct_sel(uint32_t& a, uint32_t& b, uint2 z, uint32_t p, bool) {
+        ct(a, b, z, p);
+    }
     // Gentleman-Sande: (a, b) -> (a + b, (a - b) * zinv).  a - b + p lies in
     // [1, 2p), inside the Shoup input range, so it needs no reduction first.
     __device__ static __forceinline__ void gs(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p) {
@@ -115,21 +122,33 @@ struct Arith<kSetWide> {
     }
 };
 
-// Narrow set (p < 2^30): forward values in [0, 4p), inverse values in [0, 2p).
+// Narrow set (p < 2^29): forward values in [0, 8p), inverse values in [0, 2p).
 template <>
 struct Arith<kSetNarrow> {
-    // Any 32-bit coefficient to [0, 4p): 2^32 < 8p, so one step suffices.
-    __device__ static __forceinline__ uint32_t load_coeff(uint32_t c, uint32_t p) { return min(c, c - 4u * p); }
-    // a, b in [0, 4p).  x = a reduced to [0, 2p); r = z*b - q*p in [0, 2p) is
-    // kept negated (tn = -r mod 2^32) so that both outputs take one add:
-    // x + r in [0, 4p) and x - r + 2p in (0, 4p).
+    // Any 32-bit coefficient to [0, 8p): 2^32 < 16p, so one step suffices.
+    __device__ static __forceinline__ uint32_t load_coeff(uint32_t c, uint32_t p) { return min(c, c - 8u * p); }
+    // a < 8p, b any 32-bit value.  x = a reduced to [0, 4p); r = z*b - q*p in
+    // [0, 2p) is kept negated (tn = -r mod 2^32) so that both outputs take one
+    // add: x + r in [0, 6p) and x - r + 2p in (0, 6p).
     __device__ static __forceinline__ void ct(uint32_t& a, uint32_t& b, uint2 z, uint32_t p) {
-        const uint32_t p2 = 2u * p;
-        const uint32_t x = min(a, a - p2);
+        const uint32_t x = min(a, a - 4u * p);
         const uint32_t q = __umulhi(b, z.y);
         const uint32_t tn = q * p - b * z.x;
         a = x - tn;
-        b = x + tn + p2;
+        b = x + tn + 2u * p;
+    }
+    // a < 6p, no reduction: outputs a + r in [0, 8p) and a - r + 2p in (0, 8p).
+    __device__ static __forceinline__ void ct_lazy(uint32_t& a, uint32_t& b, uint2 z, uint32_t p) {
+        const uint32_t x = a;
+        const uint32_t q = __umulhi(b, z.y);
+        const uint32_t tn = q * p - b * z.x;
+        a = x - tn;
+        b = x + tn + 2u * p;
+    }
+    // lazy: the caller knows a < 6p (a compile-time constant after unrolling).
+    __device__ static __forceinline__ void ct_sel(uint32_t& a, uint32_t& b, uint2 z, uint32_t p, bool lazy) {
+        if (lazy) ct_lazy(a, b, z, p);
+        else ct(a, b, z, p);
     }
     // a, b in [0, 2p): a + b in [0, 4p) reduced to [0, 2p); (a - b + 2p) in
     // (0, 4p) times zinv by Shoup, in [0, 2p).
@@ -142,21 +161,35 @@ struct Arith<kSetNarrow> {
         const uint32_t q = __umulhi(d, zi.y);
         b = d * zi.x - q * p;
     }
-    // x, y in [0, 4p) reduced to [0, 2p); their product is below 4p^2 < 2^32 p,
+    // x, y in [0, 8p) reduced to [0, 2p); their product is below 4p^2 < 2^32 p,
     // so the Montgomery result (t + m p) / 2^32 lies in [0, 2p) unreduced.
     __device__ static __forceinline__ uint32_t pointwise(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) {
-        const uint32_t p2 = 2u * p;
+        const uint32_t p2 = 2u * p, p4 = 4u * p;
+        x = min(x, x - p4);
         x = min(x, x - p2);
+        y = min(y, y - p4);
         y = min(y, y - p2);
         const uint64_t t = (uint64_t)x * y;
         const uint32_t m = (uint32_t)t * pinv;
         return (uint32_t)((t + (uint64_t)m * p) >> 32);
     }
-    // x * f 2^-32 (mul_fixed) in (0, 2p): inside both the forward [0, 4p) and
+    // x * f 2^-32 (mul_fixed) in (0, 2p): inside both the forward [0, 8p) and
     // the inverse [0, 2p) input ranges.
     __device__ static __forceinline__ uint32_t scale(uint32_t x, uint32_t f, uint32_t fp, uint32_t p) {
         return mul_fixed(x, f, fp, p);
     }
 };
+
+// Bound (in units of p) of forward values before forward layer l of a run that
+// starts below B0 p, when every layer whose first input may exceed 6p reduces it
+// to 4p (Arith<kSetNarrow>::ct) and the others skip the reduction (ct_lazy):
+// B -> B + 2 while B <= 6, else 6.
+__host__ __device__ constexpr int lazy_bound(int B0, int l) {
+    int b = B0;
+    for (int i = 0; i < l; ++i) b = b <= 6 ? b + 2 : 6;
+    return b;
+}
+__host__ __device__ constexpr bool lazy_ok(int B0, int l) { return lazy_bound(B0, l) <= 6; }
+static_assert(lazy_bound(2, 3) == 8 && lazy_bound(8, 1) == 6 && lazy_bound(8, 2) == 8, "lazy schedule");
 
 }  // namespace pintt

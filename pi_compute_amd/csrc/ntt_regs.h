@@ -82,8 +82,15 @@ __device__ __forceinline__ uint32_t post_block(int l, int r, uint32_t lane) {
 // Six forward layers on NB tiles sharing twiddles; entry layout in, post
 // layout out.  ZT: the group's top half (t >= 32, registers 8..15) is zero,
 // so layer 0 only copies the bottom half up.
-template <int SET, int NB, bool ZT = false, int SB = 1, class TW>
+// Forward values enter below B0 p (narrow set): layer l of the stage skips the
+// reduction of its first input when lazy_ok says that input is below 6p.  A
+// stage exits below ct6_exit(B0, ZT) p.
+__host__ __device__ constexpr int ct6_exit(int B0, bool ZT) { return lazy_bound(B0, ZT ? 5 : 6); }
+
+template <int SET, int NB, bool ZT = false, int SB = 1, int B0 = 8, class TW>
 __device__ __forceinline__ void ct6(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
+    // Bound before stage layer l: the zero-top copy of layer 0 changes none.
+    auto lazy = [](int l) { return ZT ? lazy_ok(B0, l - 1) : lazy_ok(B0, l); };
     if constexpr (ZT) {
 #pragma unroll
         for (int u = 0; u < NB; ++u)
@@ -99,7 +106,8 @@ __device__ __forceinline__ void ct6(uint32_t (&v)[NB][16], const TW& tw, uint32_
 #pragma unroll
             for (int x = 0; x < h; ++x)
 #pragma unroll
-                for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p);
+                for (int u = 0; u < NB; ++u)
+                    Arith<SET>::ct_sel(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p, lazy(l));
         }
     }
     lane_swaps<NB, SB>(v);
@@ -114,7 +122,7 @@ __device__ __forceinline__ void ct6(uint32_t (&v)[NB][16], const TW& tw, uint32_
         for (int q = 0; q < 2; ++q) {
             const int rr = r | (q << SB);
 #pragma unroll
-            for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][rr], v[u][rr | 8], z, p);
+            for (int u = 0; u < NB; ++u) Arith<SET>::ct_sel(v[u][rr], v[u][rr | 8], z, p, lazy(4));
         }
     }
 #pragma unroll
@@ -122,7 +130,7 @@ __device__ __forceinline__ void ct6(uint32_t (&v)[NB][16], const TW& tw, uint32_
         if (r & (1 << SB)) continue;
         const uint2 z = tw(5, post_block<SB>(5, r, lane));
 #pragma unroll
-        for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][r], v[u][r | (1 << SB)], z, p);
+        for (int u = 0; u < NB; ++u) Arith<SET>::ct_sel(v[u][r], v[u][r | (1 << SB)], z, p, lazy(5));
     }
 }
 
@@ -166,7 +174,7 @@ __device__ __forceinline__ void gs6(uint32_t (&v)[NB][16], const TW& tw, uint32_
 
 // Five forward layers on groups of 32 elements held by lanes l and l + 32:
 // entry t = (i3 i2 i1 i0 L5) in, post t = (L5 i2 i1 i0 i3) out.
-template <int SET, int NB, class TW>
+template <int SET, int NB, int B0 = 8, class TW>
 __device__ __forceinline__ void ct5(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
@@ -177,7 +185,8 @@ __device__ __forceinline__ void ct5(uint32_t (&v)[NB][16], const TW& tw, uint32_
 #pragma unroll
             for (int x = 0; x < h; ++x)
 #pragma unroll
-                for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p);
+                for (int u = 0; u < NB; ++u)
+                    Arith<SET>::ct_sel(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p, lazy_ok(B0, l));
         }
     }
     swap32<NB>(v);
@@ -187,7 +196,7 @@ __device__ __forceinline__ void ct5(uint32_t (&v)[NB][16], const TW& tw, uint32_
     for (int r = 0; r < 8; ++r) {
         const uint2 z = tw(4, (l5 << 3) | (uint32_t)r);
 #pragma unroll
-        for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][r], v[u][r | 8], z, p);
+        for (int u = 0; u < NB; ++u) Arith<SET>::ct_sel(v[u][r], v[u][r | 8], z, p, lazy_ok(B0, 4));
     }
 }
 
@@ -217,7 +226,7 @@ __device__ __forceinline__ void gs5(uint32_t (&v)[NB][16], const TW& tw, uint32_
 
 // K <= 4 layers inside each lane: register i = (g << K) | t holds element t of
 // the lane's group g (16 >> K groups).  tw(g, l, blk).
-template <int SET, int K, class TW>
+template <int SET, int K, int B0 = 8, class TW>
 __device__ __forceinline__ void ct_lane(uint32_t (&v)[16], const TW& tw, uint32_t p) {
     static_assert(K >= 1 && K <= 4, "in-lane stage of 1..4 layers");
 #pragma unroll
@@ -230,7 +239,8 @@ __device__ __forceinline__ void ct_lane(uint32_t (&v)[16], const TW& tw, uint32_
                 const uint2 z = tw(g, l, sub);
 #pragma unroll
                 for (int x = 0; x < h; ++x)
-                    Arith<SET>::ct(v[(g << K) + sub * 2 * h + x], v[(g << K) + sub * 2 * h + x + h], z, p);
+                    Arith<SET>::ct_sel(v[(g << K) + sub * 2 * h + x], v[(g << K) + sub * 2 * h + x + h], z, p,
+                                       lazy_ok(B0, l));
             }
         }
 }

@@ -205,21 +205,22 @@ struct RegRows {
     }
 };
 
-// Stage B of a register row kernel, forward or inverse, on NB tiles.
-template <int LC, int SET, int NB, bool INV, class TW>
+// Stage B of a register row kernel, forward or inverse, on NB tiles;
+// forward values enter below B0 p.
+template <int LC, int SET, int NB, bool INV, int B0, class TW>
 __device__ __forceinline__ void row_stage_b(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
     if constexpr (LC == 12) {
         if constexpr (INV) gs6<SET, NB, 1>(v, tw, p);
-        else ct6<SET, NB, false, 1>(v, tw, p);
+        else ct6<SET, NB, false, 1, B0>(v, tw, p);
     } else if constexpr (LC == 11) {
         if constexpr (INV) gs5<SET, NB>(v, tw, p);
-        else ct5<SET, NB>(v, tw, p);
+        else ct5<SET, NB, B0>(v, tw, p);
     } else {
         auto tl = [&](int, int l, uint32_t blk) { return tw(l, blk); };
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             if constexpr (INV) gs_lane<SET, 4>(v[u], tl, p);
-            else ct_lane<SET, 4>(v[u], tl, p);
+            else ct_lane<SET, 4, B0>(v[u], tl, p);
         }
     }
 }
@@ -235,6 +236,10 @@ __device__ __forceinline__ void row_stage_b(uint32_t (&v)[NB][16], const TW& tw,
 template <int LC, int KIND, int SET, bool SCALED>
 __global__ void __launch_bounds__(256) k_row_reg(const JobList<RowJob> jobs, int log2r, Twiddles tw) {
     using RG = RegRows<LC>;
+    // Forward values enter stage A below 2p when scaled (mul_fixed), below 8p
+    // as the column pass stored them otherwise; stage B starts where A ends.
+    constexpr int kB0A = SCALED ? 2 : 8;
+    constexpr int kB0B = ct6_exit(kB0A, false);
     constexpr bool kPair = KIND == kRowPair;
     constexpr bool kTransB = KIND == kRowTransformB;
     constexpr int NA = kPair ? 2 : 1;                       // chunk tiles
@@ -308,7 +313,7 @@ __global__ void __launch_bounds__(256) k_row_reg(const JobList<RowJob> jobs, int
                 for (int t = 0; t < NL; ++t) v[t][i] = Arith<SET>::scale(v[t][i], f.x, f.y, p);
             }
         }
-        ct6<SET, NL, false, RG::SB>(v, fA, p);
+        ct6<SET, NL, false, RG::SB, kB0A>(v, fA, p);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const uint32_t k = (post_t<RG::SB>(r, lane) << RG::GA) | j;
@@ -326,7 +331,7 @@ __global__ void __launch_bounds__(256) k_row_reg(const JobList<RowJob> jobs, int
     } else if constexpr (kPair || kTransB) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) vb[0][i] = lds[NL - 1][rw][RG::lds_at(RG::b_entry(i, lane, wr))];
-        row_stage_b<LC, SET, 1, false>(vb, fB, p);
+        row_stage_b<LC, SET, 1, false, kB0B>(vb, fB, p);
         if (job.write_b) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) gb[((uint32_t)r << (LC - 4)) | ell] = vb[0][r];
@@ -338,13 +343,13 @@ __global__ void __launch_bounds__(256) k_row_reg(const JobList<RowJob> jobs, int
         for (int i = 0; i < 16; ++i)
 #pragma unroll
             for (int t = 0; t < NA; ++t) v[t][i] = lds[t][rw][RG::lds_at(RG::b_entry(i, lane, wr))];
-        row_stage_b<LC, SET, NA, false>(v, fB, p);
+        row_stage_b<LC, SET, NA, false, kB0B>(v, fB, p);
 #pragma unroll
         for (int r = 0; r < 16; ++r)
 #pragma unroll
             for (int t = 0; t < NA; ++t)
                 v[t][r] = Arith<SET>::pointwise(v[t][r], KIND == kRowSquare ? v[t][r] : vb[0][r], p, pinv);
-        row_stage_b<LC, SET, NA, true>(v, iB, p);
+        row_stage_b<LC, SET, NA, true, kB0B>(v, iB, p);
 #pragma unroll
         for (int i = 0; i < 16; ++i)
 #pragma unroll

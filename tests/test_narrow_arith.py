@@ -1,10 +1,11 @@
 """Pins the lazy (Harvey) arithmetic of the narrow prime set in the oracle.
 
-Transforms of up to 2^22 points run over three primes below 2^30, so 4p fits
-a 32-bit word: forward values may stay in [0, 4p) and inverse values in
-[0, 2p), and each butterfly needs one conditional subtraction.  The checks
-here use only the definitions (congruences computed with Python integers,
-ranges) and the strict restatement, which the textbook DFT pins:
+Transforms of up to 2^22 points run over three primes below 2^29, so 8p fits
+a 32-bit word: forward values may stay in [0, 8p) -- a forward butterfly
+reduces its first input to [0, 4p) only when it may be 6p or more, every other
+layer -- and inverse values in [0, 2p).  The checks here use only the
+definitions (congruences computed with Python integers, ranges) and the strict
+restatement, which the textbook DFT pins:
   * every butterfly output lies in its range and is congruent to the exact
     butterfly, on edge values of the input ranges and on random values;
   * whole lazy transforms (columns then rows, and inverse) agree with the
@@ -31,15 +32,21 @@ def _twiddles(oracle, prime, rng):
 
 
 @pytest.mark.parametrize("prime", NARROW)
-def test_ct_butterfly_range_and_congruence(oracle, prime):
+@pytest.mark.parametrize("lazy", [False, True], ids=["reducing", "lazy"])
+def test_ct_butterfly_range_and_congruence(oracle, prime, lazy):
+    # reducing: a < 8p -> outputs < 6p; lazy: a < 6p -> outputs < 8p; b is any
+    # 32-bit value either way (Shoup's product accepts it).
     p = oracle.prime(prime)
-    rng = random.Random(prime)
+    rng = random.Random(prime + (100 if lazy else 0))
     ws = _twiddles(oracle, prime, rng)
-    cases = [(a, b, w) for a in _edges(p, 4 * p) for b in _edges(p, 4 * p) for w in ws]
-    cases += [(rng.randrange(4 * p), rng.randrange(4 * p), rng.randrange(p)) for _ in range(20000)]
+    a_top, out_top = (6 * p, 8 * p) if lazy else (8 * p, 6 * p)
+    f = oracle.narrow_ct_lazy if lazy else oracle.narrow_ct
+    bs = _edges(p, 8 * p) + [2**32 - 1, 2**32 - 2, 5 * p, 7 * p + 3]
+    cases = [(a, b, w) for a in _edges(p, a_top) + [a_top - 1] for b in bs for w in ws]
+    cases += [(rng.randrange(a_top), rng.randrange(2**32), rng.randrange(p)) for _ in range(20000)]
     for a, b, w in cases:
-        A, B = oracle.narrow_ct(a, b, w, p)
-        assert A < 4 * p and B < 4 * p, (a, b, w)
+        A, B = f(a, b, w, p)
+        assert A < out_top and B < out_top, (a, b, w)
         assert (A - a - w * b) % p == 0 and (B - a + w * b) % p == 0, (a, b, w)
 
 
@@ -62,14 +69,14 @@ def test_pointwise_and_load(oracle, prime):
     L = oracle.L
     r_inv = pow(2**32, p - 2, p)
     rng = random.Random(20 + prime)
-    pairs = [(x, y) for x in _edges(p, 4 * p) for y in _edges(p, 4 * p)]
-    pairs += [(rng.randrange(4 * p), rng.randrange(4 * p)) for _ in range(20000)]
+    pairs = [(x, y) for x in _edges(p, 8 * p) + [8 * p - 1] for y in _edges(p, 8 * p) + [8 * p - 1]]
+    pairs += [(rng.randrange(8 * p), rng.randrange(8 * p)) for _ in range(20000)]
     for x, y in pairs:
         r = L.orc_narrow_pointwise(x, y, p)
         assert r < 2 * p and (r - x * y * r_inv) % p == 0, (x, y)
-    for c in _edges(p, 2**32) + [4 * p - 1, 4 * p, 4 * p + 1, 2**32 - 1] + [rng.randrange(2**32) for _ in range(5000)]:
+    for c in _edges(p, 2**32) + [8 * p - 1, 8 * p, 8 * p + 1, 2**32 - 1] + [rng.randrange(2**32) for _ in range(5000)]:
         v = L.orc_narrow_load(c, p)
-        assert v < 4 * p and (v - c) % p == 0, c
+        assert v < 8 * p and (v - c) % p == 0, c
 
 
 @pytest.mark.parametrize("prime", NARROW)
@@ -77,13 +84,14 @@ def test_pointwise_and_load(oracle, prime):
 def test_lazy_transforms_match_strict(oracle, prime, log2n, split):
     p = oracle.prime(prime)
     rng = np.random.default_rng(prime * 100 + log2n)
-    x = rng.integers(0, 4 * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
+    x = rng.integers(0, 8 * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
     strict = oracle.ct_layers((x.astype(np.uint64) % p).astype(np.uint32), log2n, prime, 0, log2n)
-    # column layers, then row layers: the four-step split of the engine
+    # column layers, then row layers: the four-step split of the engine (each
+    # pass starts below 8p, as the column pass stores its values)
     cols = oracle.narrow_layers(x, log2n, prime, 0, split)
-    assert int(cols.max()) < 4 * p
+    assert int(cols.max()) < 8 * p
     lazy = oracle.narrow_layers(cols, log2n, prime, split, log2n)
-    assert int(lazy.max()) < 4 * p
+    assert int(lazy.max()) < 8 * p
     assert np.array_equal(lazy.astype(np.uint64) % p, strict.astype(np.uint64))
 
     y = rng.integers(0, 2 * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
@@ -96,11 +104,18 @@ def test_lazy_transforms_match_strict(oracle, prime, log2n, split):
 
 
 def test_out_of_contract_inputs_are_caught(oracle):
-    """The range checks have teeth: forward inputs at or above 4p break them."""
+    """The range checks have teeth: a reducing butterfly's first input at or
+    above 8p, or a lazy one's at or above 6p, breaks them."""
     p = oracle.prime(3)
     bad = 0
-    for a in range(4 * p, 2**32, 7919 * 4096):
+    for a in range(8 * p, 2**32, 7919 * 16):
         A, B = oracle.narrow_ct(a, 0, 1, p)
-        if not (A < 4 * p and B < 4 * p and (A - a) % p == 0 and (B - a) % p == 0):
+        if not (A < 6 * p and B < 6 * p and (A - a) % p == 0 and (B - a) % p == 0):
+            bad += 1
+    assert bad > 0
+    bad = 0
+    for a in range(6 * p, 2**32, 7919 * 16):
+        A, B = oracle.narrow_ct_lazy(a, 2 * p - 1, 1, p)
+        if not (A < 8 * p and B < 8 * p and (A - a - 2 * p + 1) % p == 0 and (B - a + 2 * p - 1) % p == 0):
             bad += 1
     assert bad > 0

@@ -24,7 +24,7 @@ import pi_compute_amd as pc
 
 ROOT = Path(__file__).resolve().parent.parent
 # Prime factors of p - 1; primes 0-2 form the wide set, 3-5 the narrow set.
-FACTORS = {0: (2, 3, 5), 1: (2, 3), 2: (2, 3, 7), 3: (2, 7, 17), 4: (2, 5, 47), 5: (2, 3, 5)}
+FACTORS = {0: (2, 3, 5), 1: (2, 3), 2: (2, 3, 7), 3: (2, 7), 4: (2, 3, 11), 5: (2, 3, 5)}
 
 
 def _is_prime(n):
@@ -58,8 +58,8 @@ def test_prime_set(oracle, i):
     assert (p - 1) % (1 << k) == 0 and ((p - 1) >> k) % 2 == 1
     if i < 3:
         assert k >= 25
-    else:  # narrow set: lazy butterflies need 4p below 2^32
-        assert k >= 22 and p < 2**30 and 4 * p < 2**32
+    else:  # narrow set: lazy butterflies need 8p below 2^32
+        assert k >= 22 and p < 2**29 and 8 * p < 2**32
     # g generates Z_p^*: g^((p-1)/q) != 1 for every prime q | p - 1
     m = p - 1
     for q in FACTORS[i]:
@@ -71,12 +71,18 @@ def test_prime_set(oracle, i):
 
 
 def test_crt_modulus_bounds_the_convolution(oracle):
-    # N * (2^32 - 1)^2 < M at each set's longest transform: every coefficient is
-    # reconstructed exactly
+    # Every coefficient is reconstructed exactly.  A coefficient of a product
+    # of la and lb coefficients (la + lb - 1 <= N, the transform length) sums
+    # min(la, lb) <= N/2 terms below 2^64; the wide set even covers N terms.
     M = oracle.prime(0) * oracle.prime(1) * oracle.prime(2)
     assert (1 << 25) * (2**32 - 1) ** 2 < M
     M = oracle.prime(3) * oracle.prime(4) * oracle.prime(5)
-    assert (1 << 22) * (2**32 - 1) ** 2 < M
+    assert (1 << 21) * (2**32 - 1) ** 2 < M
+    for la in (1, 2, 1000, 1 << 20, (1 << 21) - 1, 1 << 21, (1 << 21) + 1, (1 << 22) - 1):
+        lb = (1 << 22) + 1 - la  # the longest product 2^22 points hold
+        assert min(la, lb) * (2**32 - 1) ** 2 < M, la
+    # the narrow set could not hold N full terms: the bound above is the one it needs
+    assert (1 << 22) * (2**32 - 1) ** 2 > M
     assert [oracle.L.orc_set_for(k) for k in (1, 22, 23, 25)] == [1, 1, 0, 0]
 
 
@@ -163,7 +169,7 @@ def test_crt_against_integer_reconstruction(oracle, prime_set):
     P = [oracle.prime(3 * prime_set + i) for i in range(3)]
     M = P[0] * P[1] * P[2]
     rng = random.Random(5)
-    xs = [0, 1, M - 1, M // 2, 2**89 - 1, (1 << 64) + 12345] + [rng.randrange(M) for _ in range(300)]
+    xs = [0, 1, M - 1, M // 2, M - 2**40, (1 << 64) + 12345] + [rng.randrange(M) for _ in range(300)]
     for x in xs:
         assert oracle.crt3([x % P[0], x % P[1], x % P[2]], prime_set) == x
 

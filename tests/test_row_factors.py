@@ -18,7 +18,7 @@ import pytest
 
 # A narrow-set and a wide-set prime with the generators the library uses
 # (modarith.h: kPrimeSet, kGenSet, kSetMaxLog2N).
-PRIMES = [(998244353, 3, 22), (2013265921, 31, 25)]
+PRIMES = [(469762049, 3, 22), (2013265921, 31, 25)]
 
 
 def brev(x: int, bits: int) -> int:
