@@ -186,6 +186,23 @@ void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p) {
     *b = d * winv - q * p;
 }
 
+/* The shared operand of a row's products (b of a0 b and a1 b): y in [0, 8p)
+ * reduced once to [0, 2p); each x is then reduced only to [0, 4p). */
+uint32_t orc_narrow_shared_operand(uint32_t y, uint32_t p) {
+    y = min_u32(y, y - 4u * p);
+    return min_u32(y, y - 2u * p);
+}
+
+uint32_t orc_narrow_pointwise_shared(uint32_t x, uint32_t ys, uint32_t p) {
+    uint32_t pinv = p; /* -p^-1 mod 2^32 by Newton's iteration */
+    for (int i = 0; i < 5; ++i) pinv *= 2u - p * pinv;
+    pinv = 0u - pinv;
+    x = min_u32(x, x - 4u * p);
+    const uint64_t t = (uint64_t)x * ys;
+    const uint32_t m = (uint32_t)t * pinv;
+    return (uint32_t)((t + (uint64_t)m * p) >> 32);
+}
+
 uint32_t orc_narrow_pointwise(uint32_t x, uint32_t y, uint32_t p) {
     const uint32_t p2 = 2u * p, p4 = 4u * p;
     uint32_t pinv = p; /* -p^-1 mod 2^32 by Newton's iteration */

@@ -26,14 +26,9 @@ __device__ __forceinline__ uint32_t cm_compose(uint32_t later, uint32_t earlier)
 }
 constexpr uint32_t kCmIdentity = 2u;  // f(0) = 0, f(1) = 1
 
+// Last job with blk0 <= blk (jobs sorted by blk0), by every lane of a wave.
 __device__ __forceinline__ uint32_t find_job(const AddJob* __restrict__ jobs, uint32_t njobs, uint32_t blk) {
-    uint32_t lo = 0, hi = njobs - 1;  // last job with blk0 <= blk
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].blk0 <= blk) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    return wave_search(njobs, blk, [jobs](uint32_t i) { return jobs[i].blk0; });
 }
 
 __device__ __forceinline__ uint64_t width_of(const AddJob& j) { return j.xn > j.yn ? j.xn : j.yn; }

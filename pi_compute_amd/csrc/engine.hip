@@ -224,6 +224,17 @@ CrtConst crt_constants(int log2n) {
     c.k2 = shoup(mulmod(scale(p2), ip01_2, p2), p2);
     c.c20 = shoup(ip01_2, p2);
     c.c21 = shoup(invmod(p1 % p2, p2), p2);
+    const unsigned __int128 m = (unsigned __int128)p0 * p1 * p2;
+    const unsigned __int128 negm = ((unsigned __int128)1 << 96) - m;
+    const uint32_t ps[kPrimes] = {p0, p1, p2};
+    for (int i = 0; i < kPrimes; ++i) {
+        const unsigned __int128 mi = m / ps[i];
+        c.e[i] = shoup(mulmod(scale(ps[i]), invmod((uint32_t)(mi % ps[i]), ps[i]), ps[i]), ps[i]);
+        c.mlo[i] = (uint32_t)mi;
+        c.mhi[i] = (uint32_t)(mi >> 32);
+        c.inv[i] = (float)(1.0 / (double)ps[i]);
+        c.negm[i] = (uint32_t)(negm >> (32 * i));
+    }
     return c;
 }
 

@@ -115,6 +115,11 @@ struct Arith<kSetWide> {
     __device__ static __forceinline__ uint32_t pointwise(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) {
         return mul_mont(x, y, p, pinv);
     }
+    __device__ static __forceinline__ uint32_t shared_operand(uint32_t y, uint32_t) { return y; }
+    __device__ static __forceinline__ uint32_t pointwise_shared(uint32_t x, uint32_t ys, uint32_t p,
+                                                                uint32_t pinv) {
+        return mul_mont(x, ys, p, pinv);
+    }
     // x * f 2^-32 (mul_fixed), fully reduced.
     __device__ static __forceinline__ uint32_t scale(uint32_t x, uint32_t f, uint32_t fp, uint32_t p) {
         const uint32_t u = mul_fixed(x, f, fp, p);
@@ -170,6 +175,20 @@ struct Arith<kSetNarrow> {
         y = min(y, y - p4);
         y = min(y, y - p2);
         const uint64_t t = (uint64_t)x * y;
+        const uint32_t m = (uint32_t)t * pinv;
+        return (uint32_t)((t + (uint64_t)m * p) >> 32);
+    }
+    // The operand of pointwise products that one factor shares (b in a*b):
+    // y in [0, 8p) reduced to [0, 2p) once.  Then x needs only [0, 4p): x y <
+    // 8p^2 < 2^32 p keeps the Montgomery result in [0, 2p).
+    __device__ static __forceinline__ uint32_t shared_operand(uint32_t y, uint32_t p) {
+        y = min(y, y - 4u * p);
+        return min(y, y - 2u * p);
+    }
+    __device__ static __forceinline__ uint32_t pointwise_shared(uint32_t x, uint32_t ys, uint32_t p,
+                                                                uint32_t pinv) {
+        x = min(x, x - 4u * p);
+        const uint64_t t = (uint64_t)x * ys;
         const uint32_t m = (uint32_t)t * pinv;
         return (uint32_t)((t + (uint64_t)m * p) >> 32);
     }

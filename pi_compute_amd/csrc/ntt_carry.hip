@@ -82,12 +82,45 @@ __device__ __forceinline__ uint32_t block_scan_fn(uint32_t f, uint32_t* lds_wave
     return excl;
 }
 
-// Garner CRT of one position: the exact value x < p0 p1 p2 < 2^93 as 3 words.
-// Residues may be any 32-bit values (each is first multiplied by Shoup).
+// x r mod p in [0, 2p) for any 32-bit r (Shoup's product without its last
+// reduction).
+__device__ __forceinline__ uint32_t shoup_lazy(uint32_t r, uint2 w, uint32_t p) {
+    return r * w.x - __umulhi(r, w.y) * p;
+}
+
+// CRT of one position: the exact value x < p0 p1 p2 as 3 words.  Residues may
+// be any 32-bit values (each is first multiplied by Shoup).
+//
+// Narrow set: y_i in [0, 2 p_i) and sum_i y_i M_i = x + k M exactly, with
+// sum_i y_i / p_i = k + x / M.  Every product the set serves has x < 2^85 <
+// 0.526 M, so the sum lies in [k, k + 0.526); with 0.25 added its distance to
+// the integers around it is above 0.22, and the single-precision estimate
+// (three conversions and three fused multiply-adds, relative errors near 2^-24
+// on terms below 2, about 2e-6 in all) truncates to k.  The 96-bit sum then
+// needs no reduction: x = sum_i y_i M_i + k (2^96 - M) mod 2^96.  Its partial
+// sums fit 64 bits: y_i mlo_i < 2^30 2^32, three of them below 2^63.6, plus
+// k negm_0 < 6 2^32.
 template <int SET>
 __device__ __forceinline__ void crt_words(uint32_t r0, uint32_t r1, uint32_t r2, const CrtConst& cc,
                                           uint32_t& x0, uint32_t& x1, uint32_t& x2) {
     constexpr uint32_t p0 = kPrimeSet[SET][0], p1 = kPrimeSet[SET][1], p2 = kPrimeSet[SET][2];
+    if constexpr (SET == kSetNarrow) {
+        const uint32_t y0 = shoup_lazy(r0, cc.e[0], p0);
+        const uint32_t y1 = shoup_lazy(r1, cc.e[1], p1);
+        const uint32_t y2 = shoup_lazy(r2, cc.e[2], p2);
+        float f = __builtin_fmaf((float)y0, cc.inv[0], 0.25f);
+        f = __builtin_fmaf((float)y1, cc.inv[1], f);
+        f = __builtin_fmaf((float)y2, cc.inv[2], f);
+        const uint32_t k = (uint32_t)f;
+        const uint64_t lo = (uint64_t)y0 * cc.mlo[0] + (uint64_t)y1 * cc.mlo[1] + (uint64_t)y2 * cc.mlo[2] +
+                            (uint64_t)k * cc.negm[0];
+        const uint64_t hi = (uint64_t)y0 * cc.mhi[0] + (uint64_t)y1 * cc.mhi[1] + (uint64_t)y2 * cc.mhi[2] +
+                            (lo >> 32) + (uint64_t)k * cc.negm[1] + ((uint64_t)(k * cc.negm[2]) << 32);
+        x0 = (uint32_t)lo;
+        x1 = (uint32_t)hi;
+        x2 = (uint32_t)(hi >> 32);
+        return;
+    }
     uint32_t y0 = mul_shoup(r0, cc.k0.x, cc.k0.y, p0);
     uint32_t y1 = sub_mod(mul_shoup(r1, cc.k1.x, cc.k1.y, p1), mul_shoup(y0, cc.c1.x, cc.c1.y, p1), p1);
     uint32_t y2 = sub_mod(mul_shoup(r2, cc.k2.x, cc.k2.y, p2), mul_shoup(y0, cc.c20.x, cc.c20.y, p2), p2);
@@ -101,22 +134,18 @@ __device__ __forceinline__ void crt_words(uint32_t r0, uint32_t r1, uint32_t r2,
     x2 = (uint32_t)(hi >> 32);
 }
 
-// Index of the group that owns carry workgroup `blk` (jobs sorted by blk0).
+// Index of the group that owns carry block `blk` (jobs sorted by blk0, job 0
+// at block 0), called by every lane of a wave.
 __device__ __forceinline__ uint32_t find_job(const JobList<CarryJob>& jobs, uint32_t njobs, uint32_t blk) {
     uint32_t lo = 0;  // last job with blk0 <= blk
     if (jobs.ninl) {
 #pragma unroll
         for (int i = 1; i < kInlineJobs; ++i)
             if ((uint32_t)i < jobs.ninl && jobs.inl[i].blk0 <= blk) lo = i;
-    } else {
-        uint32_t hi = njobs - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (jobs.dev[mid].blk0 <= blk) lo = mid;
-            else hi = mid - 1;
-        }
+        return lo;
     }
-    return lo;
+    const CarryJob* dev = jobs.dev;
+    return wave_search(njobs, blk, [dev](uint32_t i) { return dev[i].blk0; });
 }
 
 constexpr int kWT = kCarryWordsPerThread;
@@ -140,10 +169,8 @@ __global__ void __launch_bounds__(CT)
     __shared__ uint32_t waves[kCT / 64];
     __shared__ uint64_t edge[kCT / 64][2];  // a wave's last lane's spill terms, for the next wave's lane 0
     __shared__ uint64_t head[2][2];         // the spill terms of the two positions before the first word
-    __shared__ uint32_t sh_job, sh_win;
-    if (threadIdx.x == 0) sh_job = find_job(jobs, njobs, blockIdx.x);
-    __syncthreads();
-    const CarryJob job = PINTT_JOB(CarryJob, jobs, sh_job);
+    __shared__ uint32_t sh_win;
+    const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
     const uint32_t lb = blockIdx.x - job.blk0;  // block within the group
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const long long first = (long long)lb * kWB - kCarryWindowWords;  // first word summed
@@ -295,11 +322,17 @@ template <int CT>
 __global__ void __launch_bounds__(64) k_carry_fixup(const JobList<CarryJob> jobs, uint32_t njobs) {
     constexpr int kWB = carry_block_words(CT);
     const uint32_t lane = threadIdx.x;
+    {
+        // The groups' maps lie one after another in block order (maps - blk0
+        // is the same for every job of a launch, and job 0 starts at block
+        // 0), so the flag needs no job lookup: almost every block stops here.
+        const CarryJob j0 = PINTT_JOB(CarryJob, jobs, 0u);
+        if (!(as_global(j0.maps)[blockIdx.x] & kFnUnresolved)) return;
+    }
     const CarryJob job = PINTT_JOB(CarryJob, jobs, find_job(jobs, njobs, blockIdx.x));
     const uint32_t lb = blockIdx.x - job.blk0;
-    if (lb == 0) return;
+    if (lb == 0) return;  // never flagged: its window lies before word 0
     const global_ptr<const uint32_t> maps = as_global(job.maps);
-    if (!(maps[lb] & kFnUnresolved)) return;  // its window fixed its carry-in
     // Maps of blocks 0 .. lb-1 in order, 1024 per round: 16 consecutive
     // maps per lane, all loaded before any is composed.
     uint32_t total = kFnIdentity;  // composition of the maps of the rounds so far

@@ -75,6 +75,25 @@ __device__ __forceinline__ uint32_t row_factor(RowTable t, uint32_t e, uint32_t 
 }
 #define PINTT_ROW_TABLE(tw, prime) pick_row((tw).row[0], (tw).row[1], (tw).row[2], (prime))
 
+// Last index i < n with key(i) <= v, for keys sorted ascending and key(0) <= v,
+// called by every lane of a wave: each round every lane reads one sampled key
+// and a ballot keeps the last sample at or below v, so 4,096 entries take two
+// dependent loads instead of a twelve-step binary search.
+template <class Key>
+__device__ __forceinline__ uint32_t wave_search(uint32_t n, uint32_t v, Key key) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t lo = 0, span = n;  // the answer lies in [lo, lo + span)
+    while (span > 1) {
+        const uint32_t step = (span + 63u) >> 6;
+        const bool ok = lane * step < span && key(lo + lane * step) <= v;
+        const uint64_t m = __ballot(ok);  // lane 0 is always set
+        const uint32_t nlo = lo + (63u - (uint32_t)__builtin_clzll(m)) * step;
+        span = min(step, lo + span - nlo);
+        lo = nlo;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
 // One pad word after every 16 keeps radix-16 strides off a single LDS bank.
 __device__ __forceinline__ uint32_t pad(uint32_t x) { return x + (x >> 4); }
 

@@ -33,9 +33,13 @@ struct Twiddles {
     const uint32_t* row[kPrimes];  // row factor tables (lo, then hi pairs; ntt_device.h)
 };
 
-// Constants for Garner's CRT at one transform length N.  The residues coming
-// out of the inverse transform equal N * x * 2^-32 (Montgomery pointwise
-// product), so the first multiplier of each prime also carries 2^32 / N.
+// Constants for the CRT at one transform length N.  The residues coming out of
+// the inverse transform equal N * x * 2^-32 (Montgomery pointwise product), so
+// the first multiplier of each prime also carries 2^32 / N.  The wide set uses
+// Garner's form (k0 .. c21); the narrow set the explicit sum
+// x = sum_i y_i M_i - k M, M_i = M / p_i, y_i = r_i 2^32 N^-1 M_i^-1 mod p_i
+// (e), k from a single-precision estimate of sum_i y_i / p_i (inv), with the
+// M_i split into 32-bit halves (mlo, mhi) and -M mod 2^96 as three words (negm).
 struct CrtConst {
     uint2 k0;        // 2^32 N^-1                  mod p0
     uint2 k1;        // 2^32 N^-1 p0^-1            mod p1
@@ -43,6 +47,11 @@ struct CrtConst {
     uint2 k2;        // 2^32 N^-1 (p0 p1)^-1       mod p2
     uint2 c20;       // (p0 p1)^-1                 mod p2
     uint2 c21;       // p1^-1                      mod p2
+    uint2 e[kPrimes];       // 2^32 N^-1 M_i^-1 mod p_i, with its Shoup quotient
+    uint32_t mlo[kPrimes];  // M_i mod 2^32
+    uint32_t mhi[kPrimes];  // M_i div 2^32
+    float inv[kPrimes];     // 1 / p_i rounded to single precision
+    uint32_t negm[3];       // 2^96 - M, least significant word first
 };
 
 constexpr int kElemsPerThread = 16;
@@ -99,7 +108,8 @@ struct CarryJob {           // CRT + carry of one group
     uint32_t blk0;                    // first carry block of the group in this launch
     uint32_t* out;                    // product words
     uint64_t nwords;
-    uint32_t* maps;                   // one carry map per carry workgroup of the group
+    uint32_t* maps;                   // one carry map per carry block of the group; the groups of a
+                                      // launch have theirs one after another (maps - blk0 is the same)
 };
 
 // Row/column split and tile shapes for a transform of length 2^n.

@@ -123,9 +123,13 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = pad(e * TL::NT + threadIdx.x);
         uint32_t x = ldsA[idx];
-        uint32_t y = KIND == kRowSquare ? x : ldsB[idx];
-        ldsA[idx] = Arith<SET>::pointwise(x, y, p, pinv);
-        if constexpr (kPair) ldsC[idx] = Arith<SET>::pointwise(ldsC[idx], y, p, pinv);
+        if constexpr (KIND == kRowSquare) {
+            ldsA[idx] = Arith<SET>::pointwise(x, x, p, pinv);
+        } else {
+            const uint32_t ys = Arith<SET>::shared_operand(ldsB[idx], p);
+            ldsA[idx] = Arith<SET>::pointwise_shared(x, ys, p, pinv);
+            if constexpr (kPair) ldsC[idx] = Arith<SET>::pointwise_shared(ldsC[idx], ys, p, pinv);
+        }
     }
     __syncthreads();
     const auto zi = PINTT_INV_TABLE(tw, prime);
@@ -345,10 +349,16 @@ __global__ void __launch_bounds__(256) k_row_reg(const JobList<RowJob> jobs, int
             for (int t = 0; t < NA; ++t) v[t][i] = lds[t][rw][RG::lds_at(RG::b_entry(i, lane, wr))];
         row_stage_b<LC, SET, NA, false, kB0B>(v, fB, p);
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
+        for (int r = 0; r < 16; ++r) {
+            if constexpr (KIND == kRowSquare) {
 #pragma unroll
-            for (int t = 0; t < NA; ++t)
-                v[t][r] = Arith<SET>::pointwise(v[t][r], KIND == kRowSquare ? v[t][r] : vb[0][r], p, pinv);
+                for (int t = 0; t < NA; ++t) v[t][r] = Arith<SET>::pointwise(v[t][r], v[t][r], p, pinv);
+            } else {
+                const uint32_t ys = Arith<SET>::shared_operand(vb[0][r], p);
+#pragma unroll
+                for (int t = 0; t < NA; ++t) v[t][r] = Arith<SET>::pointwise_shared(v[t][r], ys, p, pinv);
+            }
+        }
         row_stage_b<LC, SET, NA, true, kB0B>(v, iB, p);
 #pragma unroll
         for (int i = 0; i < 16; ++i)

@@ -42,6 +42,10 @@ class Oracle:
             f.argtypes = [_pu32, _pu32, ctypes.c_uint32, ctypes.c_uint32]
         L.orc_narrow_pointwise.restype = ctypes.c_uint32
         L.orc_narrow_pointwise.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_narrow_pointwise_shared.restype = ctypes.c_uint32
+        L.orc_narrow_pointwise_shared.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_narrow_shared_operand.restype = ctypes.c_uint32
+        L.orc_narrow_shared_operand.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.orc_narrow_load.restype = ctypes.c_uint32
         L.orc_narrow_load.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         for f in (L.orc_narrow_ct_layers, L.orc_narrow_gs_layers):

@@ -80,6 +80,27 @@ def test_pointwise_and_load(oracle, prime):
 
 
 @pytest.mark.parametrize("prime", NARROW)
+def test_pointwise_with_shared_operand(oracle, prime):
+    """b of a0 b and a1 b reduced once to [0, 2p), each a only to [0, 4p)."""
+    p = oracle.prime(prime)
+    L = oracle.L
+    r_inv = pow(2**32, p - 2, p)
+    rng = random.Random(40 + prime)
+    top = _edges(p, 8 * p) + [8 * p - 1]
+    pairs = [(x, y) for x in top for y in top] + [(rng.randrange(8 * p), rng.randrange(8 * p)) for _ in range(20000)]
+    for x, y in pairs:
+        ys = L.orc_narrow_shared_operand(y, p)
+        assert ys < 2 * p and (ys - y) % p == 0, y
+        r = L.orc_narrow_pointwise_shared(x, ys, p)
+        assert r < 2 * p and (r - x * y * r_inv) % p == 0, (x, y)
+        assert r == L.orc_narrow_pointwise(x, y, p) or (r - L.orc_narrow_pointwise(x, y, p)) % p == 0
+    # out of contract: a shared operand left at or above 2p breaks the range
+    bad = sum(L.orc_narrow_pointwise_shared(x, y, p) >= 2 * p
+              for x in range(4 * p - 4000, 4 * p) for y in (8 * p - 1, 6 * p + 5))
+    assert bad > 0
+
+
+@pytest.mark.parametrize("prime", NARROW)
 @pytest.mark.parametrize("log2n,split", [(1, 1), (5, 2), (10, 4), (12, 5)])
 def test_lazy_transforms_match_strict(oracle, prime, log2n, split):
     p = oracle.prime(prime)
