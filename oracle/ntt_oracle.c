@@ -176,13 +176,25 @@ void orc_narrow_ct(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p) {
 
 void orc_narrow_ct_lazy(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p) { narrow_ct_core(*a, a, b, w, p); }
 
+/* a, b in [0, 4p): the sum reduced from [0, 8p) to [0, 4p); a - b + 4p in
+ * (0, 8p) by Shoup to [0, 2p). */
 void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p) {
-    const uint32_t p2 = 2u * p, wq = orc_shoup_quotient(winv, p);
+    const uint32_t p4 = 4u * p, wq = orc_shoup_quotient(winv, p);
     const uint32_t u = *a, v = *b;
     const uint32_t s = u + v;
-    const uint32_t d = u - v + p2;
+    const uint32_t d = u - v + p4;
     const uint32_t q = (uint32_t)(((uint64_t)d * wq) >> 32);
-    *a = min_u32(s, s - p2);
+    *a = min_u32(s, s - p4);
+    *b = d * winv - q * p;
+}
+
+/* a, b in [0, 2p): the sum kept in [0, 4p); the difference as above. */
+void orc_narrow_gs_lazy(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p) {
+    const uint32_t wq = orc_shoup_quotient(winv, p);
+    const uint32_t u = *a, v = *b;
+    const uint32_t d = u - v + 4u * p;
+    const uint32_t q = (uint32_t)(((uint64_t)d * wq) >> 32);
+    *a = u + v;
     *b = d * winv - q * p;
 }
 
@@ -244,12 +256,26 @@ void orc_narrow_gs_layers(uint32_t* x, int log2n, int prime, int s_lo, int s_hi)
     uint32_t* z = (uint32_t*)malloc(n * sizeof(uint32_t));
     uint32_t* zi = (uint32_t*)malloc(n * sizeof(uint32_t));
     orc_zeta_table(prime, log2n, z, zi);
+    /* Bound of every value in units of p: 2 while it is a difference (or an
+     * input below 2p), 4 otherwise; a butterfly of two values below 2p keeps
+     * its sum unreduced. */
+    unsigned char* bnd = (unsigned char*)malloc(n);
+    for (size_t i = 0; i < n; ++i) bnd[i] = x[i] < 2u * p ? 2 : 4;
     for (int s = s_hi - 1; s >= s_lo; --s) {
         size_t half = n >> (s + 1);
         for (size_t b = 0; b < ((size_t)1 << s); ++b)
-            for (size_t j = 0; j < half; ++j)
-                orc_narrow_gs(&x[b * 2 * half + j], &x[b * 2 * half + j + half], zi[((size_t)1 << s) + b], p);
+            for (size_t j = 0; j < half; ++j) {
+                const size_t iu = b * 2 * half + j, iv = iu + half;
+                if (bnd[iu] == 2 && bnd[iv] == 2) {
+                    orc_narrow_gs_lazy(&x[iu], &x[iv], zi[((size_t)1 << s) + b], p);
+                } else {
+                    orc_narrow_gs(&x[iu], &x[iv], zi[((size_t)1 << s) + b], p);
+                }
+                bnd[iu] = 4;
+                bnd[iv] = 2;
+            }
     }
+    free(bnd);
     free(z);
     free(zi);
 }

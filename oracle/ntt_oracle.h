@@ -74,18 +74,21 @@ void orc_crt3(const uint32_t r[3], uint32_t out[3]);
  * floor(w 2^32 / p).  ct: a in [0, 8p), any b -> (a + w b, a - w b) mod p,
  * both in [0, 6p) (a reduced to [0, 4p) first).  ct_lazy: a in [0, 6p), any b
  * -> the same congruences, both in [0, 8p) (no reduction).  gs: a, b in
- * [0, 2p) -> (a + b, (a - b) winv) mod p, both in [0, 2p).  pointwise: x, y in
+ * [0, 4p) -> (a + b, (a - b) winv) mod p in [0, 4p) and [0, 2p).  gs_lazy:
+ * a, b in [0, 2p) -> the same, the sum unreduced in [0, 4p).  pointwise: x, y in
  * [0, 8p) -> x y 2^-32 mod p in [0, 2p).  shared_operand: y in [0, 8p) ->
  * y mod p in [0, 2p); pointwise_shared: x in [0, 8p), ys in [0, 2p) -> x ys
  * 2^-32 mod p in [0, 2p) (x reduced only to [0, 4p)).  load: any 32-bit c -> c mod p in
  * [0, 8p).  The layer functions run whole layers as orc_ntt_ct_layers /
  * orc_ntt_gs_layers do, with unreduced values; the forward ones start below
  * 8p and skip the reduction while every value is below 6p (every other layer),
- * as the device's register stages do. */
+ * as the device's register stages do; the inverse ones take inputs below 4p
+ * and keep the sum of two values below 2p unreduced. */
 uint32_t orc_shoup_quotient(uint32_t w, uint32_t p);
 void orc_narrow_ct(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p);
 void orc_narrow_ct_lazy(uint32_t* a, uint32_t* b, uint32_t w, uint32_t p);
 void orc_narrow_gs(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p);
+void orc_narrow_gs_lazy(uint32_t* a, uint32_t* b, uint32_t winv, uint32_t p);
 uint32_t orc_narrow_pointwise(uint32_t x, uint32_t y, uint32_t p);
 uint32_t orc_narrow_shared_operand(uint32_t y, uint32_t p);
 uint32_t orc_narrow_pointwise_shared(uint32_t x, uint32_t ys, uint32_t p);

@@ -6,7 +6,8 @@
 // serves transforms up to 2^22 points with Harvey's lazy butterflies: 8p still
 // fits a 32-bit lane, so forward values live in [0, 8p) -- a forward butterfly
 // reduces its first input only when it may exceed 6p, every other layer -- and
-// inverse values in [0, 2p).  Their product, 2^85.93, exceeds every
+// inverse values in [0, 4p) -- an inverse butterfly reduces its sum only when
+// an input may exceed 2p.  Their product, 2^85.93, exceeds every
 // coefficient of a product the set serves: a coefficient sums at most
 // min(la, lb) <= 2^21 terms below 2^64 when la + lb - 1 <= 2^22.
 // Twiddle products use Shoup's precomputed quotient (w' = floor(w * 2^32 / p)):
@@ -111,6 +112,9 @@ struct Arith<kSetWide> {
         a = add_mod(u, v, p);
         b = mul_shoup(u + p - v, zi.x, zi.y, p);
     }
+    __device__ static __forceinline__ void gs_sel(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p, bool) {
+        gs(a, b, zi, p);
+    }
     // Forward outputs (in [0, p)) to the inverse input range, multiplied.
     __device__ static __forceinline__ uint32_t pointwise(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) {
         return mul_mont(x, y, p, pinv);
@@ -127,7 +131,7 @@ struct Arith<kSetWide> {
     }
 };
 
-// Narrow set (p < 2^29): forward values in [0, 8p), inverse values in [0, 2p).
+// Narrow set (p < 2^29): forward values in [0, 8p), inverse values in [0, 4p).
 template <>
 struct Arith<kSetNarrow> {
     // Any 32-bit coefficient to [0, 8p): 2^32 < 16p, so one step suffices.
@@ -155,16 +159,30 @@ struct Arith<kSetNarrow> {
         if (lazy) ct_lazy(a, b, z, p);
         else ct(a, b, z, p);
     }
-    // a, b in [0, 2p): a + b in [0, 4p) reduced to [0, 2p); (a - b + 2p) in
-    // (0, 4p) times zinv by Shoup, in [0, 2p).
+    // a, b in [0, 4p): a + b in [0, 8p) reduced to [0, 4p); (a - b + 4p) in
+    // (0, 8p) times zinv by Shoup, in [0, 2p).
     __device__ static __forceinline__ void gs(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p) {
-        const uint32_t p2 = 2u * p;
+        const uint32_t p4 = 4u * p;
         const uint32_t u = a, v = b;
         const uint32_t s = u + v;
-        a = min(s, s - p2);
-        const uint32_t d = u - v + p2;
+        a = min(s, s - p4);
+        const uint32_t d = u - v + p4;
         const uint32_t q = __umulhi(d, zi.y);
         b = d * zi.x - q * p;
+    }
+    // a, b in [0, 2p): a + b in [0, 4p) kept as it is; b as in gs.
+    __device__ static __forceinline__ void gs_lazy(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p) {
+        const uint32_t u = a, v = b;
+        a = u + v;
+        const uint32_t d = u - v + 4u * p;
+        const uint32_t q = __umulhi(d, zi.y);
+        b = d * zi.x - q * p;
+    }
+    // lazy: the caller knows both inputs are below 2p (a compile-time constant
+    // after unrolling).
+    __device__ static __forceinline__ void gs_sel(uint32_t& a, uint32_t& b, uint2 zi, uint32_t p, bool lazy) {
+        if (lazy) gs_lazy(a, b, zi, p);
+        else gs(a, b, zi, p);
     }
     // x, y in [0, 8p) reduced to [0, 2p); their product is below 4p^2 < 2^32 p,
     // so the Montgomery result (t + m p) / 2^32 lies in [0, 2p) unreduced.
@@ -193,7 +211,7 @@ struct Arith<kSetNarrow> {
         return (uint32_t)((t + (uint64_t)m * p) >> 32);
     }
     // x * f 2^-32 (mul_fixed) in (0, 2p): inside both the forward [0, 8p) and
-    // the inverse [0, 2p) input ranges.
+    // the inverse [0, 4p) input ranges.
     __device__ static __forceinline__ uint32_t scale(uint32_t x, uint32_t f, uint32_t fp, uint32_t p) {
         return mul_fixed(x, f, fp, p);
     }

@@ -134,8 +134,17 @@ __device__ __forceinline__ void ct6(uint32_t (&v)[NB][16], const TW& tw, uint32_
     }
 }
 
+// Inverse values (narrow set) stay below 4p; a butterfly skips the reduction
+// of its sum when both inputs are below 2p.  Inputs of the stage are below 2p
+// when FRESH (pointwise products, scaled rows), else below 4p.  After a layer
+// the differences (the register of the pair with the layer's bit set) are
+// below 2p and the sums below 4p, so the next layer of the same registers
+// pairs two differences exactly where the previous layer's bit is set in the
+// register index; a lane swap mixes sums and differences of one register
+// across lanes, so the layer after it reduces everywhere.
+//
 // The six layers undone: post layout in, entry layout out.
-template <int SET, int NB, int SB = 1, class TW>
+template <int SET, int NB, int SB = 1, bool FRESH = false, class TW>
 __device__ __forceinline__ void gs6(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
     const uint32_t lane = lane_id();
 #pragma unroll
@@ -143,8 +152,10 @@ __device__ __forceinline__ void gs6(uint32_t (&v)[NB][16], const TW& tw, uint32_
         if (r & (1 << SB)) continue;
         const uint2 z = tw(5, post_block<SB>(5, r, lane));
 #pragma unroll
-        for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][r], v[u][r | (1 << SB)], z, p);
+        for (int u = 0; u < NB; ++u) Arith<SET>::gs_sel(v[u][r], v[u][r | (1 << SB)], z, p, FRESH);
     }
+    // Layer 4 pairs registers with the same bit SB (layer 5's): differences
+    // of layer 5 where it is set.
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         if (r & 8) continue;
@@ -154,10 +165,12 @@ __device__ __forceinline__ void gs6(uint32_t (&v)[NB][16], const TW& tw, uint32_
         for (int q = 0; q < 2; ++q) {
             const int rr = r | (q << SB);
 #pragma unroll
-            for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][rr], v[u][rr | 8], z, p);
+            for (int u = 0; u < NB; ++u) Arith<SET>::gs_sel(v[u][rr], v[u][rr | 8], z, p, q == 1);
         }
     }
     lane_swaps<NB, SB>(v);
+    // Layer l pairs register bit log2(h); the layer before paired bit
+    // log2(h) - 1, set in x where x & (h >> 1).
 #pragma unroll
     for (int l = 3; l >= 0; --l) {
         const int h = 8 >> l;
@@ -167,7 +180,9 @@ __device__ __forceinline__ void gs6(uint32_t (&v)[NB][16], const TW& tw, uint32_
 #pragma unroll
             for (int x = 0; x < h; ++x)
 #pragma unroll
-                for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p);
+                for (int u = 0; u < NB; ++u)
+                    Arith<SET>::gs_sel(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p,
+                                       l < 3 && (x & (h >> 1)) != 0);
         }
     }
 }
@@ -200,14 +215,15 @@ __device__ __forceinline__ void ct5(uint32_t (&v)[NB][16], const TW& tw, uint32_
     }
 }
 
-template <int SET, int NB, class TW>
+// Bounds as in gs6.
+template <int SET, int NB, bool FRESH = false, class TW>
 __device__ __forceinline__ void gs5(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
     const uint32_t l5 = (lane_id() >> 5) & 1u;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const uint2 z = tw(4, (l5 << 3) | (uint32_t)r);
 #pragma unroll
-        for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][r], v[u][r | 8], z, p);
+        for (int u = 0; u < NB; ++u) Arith<SET>::gs_sel(v[u][r], v[u][r | 8], z, p, FRESH);
     }
     swap32<NB>(v);
 #pragma unroll
@@ -219,7 +235,9 @@ __device__ __forceinline__ void gs5(uint32_t (&v)[NB][16], const TW& tw, uint32_
 #pragma unroll
             for (int x = 0; x < h; ++x)
 #pragma unroll
-                for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p);
+                for (int u = 0; u < NB; ++u)
+                    Arith<SET>::gs_sel(v[u][sub * 2 * h + x], v[u][sub * 2 * h + x + h], z, p,
+                                       l < 3 && (x & (h >> 1)) != 0);
         }
     }
 }
@@ -245,7 +263,9 @@ __device__ __forceinline__ void ct_lane(uint32_t (&v)[16], const TW& tw, uint32_
         }
 }
 
-template <int SET, int K, class TW>
+// Bounds as in gs6: the first layer is lazy when FRESH, every later one where
+// the previous layer's bit is set.
+template <int SET, int K, bool FRESH = false, class TW>
 __device__ __forceinline__ void gs_lane(uint32_t (&v)[16], const TW& tw, uint32_t p) {
     static_assert(K >= 1 && K <= 4, "in-lane stage of 1..4 layers");
 #pragma unroll
@@ -258,7 +278,8 @@ __device__ __forceinline__ void gs_lane(uint32_t (&v)[16], const TW& tw, uint32_
                 const uint2 z = tw(g, l, sub);
 #pragma unroll
                 for (int x = 0; x < h; ++x)
-                    Arith<SET>::gs(v[(g << K) + sub * 2 * h + x], v[(g << K) + sub * 2 * h + x + h], z, p);
+                    Arith<SET>::gs_sel(v[(g << K) + sub * 2 * h + x], v[(g << K) + sub * 2 * h + x + h], z, p,
+                                       l == K - 1 ? FRESH : (x & (h >> 1)) != 0);
             }
         }
 }

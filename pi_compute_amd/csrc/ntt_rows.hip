@@ -213,17 +213,18 @@ struct RegRows {
 // forward values enter below B0 p.
 template <int LC, int SET, int NB, bool INV, int B0, class TW>
 __device__ __forceinline__ void row_stage_b(uint32_t (&v)[NB][16], const TW& tw, uint32_t p) {
+    // The inverse stage B takes the pointwise products, below 2p.
     if constexpr (LC == 12) {
-        if constexpr (INV) gs6<SET, NB, 1>(v, tw, p);
+        if constexpr (INV) gs6<SET, NB, 1, true>(v, tw, p);
         else ct6<SET, NB, false, 1, B0>(v, tw, p);
     } else if constexpr (LC == 11) {
-        if constexpr (INV) gs5<SET, NB>(v, tw, p);
+        if constexpr (INV) gs5<SET, NB, true>(v, tw, p);
         else ct5<SET, NB, B0>(v, tw, p);
     } else {
         auto tl = [&](int, int l, uint32_t blk) { return tw(l, blk); };
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
-            if constexpr (INV) gs_lane<SET, 4>(v[u], tl, p);
+            if constexpr (INV) gs_lane<SET, 4, true>(v[u], tl, p);
             else ct_lane<SET, 4, B0>(v[u], tl, p);
         }
     }

@@ -38,7 +38,7 @@ class Oracle:
         L.orc_shoup_quotient.restype = ctypes.c_uint32
         L.orc_shoup_quotient.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         _pu32 = ctypes.POINTER(ctypes.c_uint32)
-        for f in (L.orc_narrow_ct, L.orc_narrow_ct_lazy, L.orc_narrow_gs):
+        for f in (L.orc_narrow_ct, L.orc_narrow_ct_lazy, L.orc_narrow_gs, L.orc_narrow_gs_lazy):
             f.argtypes = [_pu32, _pu32, ctypes.c_uint32, ctypes.c_uint32]
         L.orc_narrow_pointwise.restype = ctypes.c_uint32
         L.orc_narrow_pointwise.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
@@ -167,6 +167,11 @@ class Oracle:
     def narrow_gs(self, a, b, winv, p):
         x, y = ctypes.c_uint32(a), ctypes.c_uint32(b)
         self.L.orc_narrow_gs(ctypes.byref(x), ctypes.byref(y), winv, p)
+        return x.value, y.value
+
+    def narrow_gs_lazy(self, a, b, winv, p):
+        x, y = ctypes.c_uint32(a), ctypes.c_uint32(b)
+        self.L.orc_narrow_gs_lazy(ctypes.byref(x), ctypes.byref(y), winv, p)
         return x.value, y.value
 
     def narrow_layers(self, x, log2n, prime, lo, hi, inverse=False):

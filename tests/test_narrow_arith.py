@@ -3,7 +3,8 @@
 Transforms of up to 2^22 points run over three primes below 2^29, so 8p fits
 a 32-bit word: forward values may stay in [0, 8p) -- a forward butterfly
 reduces its first input to [0, 4p) only when it may be 6p or more, every other
-layer -- and inverse values in [0, 2p).  The checks here use only the
+layer -- and inverse values in [0, 4p) (the sum of two values below 2p kept
+unreduced).  The checks here use only the
 definitions (congruences computed with Python integers, ranges) and the strict
 restatement, which the textbook DFT pins:
   * every butterfly output lies in its range and is congruent to the exact
@@ -51,15 +52,20 @@ def test_ct_butterfly_range_and_congruence(oracle, prime, lazy):
 
 
 @pytest.mark.parametrize("prime", NARROW)
-def test_gs_butterfly_range_and_congruence(oracle, prime):
+@pytest.mark.parametrize("lazy", [False, True], ids=["reducing", "lazy"])
+def test_gs_butterfly_range_and_congruence(oracle, prime, lazy):
+    # reducing: a, b < 4p; lazy: a, b < 2p.  Either way the sum lands below
+    # 4p and the difference times winv below 2p.
     p = oracle.prime(prime)
-    rng = random.Random(10 + prime)
+    rng = random.Random(10 + prime + (100 if lazy else 0))
     ws = _twiddles(oracle, prime, rng)
-    cases = [(a, b, w) for a in _edges(p, 2 * p) for b in _edges(p, 2 * p) for w in ws]
-    cases += [(rng.randrange(2 * p), rng.randrange(2 * p), rng.randrange(p)) for _ in range(20000)]
+    top = 2 * p if lazy else 4 * p
+    f = oracle.narrow_gs_lazy if lazy else oracle.narrow_gs
+    cases = [(a, b, w) for a in _edges(p, top) + [top - 1] for b in _edges(p, top) + [top - 1] for w in ws]
+    cases += [(rng.randrange(top), rng.randrange(top), rng.randrange(p)) for _ in range(20000)]
     for a, b, w in cases:
-        A, B = oracle.narrow_gs(a, b, w, p)
-        assert A < 2 * p and B < 2 * p, (a, b, w)
+        A, B = f(a, b, w, p)
+        assert A < 4 * p and B < 2 * p, (a, b, w)
         assert (A - a - b) % p == 0 and (B - (a - b) * w) % p == 0, (a, b, w)
 
 
@@ -115,18 +121,22 @@ def test_lazy_transforms_match_strict(oracle, prime, log2n, split):
     assert int(lazy.max()) < 8 * p
     assert np.array_equal(lazy.astype(np.uint64) % p, strict.astype(np.uint64))
 
-    y = rng.integers(0, 2 * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
-    strict_inv = oracle.gs_layers((y.astype(np.uint64) % p).astype(np.uint32), log2n, prime, 0, log2n)
-    rows = oracle.narrow_layers(y, log2n, prime, split, log2n, inverse=True)
-    assert int(rows.max()) < 2 * p
-    lazy_inv = oracle.narrow_layers(rows, log2n, prime, 0, split, inverse=True)
-    assert int(lazy_inv.max()) < 2 * p
-    assert np.array_equal(lazy_inv.astype(np.uint64) % p, strict_inv.astype(np.uint64))
+    # inverse: rows from pointwise products (below 2p), then columns from
+    # whatever the rows stored (below 4p); also a pass started below 4p
+    for top in (2, 4):
+        y = rng.integers(0, top * p, 1 << log2n, dtype=np.uint64).astype(np.uint32)
+        strict_inv = oracle.gs_layers((y.astype(np.uint64) % p).astype(np.uint32), log2n, prime, 0, log2n)
+        rows = oracle.narrow_layers(y, log2n, prime, split, log2n, inverse=True)
+        assert int(rows.max()) < 4 * p
+        lazy_inv = oracle.narrow_layers(rows, log2n, prime, 0, split, inverse=True)
+        assert int(lazy_inv.max()) < 4 * p
+        assert np.array_equal(lazy_inv.astype(np.uint64) % p, strict_inv.astype(np.uint64))
 
 
 def test_out_of_contract_inputs_are_caught(oracle):
     """The range checks have teeth: a reducing butterfly's first input at or
-    above 8p, or a lazy one's at or above 6p, breaks them."""
+    above 8p, or a lazy one's at or above 6p, breaks them; so does a lazy
+    inverse butterfly given a sum of two values at or above 2p."""
     p = oracle.prime(3)
     bad = 0
     for a in range(8 * p, 2**32, 7919 * 16):
@@ -138,5 +148,11 @@ def test_out_of_contract_inputs_are_caught(oracle):
     for a in range(6 * p, 2**32, 7919 * 16):
         A, B = oracle.narrow_ct_lazy(a, 2 * p - 1, 1, p)
         if not (A < 8 * p and B < 8 * p and (A - a - 2 * p + 1) % p == 0 and (B - a + 2 * p - 1) % p == 0):
+            bad += 1
+    assert bad > 0
+    bad = 0
+    for a in range(2 * p, 4 * p, 7919 * 8):
+        A, B = oracle.narrow_gs_lazy(a, 4 * p - 1 - (a - 2 * p), 1, p)
+        if not (A < 4 * p and B < 2 * p):
             bad += 1
     assert bad > 0

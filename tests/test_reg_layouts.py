@@ -295,3 +295,108 @@ def test_col_reg_layouts():
             rows = {(entry_t(reg, l2) << (LR - 6)) | (g >> 3) for reg in range(16)
                     for l2 in (lane & 15, (lane & 15) | 16, (lane & 15) | 32, (lane & 15) | 48)}
             assert rows == {(t << (LR - 6)) | (g >> 3) for t in range(64)}
+
+
+# Inverse bounds (narrow set): values below 4p, a butterfly may skip the
+# reduction of its sum only when both inputs are below 2p.  The predicates
+# below are the ones gs6 / gs5 / gs_lane evaluate while unrolling; the model
+# carries each (register, lane) value's bound (in units of p) through the
+# butterflies and the lane swaps and checks every lazy choice.
+def _gs_butterfly(bnd, ra, rb, lazy):
+    a, b = bnd[ra], bnd[rb]
+    assert (a <= 4).all() and (b <= 4).all()
+    if lazy:
+        assert (a <= 2).all() and (b <= 2).all(), (ra, rb)
+        bnd[ra] = a + b
+    else:
+        bnd[ra] = np.full(64, 4)
+    bnd[rb] = np.full(64, 2)
+    return 1 if lazy else 0
+
+
+def _gs_post_swap_layers(bnd):
+    lazy = 0
+    for l in range(3, -1, -1):
+        h = 8 >> l
+        for sub in range(1 << l):
+            for x in range(h):
+                lazy += _gs_butterfly(bnd, sub * 2 * h + x, sub * 2 * h + x + h, l < 3 and (x & (h >> 1)) != 0)
+    return lazy
+
+
+def _gs6_bounds(sb, fresh):
+    bnd = [np.full(64, 2 if fresh else 4) for _ in range(16)]
+    lazy = 0
+    for r in range(16):
+        if r & (1 << sb):
+            continue
+        lazy += _gs_butterfly(bnd, r, r | (1 << sb), fresh)
+    for r in range(16):
+        if r & 8 or r & (1 << sb):
+            continue
+        for q in (0, 1):
+            rr = r | (q << sb)
+            lazy += _gs_butterfly(bnd, rr, rr | 8, q == 1)
+    bnd = lane_swaps(bnd, sb)
+    lazy += _gs_post_swap_layers(bnd)
+    return bnd, lazy
+
+
+def _gs5_bounds(fresh):
+    bnd = [np.full(64, 2 if fresh else 4) for _ in range(16)]
+    lazy = 0
+    for r in range(8):
+        lazy += _gs_butterfly(bnd, r, r | 8, fresh)
+    bnd = lane_swaps(bnd, with16=False)
+    lazy += _gs_post_swap_layers(bnd)
+    return bnd, lazy
+
+
+def _gs_lane_bounds(K, fresh):
+    bnd = [np.full(64, 2 if fresh else 4) for _ in range(16)]
+    lazy = 0
+    for g in range(16 >> K):
+        for l in range(K - 1, -1, -1):
+            h = (1 << K) >> (l + 1)
+            for sub in range(1 << l):
+                for x in range(h):
+                    ra = (g << K) + sub * 2 * h + x
+                    lazy += _gs_butterfly(bnd, ra, ra + h, fresh if l == K - 1 else (x & (h >> 1)) != 0)
+    return bnd, lazy
+
+
+def test_lazy_inverse_choices_hold_their_bounds():
+    for sb in (1, 2):
+        for fresh in (False, True):
+            bnd, lazy = _gs6_bounds(sb, fresh)
+            assert all((b <= 4).all() for b in bnd)
+            assert lazy == (24 if fresh else 16), (sb, fresh, lazy)
+    for fresh in (False, True):
+        bnd, lazy = _gs5_bounds(fresh)
+        assert all((b <= 4).all() for b in bnd)
+        assert lazy == (20 if fresh else 12)
+    for K in (1, 2, 3, 4):
+        for fresh in (False, True):
+            bnd, lazy = _gs_lane_bounds(K, fresh)
+            assert all((b <= 4).all() for b in bnd)
+
+
+def test_lazy_inverse_model_catches_a_wrong_choice():
+    """The first layer after a lane swap mixes sums and differences of one
+    register across lanes: calling it lazy must fail the bound check."""
+    bnd = [np.full(64, 2) for _ in range(16)]
+    for r in range(16):
+        if r & 2:
+            continue
+        _gs_butterfly(bnd, r, r | 2, True)
+    for r in range(16):
+        if r & 8 or r & 2:
+            continue
+        for q in (0, 1):
+            _gs_butterfly(bnd, r | (q << 1), r | (q << 1) | 8, q == 1)
+    bnd = lane_swaps(bnd, 1)
+    try:
+        _gs_butterfly(bnd, 0, 1, True)
+    except AssertionError:
+        return
+    raise AssertionError("a lazy butterfly on mixed bounds went unnoticed")
