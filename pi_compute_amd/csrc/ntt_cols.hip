@@ -83,12 +83,13 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 }
 
 // Column passes with register-resident stages (ntt_regs.h) for tiles of 2^LR
-// rows (LR = 9, 10, 11) by 8 columns, 2^(LR - 1) threads.  Row r of column w:
+// rows (LR = 7 .. 11) by 8 columns, 2^(LR - 1) threads.  Row r of column w:
 // stage A runs layers 0..5 on r's top six bits for the groups g = (r_low, w)
 // (r_low = r's low LR - 6 bits; g = (wave << 4) | (lane & 15), w = g & 7);
 // stage B runs the last LR - 6 layers on the groups (r >> (LR - 6), w): inside
-// each lane for 3 or 4 layers, as a five-layer lane-swap stage (lanes l and
-// l + 32, row bit 0 on lane bit 5) for LR = 11.  Global memory is read and
+// each lane for 1 to 4 layers (a lane holds 16 consecutive rows of one column,
+// 16 >> (LR - 6) groups), as a five-layer lane-swap stage (lanes l and l + 32,
+// row bit 0 on lane bit 5) for LR = 11.  Global memory is read and
 // written in the stage A entry layout (forward) or the stage B layout
 // (inverse) as whole 32-byte column rows; LDS (word k = (r << 3) | w, padded
 // by S words per 2^PSH) carries the one exchange between the stages, at 32
@@ -96,10 +97,11 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
 // stage B groups chosen per shape; tests/test_reg_layouts.py checks them).
 template <int LR>
 struct RegCols {
-    static_assert(LR >= 9 && LR <= 11, "register column tiles: 2^9 .. 2^11 rows");
+    static_assert(LR >= 7 && LR <= 11, "register column tiles: 2^7 .. 2^11 rows");
     static constexpr int NT = 1 << (LR - 1);
     static constexpr int KB = LR - 6;  // stage B layers
-    static constexpr uint32_t S = LR == 9 ? 2u : 1u;
+    static constexpr int SB = LR == 7 ? 2 : 1;  // stage A lane swap: register bit SB <-> lane bit 4
+    static constexpr uint32_t S = LR <= 8 ? 4u : (LR == 9 ? 2u : 1u);
     static constexpr int PSH = LR == 11 ? 7 : 6;
     static constexpr int WORDS = (8 << LR) + (int)S * ((8 << LR) >> PSH);
     __device__ static __forceinline__ uint32_t lds_at(uint32_t r, uint32_t w) {
@@ -111,18 +113,21 @@ struct RegCols {
         return (entry_t(i, lane) << (LR - 6)) | rlow;
     }
     __device__ static __forceinline__ uint32_t row_post(int i, uint32_t lane, uint32_t rlow) {
-        return (post_t(i, lane) << (LR - 6)) | rlow;
+        return (post_t<SB>(i, lane) << (LR - 6)) | rlow;
     }
-    // Stage B: the lane's group is hi = r >> 4 (LR = 9, 10; two groups per
-    // lane for LR = 9, told apart by register bit 3) or r >> 5 (LR = 11).
+    // Stage B: the lane's rows are hi = r >> 4 (LR <= 10; 16 >> (LR - 6)
+    // groups per lane, told apart by the register bits above LR - 6) or
+    // r >> 5 (LR = 11).
     __device__ static __forceinline__ uint32_t hi_bits(uint32_t lane, uint32_t wave) {
         const uint32_t l3 = (lane >> 3) & 1u, l4 = (lane >> 4) & 1u, l5 = (lane >> 5) & 1u;
         if constexpr (LR == 11)  // r5, r6 = wave0, wave1; r7 = L3; r8 = L4; r9, r10 = wave2, wave3 (r0 = L5)
             return (wave & 3u) | (l3 << 2) | (l4 << 3) | ((wave >> 2) << 4);
         else if constexpr (LR == 10)  // r4 = L5, r5 = wave0, r6 = L3, r7 = L4, r8, r9 = wave1, wave2
             return l5 | ((wave & 1u) << 1) | (l3 << 2) | (l4 << 3) | ((wave >> 1) << 4);
-        else  // r4 = L5, r5 = L3, r6 = L4, r7, r8 = wave0, wave1 (r3 = register bit 3)
+        else if constexpr (LR == 9)  // r4 = L5, r5 = L3, r6 = L4, r7, r8 = wave0, wave1 (r3 = register bit 3)
             return l5 | (l3 << 1) | (l4 << 2) | (wave << 3);
+        else  // LR = 7, 8: r4 = L3, r5 = L4, r6 = L5, r7 = wave0
+            return l3 | (l4 << 1) | (l5 << 2) | (wave << 3);
     }
     // Stage B row of register i, in the entry layout and (LR = 11) the post one.
     __device__ static __forceinline__ uint32_t row_b(int i, uint32_t lane, uint32_t hi) {
@@ -137,14 +142,14 @@ struct RegCols {
 };
 
 // Stage B twiddles: at layer 6 + l, block (outer << l) | blk with outer the
-// group's block at layer 6: hi (LR = 10, 11) or (hi << 1) | g (LR = 9, g =
-// register bit 3).
+// group's block at layer 6, r >> KB: hi (LR = 10, 11) or (hi << (4 - KB)) | g
+// (LR = 7 .. 9, g = the register bits above KB).
 struct TwColB {
     ConstTable z;
     uint32_t hi;
     int KB;
     __device__ __forceinline__ uint2 operator()(int g, int l, uint32_t blk) const {
-        const uint32_t outer = KB == 3 ? ((hi << 1) | (uint32_t)g) : hi;
+        const uint32_t outer = KB >= 4 ? hi : ((hi << (4 - KB)) | (uint32_t)g);
         return tw_at(z, (1u << (6 + l)) + (outer << l) + blk);
     }
     __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const { return (*this)(0, l, blk); }
@@ -189,14 +194,14 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_fwd_reg(const JobList<F
             const uint32_t k = (RC::row_entry(i, lane, rlow) << log2c) + c0 + w;
             v[0][i] = Arith<SET>::load_coeff(k < job.ncoef ? src[k] : 0u, p);
         }
-        ct6<SET, 1, true>(v, TwColA{as_const(z)}, p);
+        ct6<SET, 1, true, RC::SB>(v, TwColA{as_const(z)}, p);
     } else {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const uint32_t k = (RC::row_entry(i, lane, rlow) << log2c) + c0 + w;
             v[0][i] = Arith<SET>::load_coeff(k < job.ncoef ? src[k] : 0u, p);
         }
-        ct6<SET, 1>(v, TwColA{as_const(z)}, p);
+        ct6<SET, 1, false, RC::SB>(v, TwColA{as_const(z)}, p);
     }
 #pragma unroll
     for (int i = 0; i < 16; ++i) lds[RC::lds_at(RC::row_post(i, lane, rlow), w)] = v[0][i];
@@ -231,13 +236,14 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<I
     const uint32_t g = (wave << 4) | (lane & 15u), w = g & 7u, rlow = g >> 3;
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[0][i] = lds[RC::lds_at(RC::row_post(i, lane, rlow), w)];
-    gs6<SET, 1>(v, TwColA{as_const(z)}, p);
+    gs6<SET, 1, RC::SB>(v, TwColA{as_const(z)}, p);
 #pragma unroll
     for (int i = 0; i < 16; ++i) buf[(RC::row_entry(i, lane, rlow) << log2c) + c0 + w] = v[0][i];
 }
 
 // Column tiles of 2^9 .. 2^11 rows by 8 columns go to the register kernels
-// unless PINTT_REG_COLS=0.
+// unless PINTT_REG_COLS=0 (the forward pass also for the narrow set's 2^7-
+// and 2^8-row tiles).
 bool reg_cols_enabled() {
     static const bool on = [] {
         const char* v = std::getenv("PINTT_REG_COLS");
@@ -268,13 +274,15 @@ bool col_case_available(int set, int log2r, int log2w) {
 hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
-    if (pl.log2w == 3 && pl.log2r >= 9 && pl.log2r <= 11 && reg_cols_enabled()) {
+    if (pl.log2w == 3 && pl.log2r >= 7 && pl.log2r <= 11 && reg_cols_enabled()) {
 #define PINTT_REG_COL(LR, SET)                                                                                   \
     if (pl.log2r == LR) {                                                                                         \
         hipLaunchKernelGGL((k_col_fwd_reg<LR, SET>), grid, dim3(RegCols<LR>::NT), 0, s, jobs, pl.log2c, tw); \
         return hipGetLastError();                                                                                \
     }
         if (set_for(pl.log2n) == kSetNarrow) {
+            PINTT_REG_COL(7, kSetNarrow)
+            PINTT_REG_COL(8, kSetNarrow)
             PINTT_REG_COL(9, kSetNarrow)
             PINTT_REG_COL(10, kSetNarrow)
             PINTT_REG_COL(11, kSetNarrow)
@@ -307,6 +315,11 @@ hipError_t launch_forward_columns(const Plan& pl, const JobList<FwdJob>& jobs, u
 hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, uint32_t njobs, const Twiddles& tw,
                                   hipStream_t s) {
     dim3 grid(1u << (pl.log2c - pl.log2w), kPrimes, njobs);
+    // 2^7- and 2^8-row tiles keep the LDS kernel for the inverse: reading and
+    // writing 32-byte column rows from one-wave workgroups, the register
+    // kernel moved a 2^15-point bucket at 2.6 TB/s against 4.1 TB/s for the
+    // LDS kernel with 16-column tiles (the forward pass, which writes only,
+    // is faster on the register kernel).
     if (pl.log2w == 3 && pl.log2r >= 9 && pl.log2r <= 11 && reg_cols_enabled()) {
 #define PINTT_REG_COL(LR, SET)                                                                                   \
     if (pl.log2r == LR) {                                                                                         \

@@ -231,7 +231,7 @@ def test_row_reg_layouts():
 def _col_model(LR):
     """LDS address, stage B group bits (hi) and stage B rows of the register
     column kernels (RegCols in ntt_cols.hip)."""
-    S, PSH = {9: (2, 6), 10: (1, 6), 11: (1, 7)}[LR]
+    S, PSH = {7: (4, 6), 8: (4, 6), 9: (2, 6), 10: (1, 6), 11: (1, 7)}[LR]
     NT = 1 << (LR - 1)
     lds = lambda r, w: ((r << 3) | w) + S * (((r << 3) | w) >> PSH)
 
@@ -241,7 +241,9 @@ def _col_model(LR):
             return (wave & 3) | (l3 << 2) | (l4 << 3) | ((wave >> 2) << 4)
         if LR == 10:
             return l5 | ((wave & 1) << 1) | (l3 << 2) | (l4 << 3) | ((wave >> 1) << 4)
-        return l5 | (l3 << 1) | (l4 << 2) | (wave << 3)
+        if LR == 9:
+            return l5 | (l3 << 1) | (l4 << 2) | (wave << 3)
+        return l3 | (l4 << 1) | (l5 << 2) | (wave << 3)
 
     def row_b(i, lane, hi, post=False):
         if LR == 11:
@@ -252,8 +254,9 @@ def _col_model(LR):
 
 
 def test_col_reg_layouts():
-    for LR in (9, 10, 11):
+    for LR in (7, 8, 9, 10, 11):
         S, NT, lds, hi_bits, row_b = _col_model(LR)
+        sb = 2 if LR == 7 else 1  # RegCols::SB
         R = 1 << LR
         a_entry, a_post, b_rows, b_post = [], [], [], []
         for wave in range(NT // 64):
@@ -263,7 +266,7 @@ def test_col_reg_layouts():
                     g = (wave << 4) | (lane & 15)
                     w, rlow = g & 7, g >> 3
                     ae.append(((entry_t(reg, lane) << (LR - 6)) | rlow, w))
-                    ap.append(((post_t(reg, lane) << (LR - 6)) | rlow, w))
+                    ap.append(((post_t(reg, lane, sb) << (LR - 6)) | rlow, w))
                     hi = hi_bits(lane, wave)
                     br.append((row_b(reg, lane, hi), lane & 7))
                     bp.append((row_b(reg, lane, hi, True), lane & 7))
@@ -282,12 +285,20 @@ def test_col_reg_layouts():
         for name, seq in (("a_entry", a_entry), ("a_post", a_post), ("b", b_rows), ("b_post", b_post)):
             assert sorted(seq) == full, (LR, name)
         # Stage B groups: the rows sharing r >> (LR - 6) in one column, 2^(LR-6)
-        # of them, on one lane (LR = 10), half a lane (LR = 9, register bit 3)
-        # or lanes l and l + 32 (LR = 11).
+        # of them, on one lane (LR = 10), part of a lane (LR = 7 .. 9, the
+        # register bits above LR - 6) or lanes l and l + 32 (LR = 11).
         groups = {}
         for wave, lane, reg in itertools.product(range(NT // 64), range(64), range(16)):
             r = row_b(reg, lane, hi_bits(lane, wave))
             groups.setdefault((r >> (LR - 6), lane & 7), set()).add(r)
+            if LR <= 10:
+                # in-lane stage: register reg = (g << KB) | t holds row
+                # (outer << KB) | t, outer = TwColB's block at layer 6
+                kb = LR - 6
+                g, t = reg >> kb, reg & ((1 << kb) - 1)
+                hi = hi_bits(lane, wave)
+                outer = hi if kb >= 4 else (hi << (4 - kb)) | g
+                assert r == (outer << kb) | t, (LR, wave, lane, reg)
         assert len(groups) == 64 * 8 and all(len(v) == 1 << (LR - 6) for v in groups.values())
         # Stage A groups: the 64 rows sharing r's low LR - 6 bits, in column w.
         for wave, lane in itertools.product(range(NT // 64), range(64)):
