@@ -627,6 +627,7 @@ def main() -> int:
     prewarm = clock_warmup(stream, da, an, db, bn, dout, args.clock_warmup)
     ms_mul = time_gpu(stream, da, an, db, bn, dout, args.steps, args.warmup)
     gpu_out = dout.to_array()
+    prewarm_sq = clock_warmup(stream, da, an, da, an, dsq, args.clock_warmup)
     ms_sqr = time_gpu(stream, da, an, da, an, dsq, args.steps, args.warmup)
     passes = time_passes(stream, da, an, db, bn, dout, max(5, min(50, args.steps)))
     passes_sq = time_passes(stream, da, an, da, an, dsq, max(5, min(50, args.steps)))
@@ -665,7 +666,7 @@ def main() -> int:
         "product_roofline": {"algorithmic_bytes": b_alg,
                              "achieved_GBs": round(b_alg / (ms_mul * 1e-3) / 1e9, 1),
                              "frac": round(b_alg / (ms_mul * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "square": {"mults_per_s": round(1e3 / ms_sqr, 2), "ms": round(ms_sqr, 5),
+        "square": {"mults_per_s": round(1e3 / ms_sqr, 2), "ms": round(ms_sqr, 5), "clock_warmup_products": prewarm_sq,
                    "product_roofline_frac": round(
                        survey_bytes(an, an, log2n, True) / (ms_sqr * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "passes_ms": {k: round(v, 4) for k, v in passes.items()},

@@ -158,6 +158,84 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Whether chunk product i of `job` reaches the words [first, first + CT kWT)
+// of a block (its positions first - 2 .. spill into them).
+template <int CT>
+__device__ __forceinline__ bool chunk_reaches(const CarryJob& job, int i, long long first) {
+    const long long k0 = first - (long long)job.off[i];
+    return !(k0 + CT * kWT <= 0 || k0 - 2 >= (long long)job.len[i]);
+}
+
+// The residues of chunk product i at the thread's kWT positions (two 16-byte
+// loads per plane where all are inside the chunk) and, for threads 0 and 1,
+// at position first - 2 + t before the block's first word.
+template <int CT>
+__device__ __forceinline__ void load_chunk(const CarryJob& job, int i, long long first, int t, size_t n,
+                                           uint32_t (&r)[3][kWT], uint32_t (&h)[3]) {
+    const long long k0 = first - (long long)job.off[i];
+    const long long len = job.len[i];
+    const global_ptr<const uint32_t> res = as_global(job.res[i]);
+    const long long k = k0 + kWT * t;
+    if (k >= 0 && k + kWT <= len) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int e = 0; e < kWT; e += 4) {
+                const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k + e);
+                r[q][e] = x.x;
+                r[q][e + 1] = x.y;
+                r[q][e + 2] = x.z;
+                r[q][e + 3] = x.w;
+            }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kWT; ++u) {
+            const bool in = k + u >= 0 && k + u < len;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) r[q][u] = in ? res[q * n + k + u] : 0u;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) h[q] = 0u;
+    if (t < 2) {
+        const long long kk = k0 - 2 + t;
+        if (kk >= 0 && kk < len) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) h[q] = res[q * n + kk];
+        }
+    }
+}
+
+// CRT words of the thread's positions added into its word sums s; the terms
+// of the next thread's words 0 and 1 into ea, eb; threads 0 and 1 add the
+// terms of the positions before the block into ha, hb.
+template <int SET>
+__device__ __forceinline__ void add_chunk(const uint32_t (&r)[3][kWT], const uint32_t (&h)[3], int t,
+                                          const CrtConst& cc, uint64_t (&s)[kWT], uint64_t& ea, uint64_t& eb,
+                                          uint64_t& ha, uint64_t& hb) {
+#pragma unroll
+    for (int u = 0; u < kWT; ++u) {
+        uint32_t x0, x1, x2;
+        crt_words<SET>(r[0][u], r[1][u], r[2][u], cc, x0, x1, x2);
+        s[u] += x0;
+        if (u + 1 < kWT) s[u + 1] += x1;
+        else ea += x1;
+        if (u + 2 < kWT) s[u + 2] += x2;
+        else if (u + 2 == kWT) ea += x2;
+        else eb += x2;
+    }
+    if (t < 2) {  // positions first - 2 and first - 1
+        uint32_t x0, x1, x2;
+        crt_words<SET>(h[0], h[1], h[2], cc, x0, x1, x2);
+        if (t == 0) {
+            ha += x2;
+        } else {
+            ha += x1;
+            hb += x2;
+        }
+    }
+}
+
 // grid = carry workgroups of the launch; jobs sorted by blk0.  SET: the prime
 // set the residues belong to; CT: threads per workgroup.
 template <int SET, int CT>
@@ -182,58 +260,29 @@ __global__ void __launch_bounds__(CT)
     for (int e = 0; e < kWT; ++e) s[e] = 0;
     uint64_t ea = 0, eb = 0;  // terms of the next thread's words 0 and 1
     uint64_t ha = 0, hb = 0;  // threads 0, 1: terms of words `first` and `first` + 1
-    // Constant chunk indices keep the job (a kernel-argument copy) in registers.
+    // The first two chunk products that reach the words: every load issued
+    // before any is used, so that a wave waits for memory once.  Products cut
+    // into three or four chunks load the others after them.  Constant chunk
+    // indices keep the job (a kernel-argument copy) in registers.
+    {
+        uint32_t r[2][3][kWT], h[2][3];
+        bool use[2];
 #pragma unroll
-    for (uint32_t i = 0; i < (uint32_t)kMaxChunks; ++i) {
-        if (i >= job.m) break;
-        const long long k0 = first - (long long)job.off[i];  // chunk position of word `first`
-        const long long len = job.len[i];
-        if (k0 + kCT * kWT <= 0 || k0 - 2 >= len) continue;  // no position of this chunk reaches the words
-        const global_ptr<const uint32_t> res = as_global(job.res[i]);
-        const long long k = k0 + kWT * t;
-        uint32_t r[3][kWT];
-        if (k >= 0 && k + kWT <= len) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-#pragma unroll
-                for (int h = 0; h < kWT; h += 4) {
-                    const u32x4 x = *(global_ptr<const u32x4>)(res + q * n + k + h);
-                    r[q][h] = x.x;
-                    r[q][h + 1] = x.y;
-                    r[q][h + 2] = x.z;
-                    r[q][h + 3] = x.w;
-                }
-        } else {
-#pragma unroll
-            for (int u = 0; u < kWT; ++u) {
-                const bool in = k + u >= 0 && k + u < len;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) r[q][u] = in ? res[q * n + k + u] : 0u;
-            }
+        for (int i = 0; i < 2; ++i) {
+            use[i] = (uint32_t)i < job.m && chunk_reaches<kCT>(job, i, first);
+            if (use[i]) load_chunk<kCT>(job, i, first, t, n, r[i], h[i]);
         }
 #pragma unroll
-        for (int u = 0; u < kWT; ++u) {
-            uint32_t x0, x1, x2;
-            crt_words<SET>(r[0][u], r[1][u], r[2][u], cc, x0, x1, x2);
-            s[u] += x0;
-            if (u + 1 < kWT) s[u + 1] += x1;
-            else ea += x1;
-            if (u + 2 < kWT) s[u + 2] += x2;
-            else if (u + 2 == kWT) ea += x2;
-            else eb += x2;
-        }
-        if (t < 2) {  // positions first - 2 and first - 1
-            const long long kk = k0 - 2 + t;
-            const bool in = kk >= 0 && kk < len;
-            uint32_t x0, x1, x2;
-            crt_words<SET>(in ? res[kk] : 0u, in ? res[n + kk] : 0u, in ? res[2 * n + kk] : 0u, cc, x0, x1, x2);
-            if (t == 0) {
-                ha += x2;
-            } else {
-                ha += x1;
-                hb += x2;
-            }
-        }
+        for (int i = 0; i < 2; ++i)
+            if (use[i]) add_chunk<SET>(r[i], h[i], t, cc, s, ea, eb, ha, hb);
+    }
+#pragma unroll
+    for (int i = 2; i < kMaxChunks; ++i) {
+        if ((uint32_t)i >= job.m) break;
+        if (!chunk_reaches<kCT>(job, i, first)) continue;
+        uint32_t r[3][kWT], h[3];
+        load_chunk<kCT>(job, i, first, t, n, r, h);
+        add_chunk<SET>(r, h, t, cc, s, ea, eb, ha, hb);
     }
     // Spill terms from the thread before: lane shuffles inside a wave, LDS
     // across waves and, for the first thread, the two positions before.
