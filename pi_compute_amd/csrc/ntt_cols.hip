@@ -159,14 +159,15 @@ struct TwColA {
     __device__ __forceinline__ uint2 operator()(int l, uint32_t blk) const { return tw_at(z, (1u << l) + blk); }
 };
 
-// Stage B of a register column kernel on the lane's 16 residues.
-template <int LR, int SET, bool INV>
+// Stage B of a register column kernel on the lane's 16 residues.  FRESH
+// (inverse): the values are below 2p, as scaled rows store them.
+template <int LR, int SET, bool INV, bool FRESH = false>
 __device__ __forceinline__ void col_stage_b(uint32_t (&v)[1][16], const TwColB& tw, uint32_t p) {
     if constexpr (LR == 11) {
-        if constexpr (INV) gs5<SET, 1>(v, tw, p);
+        if constexpr (INV) gs5<SET, 1, FRESH>(v, tw, p);
         else ct5<SET, 1>(v, tw, p);
     } else {
-        if constexpr (INV) gs_lane<SET, LR - 6>(v[0], tw, p);
+        if constexpr (INV) gs_lane<SET, LR - 6, FRESH>(v[0], tw, p);
         else ct_lane<SET, LR - 6>(v[0], tw, p);
     }
 }
@@ -214,7 +215,8 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_fwd_reg(const JobList<F
     for (int i = 0; i < 16; ++i) dst[(RC::row_b_post(i, lane, hi) << log2c) + c0 + wb] = v[0][i];
 }
 
-template <int LR, int SET>
+// FRESH: every product row arrives below 2p (the row pass scaled its rows).
+template <int LR, int SET, bool FRESH>
 __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<InvJob> jobs, int log2c, Twiddles tw) {
     using RC = RegCols<LR>;
     __shared__ uint32_t lds[RC::WORDS];
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(RegCols<LR>::NT) k_col_inv_reg(const JobList<I
     uint32_t v[1][16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[0][i] = buf[(RC::row_b_post(i, lane, hi) << log2c) + c0 + wb];
-    col_stage_b<LR, SET, true>(v, TwColB{as_const(z), hi, RC::KB}, p);
+    col_stage_b<LR, SET, true, FRESH>(v, TwColB{as_const(z), hi, RC::KB}, p);
 #pragma unroll
     for (int i = 0; i < 16; ++i) lds[RC::lds_at(RC::row_b(i, lane, hi), wb)] = v[0][i];
     __syncthreads();
@@ -321,9 +323,15 @@ hipError_t launch_inverse_columns(const Plan& pl, const JobList<InvJob>& jobs, u
     // LDS kernel with 16-column tiles (the forward pass, which writes only,
     // is faster on the register kernel).
     if (pl.log2w == 3 && pl.log2r >= 9 && pl.log2r <= 11 && reg_cols_enabled()) {
+        const bool fresh = rows_scaled(pl);
 #define PINTT_REG_COL(LR, SET)                                                                                   \
     if (pl.log2r == LR) {                                                                                         \
-        hipLaunchKernelGGL((k_col_inv_reg<LR, SET>), grid, dim3(RegCols<LR>::NT), 0, s, jobs, pl.log2c, tw); \
+        if (fresh)                                                                                                \
+            hipLaunchKernelGGL((k_col_inv_reg<LR, SET, true>), grid, dim3(RegCols<LR>::NT), 0, s, jobs,       \
+                               pl.log2c, tw);                                                                    \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_col_inv_reg<LR, SET, false>), grid, dim3(RegCols<LR>::NT), 0, s, jobs,      \
+                               pl.log2c, tw);                                                                    \
         return hipGetLastError();                                                                                \
     }
         if (set_for(pl.log2n) == kSetNarrow) {

@@ -148,6 +148,8 @@ bool col_case_available(int set, int log2r, int log2w);
 bool row_case_available(int set, int log2c, int log2rows);
 // Pair row kernels hold three tiles in LDS: rows of at most 2^13 words per workgroup.
 bool pair_case_available(int set, int log2c, int log2rows);
+// Every row kernel of a plan scales its rows and stores its products below 2p.
+bool rows_scaled(const Plan& pl);
 
 // A plan with an explicit split, checked against the tile shapes compiled for
 // the prime set of its length.

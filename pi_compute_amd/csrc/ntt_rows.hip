@@ -452,6 +452,17 @@ void launch_rows(const Plan& pl, dim3 grid, const JobList<RowJob>& jobs, const T
 
 }  // namespace
 
+// Whether every row kernel at this plan scales its rows (then it stores its
+// products through Arith::scale, below 2p), whatever the row kinds of the
+// bucket: the same thresholds and shapes launch_rows applies.
+bool rows_scaled(const Plan& pl) {
+    if (set_for(pl.log2n) != kSetNarrow) return false;
+    const bool shape = pl.log2rows == 0 || (reg_rows_enabled() && ((pl.log2c == 10 && pl.log2rows == 2) ||
+                                                                   (pl.log2c == 11 && pl.log2rows == 1)));
+    return shape && pl.log2n >= kScaledRowsMinLog2N && pl.log2n >= kScaledPairMinLog2N &&
+           pl.log2n <= kScaledPairMaxLog2N;
+}
+
 // Tile shapes per prime set.  The wide set only serves 2^23 .. 2^25 points,
 // rows of 2^12 .. 2^14 (columns hold at most 2^11 rows).
 #define PINTT_ROW_CASES_NARROW(X) \
