@@ -144,7 +144,10 @@ struct Stage {
 // layer S0+l, sub-block `sub` of the group: global layer sb + S0 + l, global
 // block ((row0 + seq) << (S0 + l)) + (bblk << l) + sub, where row0 is the
 // block index of sequence 0 at layer sb (0 for columns, the row for rows).
-template <int SET, class TL, int S0, int K, int NB, class ZP>
+// The run's values enter below B0 p (narrow set): layer S0 + l skips the
+// reduction of its first input when lazy_ok(B0, S0 + l) (the bound of a layer
+// is the same for every element).
+template <int SET, class TL, int S0, int K, int NB, int B0, class ZP>
 __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
@@ -166,7 +169,8 @@ __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb,
 #pragma unroll
                 for (int i = 0; i < h; ++i)
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) Arith<SET>::ct(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
+                    for (int u = 0; u < NB; ++u)
+                        Arith<SET>::ct_sel(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p, lazy_ok(B0, S0 + l));
             }
         }
 #pragma unroll
@@ -177,7 +181,11 @@ __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb,
 }
 
 // Inverse (Gentleman-Sande) layers S0+K-1 down to S0 on NB tiles (1..2).
-template <int SET, class TL, int S0, int K, int NB, class ZP>
+// Narrow set: values below 4p; a butterfly keeps its sum unreduced when both
+// inputs are below 2p: in the stage's first layer when FRESH (every value of
+// the tile below 2p), in a later one where the layer before (register bit
+// log2(h) - 1) left two differences, i & (h >> 1).
+template <int SET, class TL, int S0, int K, int NB, bool FRESH, class ZP>
 __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb, uint32_t row0, uint32_t p) {
     using ST = Stage<TL, S0, K>;
 #pragma unroll
@@ -199,7 +207,9 @@ __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb
 #pragma unroll
                 for (int i = 0; i < h; ++i)
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) Arith<SET>::gs(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p);
+                    for (int u = 0; u < NB; ++u)
+                        Arith<SET>::gs_sel(v[u][sub * 2 * h + i], v[u][sub * 2 * h + i + h], z, p,
+                                           l == K - 1 ? FRESH : (i & (h >> 1)) != 0);
             }
         }
 #pragma unroll
@@ -209,22 +219,26 @@ __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb
     }
 }
 
-template <int SET, class TL, int S0, int NB, class ZP>
+// Forward layers S0.. of a tile whose values enter the run below B0 p.
+template <int SET, class TL, int S0, int NB, int B0 = 8, class ZP>
 __device__ __forceinline__ void ct_run(uint32_t* const (&t)[NB], ZP Z, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
-        ct_stage<SET, TL, S0, K, NB>(t, Z, sb, row0, p);
+        ct_stage<SET, TL, S0, K, NB, B0>(t, Z, sb, row0, p);
         __syncthreads();
-        ct_run<SET, TL, S0 + K, NB>(t, Z, sb, row0, p);
+        ct_run<SET, TL, S0 + K, NB, B0>(t, Z, sb, row0, p);
     }
 }
 
-template <int SET, class TL, int S0, int NB, class ZP>
+// Inverse layers down to S0; FRESH: the run's values enter below 2p (the
+// first stage it runs, the deepest, may start lazy; after an LDS exchange the
+// sums and differences of a register are mixed, so later stages reduce first).
+template <int SET, class TL, int S0, int NB, bool FRESH = false, class ZP>
 __device__ __forceinline__ void gs_run(uint32_t* const (&t)[NB], ZP Zi, int sb, uint32_t row0, uint32_t p) {
     if constexpr (S0 < TL::LOGL) {
         constexpr int K = (TL::LOGL - S0) < 4 ? (TL::LOGL - S0) : 4;
-        gs_run<SET, TL, S0 + K, NB>(t, Zi, sb, row0, p);
-        gs_stage<SET, TL, S0, K, NB>(t, Zi, sb, row0, p);
+        gs_run<SET, TL, S0 + K, NB, FRESH>(t, Zi, sb, row0, p);
+        gs_stage<SET, TL, S0, K, NB, FRESH && S0 + K == TL::LOGL>(t, Zi, sb, row0, p);
         __syncthreads();
     }
 }

@@ -102,13 +102,13 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     const auto zf = PINTT_FWD_TABLE(tw, prime);
     if constexpr (kPair) {
         uint32_t* const t[3] = {ldsA, ldsB, ldsC};
-        ct_run<SET, TL, 0, 3>(t, zf, sb, rb, p);
+        ct_run<SET, TL, 0, 3, kScaled ? 2 : 8>(t, zf, sb, rb, p);
     } else if constexpr (KIND == kRowTransformB) {
         uint32_t* const t[2] = {ldsA, ldsB};
-        ct_run<SET, TL, 0, 2>(t, zf, sb, rb, p);
+        ct_run<SET, TL, 0, 2, kScaled ? 2 : 8>(t, zf, sb, rb, p);
     } else {
         uint32_t* const t[1] = {ldsA};
-        ct_run<SET, TL, 0, 1>(t, zf, sb, rb, p);
+        ct_run<SET, TL, 0, 1, kScaled ? 2 : 8>(t, zf, sb, rb, p);
     }
     if constexpr (KIND == kRowTransformB || kPair) {
         if (job.write_b) {
@@ -135,10 +135,10 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
     const auto zi = PINTT_INV_TABLE(tw, prime);
     if constexpr (kPair) {
         uint32_t* const t[2] = {ldsA, ldsC};
-        gs_run<SET, TL, 0, 2>(t, zi, sb, rb, p);
+        gs_run<SET, TL, 0, 2, true>(t, zi, sb, rb, p);
     } else {
         uint32_t* const t[1] = {ldsA};
-        gs_run<SET, TL, 0, 1>(t, zi, sb, rb, p);
+        gs_run<SET, TL, 0, 1, true>(t, zi, sb, rb, p);
     }
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
