@@ -9,10 +9,11 @@
  * boundary; every entry point returns a status code (0 = success).
  *
  * Products are exact: the engine computes the cyclic convolution of 32-bit
- * coefficients modulo three primes below 2^31 (transform length N = 2^k,
- * 2^10 <= N <= 2^25 with N >= 2*an + 2*bn - 1), reconstructs every
- * coefficient with Garner's CRT and propagates carries on the device.  The
- * result is bit-identical to mpz_mul.
+ * coefficients modulo three primes (below 2^29 for N <= 2^22, below 2^31
+ * above; transform length N = 2^k, 2^10 <= N <= 2^25 with
+ * N >= 2*an + 2*bn - 1), reconstructs every coefficient with the Chinese
+ * remainder theorem and propagates carries on the device.  The result is
+ * bit-identical to mpz_mul.
  *
  * Thread safety: every entry point may be called from any thread; calls on one
  * device are serialised inside the library (one product at a time).
