@@ -334,10 +334,11 @@ def sweep_leg(stream, gmp: Gmp, kmin: int, kmax: int, cpu_budget: float):
     return out
 
 
-def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int):
+def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int, per_call_leg: bool = True):
     """The tree-level replay batch: 4,096 independent products (lognormal bit
     lengths, median 2^18, every third a square), operands resident in HBM.
-    Times one product per call, one batch call, and a captured-graph replay."""
+    Times one product per call (unless per_call_leg is False), one batch call,
+    and a captured-graph replay."""
     from pi_compute_amd import workloads
 
     prods = workloads.tree_level_replay()
@@ -385,7 +386,7 @@ def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int):
         for i in range(n):
             pc.mul_device(d_a[i], an[i], d_b[i], bn[i], d_o[i], stream.handle)
 
-    ms_calls = timed(per_call, 1)
+    ms_calls = timed(per_call, 1) if per_call_leg else None
     ms_batch = timed(lambda: pc.mul_batch_device(d_a, an, d_b, bn, d_o, stream.handle), reps)
     graph = pc.BatchGraph(d_a, an, d_b, bn, d_o)
     ms_graph = timed(lambda: graph.launch(stream.handle), reps)
@@ -412,7 +413,7 @@ def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int):
                     "every third a square, splitmix64 seed 7",
         "products": n, "limbs_in": int(total_in), "limbs_out": int(total_out),
         "algorithmic_bytes": int(b_alg),
-        "per_call": rate(ms_calls),
+        "per_call": rate(ms_calls) if ms_calls is not None else None,
         "batch_call": rate(ms_batch),
         "graph_replay": {**rate(ms_graph), "graph_nodes": int(graph_nodes)},
     }
@@ -567,6 +568,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the tree-level replay batch leg")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--plans-file", default="", metavar="PATH",
+                    help="load tile plans and their tuned times (gpu_ntt_load_plans) instead of tuning lengths "
+                         "at first use; every run then plans the batch the same way")
+    ap.add_argument("--no-per-call", action="store_true",
+                    help="batch leg: skip the one-product-per-call timing (4,096 calls, about 20,000 launches)")
     ap.add_argument("--plan", action="append", default=[], metavar="N:R,W,ROWS",
                     help="force the tile plan of transform length 2^N (else autotuned); repeatable")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "k26_multiply_traffic.json"),
@@ -600,8 +606,12 @@ def main() -> int:
         print("gpu_ntt_init failed: " + lib.gpu_ntt_last_error().decode(), file=sys.stderr)
         return 1
 
+    if args.plans_file:
+        if lib.gpu_ntt_load_plans(str(args.plans_file).encode()) != 0:
+            print("gpu_ntt_load_plans failed: " + lib.gpu_ntt_last_error().decode(), file=sys.stderr)
+            return 1
     specs = list(args.plan)
-    if not specs and not args.autotune:
+    if not specs and not args.autotune and not args.plans_file:
         try:
             prof = json.loads(Path(args.traffic).read_text())
             specs = [f"{n}:{','.join(map(str, v))}" for n, v in prof["tile_plans"].items()]
@@ -689,7 +699,7 @@ def main() -> int:
     if not args.no_batch:
         print("batch replay", file=sys.stderr, flush=True)
         result["batch_replay"] = batch_replay(stream, gmp, 0 if args.no_cpu else args.cpu_budget / 2,
-                                              args.batch_reps)
+                                              args.batch_reps, not args.no_per_call)
     if args.config3_digits:
         result["config3"] = config3_leg(args.config3_digits)
     if args.pi_digits:
