@@ -31,13 +31,14 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     // Neighbouring column tiles share 128-byte lines; give each XCD (workgroups
     // are dealt to the 8 XCDs round-robin) a contiguous range of tiles.
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
+    const uint32_t w0 = TL::word(threadIdx.x);  // element e: tile position e NT + threadIdx.x
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
         uint32_t k = (r << log2c) + c0 + w;
         uint32_t v = k < job.ncoef ? src[k] : 0u;
-        lds[TL::addr(w, r)] = Arith<SET>::load_coeff(v, p);
+        lds[TL::word_at(w0, e * TL::NT)] = Arith<SET>::load_coeff(v, p);
     }
     __syncthreads();
     uint32_t* const tiles[1] = {lds};
@@ -46,7 +47,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
-        dst[(r << log2c) + c0 + w] = lds[TL::addr(w, r)];
+        dst[(r << log2c) + c0 + w] = lds[TL::word_at(w0, e * TL::NT)];
     }
 }
 
@@ -65,11 +66,12 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
     const TwTable z = PINTT_INV_TABLE(tw, prime);
     for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw_at(z, i);
+    const uint32_t w0 = TL::word(threadIdx.x);  // element e: tile position e NT + threadIdx.x
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
-        lds[TL::addr(w, r)] = buf[(r << log2c) + c0 + w];
+        lds[TL::word_at(w0, e * TL::NT)] = buf[(r << log2c) + c0 + w];
     }
     __syncthreads();
     uint32_t* const tiles[1] = {lds};
@@ -78,7 +80,7 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     for (int e = 0; e < kElemsPerThread; ++e) {
         uint32_t idx = e * TL::NT + threadIdx.x;
         uint32_t w = idx & (TL::SEQ - 1), r = idx >> LOGW;
-        buf[(r << log2c) + c0 + w] = lds[TL::addr(w, r)];
+        buf[(r << log2c) + c0 + w] = lds[TL::word_at(w0, e * TL::NT)];
     }
 }
 

@@ -94,12 +94,41 @@ __device__ __forceinline__ uint32_t wave_search(uint32_t n, uint32_t v, Key key)
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// One pad word after every 16 keeps radix-16 strides off a single LDS bank.
-__device__ __forceinline__ uint32_t pad(uint32_t x) { return x + (x >> 4); }
+// Bank layout of a tile in LDS.  Linear position x of a tile sits at word
+// x ^ T(x), T(x) = ((x >> s1) & m1) << t1  ^  ((x >> s2) & m2) << t2, which
+// permutes the words of each line of 32 (T changes bits 0..4 only and is
+// linear over GF(2)).  LDS serves a wave in two halves of 32 lanes over 32
+// banks (bank = word mod 32); a radix-16 exchange has the lanes touch words a
+// power-of-two stride apart, which the identity layout puts on few banks.  The
+// terms below leave no bank conflict in any load, store or stage exchange of
+// the kernels, for every compiled shape (counted lane by lane on a model of
+// those accesses; a search over one- and two-term maps):
+//   rows: (x >> 4) & 31 into bits 0..4, and a second term by log2(L) mod 4;
+//   columns: the line number (x >> 5) into bits 0..4 as listed.
+struct LdsMap {
+    uint32_t s1, m1, t1, s2, m2, t2;
+};
+__host__ __device__ constexpr LdsMap lds_map(int logl, int logseq, bool cols) {
+    if (!cols) {
+        switch (logl & 3) {
+            case 1: return {4, 31, 0, 4, 3, 3};
+            case 2: return {4, 31, 0, 4, 7, 2};
+            case 3: return {4, 31, 0, 4, 15, 1};
+            default: return {4, 31, 0, 9, 1, 4};
+        }
+    }
+    if ((logl == 7 && logseq <= 4) || (logl == 11 && logseq == 3)) return {5, 7, 2, 0, 0, 0};
+    if (logl == 8 && logseq <= 4) return {5, 15, 1, 0, 0, 0};
+    if (logl == 9 && logseq == 3) return {5, 15, 1, 5, 1, 3};
+    if (logl == 9 && logseq == 4) return {5, 1, 4, 0, 0, 0};
+    if ((logl == 6 && logseq == 4) || (logl == 10 && logseq <= 4)) return {5, 3, 3, 0, 0, 0};
+    return {5, 1, 0, 0, 0, 0};
+}
 
 // A tile of SEQ sequences of length L in LDS.  COLS: the column pass, where
-// element (seq w, index r) sits at r * SEQ + w so that neighbouring threads
-// touch neighbouring words; otherwise sequences are stored one after another.
+// element (seq w, index r) has linear position r * SEQ + w so that
+// neighbouring threads touch neighbouring words; otherwise sequences are
+// stored one after another.
 template <int LOGL_, int LOGSEQ_, bool COLS_>
 struct Tile {
     static constexpr int LOGL = LOGL_;
@@ -108,9 +137,24 @@ struct Tile {
     static constexpr int L = 1 << LOGL;
     static constexpr int SEQ = 1 << LOGSEQ;
     static constexpr int NT = L * SEQ / kElemsPerThread;
-    static constexpr int WORDS = L * SEQ + ((L * SEQ) >> 4);
+    static constexpr int WORDS = L * SEQ;
+    static constexpr uint32_t STRIDE = COLS ? SEQ : 1;  // linear distance of neighbouring indices
+    static constexpr LdsMap M = lds_map(LOGL, LOGSEQ, COLS);
+    static_assert(LOGL + LOGSEQ >= 5 && (M.m1 << M.t1) < 32 && (M.m2 << M.t2) < 32,
+                  "the map permutes whole lines of 32 words");
+    __host__ __device__ static constexpr uint32_t word(uint32_t x) {
+        return x ^ (((x >> M.s1) & M.m1) << M.t1) ^ (((x >> M.s2) & M.m2) << M.t2);
+    }
     __device__ static __forceinline__ uint32_t addr(uint32_t seq, uint32_t idx) {
-        return COLS ? pad((idx << LOGSEQ) + seq) : pad((seq << LOGL) + idx);
+        return word(COLS ? (idx << LOGSEQ) + seq : (seq << LOGL) + idx);
+    }
+    // Word of position x0 + c, from w0 = word(x0), for a c with no set bit in
+    // common with x0: then x0 + c = x0 ^ c and word(x0 + c) = w0 ^ word(c).
+    // Above bit 4 that is an addition, so a constant c costs at most one XOR
+    // of the low bits; the rest goes into the LDS instruction's offset.
+    __device__ static __forceinline__ uint32_t word_at(uint32_t w0, uint32_t c) {
+        const uint32_t wc = word(c);
+        return (w0 ^ (wc & 31u)) + (wc & ~31u);
     }
 };
 
@@ -155,11 +199,12 @@ __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb,
         uint32_t seq, base, bblk;
         ST::locate(q * TL::NT + threadIdx.x, seq, base, bblk);
         uint32_t blk = TL::COLS ? row0 : row0 + seq;
+        const uint32_t w0 = TL::addr(seq, base);  // the group's elements: base + i DD, bits apart from base's
         uint32_t v[NB][ST::G];
 #pragma unroll
         for (int i = 0; i < ST::G; ++i)
 #pragma unroll
-            for (int u = 0; u < NB; ++u) v[u][i] = t[u][TL::addr(seq, base + i * ST::DD)];
+            for (int u = 0; u < NB; ++u) v[u][i] = t[u][TL::word_at(w0, i * ST::DD * TL::STRIDE)];
 #pragma unroll
         for (int l = 0; l < K; ++l) {
             const int h = ST::G >> (l + 1);
@@ -176,7 +221,7 @@ __device__ __forceinline__ void ct_stage(uint32_t* const (&t)[NB], ZP Z, int sb,
 #pragma unroll
         for (int i = 0; i < ST::G; ++i)
 #pragma unroll
-            for (int u = 0; u < NB; ++u) t[u][TL::addr(seq, base + i * ST::DD)] = v[u][i];
+            for (int u = 0; u < NB; ++u) t[u][TL::word_at(w0, i * ST::DD * TL::STRIDE)] = v[u][i];
     }
 }
 
@@ -193,11 +238,12 @@ __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb
         uint32_t seq, base, bblk;
         ST::locate(q * TL::NT + threadIdx.x, seq, base, bblk);
         uint32_t blk = TL::COLS ? row0 : row0 + seq;
+        const uint32_t w0 = TL::addr(seq, base);  // the group's elements: base + i DD, bits apart from base's
         uint32_t v[NB][ST::G];
 #pragma unroll
         for (int i = 0; i < ST::G; ++i)
 #pragma unroll
-            for (int u = 0; u < NB; ++u) v[u][i] = t[u][TL::addr(seq, base + i * ST::DD)];
+            for (int u = 0; u < NB; ++u) v[u][i] = t[u][TL::word_at(w0, i * ST::DD * TL::STRIDE)];
 #pragma unroll
         for (int l = K - 1; l >= 0; --l) {
             const int h = ST::G >> (l + 1);
@@ -215,7 +261,7 @@ __device__ __forceinline__ void gs_stage(uint32_t* const (&t)[NB], ZP Zi, int sb
 #pragma unroll
         for (int i = 0; i < ST::G; ++i)
 #pragma unroll
-            for (int u = 0; u < NB; ++u) t[u][TL::addr(seq, base + i * ST::DD)] = v[u][i];
+            for (int u = 0; u < NB; ++u) t[u][TL::word_at(w0, i * ST::DD * TL::STRIDE)] = v[u][i];
     }
 }
 

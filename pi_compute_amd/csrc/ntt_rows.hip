@@ -75,6 +75,9 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         }
         __syncthreads();
     }
+    // Element e of a thread has tile position e NT + threadIdx.x.
+    const uint32_t w0 = TL::word(threadIdx.x);
+    auto at = [&](int e) { return TL::word_at(w0, e * TL::NT); };
     auto factor = [&](int e, bool inverse) {
         const uint2 bq = beta[(inverse ? kElemsPerThread : 0) + e];
         const uint32_t f = mul_shoup(inverse ? h_inv : h_fwd, bq.x, bq.y, p);
@@ -85,14 +88,14 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         uint32_t idx = e * TL::NT + threadIdx.x;
         if constexpr (kScaled) {
             const uint2 f = factor(e, false);
-            ldsA[pad(idx)] = Arith<SET>::scale(a[idx], f.x, f.y, p);
-            if constexpr (KIND == kRowTransformB || kPair) ldsB[pad(idx)] = Arith<SET>::scale(b[idx], f.x, f.y, p);
-            if constexpr (KIND == kRowReuseB) ldsB[pad(idx)] = b[idx];
-            if constexpr (kPair) ldsC[pad(idx)] = Arith<SET>::scale(a2[idx], f.x, f.y, p);
+            ldsA[at(e)] = Arith<SET>::scale(a[idx], f.x, f.y, p);
+            if constexpr (KIND == kRowTransformB || kPair) ldsB[at(e)] = Arith<SET>::scale(b[idx], f.x, f.y, p);
+            if constexpr (KIND == kRowReuseB) ldsB[at(e)] = b[idx];
+            if constexpr (kPair) ldsC[at(e)] = Arith<SET>::scale(a2[idx], f.x, f.y, p);
         } else {
-            ldsA[pad(idx)] = a[idx];
-            if constexpr (KIND != kRowSquare) ldsB[pad(idx)] = b[idx];
-            if constexpr (kPair) ldsC[pad(idx)] = a2[idx];
+            ldsA[at(e)] = a[idx];
+            if constexpr (KIND != kRowSquare) ldsB[at(e)] = b[idx];
+            if constexpr (kPair) ldsC[at(e)] = a2[idx];
         }
     }
     __syncthreads();
@@ -115,13 +118,13 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
 #pragma unroll
             for (int e = 0; e < kElemsPerThread; ++e) {
                 uint32_t idx = e * TL::NT + threadIdx.x;
-                b[idx] = ldsB[pad(idx)];
+                b[idx] = ldsB[at(e)];
             }
         }
     }
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {
-        uint32_t idx = pad(e * TL::NT + threadIdx.x);
+        uint32_t idx = at(e);
         uint32_t x = ldsA[idx];
         if constexpr (KIND == kRowSquare) {
             ldsA[idx] = Arith<SET>::pointwise(x, x, p, pinv);
@@ -145,11 +148,11 @@ __global__ void __launch_bounds__((Tile<LOGC, LOGROWS, false>::NT)) __attribute_
         uint32_t idx = e * TL::NT + threadIdx.x;
         if constexpr (kScaled) {
             const uint2 f = factor(e, true);
-            a[idx] = Arith<SET>::scale(ldsA[pad(idx)], f.x, f.y, p);
-            if constexpr (kPair) a2[idx] = Arith<SET>::scale(ldsC[pad(idx)], f.x, f.y, p);
+            a[idx] = Arith<SET>::scale(ldsA[at(e)], f.x, f.y, p);
+            if constexpr (kPair) a2[idx] = Arith<SET>::scale(ldsC[at(e)], f.x, f.y, p);
         } else {
-            a[idx] = ldsA[pad(idx)];
-            if constexpr (kPair) a2[idx] = ldsC[pad(idx)];
+            a[idx] = ldsA[at(e)];
+            if constexpr (kPair) a2[idx] = ldsC[at(e)];
         }
     }
 }
