@@ -388,6 +388,15 @@ def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int, per_call_leg: b
 
     ms_calls = timed(per_call, 1) if per_call_leg else None
     ms_batch = timed(lambda: pc.mul_batch_device(d_a, an, d_b, bn, d_o, stream.handle), reps)
+    # One chain of launches, buckets one after another ...
+    pc.set_graph_branches(False)
+    graph_chain = pc.BatchGraph(d_a, an, d_b, bn, d_o)
+    pc.set_graph_branches(True)
+    ms_chain = timed(lambda: graph_chain.launch(stream.handle), reps)
+    chain_nodes = graph_chain.nodes
+    graph_chain.close()
+    # ... and every length bucket a branch of its own (the default); its outputs
+    # are the ones checked below.
     graph = pc.BatchGraph(d_a, an, d_b, bn, d_o)
     ms_graph = timed(lambda: graph.launch(stream.handle), reps)
     pc.synchronize()
@@ -415,7 +424,8 @@ def batch_replay(stream, gmp: Gmp, cpu_budget: float, reps: int, per_call_leg: b
         "algorithmic_bytes": int(b_alg),
         "per_call": rate(ms_calls) if ms_calls is not None else None,
         "batch_call": rate(ms_batch),
-        "graph_replay": {**rate(ms_graph), "graph_nodes": int(graph_nodes)},
+        "graph_replay": {**rate(ms_graph), "graph_nodes": int(graph_nodes), "capture": "bucket branches"},
+        "graph_replay_one_chain": {**rate(ms_chain), "graph_nodes": int(chain_nodes)},
     }
     if cpu_budget > 0:
         res["cpu_baseline"] = batch_cpu_baseline(gmp, prods, out, o_off, cpu_budget)

@@ -186,6 +186,15 @@ int gpu_ntt_synchronize(void);
  * either way. */
 void gpu_ntt_set_split(int mode);
 
+/* Graphs made by gpu_ntt_graph_create after this call capture each
+ * transform-length bucket (and the schoolbook products) as an independent
+ * branch with its own working memory, so a replay may run them at the same
+ * time (enable != 0, the default), or capture every launch in one chain
+ * (0).  Products are identical either way.  No reference counterpart (the
+ * reference has no batch launch; its caller is
+ * src/engine/binary_splitting.cpp:161-293). */
+void gpu_ntt_set_graph_branches(int enable);
+
 /* log2 of the transform length an unsplit an-limb by bn-limb product uses,
  * or -1 when it exceeds 2^25. */
 int gpu_ntt_transform_log2(size_t an, size_t bn);

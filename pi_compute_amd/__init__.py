@@ -80,6 +80,7 @@ ABI = {
          ctypes.c_void_p],
     ),
     "gpu_ntt_set_split": (None, [ctypes.c_int]),
+    "gpu_ntt_set_graph_branches": (None, [ctypes.c_int]),
     "gpu_ntt_graph_create": (
         ctypes.c_int,
         [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
@@ -267,6 +268,11 @@ def synchronize() -> None:
 def set_split(mode: int) -> None:
     """0 never cut operands, 1 cut when cheaper (default), 2..4 force that many chunks."""
     load_library().gpu_ntt_set_split(int(mode))
+
+
+def set_graph_branches(enable: bool) -> None:
+    """Graphs made after this capture each length bucket as its own branch (default) or one chain."""
+    load_library().gpu_ntt_set_graph_branches(1 if enable else 0)
 
 
 def mul_device(d_a: int, an: int, d_b: int, bn: int, d_out: int, stream: int = 0) -> None:

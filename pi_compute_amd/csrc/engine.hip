@@ -138,6 +138,7 @@ struct Engine {
     Plan plan[kMaxLog2N + 1] = {};
     float plan_ms[kMaxLog2N + 1] = {};
     int split_mode = 1;  // 0: never cut operands into chunks, 1: when cheaper, m >= 2: m chunks when valid
+    bool graph_branches = true;  // graphs capture each length bucket as a branch of its own
     bool pair_rows = true;  // first two chunks of a group in one row launch (PINTT_PAIR=0: off)
     size_t tune_cap = 0;      // limbs per tuning operand
     size_t tune_out_cap = 0;  // limbs of tuning outputs
@@ -444,6 +445,7 @@ int plan_for(int log2n, Plan* out);
 struct PlanMode {
     bool tune = true;
     uint32_t max_chunks = kMaxChunks;
+    bool disjoint = false;  // every bucket in its own part of the arena, so buckets may run at once
 };
 
 // The plan of a length without tuning: the tuned or loaded one, else the
@@ -571,6 +573,7 @@ struct BucketLayout {
     uint32_t max_m;
     std::vector<RowLaunch> rows;                  // in launch order
     bool pair;                                    // chunks 0 and 1 of a group share one row launch
+    size_t arena_base = 0;                        // first arena word of its slots
 };
 
 template <class T>
@@ -690,7 +693,13 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp, const PlanMode& 
         bytes = align_up<CarryJob>(bytes);
         b.carry_off = bytes;
         bytes += b.groups.size() * sizeof(CarryJob);
-        bp->arena_words = std::max(bp->arena_words, slots * kPrimes * ((size_t)1 << b.log2n));
+        const size_t words = slots * kPrimes * ((size_t)1 << b.log2n);
+        if (mode.disjoint) {
+            b.arena_base = bp->arena_words;
+            bp->arena_words += words;
+        } else {
+            bp->arena_words = std::max(bp->arena_words, words);
+        }
         bp->total_groups += b.groups.size();
         bp->total_blocks += b.nblocks;
         bp->min_log2n = std::min(bp->min_log2n, b.log2n);
@@ -732,11 +741,12 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, ui
             const Group& gr = bp.groups[b.groups[k]];
             const Request& r = gr.r;
             CarryJob car{};
-            uint32_t* bslot = r.square ? nullptr : arena + slot * (next_slot + gr.m);
+            uint32_t* const base = arena + b.arena_base;
+            uint32_t* bslot = r.square ? nullptr : base + slot * (next_slot + gr.m);
             const size_t first = next_slot;
             const bool pair = b.pair && !r.square && gr.m >= 2;
             for (uint32_t i = 0; i < gr.m; ++i) {
-                uint32_t* dst = arena + slot * next_slot++;
+                uint32_t* dst = base + slot * next_slot++;
                 size_t lo = i * gr.h;
                 size_t limbs = std::min(gr.h, r.an - lo);
                 fj[f++] = FwdJob{reinterpret_cast<const uint32_t*>(r.a + lo), (uint32_t)(2 * limbs), dst};
@@ -746,7 +756,7 @@ void fill_descriptors(const BatchPlan& bp, unsigned char* H, uint32_t* arena, ui
                 } else if (pair && i < 2) {
                     if (i == 0) {
                         const size_t li = launch_of(kRowPair, 0);
-                        rj[li][rfill[li]++] = RowJob{dst, bslot, arena + slot * (first + 1), gr.m > 2 ? 1u : 0u};
+                        rj[li][rfill[li]++] = RowJob{dst, bslot, base + slot * (first + 1), gr.m > 2 ? 1u : 0u};
                     }
                 } else if (i == 0) {
                     const size_t li = launch_of(kRowTransformB, 0);
@@ -792,47 +802,64 @@ bool fits_inline(const BatchPlan& bp) {
 // D: device copy of the descriptors (may be null when H is given and every
 // launch fits inline); H: host copy (null inside captures, which always read
 // device descriptors).  tw: twiddle tables to use instead of the library's.
-int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned char* H, hipStream_t s,
-                     bool timed, const Twiddles* tw_tables = nullptr) {
-    auto list = [&](auto* tag, size_t off, uint32_t n) {
-        using T = std::remove_pointer_t<decltype(tag)>;
+struct LaunchSource {
+    const BatchPlan& bp;
+    const unsigned char* D;
+    const unsigned char* H;
+    template <class T>
+    JobList<T> list(size_t off, uint32_t n) const {
         if (H && n <= (uint32_t)kInlineJobs) return inline_jobs<T>(reinterpret_cast<const T*>(H + off), n);
         return device_jobs<T>(reinterpret_cast<const T*>(D + off));
-    };
-    if (!bp.school.empty()) {
-        const uint32_t ns = (uint32_t)bp.school.size();
-        PINTT_HIP(launch_schoolbook(list((SchoolJob*)nullptr, bp.school_off, ns), ns, s));
     }
+};
+
+// The schoolbook products of a batch (one launch), if any.
+int enqueue_school(const LaunchSource& src, hipStream_t s) {
+    const BatchPlan& bp = src.bp;
+    if (bp.school.empty()) return PINTT_OK;
+    const uint32_t ns = (uint32_t)bp.school.size();
+    PINTT_HIP(launch_schoolbook(src.list<SchoolJob>(bp.school_off, ns), ns, s));
+    return PINTT_OK;
+}
+
+// The passes of bucket bi, in order on `s`.
+int enqueue_bucket(const LaunchSource& src, size_t bi, hipStream_t s, bool timed, const Twiddles* tw_tables) {
     // Grid z carries the job index: launches of more than 65535 jobs go in pieces.
     constexpr uint32_t kMaxJobs = 65535;
-    for (size_t bi = 0; bi < bp.buckets.size(); ++bi) {
-        const BucketLayout& b = bp.buckets[bi];
-        const Plan& pl = bp.plans[bi];
-        const Twiddles tw = tw_tables ? *tw_tables : twiddles(set_for(b.log2n));
-        if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
-        for (uint32_t k = 0; k < b.nfwd; k += kMaxJobs) {
-            const uint32_t c = std::min(kMaxJobs, b.nfwd - k);
-            PINTT_HIP(launch_forward_columns(pl, list((FwdJob*)nullptr, b.fwd_off + k * sizeof(FwdJob), c), c, tw, s));
-        }
-        if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
-        for (const RowLaunch& rl : b.rows)
-            for (uint32_t k = 0; k < rl.count; k += kMaxJobs) {
-                const uint32_t c = std::min(kMaxJobs, rl.count - k);
-                PINTT_HIP(launch_row_convolution(pl, list((RowJob*)nullptr, rl.off + k * sizeof(RowJob), c), c,
-                                                 rl.kind, tw, s));
-            }
-        if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
-        for (uint32_t k = 0; k < b.ninv; k += kMaxJobs) {
-            const uint32_t c = std::min(kMaxJobs, b.ninv - k);
-            PINTT_HIP(launch_inverse_columns(pl, list((InvJob*)nullptr, b.inv_off + k * sizeof(InvJob), c), c,
-                                             tw, s));
-        }
-        if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
-        const uint32_t ng = (uint32_t)b.groups.size();
-        PINTT_HIP(launch_crt_carry(list((CarryJob*)nullptr, b.carry_off, ng), ng, b.nblocks, b.log2n,
-                                   crt_constants(b.log2n), s));
-        if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
+    const BucketLayout& b = src.bp.buckets[bi];
+    const Plan& pl = src.bp.plans[bi];
+    const Twiddles tw = tw_tables ? *tw_tables : twiddles(set_for(b.log2n));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[0], s));
+    for (uint32_t k = 0; k < b.nfwd; k += kMaxJobs) {
+        const uint32_t c = std::min(kMaxJobs, b.nfwd - k);
+        PINTT_HIP(launch_forward_columns(pl, src.list<FwdJob>(b.fwd_off + k * sizeof(FwdJob), c), c, tw, s));
     }
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[1], s));
+    for (const RowLaunch& rl : b.rows)
+        for (uint32_t k = 0; k < rl.count; k += kMaxJobs) {
+            const uint32_t c = std::min(kMaxJobs, rl.count - k);
+            PINTT_HIP(launch_row_convolution(pl, src.list<RowJob>(rl.off + k * sizeof(RowJob), c), c, rl.kind, tw,
+                                             s));
+        }
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[2], s));
+    for (uint32_t k = 0; k < b.ninv; k += kMaxJobs) {
+        const uint32_t c = std::min(kMaxJobs, b.ninv - k);
+        PINTT_HIP(launch_inverse_columns(pl, src.list<InvJob>(b.inv_off + k * sizeof(InvJob), c), c, tw, s));
+    }
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[3], s));
+    const uint32_t ng = (uint32_t)b.groups.size();
+    PINTT_HIP(launch_crt_carry(src.list<CarryJob>(b.carry_off, ng), ng, b.nblocks, b.log2n, crt_constants(b.log2n),
+                               s));
+    if (timed) PINTT_HIP(hipEventRecord(g.ev[4], s));
+    return PINTT_OK;
+}
+
+// Every launch of a batch, in order on `s` (buckets one after another).
+int enqueue_launches(const BatchPlan& bp, const unsigned char* D, const unsigned char* H, hipStream_t s,
+                     bool timed, const Twiddles* tw_tables = nullptr) {
+    const LaunchSource src{bp, D, H};
+    PINTT_TRY(enqueue_school(src, s));
+    for (size_t bi = 0; bi < bp.buckets.size(); ++bi) PINTT_TRY(enqueue_bucket(src, bi, s, timed, tw_tables));
     return PINTT_OK;
 }
 
@@ -893,9 +920,49 @@ void free_graph(BatchGraph* bg) {
     delete bg;
 }
 
+// Capture the launches of `bp` into a graph on the capturing stream cs.  With
+// branches, the schoolbook launch and every length bucket (which then own
+// disjoint parts of the arena and of the carry maps) hang off one fork node
+// as independent chains and join at the end, so the graph may run them at
+// the same time: small buckets and the tails of large launches fill the
+// machine beside the largest bucket instead of after it.
+int capture_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t cs, bool branches) {
+    if (!branches || bp.buckets.size() + (bp.school.empty() ? 0 : 1) < 2)
+        return enqueue_launches(bp, D, nullptr, cs, false);
+    const LaunchSource src{bp, D, nullptr};
+    const size_t nb = bp.buckets.size() + 1;  // branch 0: schoolbook products
+    std::vector<hipStream_t> st(nb, nullptr);
+    std::vector<hipEvent_t> ev(nb + 1, nullptr);
+    int rc = PINTT_OK;
+    for (size_t i = 0; i < nb && !rc; ++i)
+        if (hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) != hipSuccess)
+            rc = fail(PINTT_ERR_HIP, "branch stream");
+    for (size_t i = 0; i <= nb && !rc; ++i)
+        if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+            rc = fail(PINTT_ERR_HIP, "branch event");
+    // ev[nb] forks; ev[i] joins branch i.
+    if (!rc && hipEventRecord(ev[nb], cs) != hipSuccess) rc = fail(PINTT_ERR_HIP, "fork");
+    for (size_t i = 0; i < nb && !rc; ++i) {
+        if (hipStreamWaitEvent(st[i], ev[nb], 0) != hipSuccess) {
+            rc = fail(PINTT_ERR_HIP, "branch wait");
+            break;
+        }
+        rc = i == 0 ? enqueue_school(src, st[i]) : enqueue_bucket(src, i - 1, st[i], false, nullptr);
+        if (!rc && (hipEventRecord(ev[i], st[i]) != hipSuccess || hipStreamWaitEvent(cs, ev[i], 0) != hipSuccess))
+            rc = fail(PINTT_ERR_HIP, "join");
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t x : st)
+        if (x) (void)hipStreamDestroy(x);
+    return rc;
+}
+
 int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
     BatchPlan bp;
-    PINTT_TRY(plan_batch(reqs, &bp));
+    PlanMode mode;
+    mode.disjoint = g.graph_branches;
+    PINTT_TRY(plan_batch(reqs, &bp, mode));
     if (!bp.buckets.empty()) {
         PINTT_TRY(ensure_twiddles(bp.min_log2n));
         PINTT_TRY(ensure_twiddles(bp.max_log2n));
@@ -921,7 +988,7 @@ int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
     if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         rc = fail(PINTT_ERR_HIP, "hipStreamBeginCapture");
     } else {
-        rc = enqueue_launches(bp, bg->desc, nullptr, cs, false);
+        rc = capture_launches(bp, bg->desc, cs, g.graph_branches);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(cs, &graph);
         if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
@@ -1851,6 +1918,11 @@ int gpu_ntt_list_plans(int log2n, int* out, int max_plans) {
 void gpu_ntt_set_split(int mode) {
     std::lock_guard<std::mutex> lock(g_mu);
     g.split_mode = mode < 0 ? 0 : (mode > kMaxChunks ? kMaxChunks : mode);
+}
+
+void gpu_ntt_set_graph_branches(int enable) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g.graph_branches = enable != 0;
 }
 
 void gpu_ntt_set_timing(int enable) {

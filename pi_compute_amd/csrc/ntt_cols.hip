@@ -21,16 +21,22 @@ __global__ void __launch_bounds__((Tile<LOGR, LOGW, true>::NT))
     using TL = Tile<LOGR, LOGW, true>;
     __shared__ uint32_t lds[TL::WORDS];
     __shared__ uint2 zt[TL::L];  // the R - 1 twiddles of the column layers (entry 0 unused)
-    const FwdJob job = PINTT_JOB(FwdJob, jobs, blockIdx.z);
-    const int prime = blockIdx.y;
+    // Tiles of 2^5 columns or more read whole 128-byte lines of their own: the
+    // three primes of a tile run on one XCD, one after another, and share its
+    // L2 copy of the limbs.  Narrower tiles share lines with their neighbours:
+    // each XCD takes a contiguous range of tiles.
+    constexpr bool kGroupPrimes = LOGW >= 5;
+    const ColBlock cb = kGroupPrimes ? prime_grouped_block(blockIdx.x + gridDim.x * (blockIdx.y + kPrimes * blockIdx.z),
+                                                           gridDim.x, gridDim.z)
+                                     : ColBlock{xcd_swizzle(blockIdx.x, gridDim.x), blockIdx.y, blockIdx.z};
+    const FwdJob job = PINTT_JOB(FwdJob, jobs, cb.job);
+    const int prime = (int)cb.prime;
     const uint32_t p = prime_of<SET>(prime);
     global_ptr<uint32_t> dst = as_global(job.dst) + ((size_t)prime << (LOGR + log2c));
     global_ptr<const uint32_t> src = as_global(job.src);
     const TwTable z = PINTT_FWD_TABLE(tw, prime);
     for (int i = threadIdx.x; i < TL::L; i += TL::NT) zt[i] = tw_at(z, i);
-    // Neighbouring column tiles share 128-byte lines; give each XCD (workgroups
-    // are dealt to the 8 XCDs round-robin) a contiguous range of tiles.
-    const uint32_t c0 = xcd_swizzle(blockIdx.x, gridDim.x) << LOGW;
+    const uint32_t c0 = cb.tile << LOGW;
     const uint32_t w0 = TL::word(threadIdx.x);  // element e: tile position e NT + threadIdx.x
 #pragma unroll
     for (int e = 0; e < kElemsPerThread; ++e) {

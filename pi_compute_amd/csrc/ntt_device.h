@@ -94,6 +94,33 @@ __device__ __forceinline__ uint32_t wave_search(uint32_t n, uint32_t v, Key key)
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
+// Workgroup order of a forward column grid (column tiles, prime, job).  The
+// dispatcher deals workgroups to the 8 XCDs round-robin in linear order
+// (x fastest), and every XCD has its own L2.  The three primes of one
+// (tile, job) read the same limbs, so workgroup L runs the (tile, prime, job)
+// below: the pairs u = (tile, job), u = tile + nx * job, go in groups of 8,
+// pair 8 q + x on XCD x, its primes 0, 1, 2 that XCD's workgroups 3 q .. 3 q
+// + 2; the limbs come from HBM once and from that L2 twice.  The last
+// (nx nz mod 8) pairs, fewer than 24 workgroups, go in linear order.  A
+// bijection of [0, 3 nx nz) for every nx, nz (tests/test_col_blocks.py).
+struct ColBlock {
+    uint32_t tile, prime, job;
+};
+__host__ __device__ constexpr ColBlock prime_grouped_block(uint32_t L, uint32_t nx, uint32_t nz) {
+    const uint32_t full = (nx * nz) >> 3;  // whole groups of 8 pairs (24 workgroups)
+    uint32_t u = 0, prime = 0;
+    if (L < 24 * full) {
+        const uint32_t s = L >> 3;  // L's place on its XCD
+        prime = s % 3;
+        u = (s / 3) * 8 + (L & 7u);
+    } else {
+        const uint32_t r = L - 24 * full;
+        prime = r % 3;
+        u = 8 * full + r / 3;
+    }
+    return ColBlock{u % nx, prime, u / nx};
+}
+
 // Bank layout of a tile in LDS.  Linear position x of a tile sits at word
 // x ^ T(x), T(x) = ((x >> s1) & m1) << t1  ^  ((x >> s2) & m2) << t2, which
 // permutes the words of each line of 32 (T changes bits 0..4 only and is

@@ -78,6 +78,7 @@ int gpu_ntt_decimal_leaves_device(size_t, const uint64_t* const*, const size_t*,
 int gpu_ntt_bs_leaves(uint64_t, uint64_t, size_t, uint64_t*, uint32_t*, int32_t*) { return no_device(); }
 int gpu_ntt_synchronize(void) { return PINTT_OK; }
 void gpu_ntt_set_split(int) {}
+void gpu_ntt_set_graph_branches(int) {}
 // The length rule of the engine: the least 2^k >= 2 an + 2 bn - 1 with
 // 10 <= k <= 25, -1 above.
 int gpu_ntt_transform_log2(size_t an, size_t bn) {

@@ -435,15 +435,22 @@ def _upload_batch(prods):
     return bufs, d_a, an, d_b, bn, d_o
 
 
-def test_graph_replays_match_eager(engine, oracle):
+@pytest.mark.parametrize("branches", [True, False], ids=["bucket-branches", "one-chain"])
+def test_graph_replays_match_eager(engine, oracle, branches):
     """A captured batch replayed several times, with eager products and host
-    synchronisations in between; every replay is checked."""
+    synchronisations in between; every replay is checked.  Captured with each
+    length bucket (and the schoolbook products) as a branch of its own, and as
+    one chain."""
     from pi_compute_amd import workloads
 
     prods = workloads.tree_level_replay(count=96, seed=11, median_bits=1 << 15, hi_bits=1 << 20)
     want = [oracle.gmp_mul(a, b) for a, b, _ in prods]
     bufs, d_a, an, d_b, bn, d_o = _upload_batch(prods)
-    graph = engine.BatchGraph(d_a, an, d_b, bn, d_o)
+    engine.set_graph_branches(branches)
+    try:
+        graph = engine.BatchGraph(d_a, an, d_b, bn, d_o)
+    finally:
+        engine.set_graph_branches(True)
     try:
         assert graph.nodes >= 4
         for rep in range(4):
