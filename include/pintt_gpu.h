@@ -189,8 +189,9 @@ void gpu_ntt_set_split(int mode);
 /* Graphs made by gpu_ntt_graph_create after this call capture each
  * transform-length bucket (and the schoolbook products) as an independent
  * branch with its own working memory, so a replay may run them at the same
- * time (enable != 0, the default), or capture every launch in one chain
- * (0).  Products are identical either way.  No reference counterpart (the
+ * time (enable != 0, the default; the graph then holds the sum of the
+ * buckets' working memory rather than the largest), or capture every launch
+ * in one chain (0).  Products are identical either way.  No reference counterpart (the
  * reference has no batch launch; its caller is
  * src/engine/binary_splitting.cpp:161-293). */
 void gpu_ntt_set_graph_branches(int enable);
