@@ -43,6 +43,7 @@ from pi_compute_amd import hiprt  # noqa: E402
 
 METRIC = "exact mults/s at 2^26 bits (%HBM roofline); 100M-digit pi time, GPU vs GMP"
 HBM_PEAK_GBS = 8000.0
+SHIPPED_PLANS = ROOT / "pi_compute_amd" / "data" / "plans_gfx950.txt"
 K_BITS = 26
 SEED = 26
 
@@ -579,8 +580,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the tree-level replay batch leg")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--plans-file", default="", metavar="PATH",
-                    help="load tile plans and their tuned times (gpu_ntt_load_plans) instead of tuning lengths "
-                         "at first use; every run then plans the batch the same way")
+                    help="load tile plans and their tuned times (gpu_ntt_load_plans) from this file instead of "
+                         "the shipped table pi_compute_amd/data/plans_gfx950.txt")
     ap.add_argument("--no-per-call", action="store_true",
                     help="batch leg: skip the one-product-per-call timing (4,096 calls, about 20,000 launches)")
     ap.add_argument("--plan", action="append", default=[], metavar="N:R,W,ROWS",
@@ -588,7 +589,9 @@ def main() -> int:
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "k26_multiply_traffic.json"),
                     help="measured HBM bytes per pass (rocprofv3 FETCH_SIZE/WRITE_SIZE summary); its tile "
                          "plans are used unless --plan or --autotune is given")
-    ap.add_argument("--autotune", action="store_true", help="tune the tile plans instead of using the profiled ones")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune every length at first use instead of loading a plan table and pinning the profiled "
+                         "plans")
     ap.add_argument("--pi-digits", type=int, default=100_000_000,
                     help="digits of the pi leg (GMP-only vs GPU runs of pi_compute_hip); 0 skips it")
     ap.add_argument("--no-config1", action="store_true", help="skip the 2^22 x 2^22-bit single-product leg")
@@ -616,8 +619,12 @@ def main() -> int:
         print("gpu_ntt_init failed: " + lib.gpu_ntt_last_error().decode(), file=sys.stderr)
         return 1
 
-    if args.plans_file:
-        if lib.gpu_ntt_load_plans(str(args.plans_file).encode()) != 0:
+    # Tile plans: a given file, else the shipped table (tuned alone per length,
+    # then together on the tree-level replay: pi_compute_amd/tune_plans.py),
+    # which pi_compute_hip loads too; --autotune tunes at first use instead.
+    plans_file = args.plans_file or ("" if args.autotune else str(SHIPPED_PLANS))
+    if plans_file:
+        if lib.gpu_ntt_load_plans(plans_file.encode()) != 0:
             print("gpu_ntt_load_plans failed: " + lib.gpu_ntt_last_error().decode(), file=sys.stderr)
             return 1
     specs = list(args.plan)

@@ -241,14 +241,18 @@ const char* gpu_ntt_last_error(void);
  * stream capture.  No reference counterpart: the reference had one fixed
  * radix-2 layout (src/gpu/int_ntt.cu:192-220). */
 int gpu_ntt_autotune(int log2n_min, int log2n_max);
+/* The plan of a length and its tuned time *ms: milliseconds per product of
+ * two N/4-limb operands (timed as a batch of 2^(22-n) products below 2^22
+ * points, where one product is launch-bound), 0 if the length was not
+ * tuned. */
 int gpu_ntt_get_plan(int log2n, int* log2r, int* log2w, int* log2rows, float* ms);
 /* Force a plan (log2r <= 0 selects the heuristic); PINTT_ERR_ARGS if that
  * tile shape is not compiled in. */
 int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows);
 /* Writes up to max_plans (log2r, log2w, log2rows) triples; returns the count. */
 int gpu_ntt_list_plans(int log2n, int* out, int max_plans);
-/* Tile plans (and their tuned times) to / from a text file, so that a
- * process can skip tuning lengths another run already tuned. */
+/* Tile plans (and their tuned times per product) to / from a text file, so
+ * that a process can skip tuning lengths another run already tuned. */
 int gpu_ntt_save_plans(const char* path);
 int gpu_ntt_load_plans(const char* path);
 

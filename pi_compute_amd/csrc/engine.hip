@@ -453,7 +453,7 @@ struct PlanMode {
 Plan plan_lookup(int log2n) { return g.have_plan[log2n] ? g.plan[log2n] : make_plan(log2n); }
 
 // Estimated cost of `transforms` transforms of length 2^k: a third of the
-// tuned time of a full product (three transforms) when known, else N log N.
+// tuned time of one full product (three transforms) when known, else N log N.
 double transform_cost(int k, int transforms) {
     double base = g.have_plan[k] && g.plan_ms[k] > 0 ? g.plan_ms[k] / 3.0
                                                      : 1e-7 * (double)((size_t)1 << k) * k;
@@ -1062,7 +1062,8 @@ int tune(int log2n) {
     g.school = true;
     g.timing = saved_timing;
     g.plan[log2n] = best;
-    g.plan_ms[log2n] = best_ms < 1e29f ? best_ms : 0.f;
+    // Kept per product: the chunk rule prices lengths against each other.
+    g.plan_ms[log2n] = best_ms < 1e29f ? best_ms / (float)count : 0.f;
     g.have_plan[log2n] = true;
     return rc;
 }
@@ -1867,10 +1868,10 @@ int gpu_ntt_save_plans(const char* path) {
     std::lock_guard<std::mutex> lock(g_mu);
     std::FILE* f = std::fopen(path, "w");
     if (!f) return fail(PINTT_ERR_ARGS, std::string("cannot write ") + path);
-    std::fprintf(f, "# log2n log2r log2w log2rows tuned_ms\n");
+    std::fprintf(f, "# log2n log2r log2w log2rows tuned_ms (per product)\n");
     for (int n = kMinLog2N; n <= kMaxLog2N; ++n)
         if (g.have_plan[n])
-            std::fprintf(f, "%d %d %d %d %.6f\n", n, g.plan[n].log2r, g.plan[n].log2w, g.plan[n].log2rows,
+            std::fprintf(f, "%d %d %d %d %.6g\n", n, g.plan[n].log2r, g.plan[n].log2w, g.plan[n].log2rows,
                          (double)g.plan_ms[n]);
     return std::fclose(f) == 0 ? PINTT_OK : fail(PINTT_ERR_ARGS, "write failed");
 }
