@@ -251,6 +251,14 @@ int gpu_ntt_get_plan(int log2n, int* log2r, int* log2w, int* log2rows, float* ms
 int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows);
 /* Writes up to max_plans (log2r, log2w, log2rows) triples; returns the count. */
 int gpu_ntt_list_plans(int log2n, int* out, int max_plans);
+/* Batch plans: a length bucket of a batch with 2^24 points of forward
+ * transforms or more (a tree level's hundreds of products of one length) is
+ * throughput-bound and takes its length's batch plan when one is set; other
+ * buckets, single products among them, take the plan above.  log2r <= 0
+ * clears it.  PINTT_ERR_ARGS if the tile shape is not compiled in (set) or no
+ * batch plan is set (get).  Plan files hold both tables. */
+int gpu_ntt_set_batch_plan(int log2n, int log2r, int log2w, int log2rows);
+int gpu_ntt_get_batch_plan(int log2n, int* log2r, int* log2w, int* log2rows);
 /* Tile plans (and their tuned times per product) to / from a text file, so
  * that a process can skip tuning lengths another run already tuned. */
 int gpu_ntt_save_plans(const char* path);

@@ -68,6 +68,10 @@ ABI = {
     ),
     "gpu_ntt_set_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gpu_ntt_list_plans": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+    "gpu_ntt_set_batch_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "gpu_ntt_get_batch_plan": (
+        ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                       ctypes.POINTER(ctypes.c_int)]),
     "gpu_ntt_mul_batch": (
         ctypes.c_int,
         [ctypes.c_size_t, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_u64p),
@@ -310,6 +314,21 @@ def product_plan(an: int, bn: int) -> dict:
 def set_plan(log2n: int, log2r: int, log2w: int, log2rows: int) -> None:
     lib = load_library()
     _check(lib, lib.gpu_ntt_set_plan(log2n, log2r, log2w, log2rows))
+
+
+def set_batch_plan(log2n: int, log2r: int, log2w: int, log2rows: int) -> None:
+    """Plan for large buckets of length 2^log2n (log2r <= 0 clears it)."""
+    lib = load_library()
+    _check(lib, lib.gpu_ntt_set_batch_plan(log2n, log2r, log2w, log2rows))
+
+
+def get_batch_plan(log2n: int):
+    """(log2r, log2w, log2rows) of the length's batch plan, or None."""
+    lib = load_library()
+    r, w, rows = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    if lib.gpu_ntt_get_batch_plan(log2n, ctypes.byref(r), ctypes.byref(w), ctypes.byref(rows)) != 0:
+        return None
+    return (r.value, w.value, rows.value)
 
 
 def autotune(lo: int, hi: int) -> None:

@@ -136,6 +136,10 @@ struct Engine {
     bool autotune = true;
     bool have_plan[kMaxLog2N + 1] = {};
     Plan plan[kMaxLog2N + 1] = {};
+    // Plans for large buckets (kBatchPlanMinPoints points of forward
+    // transforms or more), tuned on a tree level; the plan above otherwise.
+    bool have_bplan[kMaxLog2N + 1] = {};
+    Plan bplan[kMaxLog2N + 1] = {};
     float plan_ms[kMaxLog2N + 1] = {};
     int split_mode = 1;  // 0: never cut operands into chunks, 1: when cheaper, m >= 2: m chunks when valid
     bool graph_branches = true;  // graphs capture each length bucket as a branch of its own
@@ -445,8 +449,15 @@ int plan_for(int log2n, Plan* out);
 struct PlanMode {
     bool tune = true;
     uint32_t max_chunks = kMaxChunks;
+    bool throughput = false;  // the batch is large enough to be bound by throughput, not latency
     bool disjoint = false;  // every bucket in its own part of the arena, so buckets may run at once
 };
+
+// Buckets of at least this many points of forward transforms (2^24: some
+// hundreds of transforms of 2^15 .. 2^17 points) are throughput-bound and take
+// a length's batch plan; smaller ones, single products among them, are bound
+// by latency and take the plan tuned for the length alone.
+constexpr size_t kBatchPlanMinPoints = (size_t)1 << 24;
 
 // The plan of a length without tuning: the tuned or loaded one, else the
 // heuristic.
@@ -481,6 +492,10 @@ bool chunk_shape(size_t an, size_t bn, uint32_t m, size_t* h, int* k) {
     *k = kk;
     return true;
 }
+
+// Unbalanced products alone (or in small batches) are cut into chunks only
+// from this transform length on.
+constexpr int kChunkAloneMinLog2N = 21;
 
 int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{}) {
     Request r = req;
@@ -529,7 +544,12 @@ int plan_group(const Request& req, Group* gr, const PlanMode& mode = PlanMode{})
             }
         }
     }
-    if (!r.square && g.split_mode) {
+    // The tuned times price throughput: a product or a small batch below
+    // 2^kChunkAloneMinLog2N points is bound by latency, where the shorter
+    // transforms of a cut do not pay for its extra transform (a single 2^19-
+    // point product cut in two took 13% longer).  Forced cuts still apply.
+    const bool latency_bound = !mode.throughput && gr->log2n < kChunkAloneMinLog2N;
+    if (!r.square && g.split_mode && !(latency_bound && g.split_mode == 1)) {
         if (mode.tune && g.autotune && g.split_mode == 1 && !g.have_plan[gr->log2n]) {
             Plan tmp;
             PINTT_TRY(plan_for(gr->log2n, &tmp));
@@ -607,8 +627,20 @@ bool school_eligible(const Request& r) {
 
 // Groups, buckets, descriptor layout and resource sizes of a batch.  May tune
 // tile plans for lengths seen for the first time (which waits on the device).
-int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp, const PlanMode& mode = PlanMode{}) {
+int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp, const PlanMode& mode_in = PlanMode{}) {
     bp->groups.resize(reqs.size());
+    // Points of the batch's transforms, uncut: whether it is bound by
+    // throughput (the regime the tuned times measure) or by latency.
+    PlanMode mode = mode_in;
+    {
+        size_t points = 0;
+        for (const Request& r : reqs) {
+            if (school_eligible(r)) continue;
+            const int k = log2_for(r.an, r.bn);
+            if (k >= 0) points += (r.square ? 2u : 3u) * ((size_t)1 << k);
+        }
+        mode.throughput = points >= kBatchPlanMinPoints;
+    }
     // Schoolbook products need no transform plan: only their request is kept
     // (a tree level's first batches hold tens of thousands of them).
     for (size_t i = 0; i < reqs.size(); ++i) {
@@ -632,13 +664,20 @@ int plan_batch(const std::vector<Request>& reqs, BatchPlan* bp, const PlanMode& 
         }
         it->groups.push_back(i);
     }
-    // Tile plans first: the row launches depend on them.
+    // Tile plans first: the row launches depend on them.  A bucket with
+    // kBatchPlanMinPoints points of forward transforms or more takes the
+    // length's batch plan when there is one.
     bp->plans.resize(buckets.size());
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
-        if (mode.tune) {
-            PINTT_TRY(plan_for(buckets[bi].log2n, &bp->plans[bi]));
+        const int k = buckets[bi].log2n;
+        size_t nfwd = 0;
+        for (size_t gi : buckets[bi].groups) nfwd += groups[gi].m + (groups[gi].r.square ? 0 : 1);
+        if (g.have_bplan[k] && (nfwd << k) >= kBatchPlanMinPoints) {
+            bp->plans[bi] = g.bplan[k];
+        } else if (mode.tune) {
+            PINTT_TRY(plan_for(k, &bp->plans[bi]));
         } else {
-            bp->plans[bi] = plan_lookup(buckets[bi].log2n);
+            bp->plans[bi] = plan_lookup(k);
         }
     }
     // Buckets run one after another on one stream and reuse the arena.
@@ -1863,6 +1902,30 @@ int gpu_ntt_set_plan(int log2n, int log2r, int log2w, int log2rows) {
     return PINTT_OK;
 }
 
+int gpu_ntt_set_batch_plan(int log2n, int log2r, int log2w, int log2rows) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (log2n < kMinLog2N || log2n > kMaxLog2N) return fail(PINTT_ERR_ARGS, "length out of range");
+    if (log2r <= 0) {
+        g.have_bplan[log2n] = false;
+        return PINTT_OK;
+    }
+    Plan pl{log2n, log2r, log2n - log2r, log2w, log2rows};
+    if (!plan_valid(pl)) return fail(PINTT_ERR_ARGS, "tile plan not compiled into this library");
+    g.bplan[log2n] = pl;
+    g.have_bplan[log2n] = true;
+    return PINTT_OK;
+}
+
+int gpu_ntt_get_batch_plan(int log2n, int* log2r, int* log2w, int* log2rows) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (log2n < kMinLog2N || log2n > kMaxLog2N) return fail(PINTT_ERR_ARGS, "length out of range");
+    if (!g.have_bplan[log2n]) return fail(PINTT_ERR_ARGS, "no batch plan for this length");
+    if (log2r) *log2r = g.bplan[log2n].log2r;
+    if (log2w) *log2w = g.bplan[log2n].log2w;
+    if (log2rows) *log2rows = g.bplan[log2n].log2rows;
+    return PINTT_OK;
+}
+
 int gpu_ntt_save_plans(const char* path) {
     if (!path) return fail(PINTT_ERR_ARGS, "null path");
     std::lock_guard<std::mutex> lock(g_mu);
@@ -1873,6 +1936,10 @@ int gpu_ntt_save_plans(const char* path) {
         if (g.have_plan[n])
             std::fprintf(f, "%d %d %d %d %.6g\n", n, g.plan[n].log2r, g.plan[n].log2w, g.plan[n].log2rows,
                          (double)g.plan_ms[n]);
+    std::fprintf(f, "# batch log2n log2r log2w log2rows: plans for large buckets\n");
+    for (int n = kMinLog2N; n <= kMaxLog2N; ++n)
+        if (g.have_bplan[n])
+            std::fprintf(f, "batch %d %d %d %d\n", n, g.bplan[n].log2r, g.bplan[n].log2w, g.bplan[n].log2rows);
     return std::fclose(f) == 0 ? PINTT_OK : fail(PINTT_ERR_ARGS, "write failed");
 }
 
@@ -1887,6 +1954,17 @@ int gpu_ntt_load_plans(const char* path) {
         if (line[0] == '#') continue;
         int n, r, w, rows;
         double ms;
+        if (std::sscanf(line, "batch %d %d %d %d", &n, &r, &w, &rows) == 4) {
+            if (n < kMinLog2N || n > kMaxLog2N) continue;
+            Plan pl{n, r, n - r, w, rows};
+            if (!plan_valid(pl)) {
+                rc = fail(PINTT_ERR_ARGS, "plan file names a tile plan not compiled into this library");
+                continue;
+            }
+            g.bplan[n] = pl;
+            g.have_bplan[n] = true;
+            continue;
+        }
         if (std::sscanf(line, "%d %d %d %d %lf", &n, &r, &w, &rows, &ms) != 5) continue;
         if (n < kMinLog2N || n > kMaxLog2N) continue;
         Plan pl{n, r, n - r, w, rows};

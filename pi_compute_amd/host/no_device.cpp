@@ -120,6 +120,8 @@ int gpu_ntt_chunk_plan(size_t an, size_t bn, int chunks, size_t* chunk_limbs, in
 const char* gpu_ntt_last_error(void) { return kNoDevice; }
 int gpu_ntt_autotune(int, int) { return no_device(); }
 int gpu_ntt_get_plan(int, int*, int*, int*, float*) { return no_device(); }
+int gpu_ntt_set_batch_plan(int, int, int, int) { return no_device(); }
+int gpu_ntt_get_batch_plan(int, int*, int*, int*) { return no_device(); }
 int gpu_ntt_set_plan(int, int, int, int) { return no_device(); }
 int gpu_ntt_list_plans(int, int*, int) { return 0; }
 int gpu_ntt_save_plans(const char*) { return no_device(); }

@@ -10,14 +10,16 @@ Two steps, on the first visible GPU:
 2. The lengths a tree level uses are then tuned together, on the tree-level
    replay batch (4,096 products, lognormal bit lengths of median 2^18, every
    third a square; pi_compute_amd.workloads) captured as a graph: for each of
-   its length buckets, largest first, every compiled plan of that length is
-   timed as the replay time of the whole graph with the other buckets' plans
-   held, and the fastest is kept when it beats the current one by more than
-   half a percent.  Tuned alone, a length is timed on one shape of product
-   running by itself; in a batch its launches run beside the other buckets'
-   (each bucket is a graph branch), and the tile shape that wins can differ
-   (2^16 points: register columns of 8 columns alone, LDS columns of 32 in
-   the batch).
+   its length buckets large enough to take a batch plan (2^24 points of
+   forward transforms or more), largest first, every compiled plan of that
+   length is timed as its batch plan by the replay time of the whole graph
+   with the other buckets' plans held, and the fastest becomes the batch plan
+   when it beats the length's own plan by more than half a percent.  Tuned
+   alone, a length is timed on one shape of product running by itself; in a
+   batch its launches run beside the other buckets' (each bucket is a graph
+   branch), and the tile shape that wins can differ (2^16 points: more rows
+   per column tile in the batch).  Batch plans apply only to buckets that
+   large (gpu_ntt_set_batch_plan), so single products keep the plans of step 1.
 
 The tuned times kept with each plan are those of step 1 (per product); the
 chunk rule compares lengths with them.  Products are bit-identical under
@@ -33,6 +35,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 DEFAULT_OUT = HERE / "data" / "plans_gfx950.txt"
+BATCH_PLAN_MIN_POINTS = 1 << 24  # the engine's kBatchPlanMinPoints
 
 
 def _replay_timer(pc, hiprt, prods):
@@ -101,34 +104,36 @@ def tune(out: Path, batch: bool = True, log=print) -> None:
         f"2^{n}: {tuple(pc.get_plan(n)[k] for k in ('log2r', 'log2w', 'log2rows'))}" for n in range(10, 26)))
     if batch:
         prods = workloads.tree_level_replay()
-        work = {}
+        points = {}  # forward transform points per length bucket
         for a, b, sq in prods:
             if a.size <= 512 and b.size <= 512:
                 continue  # schoolbook products: no transform
-            k = pc.product_plan(a.size, b.size)["log2n"]
-            work[k] = work.get(k, 0) + (1 << k)
+            pl = pc.product_plan(a.size, b.size)
+            k = pl["log2n"]
+            points[k] = points.get(k, 0) + (pl["chunks"] + (0 if sq else 1) << k)
+        for k in range(10, 26):
+            pc.set_batch_plan(k, 0, 0, 0)
         timed, close = _replay_timer(pc, hiprt, prods)
         try:
-            best_ms = timed()
-            log(f"tree-level replay with those plans: {best_ms:.4f} ms")
-            for k in sorted(work, key=lambda x: -work[x]):
-                cur = pc.get_plan(k)
-                cur_t = (cur["log2r"], cur["log2w"], cur["log2rows"])
+            log(f"tree-level replay with those plans: {timed():.4f} ms")
+            for k in sorted(points, key=lambda x: -points[x]):
+                if points[k] < BATCH_PLAN_MIN_POINTS:
+                    continue
+                own = pc.get_plan(k)
+                own_t = (own["log2r"], own["log2w"], own["log2rows"])
                 results = []
                 for cand in pc.list_plans(k):
-                    pc.set_plan(k, *cand)
+                    pc.set_batch_plan(k, *cand)
                     results.append((timed(), cand))
                 results.sort()
-                pc.set_plan(k, *cur_t)
-                cur_ms = timed()
+                pc.set_batch_plan(k, 0, 0, 0)
+                own_ms = timed()
                 ms, cand = results[0]
-                if cand != cur_t and ms < cur_ms * 0.995:
-                    pc.set_plan(k, *cand)
-                    best_ms = ms
-                    log(f"2^{k}: {cur_t} -> {cand} ({cur_ms:.4f} -> {ms:.4f} ms per replay)")
+                if cand != own_t and ms < own_ms * 0.995:
+                    pc.set_batch_plan(k, *cand)
+                    log(f"2^{k}: batch plan {cand} ({own_ms:.4f} -> {ms:.4f} ms per replay; alone {own_t})")
                 else:
-                    best_ms = cur_ms
-                    log(f"2^{k}: {cur_t} kept ({cur_ms:.4f} ms; best other {cand} {ms:.4f})")
+                    log(f"2^{k}: no batch plan ({own_t} {own_ms:.4f} ms; best other {cand} {ms:.4f})")
             log(f"tree-level replay, final plans: {timed():.4f} ms")
         finally:
             close()
