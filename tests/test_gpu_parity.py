@@ -664,3 +664,37 @@ def test_concurrent_callers(engine, oracle):
         for i in range(len(sizes)):
             assert np.array_equal(got[t][i], wants[t][i]), f"thread {t} product {i}"
         assert np.array_equal(moved[t], cases[t][0][0]), f"thread {t} upload/download"
+
+
+def test_batch_plan_on_a_large_bucket(engine, oracle, tmp_path):
+    """A batch plan (gpu_ntt_set_batch_plan) differing from the length's own
+    plan, taken by a bucket of 1,024 products of 2^13 points (2^24 points of
+    forward transforms): every product exact; the plan file keeps it."""
+    k = 13
+    own = engine.get_plan(k)
+    own_t = (own["log2r"], own["log2w"], own["log2rows"])
+    saved = engine.get_batch_plan(k)
+    other = next(p for p in engine.list_plans(k) if p != own_t)
+    rng = np.random.default_rng(13)
+    pairs = []
+    for i in range(1024):
+        an = int(rng.integers(1900, 2048))
+        a, b = pc.operands(1300 + i, 64 * an, 64 * int(rng.integers(1900, 2048)))
+        assert engine.transform_log2(a.size, b.size) == k
+        pairs.append((a, b))
+    try:
+        engine.set_batch_plan(k, *other)
+        assert engine.get_batch_plan(k) == other
+        outs = engine.mul_batch(pairs)
+        for i, ((a, b), got) in enumerate(zip(pairs, outs)):
+            assert np.array_equal(got, oracle.gmp_mul(a, b)), f"product {i} under batch plan {other}"
+        path = tmp_path / "plans.txt"
+        lib = pc.load_library()
+        assert lib.gpu_ntt_save_plans(str(path).encode()) == 0
+        assert f"batch {k} {other[0]} {other[1]} {other[2]}" in path.read_text()
+        engine.set_batch_plan(k, 0, 0, 0)
+        assert engine.get_batch_plan(k) is None
+        assert lib.gpu_ntt_load_plans(str(path).encode()) == 0
+        assert engine.get_batch_plan(k) == other
+    finally:
+        engine.set_batch_plan(k, *(saved or (0, 0, 0)))
