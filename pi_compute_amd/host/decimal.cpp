@@ -645,11 +645,12 @@ std::string to_decimal_device(const uint64_t* x, size_t xn, unsigned threads, si
     DecimalStats st;
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     leaf_digits = std::min<size_t>(std::max<size_t>(leaf_digits, kLeafDigits), PINTT_DECIMAL_LEAF_MAX_DIGITS);
-    // The true length and top limb, from the top in windows of 64 limbs.
+    // The true length and top limb, from the top in windows of 64 limbs that
+    // double while they hold only zeros.
     size_t n = xn;
     uint64_t top = 0;
-    while (n > 0 && !top) {
-        const size_t k = std::min<size_t>(n, 64);
+    for (size_t w = 64; n > 0 && !top; w = std::min<size_t>(2 * w, (size_t)1 << 20)) {
+        const size_t k = std::min<size_t>(n, w);
         std::vector<uint64_t> h(k);
         uint64_t* hp = h.data();
         const uint64_t* src = x + (n - k);

@@ -412,10 +412,13 @@ std::unique_ptr<DeviceQR> BinarySplitting::compute_device_qr(unsigned long a, un
     // the trees can go.
     auto keep = [&out](const DevVal& v, dev::Num& dst) {
         // The length is an upper bound: drop zero top limbs, reading the top
-        // in windows of 64 limbs.
+        // in windows of 64 limbs that double while they hold only zeros (the
+        // root's bound can exceed its value by tens of thousands of limbs,
+        // and every window read waits for the stream).
         size_t n = v.n;
-        for (bool more = true; more && n > 1;) {
-            const size_t k = std::min<size_t>(n, 64);
+        size_t w = 64;
+        for (bool more = true; more && n > 1; w = std::min<size_t>(2 * w, (size_t)1 << 20)) {
+            const size_t k = std::min<size_t>(n, w);
             std::vector<uint64_t> h(k);
             uint64_t* hp = h.data();
             const uint64_t* src = v.d + (n - k);
