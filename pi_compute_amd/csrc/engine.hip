@@ -1608,56 +1608,75 @@ int add_batch(size_t count, const uint64_t* const* d_x, const size_t* xn, const 
                          (uint32_t)nb, 0u};
         blocks += nb;
     }
-    std::lock_guard<std::mutex> lock(g_mu);
-    PINTT_TRY(ensure_init(-1));
-    PINTT_HIP(hipSetDevice(g.device));
-    PINTT_TRY(drain());  // the staging buffers below may be reused
-    if (count > g.add_job_cap) {
-        if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
-        (void)hipFree(g.add_jobs_dev);
-        (void)hipFree(g.add_status);
-        g.add_jobs_host = nullptr;
-        g.add_jobs_dev = nullptr;
-        g.add_status = nullptr;
-        g.add_job_cap = 0;
-        const size_t cap = std::max<size_t>(count, 1024);
-        if (hipHostMalloc(reinterpret_cast<void**>(&g.add_jobs_host), cap * sizeof(AddJob)) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void**>(&g.add_jobs_dev), cap * sizeof(AddJob)) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void**>(&g.add_status), cap * sizeof(uint32_t)) != hipSuccess)
-            return fail(PINTT_ERR_NO_MEMORY, "add descriptors");
-        g.add_job_cap = cap;
+    // The error flag and the per-job signs come back through a staging slot,
+    // waited for without the engine lock: other threads keep enqueueing
+    // while this batch and the work before it finish.
+    StagingSlot st;
+    PINTT_TRY(st.acquire((count + 1) / 2 + 1));
+    uint32_t* const host = reinterpret_cast<uint32_t*>(st.out());  // [0]: error flag, [1..count]: signs
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        PINTT_TRY(ensure_init(-1));
+        PINTT_HIP(hipSetDevice(g.device));
+        // Device scratch that earlier batches may still be using grows only
+        // after the stream drains.
+        if (count > g.add_job_cap || blocks > g.add_blk_cap) {
+            PINTT_TRY(drain());
+            if (count > g.add_job_cap) {
+                (void)hipFree(g.add_status);
+                g.add_status = nullptr;
+                g.add_job_cap = 0;
+                const size_t cap = std::max<size_t>(count, 1024);
+                if (hipMalloc(reinterpret_cast<void**>(&g.add_status), cap * sizeof(uint32_t)) != hipSuccess)
+                    return fail(PINTT_ERR_NO_MEMORY, "add status");
+                if (g.add_jobs_host) (void)hipHostFree(g.add_jobs_host);
+                (void)hipFree(g.add_jobs_dev);
+                g.add_jobs_host = nullptr;
+                g.add_jobs_dev = nullptr;
+                if (hipHostMalloc(reinterpret_cast<void**>(&g.add_jobs_host), cap * sizeof(AddJob)) != hipSuccess ||
+                    hipMalloc(reinterpret_cast<void**>(&g.add_jobs_dev), cap * sizeof(AddJob)) != hipSuccess)
+                    return fail(PINTT_ERR_NO_MEMORY, "add descriptors");
+                g.add_job_cap = cap;
+            }
+            if (blocks > g.add_blk_cap) {
+                (void)hipFree(g.add_maps);
+                (void)hipFree(g.add_carry);
+                g.add_maps = g.add_carry = nullptr;
+                g.add_blk_cap = 0;
+                const size_t cap = std::max<size_t>(blocks, 1 << 16);
+                if (hipMalloc(reinterpret_cast<void**>(&g.add_maps), cap) != hipSuccess ||
+                    hipMalloc(reinterpret_cast<void**>(&g.add_carry), cap) != hipSuccess)
+                    return fail(PINTT_ERR_NO_MEMORY, "add block state");
+                g.add_blk_cap = cap;
+            }
+        }
+        if (!g.add_error && hipMalloc(reinterpret_cast<void**>(&g.add_error), sizeof(uint32_t)) != hipSuccess)
+            return fail(PINTT_ERR_NO_MEMORY, "add error flag");
+        // Descriptors through the ring that product batches use (its slot
+        // waits only for its own previous upload).
+        const int ring = g.desc_next;
+        g.desc_next = (g.desc_next + 1) % Engine::kDescRing;
+        const size_t bytes = count * sizeof(AddJob);
+        PINTT_TRY(ensure_descriptors(ring, bytes));
+        std::memcpy(g.desc_host[ring], jobs.data(), bytes);
+        PINTT_TRY(begin_on(g.stream));
+        PINTT_HIP(hipMemcpyAsync(g.desc_dev[ring], g.desc_host[ring], bytes, hipMemcpyHostToDevice, g.stream));
+        PINTT_HIP(hipEventRecord(g.desc_done[ring], g.stream));
+        // A fill kernel, not hipMemsetAsync: no memset node if a caller ever captures this.
+        PINTT_HIP(launch_fill_u32(g.add_error, 0u, 1, g.stream));
+        PINTT_HIP(launch_add_batch(reinterpret_cast<const AddJob*>(g.desc_dev[ring]), (uint32_t)count,
+                                   (uint32_t)blocks, g.add_maps, g.add_carry, g.add_error, g.add_status, g.stream));
+        PINTT_HIP(hipMemcpyAsync(host, g.add_error, sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+        if (negative)
+            PINTT_HIP(hipMemcpyAsync(host + 1, g.add_status, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     g.stream));
+        PINTT_TRY(end_on(g.stream));
+        PINTT_TRY(st.record());
     }
-    if (blocks > g.add_blk_cap) {
-        (void)hipFree(g.add_maps);
-        (void)hipFree(g.add_carry);
-        g.add_maps = g.add_carry = nullptr;
-        g.add_blk_cap = 0;
-        const size_t cap = std::max<size_t>(blocks, 1 << 16);
-        if (hipMalloc(reinterpret_cast<void**>(&g.add_maps), cap) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void**>(&g.add_carry), cap) != hipSuccess)
-            return fail(PINTT_ERR_NO_MEMORY, "add block state");
-        g.add_blk_cap = cap;
-    }
-    if (!g.add_error && hipMalloc(reinterpret_cast<void**>(&g.add_error), sizeof(uint32_t)) != hipSuccess)
-        return fail(PINTT_ERR_NO_MEMORY, "add error flag");
-    std::memcpy(g.add_jobs_host, jobs.data(), count * sizeof(AddJob));
-    PINTT_TRY(begin_on(g.stream));
-    PINTT_HIP(hipMemcpyAsync(g.add_jobs_dev, g.add_jobs_host, count * sizeof(AddJob), hipMemcpyHostToDevice, g.stream));
-    // A fill kernel, not hipMemsetAsync: no memset node if a caller ever captures this.
-    PINTT_HIP(launch_fill_u32(g.add_error, 0u, 1, g.stream));
-    PINTT_HIP(launch_add_batch(g.add_jobs_dev, (uint32_t)count, (uint32_t)blocks, g.add_maps, g.add_carry, g.add_error,
-                               g.add_status, g.stream));
-    uint32_t err = 0;
-    PINTT_HIP(hipMemcpyAsync(&err, g.add_error, sizeof(err), hipMemcpyDeviceToHost, g.stream));
-    std::vector<uint32_t> status(negative ? count : 0);
-    if (negative)
-        PINTT_HIP(hipMemcpyAsync(status.data(), g.add_status, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 g.stream));
-    PINTT_TRY(end_on(g.stream));
-    PINTT_HIP(hipStreamSynchronize(g.stream));
+    PINTT_TRY(st.wait());
     if (negative) {
-        for (size_t i = 0; i < count; ++i) negative[i] = status[i] ? 1 : 0;
-    } else if (err) {
+        for (size_t i = 0; i < count; ++i) negative[i] = host[1 + i] ? 1 : 0;
+    } else if (host[0]) {
         return fail(PINTT_ERR_ARGS, "subtraction with |x| < |y|");
     }
     return PINTT_OK;
