@@ -959,42 +959,46 @@ void free_graph(BatchGraph* bg) {
     delete bg;
 }
 
+// Streams and events of a branched capture; released after the capture has
+// ended (a stream forked from the capture stays in it until then).
+struct Branches {
+    std::vector<hipStream_t> st;
+    std::vector<hipEvent_t> ev;
+    ~Branches() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t x : st)
+            if (x) (void)hipStreamDestroy(x);
+    }
+};
+
 // Capture the launches of `bp` into a graph on the capturing stream cs.  With
 // branches, the schoolbook launch and every length bucket (which then own
 // disjoint parts of the arena and of the carry maps) hang off one fork node
 // as independent chains and join at the end, so the graph may run them at
 // the same time: small buckets and the tails of large launches fill the
 // machine beside the largest bucket instead of after it.
-int capture_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t cs, bool branches) {
-    if (!branches || bp.buckets.size() + (bp.school.empty() ? 0 : 1) < 2)
-        return enqueue_launches(bp, D, nullptr, cs, false);
+int capture_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t cs, Branches* br) {
+    if (!br || bp.buckets.size() + (bp.school.empty() ? 0 : 1) < 2) return enqueue_launches(bp, D, nullptr, cs, false);
     const LaunchSource src{bp, D, nullptr};
     const size_t nb = bp.buckets.size() + 1;  // branch 0: schoolbook products
-    std::vector<hipStream_t> st(nb, nullptr);
-    std::vector<hipEvent_t> ev(nb + 1, nullptr);
-    int rc = PINTT_OK;
-    for (size_t i = 0; i < nb && !rc; ++i)
-        if (hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) != hipSuccess)
-            rc = fail(PINTT_ERR_HIP, "branch stream");
-    for (size_t i = 0; i <= nb && !rc; ++i)
-        if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
-            rc = fail(PINTT_ERR_HIP, "branch event");
+    br->st.assign(nb, nullptr);
+    br->ev.assign(nb + 1, nullptr);
+    for (size_t i = 0; i < nb; ++i)
+        if (hipStreamCreateWithFlags(&br->st[i], hipStreamNonBlocking) != hipSuccess)
+            return fail(PINTT_ERR_HIP, "branch stream");
+    for (size_t i = 0; i <= nb; ++i)
+        if (hipEventCreateWithFlags(&br->ev[i], hipEventDisableTiming) != hipSuccess)
+            return fail(PINTT_ERR_HIP, "branch event");
     // ev[nb] forks; ev[i] joins branch i.
-    if (!rc && hipEventRecord(ev[nb], cs) != hipSuccess) rc = fail(PINTT_ERR_HIP, "fork");
-    for (size_t i = 0; i < nb && !rc; ++i) {
-        if (hipStreamWaitEvent(st[i], ev[nb], 0) != hipSuccess) {
-            rc = fail(PINTT_ERR_HIP, "branch wait");
-            break;
-        }
-        rc = i == 0 ? enqueue_school(src, st[i]) : enqueue_bucket(src, i - 1, st[i], false, nullptr);
-        if (!rc && (hipEventRecord(ev[i], st[i]) != hipSuccess || hipStreamWaitEvent(cs, ev[i], 0) != hipSuccess))
-            rc = fail(PINTT_ERR_HIP, "join");
+    if (hipEventRecord(br->ev[nb], cs) != hipSuccess) return fail(PINTT_ERR_HIP, "fork");
+    for (size_t i = 0; i < nb; ++i) {
+        if (hipStreamWaitEvent(br->st[i], br->ev[nb], 0) != hipSuccess) return fail(PINTT_ERR_HIP, "branch wait");
+        PINTT_TRY(i == 0 ? enqueue_school(src, br->st[i]) : enqueue_bucket(src, i - 1, br->st[i], false, nullptr));
+        if (hipEventRecord(br->ev[i], br->st[i]) != hipSuccess || hipStreamWaitEvent(cs, br->ev[i], 0) != hipSuccess)
+            return fail(PINTT_ERR_HIP, "join");
     }
-    for (hipEvent_t e : ev)
-        if (e) (void)hipEventDestroy(e);
-    for (hipStream_t x : st)
-        if (x) (void)hipStreamDestroy(x);
-    return rc;
+    return PINTT_OK;
 }
 
 int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
@@ -1024,14 +1028,17 @@ int create_graph(const std::vector<Request>& reqs, BatchGraph** out) {
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(PINTT_ERR_HIP, "capture stream"));
     int rc = PINTT_OK;
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        rc = fail(PINTT_ERR_HIP, "hipStreamBeginCapture");
-    } else {
-        rc = capture_launches(bp, bg->desc, cs, g.graph_branches);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(cs, &graph);
-        if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
-        bg->graph = graph;
+    {
+        Branches br;  // outlives the capture
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = fail(PINTT_ERR_HIP, "hipStreamBeginCapture");
+        } else {
+            rc = capture_launches(bp, bg->desc, cs, g.graph_branches ? &br : nullptr);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(cs, &graph);
+            if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+            bg->graph = graph;
+        }
     }
     (void)hipStreamDestroy(cs);
     if (rc) return cleanup(rc);
