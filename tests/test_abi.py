@@ -92,3 +92,42 @@ def test_cost_model_abi_without_device():
     if hiprt.device_count() == 0:
         # no device: the kernels refuse rather than computing on the host
         assert lib.gpu_mlp_forward(w.ctypes.data_as(f), 2, x.ctypes.data_as(f), 3, y.ctypes.data_as(f)) == 1
+
+
+def test_shipped_plan_table_loads(tmp_path):
+    """The shipped tile-plan table names only compiled tile shapes (gpu_ntt_load_plans
+    checks each; no device needed), holds a per-product tuned time for every length
+    2^10 .. 2^25 and its batch plans survive a save/load round trip."""
+    lib = pc.load_library()
+    path = ROOT / "pi_compute_amd" / "data" / "plans_gfx950.txt"
+    assert lib.gpu_ntt_load_plans(str(path).encode()) == 0, lib.gpu_ntt_last_error().decode()
+    lines = [ln.split() for ln in path.read_text().splitlines() if ln and not ln.startswith("#")]
+    own = {int(f[0]): f for f in lines if f[0] != "batch"}
+    batch = {int(f[1]): tuple(int(x) for x in f[2:5]) for f in lines if f[0] == "batch"}
+    assert sorted(own) == list(range(10, 26))
+    # Per product: a length's time grows with its length from 2^12 on (batch times
+    # of 2^(22-n) products would all sit near the same value).
+    ms = [float(own[n][4]) for n in range(12, 26)]
+    assert all(x < y for x, y in zip(ms, ms[1:])), ms
+    for n in range(10, 26):
+        p = pc.get_plan(n)
+        assert (p["log2r"], p["log2w"], p["log2rows"]) == tuple(int(x) for x in own[n][1:4])
+        assert pc.get_batch_plan(n) == batch.get(n)
+    saved = tmp_path / "plans.txt"
+    assert lib.gpu_ntt_save_plans(str(saved).encode()) == 0
+    for n in batch:
+        assert pc.set_batch_plan(n, 0, 0, 0) is None and pc.get_batch_plan(n) is None
+    assert lib.gpu_ntt_load_plans(str(saved).encode()) == 0
+    assert {n: pc.get_batch_plan(n) for n in batch} == batch
+    # A tile shape that is not compiled in is refused.
+    assert lib.gpu_ntt_set_batch_plan(16, 3, 3, 0) != 0
+
+
+def test_plan_tuner_cli():
+    """The tile-plan tuner's command line parses without a device."""
+    import subprocess
+    import sys
+
+    out = subprocess.run([sys.executable, "-m", "pi_compute_amd.tune_plans", "--help"], cwd=ROOT,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "--no-batch" in out.stdout
