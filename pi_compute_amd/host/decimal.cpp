@@ -553,14 +553,36 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
     }
     d.powers_s = std::chrono::duration<double>(Clock::now() - t).count();
     t = Clock::now();
-    for (int j = k; j >= 0 && ((size_t)1 << j) >= leaf_digits; --j) {
-        const size_t np = mpz_sizeinbase(&d.pw[j], 2);
-        const size_t e2 = split_shifts(np).e2;
-        if (device)  // whole limbs: e2 / 64 = limbs(P) + B
-            device_reciprocal_limbs(&d.inv[j], &d.pw[j], e2 / 64 - mpz_size(&d.pw[j]));
-        else
-            newton_reciprocal(&d.inv[j], &d.pw[j], e2 - np, m);  // ~ 2^e2 / P, within 4
-        mpz_sub_ui(&d.inv[j], &d.inv[j], 5);                     // below 2^e2 / P
+    int jlo = k + 1;  // the reciprocals made: j = jlo .. k
+    while (jlo > 0 && ((size_t)1 << (jlo - 1)) >= leaf_digits) --jlo;
+    auto shift_limbs = [&d](int j) {  // whole limbs: e2 / 64 = limbs(P) + B
+        return split_shifts(mpz_sizeinbase(&d.pw[j], 2)).e2 / 64 - mpz_size(&d.pw[j]);
+    };
+    // On the device, the host steps that start every reciprocal (GMP alone,
+    // up to 2^14 limbs) run first, all at once on threads of their own; the
+    // device steps then run one reciprocal after another.
+    struct Start {
+        mpz_t v;
+        Start() { mpz_init(v); }
+        ~Start() { mpz_clear(v); }
+    };
+    std::unique_ptr<Start[]> starts(device && jlo <= k ? new Start[k + 1 - jlo] : nullptr);
+    if (starts) {
+        std::vector<std::future<void>> pending;
+        for (int j = k; j >= jlo; --j)
+            pending.push_back(std::async(std::launch::async, [&d, &starts, &shift_limbs, j, jlo] {
+                device_reciprocal_start(starts[j - jlo].v, &d.pw[j], shift_limbs(j));
+            }));
+        for (auto& f : pending) f.get();
+    }
+    for (int j = k; j >= jlo; --j) {
+        if (device) {
+            device_reciprocal_limbs(&d.inv[j], &d.pw[j], shift_limbs(j), starts[j - jlo].v);
+        } else {
+            const size_t np = mpz_sizeinbase(&d.pw[j], 2);
+            newton_reciprocal(&d.inv[j], &d.pw[j], split_shifts(np).e2 - np, m);  // ~ 2^e2 / P, within 4
+        }
+        mpz_sub_ui(&d.inv[j], &d.inv[j], 5);  // below 2^e2 / P
     }
     if (device) {
         int jmin = k + 1;

@@ -165,32 +165,51 @@ Num newton_step(Dev& g, Num Y, Num p, size_t K, size_t c, size_t cp, bool half) 
     return Yn;
 }
 
-// Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
-// and its top limbs on the host (at least min(Ld, kHostLimbs + kGuard + 1)
-// of them, or all): the steps up to kHostLimbs read no others.
-Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B) {
-    const size_t Ld = D.n;
+// The precisions of a reciprocal's device steps (descending) and the one its
+// host start reaches: the steps up to kHostLimbs run on the host
+// (newton_reciprocal_limbs to that precision is the same integer the device
+// steps would reach; small products are cheaper there than device round
+// trips).
+std::vector<size_t> reciprocal_schedule(size_t B, size_t* c_start) {
     std::vector<size_t> sched;
     for (size_t c = B; c > kExactLimbs; c = (c + 3) / 2) sched.push_back(c);
-    // The steps up to kHostLimbs run on the host (newton_reciprocal_limbs to
-    // that precision is the same integer the device steps would reach; small
-    // products are cheaper there than device round trips).
     size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
     while (!sched.empty() && sched.back() <= kHostLimbs) {
         c = sched.back();
         sched.pop_back();
     }
+    *c_start = c;
+    return sched;
+}
+
+// The host start: beta^(Ld + c) / D to precision c, from D's top limbs Dh (at
+// least min(Ld, c + kGuard) of them, or all).
+void reciprocal_start(mpz_t t, mpz_srcptr Dh, size_t Ld, size_t c) {
+    const size_t need = std::min(Ld, c + kGuard);
+    const size_t have = mpz_size(Dh);
+    if (have < need || have > Ld) throw std::runtime_error("device reciprocal: host top limbs do not fit");
+    // The top `need` limbs (the same integer the whole divisor gives the host
+    // steps: each reads at most c + kGuard top limbs).
+    mpz_t top;
+    mpz_roinit_n(top, mpz_limbs_read(Dh) + (have - need), (mp_size_t)need);
+    newton_reciprocal_limbs(t, need == Ld ? Dh : top, c);
+}
+
+// Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
+// and its top limbs on the host (at least min(Ld, kHostLimbs + kGuard + 1)
+// of them, or all): the steps up to kHostLimbs read no others.  start: the
+// host start (reciprocal_start) computed beforehand, or null.
+Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B, mpz_srcptr start = nullptr) {
+    const size_t Ld = D.n;
+    size_t c = 0;
+    const std::vector<size_t> sched = reciprocal_schedule(B, &c);
     Num Y;
-    {
-        const size_t need = std::min(Ld, c + kGuard);
-        const size_t have = mpz_size(Dh);
-        if (have < need || have > Ld) throw std::runtime_error("device reciprocal: host top limbs do not fit");
-        // The top `need` limbs (the same integer the whole divisor gives the
-        // host steps: each reads at most c + kGuard top limbs).
-        mpz_t top, t;
-        mpz_roinit_n(top, mpz_limbs_read(Dh) + (have - need), (mp_size_t)need);
+    if (start) {
+        Y = g.upload(start, 0, mpz_size(start));
+    } else {
+        mpz_t t;
         mpz_init(t);
-        newton_reciprocal_limbs(t, need == Ld ? Dh : top, c);
+        reciprocal_start(t, Dh, Ld, c);
         Y = g.upload(t, 0, mpz_size(t));
         mpz_clear(t);
     }
@@ -246,12 +265,19 @@ void device_inv_sqrt_limbs(mpz_t y, unsigned long a, size_t B) {
     mpz_limbs_finish(y, (mp_size_t)Y.n);
 }
 
-void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B) {
+void device_reciprocal_start(mpz_t t, mpz_srcptr d, size_t B) {
+    if (mpz_sgn(d) <= 0) throw std::runtime_error("device reciprocal: divisor must be positive");
+    size_t c = 0;
+    (void)reciprocal_schedule(B, &c);
+    reciprocal_start(t, d, mpz_size(d), c);
+}
+
+void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B, mpz_srcptr start) {
     if (mpz_sgn(d) <= 0) throw std::runtime_error("device reciprocal: divisor must be positive");
     const size_t Ld = mpz_size(d);
     Dev g(8 * (Ld + B) + 64);
     const Num D = g.upload(d, 0, Ld);
-    const Num Y = reciprocal(g, D, d, B);
+    const Num Y = reciprocal(g, D, d, B, start);
     uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(y, (mp_size_t)Y.n));
     const uint64_t* src = Y.d;
     size_t n = Y.n;
