@@ -134,8 +134,9 @@ int gpu_ntt_add_batch_device(size_t count, const uint64_t* const* d_x, const siz
 
 /* Same batch enqueued without waiting (ordered with products on the library
  * stream): a subtraction with x < y sets the device word *d_negative to 1
- * (it is never cleared here; the caller zeroes and reads it) and leaves that
- * output undefined.  Lets a caller enqueue a whole stretch of binary-splitting
+ * (it is never cleared here; the caller zeroes and reads it) and writes
+ * x - y + 2^(64 max(xn, yn)) to that output, as gpu_ntt_add_batch_device_flags
+ * does.  Lets a caller enqueue a whole stretch of binary-splitting
  * levels and check once.  No reference counterpart. */
 int gpu_ntt_add_batch_device_async(size_t count, const uint64_t* const* d_x, const size_t* xn,
                                    const uint64_t* const* d_y, const size_t* yn, const int* subtract,

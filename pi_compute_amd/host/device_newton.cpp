@@ -42,14 +42,15 @@ using dev::Num;
 class Dev {
 public:
     explicit Dev(size_t limbs) : base_(mem_.alloc(limbs)), cap_(limbs) {
-        const uint64_t c[3] = {0, 1, 1ull << 63};
+        const uint64_t c[4] = {0, 1, 1ull << 63, 0};
         zero = take(1);
         one = take(1);
         half = take(1);
-        const uint64_t* src[3] = {&c[0], &c[1], &c[2]};
-        uint64_t* dst[3] = {zero.d, one.d, half.d};
-        const size_t n[3] = {1, 1, 1};
-        dev::check(gpu_ntt_upload(3, dst, src, n), "gpu_ntt_upload");
+        sink_ = reinterpret_cast<uint32_t*>(take(1).d);
+        const uint64_t* src[4] = {&c[0], &c[1], &c[2], &c[3]};
+        uint64_t* dst[4] = {zero.d, one.d, half.d, reinterpret_cast<uint64_t*>(sink_)};
+        const size_t n[4] = {1, 1, 1, 1};
+        dev::check(gpu_ntt_upload(4, dst, src, n), "gpu_ntt_upload");
     }
     Num take(size_t n) {
         n = std::max<size_t>(n, 1);
@@ -93,9 +94,20 @@ public:
                    "gpu_ntt_add_batch_device_flags");
         return neg != 0;
     }
+    // out = x + y or x - y as addsub, enqueued without a wait: for a result
+    // whose sign is known (x - y below zero is still x - y + beta^max; its
+    // flag goes to a device word nobody reads).
+    void addsub_async(uint64_t* out, Num x, Num y, bool sub) {
+        const uint64_t* X = x.d;
+        const uint64_t* Y = y.d;
+        const int s = sub ? 1 : 0;
+        dev::check(gpu_ntt_add_batch_device_async(1, &X, &x.n, &Y, &y.n, &s, &out, sink_),
+                   "gpu_ntt_add_batch_device_async");
+    }
     Num zero, one, half;  // 0, 1 and 2^63 (beta / 2)
 
 private:
+    uint32_t* sink_ = nullptr;  // the negative flag of addsub_async
     dev::Buffers mem_;
     uint64_t* base_;
     size_t cap_, used_ = 0;
@@ -124,7 +136,7 @@ Num newton_step(Dev& g, Num Y, Num p, size_t K, size_t c, size_t cp, bool half) 
     } else if (!above && top[1] == 0 && top[0] == ~0ull) {  // p < beta^K
         positive = true;
         E = g.take(K);
-        (void)g.addsub(E.d, g.zero, Num{p.d, K}, true);  // beta^K - p
+        g.addsub_async(E.d, g.zero, Num{p.d, K}, true);  // beta^K - p (0 - p + beta^K)
     } else {
         throw std::runtime_error("device Newton step: product far from beta^K");
     }
@@ -147,7 +159,7 @@ Num newton_step(Dev& g, Num Y, Num p, size_t K, size_t c, size_t cp, bool half) 
     Num Yn = g.take(s + hn);
     if (positive) {
         dev::check(gpu_ntt_dev_copy(Yn.d, term.d, s), "gpu_ntt_dev_copy");
-        (void)g.addsub(Yn.d + s, Y, hi, false);
+        g.addsub_async(Yn.d + s, Y, hi, false);
     } else {
         // Y beta^s - term: the low limbs are -term mod beta^s, with a borrow
         // into the high part when they are not all zero.
