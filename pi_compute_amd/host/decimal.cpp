@@ -532,6 +532,11 @@ DevicePowers::DevicePowers(const DecimalPowers& d, int jmin_) : jmin(jmin_) {
     upload(src, dst);
 }
 
+// The conversion's reciprocals take their host steps up to 2^16 limbs: those
+// run for every reciprocal at once on threads of their own, while each device
+// step waits on the library stream behind binary splitting.
+constexpr size_t kConvHostLimbs = 65536;
+
 void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_digits, bool device) {
     using Clock = std::chrono::steady_clock;
     auto t = Clock::now();
@@ -571,13 +576,13 @@ void decimal_powers(DecimalPowers& d, size_t width, Multiplier& m, size_t leaf_d
         std::vector<std::future<void>> pending;
         for (int j = k; j >= jlo; --j)
             pending.push_back(std::async(std::launch::async, [&d, &starts, &shift_limbs, j, jlo] {
-                device_reciprocal_start(starts[j - jlo].v, &d.pw[j], shift_limbs(j));
+                device_reciprocal_start(starts[j - jlo].v, &d.pw[j], shift_limbs(j), kConvHostLimbs);
             }));
         for (auto& f : pending) f.get();
     }
     for (int j = k; j >= jlo; --j) {
         if (device) {
-            device_reciprocal_limbs(&d.inv[j], &d.pw[j], shift_limbs(j), starts[j - jlo].v);
+            device_reciprocal_limbs(&d.inv[j], &d.pw[j], shift_limbs(j), starts[j - jlo].v, kConvHostLimbs);
         } else {
             const size_t np = mpz_sizeinbase(&d.pw[j], 2);
             newton_reciprocal(&d.inv[j], &d.pw[j], split_shifts(np).e2 - np, m);  // ~ 2^e2 / P, within 4

@@ -115,7 +115,7 @@ private:
 
 constexpr size_t kGuard = 3;
 constexpr size_t kExactLimbs = 16;   // the host mirror's exact start (newton.cpp)
-constexpr size_t kHostLimbs = 16384;
+constexpr size_t kHostLimbs = kReciprocalHostLimbs;
 
 // One Newton step from precision c to cp limbs, given p ~ beta^K on the
 // device (p.n >= K + 1): E = beta^K - p, E_t = the limbs of |E| from m - T to
@@ -182,11 +182,11 @@ Num newton_step(Dev& g, Num Y, Num p, size_t K, size_t c, size_t cp, bool half) 
 // (newton_reciprocal_limbs to that precision is the same integer the device
 // steps would reach; small products are cheaper there than device round
 // trips).
-std::vector<size_t> reciprocal_schedule(size_t B, size_t* c_start) {
+std::vector<size_t> reciprocal_schedule(size_t B, size_t* c_start, size_t host_limbs = kHostLimbs) {
     std::vector<size_t> sched;
     for (size_t c = B; c > kExactLimbs; c = (c + 3) / 2) sched.push_back(c);
     size_t c = sched.empty() ? B : (sched.back() + 3) / 2;
-    while (!sched.empty() && sched.back() <= kHostLimbs) {
+    while (!sched.empty() && sched.back() <= host_limbs) {
         c = sched.back();
         sched.pop_back();
     }
@@ -210,11 +210,13 @@ void reciprocal_start(mpz_t t, mpz_srcptr Dh, size_t Ld, size_t c) {
 // Y ~ beta^(Ld + B) / D; D on the device (exactly Ld limbs, top limb nonzero)
 // and its top limbs on the host (at least min(Ld, kHostLimbs + kGuard + 1)
 // of them, or all): the steps up to kHostLimbs read no others.  start: the
-// host start (reciprocal_start) computed beforehand, or null.
-Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B, mpz_srcptr start = nullptr) {
+// host start (reciprocal_start) computed beforehand, or null; host_limbs: the
+// precision the host steps go to (the same integer whatever it is).
+Num reciprocal(Dev& g, Num D, mpz_srcptr Dh, size_t B, mpz_srcptr start = nullptr,
+               size_t host_limbs = kHostLimbs) {
     const size_t Ld = D.n;
     size_t c = 0;
-    const std::vector<size_t> sched = reciprocal_schedule(B, &c);
+    const std::vector<size_t> sched = reciprocal_schedule(B, &c, host_limbs);
     Num Y;
     if (start) {
         Y = g.upload(start, 0, mpz_size(start));
@@ -277,19 +279,25 @@ void device_inv_sqrt_limbs(mpz_t y, unsigned long a, size_t B) {
     mpz_limbs_finish(y, (mp_size_t)Y.n);
 }
 
-void device_reciprocal_start(mpz_t t, mpz_srcptr d, size_t B) {
+void device_reciprocal_start(mpz_t t, mpz_srcptr d, size_t B, size_t host_limbs) {
     if (mpz_sgn(d) <= 0) throw std::runtime_error("device reciprocal: divisor must be positive");
     size_t c = 0;
-    (void)reciprocal_schedule(B, &c);
+    (void)reciprocal_schedule(B, &c, host_limbs);
     reciprocal_start(t, d, mpz_size(d), c);
 }
 
-void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B, mpz_srcptr start) {
+void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B, mpz_srcptr start, size_t host_limbs) {
     if (mpz_sgn(d) <= 0) throw std::runtime_error("device reciprocal: divisor must be positive");
     const size_t Ld = mpz_size(d);
+    size_t c = 0;
+    if (reciprocal_schedule(B, &c, host_limbs).empty() && start) {
+        // Every step ran on the host: the start is the reciprocal.
+        mpz_set(y, start);
+        return;
+    }
     Dev g(8 * (Ld + B) + 64);
     const Num D = g.upload(d, 0, Ld);
-    const Num Y = reciprocal(g, D, d, B, start);
+    const Num Y = reciprocal(g, D, d, B, start, host_limbs);
     uint64_t* h = reinterpret_cast<uint64_t*>(mpz_limbs_write(y, (mp_size_t)Y.n));
     const uint64_t* src = Y.d;
     size_t n = Y.n;

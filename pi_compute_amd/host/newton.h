@@ -29,12 +29,15 @@ void newton_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B);
 
 // The same steps on device numbers through the GPU engine's device entry
 // points (device_newton.cpp): d uploaded, y downloaded, the same integer as
-// newton_reciprocal_limbs(y, d, B).  Its first steps, up to 2^14 limbs, run
-// on the host: device_reciprocal_start computes them alone (no device, any
-// thread), and `start` hands that result in (null: computed here).  Throws
-// std::runtime_error if the device fails.
-void device_reciprocal_start(mpz_t t, mpz_srcptr d, size_t B);
-void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B, mpz_srcptr start = nullptr);
+// newton_reciprocal_limbs(y, d, B).  Its first steps, up to host_limbs
+// precision, run on the host (the same integer whichever side a step runs
+// on): device_reciprocal_start computes them alone (no device, any thread),
+// and `start` hands that result in (null: computed here; the same host_limbs
+// both times).  Throws std::runtime_error if the device fails.
+constexpr size_t kReciprocalHostLimbs = 16384;
+void device_reciprocal_start(mpz_t t, mpz_srcptr d, size_t B, size_t host_limbs = kReciprocalHostLimbs);
+void device_reciprocal_limbs(mpz_t y, mpz_srcptr d, size_t B, mpz_srcptr start = nullptr,
+                             size_t host_limbs = kReciprocalHostLimbs);
 
 // y ~ beta^B / sqrt(a) in whole limbs, a > 0 (the step: E = beta^(2c) - a y^2,
 // y' = y beta^(c'-c) + y E_t 2^63 >> limbs, 2^63 being the step's halving as
