@@ -1,4 +1,4 @@
-"""The shipped dispatch cost model (pi_compute_amd/data/cost_model.txt) judged on
+"""The shipped dispatch cost model (pi_compute_amd/data/cost_model.txt) evaluated on
 its held-out calibration slice (pi_compute_amd/data/cost_model_holdout.jsonl.gz,
 the shapes pi_train_mlp kept out of training), on the CPU.
 
