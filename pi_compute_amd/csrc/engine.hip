@@ -981,7 +981,7 @@ struct Branches {
 int capture_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t cs, Branches* br) {
     if (!br || bp.buckets.size() + (bp.school.empty() ? 0 : 1) < 2) return enqueue_launches(bp, D, nullptr, cs, false);
     const LaunchSource src{bp, D, nullptr};
-    const size_t nb = bp.buckets.size() + 1;  // branch 0: schoolbook products
+    const size_t nb = bp.buckets.size() + 1;  // the last branch: schoolbook products
     br->st.assign(nb, nullptr);
     br->ev.assign(nb + 1, nullptr);
     for (size_t i = 0; i < nb; ++i)
@@ -990,11 +990,17 @@ int capture_launches(const BatchPlan& bp, const unsigned char* D, hipStream_t cs
     for (size_t i = 0; i <= nb; ++i)
         if (hipEventCreateWithFlags(&br->ev[i], hipEventDisableTiming) != hipSuccess)
             return fail(PINTT_ERR_HIP, "branch event");
-    // ev[nb] forks; ev[i] joins branch i.
+    // ev[nb] forks; ev[i] joins branch i.  The buckets with the most
+    // transform work are captured first.
+    std::vector<size_t> order(bp.buckets.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    auto work = [&bp](size_t i) { return (size_t)bp.buckets[i].nfwd << bp.buckets[i].log2n; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
     if (hipEventRecord(br->ev[nb], cs) != hipSuccess) return fail(PINTT_ERR_HIP, "fork");
     for (size_t i = 0; i < nb; ++i) {
         if (hipStreamWaitEvent(br->st[i], br->ev[nb], 0) != hipSuccess) return fail(PINTT_ERR_HIP, "branch wait");
-        PINTT_TRY(i == 0 ? enqueue_school(src, br->st[i]) : enqueue_bucket(src, i - 1, br->st[i], false, nullptr));
+        PINTT_TRY(i == nb - 1 ? enqueue_school(src, br->st[i])
+                              : enqueue_bucket(src, order[i], br->st[i], false, nullptr));
         if (hipEventRecord(br->ev[i], br->st[i]) != hipSuccess || hipStreamWaitEvent(cs, br->ev[i], 0) != hipSuccess)
             return fail(PINTT_ERR_HIP, "join");
     }
